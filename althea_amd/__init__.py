@@ -1,0 +1,300 @@
+"""Python binding of the MI355X IBL precompute + deferred-resolve library.
+
+The surface mirrors nithinp7/Althea's host API for this path
+(Include/Althea/ImageBasedLighting.h:17-54, Include/Althea/DeferredRendering.h:26-102)
+with torch device tensors in place of VkImages:
+
+- ``ImageBasedLighting.createResources(env, desc)`` -> ``IBLResources``
+  (environment pyramid, prefiltered cube, irradiance cube, BRDF LUT);
+  ``IBLResources.getHandles()`` returns the device views in the reference's
+  order (environment, prefiltered, irradiance, BRDF LUT).
+- ``GBufferResources`` wraps the four G-buffer planes in the order of
+  GBufferResources::buildMaterial (Src/DeferredRendering.cpp:23-35).
+- ``DeferredRendering.resolve(...)`` runs DeferredPass.frag's lighting.
+
+Every call goes through the C ABI of ``libalthea_amd.so`` onto torch's
+current HIP stream.  There is no fallback: without the built library or a
+GPU these calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from . import synth
+from ._lib import AltheaError, check, lib
+
+__all__ = ["AltheaError", "IBLDesc", "IBLResources", "ImageBasedLighting", "GBufferResources",
+           "DeferredRendering", "synth", "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
+           "brdf_lut", "cube_sample", "env_sample", "prefilter_points", "env_levels", "cube_texels"]
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    assert t.is_cuda, "device tensor required"
+    assert t.is_contiguous(), "contiguous tensor required"
+    return t.data_ptr()
+
+
+_hip = None
+
+
+def _hip_copy_dtod(dst: int, src: int, nbytes: int, stream=None) -> None:
+    """hipMemcpyAsync device-to-device on torch's current stream."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        _hip.hipMemcpyAsync.restype = C.c_int
+    rc = _hip.hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(src), nbytes, 3, C.c_void_p(_stream(stream)))
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpyAsync failed ({rc})")
+
+
+def env_levels(width: int, height: int) -> int:
+    return lib().althea_env_levels(width, height)
+
+
+def cube_texels(size: int, mips: int) -> int:
+    return lib().althea_cube_texels(size, mips)
+
+
+# ------------------------------------------------------------- primitives
+def hammersley(n: int, first: int = 0, count: int | None = None, stream=None):
+    """(bits uint32 tensor, xi float32 (count, 2)) from the device Hammersley."""
+    torch = _torch()
+    count = n - first if count is None else count
+    bits = torch.empty(count, dtype=torch.int32, device="cuda")
+    xi = torch.empty((count, 2), dtype=torch.float32, device="cuda")
+    check(lib().althea_hammersley(n, first, count, _ptr(bits), _ptr(xi), _stream(stream)), "althea_hammersley")
+    return bits, xi
+
+
+def env_pyramid(level0, stream=None):
+    """Full pyramid of an equirect RGBA32F map (h, w, 4) -> flat (texels, 4)."""
+    torch = _torch()
+    h, w = int(level0.shape[0]), int(level0.shape[1])
+    out = torch.empty((lib().althea_env_pyramid_texels(w, h), 4), dtype=torch.float32, device=level0.device)
+    check(lib().althea_env_pyramid_build(_ptr(level0), w, h, _ptr(out), _stream(stream)), "althea_env_pyramid_build")
+    return out
+
+
+def prefilter_cube(pyramid, env_w: int, env_h: int, size: int, mips: int, samples: int, fp32_sums: bool = False,
+                   stream=None):
+    torch = _torch()
+    out = torch.empty((cube_texels(size, mips), 4), dtype=torch.float32, device=pyramid.device)
+    flags = _lib.PRECOMPUTE_FP32_SUMS if fp32_sums else 0
+    check(lib().althea_prefilter_cube(_ptr(pyramid), env_w, env_h, size, mips, samples, flags, _ptr(out),
+                                      _stream(stream)), "althea_prefilter_cube")
+    return out
+
+
+def prefilter_points(pyramid, env_w: int, env_h: int, samples: int, dir_rough, stream=None):
+    torch = _torch()
+    out = torch.empty((dir_rough.shape[0], 4), dtype=torch.float32, device=pyramid.device)
+    check(lib().althea_prefilter_points(_ptr(pyramid), env_w, env_h, samples, _ptr(dir_rough),
+                                        int(dir_rough.shape[0]), _ptr(out), _stream(stream)),
+          "althea_prefilter_points")
+    return out
+
+
+def irradiance_cube(pyramid, env_w: int, env_h: int, size: int, theta_samples: int = 300, fp32_sums: bool = False,
+                    stream=None):
+    torch = _torch()
+    out = torch.empty((6 * size * size, 4), dtype=torch.float32, device=pyramid.device)
+    flags = _lib.PRECOMPUTE_FP32_SUMS if fp32_sums else 0
+    check(lib().althea_irradiance_cube(_ptr(pyramid), env_w, env_h, size, theta_samples, flags, _ptr(out),
+                                       _stream(stream)), "althea_irradiance_cube")
+    return out
+
+
+def brdf_lut(size: int, samples: int, stream=None):
+    torch = _torch()
+    out = torch.empty((size, size, 2), dtype=torch.float32, device="cuda")
+    check(lib().althea_brdf_lut(size, samples, _ptr(out), _stream(stream)), "althea_brdf_lut")
+    return out
+
+
+def cube_sample(cube, size: int, mips: int, dir_lod, stream=None):
+    torch = _torch()
+    out = torch.empty((dir_lod.shape[0], 4), dtype=torch.float32, device=cube.device)
+    check(lib().althea_cube_sample(_ptr(cube), size, mips, _ptr(dir_lod), int(dir_lod.shape[0]), _ptr(out),
+                                   _stream(stream)), "althea_cube_sample")
+    return out
+
+
+def env_sample(pyramid, env_w: int, env_h: int, dir_lod, stream=None):
+    torch = _torch()
+    out = torch.empty((dir_lod.shape[0], 4), dtype=torch.float32, device=pyramid.device)
+    check(lib().althea_env_sample(_ptr(pyramid), env_w, env_h, _ptr(dir_lod), int(dir_lod.shape[0]), _ptr(out),
+                                  _stream(stream)), "althea_env_sample")
+    return out
+
+
+# ------------------------------------------------ ImageBasedLighting mirror
+@dataclass
+class IBLDesc:
+    irradiance_size: int = 32
+    irradiance_theta_samples: int = 300
+    prefiltered_size: int = 128
+    prefiltered_mips: int = 5
+    prefilter_samples: int = 1024
+    brdf_lut_size: int = 128
+    brdf_lut_samples: int = 1024
+    fp32_sums: bool = False
+
+    def c(self) -> _lib.IBLDescC:
+        return _lib.IBLDescC(self.irradiance_size, self.irradiance_theta_samples, self.prefiltered_size,
+                             self.prefiltered_mips, self.prefilter_samples, self.brdf_lut_size,
+                             self.brdf_lut_samples, _lib.PRECOMPUTE_FP32_SUMS if self.fp32_sums else 0)
+
+
+class IBLResources:
+    """Owns an ``althea_ibl`` handle (IBLResources, ImageBasedLighting.h:24-43)."""
+
+    def __init__(self, desc: IBLDesc):
+        self.desc = desc
+        h = C.c_void_p()
+        check(lib().althea_ibl_create(C.byref(desc.c()), C.byref(h)), "althea_ibl_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().althea_ibl_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def precompute(self, env, stream=None) -> None:
+        h, w = int(env.shape[0]), int(env.shape[1])
+        self._env_keepalive = env
+        check(lib().althea_ibl_precompute(self._h, _ptr(env), w, h, _stream(stream)), "althea_ibl_precompute")
+
+    def getHandles(self) -> _lib.IBLViewsC:
+        v = _lib.IBLViewsC()
+        check(lib().althea_ibl_get_views(self._h, C.byref(v)), "althea_ibl_get_views")
+        return v
+
+    views = getHandles
+
+    def state_bytes(self) -> int:
+        return lib().althea_ibl_state_bytes(self._h)
+
+    def _tensor_of(self, ptr: int, count: int, comps: int):
+        """Device copy (stream-ordered) of `count` x `comps` floats at `ptr`."""
+        torch = _torch()
+        out = torch.empty((count, comps), dtype=torch.float32, device="cuda")
+        _hip_copy_dtod(out.data_ptr(), ptr, count * comps * 4)
+        return out
+
+    def prefiltered_tensor(self):
+        v = self.views()
+        return self._tensor_of(v.prefiltered, cube_texels(v.prefiltered_size, v.prefiltered_mips), 4)
+
+    def irradiance_tensor(self):
+        v = self.views()
+        return self._tensor_of(v.irradiance, 6 * v.irradiance_size ** 2, 4)
+
+    def lut_tensor(self):
+        v = self.views()
+        return self._tensor_of(v.brdf_lut, v.brdf_lut_size ** 2, 2)
+
+    def environment_tensor(self):
+        v = self.views()
+        return self._tensor_of(v.environment, lib().althea_env_pyramid_texels(v.env_width, v.env_height), 4)
+
+    def views_numpy(self) -> dict:
+        v = self.views()
+        return {
+            "env_w": v.env_width, "env_h": v.env_height,
+            "prefiltered": self.prefiltered_tensor().cpu().numpy(), "pref_size": v.prefiltered_size,
+            "pref_mips": v.prefiltered_mips,
+            "irradiance": self.irradiance_tensor().cpu().numpy(), "irr_size": v.irradiance_size,
+            "lut": self.lut_tensor().cpu().numpy(), "lut_size": v.brdf_lut_size,
+        }
+
+    def prefiltered_texels(self, ids: np.ndarray) -> np.ndarray:
+        """RGB of cube texels ids[:, (mip, face, x, y)] (host numpy)."""
+        v = self.views()
+        cube = self.prefiltered_tensor().cpu().numpy()
+        out = np.empty((ids.shape[0], 3), np.float32)
+        for k, (m, f, x, y) in enumerate(ids):
+            s = max(1, v.prefiltered_size >> int(m))
+            off = cube_texels(v.prefiltered_size, int(m)) + (int(f) * s + int(y)) * s + int(x)
+            out[k] = cube[off, :3]
+        return out
+
+
+class ImageBasedLighting:
+    """ImageBasedLighting::createResources (ImageBasedLighting.h:45-49): the
+    environment comes in as a device tensor instead of a file name."""
+
+    @staticmethod
+    def createResources(env, desc: IBLDesc | None = None, stream=None) -> IBLResources:
+        res = IBLResources(desc or IBLDesc())
+        res.precompute(env, stream)
+        return res
+
+    create_resources = createResources
+
+
+# ------------------------------------------------------ G-buffer / resolve
+class GBufferResources:
+    """The four G-buffer planes (position, normal, albedo, metallic-roughness-
+    occlusion) as (h, w, 4) device tensors, FP32 or FP16."""
+
+    def __init__(self, position, normal, albedo, mro):
+        self.planes = [position, normal, albedo, mro]
+        dt = position.dtype
+        torch = _torch()
+        assert all(p.dtype == dt and p.shape == position.shape for p in self.planes)
+        self.height, self.width = int(position.shape[0]), int(position.shape[1])
+        self.format = _lib.GBUFFER_FP16 if dt == torch.float16 else _lib.GBUFFER_FP32
+
+    def getHandles(self) -> _lib.GBufferViewC:
+        p = self.planes
+        return _lib.GBufferViewC(_ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(p[3]), self.width, self.height,
+                                 self.format)
+
+
+class DeferredRendering:
+    @staticmethod
+    def resolve(gbuffer, ibl, lights, globals_, ssao=None, tonemap: bool = False, out_fp16: bool = False,
+                ao_out=None, out=None, stream=None):
+        """One frame of the deferred resolve.  `gbuffer` is GBufferResources
+        or a list of the four planes; `ibl` an IBLResources or IBLViewsC;
+        `lights` an (n, 8) float32 device tensor."""
+        torch = _torch()
+        if not isinstance(gbuffer, GBufferResources):
+            gbuffer = GBufferResources(*gbuffer)
+        views = ibl.views() if isinstance(ibl, IBLResources) else ibl
+        gv = gbuffer.getHandles()
+        n = int(lights.shape[0]) if lights is not None else 0
+        if out is None:
+            dt = torch.float16 if out_fp16 else torch.float32
+            out = torch.empty((gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
+        flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)
+        check(lib().althea_deferred_resolve(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
+                                            C.byref(globals_), C.byref(ssao) if ssao is not None else None,
+                                            flags, _ptr(out), _ptr(ao_out), _stream(stream)),
+              "althea_deferred_resolve")
+        return out

@@ -1,0 +1,116 @@
+"""ctypes binding of ``libalthea_amd.so`` (the C ABI of include/althea/althea.h).
+
+Loading fails loudly when the library has not been built: there is no CPU or
+Python fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libalthea_amd.so"
+
+ALTHEA_OK = 0
+PRECOMPUTE_FP32_SUMS = 0x1
+RESOLVE_TONEMAP = 0x1
+RESOLVE_OUT_FP16 = 0x2
+GBUFFER_FP32 = 0
+GBUFFER_FP16 = 1
+SSAO_MAX_SAMPLES = 64
+
+vp = C.c_void_p
+i32 = C.c_int
+u32 = C.c_uint32
+sz = C.c_size_t
+
+
+class IBLDescC(C.Structure):
+    _fields_ = [("irradiance_size", i32), ("irradiance_theta_samples", i32), ("prefiltered_size", i32),
+                ("prefiltered_mips", i32), ("prefilter_samples", i32), ("brdf_lut_size", i32),
+                ("brdf_lut_samples", i32), ("flags", C.c_uint)]
+
+
+class IBLViewsC(C.Structure):
+    _fields_ = [("environment", vp), ("env_width", i32), ("env_height", i32), ("env_levels", i32),
+                ("prefiltered", vp), ("prefiltered_size", i32), ("prefiltered_mips", i32),
+                ("irradiance", vp), ("irradiance_size", i32), ("brdf_lut", vp), ("brdf_lut_size", i32)]
+
+
+class GBufferViewC(C.Structure):
+    _fields_ = [("position", vp), ("normal", vp), ("albedo", vp), ("mro", vp), ("width", i32), ("height", i32),
+                ("format", i32)]
+
+
+class GlobalsC(C.Structure):
+    _fields_ = [("projection", C.c_float * 16), ("inverse_projection", C.c_float * 16), ("view", C.c_float * 16),
+                ("prev_view", C.c_float * 16), ("inverse_view", C.c_float * 16),
+                ("prev_inverse_view", C.c_float * 16), ("mouse_uv", C.c_float * 2), ("light_count", C.c_int32),
+                ("light_buffer_handle", u32), ("time", C.c_float), ("exposure", C.c_float), ("input_mask", u32),
+                ("frame_count", u32)]
+
+
+class SsaoDescC(C.Structure):
+    _fields_ = [("enabled", i32), ("samples", i32), ("radius", C.c_float), ("bias", C.c_float),
+                ("kernel", (C.c_float * 4) * SSAO_MAX_SAMPLES), ("noise", (C.c_float * 4) * 16)]
+
+
+assert C.sizeof(GlobalsC) == 416, C.sizeof(GlobalsC)
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "althea_status_string": (C.c_char_p, [i32]),
+    "althea_last_error": (C.c_char_p, []),
+    "althea_version": (i32, []),
+    "althea_hammersley": (i32, [u32, u32, u32, vp, vp, vp]),
+    "althea_env_levels": (i32, [i32, i32]),
+    "althea_env_pyramid_texels": (sz, [i32, i32]),
+    "althea_env_pyramid_build": (i32, [vp, i32, i32, vp, vp]),
+    "althea_cube_texels": (sz, [i32, i32]),
+    "althea_cube_sample": (i32, [vp, i32, i32, vp, i32, vp, vp]),
+    "althea_env_sample": (i32, [vp, i32, i32, vp, i32, vp, vp]),
+    "althea_prefilter_cube": (i32, [vp, i32, i32, i32, i32, i32, C.c_uint, vp, vp]),
+    "althea_prefilter_points": (i32, [vp, i32, i32, i32, vp, i32, vp, vp]),
+    "althea_irradiance_cube": (i32, [vp, i32, i32, i32, i32, C.c_uint, vp, vp]),
+    "althea_brdf_lut": (i32, [i32, i32, vp, vp]),
+    "althea_ibl_desc_default": (None, [C.POINTER(IBLDescC)]),
+    "althea_ibl_create": (i32, [C.POINTER(IBLDescC), C.POINTER(vp)]),
+    "althea_ibl_destroy": (i32, [vp]),
+    "althea_ibl_precompute": (i32, [vp, vp, i32, i32, vp]),
+    "althea_ibl_get_views": (i32, [vp, C.POINTER(IBLViewsC)]),
+    "althea_ibl_state_bytes": (sz, [vp]),
+    "althea_deferred_resolve": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
+                                      C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded C ABI library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first "
+                               "(there is no fallback path)")
+        lib_ = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(lib_, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib_
+    return _lib
+
+
+class AltheaError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        l = lib()
+        super().__init__(f"{where}: {l.althea_status_string(status).decode()} ({status}): "
+                         f"{l.althea_last_error().decode()}")
+        self.status = status
+
+
+def check(status: int, where: str) -> None:
+    if status != ALTHEA_OK:
+        raise AltheaError(status, where)

@@ -1,0 +1,359 @@
+// C ABI entry points: argument validation, handle management and workspace
+// plumbing around the kernel launchers.  No exception crosses this boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "althea/althea.h"
+#include "internal.hpp"
+
+namespace althea {
+
+namespace {
+thread_local char g_error[512] = "";
+}
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_error, sizeof(g_error), fmt, ap);
+  va_end(ap);
+}
+
+althea_status check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s launch failed: %s", what, hipGetErrorString(e));
+    return ALTHEA_ERROR_HIP;
+  }
+  return ALTHEA_OK;
+}
+
+PyramidGeometry pyramid_geometry(int width, int height) {
+  PyramidGeometry g{};
+  g.levels = althea_env_levels(width, height);
+  size_t total = 0;
+  for (int l = 0; l < g.levels; ++l) {
+    g.w[l] = std::max(1, width >> l);
+    g.h[l] = std::max(1, height >> l);
+    g.off[l] = static_cast<uint32_t>(total);
+    total += static_cast<size_t>(g.w[l]) * g.h[l];
+  }
+  g.texels = total;
+  return g;
+}
+
+size_t cube_mip_offset(int size, int mip) {
+  size_t off = 0;
+  for (int m = 0; m < mip; ++m) {
+    size_t s = static_cast<size_t>(std::max(1, size >> m));
+    off += 6 * s * s;
+  }
+  return off;
+}
+
+// Stream-ordered workspace: allocated and released on the caller's stream.
+class Workspace {
+ public:
+  Workspace(size_t bytes, hipStream_t s) : s_(s) {
+    if (bytes == 0) return;
+    e_ = hipMallocAsync(&ptr_, bytes, s);
+    if (e_ != hipSuccess) {
+      ptr_ = nullptr;
+      set_error("hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e_));
+    }
+  }
+  ~Workspace() {
+    if (ptr_) (void)hipFreeAsync(ptr_, s_);
+  }
+  bool ok() const { return e_ == hipSuccess; }
+  void* get() const { return ptr_; }
+
+ private:
+  void* ptr_ = nullptr;
+  hipStream_t s_;
+  hipError_t e_ = hipSuccess;
+};
+
+static bool env_dims_ok(int w, int h) {
+  return is_pow2(w) && is_pow2(h) && w <= (1 << 15) && h <= (1 << 15);
+}
+
+}  // namespace althea
+
+using namespace althea;
+
+// ---------------------------------------------------------------- handles
+struct althea_ibl {
+  althea_ibl_desc desc{};
+  int env_w = 0, env_h = 0;
+  float* env = nullptr;  // pyramid
+  size_t env_texels = 0;
+  float* prefiltered = nullptr;
+  float* irradiance = nullptr;
+  float* lut = nullptr;
+  void* workspace = nullptr;
+  size_t workspace_bytes = 0;
+};
+
+extern "C" {
+
+const char* althea_status_string(althea_status status) {
+  switch (status) {
+    case ALTHEA_OK: return "ok";
+    case ALTHEA_ERROR_INVALID_ARGUMENT: return "invalid argument";
+    case ALTHEA_ERROR_OUT_OF_MEMORY: return "out of device memory";
+    case ALTHEA_ERROR_HIP: return "HIP runtime error";
+    case ALTHEA_ERROR_NOT_FOUND: return "not found";
+    case ALTHEA_ERROR_IO: return "I/O error";
+    case ALTHEA_ERROR_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown status";
+}
+
+const char* althea_last_error(void) { return g_error; }
+
+int althea_version(void) { return 100; }
+
+althea_status althea_hammersley(uint32_t n, uint32_t first, uint32_t count, uint32_t* d_bits, float* d_xi,
+                                hipStream_t stream) {
+  ALTHEA_REQUIRE(n > 0, "hammersley: n must be positive");
+  ALTHEA_REQUIRE(d_bits || d_xi || count == 0, "hammersley: no output buffer");
+  return launch_hammersley(n, first, count, d_bits, d_xi, stream);
+}
+
+int althea_env_levels(int width, int height) {
+  if (width <= 0 || height <= 0) return 0;
+  return 1 + static_cast<int>(std::floor(std::log2(static_cast<double>(std::max(width, height)))));
+}
+
+size_t althea_env_pyramid_texels(int width, int height) {
+  if (width <= 0 || height <= 0) return 0;
+  return pyramid_geometry(width, height).texels;
+}
+
+althea_status althea_env_pyramid_build(const float* d_level0, int width, int height, float* d_pyramid,
+                                       hipStream_t stream) {
+  ALTHEA_REQUIRE(d_level0 && d_pyramid, "env_pyramid_build: null buffer");
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "env_pyramid_build: %dx%d is not a power-of-two extent", width, height);
+  return launch_pyramid(d_level0, width, height, d_pyramid, stream);
+}
+
+size_t althea_cube_texels(int size, int mips) {
+  if (size <= 0 || mips <= 0) return 0;
+  return cube_mip_offset(size, mips);
+}
+
+static bool cube_ok(int size, int mips) {
+  return is_pow2(size) && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1;
+}
+
+althea_status althea_cube_sample(const float* d_cube, int size, int mips, const float* d_dir_lod, int count,
+                                 float* d_out, hipStream_t stream) {
+  ALTHEA_REQUIRE(count >= 0, "cube_sample: negative count");
+  ALTHEA_REQUIRE(count == 0 || (d_cube && d_dir_lod && d_out), "cube_sample: null buffer");
+  ALTHEA_REQUIRE(cube_ok(size, mips), "cube_sample: size %d / mips %d unsupported", size, mips);
+  return launch_cube_sample(d_cube, size, mips, d_dir_lod, count, d_out, nullptr, stream);
+}
+
+althea_status althea_env_sample(const float* d_pyramid, int width, int height, const float* d_dir_lod, int count,
+                                float* d_out, hipStream_t stream) {
+  ALTHEA_REQUIRE(count >= 0, "env_sample: negative count");
+  ALTHEA_REQUIRE(count == 0 || (d_pyramid && d_dir_lod && d_out), "env_sample: null buffer");
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "env_sample: %dx%d is not a power-of-two extent", width, height);
+  return launch_env_sample(d_pyramid, width, height, d_dir_lod, count, d_out, nullptr, stream);
+}
+
+static bool samples_ok(int samples) { return samples >= 1 && samples <= (1 << 20); }
+
+althea_status althea_prefilter_cube(const float* d_pyramid, int env_width, int env_height, int size, int mips,
+                                    int samples, unsigned flags, float* d_cube, hipStream_t stream) {
+  ALTHEA_REQUIRE(d_pyramid && d_cube, "prefilter_cube: null buffer");
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_cube: %dx%d is not a power-of-two extent",
+                 env_width, env_height);
+  ALTHEA_REQUIRE(cube_ok(size, mips), "prefilter_cube: size %d / mips %d unsupported", size, mips);
+  ALTHEA_REQUIRE(samples_ok(samples), "prefilter_cube: samples %d out of range", samples);
+  Workspace ws(prefilter_workspace_bytes(env_width, env_height, size, mips, samples), stream);
+  if (!ws.ok()) return ALTHEA_ERROR_OUT_OF_MEMORY;
+  return launch_prefilter_cube(d_pyramid, env_width, env_height, size, mips, samples, flags, d_cube, ws.get(),
+                               stream);
+}
+
+althea_status althea_prefilter_points(const float* d_pyramid, int env_width, int env_height, int samples,
+                                      const float* d_dir_rough, int count, float* d_out, hipStream_t stream) {
+  ALTHEA_REQUIRE(count >= 0, "prefilter_points: negative count");
+  ALTHEA_REQUIRE(count == 0 || (d_pyramid && d_dir_rough && d_out), "prefilter_points: null buffer");
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_points: %dx%d is not a power-of-two extent",
+                 env_width, env_height);
+  ALTHEA_REQUIRE(samples_ok(samples), "prefilter_points: samples %d out of range", samples);
+  return launch_prefilter_points(d_pyramid, env_width, env_height, samples, d_dir_rough, count, d_out, nullptr,
+                                 stream);
+}
+
+althea_status althea_irradiance_cube(const float* d_pyramid, int env_width, int env_height, int size,
+                                     int theta_samples, unsigned flags, float* d_cube, hipStream_t stream) {
+  ALTHEA_REQUIRE(d_pyramid && d_cube, "irradiance_cube: null buffer");
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "irradiance_cube: %dx%d is not a power-of-two extent",
+                 env_width, env_height);
+  ALTHEA_REQUIRE(size >= 1 && size <= 4096, "irradiance_cube: size %d out of range", size);
+  ALTHEA_REQUIRE(theta_samples >= 1 && theta_samples <= 100000, "irradiance_cube: theta_samples %d out of range",
+                 theta_samples);
+  Workspace ws(irradiance_workspace_bytes(env_width, env_height, theta_samples), stream);
+  if (!ws.ok()) return ALTHEA_ERROR_OUT_OF_MEMORY;
+  return launch_irradiance(d_pyramid, env_width, env_height, size, theta_samples, flags, d_cube, ws.get(), stream);
+}
+
+althea_status althea_brdf_lut(int size, int samples, float* d_rg, hipStream_t stream) {
+  ALTHEA_REQUIRE(d_rg, "brdf_lut: null buffer");
+  ALTHEA_REQUIRE(size >= 1 && size <= 4096, "brdf_lut: size %d out of range", size);
+  ALTHEA_REQUIRE(samples_ok(samples), "brdf_lut: samples %d out of range", samples);
+  return launch_brdf_lut(size, samples, d_rg, stream);
+}
+
+// ------------------------------------------------------------- IBL handle
+void althea_ibl_desc_default(althea_ibl_desc* desc) {
+  if (!desc) return;
+  desc->irradiance_size = 32;
+  desc->irradiance_theta_samples = 300;
+  desc->prefiltered_size = 128;
+  desc->prefiltered_mips = 5;
+  desc->prefilter_samples = 1024;
+  desc->brdf_lut_size = 128;
+  desc->brdf_lut_samples = 1024;
+  desc->flags = 0;
+}
+
+althea_status althea_ibl_create(const althea_ibl_desc* desc, althea_ibl** out) {
+  ALTHEA_REQUIRE(desc && out, "ibl_create: null argument");
+  *out = nullptr;
+  ALTHEA_REQUIRE(desc->irradiance_size >= 1 && desc->irradiance_size <= 4096, "ibl_create: irradiance_size %d",
+                 desc->irradiance_size);
+  ALTHEA_REQUIRE(desc->irradiance_theta_samples >= 1, "ibl_create: irradiance_theta_samples %d",
+                 desc->irradiance_theta_samples);
+  ALTHEA_REQUIRE(cube_ok(desc->prefiltered_size, desc->prefiltered_mips), "ibl_create: prefiltered %d / %d mips",
+                 desc->prefiltered_size, desc->prefiltered_mips);
+  ALTHEA_REQUIRE(samples_ok(desc->prefilter_samples), "ibl_create: prefilter_samples %d", desc->prefilter_samples);
+  ALTHEA_REQUIRE(desc->brdf_lut_size >= 1 && desc->brdf_lut_size <= 4096, "ibl_create: brdf_lut_size %d",
+                 desc->brdf_lut_size);
+  ALTHEA_REQUIRE(samples_ok(desc->brdf_lut_samples), "ibl_create: brdf_lut_samples %d", desc->brdf_lut_samples);
+  althea_ibl* ibl = new (std::nothrow) althea_ibl;
+  if (!ibl) return ALTHEA_ERROR_OUT_OF_MEMORY;
+  ibl->desc = *desc;
+  size_t pref = sizeof(float) * 4 * althea_cube_texels(desc->prefiltered_size, desc->prefiltered_mips);
+  size_t irr = sizeof(float) * 4 * 6 * static_cast<size_t>(desc->irradiance_size) * desc->irradiance_size;
+  size_t lut = sizeof(float) * 2 * static_cast<size_t>(desc->brdf_lut_size) * desc->brdf_lut_size;
+  if (hipMalloc(&ibl->prefiltered, pref) != hipSuccess || hipMalloc(&ibl->irradiance, irr) != hipSuccess ||
+      hipMalloc(&ibl->lut, lut) != hipSuccess) {
+    set_error("ibl_create: device allocation failed");
+    althea_ibl_destroy(ibl);
+    return ALTHEA_ERROR_OUT_OF_MEMORY;
+  }
+  *out = ibl;
+  return ALTHEA_OK;
+}
+
+althea_status althea_ibl_destroy(althea_ibl* ibl) {
+  if (!ibl) return ALTHEA_OK;
+  (void)hipFree(ibl->env);
+  (void)hipFree(ibl->prefiltered);
+  (void)hipFree(ibl->irradiance);
+  (void)hipFree(ibl->lut);
+  (void)hipFree(ibl->workspace);
+  delete ibl;
+  return ALTHEA_OK;
+}
+
+althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, int width, int height,
+                                    hipStream_t stream) {
+  ALTHEA_REQUIRE(ibl && d_equirect, "ibl_precompute: null argument");
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_precompute: %dx%d is not a power-of-two extent", width, height);
+  const althea_ibl_desc& d = ibl->desc;
+  size_t texels = althea_env_pyramid_texels(width, height);
+  if (texels != ibl->env_texels) {
+    // The pyramid is reallocated only when the source extent changes.
+    ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(ibl->env);
+    ibl->env = nullptr;
+    ibl->env_texels = 0;
+    ALTHEA_TRY_HIP(hipMalloc(&ibl->env, sizeof(float) * 4 * texels));
+    ibl->env_texels = texels;
+  }
+  ibl->env_w = width;
+  ibl->env_h = height;
+  size_t ws_bytes = std::max(prefilter_workspace_bytes(width, height, d.prefiltered_size, d.prefiltered_mips,
+                                                       d.prefilter_samples),
+                             irradiance_workspace_bytes(width, height, d.irradiance_theta_samples));
+  if (ws_bytes > ibl->workspace_bytes) {
+    ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(ibl->workspace);
+    ibl->workspace = nullptr;
+    ibl->workspace_bytes = 0;
+    ALTHEA_TRY_HIP(hipMalloc(&ibl->workspace, ws_bytes));
+    ibl->workspace_bytes = ws_bytes;
+  }
+  althea_status st = launch_pyramid(d_equirect, width, height, ibl->env, stream);
+  if (st != ALTHEA_OK) return st;
+  st = launch_irradiance(ibl->env, width, height, d.irradiance_size, d.irradiance_theta_samples, d.flags,
+                         ibl->irradiance, ibl->workspace, stream);
+  if (st != ALTHEA_OK) return st;
+  st = launch_prefilter_cube(ibl->env, width, height, d.prefiltered_size, d.prefiltered_mips, d.prefilter_samples,
+                             d.flags, ibl->prefiltered, ibl->workspace, stream);
+  if (st != ALTHEA_OK) return st;
+  return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
+}
+
+althea_status althea_ibl_get_views(const althea_ibl* ibl, althea_ibl_views* v) {
+  ALTHEA_REQUIRE(ibl && v, "ibl_get_views: null argument");
+  v->environment = ibl->env;
+  v->env_width = ibl->env_w;
+  v->env_height = ibl->env_h;
+  v->env_levels = ibl->env ? althea_env_levels(ibl->env_w, ibl->env_h) : 0;
+  v->prefiltered = ibl->prefiltered;
+  v->prefiltered_size = ibl->desc.prefiltered_size;
+  v->prefiltered_mips = ibl->desc.prefiltered_mips;
+  v->irradiance = ibl->irradiance;
+  v->irradiance_size = ibl->desc.irradiance_size;
+  v->brdf_lut = ibl->lut;
+  v->brdf_lut_size = ibl->desc.brdf_lut_size;
+  return ALTHEA_OK;
+}
+
+size_t althea_ibl_state_bytes(const althea_ibl* ibl) {
+  if (!ibl) return 0;
+  const althea_ibl_desc& d = ibl->desc;
+  return sizeof(float) * 4 * althea_cube_texels(d.prefiltered_size, d.prefiltered_mips) +
+         sizeof(float) * 4 * 6 * static_cast<size_t>(d.irradiance_size) * d.irradiance_size +
+         sizeof(float) * 2 * static_cast<size_t>(d.brdf_lut_size) * d.brdf_lut_size;
+}
+
+// ---------------------------------------------------------------- resolve
+althea_status althea_deferred_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                      const althea_point_light* d_lights, int light_count,
+                                      const althea_globals* globals, const althea_ssao_desc* ssao, unsigned flags,
+                                      void* d_out, float* d_ao_out, hipStream_t stream) {
+  ALTHEA_REQUIRE(g && ibl && globals && d_out, "deferred_resolve: null argument");
+  ALTHEA_REQUIRE(g->position && g->normal && g->albedo && g->mro, "deferred_resolve: null G-buffer plane");
+  ALTHEA_REQUIRE(g->width >= 1 && g->height >= 1 && g->width <= 32768 && g->height <= 32768,
+                 "deferred_resolve: extent %dx%d", g->width, g->height);
+  ALTHEA_REQUIRE(g->format == ALTHEA_GBUFFER_FP32 || g->format == ALTHEA_GBUFFER_FP16,
+                 "deferred_resolve: G-buffer format %d", g->format);
+  ALTHEA_REQUIRE(ibl->environment && ibl->prefiltered && ibl->irradiance && ibl->brdf_lut,
+                 "deferred_resolve: incomplete IBL views");
+  ALTHEA_REQUIRE(env_dims_ok(ibl->env_width, ibl->env_height), "deferred_resolve: environment %dx%d",
+                 ibl->env_width, ibl->env_height);
+  ALTHEA_REQUIRE(cube_ok(ibl->prefiltered_size, ibl->prefiltered_mips), "deferred_resolve: prefiltered %d / %d",
+                 ibl->prefiltered_size, ibl->prefiltered_mips);
+  ALTHEA_REQUIRE(ibl->irradiance_size >= 1 && ibl->brdf_lut_size >= 1, "deferred_resolve: IBL sizes");
+  ALTHEA_REQUIRE(light_count >= 0 && (light_count == 0 || d_lights), "deferred_resolve: lights");
+  if (ssao && ssao->enabled)
+    ALTHEA_REQUIRE(ssao->samples >= 1 && ssao->samples <= ALTHEA_SSAO_MAX_SAMPLES && ssao->radius > 0.0f,
+                   "deferred_resolve: ssao samples %d radius %g", ssao->samples, ssao->radius);
+  return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,302 @@
+// Device-side math shared by the gfx950 kernels: Hammersley points, tangent
+// frames, the software equirect and seamless-cube texture filters, and the
+// GGX / Fresnel terms of the shading model.
+//
+// Texture filtering follows the public Vulkan texel-filtering rules the
+// reference's samplers use (Src/Sampler.cpp:90-125): coordinates u*W - 0.5,
+// floor, linear weights, REPEAT or CLAMP on integer texel indices, LINEAR
+// mipmapping between floor(lod) and floor(lod)+1.  Texel coordinates are formed
+// with one FMA from the angle (x = yaw * W/2pi + (W/2 - 0.5)) so FP32 rounding
+// is applied once, in texel units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace althea {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kInv2Pi = 0.159154943091895335769f;
+constexpr int kMaxLevels = 16;
+
+// ------------------------------------------------------------------ basics
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 mul(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ f3 lerp3(f3 a, f3 b, float t) {
+  return {fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z)};
+}
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+  return {fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z), fmaf(b.w - a.w, t, a.w)};
+}
+
+// -------------------------------------------------------------- Hammersley
+// Point i of an n-point set: (i/n correctly rounded, float(bitreverse(i)) * 2^-32).
+__device__ __forceinline__ uint32_t vdc_bits(uint32_t i) { return __builtin_bitreverse32(i); }
+__device__ __forceinline__ void hammersley(uint32_t i, uint32_t n, float& x1, float& x2) {
+  x1 = static_cast<float>(i) / static_cast<float>(n);
+  x2 = static_cast<float>(vdc_bits(i)) * 2.3283064365386963e-10f;
+}
+
+// ------------------------------------------------------------ tangent frame
+// coordinateSystem / LocalToWorld (Shaders/Misc/Sampling.glsl:58-70).  The
+// |x| > |y| branch is taken on the raw vector so every precision agrees on it.
+struct Frame {
+  f3 t, b, n;
+};
+__device__ __forceinline__ Frame local_to_world(f3 raw) {
+  Frame f;
+  f.n = normalize(raw);
+  f3 n = f.n;
+  if (fabsf(raw.x) > fabsf(raw.y)) {
+    float inv = 1.0f / sqrtf(fmaf(n.x, n.x, n.z * n.z));
+    f.t = mk(-n.z * inv, 0.0f, n.x * inv);
+  } else {
+    float inv = 1.0f / sqrtf(fmaf(n.y, n.y, n.z * n.z));
+    f.t = mk(0.0f, n.z * inv, -n.y * inv);
+  }
+  f.b = cross(n, f.t);
+  return f;
+}
+
+// -------------------------------------------------------- environment maps
+struct EnvLevels {
+  int levels;
+  int w[kMaxLevels], h[kMaxLevels];
+  uint32_t off[kMaxLevels];  // texel offset of each level in the pyramid
+};
+
+// Bilinear REPEAT fetch at texel coordinates (x, y) of a power-of-two level.
+__device__ __forceinline__ f3 env_bilinear_repeat(const float4* __restrict__ level, int w, int h, float x,
+                                                  float y) {
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  int x0 = ix & (w - 1), x1 = (ix + 1) & (w - 1);
+  int y0 = iy & (h - 1), y1 = (iy + 1) & (h - 1);
+  float4 t00 = level[y0 * w + x0];
+  float4 t10 = level[y0 * w + x1];
+  float4 t01 = level[y1 * w + x0];
+  float4 t11 = level[y1 * w + x1];
+  f3 top = lerp3(xyz(t00), xyz(t10), ax);
+  f3 bot = lerp3(xyz(t01), xyz(t11), ax);
+  return lerp3(top, bot, ay);
+}
+
+// Bilinear CLAMP_TO_EDGE fetch (the shading-side samplers).
+__device__ __forceinline__ f3 env_bilinear_clamp(const float4* __restrict__ level, int w, int h, float x,
+                                                 float y) {
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  int x0 = min(max(ix, 0), w - 1), x1 = min(max(ix + 1, 0), w - 1);
+  int y0 = min(max(iy, 0), h - 1), y1 = min(max(iy + 1, 0), h - 1);
+  float4 t00 = level[y0 * w + x0];
+  float4 t10 = level[y0 * w + x1];
+  float4 t01 = level[y1 * w + x0];
+  float4 t11 = level[y1 * w + x1];
+  f3 top = lerp3(xyz(t00), xyz(t10), ax);
+  f3 bot = lerp3(xyz(t01), xyz(t11), ax);
+  return lerp3(top, bot, ay);
+}
+
+// Precompute-side equirect angles (PreFilterEnvMap.comp:98-122): pole branch
+// at |xz| <= 0.001, yaw = atan2(z, x), pitch = atan(y / |xz|).  Returns the
+// angles; the caller turns them into texel coordinates per level.
+__device__ __forceinline__ void equirect_angles(f3 d, float& yaw, float& pitch) {
+  float len_xz = sqrtf(fmaf(d.x, d.x, d.z * d.z));
+  if (len_xz > 0.001f) {
+    yaw = atan2f(d.z, d.x);
+    pitch = atanf(d.y / len_xz);
+  } else {
+    yaw = 0.0f;
+    pitch = d.y > 0.0f ? 0.5f * kPi : -0.5f * kPi;
+  }
+}
+
+// Texel coordinates of (yaw, pitch) on a w x h level: u = yaw/2pi + 0.5,
+// v = 0.5 - pitch/pi, x = u*w - 0.5, y = v*h - 0.5.
+__device__ __forceinline__ float equirect_x(float yaw, int w) {
+  float fw = static_cast<float>(w);
+  return fmaf(yaw, fw * kInv2Pi, 0.5f * fw - 0.5f);
+}
+__device__ __forceinline__ float equirect_y(float pitch, int h) {
+  float fh = static_cast<float>(h);
+  return fmaf(pitch, -fh * kInvPi, 0.5f * fh - 0.5f);
+}
+
+// Trilinear REPEAT lookup of the pyramid at (yaw, pitch, lod) with lod
+// clamped to [0, levels-1] (maxLod = mipCount - 1, Src/Sampler.cpp:117).
+__device__ __forceinline__ f3 env_trilinear(const float4* __restrict__ pyr, const EnvLevels& lv, float yaw,
+                                            float pitch, float lod) {
+  float d = fminf(fmaxf(lod, 0.0f), static_cast<float>(lv.levels - 1));
+  float fl = floorf(d);
+  int l0 = static_cast<int>(fl);
+  int l1 = min(l0 + 1, lv.levels - 1);
+  float delta = d - fl;
+  f3 a = env_bilinear_repeat(pyr + lv.off[l0], lv.w[l0], lv.h[l0], equirect_x(yaw, lv.w[l0]),
+                             equirect_y(pitch, lv.h[l0]));
+  if (delta == 0.0f) return a;
+  f3 b = env_bilinear_repeat(pyr + lv.off[l1], lv.w[l1], lv.h[l1], equirect_x(yaw, lv.w[l1]),
+                             equirect_y(pitch, lv.h[l1]));
+  return lerp3(a, b, delta);
+}
+
+// -------------------------------------------------------------- cube maps
+// Faces +X,-X,+Y,-Y,+Z,-Z; (sc, tc, ma) per the Vulkan cube-face-selection
+// table.  Storage [mip][face][y][x] float4.
+__device__ __forceinline__ f3 cube_face_dir(int face, float sc, float tc) {
+  switch (face) {
+    case 0: return mk(1.0f, -tc, -sc);
+    case 1: return mk(-1.0f, -tc, sc);
+    case 2: return mk(sc, 1.0f, tc);
+    case 3: return mk(sc, -1.0f, -tc);
+    case 4: return mk(sc, -tc, 1.0f);
+    default: return mk(-sc, -tc, -1.0f);
+  }
+}
+// Raw direction of texel (x, y) of an S x S face (exact for power-of-two S).
+__device__ __forceinline__ f3 cube_texel_raw_dir(int face, int x, int y, int S) {
+  float inv = 1.0f / static_cast<float>(S);
+  float sc = static_cast<float>(2 * x + 1) * inv - 1.0f;
+  float tc = static_cast<float>(2 * y + 1) * inv - 1.0f;
+  return cube_face_dir(face, sc, tc);
+}
+
+// Major-axis selection: face and the projected (sc/ma, tc/ma) in [-1, 1].
+__device__ __forceinline__ void cube_select(f3 d, int& face, float& pc, float& pt) {
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  float sc, tc, ma;
+  if (ax >= ay && ax >= az) {
+    ma = ax;
+    if (d.x >= 0.0f) { face = 0; sc = -d.z; tc = -d.y; }
+    else { face = 1; sc = d.z; tc = -d.y; }
+  } else if (ay >= az) {
+    ma = ay;
+    if (d.y >= 0.0f) { face = 2; sc = d.x; tc = d.z; }
+    else { face = 3; sc = d.x; tc = -d.z; }
+  } else {
+    ma = az;
+    if (d.z >= 0.0f) { face = 4; sc = d.x; tc = -d.y; }
+    else { face = 5; sc = -d.x; tc = -d.y; }
+  }
+  pc = sc / ma;
+  pt = tc / ma;
+}
+
+// Seamless edge handling on the integer lattice of half-texel units: a tap
+// one texel beyond an edge is folded onto the neighbouring face.
+__device__ inline void cube_fold(int face, int i, int j, int S, int& fo, int& io, int& jo) {
+  int sc = 2 * i + 1 - S, tc = 2 * j + 1 - S;
+  int p[3];
+  switch (face) {
+    case 0: p[0] = S; p[1] = -tc; p[2] = -sc; break;
+    case 1: p[0] = -S; p[1] = -tc; p[2] = sc; break;
+    case 2: p[0] = sc; p[1] = S; p[2] = tc; break;
+    case 3: p[0] = sc; p[1] = -S; p[2] = -tc; break;
+    case 4: p[0] = sc; p[1] = -tc; p[2] = S; break;
+    default: p[0] = -sc; p[1] = -tc; p[2] = -S; break;
+  }
+  int major = (face >> 1);
+  int over = -1;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (abs(p[a]) == S + 1) over = a;
+  p[over] = p[over] > 0 ? S : -S;
+  p[major] = p[major] > 0 ? S - 1 : -(S - 1);
+  int nsc, ntc;
+  if (p[0] == S) { fo = 0; nsc = -p[2]; ntc = -p[1]; }
+  else if (p[0] == -S) { fo = 1; nsc = p[2]; ntc = -p[1]; }
+  else if (p[1] == S) { fo = 2; nsc = p[0]; ntc = p[2]; }
+  else if (p[1] == -S) { fo = 3; nsc = p[0]; ntc = -p[2]; }
+  else if (p[2] == S) { fo = 4; nsc = p[0]; ntc = -p[1]; }
+  else { fo = 5; nsc = -p[0]; ntc = -p[1]; }
+  io = (nsc + S - 1) >> 1;
+  jo = (ntc + S - 1) >> 1;
+}
+
+__device__ __forceinline__ f3 cube_texel(const float4* __restrict__ mip, int S, int face, int i, int j) {
+  return xyz(mip[(face * S + j) * S + i]);
+}
+
+__device__ inline f3 cube_tap(const float4* __restrict__ mip, int S, int face, int i, int j) {
+  bool in_i = static_cast<unsigned>(i) < static_cast<unsigned>(S);
+  bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(S);
+  if (in_i && in_j) return cube_texel(mip, S, face, i, j);
+  int f, a, b;
+  if (in_i || in_j) {
+    cube_fold(face, i, j, S, f, a, b);
+    return cube_texel(mip, S, f, a, b);
+  }
+  int ic = min(max(i, 0), S - 1), jc = min(max(j, 0), S - 1);
+  f3 acc = cube_texel(mip, S, face, ic, jc);
+  cube_fold(face, i, jc, S, f, a, b);
+  acc = acc + cube_texel(mip, S, f, a, b);
+  cube_fold(face, ic, j, S, f, a, b);
+  acc = acc + cube_texel(mip, S, f, a, b);
+  return acc * (1.0f / 3.0f);
+}
+
+__device__ inline f3 cube_bilinear(const float4* __restrict__ mip, int S, int face, float pc, float pt) {
+  float half = 0.5f * static_cast<float>(S);
+  float x = fmaf(pc, half, half - 0.5f), y = fmaf(pt, half, half - 0.5f);
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+  f3 t00, t10, t01, t11;
+  if (i0 >= 0 && j0 >= 0 && i0 + 1 < S && j0 + 1 < S) {
+    const float4* r0 = mip + (face * S + j0) * S + i0;
+    t00 = xyz(r0[0]);
+    t10 = xyz(r0[1]);
+    t01 = xyz(r0[S]);
+    t11 = xyz(r0[S + 1]);
+  } else {
+    t00 = cube_tap(mip, S, face, i0, j0);
+    t10 = cube_tap(mip, S, face, i0 + 1, j0);
+    t01 = cube_tap(mip, S, face, i0, j0 + 1);
+    t11 = cube_tap(mip, S, face, i0 + 1, j0 + 1);
+  }
+  f3 top = lerp3(t00, t10, ax), bot = lerp3(t01, t11, ax);
+  return lerp3(top, bot, ay);
+}
+
+struct CubeMips {
+  int size, mips;
+  uint32_t off[kMaxLevels];  // texel offset of each mip
+};
+
+__device__ inline f3 cube_sample(const float4* __restrict__ cube, const CubeMips& cm, f3 dir, float lod) {
+  int face;
+  float pc, pt;
+  cube_select(dir, face, pc, pt);
+  float d = fminf(fmaxf(lod, 0.0f), static_cast<float>(cm.mips - 1));
+  float fl = floorf(d);
+  int l0 = static_cast<int>(fl);
+  int l1 = min(l0 + 1, cm.mips - 1);
+  float delta = d - fl;
+  f3 a = cube_bilinear(cube + cm.off[l0], max(1, cm.size >> l0), face, pc, pt);
+  if (delta == 0.0f) return a;
+  f3 b = cube_bilinear(cube + cm.off[l1], max(1, cm.size >> l1), face, pc, pt);
+  return lerp3(a, b, delta);
+}
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace althea

@@ -1,0 +1,651 @@
+// IBL precompute kernels for gfx950: Hammersley, the environment pyramid, the
+// per-roughness GGX sample tables, the prefiltered cube, the irradiance cube,
+// the split-sum BRDF LUT, point prefiltering and sampler probes.
+//
+// The GGX prefilter (Shaders/IBL_Precompute/PreFilterEnvMap.comp:126-177)
+// evaluated at a cube texel with N = V = R has every per-sample quantity except
+// the frame rotation independent of the texel: with h the local half vector,
+// V.H = h.z, so L = frame * (2 h.z h.x, 2 h.z h.y, 2 h.z^2 - 1), NdotL = the z
+// term and the source LOD depends on h.z only.  A one-block kernel per
+// roughness builds these entries in FP64 (rounded once to FP32), drops samples
+// with NdotL <= 0 in sample order, and sums the weights in FP64.  The texel
+// kernel then reads one wave-uniform entry per sample (scalar loads), rotates
+// it by its frame and filters the source in software.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "device_math.hpp"
+#include "internal.hpp"
+
+namespace althea {
+
+// -------------------------------------------------------------- Hammersley
+__global__ void hammersley_kernel(uint32_t n, uint32_t first, uint32_t count, uint32_t* __restrict__ bits,
+                                  float2* __restrict__ xi) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  uint32_t i = first + k;
+  if (bits) bits[k] = vdc_bits(i);
+  if (xi) {
+    float x1, x2;
+    hammersley(i, n, x1, x2);
+    xi[k] = make_float2(x1, x2);
+  }
+}
+
+althea_status launch_hammersley(uint32_t n, uint32_t first, uint32_t count, uint32_t* bits, float* xi,
+                                hipStream_t s) {
+  if (count == 0) return ALTHEA_OK;
+  dim3 grid((count + 255) / 256);
+  hipLaunchKernelGGL(hammersley_kernel, grid, dim3(256), 0, s, n, first, count, bits,
+                     reinterpret_cast<float2*>(xi));
+  return check_launch("hammersley_kernel");
+}
+
+// ----------------------------------------------------------------- pyramid
+// Level l = mean of the 2x2 (2x1 / 1x2 at unit extents) block of level l-1,
+// the LINEAR half-size blit of Image::generateMipMaps (Src/Image.cpp:135-239).
+__global__ void pyramid_level_kernel(const float4* __restrict__ src, int sw, int sh, float4* __restrict__ dst,
+                                     int dw, int dh) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= dw) return;
+  int fx = sw > dw ? 2 : 1, fy = sh > dh ? 2 : 1;
+  const float4* r0 = src + (size_t)(y * fy) * sw + x * fx;
+  float4 a = r0[0];
+  float4 acc = a;
+  float scale = 1.0f;
+  if (fx == 2 && fy == 2) {
+    float4 b = r0[1], c = r0[sw], d = r0[sw + 1];
+    acc = make_float4(((a.x + b.x) + (c.x + d.x)), ((a.y + b.y) + (c.y + d.y)), ((a.z + b.z) + (c.z + d.z)),
+                      ((a.w + b.w) + (c.w + d.w)));
+    scale = 0.25f;
+  } else if (fx == 2) {
+    float4 b = r0[1];
+    acc = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+    scale = 0.5f;
+  } else if (fy == 2) {
+    float4 b = r0[sw];
+    acc = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+    scale = 0.5f;
+  }
+  dst[(size_t)y * dw + x] = make_float4(acc.x * scale, acc.y * scale, acc.z * scale, acc.w * scale);
+}
+
+althea_status launch_pyramid(const float* level0, int w, int h, float* pyr, hipStream_t s) {
+  PyramidGeometry g = pyramid_geometry(w, h);
+  if (level0 != pyr)
+    ALTHEA_TRY_HIP(hipMemcpyAsync(pyr, level0, sizeof(float4) * (size_t)w * h, hipMemcpyDeviceToDevice, s));
+  float4* p = reinterpret_cast<float4*>(pyr);
+  for (int l = 1; l < g.levels; ++l) {
+    dim3 grid((g.w[l] + 255) / 256, g.h[l]);
+    hipLaunchKernelGGL(pyramid_level_kernel, grid, dim3(256), 0, s, p + g.off[l - 1], g.w[l - 1], g.h[l - 1],
+                       p + g.off[l], g.w[l], g.h[l]);
+    althea_status st = check_launch("pyramid_level_kernel");
+    if (st != ALTHEA_OK) return st;
+  }
+  return ALTHEA_OK;
+}
+
+// ------------------------------------------------------- GGX sample tables
+// One block per roughness level.  Entry e (in sample order, NdotL > 0 only):
+// float4 (L.t, L.b, L.n, lod) in the frame (T, B, N), lod clamped to the
+// pyramid's [0, levels-1].  counts[m] = kept entries, wsum[m] = FP64 sum of
+// L.n, wsum32[m] = the same sum accumulated in FP32 in sample order.
+struct TableParams {
+  int samples;
+  int mips;
+  int env_levels;
+  double sa_texel;  // 4 pi / (6 W H)  (PreFilterEnvMap.comp:158)
+};
+
+constexpr int kTableThreads = 256;
+
+__global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p, float4* __restrict__ tables,
+                                                                  int* __restrict__ counts,
+                                                                  double* __restrict__ wsum,
+                                                                  float* __restrict__ wsum32) {
+  const int m = blockIdx.x;
+  const int tid = threadIdx.x;
+  const double rough = p.mips > 1 ? static_cast<double>(m) / (p.mips - 1) : 0.0;
+  const double a2 = rough * rough;  // alpha = roughness (PreFilterEnvMap.comp:139-140)
+  const double two_pi = 6.283185307179586476925;
+  __shared__ int s_scan[kTableThreads];
+  __shared__ double s_sum[kTableThreads];
+  __shared__ float s_keep_w[kTableThreads];
+  __shared__ int s_base;
+  __shared__ float s_w32;
+  if (tid == 0) {
+    s_base = 0;
+    s_w32 = 0.0f;
+  }
+  double my_sum = 0.0;
+  float4* tab = tables + static_cast<size_t>(m) * p.samples;
+  __syncthreads();
+  for (int chunk = 0; chunk < p.samples; chunk += kTableThreads) {
+    int s = chunk + tid;
+    bool keep = false;
+    float4 entry = make_float4(0.f, 0.f, 0.f, 0.f);
+    float w32 = 0.0f;
+    if (s < p.samples) {
+      double x1 = static_cast<double>(s) / p.samples;
+      double x2 = static_cast<double>(__builtin_bitreverse32(static_cast<uint32_t>(s))) * (1.0 / 4294967296.0);
+      double den = 1.0 + (a2 - 1.0) * x2;
+      double cos_t = sqrt((1.0 - x2) / den);
+      double sin_t = sqrt(a2 * x2 / den);  // = sqrt(1 - cos_t^2) without cancellation
+      double phi = two_pi * x1;
+      double hx = cos(phi) * sin_t, hy = sin(phi) * sin_t, hz = cos_t;
+      double lz = 2.0 * hz * hz - 1.0;
+      if (lz > 0.0) {
+        keep = true;
+        double lod = 0.0;
+        if (rough != 0.0) {
+          double t = hz * hz * (a2 - 1.0) + 1.0;
+          double D = a2 / (3.14159265358979323846 * t * t);
+          double pdf = D * hz / (4.0 * hz + 0.00001);
+          double sa_sample = 1.0 / (p.samples * pdf + 0.0001);
+          lod = 0.5 * log2(sa_sample / p.sa_texel);
+        }
+        lod = fmin(fmax(lod, 0.0), static_cast<double>(p.env_levels - 1));
+        entry = make_float4(static_cast<float>(2.0 * hz * hx), static_cast<float>(2.0 * hz * hy),
+                            static_cast<float>(lz), static_cast<float>(lod));
+        my_sum += lz;
+        w32 = static_cast<float>(lz);
+      }
+    }
+    // Block-wide exclusive scan of keep flags (Hillis-Steele in LDS).
+    s_scan[tid] = keep ? 1 : 0;
+    s_keep_w[tid] = w32;
+    __syncthreads();
+    for (int off = 1; off < kTableThreads; off <<= 1) {
+      int v = tid >= off ? s_scan[tid - off] : 0;
+      __syncthreads();
+      s_scan[tid] += v;
+      __syncthreads();
+    }
+    int incl = s_scan[tid];
+    if (keep) tab[s_base + incl - 1] = entry;
+    if (tid == 0) {
+      // FP32 weight sum in sample order, the reference's totalWeight += NdotL.
+      float acc = s_w32;
+      for (int k = 0; k < kTableThreads; ++k) acc += s_keep_w[k];
+      s_w32 = acc;
+    }
+    __syncthreads();
+    if (tid == kTableThreads - 1) s_base += incl;
+    __syncthreads();
+  }
+  s_sum[tid] = my_sum;
+  __syncthreads();
+  for (int off = kTableThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) s_sum[tid] += s_sum[tid + off];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    counts[m] = s_base;
+    wsum[m] = s_sum[0];
+    wsum32[m] = s_w32;
+  }
+}
+
+// ------------------------------------------------------------ prefilter
+struct PrefilterParams {
+  EnvLevels env;
+  CubeMips cube;
+  int samples;
+  int tile_start[kMaxLevels + 1];  // 8x8-texel tiles before each mip
+};
+
+constexpr int kPrefilterTile = 8;
+constexpr int kBlockSums = 32;  // FP32 partial sums flushed to FP64 every 32 samples
+
+template <bool kF64Sums>
+__global__ __launch_bounds__(64) void prefilter_cube_kernel(const float4* __restrict__ pyr,
+                                                            const float4* __restrict__ tables,
+                                                            const int* __restrict__ counts,
+                                                            const double* __restrict__ wsum,
+                                                            const PrefilterParams p, float4* __restrict__ cube) {
+  const int b = blockIdx.x;
+  int m = 0;
+  while (m + 1 < p.cube.mips && b >= p.tile_start[m + 1]) ++m;
+  const int S = max(1, p.cube.size >> m);
+  const int tpf = (S + kPrefilterTile - 1) / kPrefilterTile;
+  const int local = b - p.tile_start[m];
+  const int face = local / (tpf * tpf);
+  const int t = local - face * tpf * tpf;
+  const int x = (t % tpf) * kPrefilterTile + (threadIdx.x & 7);
+  const int y = (t / tpf) * kPrefilterTile + (threadIdx.x >> 3);
+  const bool active = x < S && y < S;
+  const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+
+  const float4* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
+  const int count = counts[m];
+  double ar = 0.0, ag = 0.0, ab = 0.0;
+  float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+  for (int e = 0; e < count; ++e) {
+    const float4 en = tab[e];
+    f3 L = mk(fmaf(fr.t.x, en.x, fmaf(fr.b.x, en.y, fr.n.x * en.z)),
+              fmaf(fr.t.y, en.x, fmaf(fr.b.y, en.y, fr.n.y * en.z)),
+              fmaf(fr.t.z, en.x, fmaf(fr.b.z, en.y, fr.n.z * en.z)));
+    float yaw, pitch;
+    equirect_angles(L, yaw, pitch);
+    f3 v = env_trilinear(pyr, p.env, yaw, pitch, en.w);
+    sr = fmaf(v.x, en.z, sr);
+    sg = fmaf(v.y, en.z, sg);
+    sb = fmaf(v.z, en.z, sb);
+    if (kF64Sums) {
+      if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
+        ar += sr; ag += sg; ab += sb;
+        sr = sg = sb = 0.0f;
+      }
+    } else {
+      sw += en.z;
+    }
+  }
+  float4 out;
+  if (kF64Sums) {
+    ar += sr; ag += sg; ab += sb;
+    const double inv = 1.0 / wsum[m];
+    out = make_float4(static_cast<float>(ar * inv), static_cast<float>(ag * inv), static_cast<float>(ab * inv), 1.0f);
+  } else {
+    out = make_float4(sr / sw, sg / sw, sb / sw, 1.0f);
+  }
+  if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
+}
+
+static EnvLevels env_levels_of(int w, int h) {
+  PyramidGeometry g = pyramid_geometry(w, h);
+  EnvLevels e{};
+  e.levels = g.levels;
+  for (int l = 0; l < g.levels; ++l) {
+    e.w[l] = g.w[l];
+    e.h[l] = g.h[l];
+    e.off[l] = g.off[l];
+  }
+  return e;
+}
+
+static CubeMips cube_mips_of(int size, int mips) {
+  CubeMips c{};
+  c.size = size;
+  c.mips = mips;
+  for (int m = 0; m < mips; ++m) c.off[m] = static_cast<uint32_t>(cube_mip_offset(size, m));
+  return c;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+size_t prefilter_workspace_bytes(int, int, int, int mips, int samples) {
+  return align256(sizeof(float4) * static_cast<size_t>(mips) * samples) + align256(sizeof(int) * mips) +
+         align256(sizeof(double) * mips) + align256(sizeof(float) * mips);
+}
+
+althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,
+                                    unsigned flags, float* cube, void* workspace, hipStream_t s) {
+  char* ws = static_cast<char*>(workspace);
+  float4* tables = reinterpret_cast<float4*>(ws);
+  ws += align256(sizeof(float4) * static_cast<size_t>(mips) * samples);
+  int* counts = reinterpret_cast<int*>(ws);
+  ws += align256(sizeof(int) * mips);
+  double* wsum = reinterpret_cast<double*>(ws);
+  ws += align256(sizeof(double) * mips);
+  float* wsum32 = reinterpret_cast<float*>(ws);
+
+  TableParams tp;
+  tp.samples = samples;
+  tp.mips = mips;
+  tp.env_levels = althea_env_levels(env_w, env_h);
+  tp.sa_texel = 4.0 * M_PI / (6.0 * static_cast<double>(env_w) * env_h);
+  hipLaunchKernelGGL(ggx_table_kernel, dim3(mips), dim3(kTableThreads), 0, s, tp, tables, counts, wsum, wsum32);
+  althea_status st = check_launch("ggx_table_kernel");
+  if (st != ALTHEA_OK) return st;
+
+  PrefilterParams p{};
+  p.env = env_levels_of(env_w, env_h);
+  p.cube = cube_mips_of(size, mips);
+  p.samples = samples;
+  int tiles = 0;
+  for (int m = 0; m < mips; ++m) {
+    p.tile_start[m] = tiles;
+    int S = std::max(1, size >> m);
+    int tpf = (S + kPrefilterTile - 1) / kPrefilterTile;
+    tiles += 6 * tpf * tpf;
+  }
+  p.tile_start[mips] = tiles;
+  const float4* pyr4 = reinterpret_cast<const float4*>(pyr);
+  float4* cube4 = reinterpret_cast<float4*>(cube);
+  if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
+    hipLaunchKernelGGL(prefilter_cube_kernel<false>, dim3(tiles), dim3(64), 0, s, pyr4, tables, counts, wsum, p,
+                       cube4);
+  else
+    hipLaunchKernelGGL(prefilter_cube_kernel<true>, dim3(tiles), dim3(64), 0, s, pyr4, tables, counts, wsum, p,
+                       cube4);
+  return check_launch("prefilter_cube_kernel");
+}
+
+// -------------------------------------------------------- point prefilter
+// Same estimator at per-lane (direction, roughness): the sample quantities are
+// evaluated per lane in FP32 (sin^2 theta formed without cancellation), sums
+// in FP32 blocks flushed to FP64.
+struct PointParams {
+  EnvLevels env;
+  int samples;
+  int count;
+  float log2_sa_texel;
+};
+
+__global__ __launch_bounds__(256) void prefilter_points_kernel(const float4* __restrict__ pyr, PointParams p,
+                                                               const float4* __restrict__ dir_rough,
+                                                               float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.count) return;
+  const float4 q = dir_rough[i];
+  const Frame fr = local_to_world(mk(q.x, q.y, q.z));
+  const float rough = q.w;
+  const float a2 = rough * rough;
+  const float fn = static_cast<float>(p.samples);
+  const float max_lod = static_cast<float>(p.env.levels - 1);
+  double ar = 0.0, ag = 0.0, ab = 0.0, aw = 0.0;
+  float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+  for (int s = 0; s < p.samples; ++s) {
+    float x1, x2;
+    hammersley(static_cast<uint32_t>(s), static_cast<uint32_t>(p.samples), x1, x2);
+    const float den = fmaf(a2 - 1.0f, x2, 1.0f);
+    const float cos_t = sqrtf((1.0f - x2) / den);
+    const float sin_t = sqrtf(a2 * x2 / den);
+    float sp, cp;
+    sincospif(2.0f * x1, &sp, &cp);
+    const float hz = cos_t;
+    const float lz = fmaf(2.0f * hz, hz, -1.0f);
+    if (lz > 0.0f) {
+      const float lx = 2.0f * hz * (cp * sin_t), ly = 2.0f * hz * (sp * sin_t);
+      float lod = 0.0f;
+      if (rough != 0.0f) {
+        const float t = fmaf(hz * hz, a2 - 1.0f, 1.0f);
+        const float D = a2 / (kPi * t * t);
+        const float pdf = D * hz / fmaf(4.0f, hz, 0.00001f);
+        lod = 0.5f * (-log2f(fmaf(fn, pdf, 0.0001f)) - p.log2_sa_texel);
+      }
+      lod = fminf(fmaxf(lod, 0.0f), max_lod);
+      f3 L = fr.t * lx + fr.b * ly + fr.n * lz;
+      float yaw, pitch;
+      equirect_angles(L, yaw, pitch);
+      f3 v = env_trilinear(pyr, p.env, yaw, pitch, lod);
+      sr = fmaf(v.x, lz, sr);
+      sg = fmaf(v.y, lz, sg);
+      sb = fmaf(v.z, lz, sb);
+      sw += lz;
+    }
+    if ((s & (kBlockSums - 1)) == kBlockSums - 1) {
+      ar += sr; ag += sg; ab += sb; aw += sw;
+      sr = sg = sb = sw = 0.0f;
+    }
+  }
+  ar += sr; ag += sg; ab += sb; aw += sw;
+  const double inv = 1.0 / aw;
+  out[i] = make_float4(static_cast<float>(ar * inv), static_cast<float>(ag * inv), static_cast<float>(ab * inv),
+                       1.0f);
+}
+
+althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, int samples, const float* dir_rough,
+                                      int count, float* out, void*, hipStream_t s) {
+  if (count == 0) return ALTHEA_OK;
+  PointParams p{};
+  p.env = env_levels_of(env_w, env_h);
+  p.samples = samples;
+  p.count = count;
+  p.log2_sa_texel = static_cast<float>(std::log2(4.0 * M_PI / (6.0 * static_cast<double>(env_w) * env_h)));
+  hipLaunchKernelGGL(prefilter_points_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(pyr), p, reinterpret_cast<const float4*>(dir_rough),
+                     reinterpret_cast<float4*>(out));
+  return check_launch("prefilter_points_kernel");
+}
+
+// ------------------------------------------------------------ irradiance
+// GenIrradianceMap.comp:106-155 per cube texel.  A 256-thread block owns a
+// 4x4 texel tile; lane = (texel 0..15, theta slice 0..15).  Slice k walks
+// theta_i for i = k, k+16, ... with the phi loop wave-uniform (scalar table
+// loads); per-theta FP32 sums are flushed into FP64, then reduced over the
+// four slices of a wave with shuffles and over the four waves in LDS.
+struct IrradianceParams {
+  EnvLevels env;
+  int size;
+  int theta_samples;
+  int phi_samples;
+  int l0, l1;
+  float delta;
+  double scale;  // pi / theta_samples / phi_samples
+};
+
+__global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float2* __restrict__ theta_tab,
+                                         float4* __restrict__ phi_tab) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double pi = 3.14159265358979323846;
+  if (i < theta_samples) {
+    double th = i * 2.0 * pi / theta_samples;
+    theta_tab[i] = make_float2(static_cast<float>(cos(th)), static_cast<float>(sin(th)));
+  }
+  if (i < phi_samples) {
+    double ph = i * 0.5 * pi / phi_samples;
+    double c = cos(ph), s = sin(ph);
+    phi_tab[i] = make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(c * s), 0.0f);
+  }
+}
+
+template <bool kF64Sums>
+__global__ __launch_bounds__(256) void irradiance_cube_kernel(const float4* __restrict__ pyr,
+                                                              const float2* __restrict__ theta_tab,
+                                                              const float4* __restrict__ phi_tab,
+                                                              const IrradianceParams p, float4* __restrict__ cube) {
+  const int S = p.size;
+  const int tpf = (S + 3) / 4;
+  const int face = blockIdx.x / (tpf * tpf);
+  const int t = blockIdx.x - face * tpf * tpf;
+  const int lane_texel = threadIdx.x & 15;
+  const int slice = threadIdx.x >> 4;
+  const int x = (t % tpf) * 4 + (lane_texel & 3);
+  const int y = (t / tpf) * 4 + (lane_texel >> 2);
+  const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+
+  const float4* lv0 = pyr + p.env.off[p.l0];
+  const float4* lv1 = pyr + p.env.off[p.l1];
+  const int w0 = p.env.w[p.l0], h0 = p.env.h[p.l0], w1 = p.env.w[p.l1], h1 = p.env.h[p.l1];
+  double ar = 0.0, ag = 0.0, ab = 0.0;
+  float fr32 = 0.0f, fg32 = 0.0f, fb32 = 0.0f;
+  for (int i = slice; i < p.theta_samples; i += 16) {
+    const float2 th = theta_tab[i];
+    const f3 a = fr.t * th.x + fr.b * th.y;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+    for (int j = 0; j < p.phi_samples; ++j) {
+      const float4 ph = phi_tab[j];
+      const f3 d = mk(fmaf(a.x, ph.y, fr.n.x * ph.x), fmaf(a.y, ph.y, fr.n.y * ph.x),
+                      fmaf(a.z, ph.y, fr.n.z * ph.x));
+      float yaw, pitch;
+      equirect_angles(d, yaw, pitch);
+      f3 v = env_bilinear_repeat(lv0, w0, h0, equirect_x(yaw, w0), equirect_y(pitch, h0));
+      if (p.delta != 0.0f) {
+        f3 v1 = env_bilinear_repeat(lv1, w1, h1, equirect_x(yaw, w1), equirect_y(pitch, h1));
+        v = lerp3(v, v1, p.delta);
+      }
+      sr = fmaf(v.x, ph.z, sr);
+      sg = fmaf(v.y, ph.z, sg);
+      sb = fmaf(v.z, ph.z, sb);
+    }
+    if (kF64Sums) {
+      ar += sr; ag += sg; ab += sb;
+    } else {
+      fr32 += sr; fg32 += sg; fb32 += sb;
+    }
+  }
+  if (!kF64Sums) {
+    ar = fr32; ag = fg32; ab = fb32;
+  }
+  // Reduce the four theta slices of this wave (lanes l, l^16, l^32, l^48).
+  ar += __shfl_xor(ar, 16, 64); ag += __shfl_xor(ag, 16, 64); ab += __shfl_xor(ab, 16, 64);
+  ar += __shfl_xor(ar, 32, 64); ag += __shfl_xor(ag, 32, 64); ab += __shfl_xor(ab, 32, 64);
+  __shared__ double s_red[4][16][3];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) < 16) {
+    s_red[wave][lane_texel][0] = ar;
+    s_red[wave][lane_texel][1] = ag;
+    s_red[wave][lane_texel][2] = ab;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16 && x < S && y < S) {
+    double r = 0.0, g = 0.0, bl = 0.0;
+    for (int w = 0; w < 4; ++w) {
+      r += s_red[w][lane_texel][0];
+      g += s_red[w][lane_texel][1];
+      bl += s_red[w][lane_texel][2];
+    }
+    cube[(static_cast<size_t>(face) * S + y) * S + x] =
+        make_float4(static_cast<float>(r * p.scale), static_cast<float>(g * p.scale),
+                    static_cast<float>(bl * p.scale), 1.0f);
+  }
+}
+
+static int irradiance_phi_samples(int w, int h, int theta) {
+  // int(pushConstants.height * thetaSamples / pushConstants.width) in FP32
+  // (GenIrradianceMap.comp:135-136).
+  return static_cast<int>(static_cast<float>(h) * static_cast<float>(theta) / static_cast<float>(w));
+}
+
+size_t irradiance_workspace_bytes(int w, int h, int theta) {
+  int phi = irradiance_phi_samples(w, h, theta);
+  return align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
+}
+
+althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
+                                unsigned flags, float* cube, void* workspace, hipStream_t s) {
+  const int phi = irradiance_phi_samples(env_w, env_h, theta_samples);
+  ALTHEA_REQUIRE(phi >= 1, "irradiance: int(H*theta/W) = %d phi samples", phi);
+  char* ws = static_cast<char*>(workspace);
+  float2* theta_tab = reinterpret_cast<float2*>(ws);
+  float4* phi_tab = reinterpret_cast<float4*>(ws + align256(sizeof(float2) * theta_samples));
+  int n = std::max(theta_samples, phi);
+  hipLaunchKernelGGL(irradiance_tables_kernel, dim3((n + 255) / 256), dim3(256), 0, s, theta_samples, phi,
+                     theta_tab, phi_tab);
+  althea_status st = check_launch("irradiance_tables_kernel");
+  if (st != ALTHEA_OK) return st;
+
+  IrradianceParams p{};
+  p.env = env_levels_of(env_w, env_h);
+  p.size = size;
+  p.theta_samples = theta_samples;
+  p.phi_samples = phi;
+  // Source LOD log2(W / thetaSamples) (GenIrradianceMap.comp:139) in FP32 as
+  // the shader computes it, clamped to the pyramid.
+  float lod = std::log2(static_cast<float>(env_w) / static_cast<float>(theta_samples));
+  lod = std::min(std::max(lod, 0.0f), static_cast<float>(p.env.levels - 1));
+  p.l0 = static_cast<int>(std::floor(lod));
+  p.l1 = std::min(p.l0 + 1, p.env.levels - 1);
+  p.delta = lod - static_cast<float>(p.l0);
+  p.scale = M_PI / theta_samples / phi;
+  int tpf = (size + 3) / 4;
+  dim3 grid(6 * tpf * tpf);
+  const float4* pyr4 = reinterpret_cast<const float4*>(pyr);
+  float4* cube4 = reinterpret_cast<float4*>(cube);
+  if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
+    hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+  else
+    hipLaunchKernelGGL(irradiance_cube_kernel<true>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+  return check_launch("irradiance_cube_kernel");
+}
+
+// --------------------------------------------------------------- BRDF LUT
+// Karis split-sum (A, B) at texel centre (NdotV, roughness); a 64-lane wave
+// covers 64 NdotV columns of one roughness row so the sample quantities are
+// wave-uniform.  FP32 per-sample arithmetic, FP64 sums and normalisation.
+__global__ __launch_bounds__(64) void brdf_lut_kernel(int size, int samples, float2* __restrict__ rg) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y;
+  const float nv = (static_cast<float>(x) + 0.5f) / static_cast<float>(size);
+  const float r = (static_cast<float>(y) + 0.5f) / static_cast<float>(size);
+  const float alpha = r * r;
+  const float a2 = alpha * alpha;
+  const float k = 0.5f * alpha;
+  const float vx = sqrtf(fmaxf(0.0f, fmaf(-nv, nv, 1.0f)));
+  const float gv = nv / fmaf(nv, 1.0f - k, k);
+  double sa = 0.0, sb = 0.0;
+  float fa = 0.0f, fb = 0.0f;
+  for (int s = 0; s < samples; ++s) {
+    float x1, x2;
+    hammersley(static_cast<uint32_t>(s), static_cast<uint32_t>(samples), x1, x2);
+    const float den = fmaf(a2 - 1.0f, x2, 1.0f);
+    const float cos_t = sqrtf((1.0f - x2) / den);
+    const float sin_t = sqrtf(a2 * x2 / den);
+    float sp, cp;
+    sincospif(2.0f * x1, &sp, &cp);
+    const float hx = cp * sin_t, hz = cos_t;
+    const float vdh = fmaf(vx, hx, nv * hz);
+    const float ndl = fmaf(2.0f * vdh, hz, -nv);
+    if (ndl > 0.0f) {
+      const float gl = ndl / fmaf(ndl, 1.0f - k, k);
+      const float g_vis = gv * gl * vdh / (hz * nv);
+      const float om = 1.0f - vdh;
+      const float om2 = om * om;
+      const float fc = om2 * om2 * om;
+      fa = fmaf(1.0f - fc, g_vis, fa);
+      fb = fmaf(fc, g_vis, fb);
+    }
+    if ((s & (kBlockSums - 1)) == kBlockSums - 1) {
+      sa += fa;
+      sb += fb;
+      fa = fb = 0.0f;
+    }
+  }
+  sa += fa;
+  sb += fb;
+  if (x < size) rg[y * size + x] = make_float2(static_cast<float>(sa / samples), static_cast<float>(sb / samples));
+}
+
+althea_status launch_brdf_lut(int size, int samples, float* rg, hipStream_t s) {
+  dim3 grid((size + 63) / 64, size);
+  hipLaunchKernelGGL(brdf_lut_kernel, grid, dim3(64), 0, s, size, samples, reinterpret_cast<float2*>(rg));
+  return check_launch("brdf_lut_kernel");
+}
+
+// ---------------------------------------------------------- sampler probes
+__global__ void cube_sample_kernel(const float4* __restrict__ cube, const CubeMips cm,
+                                   const float4* __restrict__ dir_lod, int count, float4* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float4 q = dir_lod[i];
+  f3 v = cube_sample(cube, cm, mk(q.x, q.y, q.z), q.w);
+  out[i] = make_float4(v.x, v.y, v.z, 1.0f);
+}
+
+althea_status launch_cube_sample(const float* cube, int size, int mips, const float* dir_lod, int count,
+                                 float* out, void*, hipStream_t s) {
+  if (count == 0) return ALTHEA_OK;
+  CubeMips cm = cube_mips_of(size, mips);
+  hipLaunchKernelGGL(cube_sample_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(cube), cm, reinterpret_cast<const float4*>(dir_lod), count,
+                     reinterpret_cast<float4*>(out));
+  return check_launch("cube_sample_kernel");
+}
+
+__global__ void env_sample_kernel(const float4* __restrict__ pyr, const EnvLevels lv,
+                                  const float4* __restrict__ dir_lod, int count, float4* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float4 q = dir_lod[i];
+  float yaw, pitch;
+  equirect_angles(mk(q.x, q.y, q.z), yaw, pitch);
+  f3 v = env_trilinear(pyr, lv, yaw, pitch, q.w);
+  out[i] = make_float4(v.x, v.y, v.z, 1.0f);
+}
+
+althea_status launch_env_sample(const float* pyr, int w, int h, const float* dir_lod, int count, float* out, void*,
+                                hipStream_t s) {
+  if (count == 0) return ALTHEA_OK;
+  EnvLevels lv = env_levels_of(w, h);
+  hipLaunchKernelGGL(env_sample_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(pyr), lv, reinterpret_cast<const float4*>(dir_lod), count,
+                     reinterpret_cast<float4*>(out));
+  return check_launch("env_sample_kernel");
+}
+
+}  // namespace althea

@@ -1,0 +1,75 @@
+// Host-side internals shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#include "althea/althea.h"
+
+namespace althea {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define ALTHEA_TRY_HIP(expr)                                                       \
+  do {                                                                             \
+    hipError_t althea_e_ = (expr);                                                 \
+    if (althea_e_ != hipSuccess) {                                                 \
+      ::althea::set_error("%s failed: %s", #expr, hipGetErrorString(althea_e_));    \
+      return ALTHEA_ERROR_HIP;                                                     \
+    }                                                                              \
+  } while (0)
+
+#define ALTHEA_REQUIRE(cond, ...)                 \
+  do {                                            \
+    if (!(cond)) {                                \
+      ::althea::set_error(__VA_ARGS__);           \
+      return ALTHEA_ERROR_INVALID_ARGUMENT;       \
+    }                                             \
+  } while (0)
+
+inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Launch-error check after a kernel launch.
+althea_status check_launch(const char* what);
+
+// Pyramid geometry: per level (w, h, texel offset) plus the total texel count.
+struct PyramidGeometry {
+  int levels;
+  int w[16], h[16];
+  uint32_t off[16];
+  size_t texels;
+};
+PyramidGeometry pyramid_geometry(int width, int height);
+
+// Cube mip offsets (texels).
+size_t cube_mip_offset(int size, int mip);
+
+// Kernel launchers (ibl_kernels.hip).
+althea_status launch_hammersley(uint32_t n, uint32_t first, uint32_t count, uint32_t* bits, float* xi,
+                                hipStream_t s);
+althea_status launch_pyramid(const float* level0, int w, int h, float* pyr, hipStream_t s);
+althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,
+                                    unsigned flags, float* cube, void* workspace, hipStream_t s);
+size_t prefilter_workspace_bytes(int env_w, int env_h, int size, int mips, int samples);
+althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, int samples,
+                                      const float* dir_rough, int count, float* out, void* workspace,
+                                      hipStream_t s);
+althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
+                                unsigned flags, float* cube, void* workspace, hipStream_t s);
+size_t irradiance_workspace_bytes(int env_w, int env_h, int theta_samples);
+althea_status launch_brdf_lut(int size, int samples, float* rg, hipStream_t s);
+althea_status launch_cube_sample(const float* cube, int size, int mips, const float* dir_lod, int count,
+                                 float* out, void* workspace, hipStream_t s);
+althea_status launch_env_sample(const float* pyr, int w, int h, const float* dir_lod, int count, float* out,
+                                void* workspace, hipStream_t s);
+
+// Deferred resolve (resolve_kernels.hip).
+althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                             const althea_point_light* lights, int light_count, const althea_globals* globals,
+                             const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
+                             hipStream_t s);
+
+}  // namespace althea

@@ -1,0 +1,323 @@
+// Deferred resolve for gfx950: one lane per pixel, a 256-lane block covers a
+// 64 x 4 pixel tile so each wave reads one 64-pixel run of every G-buffer
+// plane (1 KiB contiguous per plane in FP32).
+//
+// Per pixel (DeferredPass.vert:10-22, DeferredPass.frag:41-94):
+//   view ray, affine in the pixel centre and precomputed on the host from
+//     mat3(inverseView) * (inverseProjection * (ndc, 0, 1)).xyz;
+//   background equirect lookup (LOD 0, CLAMP) when position.w == 0 (Q13);
+//   reflection = seamless trilinear prefiltered-cube lookup at
+//     LOD roughness*(mips-1) (PBRMaterial.glsl:4-11 with cube maps);
+//   irradiance = seamless bilinear irradiance-cube lookup (PBRMaterial.glsl:13-19);
+//   SSAO hemisphere kernel into the occlusion channel (Q9);
+//   pbrMaterial (PBRMaterial.glsl:72-162, live convention of SSR.frag:65-75);
+//   tonemap 1 - exp(-c * exposure) (DeferredPass.frag:86-88).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+
+#include "device_math.hpp"
+#include "internal.hpp"
+
+namespace althea {
+
+struct ResolveParams {
+  const void* position;
+  const void* normal;
+  const void* albedo;
+  const void* mro;
+  int width, height;
+  const float4* env;  // equirect level 0
+  int env_w, env_h;
+  const float4* prefiltered;
+  CubeMips pref;
+  const float4* irradiance;
+  int irr_size;
+  const float2* lut;
+  int lut_size;
+  const althea_point_light* lights;
+  int light_count;
+  float ray0[3], rayx[3], rayy[3];  // view ray = ray0 + rayx*(px+0.5) + rayy*(py+0.5)
+  float view_z[4];                  // row 2 of the view matrix
+  float vp[16];                     // projection * view, column-major
+  float exposure;
+  int tonemap;
+  int ssao_on;
+  int ssao_samples;
+  float ssao_radius, ssao_bias;
+  float kernel[ALTHEA_SSAO_MAX_SAMPLES][4];
+  float noise[16][4];
+  void* out;
+  float* ao_out;
+};
+
+template <int kFmt>
+__device__ __forceinline__ float4 load_gbuf(const void* plane, size_t idx) {
+  if (kFmt == ALTHEA_GBUFFER_FP32) {
+    return static_cast<const float4*>(plane)[idx];
+  } else {
+    const ushort4 h = static_cast<const ushort4*>(plane)[idx];
+    return make_float4(__half2float(__ushort_as_half(h.x)), __half2float(__ushort_as_half(h.y)),
+                       __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
+  }
+}
+
+template <bool kOutF16>
+__device__ __forceinline__ void store_out(void* out, size_t idx, f3 c) {
+  if (kOutF16) {
+    ushort4 h;
+    h.x = __half_as_ushort(__float2half_rn(c.x));
+    h.y = __half_as_ushort(__float2half_rn(c.y));
+    h.z = __half_as_ushort(__float2half_rn(c.z));
+    h.w = __half_as_ushort(__float2half_rn(1.0f));
+    static_cast<ushort4*>(out)[idx] = h;
+  } else {
+    static_cast<float4*>(out)[idx] = make_float4(c.x, c.y, c.z, 1.0f);
+  }
+}
+
+__device__ __forceinline__ float pow5(float x) {
+  float x2 = x * x;
+  return x2 * x2 * x;
+}
+
+__device__ __forceinline__ f3 tonemap(f3 c, float exposure) {
+  return mk(1.0f - expf(-c.x * exposure), 1.0f - expf(-c.y * exposure), 1.0f - expf(-c.z * exposure));
+}
+
+__device__ __forceinline__ float smoothstep01(float x) {
+  float t = fminf(fmaxf(x, 0.0f), 1.0f);
+  return t * t * fmaf(-2.0f, t, 3.0f);
+}
+
+__device__ __forceinline__ float view_depth(const ResolveParams& p, f3 q) {
+  return -fmaf(p.view_z[0], q.x, fmaf(p.view_z[1], q.y, fmaf(p.view_z[2], q.z, p.view_z[3])));
+}
+
+template <int kFmt>
+__device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int py, f3 P, f3 N) {
+  const float4 nz = s_noise[(px & 3) + 4 * (py & 3)];
+  const f3 r = mk(nz.x, nz.y, nz.z);
+  const f3 T = normalize(r - N * dot(r, N));
+  const f3 B = cross(N, T);
+  const float frag_depth = view_depth(p, P);
+  const float fw = static_cast<float>(p.width), fh = static_cast<float>(p.height);
+  float occ = 0.0f;
+  for (int k = 0; k < p.ssao_samples; ++k) {
+    const float kx = p.kernel[k][0], ky = p.kernel[k][1], kz = p.kernel[k][2];
+    const f3 off = mk(fmaf(T.x, kx, fmaf(B.x, ky, N.x * kz)), fmaf(T.y, kx, fmaf(B.y, ky, N.y * kz)),
+                      fmaf(T.z, kx, fmaf(B.z, ky, N.z * kz)));
+    const f3 S = mk(fmaf(off.x, p.ssao_radius, P.x), fmaf(off.y, p.ssao_radius, P.y),
+                    fmaf(off.z, p.ssao_radius, P.z));
+    const float sd = view_depth(p, S);
+    const float cx = fmaf(p.vp[0], S.x, fmaf(p.vp[4], S.y, fmaf(p.vp[8], S.z, p.vp[12])));
+    const float cy = fmaf(p.vp[1], S.x, fmaf(p.vp[5], S.y, fmaf(p.vp[9], S.z, p.vp[13])));
+    const float cw = fmaf(p.vp[3], S.x, fmaf(p.vp[7], S.y, fmaf(p.vp[11], S.z, p.vp[15])));
+    if (!(cw > 0.0f)) continue;
+    const float u = 0.5f * (cx / cw) + 0.5f, v = 0.5f * (cy / cw) + 0.5f;
+    if (!(u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f)) continue;
+    const int qx = min(static_cast<int>(floorf(u * fw)), p.width - 1);
+    const int qy = min(static_cast<int>(floorf(v * fh)), p.height - 1);
+    const float4 q = load_gbuf<kFmt>(p.position, static_cast<size_t>(qy) * p.width + qx);
+    if (q.w == 0.0f) continue;
+    const float qd = view_depth(p, mk(q.x, q.y, q.z));
+    if (qd <= sd - p.ssao_bias) occ += smoothstep01(p.ssao_radius / fabsf(frag_depth - qd));
+  }
+  return 1.0f - occ / static_cast<float>(p.ssao_samples);
+}
+
+__device__ __forceinline__ void lut_sample(const ResolveParams& p, float u, float v, float& A, float& B) {
+  const int n = p.lut_size;
+  const float x = fmaf(u, static_cast<float>(n), -0.5f), y = fmaf(v, static_cast<float>(n), -0.5f);
+  const float fx = floorf(x), fy = floorf(y);
+  const float ax = x - fx, ay = y - fy;
+  const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  const int x0 = min(max(ix, 0), n - 1), x1 = min(max(ix + 1, 0), n - 1);
+  const int y0 = min(max(iy, 0), n - 1), y1 = min(max(iy + 1, 0), n - 1);
+  const float2 t00 = p.lut[y0 * n + x0], t10 = p.lut[y0 * n + x1];
+  const float2 t01 = p.lut[y1 * n + x0], t11 = p.lut[y1 * n + x1];
+  const float ta = fmaf(t10.x - t00.x, ax, t00.x), ba = fmaf(t11.x - t01.x, ax, t01.x);
+  const float tb = fmaf(t10.y - t00.y, ax, t00.y), bb = fmaf(t11.y - t01.y, ax, t01.y);
+  A = fmaf(ba - ta, ay, ta);
+  B = fmaf(bb - tb, ay, tb);
+}
+
+template <int kFmt, bool kOutF16>
+__global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
+  __shared__ float4 s_noise[16];
+  if (threadIdx.x < 16)
+    s_noise[threadIdx.x] =
+        make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
+  __syncthreads();
+  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (px >= p.width || py >= p.height) return;
+  const size_t idx = static_cast<size_t>(py) * p.width + px;
+  const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
+  const f3 dir = mk(fmaf(p.rayy[0], cy, fmaf(p.rayx[0], cx, p.ray0[0])),
+                    fmaf(p.rayy[1], cy, fmaf(p.rayx[1], cx, p.ray0[1])),
+                    fmaf(p.rayy[2], cy, fmaf(p.rayx[2], cx, p.ray0[2])));
+  const float4 pos = load_gbuf<kFmt>(p.position, idx);
+  if (pos.w == 0.0f) {
+    // Background (DeferredPass.frag:44-53): env LOD 0, CLAMP_TO_EDGE.
+    const float yaw = atan2f(dir.z, dir.x);
+    const float pitch = atan2f(dir.y, sqrtf(fmaf(dir.x, dir.x, dir.z * dir.z)));
+    f3 c = env_bilinear_clamp(p.env, p.env_w, p.env_h, equirect_x(yaw, p.env_w), equirect_y(pitch, p.env_h));
+    if (p.tonemap) c = tonemap(c, p.exposure);
+    if (p.ao_out) p.ao_out[idx] = 1.0f;
+    store_out<kOutF16>(p.out, idx, c);
+    return;
+  }
+  const float4 nrm = load_gbuf<kFmt>(p.normal, idx);
+  const float4 alb = load_gbuf<kFmt>(p.albedo, idx);
+  const float4 mro = load_gbuf<kFmt>(p.mro, idx);
+  const f3 P = mk(pos.x, pos.y, pos.z);
+  const f3 N = normalize(mk(nrm.x, nrm.y, nrm.z));
+  const f3 base = mk(alb.x, alb.y, alb.z);
+  const float metallic = mro.x, roughness = mro.y;
+  float ao = mro.z;
+
+  const f3 I = normalize(dir);
+  const float nd = 2.0f * dot(N, I);
+  const f3 R = mk(fmaf(-nd, N.x, I.x), fmaf(-nd, N.y, I.y), fmaf(-nd, N.z, I.z));
+  const f3 refl = cube_sample(p.prefiltered, p.pref, R, roughness * static_cast<float>(p.pref.mips - 1));
+  int face;
+  float pc, pt;
+  cube_select(N, face, pc, pt);
+  const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
+  if (p.ssao_on) ao = ssao<kFmt>(p, s_noise, px, py, P, N);
+  if (p.ao_out) p.ao_out[idx] = ao;
+
+  // pbrMaterial (PBRMaterial.glsl:72-162); V = I (camera -> surface).
+  const float n_dot_v = fmaxf(-dot(N, I), 0.0f);
+  const f3 F0 = mk(fmaf(metallic, base.x - 0.04f, 0.04f), fmaf(metallic, base.y - 0.04f, 0.04f),
+                   fmaf(metallic, base.z - 0.04f, 0.04f));
+  const float a = roughness * roughness;
+  const float a2 = a * a;
+  const float kdir = (a + 1.0f) * (a + 1.0f) * 0.125f;
+  const float om_r = 1.0f - roughness;
+  const f3 dF = mk(fmaxf(om_r, F0.x) - F0.x, fmaxf(om_r, F0.y) - F0.y, fmaxf(om_r, F0.z) - F0.z);
+  const f3 kd = base * (1.0f - metallic);
+  f3 color;
+  {
+    const float fx = pow5(1.0f - n_dot_v);
+    const f3 F = mk(fmaf(dF.x, fx, F0.x), fmaf(dF.y, fx, F0.y), fmaf(dF.z, fx, F0.z));
+    float A, B;
+    lut_sample(p, n_dot_v, roughness, A, B);
+    color = mk((irr.x * ((1.0f - F.x) * kd.x) + refl.x * fmaf(F.x, A, B)) * ao,
+               (irr.y * ((1.0f - F.y) * kd.y) + refl.y * fmaf(F.y, A, B)) * ao,
+               (irr.z * ((1.0f - F.z) * kd.z) + refl.z * fmaf(F.z, A, B)) * ao);
+  }
+  const float gv = n_dot_v / fmaf(n_dot_v, 1.0f - kdir, kdir);
+  for (int i = 0; i < p.light_count; ++i) {
+    const althea_point_light lt = p.lights[i];
+    f3 L = mk(lt.position[0] - P.x, lt.position[1] - P.y, lt.position[2] - P.z);
+    const float d2 = dot(L, L);
+    const float d = sqrtf(d2);
+    L = L * (1.0f / d);
+    const float inv_d2 = 1.0f / d2;
+    const f3 H = normalize(I + L);
+    const float n_dot_l = fmaxf(dot(N, L), 0.0f);
+    const float n_dot_h = fmaxf(dot(N, H), 0.0f);
+    const float fh = pow5(1.0f - n_dot_h);
+    const f3 F = mk(fmaf(dF.x, fh, F0.x), fmaf(dF.y, fh, F0.y), fmaf(dF.z, fh, F0.z));
+    const float t = fmaf(n_dot_h * n_dot_h, a2 - 1.0f, 1.0f);
+    const float D = a2 / (kPi * t * t);
+    const float gl = n_dot_l / fmaf(n_dot_l, 1.0f - kdir, kdir);
+    const float s = D * gv * gl / fmaf(4.0f * n_dot_l, n_dot_v, 0.0001f);
+    const float w = inv_d2 * n_dot_l;
+    color.x = fmaf(fmaf(F.x, s, (1.0f - F.x) * kd.x * kInvPi) * lt.emission[0], w, color.x);
+    color.y = fmaf(fmaf(F.y, s, (1.0f - F.y) * kd.y * kInvPi) * lt.emission[1], w, color.y);
+    color.z = fmaf(fmaf(F.z, s, (1.0f - F.z) * kd.z * kInvPi) * lt.emission[2], w, color.z);
+  }
+  if (p.tonemap) color = tonemap(color, p.exposure);
+  store_out<kOutF16>(p.out, idx, color);
+}
+
+// Host: the view ray of DeferredPass.vert is linear in NDC, so it is affine in
+// the pixel centre; fold mat3(inverseView) * inverseProjection into it.
+static void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]) {
+  const float* ip = g.inverse_projection;
+  const float* iv = g.inverse_view;
+  auto e_of = [&](double nx, double ny, double out[3]) {
+    double e[3];
+    for (int r = 0; r < 3; ++r) e[r] = ip[0 * 4 + r] * nx + ip[1 * 4 + r] * ny + ip[3 * 4 + r];
+    for (int r = 0; r < 3; ++r) out[r] = iv[0 * 4 + r] * e[0] + iv[1 * 4 + r] * e[1] + iv[2 * 4 + r] * e[2];
+  };
+  double o[3], ax[3], ay[3];
+  e_of(-1.0, -1.0, o);
+  e_of(1.0, -1.0, ax);
+  e_of(-1.0, 1.0, ay);
+  for (int r = 0; r < 3; ++r) {
+    ray0[r] = static_cast<float>(o[r]);
+    rayx[r] = static_cast<float>((ax[r] - o[r]) / W);
+    rayy[r] = static_cast<float>((ay[r] - o[r]) / H);
+  }
+}
+
+althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                             const althea_point_light* lights, int light_count, const althea_globals* globals,
+                             const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
+                             hipStream_t s) {
+  ResolveParams p{};
+  p.position = g->position;
+  p.normal = g->normal;
+  p.albedo = g->albedo;
+  p.mro = g->mro;
+  p.width = g->width;
+  p.height = g->height;
+  p.env = reinterpret_cast<const float4*>(ibl->environment);
+  p.env_w = ibl->env_width;
+  p.env_h = ibl->env_height;
+  p.prefiltered = reinterpret_cast<const float4*>(ibl->prefiltered);
+  p.pref.size = ibl->prefiltered_size;
+  p.pref.mips = ibl->prefiltered_mips;
+  for (int m = 0; m < ibl->prefiltered_mips; ++m)
+    p.pref.off[m] = static_cast<uint32_t>(cube_mip_offset(ibl->prefiltered_size, m));
+  p.irradiance = reinterpret_cast<const float4*>(ibl->irradiance);
+  p.irr_size = ibl->irradiance_size;
+  p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
+  p.lut_size = ibl->brdf_lut_size;
+  p.lights = lights;
+  p.light_count = light_count;
+  view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy);
+  for (int c = 0; c < 4; ++c) p.view_z[c] = globals->view[c * 4 + 2];
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double acc = 0.0;
+      for (int k = 0; k < 4; ++k)
+        acc += static_cast<double>(globals->projection[k * 4 + r]) * globals->view[c * 4 + k];
+      p.vp[c * 4 + r] = static_cast<float>(acc);
+    }
+  p.exposure = globals->exposure;
+  p.tonemap = (flags & ALTHEA_RESOLVE_TONEMAP) ? 1 : 0;
+  p.ssao_on = (ssao && ssao->enabled) ? 1 : 0;
+  if (p.ssao_on) {
+    p.ssao_samples = ssao->samples;
+    p.ssao_radius = ssao->radius;
+    p.ssao_bias = ssao->bias;
+    for (int k = 0; k < ssao->samples; ++k)
+      for (int c = 0; c < 4; ++c) p.kernel[k][c] = ssao->kernel[k][c];
+    for (int k = 0; k < 16; ++k)
+      for (int c = 0; c < 4; ++c) p.noise[k][c] = ssao->noise[k][c];
+  }
+  p.out = out;
+  p.ao_out = ao_out;
+  dim3 grid((g->width + 63) / 64, (g->height + 3) / 4);
+  const bool f16_out = (flags & ALTHEA_RESOLVE_OUT_FP16) != 0;
+  if (g->format == ALTHEA_GBUFFER_FP32) {
+    if (f16_out)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p);
+  } else {
+    if (f16_out)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p);
+  }
+  return check_launch("resolve_kernel");
+}
+
+}  // namespace althea

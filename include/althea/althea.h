@@ -1,0 +1,260 @@
+/*
+ * althea.h - C ABI of the MI355X (gfx950) image-based-lighting precompute and
+ * deferred-resolve library.
+ *
+ * It replaces the Vulkan compute/fragment path of nithinp7/Althea for:
+ *   - ImageBasedLighting::createResources / precomputeResources
+ *     (Include/Althea/ImageBasedLighting.h:45-49, Src/ImageBasedLighting.cpp:137-605)
+ *     and the shaders it dispatches (Shaders/IBL_Precompute/GenIrradianceMap.comp,
+ *     Shaders/IBL_Precompute/PreFilterEnvMap.comp), plus a computed BRDF LUT in
+ *     place of Content/PrecomputedMaps/brdf_lut.png (Src/ImageBasedLighting.cpp:571-602);
+ *   - the deferred lighting resolve DeferredPass.vert/DeferredPass.frag with
+ *     pbrMaterial (Shaders/PBR/PBRMaterial.glsl:72-162) and SSAO (Shaders/SSAO.glsl),
+ *     reading the G-buffer of GBufferResources (Include/Althea/DeferredRendering.h:38-102).
+ *
+ * Conventions
+ *   - Every buffer is caller-owned device memory unless a handle says otherwise.
+ *   - Every call is stream-ordered and asynchronous on the given hipStream_t
+ *     (0 = the legacy default stream); results are ready when the stream is.
+ *   - Errors come back as althea_status; no C++ exception crosses this ABI.
+ *     althea_last_error() gives a thread-local message for the last failure.
+ *   - Handles are not thread-safe: calls on one handle need external
+ *     synchronisation (the reference is single-threaded too,
+ *     Src/SingleTimeCommandBuffer.cpp:28-54).
+ *
+ * Layouts in HBM
+ *   - Equirect environment map: RGBA32F rows top-down; column u = yaw/2pi + 0.5
+ *     with yaw = atan2(z, x); row v = 0.5 - atan2(y, |xz|)/pi (the shading
+ *     convention of Shaders/Global/GlobalResources.glsl:48-54).  Power-of-two
+ *     extents.  Its pyramid holds althea_env_levels(w,h) levels
+ *     (Src/Utilities.cpp:97-100) concatenated, level l of max(1,w>>l) x max(1,h>>l).
+ *   - Cube map: RGBA32F [mip][face][y][x], faces +X,-X,+Y,-Y,+Z,-Z with the
+ *     Vulkan cube-face-selection table; mip m has faces of max(1, size>>m).
+ *   - BRDF LUT: RG32F [roughness row][NdotV column], texel centres at (i+0.5)/size.
+ *   - G-buffer: four planes of vec4 per pixel (position.w = coverage, 0 means
+ *     sky), FP32 or FP16, in the order of GBufferResources::buildMaterial
+ *     (Src/DeferredRendering.cpp:23-35).
+ */
+#ifndef ALTHEA_ALTHEA_H_
+#define ALTHEA_ALTHEA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define ALTHEA_API __attribute__((visibility("default")))
+#else
+#define ALTHEA_API
+#endif
+
+/* Status codes.  The reference throws std::runtime_error (Src/ImageBasedLighting.cpp:424,
+ * Src/ComputePipeline.cpp:49); the ABI returns these instead. */
+typedef enum althea_status {
+  ALTHEA_OK = 0,
+  ALTHEA_ERROR_INVALID_ARGUMENT = 1,
+  ALTHEA_ERROR_OUT_OF_MEMORY = 2,
+  ALTHEA_ERROR_HIP = 3,
+  ALTHEA_ERROR_NOT_FOUND = 4, /* missing environment map (ImageBasedLighting.cpp:423-425) */
+  ALTHEA_ERROR_IO = 5,
+  ALTHEA_ERROR_UNSUPPORTED = 6
+} althea_status;
+
+ALTHEA_API const char* althea_status_string(althea_status status);
+ALTHEA_API const char* althea_last_error(void);
+ALTHEA_API int althea_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Sampling primitives                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Hammersley points i = first .. first+count-1 of an n-point set, computed by
+ * the device function every sampling kernel uses: bits[k] = bitreverse(i) and
+ * xi[2k..2k+1] = (i/n correctly rounded, float(bits) * 2^-32).  The reference
+ * samples with a hash RNG instead (Shaders/IBL_Precompute/PreFilterEnvMap.comp:38-44). */
+ALTHEA_API althea_status althea_hammersley(uint32_t n, uint32_t first, uint32_t count, uint32_t* d_bits,
+                                           float* d_xi, hipStream_t stream);
+
+/* Number of levels of an environment pyramid: 1 + floor(log2(max(w, h)))
+ * (Src/Utilities.cpp:97-100), and the texel count of all levels together. */
+ALTHEA_API int althea_env_levels(int width, int height);
+ALTHEA_API size_t althea_env_pyramid_texels(int width, int height);
+
+/* Builds the environment pyramid: level 0 is copied from d_level0 and level l
+ * is the 2x2 box mean of level l-1, the LINEAR half-size blit chain of
+ * Image::generateMipMaps (Src/Image.cpp:135-239). */
+ALTHEA_API althea_status althea_env_pyramid_build(const float* d_level0, int width, int height,
+                                                  float* d_pyramid, hipStream_t stream);
+
+/* Texel count of a cube with `mips` levels and faces of `size` at mip 0. */
+ALTHEA_API size_t althea_cube_texels(int size, int mips);
+
+/* Seamless trilinear cube lookups: d_dir_lod holds float4 (dir.xyz, lod) per
+ * query; d_out receives float4 (rgb, 1).  The software filter used by the
+ * resolve for sampleEnvMap / sampleIrrMap (Shaders/PBR/PBRMaterial.glsl:4-19). */
+ALTHEA_API althea_status althea_cube_sample(const float* d_cube, int size, int mips, const float* d_dir_lod,
+                                            int count, float* d_out, hipStream_t stream);
+
+/* Precompute lookups into an environment pyramid (REPEAT sampler, trilinear,
+ * the pole branch of PreFilterEnvMap.comp:98-122): d_dir_lod float4 per
+ * query, d_out float4. */
+ALTHEA_API althea_status althea_env_sample(const float* d_pyramid, int width, int height,
+                                           const float* d_dir_lod, int count, float* d_out,
+                                           hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* IBL precompute kernels on caller buffers                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Flags for the precompute kernels. */
+#define ALTHEA_PRECOMPUTE_FP32_SUMS 0x1u /* all-FP32 reductions (precision study variant) */
+
+/* GGX-prefiltered cube (PreFilterEnvMap.comp:126-177 per output texel, one
+ * mip per roughness m/(mips-1) as ImageBasedLighting.cpp:352-402 does with i/4):
+ * `samples` Hammersley half vectors, alpha = roughness, NdotL-weighted mean,
+ * source LOD from the sample pdf.  d_cube holds althea_cube_texels(size, mips)
+ * float4.  Sums over samples are FP64 unless ALTHEA_PRECOMPUTE_FP32_SUMS. */
+ALTHEA_API althea_status althea_prefilter_cube(const float* d_pyramid, int env_width, int env_height,
+                                               int size, int mips, int samples, unsigned flags,
+                                               float* d_cube, hipStream_t stream);
+
+/* Prefiltered radiance at arbitrary points: d_dir_rough float4 (dir.xyz,
+ * roughness) per point, d_out float4.  Used for the learned surrogate's labels. */
+ALTHEA_API althea_status althea_prefilter_points(const float* d_pyramid, int env_width, int env_height,
+                                                 int samples, const float* d_dir_rough, int count,
+                                                 float* d_out, hipStream_t stream);
+
+/* Diffuse irradiance cube (GenIrradianceMap.comp:106-155 per output texel):
+ * theta_samples x int(H*theta_samples/W) Riemann sum (300 in the reference)
+ * at source LOD log2(W/theta_samples).  d_cube holds 6*size*size float4. */
+ALTHEA_API althea_status althea_irradiance_cube(const float* d_pyramid, int env_width, int env_height,
+                                                int size, int theta_samples, unsigned flags, float* d_cube,
+                                                hipStream_t stream);
+
+/* Split-sum BRDF LUT (Karis 2013; consumed at PBRMaterial.glsl:110-111):
+ * size x size RG32F, `samples` Hammersley points per texel, FP64 sums and
+ * normalisation. */
+ALTHEA_API althea_status althea_brdf_lut(int size, int samples, float* d_rg, hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* IBL resources (ImageBasedLighting::createResources / IBLResources)        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct althea_ibl althea_ibl;
+
+typedef struct althea_ibl_desc {
+  int irradiance_size;        /* cube face size of the irradiance map (32) */
+  int irradiance_theta_samples; /* 300 in GenIrradianceMap.comp:133 */
+  int prefiltered_size;       /* cube face size of prefiltered mip 0 (128, 512) */
+  int prefiltered_mips;       /* roughness levels (5 in ImageBasedLighting.cpp:192) */
+  int prefilter_samples;      /* Hammersley samples per texel (1024) */
+  int brdf_lut_size;          /* 128 */
+  int brdf_lut_samples;       /* 1024 */
+  unsigned flags;             /* ALTHEA_PRECOMPUTE_* */
+} althea_ibl_desc;
+
+/* Device views of an IBL set: IBLResources::getHandles (ImageBasedLighting.h:36-42)
+ * with device pointers in place of bindless indices, in the same order
+ * (environment, prefiltered, irradiance, BRDF LUT). */
+typedef struct althea_ibl_views {
+  const float* environment; /* pyramid, level 0 first */
+  int env_width, env_height, env_levels;
+  const float* prefiltered;
+  int prefiltered_size, prefiltered_mips;
+  const float* irradiance;
+  int irradiance_size;
+  const float* brdf_lut;
+  int brdf_lut_size;
+} althea_ibl_views;
+
+ALTHEA_API void althea_ibl_desc_default(althea_ibl_desc* desc);
+ALTHEA_API althea_status althea_ibl_create(const althea_ibl_desc* desc, althea_ibl** out);
+ALTHEA_API althea_status althea_ibl_destroy(althea_ibl* ibl);
+/* Builds the pyramid of d_equirect (copied into the handle), then irradiance,
+ * prefiltered mips and the BRDF LUT (ImageBasedLighting.cpp:137-412). */
+ALTHEA_API althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, int width, int height,
+                                               hipStream_t stream);
+ALTHEA_API althea_status althea_ibl_get_views(const althea_ibl* ibl, althea_ibl_views* views);
+/* Device bytes the shading pass reads for this IBL set (prefiltered +
+ * irradiance + LUT), the "IBL state per environment". */
+ALTHEA_API size_t althea_ibl_state_bytes(const althea_ibl* ibl);
+
+/* ------------------------------------------------------------------------ */
+/* Deferred resolve (DeferredPass.frag + pbrMaterial + SSAO)                 */
+/* ------------------------------------------------------------------------ */
+
+#define ALTHEA_GBUFFER_FP32 0
+#define ALTHEA_GBUFFER_FP16 1
+
+typedef struct althea_gbuffer_view {
+  const void* position; /* xyz world position, w = coverage (0 -> background) */
+  const void* normal;   /* xyz normal (normalised by the resolve) */
+  const void* albedo;   /* rgb base colour */
+  const void* mro;      /* metallic, roughness, occlusion */
+  int width, height;
+  int format;           /* ALTHEA_GBUFFER_FP32 or ALTHEA_GBUFFER_FP16 */
+} althea_gbuffer_view;
+
+/* PointLight of Include/Althea/PointLight.h:31-34 (alignas(16) vec3 pairs). */
+typedef struct althea_point_light {
+  float position[3];
+  float pad0;
+  float emission[3];
+  float pad1;
+} althea_point_light;
+
+/* GlobalUniforms of Include/Althea/GlobalUniforms.h:15-31: column-major mat4s
+ * as glm stores them. */
+typedef struct althea_globals {
+  float projection[16];
+  float inverse_projection[16];
+  float view[16];
+  float prev_view[16];
+  float inverse_view[16];
+  float prev_inverse_view[16];
+  float mouse_uv[2];
+  int32_t light_count;
+  uint32_t light_buffer_handle;
+  float time;
+  float exposure;
+  uint32_t input_mask;
+  uint32_t frame_count;
+} althea_globals;
+
+#define ALTHEA_SSAO_MAX_SAMPLES 64
+
+/* Hemisphere-kernel SSAO (replaces Shaders/SSAO.glsl's 24x12 ray march). */
+typedef struct althea_ssao_desc {
+  int enabled;    /* 0: keep the G-buffer occlusion channel */
+  int samples;    /* <= ALTHEA_SSAO_MAX_SAMPLES */
+  float radius;   /* world units (0.5) */
+  float bias;     /* view-depth bias */
+  float kernel[ALTHEA_SSAO_MAX_SAMPLES][4]; /* tangent-space offsets, z >= 0 */
+  float noise[16][4];                       /* 4x4 rotation tile, indexed (x&3) + 4 (y&3) */
+} althea_ssao_desc;
+
+#define ALTHEA_RESOLVE_TONEMAP 0x1u  /* 1 - exp(-c * exposure), DeferredPass.frag:86-88 */
+#define ALTHEA_RESOLVE_OUT_FP16 0x2u /* RGBA16F output instead of RGBA32F */
+
+/* One frame of the deferred resolve: per pixel the view ray of
+ * DeferredPass.vert:10-22, background environment when position.w == 0,
+ * reflection from the prefiltered cube at LOD roughness*(mips-1), irradiance,
+ * SSAO into the occlusion channel, pbrMaterial with `light_count` point lights,
+ * optional tonemap.  d_out: width*height RGBA (FP32 or FP16).  d_ao_out
+ * (optional, may be NULL): width*height floats of the occlusion used. */
+ALTHEA_API althea_status althea_deferred_resolve(const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl,
+                                                 const althea_point_light* d_lights, int light_count,
+                                                 const althea_globals* globals, const althea_ssao_desc* ssao,
+                                                 unsigned flags, void* d_out, float* d_ao_out,
+                                                 hipStream_t stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* ALTHEA_ALTHEA_H_ */

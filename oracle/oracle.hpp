@@ -1,0 +1,740 @@
+// FP64 CPU oracle for the IBL precompute + deferred-resolve path.
+//
+// TEST INFRASTRUCTURE ONLY.  This restatement is the correctness checker for
+// the HIP kernels in althea_amd/csrc.  Only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load it; no product code path calls it.
+//
+// Every function restates the reference algorithm in double precision from
+// FP32 inputs, citing the reference file:line it follows.  Where the build
+// deliberately differs from the reference (cube-map outputs, Hammersley
+// samples, a computed BRDF LUT, the hemisphere-kernel SSAO) the deviation is
+// stated at the function.
+//
+// Parity pins (what checks each function, besides the GPU comparison):
+//   hash_rng            hand-computed uint32 vectors (tests/test_oracle_core.py)
+//   hammersley          brute-force bit-loop radical inverse, all i < 2^16
+//   local_to_world      orthonormality / handedness invariants
+//   ggx sample + D      quadrature of D*cos = 1, alpha->0 identity, marginal CDF
+//   equirect sampling   constant map, texel-centre round trip, REPEAT seam
+//   env pyramid         numpy box-mean of each level
+//   cube mapping        Vulkan face table round trip, seamless edge continuity
+//   irradiance          closed-form Riemann sum for constant radiance
+//   prefilter           alpha = 0 identity, constant env, dense-quadrature limit
+//   brdf_lut            scipy quadrature of the split-sum integral, PNG fixture
+//   pbr_material        closed-form single light at normal incidence
+//   ssao                flat plane -> 1, fully covered point -> 0
+//   mlp                 central finite differences, Adam closed form
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace oracle {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct dvec3 {
+  double x = 0, y = 0, z = 0;
+};
+inline dvec3 operator+(dvec3 a, dvec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline dvec3 operator-(dvec3 a, dvec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline dvec3 operator*(dvec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline dvec3 operator*(double s, dvec3 a) { return {a.x * s, a.y * s, a.z * s}; }
+inline dvec3 mul(dvec3 a, dvec3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline double dot(dvec3 a, dvec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline dvec3 cross(dvec3 a, dvec3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double length(dvec3 a) { return std::sqrt(dot(a, a)); }
+inline dvec3 normalize(dvec3 a) { return a * (1.0 / length(a)); }
+inline dvec3 lerp3(dvec3 a, dvec3 b, double t) { return a + (b - a) * t; }
+inline double clampd(double x, double lo, double hi) { return std::min(std::max(x, lo), hi); }
+
+// Column-major 4x4 matrix, glm convention (m[col*4 + row]).
+struct dmat4 {
+  double m[16];
+  static dmat4 from_float(const float* f) {
+    dmat4 r;
+    for (int i = 0; i < 16; ++i) r.m[i] = f[i];
+    return r;
+  }
+  void apply(const double v[4], double out[4]) const {
+    for (int r = 0; r < 4; ++r)
+      out[r] = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1] + m[2 * 4 + r] * v[2] + m[3 * 4 + r] * v[3];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// a1. Reference hash RNG.  Shaders/Misc/Sampling.glsl:13-24 (copies at
+// GenIrradianceMap.comp:35-41, PreFilterEnvMap.comp:38-44, SSAO.glsl:5-11).
+// float(n) * (1.0 / float(0xffffffffU)): float(0xffffffff) rounds to 2^32, so
+// the scale is exactly 2^-32 and float(n) rounds to nearest (Q17: xi can be 1).
+// ---------------------------------------------------------------------------
+struct HashRng {
+  uint32_t sx, sy;
+  uint32_t nextu() {
+    sx += 1u;
+    sy += 1u;
+    uint32_t qx = 1103515245u * ((sx >> 1u) ^ sy);
+    uint32_t qy = 1103515245u * ((sy >> 1u) ^ sx);
+    return 1103515245u * (qx ^ (qy >> 3u));
+  }
+  float nextf() { return static_cast<float>(nextu()) * (1.0f / 4294967296.0f); }
+};
+
+// ---------------------------------------------------------------------------
+// a2. Hammersley point i of n: (i/n, radical inverse base 2 of i).  Absent
+// from the reference (it uses the hash RNG, PreFilterEnvMap.comp:82-84); the
+// build samples with it as the task requires.
+// ---------------------------------------------------------------------------
+inline uint32_t bitreverse32(uint32_t v) {
+  v = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+  v = ((v >> 2) & 0x33333333u) | ((v & 0x33333333u) << 2);
+  v = ((v >> 4) & 0x0F0F0F0Fu) | ((v & 0x0F0F0F0Fu) << 4);
+  v = ((v >> 8) & 0x00FF00FFu) | ((v & 0x00FF00FFu) << 8);
+  return (v >> 16) | (v << 16);
+}
+inline void hammersley(uint32_t i, uint32_t n, double& x1, double& x2) {
+  x1 = static_cast<double>(i) / static_cast<double>(n);
+  x2 = static_cast<double>(bitreverse32(i)) * (1.0 / 4294967296.0);
+}
+// The FP32 values every kernel must reproduce bit-exactly: correctly rounded
+// i/n and float(bitreverse(i)) * 2^-32 (round to nearest even).
+inline void hammersley_f32(uint32_t i, uint32_t n, float& x1, float& x2) {
+  x1 = static_cast<float>(static_cast<double>(i) / static_cast<double>(n));
+  x2 = static_cast<float>(static_cast<double>(bitreverse32(i)) * (1.0 / 4294967296.0));
+}
+
+// ---------------------------------------------------------------------------
+// a3. Tangent frame.  GenIrradianceMap.comp:19-31 / Misc/Sampling.glsl:58-70.
+// The branch |x| > |y| is decided on the caller's raw (unnormalised) vector so
+// FP32 and FP64 take the same branch; the frame itself uses unit n.
+// ---------------------------------------------------------------------------
+struct Frame {
+  dvec3 t, b, n;
+  dvec3 to_world(dvec3 v) const { return t * v.x + b * v.y + n * v.z; }
+};
+inline Frame local_to_world(dvec3 raw) {
+  dvec3 n = normalize(raw);
+  Frame f;
+  if (std::fabs(raw.x) > std::fabs(raw.y))
+    f.t = dvec3{-n.z, 0.0, n.x} * (1.0 / std::sqrt(n.x * n.x + n.z * n.z));
+  else
+    f.t = dvec3{0.0, n.z, -n.y} * (1.0 / std::sqrt(n.y * n.y + n.z * n.z));
+  f.b = cross(n, f.t);
+  f.n = n;
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// a4. GGX half-vector sample and NDF.  PreFilterEnvMap.comp:78-92.
+// ---------------------------------------------------------------------------
+inline dvec3 sample_ggx(double x1, double x2, double a2) {
+  double phi = 2.0 * kPi * x1;
+  double cos_t = std::sqrt((1.0 - x2) / (1.0 + (a2 - 1.0) * x2));
+  double sin_t = std::sqrt(std::max(0.0, 1.0 - cos_t * cos_t));
+  return {std::cos(phi) * sin_t, std::sin(phi) * sin_t, cos_t};
+}
+inline double distribution_ggx(double n_dot_h, double a2) {
+  double denom = n_dot_h * n_dot_h * (a2 - 1.0) + 1.0;
+  return a2 / (kPi * denom * denom);
+}
+
+// ---------------------------------------------------------------------------
+// a6. Environment pyramid.  Image::generateMipMaps (Src/Image.cpp:135-239)
+// blits level l-1 into level l with VK_FILTER_LINEAR (:205-213); extents are
+// max(1, dim >> l) (:158-166); the level count is 1 + floor(log2(max(W,H)))
+// (Src/Utilities.cpp:97-100).  For power-of-two extents the half-size LINEAR
+// blit samples every destination texel centre midway between source texels,
+// so it is the mean of the 2x2 (or 2x1 / 1x2 once an extent reaches 1) block.
+// ---------------------------------------------------------------------------
+inline int mip_count(int w, int h) {
+  return 1 + static_cast<int>(std::floor(std::log2(static_cast<double>(std::max(w, h)))));
+}
+struct Pyramid {
+  int levels = 0;
+  std::vector<int> w, h;
+  std::vector<size_t> offset;  // texel offsets
+  std::vector<double> rgba;    // levels concatenated, 4 doubles per texel
+
+  const double* texel(int l, int x, int y) const {
+    return &rgba[4 * (offset[l] + static_cast<size_t>(y) * w[l] + x)];
+  }
+};
+inline Pyramid build_pyramid(const float* level0, int width, int height) {
+  Pyramid p;
+  p.levels = mip_count(width, height);
+  size_t total = 0;
+  for (int l = 0; l < p.levels; ++l) {
+    p.w.push_back(std::max(1, width >> l));
+    p.h.push_back(std::max(1, height >> l));
+    p.offset.push_back(total);
+    total += static_cast<size_t>(p.w[l]) * p.h[l];
+  }
+  p.rgba.assign(4 * total, 0.0);
+  for (size_t i = 0; i < 4 * static_cast<size_t>(width) * height; ++i) p.rgba[i] = level0[i];
+  for (int l = 1; l < p.levels; ++l) {
+    int pw = p.w[l - 1], ph = p.h[l - 1];
+    int fx = pw > p.w[l] ? 2 : 1, fy = ph > p.h[l] ? 2 : 1;
+    double inv = 1.0 / (fx * fy);
+    for (int y = 0; y < p.h[l]; ++y)
+      for (int x = 0; x < p.w[l]; ++x) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int dy = 0; dy < fy; ++dy)
+          for (int dx = 0; dx < fx; ++dx) {
+            const double* s = p.texel(l - 1, x * fx + dx, y * fy + dy);
+            for (int c = 0; c < 4; ++c) acc[c] += s[c];
+          }
+        double* d = &p.rgba[4 * (p.offset[l] + static_cast<size_t>(y) * p.w[l] + x)];
+        for (int c = 0; c < 4; ++c) d[c] = acc[c] * inv;
+      }
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// a5. Equirect lookups with the Vulkan texel-filtering rules (public Vulkan
+// spec, "Texel Filtering" / "Wrapping Operation"): u*W - 0.5, floor, linear
+// weights, wrap or clamp the integer texel coordinates; mipmap LINEAR blends
+// floor(lod) and floor(lod)+1 clamped to the last level.
+//
+// Source orientation (Q7): the shading convention of
+// Shaders/Global/GlobalResources.glsl:48-54 (v = 0.5 - atan(y, |xz|)/pi, row 0
+// is straight up).  The precompute lookup keeps the reference's pole branch
+// (PreFilterEnvMap.comp:98-122, GenIrradianceMap.comp:78-102) and its REPEAT
+// sampler (Src/ImageBasedLighting.cpp:168-174).
+// ---------------------------------------------------------------------------
+enum class Wrap { Repeat, Clamp };
+
+inline int wrap_index(int i, int n, Wrap mode) {
+  if (mode == Wrap::Repeat) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+  }
+  return std::min(std::max(i, 0), n - 1);
+}
+
+inline dvec3 bilinear(const Pyramid& p, int l, double u, double v, Wrap mode) {
+  double x = u * p.w[l] - 0.5, y = v * p.h[l] - 0.5;
+  double fx0 = std::floor(x), fy0 = std::floor(y);
+  double ax = x - fx0, ay = y - fy0;
+  int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+  int xa = wrap_index(x0, p.w[l], mode), xb = wrap_index(x0 + 1, p.w[l], mode);
+  int ya = wrap_index(y0, p.h[l], mode), yb = wrap_index(y0 + 1, p.h[l], mode);
+  const double* t00 = p.texel(l, xa, ya);
+  const double* t10 = p.texel(l, xb, ya);
+  const double* t01 = p.texel(l, xa, yb);
+  const double* t11 = p.texel(l, xb, yb);
+  dvec3 r;
+  double* out = &r.x;
+  for (int c = 0; c < 3; ++c) {
+    double top = t00[c] + (t10[c] - t00[c]) * ax;
+    double bot = t01[c] + (t11[c] - t01[c]) * ax;
+    out[c] = top + (bot - top) * ay;
+  }
+  return r;
+}
+
+inline dvec3 trilinear(const Pyramid& p, double u, double v, double lod, Wrap mode) {
+  double d = clampd(lod, 0.0, static_cast<double>(p.levels - 1));
+  int l0 = static_cast<int>(std::floor(d));
+  int l1 = std::min(l0 + 1, p.levels - 1);
+  double delta = d - l0;
+  dvec3 a = bilinear(p, l0, u, v, mode);
+  if (delta == 0.0) return a;
+  dvec3 b = bilinear(p, l1, u, v, mode);
+  return lerp3(a, b, delta);
+}
+
+// Precompute lookup (PreFilterEnvMap.comp:98-122 with the Q7 orientation).
+inline void equirect_uv_precompute(dvec3 dir, double& u, double& v) {
+  double pitch = 0.0, yaw = 0.0;
+  double len_xz = std::sqrt(dir.x * dir.x + dir.z * dir.z);
+  if (len_xz > 0.001) {
+    yaw = std::atan2(dir.z, dir.x);
+    pitch = std::atan(dir.y / len_xz);
+  } else if (dir.y > 0.0) {
+    pitch = 0.5 * kPi;
+  } else {
+    pitch = -0.5 * kPi;
+  }
+  u = yaw / (2.0 * kPi) + 0.5;
+  v = 0.5 - pitch / kPi;
+}
+inline dvec3 sample_env_precompute(const Pyramid& p, dvec3 dir, double lod) {
+  double u, v;
+  equirect_uv_precompute(dir, u, v);
+  return trilinear(p, u, v, lod, Wrap::Repeat);
+}
+
+// Shading-side background lookup: DeferredPass.frag:33-39 (atan2 form, LOD 0,
+// CLAMP_TO_EDGE sampler Src/ImageBasedLighting.cpp:469-475).
+inline dvec3 sample_env_background(const Pyramid& p, dvec3 dir) {
+  double yaw = std::atan2(dir.z, dir.x);
+  double pitch = -std::atan2(dir.y, std::sqrt(dir.x * dir.x + dir.z * dir.z));
+  double u = 0.5 * yaw / kPi + 0.5, v = pitch / kPi + 0.5;
+  return bilinear(p, 0, u, v, Wrap::Clamp);
+}
+
+// ---------------------------------------------------------------------------
+// Cube maps (build layout; the reference stores equirect 2:1 images, the task
+// specifies cube outputs).  Face order and (sc, tc, ma) follow the public
+// Vulkan spec cube-face-selection table: +X, -X, +Y, -Y, +Z, -Z.
+// Texel (f, x, y) of an S x S face has centre s = (x+0.5)/S, t = (y+0.5)/S.
+// Storage: [mip][face][y][x] RGBA32F, mip m of size max(1, S >> m).
+// Seamless filtering: a bilinear tap one texel outside the face is fetched from
+// the adjacent face by folding its lattice point over the shared edge; a tap
+// outside both edges (a corner) is the mean of the three texels meeting there.
+// ---------------------------------------------------------------------------
+inline dvec3 face_dir(int face, double sc, double tc) {
+  switch (face) {
+    case 0: return {1.0, -tc, -sc};
+    case 1: return {-1.0, -tc, sc};
+    case 2: return {sc, 1.0, tc};
+    case 3: return {sc, -1.0, -tc};
+    case 4: return {sc, -tc, 1.0};
+    default: return {-sc, -tc, -1.0};
+  }
+}
+// Raw (unnormalised) direction of a texel centre: exact dyadic arithmetic for
+// power-of-two faces.
+inline dvec3 cube_texel_raw_dir(int face, int x, int y, int size) {
+  double sc = (2.0 * x + 1.0) / size - 1.0;
+  double tc = (2.0 * y + 1.0) / size - 1.0;
+  return face_dir(face, sc, tc);
+}
+inline void cube_select(dvec3 d, int& face, double& s, double& t) {
+  double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  double sc, tc, ma;
+  if (ax >= ay && ax >= az) {
+    ma = ax;
+    if (d.x >= 0) { face = 0; sc = -d.z; tc = -d.y; }
+    else { face = 1; sc = d.z; tc = -d.y; }
+  } else if (ay >= az) {
+    ma = ay;
+    if (d.y >= 0) { face = 2; sc = d.x; tc = d.z; }
+    else { face = 3; sc = d.x; tc = -d.z; }
+  } else {
+    ma = az;
+    if (d.z >= 0) { face = 4; sc = d.x; tc = -d.y; }
+    else { face = 5; sc = -d.x; tc = -d.y; }
+  }
+  s = 0.5 * (sc / ma + 1.0);
+  t = 0.5 * (tc / ma + 1.0);
+}
+
+// Integer lattice point (half-texel units) of texel (face, i, j) on a face of
+// size S: coordinates in [-S, S], the major axis at +-S.
+inline void cube_lattice(int face, int i, int j, int S, int p[3]) {
+  int sc = 2 * i + 1 - S, tc = 2 * j + 1 - S;
+  switch (face) {
+    case 0: p[0] = S; p[1] = -tc; p[2] = -sc; break;
+    case 1: p[0] = -S; p[1] = -tc; p[2] = sc; break;
+    case 2: p[0] = sc; p[1] = S; p[2] = tc; break;
+    case 3: p[0] = sc; p[1] = -S; p[2] = -tc; break;
+    case 4: p[0] = sc; p[1] = -tc; p[2] = S; break;
+    default: p[0] = -sc; p[1] = -tc; p[2] = -S; break;
+  }
+}
+inline void cube_from_lattice(const int p[3], int S, int& face, int& i, int& j) {
+  int sc, tc;
+  if (p[0] == S) { face = 0; sc = -p[2]; tc = -p[1]; }
+  else if (p[0] == -S) { face = 1; sc = p[2]; tc = -p[1]; }
+  else if (p[1] == S) { face = 2; sc = p[0]; tc = p[2]; }
+  else if (p[1] == -S) { face = 3; sc = p[0]; tc = -p[2]; }
+  else if (p[2] == S) { face = 4; sc = p[0]; tc = -p[1]; }
+  else { face = 5; sc = -p[0]; tc = -p[1]; }
+  i = (sc + S - 1) / 2;
+  j = (tc + S - 1) / 2;
+}
+// Resolve a tap with i or j (not both) one texel outside [0, S).
+inline void cube_fold(int face, int i, int j, int S, int& fo, int& io, int& jo) {
+  if (i >= 0 && i < S && j >= 0 && j < S) { fo = face; io = i; jo = j; return; }
+  int p[3];
+  cube_lattice(face, i, j, S, p);
+  int major = -1, over = -1;
+  for (int a = 0; a < 3; ++a) {
+    if (std::abs(p[a]) == S) major = a;
+    if (std::abs(p[a]) == S + 1) over = a;
+  }
+  p[over] = p[over] > 0 ? S : -S;
+  p[major] = p[major] > 0 ? S - 1 : -(S - 1);
+  cube_from_lattice(p, S, fo, io, jo);
+}
+
+struct CubeView {
+  const float* data;  // [mip][face][y][x][4]
+  int size, mips;
+  size_t mip_offset(int m) const {
+    size_t off = 0;
+    for (int k = 0; k < m; ++k) {
+      size_t s = std::max(1, size >> k);
+      off += 6 * s * s;
+    }
+    return off;
+  }
+  dvec3 texel(int m, int face, int i, int j) const {
+    int s = std::max(1, size >> m);
+    const float* t = data + 4 * (mip_offset(m) + (static_cast<size_t>(face) * s + j) * s + i);
+    return {t[0], t[1], t[2]};
+  }
+};
+inline dvec3 cube_tap(const CubeView& c, int m, int face, int i, int j) {
+  int S = std::max(1, c.size >> m);
+  bool in_i = i >= 0 && i < S, in_j = j >= 0 && j < S;
+  if (in_i && in_j) return c.texel(m, face, i, j);
+  int f, a, b;
+  if (in_i || in_j) {
+    cube_fold(face, i, j, S, f, a, b);
+    return c.texel(m, f, a, b);
+  }
+  int ic = std::min(std::max(i, 0), S - 1), jc = std::min(std::max(j, 0), S - 1);
+  dvec3 acc = c.texel(m, face, ic, jc);
+  cube_fold(face, i, jc, S, f, a, b);
+  acc = acc + c.texel(m, f, a, b);
+  cube_fold(face, ic, j, S, f, a, b);
+  acc = acc + c.texel(m, f, a, b);
+  return acc * (1.0 / 3.0);
+}
+inline dvec3 cube_bilinear(const CubeView& c, int m, int face, double s, double t) {
+  int S = std::max(1, c.size >> m);
+  double x = s * S - 0.5, y = t * S - 0.5;
+  double fx = std::floor(x), fy = std::floor(y);
+  double ax = x - fx, ay = y - fy;
+  int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+  dvec3 t00 = cube_tap(c, m, face, i0, j0), t10 = cube_tap(c, m, face, i0 + 1, j0);
+  dvec3 t01 = cube_tap(c, m, face, i0, j0 + 1), t11 = cube_tap(c, m, face, i0 + 1, j0 + 1);
+  dvec3 top = lerp3(t00, t10, ax), bot = lerp3(t01, t11, ax);
+  return lerp3(top, bot, ay);
+}
+inline dvec3 cube_sample(const CubeView& c, dvec3 dir, double lod) {
+  int face;
+  double s, t;
+  cube_select(dir, face, s, t);
+  double d = clampd(lod, 0.0, static_cast<double>(c.mips - 1));
+  int l0 = static_cast<int>(std::floor(d));
+  int l1 = std::min(l0 + 1, c.mips - 1);
+  double delta = d - l0;
+  dvec3 a = cube_bilinear(c, l0, face, s, t);
+  if (delta == 0.0) return a;
+  return lerp3(a, cube_bilinear(c, l1, face, s, t), delta);
+}
+
+// ---------------------------------------------------------------------------
+// a8. GGX prefilter of one direction.  PreFilterEnvMap.comp:126-177:
+// N = V = R; H = frame * sampleGGX(a2) with a = roughness (Q4); L = normalize
+// (2 (V.H) H - V); source LOD 0.5 log2(saSample / saTexel) with
+// saTexel = 4 pi / (6 envW envH) and saSample = 1 / (N pdf + 1e-4) (Q8), LOD 0
+// when roughness == 0; NdotL-weighted mean.  Samples are Hammersley(i, n).
+// `radiance(dir, lod)` is the source lookup (the pyramid, or an analytic field
+// for the closed-form pins).
+// ---------------------------------------------------------------------------
+template <class Radiance>
+inline dvec3 prefilter_dir(const Radiance& radiance, dvec3 raw_dir, double roughness, int samples,
+                           int env_w, int env_h, double* weight_out = nullptr) {
+  Frame fr = local_to_world(raw_dir);
+  dvec3 N = fr.n, V = N;
+  double a = roughness, a2 = a * a;
+  double sa_texel = 4.0 * kPi / (6.0 * env_w * env_h);
+  dvec3 acc;
+  double total = 0.0;
+  for (int i = 0; i < samples; ++i) {
+    double x1, x2;
+    hammersley(static_cast<uint32_t>(i), static_cast<uint32_t>(samples), x1, x2);
+    dvec3 H = fr.to_world(sample_ggx(x1, x2, a2));
+    dvec3 L = normalize(H * (2.0 * dot(V, H)) - V);
+    double n_dot_l = std::max(dot(N, L), 0.0);
+    double n_dot_h = std::max(dot(N, H), 0.0);
+    double h_dot_v = std::max(dot(H, V), 0.0);
+    double lod = 0.0;
+    if (roughness != 0.0) {
+      double D = distribution_ggx(n_dot_h, a2);
+      double pdf = D * n_dot_h / (4.0 * h_dot_v + 0.00001);
+      double sa_sample = 1.0 / (static_cast<double>(samples) * pdf + 0.0001);
+      lod = 0.5 * std::log2(sa_sample / sa_texel);
+    }
+    if (n_dot_l > 0.0) {
+      acc = acc + radiance(L, lod) * n_dot_l;
+      total += n_dot_l;
+    }
+  }
+  if (weight_out) *weight_out = total;
+  return acc * (1.0 / total);
+}
+
+// ---------------------------------------------------------------------------
+// a7. Irradiance of one direction.  GenIrradianceMap.comp:106-155: a left
+// Riemann sum over theta_i = 2 pi i / 300 and phi_j = (pi/2) j / P with
+// P = int(H * 300 / W), weight cos(phi) sin(phi), source LOD log2(W / 300),
+// output pi * sum / 300 / P.
+// ---------------------------------------------------------------------------
+inline int irradiance_phi_samples(int env_w, int env_h, int theta_samples) {
+  return static_cast<int>(static_cast<float>(env_h) * static_cast<float>(theta_samples) /
+                          static_cast<float>(env_w));
+}
+template <class Radiance>
+inline dvec3 irradiance_dir(const Radiance& radiance, dvec3 raw_dir, int env_w, int env_h,
+                            int theta_samples = 300) {
+  Frame fr = local_to_world(raw_dir);
+  int phi_samples = irradiance_phi_samples(env_w, env_h, theta_samples);
+  double lod = std::log2(static_cast<double>(env_w) / theta_samples);
+  dvec3 acc;
+  for (int i = 0; i < theta_samples; ++i) {
+    double theta = i * 2.0 * kPi / theta_samples;
+    double ct = std::cos(theta), st = std::sin(theta);
+    for (int j = 0; j < phi_samples; ++j) {
+      double phi = j * 0.5 * kPi / phi_samples;
+      double cp = std::cos(phi), sp = std::sin(phi);
+      dvec3 d = fr.to_world({ct * sp, st * sp, cp});
+      acc = acc + radiance(d, lod) * (cp * sp);
+    }
+  }
+  return acc * (kPi / theta_samples / phi_samples);
+}
+
+// ---------------------------------------------------------------------------
+// a10. Split-sum BRDF LUT texel (absent in the reference: it loads
+// Content/PrecomputedMaps/brdf_lut.png, Src/ImageBasedLighting.cpp:571-602,
+// and reads it at vec2(NdotV, roughness), Shaders/PBR/PBRMaterial.glsl:110).
+// Karis 2013: alpha = r^2, GGX half vectors with a2 = alpha^2, V in the xz
+// plane, k = alpha / 2, G_vis = G VdotH / (NdotH NdotV), Fc = (1 - VdotH)^5;
+// A = mean (1 - Fc) G_vis, B = mean Fc G_vis over samples with NdotL > 0,
+// normalised by the full sample count.
+// ---------------------------------------------------------------------------
+inline void brdf_lut_point(double n_dot_v, double roughness, int samples, double& A, double& B) {
+  dvec3 V{std::sqrt(1.0 - n_dot_v * n_dot_v), 0.0, n_dot_v};
+  double alpha = roughness * roughness;
+  double a2 = alpha * alpha;
+  double k = alpha / 2.0;
+  double sa = 0.0, sb = 0.0;
+  for (int i = 0; i < samples; ++i) {
+    double x1, x2;
+    hammersley(static_cast<uint32_t>(i), static_cast<uint32_t>(samples), x1, x2);
+    dvec3 H = sample_ggx(x1, x2, a2);
+    double v_dot_h = dot(V, H);
+    dvec3 L = H * (2.0 * v_dot_h) - V;
+    double n_dot_l = L.z, n_dot_h = H.z;
+    if (n_dot_l > 0.0) {
+      double gv = n_dot_v / (n_dot_v * (1.0 - k) + k);
+      double gl = n_dot_l / (n_dot_l * (1.0 - k) + k);
+      double g_vis = gv * gl * v_dot_h / (n_dot_h * n_dot_v);
+      double fc = std::pow(1.0 - v_dot_h, 5.0);
+      sa += (1.0 - fc) * g_vis;
+      sb += fc * g_vis;
+    }
+  }
+  A = sa / samples;
+  B = sb / samples;
+}
+
+// ---------------------------------------------------------------------------
+// a12. pbrMaterial, Shaders/PBR/PBRMaterial.glsl:72-162 with the live call
+// convention of Shaders/SSR.frag:65-75 (Q1): V = normalize(view ray)
+// (camera->surface), NdotV = max(dot(N, -V), 0) (:82), H = normalize(V + L)
+// (Q2, :145), alpha = r^2, kDirect = (alpha+1)^2/8 (Q3), Schlick-roughness
+// Fresnel (:51-53), inverse-square point lights (:142).  No shadow cubes in
+// this path, so the test at :138 always passes (Q11).
+// ---------------------------------------------------------------------------
+struct Light {
+  dvec3 position, emission;
+};
+struct Lut {
+  const float* rg;  // [size][size][2], row = roughness
+  int size;
+  void sample(double u, double v, double& A, double& B) const {
+    double x = u * size - 0.5, y = v * size - 0.5;
+    double fx = std::floor(x), fy = std::floor(y);
+    double ax = x - fx, ay = y - fy;
+    int x0 = std::min(std::max(static_cast<int>(fx), 0), size - 1);
+    int x1 = std::min(std::max(static_cast<int>(fx) + 1, 0), size - 1);
+    int y0 = std::min(std::max(static_cast<int>(fy), 0), size - 1);
+    int y1 = std::min(std::max(static_cast<int>(fy) + 1, 0), size - 1);
+    double r[2];
+    for (int c = 0; c < 2; ++c) {
+      double t00 = rg[2 * (y0 * size + x0) + c], t10 = rg[2 * (y0 * size + x1) + c];
+      double t01 = rg[2 * (y1 * size + x0) + c], t11 = rg[2 * (y1 * size + x1) + c];
+      double top = t00 + (t10 - t00) * ax, bot = t01 + (t11 - t01) * ax;
+      r[c] = top + (bot - top) * ay;
+    }
+    A = r[0];
+    B = r[1];
+  }
+};
+inline dvec3 fresnel_schlick(double x, dvec3 F0, double roughness) {
+  double k = std::pow(1.0 - x, 5.0);
+  dvec3 fmax{std::max(1.0 - roughness, F0.x), std::max(1.0 - roughness, F0.y),
+             std::max(1.0 - roughness, F0.z)};
+  return F0 + (fmax - F0) * k;
+}
+inline double geometry_schlick_ggx(double n_dot_v, double k) { return n_dot_v / (n_dot_v * (1.0 - k) + k); }
+inline double ndf_ggx(double n_dot_h, double a2) {
+  double tmp = n_dot_h * n_dot_h * (a2 - 1.0) + 1.0;
+  return a2 / (kPi * tmp * tmp);
+}
+inline dvec3 pbr_material(dvec3 world_pos, dvec3 V, dvec3 N, dvec3 base, dvec3 reflected,
+                          dvec3 irradiance, double metallic, double roughness, double ao,
+                          const Lut& lut, const Light* lights, int light_count) {
+  double n_dot_v = std::max(dot(N, V * -1.0), 0.0);
+  dvec3 F0 = lerp3(dvec3{0.04, 0.04, 0.04}, base, metallic);
+  double a = roughness * roughness;
+  double a2 = a * a;
+  double k_direct = (a + 1.0) * (a + 1.0) / 8.0;
+  dvec3 kd = base * (1.0 - metallic);  // mix(baseColor, vec3(0), metallic)
+  dvec3 color;
+  {
+    dvec3 F = fresnel_schlick(n_dot_v, F0, roughness);
+    dvec3 diffuse = mul(dvec3{1.0, 1.0, 1.0} - F, kd);
+    double A, B;
+    lut.sample(n_dot_v, roughness, A, B);
+    dvec3 spec = mul(reflected, F * A + dvec3{B, B, B});
+    color = color + (mul(irradiance, diffuse) + spec) * ao;
+  }
+  for (int i = 0; i < light_count; ++i) {
+    dvec3 L = lights[i].position - world_pos;
+    double d2 = dot(L, L);
+    double d = std::sqrt(d2);
+    L = L * (1.0 / d);
+    dvec3 radiance = lights[i].emission * (1.0 / d2);
+    dvec3 H = normalize(V + L);
+    double n_dot_l = std::max(dot(N, L), 0.0);
+    double n_dot_h = std::max(dot(N, H), 0.0);
+    dvec3 F = fresnel_schlick(n_dot_h, F0, roughness);
+    dvec3 diffuse = mul(dvec3{1.0, 1.0, 1.0} - F, kd) * (1.0 / kPi);
+    double s = ndf_ggx(n_dot_h, a2) * geometry_schlick_ggx(n_dot_v, k_direct) *
+               geometry_schlick_ggx(n_dot_l, k_direct) / (4.0 * n_dot_l * n_dot_v + 0.0001);
+    dvec3 spec = F * s;
+    color = color + mul(diffuse + spec, radiance) * n_dot_l;
+  }
+  return color;
+}
+
+// ---------------------------------------------------------------------------
+// a13. SSAO.  The reference marches 24 hashed rays x 12 screen steps
+// (Shaders/SSAO.glsl:29-85); the task replaces it with a hemisphere kernel:
+// K samples (cosine-weighted, seeded, scaled into the hemisphere) rotated by a
+// 4x4 tiled noise vector (Gram-Schmidt TBN), offset by `radius`, projected with
+// projection * view (GlobalUniforms.glsl:9-12, uv = 0.5 ndc + 0.5 as at
+// SSAO.glsl:42-43), compared in view depth against the nearest G-buffer
+// position texel with a bias, weighted by smoothstep(0,1, r / |depth delta|).
+// AO = 1 - occlusion / K.  It replaces the occlusion channel (Q9).
+// `margin` receives the smallest distance of any discrete decision from its
+// threshold (depth compare in view units, texel-boundary in pixels) so tests
+// can separate FP32/FP64 decision flips from arithmetic error.
+// ---------------------------------------------------------------------------
+struct SsaoDesc {
+  const float* kernel;  // [K][4]
+  const float* noise;   // [16][4]
+  int samples;
+  double radius, bias;
+};
+inline double smoothstep01(double x) {
+  double t = clampd(x, 0.0, 1.0);
+  return t * t * (3.0 - 2.0 * t);
+}
+inline double view_depth(const dmat4& view, dvec3 p) {
+  double v[4] = {p.x, p.y, p.z, 1.0}, o[4];
+  view.apply(v, o);
+  return -o[2];
+}
+inline double ssao(const float* position, int W, int H, int px, int py, dvec3 P, dvec3 N,
+                   const dmat4& view, const dmat4& proj, const SsaoDesc& d, double* margin) {
+  const float* nz = d.noise + 4 * ((px & 3) + 4 * (py & 3));
+  dvec3 r{nz[0], nz[1], nz[2]};
+  dvec3 T = normalize(r - N * dot(r, N));
+  dvec3 B = cross(N, T);
+  double frag_depth = view_depth(view, P);
+  double occ = 0.0;
+  double m = 1e30;
+  for (int k = 0; k < d.samples; ++k) {
+    const float* kk = d.kernel + 4 * k;
+    dvec3 S = P + (T * kk[0] + B * kk[1] + N * kk[2]) * d.radius;
+    double sd = view_depth(view, S);
+    double v4[4] = {S.x, S.y, S.z, 1.0}, vs[4], clip[4];
+    view.apply(v4, vs);
+    proj.apply(vs, clip);
+    if (!(clip[3] > 0.0)) continue;
+    double u = 0.5 * clip[0] / clip[3] + 0.5, v = 0.5 * clip[1] / clip[3] + 0.5;
+    double fx = u * W, fy = v * H;
+    m = std::min(m, std::fabs(fx - std::round(fx)));
+    m = std::min(m, std::fabs(fy - std::round(fy)));
+    if (!(u >= 0.0 && u < 1.0 && v >= 0.0 && v < 1.0)) continue;
+    int qx = std::min(static_cast<int>(std::floor(fx)), W - 1);
+    int qy = std::min(static_cast<int>(std::floor(fy)), H - 1);
+    const float* q = position + 4 * (static_cast<size_t>(qy) * W + qx);
+    if (q[3] == 0.0f) continue;
+    double qd = view_depth(view, {q[0], q[1], q[2]});
+    double thr = sd - d.bias;
+    m = std::min(m, std::fabs(qd - thr) / std::max(1.0, std::fabs(thr)));
+    if (qd <= thr) occ += smoothstep01(d.radius / std::fabs(frag_depth - qd));
+  }
+  if (margin) *margin = m;
+  return 1.0 - occ / d.samples;
+}
+
+// ---------------------------------------------------------------------------
+// a14. Deferred resolve of one pixel.  DeferredPass.vert:10-22 (view ray
+// mat3(invView) * (invProj * (ndc, 0, 1)).xyz, linear across the screen),
+// DeferredPass.frag:41-94: background when position.a == 0 (Q13), reflection
+// from the prefiltered map at LOD roughness*(mips-1) (no SSR buffer: Q10),
+// irradiance at LOD 0, SSAO into the occlusion channel (Q9), pbrMaterial (Q1),
+// tonemap 1 - exp(-c * exposure) (Q12).  Cube maps replace the equirect
+// lookups of PBRMaterial.glsl:4-19.
+// ---------------------------------------------------------------------------
+struct ResolveInputs {
+  const float *position, *normal, *albedo, *mro;  // RGBA32F planes
+  int W, H;
+  const Pyramid* env;  // background (level 0)
+  CubeView prefiltered, irradiance;
+  Lut lut;
+  const Light* lights;
+  int light_count;
+  dmat4 proj, inv_proj, view, inv_view;
+  double exposure;
+  bool tonemap, ssao_on;
+  SsaoDesc ssao;
+};
+inline dvec3 view_ray(const ResolveInputs& in, int px, int py) {
+  double u = (px + 0.5) / in.W, v = (py + 0.5) / in.H;
+  double ndc[4] = {u * 2.0 - 1.0, v * 2.0 - 1.0, 0.0, 1.0}, e[4];
+  in.inv_proj.apply(ndc, e);
+  const double* m = in.inv_view.m;
+  return {m[0] * e[0] + m[4] * e[1] + m[8] * e[2], m[1] * e[0] + m[5] * e[1] + m[9] * e[2],
+          m[2] * e[0] + m[6] * e[1] + m[10] * e[2]};
+}
+inline dvec3 tonemap(dvec3 c, double exposure) {
+  return {1.0 - std::exp(-c.x * exposure), 1.0 - std::exp(-c.y * exposure), 1.0 - std::exp(-c.z * exposure)};
+}
+inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_out, double* margin) {
+  size_t idx = static_cast<size_t>(py) * in.W + px;
+  dvec3 dir = view_ray(in, px, py);
+  const float* p = in.position + 4 * idx;
+  if (margin) *margin = 1e30;
+  if (p[3] == 0.0f) {
+    dvec3 bg = sample_env_background(*in.env, dir);
+    if (in.tonemap) bg = tonemap(bg, in.exposure);
+    if (ao_out) *ao_out = 1.0;
+    return bg;
+  }
+  const float* n = in.normal + 4 * idx;
+  const float* a = in.albedo + 4 * idx;
+  const float* m = in.mro + 4 * idx;
+  dvec3 P{p[0], p[1], p[2]};
+  dvec3 N = normalize({n[0], n[1], n[2]});
+  dvec3 base{a[0], a[1], a[2]};
+  double metallic = m[0], roughness = m[1], ao = m[2];
+  dvec3 I = normalize(dir);
+  dvec3 R = I - N * (2.0 * dot(N, I));
+  dvec3 refl = cube_sample(in.prefiltered, R, roughness * (in.prefiltered.mips - 1));
+  dvec3 irr = cube_sample(in.irradiance, N, 0.0);
+  if (in.ssao_on) ao = ssao(in.position, in.W, in.H, px, py, P, N, in.view, in.proj, in.ssao, margin);
+  if (ao_out) *ao_out = ao;
+  dvec3 c = pbr_material(P, I, N, base, refl, irr, metallic, roughness, ao, in.lut, in.lights,
+                         in.light_count);
+  if (in.tonemap) c = tonemap(c, in.exposure);
+  return c;
+}
+
+}  // namespace oracle

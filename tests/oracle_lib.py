@@ -1,0 +1,318 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py only.  Arrays are numpy; float32 inputs are read
+as given and every result is float64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_PATH = Path(__file__).resolve().parent.parent / "oracle" / "liboracle.so"
+
+vp, i32, u32, dbl = C.c_void_p, C.c_int, C.c_uint32, C.c_double
+
+
+class ResolveArgs(C.Structure):
+    _fields_ = [("position", vp), ("normal", vp), ("albedo", vp), ("mro", vp), ("width", i32), ("height", i32),
+                ("env", vp), ("prefiltered", vp), ("pref_size", i32), ("pref_mips", i32), ("irradiance", vp),
+                ("irr_size", i32), ("lut", vp), ("lut_size", i32), ("lights", vp), ("light_count", i32),
+                ("projection", vp), ("inverse_projection", vp), ("view", vp), ("inverse_view", vp),
+                ("exposure", C.c_float), ("tonemap", i32), ("ssao_on", i32), ("ssao_kernel", vp),
+                ("ssao_noise", vp), ("ssao_samples", i32), ("ssao_radius", C.c_float), ("ssao_bias", C.c_float)]
+
+
+_L = None
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is None:
+        if not ORACLE_PATH.exists():
+            raise RuntimeError(f"{ORACLE_PATH} missing: run __graft_entry__.build()")
+        L = C.CDLL(str(ORACLE_PATH))
+        L.oracle_hash_rng.argtypes = [u32, u32, i32, vp, vp]
+        L.oracle_hammersley.argtypes = [u32, u32, u32, vp, vp, vp]
+        L.oracle_frame.argtypes = [vp, i32, vp]
+        L.oracle_ggx_sample.argtypes = [vp, i32, dbl, vp]
+        L.oracle_ggx_d.argtypes = [dbl, dbl]
+        L.oracle_ggx_d.restype = dbl
+        L.oracle_mip_count.argtypes = [i32, i32]
+        L.oracle_env_create.argtypes = [vp, i32, i32]
+        L.oracle_env_create.restype = vp
+        L.oracle_env_destroy.argtypes = [vp]
+        L.oracle_env_texels.argtypes = [vp]
+        L.oracle_env_texels.restype = C.c_size_t
+        L.oracle_env_pyramid.argtypes = [vp, vp]
+        L.oracle_env_lookup.argtypes = [vp, vp, vp, i32, i32, vp]
+        L.oracle_cube_texel_dirs.argtypes = [i32, vp, i32, vp]
+        L.oracle_cube_select.argtypes = [vp, i32, vp, vp]
+        L.oracle_cube_fold.argtypes = [i32, i32, i32, i32, vp]
+        L.oracle_cube_sample.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+        L.oracle_prefilter_texels.argtypes = [vp, i32, i32, i32, vp, i32, vp]
+        L.oracle_prefilter_points.argtypes = [vp, i32, vp, i32, vp]
+        L.oracle_irradiance_texels.argtypes = [vp, i32, i32, vp, i32, vp]
+        L.oracle_prefilter_analytic.argtypes = [i32, vp, vp, i32, dbl, i32, i32, i32, vp]
+        L.oracle_irradiance_analytic.argtypes = [i32, vp, vp, i32, i32, i32, i32, vp]
+        L.oracle_brdf_lut.argtypes = [i32, i32, vp, i32, vp]
+        L.oracle_brdf_lut_point.argtypes = [dbl, dbl, i32, vp]
+        L.oracle_resolve.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp]
+        L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
+        L.oracle_mlp_param_count.restype = i32
+        L.oracle_mlp_encode.argtypes = [vp, i32, vp]
+        L.oracle_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
+        L.oracle_mlp_loss_grad.argtypes = [vp, vp, vp, i32, i32, vp]
+        L.oracle_mlp_loss_grad.restype = dbl
+        L.oracle_adam.argtypes = [vp, vp, vp, vp, i32, i32, dbl, dbl, dbl, dbl]
+        L.oracle_bf16_round.argtypes = [C.c_float]
+        L.oracle_bf16_round.restype = C.c_float
+        L.oracle_max_threads.restype = i32
+        _L = L
+    return _L
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous, "contiguous array required"
+    return a.ctypes.data
+
+
+def _c(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ------------------------------------------------------------- primitives
+def hash_rng(sx: int, sy: int, count: int):
+    u = np.empty(count, np.uint32)
+    f = np.empty(count, np.float32)
+    lib().oracle_hash_rng(sx, sy, count, _p(u), _p(f))
+    return u, f
+
+
+def hammersley(n: int, first: int, count: int):
+    bits = np.empty(count, np.uint32)
+    x32 = np.empty((count, 2), np.float32)
+    x64 = np.empty((count, 2), np.float64)
+    lib().oracle_hammersley(n, first, count, _p(bits), _p(x32), _p(x64))
+    return bits, x32, x64
+
+
+def frame(raw_dirs) -> np.ndarray:
+    d = _c(raw_dirs, np.float64)
+    out = np.empty((d.shape[0], 3, 3), np.float64)
+    lib().oracle_frame(_p(d), d.shape[0], _p(out))
+    return out  # rows: t, b, n
+
+
+def ggx_sample(xi, a2: float) -> np.ndarray:
+    x = _c(xi, np.float64)
+    out = np.empty((x.shape[0], 3), np.float64)
+    lib().oracle_ggx_sample(_p(x), x.shape[0], a2, _p(out))
+    return out
+
+
+def ggx_d(n_dot_h: float, a2: float) -> float:
+    return lib().oracle_ggx_d(n_dot_h, a2)
+
+
+def mip_count(w: int, h: int) -> int:
+    return lib().oracle_mip_count(w, h)
+
+
+def cube_texel_dirs(size: int, ids) -> np.ndarray:
+    i = _c(ids, np.int32)
+    out = np.empty((i.shape[0], 3), np.float64)
+    lib().oracle_cube_texel_dirs(size, _p(i), i.shape[0], _p(out))
+    return out
+
+
+def cube_select(dirs):
+    d = _c(dirs, np.float64)
+    face = np.empty(d.shape[0], np.int32)
+    st = np.empty((d.shape[0], 2), np.float64)
+    lib().oracle_cube_select(_p(d), d.shape[0], _p(face), _p(st))
+    return face, st
+
+
+def cube_fold(face: int, i: int, j: int, size: int):
+    out = np.empty(3, np.int32)
+    lib().oracle_cube_fold(face, i, j, size, _p(out))
+    return tuple(int(v) for v in out)
+
+
+def cube_sample(cube, size: int, mips: int, dirs, lods) -> np.ndarray:
+    c = _c(cube, np.float32)
+    d = _c(dirs, np.float64)
+    l = _c(lods, np.float64)
+    out = np.empty((d.shape[0], 3), np.float64)
+    lib().oracle_cube_sample(_p(c), size, mips, _p(d), _p(l), d.shape[0], _p(out))
+    return out
+
+
+def prefilter_analytic(kind: int, params, raw_dirs, roughness: float, samples: int, env_w: int = 512,
+                       env_h: int = 256) -> np.ndarray:
+    p = _c(params, np.float64)
+    d = _c(raw_dirs, np.float64)
+    out = np.empty((d.shape[0], 3), np.float64)
+    lib().oracle_prefilter_analytic(kind, _p(p), _p(d), d.shape[0], roughness, samples, env_w, env_h, _p(out))
+    return out
+
+
+def irradiance_analytic(kind: int, params, raw_dirs, env_w: int, env_h: int, theta: int = 300) -> np.ndarray:
+    p = _c(params, np.float64)
+    d = _c(raw_dirs, np.float64)
+    out = np.empty((d.shape[0], 3), np.float64)
+    lib().oracle_irradiance_analytic(kind, _p(p), _p(d), d.shape[0], env_w, env_h, theta, _p(out))
+    return out
+
+
+def brdf_lut(size: int, samples: int, ids=None) -> np.ndarray:
+    if ids is None:
+        yy, xx = np.meshgrid(np.arange(size), np.arange(size), indexing="ij")
+        ids = np.stack([xx.ravel(), yy.ravel()], 1)
+    i = _c(ids, np.int32)
+    out = np.empty((i.shape[0], 2), np.float64)
+    lib().oracle_brdf_lut(size, samples, _p(i), i.shape[0], _p(out))
+    return out
+
+
+def brdf_lut_point(n_dot_v: float, roughness: float, samples: int):
+    out = np.empty(2, np.float64)
+    lib().oracle_brdf_lut_point(n_dot_v, roughness, samples, _p(out))
+    return out
+
+
+# ------------------------------------------------------------ environment
+class Env:
+    """FP64 pyramid of an equirect RGBA32F map (h, w, 4)."""
+
+    def __init__(self, level0: np.ndarray):
+        self.level0 = _c(level0, np.float32)
+        self.h, self.w = self.level0.shape[:2]
+        self._h = lib().oracle_env_create(_p(self.level0), self.w, self.h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_env_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def pyramid(self) -> np.ndarray:
+        n = lib().oracle_env_texels(self._h)
+        out = np.empty((n, 4), np.float64)
+        lib().oracle_env_pyramid(self._h, _p(out))
+        return out
+
+    def lookup(self, dirs, lods=None, mode: int = 0) -> np.ndarray:
+        d = _c(dirs, np.float64)
+        l = _c(np.zeros(d.shape[0]) if lods is None else lods, np.float64)
+        out = np.empty((d.shape[0], 3), np.float64)
+        lib().oracle_env_lookup(self._h, _p(d), _p(l), d.shape[0], mode, _p(out))
+        return out
+
+    def prefilter_texels(self, face_size: int, mips: int, samples: int, ids) -> np.ndarray:
+        i = _c(ids, np.int32)
+        out = np.empty((i.shape[0], 4), np.float64)
+        lib().oracle_prefilter_texels(self._h, face_size, mips, samples, _p(i), i.shape[0], _p(out))
+        return out
+
+    def prefilter_points(self, samples: int, dir_rough) -> np.ndarray:
+        d = _c(dir_rough, np.float32)
+        out = np.empty((d.shape[0], 3), np.float64)
+        lib().oracle_prefilter_points(self._h, samples, _p(d), d.shape[0], _p(out))
+        return out
+
+    def irradiance_texels(self, face_size: int, ids, theta: int = 300) -> np.ndarray:
+        i = _c(ids, np.int32)
+        out = np.empty((i.shape[0], 3), np.float64)
+        lib().oracle_irradiance_texels(self._h, face_size, theta, _p(i), i.shape[0], _p(out))
+        return out
+
+
+# ----------------------------------------------------------------- resolve
+def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer=None):
+    """Oracle resolve of pixels[:, (x, y)] for a synth.Scene and IBL views
+    (numpy dict with prefiltered / irradiance / lut arrays and sizes).
+    Returns (rgb, ao, decision margin)."""
+    planes = gbuffer if gbuffer is not None else (scene.position, scene.normal, scene.albedo, scene.material)
+    planes = [_c(p, np.float32) for p in planes]
+    pref = _c(views["prefiltered"], np.float32)
+    irr = _c(views["irradiance"], np.float32)
+    lut = _c(views["lut"], np.float32)
+    lights = _c(scene.lights, np.float32)
+    g = _c(scene.globals_arr, np.float32)
+    mats = [_c(g[16 * k:16 * k + 16], np.float32) for k in (0, 1, 2, 4)]
+    kern = _c(scene.ssao_kernel, np.float32)
+    noise = _c(scene.ssao_noise, np.float32)
+    a = ResolveArgs(_p(planes[0]), _p(planes[1]), _p(planes[2]), _p(planes[3]), scene.width, scene.height,
+                    env.handle, _p(pref), views["pref_size"], views["pref_mips"], _p(irr), views["irr_size"],
+                    _p(lut), views["lut_size"], _p(lights), lights.shape[0], _p(mats[0]), _p(mats[1]),
+                    _p(mats[2]), _p(mats[3]), float(g[101]), 1 if tonemap else 0, 1 if scene.ssao_enabled else 0,
+                    _p(kern), _p(noise), kern.shape[0], scene.ssao_radius, scene.ssao_bias)
+    px = _c(pixels, np.int32)
+    rgb = np.empty((px.shape[0], 3), np.float64)
+    ao = np.empty(px.shape[0], np.float64)
+    margin = np.empty(px.shape[0], np.float64)
+    lib().oracle_resolve(C.byref(a), _p(px), px.shape[0], _p(rgb), _p(ao), _p(margin))
+    return rgb, ao, margin
+
+
+def pbr_material(pos, V, N, base, refl, irr, metallic: float, roughness: float, ao: float, lut, lut_size: int,
+                 lights) -> np.ndarray:
+    vec = _c(np.concatenate([pos, V, N, base, refl, irr]), np.float64)
+    l = _c(lut, np.float32)
+    lt = _c(lights, np.float32).reshape(-1, 8)
+    out = np.empty(3, np.float64)
+    lib().oracle_pbr_material(_p(vec), metallic, roughness, ao, _p(l), lut_size, _p(lt), lt.shape[0], _p(out))
+    return out
+
+
+# --------------------------------------------------------------------- mlp
+def mlp_param_count() -> int:
+    return lib().oracle_mlp_param_count()
+
+
+def mlp_encode(dir_rough) -> np.ndarray:
+    d = _c(dir_rough, np.float64)
+    out = np.empty((d.shape[0], 28), np.float64)
+    lib().oracle_mlp_encode(_p(d), d.shape[0], _p(out))
+    return out
+
+
+def mlp_forward(params, dir_rough, bf16: bool) -> np.ndarray:
+    p = _c(params, np.float32)
+    d = _c(dir_rough, np.float32)
+    out = np.empty((d.shape[0], 3), np.float64)
+    lib().oracle_mlp_forward(_p(p), _p(d), d.shape[0], 1 if bf16 else 0, _p(out))
+    return out
+
+
+def mlp_loss_grad(params, dir_rough, targets, bf16: bool):
+    p = _c(params, np.float32)
+    d = _c(dir_rough, np.float32)
+    t = _c(targets, np.float32)
+    g = np.empty(mlp_param_count(), np.float64)
+    loss = lib().oracle_mlp_loss_grad(_p(p), _p(d), _p(t), d.shape[0], 1 if bf16 else 0, _p(g))
+    return loss, g
+
+
+def adam(params, m, v, grad, step: int, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    p = _c(params, np.float64).copy()
+    m = _c(m, np.float64).copy()
+    v = _c(v, np.float64).copy()
+    g = _c(grad, np.float64)
+    lib().oracle_adam(_p(p), _p(m), _p(v), _p(g), p.size, step, lr, b1, b2, eps)
+    return p, m, v
+
+
+def bf16_round(x: float) -> float:
+    return lib().oracle_bf16_round(x)
+
+
+def max_threads() -> int:
+    return lib().oracle_max_threads()

@@ -1,0 +1,493 @@
+"""Pins of the FP64 oracle against closed forms, invariants, brute force and
+independent quadrature (CPU only).  Each test names the oracle function it
+pins and the reference file:line that function restates."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from tests import oracle_lib as ol
+
+
+# ------------------------------------------------------------ a1 hash RNG
+def _hash_rng_py(sx, sy, count):
+    """Independent uint32 statement of Shaders/Misc/Sampling.glsl:13-24."""
+    M = 0xFFFFFFFF
+    out = []
+    for _ in range(count):
+        sx = (sx + 1) & M
+        sy = (sy + 1) & M
+        qx = (1103515245 * ((sx >> 1) ^ sy)) & M
+        qy = (1103515245 * ((sy >> 1) ^ sx)) & M
+        out.append((1103515245 * (qx ^ (qy >> 3))) & M)
+    return np.array(out, np.uint32)
+
+
+@pytest.mark.parametrize("seed", [(0, 0), (1, 2), (123456, 789), (0xFFFFFFFF, 7)])
+def test_hash_rng_bit_exact(seed):
+    u, f = ol.hash_rng(seed[0], seed[1], 64)
+    np.testing.assert_array_equal(u, _hash_rng_py(seed[0], seed[1], 64))
+    # float(n) * (1 / float(0xffffffff)) == float32(n) * 2^-32 exactly (Q17)
+    np.testing.assert_array_equal(f, u.astype(np.float32) * np.float32(2.0 ** -32))
+
+
+def test_hash_rng_can_return_one():
+    # float(0xffffffff) rounds to 2^32 in FP32, so xi == 1.0 is reachable (Q17).
+    assert np.float32(np.uint32(0xFFFFFFFF)) * np.float32(2.0 ** -32) == np.float32(1.0)
+
+
+# ----------------------------------------------------------- a2 Hammersley
+def test_hammersley_bit_exact_all_i_below_2_16():
+    n = 1 << 16
+    bits, x32, x64 = ol.hammersley(1024, 0, n)
+    i = np.arange(n, dtype=np.uint64)
+    brute = np.zeros(n, np.uint64)
+    for b in range(32):
+        brute |= ((i >> np.uint64(b)) & np.uint64(1)) << np.uint64(31 - b)
+    np.testing.assert_array_equal(bits, brute.astype(np.uint32))
+    np.testing.assert_array_equal(x32[:, 1], brute.astype(np.uint32).astype(np.float32) * np.float32(2.0 ** -32))
+    np.testing.assert_array_equal(x64[:, 1], brute.astype(np.float64) / 2.0 ** 32)
+    np.testing.assert_array_equal(x32[:, 0], (i.astype(np.float64) / 1024).astype(np.float32))
+
+
+def test_hammersley_radical_inverse_small():
+    _, _, x = ol.hammersley(8, 0, 8)
+    np.testing.assert_array_equal(x[:, 1], [0, 0.5, 0.25, 0.75, 0.125, 0.625, 0.375, 0.875])
+    np.testing.assert_array_equal(x[:, 0], np.arange(8) / 8)
+
+
+# -------------------------------------------------------------- a3 frame
+def test_frame_orthonormal_right_handed():
+    rng = np.random.default_rng(0)
+    raw = rng.normal(size=(500, 3)) * rng.uniform(0.1, 10, size=(500, 1))
+    raw[:4] = [[0, 0, 1], [1, 0, 0], [0, -1, 0], [1, 1, 0.5]]
+    f = ol.frame(raw)
+    for k in range(len(raw)):
+        R = f[k]
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+        assert abs(np.linalg.det(R) - 1.0) < 1e-12
+        np.testing.assert_allclose(R[2], raw[k] / np.linalg.norm(raw[k]), atol=1e-14)
+        # GenIrradianceMap.comp:19-23: tangent has a zero y (|x|>|y|) or x component.
+        if abs(raw[k, 0]) > abs(raw[k, 1]):
+            assert R[0, 1] == 0.0
+        else:
+            assert R[0, 0] == 0.0
+
+
+# ---------------------------------------------------------------- a4 GGX
+@pytest.mark.parametrize("a2", [0.0625 ** 2, 0.0625, 0.25, 1.0])
+def test_ggx_ndf_projected_area_is_one(a2):
+    val, _ = integrate.quad(lambda t: ol.ggx_d(math.cos(t), a2) * math.cos(t) * math.sin(t) * 2 * math.pi,
+                            0, math.pi / 2, limit=400, points=[0.0] if a2 < 0.01 else None)
+    assert abs(val - 1.0) < 1e-7
+
+
+@pytest.mark.parametrize("a2", [0.01, 0.25, 0.81])
+def test_ggx_sampling_matches_ndf_marginal(a2):
+    # CDF of theta implied by the oracle's warp vs quadrature of D cos.
+    x2 = np.linspace(0.0, 0.999, 400)
+    xi = np.stack([np.full_like(x2, 0.3), x2], 1)
+    h = ol.ggx_sample(xi, a2)
+    for k in range(0, 400, 37):
+        theta_c = math.acos(h[k, 2])
+        cdf, _ = integrate.quad(lambda t: ol.ggx_d(math.cos(t), a2) * math.cos(t) * math.sin(t) * 2 * math.pi,
+                                0, theta_c, limit=400)
+        assert abs(cdf - x2[k]) < 1e-7
+
+
+def test_ggx_alpha_zero_is_normal():
+    xi = np.random.default_rng(1).uniform(0, 0.999, size=(64, 2))
+    h = ol.ggx_sample(xi, 0.0)
+    np.testing.assert_allclose(h, np.tile([0, 0, 1.0], (64, 1)), atol=0)
+
+
+# ------------------------------------------------------------ a6 pyramid
+def test_pyramid_is_box_mean():
+    rng = np.random.default_rng(2)
+    env = rng.lognormal(0, 1.5, size=(32, 64, 4)).astype(np.float32)
+    e = ol.Env(env)
+    pyr = e.pyramid()
+    assert ol.mip_count(64, 32) == 7
+    lvl = env.astype(np.float64)
+    off = 0
+    for l in range(7):
+        h, w = lvl.shape[:2]
+        np.testing.assert_allclose(pyr[off:off + h * w].reshape(h, w, 4), lvl, rtol=1e-14)
+        off += h * w
+        fh = 2 if h > 1 else 1
+        fw = 2 if w > 1 else 1
+        lvl = lvl.reshape(h // fh, fh, w // fw, fw, 4).mean(axis=(1, 3))
+    assert off == pyr.shape[0]
+
+
+def test_mip_count_matches_computeMipCount():
+    # Src/Utilities.cpp:97-100
+    for w, h in [(512, 256), (4096, 2048), (1, 1), (256, 128), (3, 1)]:
+        assert ol.mip_count(w, h) == 1 + int(math.floor(math.log2(max(w, h))))
+
+
+# --------------------------------------------------- a5 equirect lookups
+def _texel_dir(x, y, w, h):
+    yaw = ((x + 0.5) / w - 0.5) * 2 * math.pi
+    pitch = (0.5 - (y + 0.5) / h) * math.pi
+    return [math.cos(pitch) * math.cos(yaw), math.sin(pitch), math.cos(pitch) * math.sin(yaw)]
+
+
+def test_equirect_constant_and_texel_centres():
+    rng = np.random.default_rng(3)
+    env = rng.lognormal(0, 1.0, size=(16, 32, 4)).astype(np.float32)
+    e = ol.Env(env)
+    xs, ys = np.meshgrid(np.arange(32), np.arange(1, 15))
+    dirs = np.array([_texel_dir(x, y, 32, 16) for x, y in zip(xs.ravel(), ys.ravel())])
+    got = e.lookup(dirs, np.zeros(len(dirs)), mode=0)
+    np.testing.assert_allclose(got, env[ys.ravel(), xs.ravel(), :3], rtol=1e-9)
+    got_bg = e.lookup(dirs, mode=1)  # background convention, CLAMP
+    np.testing.assert_allclose(got_bg, env[ys.ravel(), xs.ravel(), :3], rtol=1e-9)
+    const = np.full((16, 32, 4), 3.25, np.float32)
+    ec = ol.Env(const)
+    d = rng.normal(size=(200, 3))
+    np.testing.assert_array_equal(ec.lookup(d, rng.uniform(0, 6, 200), mode=0), 3.25)
+
+
+def test_equirect_repeat_seam_is_continuous():
+    rng = np.random.default_rng(4)
+    env = rng.lognormal(0, 1.0, size=(16, 32, 4)).astype(np.float32)
+    e = ol.Env(env)
+    eps = 1e-9
+    a = e.lookup([[-1, 0.3, eps]], [0.0])
+    b = e.lookup([[-1, 0.3, -eps]], [0.0])
+    np.testing.assert_allclose(a, b, rtol=1e-6)
+
+
+def test_equirect_pole_branch():
+    # |xz| <= 0.001 -> yaw 0, pitch +-pi/2 (PreFilterEnvMap.comp:104-112):
+    # u = 0.5, v = 0 (top) with REPEAT in v blending row 0 and row H-1.
+    env = np.zeros((8, 16, 4), np.float32)
+    env[0, :, :3] = 10.0
+    env[7, :, :3] = 2.0
+    e = ol.Env(env)
+    up = e.lookup([[0.0, 1.0, 0.0005]], [0.0])[0]
+    np.testing.assert_allclose(up, [6.0, 6.0, 6.0], rtol=1e-12)
+
+
+# ------------------------------------------------------------ cube maps
+def test_cube_texel_round_trip():
+    S = 16
+    ids = np.array([[f, x, y] for f in range(6) for x in range(S) for y in range(S)], np.int32)
+    dirs = ol.cube_texel_dirs(S, ids)
+    face, st = ol.cube_select(dirs)
+    np.testing.assert_array_equal(face, ids[:, 0])
+    np.testing.assert_allclose(st[:, 0] * S - 0.5, ids[:, 1], atol=1e-12)
+    np.testing.assert_allclose(st[:, 1] * S - 0.5, ids[:, 2], atol=1e-12)
+
+
+def test_cube_face_table_is_vulkan():
+    # Vulkan cube-map face selection: major axis -> face, (sc, tc).
+    face, st = ol.cube_select([[1, 0.2, -0.4], [-1, 0.2, 0.4], [0.3, 1, 0.1], [0.3, -1, 0.1], [0.2, 0.1, 1],
+                               [0.2, 0.1, -1]])
+    np.testing.assert_array_equal(face, [0, 1, 2, 3, 4, 5])
+    sc = st[:, 0] * 2 - 1
+    tc = st[:, 1] * 2 - 1
+    np.testing.assert_allclose(sc, [0.4, 0.4, 0.3, 0.3, 0.2, -0.2])
+    np.testing.assert_allclose(tc, [-0.2, -0.2, 0.1, -0.1, -0.1, -0.1])
+
+
+def test_cube_fold_lands_on_adjacent_edge_and_is_involutive():
+    S = 8
+    for f in range(6):
+        for k in range(S):
+            for (i, j) in [(-1, k), (S, k), (k, -1), (k, S)]:
+                fo, io, jo = ol.cube_fold(f, i, j, S)
+                assert fo != f and fo != (f ^ 1)
+                assert io in (0, S - 1) or jo in (0, S - 1)
+                # The folded texel is one texel away across the edge.
+                d0 = ol.cube_texel_dirs(S, [[f, min(max(i, 0), S - 1), min(max(j, 0), S - 1)]])[0]
+                d1 = ol.cube_texel_dirs(S, [[fo, io, jo]])[0]
+                ang = math.degrees(math.acos(min(1.0, float(d0 @ d1))))
+                assert ang < 1.5 * 90.0 / S
+                # Stepping back over the edge from the folded texel returns home.
+                back = []
+                for (di, dj) in [(-1, 0), (1, 0), (0, -1), (0, 1)]:
+                    ii, jj = io + di, jo + dj
+                    if 0 <= ii < S and 0 <= jj < S:
+                        continue
+                    back.append(ol.cube_fold(fo, ii, jj, S))
+                assert (f, min(max(i, 0), S - 1), min(max(j, 0), S - 1)) in back
+
+
+def test_cube_sample_seamless_across_edges_and_constant():
+    rng = np.random.default_rng(5)
+    S, mips = 8, 3
+    n = sum(6 * max(1, S >> m) ** 2 for m in range(mips))
+    cube = rng.uniform(0.5, 2.0, size=(n, 4)).astype(np.float32)
+    # Points on face edges, nudged to either side.
+    base = rng.normal(size=(300, 3))
+    base[:, 0] = np.sign(base[:, 0]) * np.maximum(np.abs(base[:, 1]), np.abs(base[:, 2]))  # |x| = max(|y|,|z|)
+    for eps in (1e-9,):
+        a = base.copy()
+        a[:, 0] *= (1 + eps)
+        b = base.copy()
+        b[:, 0] *= (1 - eps)
+        lods = rng.uniform(0, mips - 1, 300)
+        ra = ol.cube_sample(cube, S, mips, a, lods)
+        rb = ol.cube_sample(cube, S, mips, b, lods)
+        np.testing.assert_allclose(ra, rb, rtol=1e-6)
+    const = np.full((n, 4), 1.5, np.float32)
+    np.testing.assert_allclose(ol.cube_sample(const, S, mips, base, lods), 1.5, rtol=1e-15)
+
+
+# ------------------------------------------------------------ a7 irradiance
+def test_irradiance_constant_closed_form():
+    # pi/(300P) sum_ij cos(phi_j) sin(phi_j) = (pi/(2P)) cot(pi/(2P))
+    # (GenIrradianceMap.comp:126-147, left Riemann sum, P = 150 for 2:1 maps).
+    dirs = np.random.default_rng(6).normal(size=(5, 3))
+    out = ol.irradiance_analytic(0, [2.0, 3.0, 5.0], dirs, 512, 256)
+    P = 150
+    k = (math.pi / (2 * P)) / math.tan(math.pi / (2 * P))
+    np.testing.assert_allclose(out, np.tile([2.0 * k, 3.0 * k, 5.0 * k], (5, 1)), rtol=1e-13)
+
+
+def test_irradiance_linear_field():
+    # For L(w) = a.w the theta sum over a full period cancels the tangent terms:
+    # out = (pi / P) (a.n) sum_j cos^2(phi_j) sin(phi_j).
+    rng = np.random.default_rng(7)
+    dirs = rng.normal(size=(6, 3))
+    a = np.array([0.3, -1.2, 0.7])
+    params = np.concatenate([a, 2 * a, -a])
+    out = ol.irradiance_analytic(1, params, dirs, 512, 256)
+    P = 150
+    phi = np.arange(P) * 0.5 * np.pi / P
+    s = np.sum(np.cos(phi) ** 2 * np.sin(phi))
+    n = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    expect = (np.pi / P) * (n @ a) * s
+    np.testing.assert_allclose(out[:, 0], expect, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(out[:, 1], 2 * expect, rtol=1e-11, atol=1e-13)
+
+
+# ------------------------------------------------------------ a8 prefilter
+def test_prefilter_roughness_zero_is_identity():
+    rng = np.random.default_rng(8)
+    env = rng.lognormal(0, 1.5, size=(64, 128, 4)).astype(np.float32)
+    e = ol.Env(env)
+    ids = np.array([[0, f, x, y] for f in range(6) for x in range(0, 16, 5) for y in range(0, 16, 3)], np.int32)
+    out = e.prefilter_texels(16, 5, 256, ids)
+    dirs = ol.cube_texel_dirs(16, ids[:, 1:])
+    direct = e.lookup(dirs, np.zeros(len(dirs)))
+    np.testing.assert_allclose(out[:, :3], direct, rtol=1e-12)
+    np.testing.assert_allclose(out[:, 3], 256.0, rtol=1e-12)
+
+
+def test_prefilter_constant_env():
+    e = ol.Env(np.full((32, 64, 4), 7.0, np.float32))
+    ids = np.array([[m, f, 1, 2] for m in range(5) for f in range(6)], np.int32)
+    out = e.prefilter_texels(8, 5, 128, ids)
+    np.testing.assert_allclose(out[:, :3], 7.0, rtol=1e-13)
+
+
+def _prefilter_linear_factor(alpha):
+    """E[lz^2 1(lz>0)] / E[lz 1(lz>0)] with cos(theta_h) ~ D cos, lz = 2cos^2-1."""
+    a2 = alpha * alpha
+
+    def pdf(t):
+        return ol.ggx_d(math.cos(t), a2) * math.cos(t) * math.sin(t) * 2 * math.pi
+
+    tmax = math.pi / 4  # lz > 0  <=>  theta_h < 45 deg
+    num, _ = integrate.quad(lambda t: (2 * math.cos(t) ** 2 - 1) ** 2 * pdf(t), 0, tmax, limit=400)
+    den, _ = integrate.quad(lambda t: (2 * math.cos(t) ** 2 - 1) * pdf(t), 0, tmax, limit=400)
+    return num / den
+
+
+@pytest.mark.parametrize("rough", [0.25, 0.5, 1.0])
+def test_prefilter_linear_field_matches_quadrature(rough):
+    rng = np.random.default_rng(9)
+    dirs = rng.normal(size=(4, 3))
+    a = np.array([0.5, -0.8, 1.1])
+    out = ol.prefilter_analytic(1, np.concatenate([a, a, a]), dirs, rough, 4096)
+    n = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    expect = (n @ a) * _prefilter_linear_factor(rough)
+    np.testing.assert_allclose(out[:, 0], expect, atol=2e-3 * np.linalg.norm(a))
+
+
+# ------------------------------------------------------------ a10 BRDF LUT
+def _split_sum_quadrature(nv, r):
+    alpha = r * r
+    a2 = alpha * alpha
+    k = alpha / 2
+    V = np.array([math.sqrt(1 - nv * nv), 0.0, nv])
+
+    def integrand(phi, t, which):
+        H = np.array([math.sin(t) * math.cos(phi), math.sin(t) * math.sin(phi), math.cos(t)])
+        vdh = V @ H
+        L = 2 * vdh * H - V
+        if L[2] <= 0:
+            return 0.0
+        g = nv / (nv * (1 - k) + k) * L[2] / (L[2] * (1 - k) + k)
+        gvis = g * vdh / (H[2] * nv)
+        fc = (1 - vdh) ** 5
+        w = ol.ggx_d(math.cos(t), a2) * math.cos(t) * math.sin(t)
+        return (fc if which else 1 - fc) * gvis * w
+
+    A, _ = integrate.dblquad(lambda p, t: integrand(p, t, 0), 0, math.pi / 2, 0, 2 * math.pi, epsabs=1e-7)
+    B, _ = integrate.dblquad(lambda p, t: integrand(p, t, 1), 0, math.pi / 2, 0, 2 * math.pi, epsabs=1e-7)
+    return A, B
+
+
+@pytest.mark.parametrize("nv,r", [(0.5, 0.5), (0.2, 0.8), (0.9, 0.6)])
+def test_brdf_lut_matches_quadrature(nv, r):
+    A, B = _split_sum_quadrature(nv, r)
+    got = ol.brdf_lut_point(nv, r, 4096)
+    assert abs(got[0] - A) < 3e-3 and abs(got[1] - B) < 3e-3, (got, A, B)
+
+
+def test_brdf_lut_mirror_limit():
+    # r -> 0: G -> 1, VdotH = NdotH... A + B -> 1.
+    for nv in (0.1, 0.5, 0.9):
+        a, b = ol.brdf_lut_point(nv, 1e-3, 1024)
+        assert abs(a + b - 1.0) < 1e-3
+
+
+def test_brdf_lut_png_fixture_loose():
+    """Content/PrecomputedMaps/brdf_lut.png (450x450, roughness = 0 at the
+    bottom row, Q5), sampled on a grid by tests/golden/make_brdf_png_fixture.py:
+    a loose fixture (8-bit palette, unknown generator)."""
+    import json
+    from pathlib import Path
+
+    fx = json.loads((Path(__file__).parent / "golden" / "brdf_lut_png.json").read_text())
+    pts = np.array(fx["points"])  # (n, 4): nv, r, A, B
+    ours = np.array([ol.brdf_lut_point(nv, r, 1024) for nv, r in pts[:, :2]])
+    err = np.abs(ours - pts[:, 2:]).ravel()
+    assert err.mean() < 0.01 and np.percentile(err, 95) < 0.03, (err.mean(), np.percentile(err, 95))
+
+
+# ---------------------------------------------------------- a12 pbrMaterial
+def _lut_const(a, b, size=4):
+    lut = np.zeros((size, size, 2), np.float32)
+    lut[..., 0] = a
+    lut[..., 1] = b
+    return lut
+
+
+def test_pbr_material_metal_rough_one_closed_form():
+    # metallic 1 -> no diffuse; roughness 1 -> D = 1/pi, F = F0 = base
+    # (max(1-r, F0) - F0 = 0), kDirect = 0.5.
+    base = np.array([0.8, 0.5, 0.2])
+    N = np.array([0, 0, 1.0])
+    V = np.array([math.sin(0.3), 0, -math.cos(0.3)])
+    P = np.zeros(3)
+    lp = np.array([1.0, 2.0, 3.0])
+    E = 20.0
+    light = np.array([*lp, 0, E, E, E, 0], np.float32)
+    out = ol.pbr_material(P, V, N, base, np.zeros(3), np.zeros(3), 1.0, 1.0, 1.0, _lut_const(0, 0), 4, light)
+    L = lp / np.linalg.norm(lp)
+    d2 = lp @ lp
+    nv, nl = -(N @ V), N @ L
+    g = lambda x: x / (0.5 * x + 0.5)
+    spec = (1 / math.pi) * base * g(nv) * g(nl) / (4 * nl * nv + 1e-4)
+    np.testing.assert_allclose(out, spec * E / d2 * nl, rtol=1e-12)
+
+
+def test_pbr_material_constant_ibl():
+    # IBL term = (irr (1-F)(1-m) base + refl (F A + B)) ao (PBRMaterial.glsl:101-114).
+    base = np.array([0.3, 0.6, 0.9])
+    N = np.array([0, 1.0, 0])
+    V = np.array([0, -1.0, 0])  # NdotV = 1 -> F = F0
+    m, r, ao = 0.25, 0.5, 0.8
+    A, B = 0.7, 0.1
+    out = ol.pbr_material(np.zeros(3), V, N, base, np.full(3, 2.0), np.full(3, 3.0), m, r, ao, _lut_const(A, B), 4,
+                          np.zeros((0, 8), np.float32))
+    F0 = 0.04 * (1 - m) + base * m
+    A, B = float(np.float32(A)), float(np.float32(B))  # the LUT stores float32
+    expect = (3.0 * (1 - F0) * (1 - m) * base + 2.0 * (F0 * A + B)) * ao
+    np.testing.assert_allclose(out, expect, rtol=1e-12)
+
+
+def test_pbr_material_light_below_horizon_contributes_nothing():
+    N = np.array([0, 1.0, 0])
+    V = np.array([0.3, -0.9, 0.1])
+    V /= np.linalg.norm(V)
+    light = np.array([0.0, -5.0, 1.0, 0, 9, 9, 9, 0], np.float32)
+    out = ol.pbr_material(np.zeros(3), V, N, np.ones(3) * 0.5, np.zeros(3), np.zeros(3), 0.0, 0.4, 1.0,
+                          _lut_const(0, 0), 4, light)
+    np.testing.assert_array_equal(out, 0.0)
+
+
+# ------------------------------------------------- a13/a14 SSAO and resolve
+def _views_const(pref=1.0, irr=1.0, lut=(0.5, 0.1)):
+    import althea_amd as aa
+
+    return dict(prefiltered=np.full((aa.cube_texels(8, 3), 4), pref, np.float32), pref_size=8, pref_mips=3,
+                irradiance=np.full((6 * 16, 4), irr, np.float32), irr_size=4,
+                lut=_lut_const(*lut, size=8), lut_size=8)
+
+
+def test_ssao_flat_plane_is_unoccluded():
+    # A plane facing the camera has constant view depth, so every kernel point
+    # above it (z >= 0) is nearer than the surface it projects onto.  (A
+    # grazing plane is not: nearest-pixel depth quantisation exceeds the bias.)
+    import althea_amd as aa
+
+    sc = aa.synth.scene(seed=3, width=96, height=64, lights=2, amplitude=0.0, eye=(0.0, 10.0, 0.01))
+    env = ol.Env(np.ones((16, 32, 4), np.float32))
+    cov = sc.position[..., 3] > 0
+    ys, xs = np.nonzero(cov)
+    pick = np.random.default_rng(0).choice(len(xs), 200, replace=False)
+    pix = np.stack([xs[pick], ys[pick]], 1)
+    _, ao, _ = ol.resolve(sc, _views_const(), env, pix)
+    np.testing.assert_array_equal(ao, 1.0)
+
+
+def test_ssao_covered_point_closed_form():
+    import althea_amd as aa
+
+    sc = aa.synth.scene(seed=3, width=64, height=48, lights=1, amplitude=0.0)
+    # Replace the G-buffer: every pixel is a wall 0.1 in front of the camera
+    # except the centre pixel, a surface at view depth 5 facing the camera.
+    view = sc.mat(2).reshape(4, 4).T.astype(np.float64)
+    inv_view = np.linalg.inv(view)
+    cam_fwd = -inv_view[:3, 2]
+    eye = inv_view[:3, 3]
+    pos = np.zeros_like(sc.position)
+    nrm = np.zeros_like(sc.normal)
+    wall = eye + 0.1 * cam_fwd
+    pos[..., :3] = wall
+    pos[..., 3] = 1
+    nrm[..., :3] = -cam_fwd
+    cx, cy = 32, 24
+    P = eye + 5.0 * cam_fwd
+    pos[cy, cx, :3] = P
+    sc.position, sc.normal = pos, nrm
+    env = ol.Env(np.ones((16, 32, 4), np.float32))
+    _, ao, _ = ol.resolve(sc, _views_const(), env, [[cx, cy]])
+    # The wall's view depth: 0.1 (plus the float32 rounding of its position).
+    qd = -(view[2, :3] @ pos[0, 0, :3].astype(np.float64) + view[2, 3])
+    fd = -(view[2, :3] @ pos[cy, cx, :3].astype(np.float64) + view[2, 3])
+    t = min(1.0, 0.5 / abs(fd - qd))
+    w = t * t * (3 - 2 * t)
+    # Samples that project onto the wall are occluded with range weight w;
+    # samples close to the normal land back on the fragment's own pixel and
+    # are not.  So AO = 1 - k w / 16 for an integer k, most samples occluded.
+    k = (1 - ao[0]) * 16 / w
+    assert abs(k - round(k)) < 1e-6 and 12 <= round(k) <= 16, k
+
+
+def test_resolve_background_and_tonemap():
+    import althea_amd as aa
+
+    sc = aa.synth.scene(seed=5, width=64, height=48, lights=3)
+    rng = np.random.default_rng(10)
+    envmap = rng.lognormal(0, 1.0, size=(32, 64, 4)).astype(np.float32)
+    env = ol.Env(envmap)
+    sky = np.argwhere(sc.position[..., 3] == 0)[:20][:, ::-1]
+    rgb, _, _ = ol.resolve(sc, _views_const(), env, sky)
+    # View ray of DeferredPass.vert:10-22 -> background lookup (DeferredPass.frag:33-39).
+    ip = sc.mat(1).reshape(4, 4).T.astype(np.float64)
+    iv = sc.mat(4).reshape(4, 4).T.astype(np.float64)
+    dirs = []
+    for x, y in sky:
+        ndc = np.array([(x + 0.5) / 64 * 2 - 1, (y + 0.5) / 48 * 2 - 1, 0, 1])
+        dirs.append(iv[:3, :3] @ (ip @ ndc)[:3])
+    np.testing.assert_allclose(rgb, env.lookup(np.array(dirs), mode=1), rtol=1e-12)
+    rgb_t, _, _ = ol.resolve(sc, _views_const(), env, sky, tonemap=True)
+    np.testing.assert_allclose(rgb_t, 1 - np.exp(-rgb * float(sc.globals_arr[101])), rtol=1e-12)
