@@ -54,6 +54,11 @@ class SsaoDescC(C.Structure):
                 ("kernel", (C.c_float * 4) * SSAO_MAX_SAMPLES), ("noise", (C.c_float * 4) * 16)]
 
 
+class MlpDescC(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("epsilon", C.c_float), ("init_output_bias", C.c_float)]
+
+
 assert C.sizeof(GlobalsC) == 416, C.sizeof(GlobalsC)
 
 # name -> (restype, argtypes)
@@ -80,6 +85,17 @@ PROTOTYPES = {
     "althea_ibl_state_bytes": (sz, [vp]),
     "althea_deferred_resolve": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
                                       C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),
+    "althea_mlp_desc_default": (None, [C.POINTER(MlpDescC)]),
+    "althea_mlp_param_count": (i32, []),
+    "althea_mlp_create": (i32, [C.POINTER(MlpDescC), C.POINTER(vp)]),
+    "althea_mlp_destroy": (i32, [vp]),
+    "althea_mlp_set_params": (i32, [vp, vp, vp]),
+    "althea_mlp_get_params": (i32, [vp, vp, vp]),
+    "althea_mlp_train_step": (i32, [vp, vp, vp, i32, vp, vp]),
+    "althea_mlp_gradients": (i32, [vp, vp, vp, i32, vp, vp, vp]),
+    "althea_mlp_infer": (i32, [vp, vp, i32, vp, vp]),
+    "althea_mlp_state_bytes": (sz, [vp]),
+    "althea_mfma_bf16_probe": (i32, [vp, vp, vp, vp]),
 }
 
 _lib = None

@@ -167,12 +167,20 @@ __device__ __forceinline__ f3 cube_face_dir(int face, float sc, float tc) {
     default: return mk(-sc, -tc, -1.0f);
   }
 }
-// Raw direction of texel (x, y) of an S x S face (exact for power-of-two S).
+// Raw direction of texel (x, y) of an S x S face on the integer lattice of
+// half-texels, (2x+1-S, 2y+1-S, +-S) through the face table: exact, so the
+// frame's |x| > |y| tie-break matches the FP64 oracle for any S.
 __device__ __forceinline__ f3 cube_texel_raw_dir(int face, int x, int y, int S) {
-  float inv = 1.0f / static_cast<float>(S);
-  float sc = static_cast<float>(2 * x + 1) * inv - 1.0f;
-  float tc = static_cast<float>(2 * y + 1) * inv - 1.0f;
-  return cube_face_dir(face, sc, tc);
+  const float sc = static_cast<float>(2 * x + 1 - S), tc = static_cast<float>(2 * y + 1 - S);
+  const float m = static_cast<float>(S);
+  switch (face) {
+    case 0: return mk(m, -tc, -sc);
+    case 1: return mk(-m, -tc, sc);
+    case 2: return mk(sc, m, tc);
+    case 3: return mk(sc, -m, -tc);
+    case 4: return mk(sc, -tc, m);
+    default: return mk(-sc, -tc, -m);
+  }
 }
 
 // Major-axis selection: face and the projected (sc/ma, tc/ma) in [-1, 1].

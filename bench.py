@@ -1,0 +1,168 @@
+"""Benchmark: 4K deferred PBR+IBL+SSAO resolve on one MI355X.
+
+Workload (the full-size plain method): 3840x2160 G-buffer of four RGBA32F
+planes (fBm heightfield seen by a 60-degree pinhole camera), 64 point lights
+with inverse-square falloff, 16-sample hemisphere SSAO (radius 0.5, 4x4
+rotation noise), IBL from a 512^2 x 6 prefiltered cube with 6 mips (1024
+Hammersley samples, from a 2048x1024 synthetic HDR map), 32^2 irradiance cube
+and 128^2 BRDF LUT.  A step is one resolve of the frame; inputs are resident
+in HBM before the timed region.
+
+Also measured (reported under "secondary"): the IBL precompute kernels of the
+small configuration (512x256 source -> 128^2 x 6 x 5 mips at 1024 samples,
+32^2 irradiance, 128^2 LUT) and of the 512^2 cube.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+SEED0 = 0x414C0000
+WIDTH, HEIGHT, LIGHTS = 3840, 2160, 64
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
+    p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
+    p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
+    return p.parse_args()
+
+
+def algorithmic_bytes(ibl, fp16: bool) -> int:
+    """Compulsory HBM bytes of one resolve: every G-buffer plane read once,
+    the output written once, the IBL state and lights read once."""
+    px = WIDTH * HEIGHT
+    gbuf = 4 * (8 if fp16 else 16)
+    out = 16
+    return px * (gbuf + out) + ibl.state_bytes() + LIGHTS * 32
+
+
+def time_events(torch, fn, reps: int) -> float:
+    """Average ms per call of fn() on the current stream, HIP events."""
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def secondary(torch, aa) -> dict:
+    res = {}
+    env0 = aa.synth.env(seed=SEED0, width=512, height=256)
+    dev0 = torch.from_numpy(env0).cuda()
+    pyr0 = aa.env_pyramid(dev0)
+    texels0 = aa.cube_texels(128, 5)
+    ms = time_events(torch, lambda: aa.prefilter_cube(pyr0, 512, 256, 128, 5, 1024), 5)
+    res["prefilter_128x6x5_1024spp"] = {"ms": ms, "texels_per_s": texels0 / (ms * 1e-3),
+                                        "samples_per_s": texels0 * 1024 / (ms * 1e-3)}
+    ms = time_events(torch, lambda: aa.irradiance_cube(pyr0, 512, 256, 32), 5)
+    res["irradiance_32x6_45000"] = {"ms": ms, "texels_per_s": 6 * 32 * 32 / (ms * 1e-3),
+                                    "samples_per_s": 6 * 32 * 32 * 45000 / (ms * 1e-3)}
+    ms = time_events(torch, lambda: aa.brdf_lut(128, 1024), 5)
+    res["brdf_lut_128_1024"] = {"ms": ms}
+    env1 = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    pyr1 = aa.env_pyramid(torch.from_numpy(env1).cuda())
+    texels1 = aa.cube_texels(512, 6)
+    ms = time_events(torch, lambda: aa.prefilter_cube(pyr1, 2048, 1024, 512, 6, 1024), 2)
+    res["prefilter_512x6x6_1024spp"] = {"ms": ms, "texels_per_s": texels1 / (ms * 1e-3),
+                                        "samples_per_s": texels1 * 1024 / (ms * 1e-3)}
+    return res
+
+
+def cpu_baseline(aa, scene, ibl, env) -> dict:
+    """The FP64 oracle (a from-source restatement of the reference shaders) on
+    the host cores: the whole 3840x2160 frame, 64 lights, SSAO."""
+    from tests import oracle_lib as ol
+
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    views = ibl.views_numpy()
+    oenv = ol.Env(env)
+    ys, xs = np.mgrid[0:HEIGHT, 0:WIDTH]
+    pix = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+    t0 = time.perf_counter()
+    ol.resolve(scene, views, oenv, pix)
+    dt = time.perf_counter() - t0
+    return {"value": pix.shape[0] / dt / 1e6, "unit": "Mpix/s", "cores": ol.max_threads(), "kind": "port",
+            "sample": f"full {WIDTH}x{HEIGHT} frame, {LIGHTS} lights, SSAO 16 (FP64 oracle, {dt:.1f} s)"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the resolve and precompute are single-GPU kernels here "
+              "(see DESIGN.md for how the path would shard); use --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import althea_amd as aa
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
+                      brdf_lut_size=128, brdf_lut_samples=1024)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
+    gbuf = aa.GBufferResources(*scene.gbuffer_tensors("cuda", fp16=args.fp16_gbuffer))
+    lights = scene.lights_tensor("cuda")
+    glob, ssao = scene.globals, scene.ssao
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+
+    def step():
+        aa.DeferredRendering.resolve(gbuf, ibl, lights, glob, ssao, out=out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        step()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end) / args.steps
+    px = WIDTH * HEIGHT
+    value = px / (ms * 1e-3) / 1e6
+    nbytes = algorithmic_bytes(ibl, args.fp16_gbuffer)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "4K deferred PBR+IBL Mpix/s & prefilter texels/s; MLP vs cubemap IBL @<=1e-3 err",
+        "value": value, "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "wall_s": wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded fBm G-buffer, log-normal + sun-lobe HDR map)",
+        "config": {"workload": f"deferred resolve {WIDTH}x{HEIGHT}, {'FP16' if args.fp16_gbuffer else 'FP32'} "
+                               f"G-buffer 4xvec4, {LIGHTS} point lights, SSAO 16, IBL 512^2x6 cube 6 mips",
+                   "pixels": px, "lights": LIGHTS, "gbuffer": "fp16" if args.fp16_gbuffer else "fp32"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": nbytes, "kernel": "resolve_kernel"},
+    }
+    if not args.no_secondary:
+        line["secondary"] = secondary(torch, aa)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

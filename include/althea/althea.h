@@ -253,6 +253,52 @@ ALTHEA_API althea_status althea_deferred_resolve(const althea_gbuffer_view* gbuf
                                                  unsigned flags, void* d_out, float* d_ao_out,
                                                  hipStream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* Learned prefiltered radiance (no counterpart in the reference)            */
+/* ------------------------------------------------------------------------ */
+
+/* A 28 -> 64 -> 64 -> 64 -> 3 MLP standing in for the prefiltered cube:
+ * input (reflection dir, sin/cos(2^k pi dir) for k = 0..3, roughness), SiLU
+ * hidden layers, log-radiance output.  BF16 weights and activations, FP32
+ * accumulation (gfx950 MFMA), FP32 master weights trained with Adam on the
+ * mean squared log-radiance error.  Parameters in the order W1[64][28] b1[64]
+ * W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3] (10371 floats). */
+typedef struct althea_mlp althea_mlp;
+
+typedef struct althea_mlp_desc {
+  uint64_t seed;          /* weight initialisation */
+  float learning_rate;    /* Adam (1e-3) */
+  float beta1, beta2, epsilon;
+  float init_output_bias; /* initial log-radiance bias */
+} althea_mlp_desc;
+
+ALTHEA_API void althea_mlp_desc_default(althea_mlp_desc* desc);
+ALTHEA_API int althea_mlp_param_count(void);
+ALTHEA_API althea_status althea_mlp_create(const althea_mlp_desc* desc, althea_mlp** out);
+ALTHEA_API althea_status althea_mlp_destroy(althea_mlp* mlp);
+/* Host copies of the FP32 master parameters (synchronous on `stream`);
+ * set_params also resets the Adam moments and step. */
+ALTHEA_API althea_status althea_mlp_set_params(althea_mlp* mlp, const float* host_params, hipStream_t stream);
+ALTHEA_API althea_status althea_mlp_get_params(const althea_mlp* mlp, float* host_params, hipStream_t stream);
+/* One Adam step on a batch: d_dir_rough float4 (dir.xyz, roughness) and
+ * d_targets float4 (linear rgb radiance > 0) per sample; d_loss (optional)
+ * receives the batch's mean squared log error before the step. */
+ALTHEA_API althea_status althea_mlp_train_step(althea_mlp* mlp, const float* d_dir_rough, const float* d_targets,
+                                               int count, float* d_loss, hipStream_t stream);
+/* The batch gradient (10371 floats, parameter order) without an update. */
+ALTHEA_API althea_status althea_mlp_gradients(althea_mlp* mlp, const float* d_dir_rough, const float* d_targets,
+                                              int count, float* d_grad, float* d_loss, hipStream_t stream);
+/* Radiance exp(y) per query: d_out float4 (rgb, 1). */
+ALTHEA_API althea_status althea_mlp_infer(const althea_mlp* mlp, const float* d_dir_rough, int count, float* d_out,
+                                          hipStream_t stream);
+/* Bytes of inference state (BF16 weights + FP32 biases). */
+ALTHEA_API size_t althea_mlp_state_bytes(const althea_mlp* mlp);
+
+/* Diagnostic: one 16x16x32 BF16 MFMA through the lane layout the MLP kernels
+ * assume.  d_a 16x32, d_b 32x16 (row-major FP32, rounded to BF16 on load),
+ * d_d 16x16 row-major FP32 = A . B when the assumed layout holds. */
+ALTHEA_API althea_status althea_mfma_bf16_probe(const float* d_a, const float* d_b, float* d_d, hipStream_t stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -299,12 +299,19 @@ inline dvec3 face_dir(int face, double sc, double tc) {
     default: return {-sc, -tc, -1.0};
   }
 }
-// Raw (unnormalised) direction of a texel centre: exact dyadic arithmetic for
-// power-of-two faces.
+// Raw (unnormalised) direction of a texel centre on the integer lattice of
+// half-texels, (2x+1-S, 2y+1-S, S) mapped through the face table: exact in any
+// precision, so the frame's |x| > |y| tie-break agrees everywhere.
 inline dvec3 cube_texel_raw_dir(int face, int x, int y, int size) {
-  double sc = (2.0 * x + 1.0) / size - 1.0;
-  double tc = (2.0 * y + 1.0) / size - 1.0;
-  return face_dir(face, sc, tc);
+  dvec3 d = face_dir(face, static_cast<double>(2 * x + 1 - size), static_cast<double>(2 * y + 1 - size));
+  // face_dir puts +-1 on the major axis; scale it to +-size.
+  double s = static_cast<double>(size);
+  switch (face) {
+    case 0: case 1: d.x *= s; break;
+    case 2: case 3: d.y *= s; break;
+    default: d.z *= s; break;
+  }
+  return d;
 }
 inline void cube_select(dvec3 d, int& face, double& s, double& t) {
   double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);

@@ -297,8 +297,11 @@ void oracle_mlp_forward(const float* params, const float* dir_rough, int count, 
   }
 }
 
-double oracle_mlp_loss_grad(const float* params, const float* dir_rough, const float* targets, int count,
-                            int bf16, double* grad) {
+}  // extern "C"
+
+template <class P>
+static double mlp_loss_grad_all(const P* params, const float* dir_rough, const float* targets, int count, int bf16,
+                                double* grad) {
   int threads = omp_get_max_threads();
   std::vector<double> partial(static_cast<size_t>(threads) * mlp::kParams, 0.0);
   std::vector<double> losses(threads, 0.0);
@@ -322,9 +325,22 @@ double oracle_mlp_loss_grad(const float* params, const float* dir_rough, const f
   return loss;
 }
 
+extern "C" {
+
 void oracle_adam(double* params, double* m, double* v, const double* grad, int n, int step, double lr,
                  double b1, double b2, double eps) {
   mlp::adam(params, m, v, grad, n, step, lr, b1, b2, eps);
+}
+
+double oracle_mlp_loss_grad(const float* params, const float* dir_rough, const float* targets, int count,
+                            int bf16, double* grad) {
+  return mlp_loss_grad_all(params, dir_rough, targets, count, bf16, grad);
+}
+
+// FP64 parameters (exact mode), for finite-difference checks.
+double oracle_mlp_loss_grad64(const double* params, const float* dir_rough, const float* targets, int count,
+                              double* grad) {
+  return mlp_loss_grad_all(params, dir_rough, targets, count, 0, grad);
 }
 
 float oracle_bf16_round(float x) { return mlp::bf16_round(x); }

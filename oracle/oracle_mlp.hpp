@@ -73,7 +73,9 @@ struct Activations {
 };
 
 // Forward pass of one sample.  Inputs: unit-ish direction and roughness.
-inline void forward(const float* params, const double dir[3], double roughness, bool bf16,
+// P is float (the library's FP32 master copy) or double (finite differences).
+template <class P>
+inline void forward(const P* params, const double dir[3], double roughness, bool bf16,
                     Activations& a) {
   double enc[kIn];
   encode(dir, roughness, enc);
@@ -108,7 +110,8 @@ inline void forward(const float* params, const double dir[3], double roughness, 
 // the network predicts log radiance.  grad has kParams entries (overwritten).
 // `norm_count` is the batch size the mean is taken over (count for a whole
 // batch, larger when this call covers a slice of it).
-inline double loss_grad(const float* params, const float* dir_rough, const float* targets, int count,
+template <class P>
+inline double loss_grad(const P* params, const float* dir_rough, const float* targets, int count,
                         int norm_count, bool bf16, double* grad) {
   std::memset(grad, 0, sizeof(double) * kParams);
   double loss = 0.0;

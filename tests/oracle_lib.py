@@ -66,6 +66,8 @@ def lib() -> C.CDLL:
         L.oracle_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
         L.oracle_mlp_loss_grad.argtypes = [vp, vp, vp, i32, i32, vp]
         L.oracle_mlp_loss_grad.restype = dbl
+        L.oracle_mlp_loss_grad64.argtypes = [vp, vp, vp, i32, vp]
+        L.oracle_mlp_loss_grad64.restype = dbl
         L.oracle_adam.argtypes = [vp, vp, vp, vp, i32, i32, dbl, dbl, dbl, dbl]
         L.oracle_bf16_round.argtypes = [C.c_float]
         L.oracle_bf16_round.restype = C.c_float
@@ -298,6 +300,16 @@ def mlp_loss_grad(params, dir_rough, targets, bf16: bool):
     t = _c(targets, np.float32)
     g = np.empty(mlp_param_count(), np.float64)
     loss = lib().oracle_mlp_loss_grad(_p(p), _p(d), _p(t), d.shape[0], 1 if bf16 else 0, _p(g))
+    return loss, g
+
+
+def mlp_loss_grad64(params, dir_rough, targets):
+    """Exact-mode loss and gradient with FP64 parameters."""
+    p = _c(params, np.float64)
+    d = _c(dir_rough, np.float32)
+    t = _c(targets, np.float32)
+    g = np.empty(mlp_param_count(), np.float64)
+    loss = lib().oracle_mlp_loss_grad64(_p(p), _p(d), _p(t), d.shape[0], _p(g))
     return loss, g
 
 
