@@ -91,9 +91,16 @@ althea_status launch_pyramid(const float* level0, int w, int h, float* pyr, hipS
 
 // ------------------------------------------------------- GGX sample tables
 // One block per roughness level.  Entry e (in sample order, NdotL > 0 only):
-// float4 (L.t, L.b, L.n, lod) in the frame (T, B, N), lod clamped to the
-// pyramid's [0, levels-1].  counts[m] = kept entries, wsum[m] = FP64 sum of
-// L.n, wsum32[m] = the same sum accumulated in FP32 in sample order.
+// L in the frame (T, B, N), and the source LOD clamped to the pyramid's
+// [0, levels-1] split into (l0, l1 = min(l0+1, levels-1), delta).
+// counts[m] = kept entries, wsum[m] = FP64 sum of L.n, wsum32[m] = the same
+// sum accumulated in FP32 in sample order.
+struct GgxEntry {
+  float lx, ly, lz, delta;
+  int l0, l1, pad0, pad1;
+};
+static_assert(sizeof(GgxEntry) == 32, "one scalar load per entry");
+
 struct TableParams {
   int samples;
   int mips;
@@ -103,7 +110,7 @@ struct TableParams {
 
 constexpr int kTableThreads = 256;
 
-__global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p, float4* __restrict__ tables,
+__global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p, GgxEntry* __restrict__ tables,
                                                                   int* __restrict__ counts,
                                                                   double* __restrict__ wsum,
                                                                   float* __restrict__ wsum32) {
@@ -122,12 +129,12 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
     s_w32 = 0.0f;
   }
   double my_sum = 0.0;
-  float4* tab = tables + static_cast<size_t>(m) * p.samples;
+  GgxEntry* tab = tables + static_cast<size_t>(m) * p.samples;
   __syncthreads();
   for (int chunk = 0; chunk < p.samples; chunk += kTableThreads) {
     int s = chunk + tid;
     bool keep = false;
-    float4 entry = make_float4(0.f, 0.f, 0.f, 0.f);
+    GgxEntry entry{};
     float w32 = 0.0f;
     if (s < p.samples) {
       double x1 = static_cast<double>(s) / p.samples;
@@ -149,8 +156,15 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
           lod = 0.5 * log2(sa_sample / p.sa_texel);
         }
         lod = fmin(fmax(lod, 0.0), static_cast<double>(p.env_levels - 1));
-        entry = make_float4(static_cast<float>(2.0 * hz * hx), static_cast<float>(2.0 * hz * hy),
-                            static_cast<float>(lz), static_cast<float>(lod));
+        // Rounded to FP32 as a sampler LOD would be, then split.
+        const float lodf = static_cast<float>(lod);
+        const float fl = floorf(lodf);
+        entry.lx = static_cast<float>(2.0 * hz * hx);
+        entry.ly = static_cast<float>(2.0 * hz * hy);
+        entry.lz = static_cast<float>(lz);
+        entry.delta = lodf - fl;
+        entry.l0 = static_cast<int>(fl);
+        entry.l1 = min(entry.l0 + 1, p.env_levels - 1);
         my_sum += lz;
         w32 = static_cast<float>(lz);
       }
@@ -192,7 +206,7 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
 
 // ------------------------------------------------------------ prefilter
 struct PrefilterParams {
-  EnvLevels env;
+  LevelC lv[kMaxLevels];  // source pyramid levels
   CubeMips cube;
   int samples;
   int tile_start[kMaxLevels + 1];  // 8x8-texel tiles before each mip
@@ -202,8 +216,8 @@ constexpr int kPrefilterTile = 8;
 constexpr int kBlockSums = 32;  // FP32 partial sums flushed to FP64 every 32 samples
 
 template <bool kF64Sums>
-__global__ __launch_bounds__(64) void prefilter_cube_kernel(const float4* __restrict__ pyr,
-                                                            const float4* __restrict__ tables,
+__global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restrict__ pyr,
+                                                            const GgxEntry* __restrict__ tables,
                                                             const int* __restrict__ counts,
                                                             const double* __restrict__ wsum,
                                                             const PrefilterParams p, float4* __restrict__ cube) {
@@ -220,28 +234,29 @@ __global__ __launch_bounds__(64) void prefilter_cube_kernel(const float4* __rest
   const bool active = x < S && y < S;
   const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
 
-  const float4* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
+  const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
   const int count = counts[m];
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
   for (int e = 0; e < count; ++e) {
-    const float4 en = tab[e];
-    f3 L = mk(fmaf(fr.t.x, en.x, fmaf(fr.b.x, en.y, fr.n.x * en.z)),
-              fmaf(fr.t.y, en.x, fmaf(fr.b.y, en.y, fr.n.y * en.z)),
-              fmaf(fr.t.z, en.x, fmaf(fr.b.z, en.y, fr.n.z * en.z)));
+    const GgxEntry en = tab[e];
+    f3 L = mk(fmaf(fr.t.x, en.lx, fmaf(fr.b.x, en.ly, fr.n.x * en.lz)),
+              fmaf(fr.t.y, en.lx, fmaf(fr.b.y, en.ly, fr.n.y * en.lz)),
+              fmaf(fr.t.z, en.lx, fmaf(fr.b.z, en.ly, fr.n.z * en.lz)));
     float yaw, pitch;
     equirect_angles(L, yaw, pitch);
-    f3 v = env_trilinear(pyr, p.env, yaw, pitch, en.w);
-    sr = fmaf(v.x, en.z, sr);
-    sg = fmaf(v.y, en.z, sg);
-    sb = fmaf(v.z, en.z, sb);
+    f3 v = level_bilinear(pyr, p.lv[en.l0], yaw, pitch);
+    if (en.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, p.lv[en.l1], yaw, pitch), en.delta);
+    sr = fmaf(v.x, en.lz, sr);
+    sg = fmaf(v.y, en.lz, sg);
+    sb = fmaf(v.z, en.lz, sb);
     if (kF64Sums) {
       if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
         ar += sr; ag += sg; ab += sb;
         sr = sg = sb = 0.0f;
       }
     } else {
-      sw += en.z;
+      sw += en.lz;
     }
   }
   float4 out;
@@ -255,16 +270,9 @@ __global__ __launch_bounds__(64) void prefilter_cube_kernel(const float4* __rest
   if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
 }
 
-static EnvLevels env_levels_of(int w, int h) {
+static void levels_of(int w, int h, LevelC* out) {
   PyramidGeometry g = pyramid_geometry(w, h);
-  EnvLevels e{};
-  e.levels = g.levels;
-  for (int l = 0; l < g.levels; ++l) {
-    e.w[l] = g.w[l];
-    e.h[l] = g.h[l];
-    e.off[l] = g.off[l];
-  }
-  return e;
+  for (int l = 0; l < g.levels; ++l) out[l] = make_level(g.w[l], g.h[l], g.off[l]);
 }
 
 static CubeMips cube_mips_of(int size, int mips) {
@@ -278,15 +286,15 @@ static CubeMips cube_mips_of(int size, int mips) {
 static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 size_t prefilter_workspace_bytes(int, int, int, int mips, int samples) {
-  return align256(sizeof(float4) * static_cast<size_t>(mips) * samples) + align256(sizeof(int) * mips) +
+  return align256(sizeof(GgxEntry) * static_cast<size_t>(mips) * samples) + align256(sizeof(int) * mips) +
          align256(sizeof(double) * mips) + align256(sizeof(float) * mips);
 }
 
 althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,
                                     unsigned flags, float* cube, void* workspace, hipStream_t s) {
   char* ws = static_cast<char*>(workspace);
-  float4* tables = reinterpret_cast<float4*>(ws);
-  ws += align256(sizeof(float4) * static_cast<size_t>(mips) * samples);
+  GgxEntry* tables = reinterpret_cast<GgxEntry*>(ws);
+  ws += align256(sizeof(GgxEntry) * static_cast<size_t>(mips) * samples);
   int* counts = reinterpret_cast<int*>(ws);
   ws += align256(sizeof(int) * mips);
   double* wsum = reinterpret_cast<double*>(ws);
@@ -303,7 +311,7 @@ althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int 
   if (st != ALTHEA_OK) return st;
 
   PrefilterParams p{};
-  p.env = env_levels_of(env_w, env_h);
+  levels_of(env_w, env_h, p.lv);
   p.cube = cube_mips_of(size, mips);
   p.samples = samples;
   int tiles = 0;
@@ -314,13 +322,13 @@ althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int 
     tiles += 6 * tpf * tpf;
   }
   p.tile_start[mips] = tiles;
-  const float4* pyr4 = reinterpret_cast<const float4*>(pyr);
+  const char* pyr_b = reinterpret_cast<const char*>(pyr);
   float4* cube4 = reinterpret_cast<float4*>(cube);
   if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
-    hipLaunchKernelGGL(prefilter_cube_kernel<false>, dim3(tiles), dim3(64), 0, s, pyr4, tables, counts, wsum, p,
+    hipLaunchKernelGGL(prefilter_cube_kernel<false>, dim3(tiles), dim3(64), 0, s, pyr_b, tables, counts, wsum, p,
                        cube4);
   else
-    hipLaunchKernelGGL(prefilter_cube_kernel<true>, dim3(tiles), dim3(64), 0, s, pyr4, tables, counts, wsum, p,
+    hipLaunchKernelGGL(prefilter_cube_kernel<true>, dim3(tiles), dim3(64), 0, s, pyr_b, tables, counts, wsum, p,
                        cube4);
   return check_launch("prefilter_cube_kernel");
 }
@@ -330,15 +338,31 @@ althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int 
 // evaluated per lane in FP32 (sin^2 theta formed without cancellation), sums
 // in FP32 blocks flushed to FP64.
 struct PointParams {
-  EnvLevels env;
+  LevelC lv[kMaxLevels];
+  int levels;
   int samples;
   int count;
   float log2_sa_texel;
 };
 
-__global__ __launch_bounds__(256) void prefilter_points_kernel(const float4* __restrict__ pyr, PointParams p,
+// Trilinear lookup with per-lane LOD: level constants from LDS.
+__device__ __forceinline__ f3 lds_trilinear(const char* __restrict__ pyr, const LevelC* s_lv, int levels, float yaw,
+                                            float pitch, float lod) {
+  const float d = fminf(fmaxf(lod, 0.0f), static_cast<float>(levels - 1));
+  const float fl = floorf(d);
+  const int l0 = static_cast<int>(fl);
+  const float delta = d - fl;
+  f3 v = level_bilinear(pyr, s_lv[l0], yaw, pitch);
+  if (delta != 0.0f) v = lerp3(v, level_bilinear(pyr, s_lv[min(l0 + 1, levels - 1)], yaw, pitch), delta);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void prefilter_points_kernel(const char* __restrict__ pyr, const PointParams p,
                                                                const float4* __restrict__ dir_rough,
                                                                float4* __restrict__ out) {
+  __shared__ LevelC s_lv[kMaxLevels];
+  if (threadIdx.x < kMaxLevels) s_lv[threadIdx.x] = p.lv[threadIdx.x];
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.count) return;
   const float4 q = dir_rough[i];
@@ -346,7 +370,7 @@ __global__ __launch_bounds__(256) void prefilter_points_kernel(const float4* __r
   const float rough = q.w;
   const float a2 = rough * rough;
   const float fn = static_cast<float>(p.samples);
-  const float max_lod = static_cast<float>(p.env.levels - 1);
+  const float max_lod = static_cast<float>(p.levels - 1);
   double ar = 0.0, ag = 0.0, ab = 0.0, aw = 0.0;
   float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
   for (int s = 0; s < p.samples; ++s) {
@@ -372,7 +396,7 @@ __global__ __launch_bounds__(256) void prefilter_points_kernel(const float4* __r
       f3 L = fr.t * lx + fr.b * ly + fr.n * lz;
       float yaw, pitch;
       equirect_angles(L, yaw, pitch);
-      f3 v = env_trilinear(pyr, p.env, yaw, pitch, lod);
+      f3 v = lds_trilinear(pyr, s_lv, p.levels, yaw, pitch, lod);
       sr = fmaf(v.x, lz, sr);
       sg = fmaf(v.y, lz, sg);
       sb = fmaf(v.z, lz, sb);
@@ -393,12 +417,13 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
                                       int count, float* out, void*, hipStream_t s) {
   if (count == 0) return ALTHEA_OK;
   PointParams p{};
-  p.env = env_levels_of(env_w, env_h);
+  levels_of(env_w, env_h, p.lv);
+  p.levels = althea_env_levels(env_w, env_h);
   p.samples = samples;
   p.count = count;
   p.log2_sa_texel = static_cast<float>(std::log2(4.0 * M_PI / (6.0 * static_cast<double>(env_w) * env_h)));
   hipLaunchKernelGGL(prefilter_points_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
-                     reinterpret_cast<const float4*>(pyr), p, reinterpret_cast<const float4*>(dir_rough),
+                     reinterpret_cast<const char*>(pyr), p, reinterpret_cast<const float4*>(dir_rough),
                      reinterpret_cast<float4*>(out));
   return check_launch("prefilter_points_kernel");
 }
@@ -410,7 +435,8 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
 // loads); per-theta FP32 sums are flushed into FP64, then reduced over the
 // four slices of a wave with shuffles and over the four waves in LDS.
 struct IrradianceParams {
-  EnvLevels env;
+  LevelC lv0, lv1;  // the two source levels around LOD log2(W / theta_samples)
+  int levels;
   int size;
   int theta_samples;
   int phi_samples;
@@ -435,7 +461,7 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
 }
 
 template <bool kF64Sums>
-__global__ __launch_bounds__(256) void irradiance_cube_kernel(const float4* __restrict__ pyr,
+__global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ pyr,
                                                               const float2* __restrict__ theta_tab,
                                                               const float4* __restrict__ phi_tab,
                                                               const IrradianceParams p, float4* __restrict__ cube) {
@@ -449,9 +475,6 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const float4* __re
   const int y = (t / tpf) * 4 + (lane_texel >> 2);
   const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
 
-  const float4* lv0 = pyr + p.env.off[p.l0];
-  const float4* lv1 = pyr + p.env.off[p.l1];
-  const int w0 = p.env.w[p.l0], h0 = p.env.h[p.l0], w1 = p.env.w[p.l1], h1 = p.env.h[p.l1];
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float fr32 = 0.0f, fg32 = 0.0f, fb32 = 0.0f;
   for (int i = slice; i < p.theta_samples; i += 16) {
@@ -464,11 +487,8 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const float4* __re
                       fmaf(a.z, ph.y, fr.n.z * ph.x));
       float yaw, pitch;
       equirect_angles(d, yaw, pitch);
-      f3 v = env_bilinear_repeat(lv0, w0, h0, equirect_x(yaw, w0), equirect_y(pitch, h0));
-      if (p.delta != 0.0f) {
-        f3 v1 = env_bilinear_repeat(lv1, w1, h1, equirect_x(yaw, w1), equirect_y(pitch, h1));
-        v = lerp3(v, v1, p.delta);
-      }
+      f3 v = level_bilinear(pyr, p.lv0, yaw, pitch);
+      if (p.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, p.lv1, yaw, pitch), p.delta);
       sr = fmaf(v.x, ph.z, sr);
       sg = fmaf(v.y, ph.z, sg);
       sb = fmaf(v.z, ph.z, sb);
@@ -531,21 +551,24 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   if (st != ALTHEA_OK) return st;
 
   IrradianceParams p{};
-  p.env = env_levels_of(env_w, env_h);
+  PyramidGeometry g = pyramid_geometry(env_w, env_h);
+  p.levels = g.levels;
   p.size = size;
   p.theta_samples = theta_samples;
   p.phi_samples = phi;
   // Source LOD log2(W / thetaSamples) (GenIrradianceMap.comp:139) in FP32 as
   // the shader computes it, clamped to the pyramid.
   float lod = std::log2(static_cast<float>(env_w) / static_cast<float>(theta_samples));
-  lod = std::min(std::max(lod, 0.0f), static_cast<float>(p.env.levels - 1));
+  lod = std::min(std::max(lod, 0.0f), static_cast<float>(p.levels - 1));
   p.l0 = static_cast<int>(std::floor(lod));
-  p.l1 = std::min(p.l0 + 1, p.env.levels - 1);
+  p.l1 = std::min(p.l0 + 1, p.levels - 1);
   p.delta = lod - static_cast<float>(p.l0);
+  p.lv0 = make_level(g.w[p.l0], g.h[p.l0], g.off[p.l0]);
+  p.lv1 = make_level(g.w[p.l1], g.h[p.l1], g.off[p.l1]);
   p.scale = M_PI / theta_samples / phi;
   int tpf = (size + 3) / 4;
   dim3 grid(6 * tpf * tpf);
-  const float4* pyr4 = reinterpret_cast<const float4*>(pyr);
+  const char* pyr4 = reinterpret_cast<const char*>(pyr);
   float4* cube4 = reinterpret_cast<float4*>(cube);
   if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
     hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
@@ -627,23 +650,36 @@ althea_status launch_cube_sample(const float* cube, int size, int mips, const fl
   return check_launch("cube_sample_kernel");
 }
 
-__global__ void env_sample_kernel(const float4* __restrict__ pyr, const EnvLevels lv,
-                                  const float4* __restrict__ dir_lod, int count, float4* __restrict__ out) {
+struct EnvSampleParams {
+  LevelC lv[kMaxLevels];
+  int levels;
+  int count;
+};
+
+__global__ __launch_bounds__(256) void env_sample_kernel(const char* __restrict__ pyr, const EnvSampleParams p,
+                                                         const float4* __restrict__ dir_lod,
+                                                         float4* __restrict__ out) {
+  __shared__ LevelC s_lv[kMaxLevels];
+  if (threadIdx.x < kMaxLevels) s_lv[threadIdx.x] = p.lv[threadIdx.x];
+  __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+  if (i >= p.count) return;
   float4 q = dir_lod[i];
   float yaw, pitch;
   equirect_angles(mk(q.x, q.y, q.z), yaw, pitch);
-  f3 v = env_trilinear(pyr, lv, yaw, pitch, q.w);
+  f3 v = lds_trilinear(pyr, s_lv, p.levels, yaw, pitch, q.w);
   out[i] = make_float4(v.x, v.y, v.z, 1.0f);
 }
 
 althea_status launch_env_sample(const float* pyr, int w, int h, const float* dir_lod, int count, float* out, void*,
                                 hipStream_t s) {
   if (count == 0) return ALTHEA_OK;
-  EnvLevels lv = env_levels_of(w, h);
+  EnvSampleParams p{};
+  levels_of(w, h, p.lv);
+  p.levels = althea_env_levels(w, h);
+  p.count = count;
   hipLaunchKernelGGL(env_sample_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
-                     reinterpret_cast<const float4*>(pyr), lv, reinterpret_cast<const float4*>(dir_lod), count,
+                     reinterpret_cast<const char*>(pyr), p, reinterpret_cast<const float4*>(dir_lod),
                      reinterpret_cast<float4*>(out));
   return check_launch("env_sample_kernel");
 }

@@ -45,6 +45,7 @@ struct ResolveParams {
   int tonemap;
   int ssao_on;
   int ssao_samples;
+  float ssao_inv_samples;
   float ssao_radius, ssao_bias;
   float kernel[ALTHEA_SSAO_MAX_SAMPLES][4];
   float noise[16][4];
@@ -77,6 +78,11 @@ __device__ __forceinline__ void store_out(void* out, size_t idx, f3 c) {
   }
 }
 
+// Smith-Schlick view term gv = NdotV / (NdotV (1 - k) + k) (PBRMaterial.glsl:55-57).
+__device__ __forceinline__ float gv_of(float n_dot_v, float k) {
+  return n_dot_v * rcp_fast(fmaf(n_dot_v, 1.0f - k, k));
+}
+
 __device__ __forceinline__ float pow5(float x) {
   float x2 = x * x;
   return x2 * x2 * x;
@@ -99,32 +105,44 @@ template <int kFmt>
 __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int py, f3 P, f3 N) {
   const float4 nz = s_noise[(px & 3) + 4 * (py & 3)];
   const f3 r = mk(nz.x, nz.y, nz.z);
-  const f3 T = normalize(r - N * dot(r, N));
+  const f3 T = normalize_fast(r - N * dot(r, N));
   const f3 B = cross(N, T);
   const float frag_depth = view_depth(p, P);
   const float fw = static_cast<float>(p.width), fh = static_cast<float>(p.height);
+  // Clip coordinates of S = P + radius * TBN k are affine in k: precompute
+  // the clip and depth of P and of the three scaled basis vectors.
+  const f3 Tr = T * p.ssao_radius, Br = B * p.ssao_radius, Nr = N * p.ssao_radius;
+  auto clip3 = [&](f3 v, float w1, float& cx, float& cy, float& cw) {
+    cx = fmaf(p.vp[0], v.x, fmaf(p.vp[4], v.y, fmaf(p.vp[8], v.z, p.vp[12] * w1)));
+    cy = fmaf(p.vp[1], v.x, fmaf(p.vp[5], v.y, fmaf(p.vp[9], v.z, p.vp[13] * w1)));
+    cw = fmaf(p.vp[3], v.x, fmaf(p.vp[7], v.y, fmaf(p.vp[11], v.z, p.vp[15] * w1)));
+  };
+  float px0, py0, pw0, tx, ty, tw, bx, by, bw, nx, ny, nw;
+  clip3(P, 1.0f, px0, py0, pw0);
+  clip3(Tr, 0.0f, tx, ty, tw);
+  clip3(Br, 0.0f, bx, by, bw);
+  clip3(Nr, 0.0f, nx, ny, nw);
+  auto vz = [&](f3 v) { return -fmaf(p.view_z[0], v.x, fmaf(p.view_z[1], v.y, p.view_z[2] * v.z)); };
+  const float dT = vz(Tr), dB = vz(Br), dN = vz(Nr);
   float occ = 0.0f;
   for (int k = 0; k < p.ssao_samples; ++k) {
     const float kx = p.kernel[k][0], ky = p.kernel[k][1], kz = p.kernel[k][2];
-    const f3 off = mk(fmaf(T.x, kx, fmaf(B.x, ky, N.x * kz)), fmaf(T.y, kx, fmaf(B.y, ky, N.y * kz)),
-                      fmaf(T.z, kx, fmaf(B.z, ky, N.z * kz)));
-    const f3 S = mk(fmaf(off.x, p.ssao_radius, P.x), fmaf(off.y, p.ssao_radius, P.y),
-                    fmaf(off.z, p.ssao_radius, P.z));
-    const float sd = view_depth(p, S);
-    const float cx = fmaf(p.vp[0], S.x, fmaf(p.vp[4], S.y, fmaf(p.vp[8], S.z, p.vp[12])));
-    const float cy = fmaf(p.vp[1], S.x, fmaf(p.vp[5], S.y, fmaf(p.vp[9], S.z, p.vp[13])));
-    const float cw = fmaf(p.vp[3], S.x, fmaf(p.vp[7], S.y, fmaf(p.vp[11], S.z, p.vp[15])));
+    const float sd = fmaf(dT, kx, fmaf(dB, ky, fmaf(dN, kz, frag_depth)));
+    const float cx = fmaf(tx, kx, fmaf(bx, ky, fmaf(nx, kz, px0)));
+    const float cy = fmaf(ty, kx, fmaf(by, ky, fmaf(ny, kz, py0)));
+    const float cw = fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
     if (!(cw > 0.0f)) continue;
-    const float u = 0.5f * (cx / cw) + 0.5f, v = 0.5f * (cy / cw) + 0.5f;
+    const float inv_w = rcp_fast(cw);
+    const float u = fmaf(0.5f * cx, inv_w, 0.5f), v = fmaf(0.5f * cy, inv_w, 0.5f);
     if (!(u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f)) continue;
-    const int qx = min(static_cast<int>(floorf(u * fw)), p.width - 1);
-    const int qy = min(static_cast<int>(floorf(v * fh)), p.height - 1);
+    const int qx = min(static_cast<int>(u * fw), p.width - 1);
+    const int qy = min(static_cast<int>(v * fh), p.height - 1);
     const float4 q = load_gbuf<kFmt>(p.position, static_cast<size_t>(qy) * p.width + qx);
     if (q.w == 0.0f) continue;
     const float qd = view_depth(p, mk(q.x, q.y, q.z));
-    if (qd <= sd - p.ssao_bias) occ += smoothstep01(p.ssao_radius / fabsf(frag_depth - qd));
+    if (qd <= sd - p.ssao_bias) occ += smoothstep01(p.ssao_radius * rcp_fast(fabsf(frag_depth - qd)));
   }
-  return 1.0f - occ / static_cast<float>(p.ssao_samples);
+  return fmaf(-occ, p.ssao_inv_samples, 1.0f);
 }
 
 __device__ __forceinline__ void lut_sample(const ResolveParams& p, float u, float v, float& A, float& B) {
@@ -143,54 +161,55 @@ __device__ __forceinline__ void lut_sample(const ResolveParams& p, float u, floa
   B = fmaf(bb - tb, ay, tb);
 }
 
+constexpr int kLightChunk = 256;  // lights staged in LDS per pass
+
 template <int kFmt, bool kOutF16>
 __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
   __shared__ float4 s_noise[16];
+  __shared__ float4 s_light_pos[kLightChunk];  // xyz position, w = 1 if the emission is grey
+  __shared__ float4 s_light_em[kLightChunk];
   if (threadIdx.x < 16)
     s_noise[threadIdx.x] =
         make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
-  __syncthreads();
   const int px = blockIdx.x * 64 + (threadIdx.x & 63);
   const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (px >= p.width || py >= p.height) return;
-  const size_t idx = static_cast<size_t>(py) * p.width + px;
-  const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
-  const f3 dir = mk(fmaf(p.rayy[0], cy, fmaf(p.rayx[0], cx, p.ray0[0])),
-                    fmaf(p.rayy[1], cy, fmaf(p.rayx[1], cx, p.ray0[1])),
-                    fmaf(p.rayy[2], cy, fmaf(p.rayx[2], cx, p.ray0[2])));
+  const bool inside = px < p.width && py < p.height;
+  const size_t idx = static_cast<size_t>(min(py, p.height - 1)) * p.width + min(px, p.width - 1);
+  const float cxp = static_cast<float>(px) + 0.5f, cyp = static_cast<float>(py) + 0.5f;
+  const f3 dir = mk(fmaf(p.rayy[0], cyp, fmaf(p.rayx[0], cxp, p.ray0[0])),
+                    fmaf(p.rayy[1], cyp, fmaf(p.rayx[1], cxp, p.ray0[1])),
+                    fmaf(p.rayy[2], cyp, fmaf(p.rayx[2], cxp, p.ray0[2])));
   const float4 pos = load_gbuf<kFmt>(p.position, idx);
-  if (pos.w == 0.0f) {
+  const bool background = pos.w == 0.0f;
+  if (background) {
     // Background (DeferredPass.frag:44-53): env LOD 0, CLAMP_TO_EDGE.
-    const float yaw = atan2f(dir.z, dir.x);
-    const float pitch = atan2f(dir.y, sqrtf(fmaf(dir.x, dir.x, dir.z * dir.z)));
+    const float yaw = atan2_fast(dir.z, dir.x);
+    const float pitch = atan2_xpos(dir.y, sqrt_fast(fmaf(dir.x, dir.x, dir.z * dir.z)));
     f3 c = env_bilinear_clamp(p.env, p.env_w, p.env_h, equirect_x(yaw, p.env_w), equirect_y(pitch, p.env_h));
     if (p.tonemap) c = tonemap(c, p.exposure);
-    if (p.ao_out) p.ao_out[idx] = 1.0f;
-    store_out<kOutF16>(p.out, idx, c);
-    return;
+    if (inside) {
+      if (p.ao_out) p.ao_out[idx] = 1.0f;
+      store_out<kOutF16>(p.out, idx, c);
+    }
   }
-  const float4 nrm = load_gbuf<kFmt>(p.normal, idx);
-  const float4 alb = load_gbuf<kFmt>(p.albedo, idx);
-  const float4 mro = load_gbuf<kFmt>(p.mro, idx);
+  // Every lane stays for the light staging barriers; background lanes and
+  // lanes outside the image shade nothing, store nothing and skip the other
+  // G-buffer planes.
+  const bool shade = inside && !background;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 nrm = shade ? load_gbuf<kFmt>(p.normal, idx) : zero4;
+  const float4 alb = shade ? load_gbuf<kFmt>(p.albedo, idx) : zero4;
+  const float4 mro = shade ? load_gbuf<kFmt>(p.mro, idx) : zero4;
   const f3 P = mk(pos.x, pos.y, pos.z);
-  const f3 N = normalize(mk(nrm.x, nrm.y, nrm.z));
+  const f3 N = shade ? normalize_fast(mk(nrm.x, nrm.y, nrm.z)) : mk(0.0f, 0.0f, 1.0f);
   const f3 base = mk(alb.x, alb.y, alb.z);
-  const float metallic = mro.x, roughness = mro.y;
+  const float metallic = mro.x, roughness = shade ? mro.y : 1.0f;
   float ao = mro.z;
 
-  const f3 I = normalize(dir);
-  const float nd = 2.0f * dot(N, I);
-  const f3 R = mk(fmaf(-nd, N.x, I.x), fmaf(-nd, N.y, I.y), fmaf(-nd, N.z, I.z));
-  const f3 refl = cube_sample(p.prefiltered, p.pref, R, roughness * static_cast<float>(p.pref.mips - 1));
-  int face;
-  float pc, pt;
-  cube_select(N, face, pc, pt);
-  const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
-  if (p.ssao_on) ao = ssao<kFmt>(p, s_noise, px, py, P, N);
-  if (p.ao_out) p.ao_out[idx] = ao;
-
-  // pbrMaterial (PBRMaterial.glsl:72-162); V = I (camera -> surface).
-  const float n_dot_v = fmaxf(-dot(N, I), 0.0f);
+  const f3 I = normalize_fast(dir);
+  f3 color = mk(0.0f, 0.0f, 0.0f);
+  const float NI = dot(N, I);
+  const float n_dot_v = fmaxf(-NI, 0.0f);
   const f3 F0 = mk(fmaf(metallic, base.x - 0.04f, 0.04f), fmaf(metallic, base.y - 0.04f, 0.04f),
                    fmaf(metallic, base.z - 0.04f, 0.04f));
   const float a = roughness * roughness;
@@ -199,8 +218,17 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
   const float om_r = 1.0f - roughness;
   const f3 dF = mk(fmaxf(om_r, F0.x) - F0.x, fmaxf(om_r, F0.y) - F0.y, fmaxf(om_r, F0.z) - F0.z);
   const f3 kd = base * (1.0f - metallic);
-  f3 color;
-  {
+  if (shade) {
+    const float nd = 2.0f * NI;
+    const f3 R = mk(fmaf(-nd, N.x, I.x), fmaf(-nd, N.y, I.y), fmaf(-nd, N.z, I.z));
+    const f3 refl = cube_sample(p.prefiltered, p.pref, R, roughness * static_cast<float>(p.pref.mips - 1));
+    int face;
+    float pc, pt;
+    cube_select(N, face, pc, pt);
+    const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
+    if (p.ssao_on) ao = ssao<kFmt>(p, s_noise, px, py, P, N);
+    if (p.ao_out) p.ao_out[idx] = ao;
+    // Environment term of pbrMaterial (PBRMaterial.glsl:101-114).
     const float fx = pow5(1.0f - n_dot_v);
     const f3 F = mk(fmaf(dF.x, fx, F0.x), fmaf(dF.y, fx, F0.y), fmaf(dF.z, fx, F0.z));
     float A, B;
@@ -209,28 +237,74 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
                (irr.y * ((1.0f - F.y) * kd.y) + refl.y * fmaf(F.y, A, B)) * ao,
                (irr.z * ((1.0f - F.z) * kd.z) + refl.z * fmaf(F.z, A, B)) * ao);
   }
-  const float gv = n_dot_v / fmaf(n_dot_v, 1.0f - kdir, kdir);
-  for (int i = 0; i < p.light_count; ++i) {
-    const althea_point_light lt = p.lights[i];
-    f3 L = mk(lt.position[0] - P.x, lt.position[1] - P.y, lt.position[2] - P.z);
-    const float d2 = dot(L, L);
-    const float d = sqrtf(d2);
-    L = L * (1.0f / d);
-    const float inv_d2 = 1.0f / d2;
-    const f3 H = normalize(I + L);
-    const float n_dot_l = fmaxf(dot(N, L), 0.0f);
-    const float n_dot_h = fmaxf(dot(N, H), 0.0f);
-    const float fh = pow5(1.0f - n_dot_h);
-    const f3 F = mk(fmaf(dF.x, fh, F0.x), fmaf(dF.y, fh, F0.y), fmaf(dF.z, fh, F0.z));
-    const float t = fmaf(n_dot_h * n_dot_h, a2 - 1.0f, 1.0f);
-    const float D = a2 / (kPi * t * t);
-    const float gl = n_dot_l / fmaf(n_dot_l, 1.0f - kdir, kdir);
-    const float s = D * gv * gl / fmaf(4.0f * n_dot_l, n_dot_v, 0.0001f);
-    const float w = inv_d2 * n_dot_l;
-    color.x = fmaf(fmaf(F.x, s, (1.0f - F.x) * kd.x * kInvPi) * lt.emission[0], w, color.x);
-    color.y = fmaf(fmaf(F.y, s, (1.0f - F.y) * kd.y * kInvPi) * lt.emission[1], w, color.y);
-    color.z = fmaf(fmaf(F.z, s, (1.0f - F.z) * kd.z * kInvPi) * lt.emission[2], w, color.z);
+
+  // Point lights (PBRMaterial.glsl:117-159), per light:
+  //   d = |Lp - P|, L = (Lp - P)/d, NdotL = max(N.L, 0),
+  //   |V + L|^2 = 2 + 2 V.L (unit V, L), NdotH = max((N.V + N.L)/|V + L|, 0),
+  //   F = F0 + dF (1 - NdotH)^5, D = a2 / (pi t^2), t = NdotH^2 (a2 - 1) + 1,
+  //   spec = D F G / (4 NdotL NdotV + 1e-4), G = gv gl, gl = NdotL / (NdotL (1-k) + k),
+  //   diffuse = (1 - F) kd / pi, color += (diffuse + spec) E NdotL / d^2.
+  // The three denominators share one reciprocal.  Grey lights (E.x = E.y =
+  // E.z) accumulate four per-pixel scalars instead of three channels:
+  //   sum (F_c s + (1 - F_c) kdp_c) e w
+  //     = F0_c (A2 - kdp_c A1) + dF_c (A4 - kdp_c A3) + kdp_c A1
+  //   with ew = e w, S = s ew, A1 = sum ew, A2 = sum S, A3 = sum x5 ew, A4 = sum x5 S.
+  // A light below the horizon of every lane of the wave adds exactly zero and
+  // is skipped with a wave-uniform branch.
+  const float c1 = a2 * gv_of(n_dot_v, kdir) * kInvPi;
+  const float four_nv = 4.0f * n_dot_v;
+  const float om_k = 1.0f - kdir;
+  const float a2m1 = a2 - 1.0f;
+  const f3 kdp = kd * kInvPi;
+  float A1 = 0.0f, A2 = 0.0f, A3 = 0.0f, A4 = 0.0f;
+  for (int base_i = 0; base_i < p.light_count; base_i += kLightChunk) {
+    const int n = min(kLightChunk, p.light_count - base_i);
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const althea_point_light lt = p.lights[base_i + k];
+      const bool grey = lt.emission[0] == lt.emission[1] && lt.emission[1] == lt.emission[2];
+      s_light_pos[k] = make_float4(lt.position[0], lt.position[1], lt.position[2], grey ? 1.0f : 0.0f);
+      s_light_em[k] = make_float4(lt.emission[0], lt.emission[1], lt.emission[2], 0.0f);
+    }
+    __syncthreads();
+    if (!__any(shade)) continue;
+    for (int i = 0; i < n; ++i) {
+      const float4 lp = s_light_pos[i];
+      const f3 L = mk(lp.x - P.x, lp.y - P.y, lp.z - P.z);
+      const float nl_raw = dot(N, L);
+      if (!__any(shade && nl_raw > 0.0f)) continue;
+      const float d2 = dot(L, L);
+      const float inv_d = rsq_fast(d2);
+      const float vl = dot(I, L);
+      const float h2 = fmaf(2.0f * vl, inv_d, 2.0f);
+      const float n_dot_l = fmaxf(nl_raw * inv_d, 0.0f);
+      const float n_dot_h = fmaxf(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2), 0.0f);
+      const float x = 1.0f - n_dot_h;
+      const float x5 = pow5(x);
+      const float t = fmaf(n_dot_h * n_dot_h, a2m1, 1.0f);
+      const float den = (t * t) * fmaf(n_dot_l, om_k, kdir) * fmaf(n_dot_l, four_nv, 0.0001f);
+      const float s = c1 * n_dot_l * rcp_fast(den);
+      const float w = (inv_d * inv_d) * n_dot_l;
+      const float4 em = s_light_em[i];
+      if (lp.w != 0.0f) {
+        const float ew = em.x * w;
+        const float S = s * ew;
+        A1 += ew;
+        A2 += S;
+        A3 = fmaf(x5, ew, A3);
+        A4 = fmaf(x5, S, A4);
+      } else {
+        const f3 F = mk(fmaf(dF.x, x5, F0.x), fmaf(dF.y, x5, F0.y), fmaf(dF.z, x5, F0.z));
+        color.x = fmaf(fmaf(F.x, s, fmaf(-F.x, kdp.x, kdp.x)) * em.x, w, color.x);
+        color.y = fmaf(fmaf(F.y, s, fmaf(-F.y, kdp.y, kdp.y)) * em.y, w, color.y);
+        color.z = fmaf(fmaf(F.z, s, fmaf(-F.z, kdp.z, kdp.z)) * em.z, w, color.z);
+      }
+    }
   }
+  if (!shade) return;
+  color.x += fmaf(F0.x, fmaf(-kdp.x, A1, A2), fmaf(dF.x, fmaf(-kdp.x, A3, A4), kdp.x * A1));
+  color.y += fmaf(F0.y, fmaf(-kdp.y, A1, A2), fmaf(dF.y, fmaf(-kdp.y, A3, A4), kdp.y * A1));
+  color.z += fmaf(F0.z, fmaf(-kdp.z, A1, A2), fmaf(dF.z, fmaf(-kdp.z, A3, A4), kdp.z * A1));
   if (p.tonemap) color = tonemap(color, p.exposure);
   store_out<kOutF16>(p.out, idx, color);
 }
@@ -295,6 +369,7 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   p.ssao_on = (ssao && ssao->enabled) ? 1 : 0;
   if (p.ssao_on) {
     p.ssao_samples = ssao->samples;
+    p.ssao_inv_samples = 1.0f / static_cast<float>(ssao->samples);
     p.ssao_radius = ssao->radius;
     p.ssao_bias = ssao->bias;
     for (int k = 0; k < ssao->samples; ++k)
