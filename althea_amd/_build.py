@@ -83,11 +83,22 @@ def build_oracle() -> Path:
     return out
 
 
+def build_facade_demo() -> Path:
+    """C++ consumer of include/althea/ImageBasedLighting.hpp (run by the GPU tests)."""
+    out_dir = ROOT / "tests" / "cpp" / "build"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    out = out_dir / "facade_demo"
+    _run([_hipcc(), "-std=c++17", "-O2", "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "facade_demo.cpp"),
+          "-L", str(PKG), "-lalthea_amd", "-Wl,-rpath,$ORIGIN/../../../althea_amd", "-o", str(out)])
+    return out
+
+
 def build_all() -> None:
     with cf.ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_hip), ex.submit(build_synth), ex.submit(build_oracle)]
         for f in futs:
             f.result()
+    build_facade_demo()
 
 
 if __name__ == "__main__":

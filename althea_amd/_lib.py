@@ -59,6 +59,14 @@ class MlpDescC(C.Structure):
                 ("epsilon", C.c_float), ("init_output_bias", C.c_float)]
 
 
+class EnvViewC(C.Structure):
+    _fields_ = [("rgba", vp), ("width", i32), ("height", i32)]
+
+
+class TuningC(C.Structure):
+    _fields_ = [("tile_w", i32), ("tile_h", i32), ("waves_per_cu", i32), ("reserved", i32)]
+
+
 assert C.sizeof(GlobalsC) == 416, C.sizeof(GlobalsC)
 
 # name -> (restype, argtypes)
@@ -96,6 +104,15 @@ PROTOTYPES = {
     "althea_mlp_infer": (i32, [vp, vp, i32, vp, vp]),
     "althea_mlp_state_bytes": (sz, [vp]),
     "althea_mfma_bf16_probe": (i32, [vp, vp, vp, vp]),
+    "althea_prefilter_batch_create": (i32, [C.POINTER(EnvViewC), i32, C.POINTER(i32), i32, i32, C.c_uint,
+                                            C.POINTER(vp)]),
+    "althea_prefilter_batch_destroy": (i32, [vp]),
+    "althea_prefilter_batch_run": (i32, [vp, vp]),
+    "althea_prefilter_batch_autotune": (i32, [vp, C.POINTER(TuningC), vp]),
+    "althea_prefilter_batch_set_tuning": (i32, [vp, C.POINTER(TuningC)]),
+    "althea_prefilter_batch_get_tuning": (i32, [vp, C.POINTER(TuningC)]),
+    "althea_prefilter_batch_cube": (i32, [vp, i32, C.POINTER(vp), C.POINTER(i32), C.POINTER(i32)]),
+    "althea_prefilter_batch_texels": (sz, [vp]),
 }
 
 _lib = None

@@ -18,6 +18,7 @@
 
 #include "device_math.hpp"
 #include "internal.hpp"
+#include "prefilter_common.hpp"
 
 namespace althea {
 
@@ -95,12 +96,6 @@ althea_status launch_pyramid(const float* level0, int w, int h, float* pyr, hipS
 // [0, levels-1] split into (l0, l1 = min(l0+1, levels-1), delta).
 // counts[m] = kept entries, wsum[m] = FP64 sum of L.n, wsum32[m] = the same
 // sum accumulated in FP32 in sample order.
-struct GgxEntry {
-  float lx, ly, lz, delta;
-  int l0, l1, pad0, pad1;
-};
-static_assert(sizeof(GgxEntry) == 32, "one scalar load per entry");
-
 struct TableParams {
   int samples;
   int mips;
@@ -213,7 +208,6 @@ struct PrefilterParams {
 };
 
 constexpr int kPrefilterTile = 8;
-constexpr int kBlockSums = 32;  // FP32 partial sums flushed to FP64 every 32 samples
 
 template <bool kF64Sums>
 __global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restrict__ pyr,
@@ -235,42 +229,12 @@ __global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restri
   const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
 
   const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
-  const int count = counts[m];
-  double ar = 0.0, ag = 0.0, ab = 0.0;
-  float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
-  for (int e = 0; e < count; ++e) {
-    const GgxEntry en = tab[e];
-    f3 L = mk(fmaf(fr.t.x, en.lx, fmaf(fr.b.x, en.ly, fr.n.x * en.lz)),
-              fmaf(fr.t.y, en.lx, fmaf(fr.b.y, en.ly, fr.n.y * en.lz)),
-              fmaf(fr.t.z, en.lx, fmaf(fr.b.z, en.ly, fr.n.z * en.lz)));
-    float yaw, pitch;
-    equirect_angles(L, yaw, pitch);
-    f3 v = level_bilinear(pyr, p.lv[en.l0], yaw, pitch);
-    if (en.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, p.lv[en.l1], yaw, pitch), en.delta);
-    sr = fmaf(v.x, en.lz, sr);
-    sg = fmaf(v.y, en.lz, sg);
-    sb = fmaf(v.z, en.lz, sb);
-    if (kF64Sums) {
-      if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
-        ar += sr; ag += sg; ab += sb;
-        sr = sg = sb = 0.0f;
-      }
-    } else {
-      sw += en.lz;
-    }
-  }
-  float4 out;
-  if (kF64Sums) {
-    ar += sr; ag += sg; ab += sb;
-    const double inv = 1.0 / wsum[m];
-    out = make_float4(static_cast<float>(ar * inv), static_cast<float>(ag * inv), static_cast<float>(ab * inv), 1.0f);
-  } else {
-    out = make_float4(sr / sw, sg / sw, sb / sw, 1.0f);
-  }
+  const float4 out = prefilter_texel<kF64Sums>(
+      pyr, [&](int l) -> const LevelC& { return p.lv[l]; }, tab, counts[m], wsum[m], fr);
   if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
 }
 
-static void levels_of(int w, int h, LevelC* out) {
+void pyramid_levels(int w, int h, LevelC* out) {
   PyramidGeometry g = pyramid_geometry(w, h);
   for (int l = 0; l < g.levels; ++l) out[l] = make_level(g.w[l], g.h[l], g.off[l]);
 }
@@ -284,6 +248,17 @@ static CubeMips cube_mips_of(int size, int mips) {
 }
 
 static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, GgxEntry* tables, int* counts,
+                                double* wsum, float* wsum32, hipStream_t s) {
+  TableParams tp;
+  tp.samples = samples;
+  tp.mips = mips;
+  tp.env_levels = althea_env_levels(env_w, env_h);
+  tp.sa_texel = 4.0 * M_PI / (6.0 * static_cast<double>(env_w) * env_h);
+  hipLaunchKernelGGL(ggx_table_kernel, dim3(mips), dim3(kTableThreads), 0, s, tp, tables, counts, wsum, wsum32);
+  return check_launch("ggx_table_kernel");
+}
 
 size_t prefilter_workspace_bytes(int, int, int, int mips, int samples) {
   return align256(sizeof(GgxEntry) * static_cast<size_t>(mips) * samples) + align256(sizeof(int) * mips) +
@@ -301,17 +276,11 @@ althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int 
   ws += align256(sizeof(double) * mips);
   float* wsum32 = reinterpret_cast<float*>(ws);
 
-  TableParams tp;
-  tp.samples = samples;
-  tp.mips = mips;
-  tp.env_levels = althea_env_levels(env_w, env_h);
-  tp.sa_texel = 4.0 * M_PI / (6.0 * static_cast<double>(env_w) * env_h);
-  hipLaunchKernelGGL(ggx_table_kernel, dim3(mips), dim3(kTableThreads), 0, s, tp, tables, counts, wsum, wsum32);
-  althea_status st = check_launch("ggx_table_kernel");
+  althea_status st = launch_ggx_tables(samples, mips, env_w, env_h, tables, counts, wsum, wsum32, s);
   if (st != ALTHEA_OK) return st;
 
   PrefilterParams p{};
-  levels_of(env_w, env_h, p.lv);
+  pyramid_levels(env_w, env_h, p.lv);
   p.cube = cube_mips_of(size, mips);
   p.samples = samples;
   int tiles = 0;
@@ -417,7 +386,7 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
                                       int count, float* out, void*, hipStream_t s) {
   if (count == 0) return ALTHEA_OK;
   PointParams p{};
-  levels_of(env_w, env_h, p.lv);
+  pyramid_levels(env_w, env_h, p.lv);
   p.levels = althea_env_levels(env_w, env_h);
   p.samples = samples;
   p.count = count;
@@ -460,24 +429,28 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
   }
 }
 
-template <bool kF64Sums>
+// kT x kT texels per 256-lane block, 256 / kT^2 theta slices; lane = slice *
+// kT^2 + texel, so a wave holds 64 / kT^2 slices of every texel of the tile.
+template <bool kF64Sums, int kT>
 __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ pyr,
                                                               const float2* __restrict__ theta_tab,
                                                               const float4* __restrict__ phi_tab,
                                                               const IrradianceParams p, float4* __restrict__ cube) {
+  constexpr int kTexels = kT * kT;
+  constexpr int kSlices = 256 / kTexels;
   const int S = p.size;
-  const int tpf = (S + 3) / 4;
+  const int tpf = (S + kT - 1) / kT;
   const int face = blockIdx.x / (tpf * tpf);
   const int t = blockIdx.x - face * tpf * tpf;
-  const int lane_texel = threadIdx.x & 15;
-  const int slice = threadIdx.x >> 4;
-  const int x = (t % tpf) * 4 + (lane_texel & 3);
-  const int y = (t / tpf) * 4 + (lane_texel >> 2);
+  const int lane_texel = threadIdx.x % kTexels;
+  const int slice = threadIdx.x / kTexels;
+  const int x = (t % tpf) * kT + (lane_texel % kT);
+  const int y = (t / tpf) * kT + (lane_texel / kT);
   const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
 
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float fr32 = 0.0f, fg32 = 0.0f, fb32 = 0.0f;
-  for (int i = slice; i < p.theta_samples; i += 16) {
+  for (int i = slice; i < p.theta_samples; i += kSlices) {
     const float2 th = theta_tab[i];
     const f3 a = fr.t * th.x + fr.b * th.y;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f;
@@ -502,18 +475,23 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __rest
   if (!kF64Sums) {
     ar = fr32; ag = fg32; ab = fb32;
   }
-  // Reduce the four theta slices of this wave (lanes l, l^16, l^32, l^48).
-  ar += __shfl_xor(ar, 16, 64); ag += __shfl_xor(ag, 16, 64); ab += __shfl_xor(ab, 16, 64);
-  ar += __shfl_xor(ar, 32, 64); ag += __shfl_xor(ag, 32, 64); ab += __shfl_xor(ab, 32, 64);
-  __shared__ double s_red[4][16][3];
+  // Reduce the theta slices of this wave (lanes l, l ^ kTexels, ...) with
+  // shuffles, then the four waves in LDS, in a fixed order.
+#pragma unroll
+  for (int o = kTexels; o < 64; o <<= 1) {
+    ar += __shfl_xor(ar, o, 64);
+    ag += __shfl_xor(ag, o, 64);
+    ab += __shfl_xor(ab, o, 64);
+  }
+  __shared__ double s_red[4][kTexels][3];
   const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) < 16) {
+  if ((threadIdx.x & 63) < kTexels) {
     s_red[wave][lane_texel][0] = ar;
     s_red[wave][lane_texel][1] = ag;
     s_red[wave][lane_texel][2] = ab;
   }
   __syncthreads();
-  if (threadIdx.x < 16 && x < S && y < S) {
+  if (threadIdx.x < kTexels && x < S && y < S) {
     double r = 0.0, g = 0.0, bl = 0.0;
     for (int w = 0; w < 4; ++w) {
       r += s_red[w][lane_texel][0];
@@ -566,14 +544,19 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.lv0 = make_level(g.w[p.l0], g.h[p.l0], g.off[p.l0]);
   p.lv1 = make_level(g.w[p.l1], g.h[p.l1], g.off[p.l1]);
   p.scale = M_PI / theta_samples / phi;
-  int tpf = (size + 3) / 4;
+  // 2x2-texel tiles (64 theta slices per texel) keep a 32^2 cube at 1536
+  // blocks, enough waves to hide the gather latency.
+  constexpr int kT = 2;
+  int tpf = (size + kT - 1) / kT;
   dim3 grid(6 * tpf * tpf);
   const char* pyr4 = reinterpret_cast<const char*>(pyr);
   float4* cube4 = reinterpret_cast<float4*>(cube);
   if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
-    hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+    hipLaunchKernelGGL((irradiance_cube_kernel<false, kT>), grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p,
+                       cube4);
   else
-    hipLaunchKernelGGL(irradiance_cube_kernel<true>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+    hipLaunchKernelGGL((irradiance_cube_kernel<true, kT>), grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p,
+                       cube4);
   return check_launch("irradiance_cube_kernel");
 }
 
@@ -675,7 +658,7 @@ althea_status launch_env_sample(const float* pyr, int w, int h, const float* dir
                                 hipStream_t s) {
   if (count == 0) return ALTHEA_OK;
   EnvSampleParams p{};
-  levels_of(w, h, p.lv);
+  pyramid_levels(w, h, p.lv);
   p.levels = althea_env_levels(w, h);
   p.count = count;
   hipLaunchKernelGGL(env_sample_kernel, dim3((count + 255) / 256), dim3(256), 0, s,

@@ -47,7 +47,13 @@ PyramidGeometry pyramid_geometry(int width, int height);
 // Cube mip offsets (texels).
 size_t cube_mip_offset(int size, int mip);
 
+struct GgxEntry;
+struct LevelC;
+
 // Kernel launchers (ibl_kernels.hip).
+althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, GgxEntry* tables, int* counts,
+                                double* wsum, float* wsum32, hipStream_t s);
+void pyramid_levels(int w, int h, LevelC* out);
 althea_status launch_hammersley(uint32_t n, uint32_t first, uint32_t count, uint32_t* bits, float* xi,
                                 hipStream_t s);
 althea_status launch_pyramid(const float* level0, int w, int h, float* pyr, hipStream_t s);

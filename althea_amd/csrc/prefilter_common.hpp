@@ -1,0 +1,63 @@
+// Per-texel GGX prefilter shared by the single-map and the batched
+// (persistent) kernels, so both produce bit-identical texels.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace althea {
+
+// Sample-table entry (one per Hammersley sample with NdotL > 0, in sample
+// order): L in the texel frame (T, B, N) and the source LOD split into
+// (l0, l1, delta).  32 bytes, one scalar load.
+struct GgxEntry {
+  float lx, ly, lz, delta;
+  int l0, l1, pad0, pad1;
+};
+static_assert(sizeof(GgxEntry) == 32, "one scalar load per entry");
+
+constexpr int kBlockSums = 32;  // FP32 partial sums flushed to FP64 every 32 samples
+
+// NdotL-weighted mean of the source over the table (PreFilterEnvMap.comp:147-169):
+// the FP32 per-sample body, FP64 running sums (kF64Sums) or all-FP32 sums.
+// `level(l)` returns the LevelC of source level l.
+template <bool kF64Sums, class LevelOf>
+__device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, const LevelOf& level,
+                                                  const GgxEntry* __restrict__ tab, int count, double wsum,
+                                                  const Frame& fr) {
+  double ar = 0.0, ag = 0.0, ab = 0.0;
+  float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+  for (int e = 0; e < count; ++e) {
+    const GgxEntry en = tab[e];
+    const f3 L = mk(fmaf(fr.t.x, en.lx, fmaf(fr.b.x, en.ly, fr.n.x * en.lz)),
+                    fmaf(fr.t.y, en.lx, fmaf(fr.b.y, en.ly, fr.n.y * en.lz)),
+                    fmaf(fr.t.z, en.lx, fmaf(fr.b.z, en.ly, fr.n.z * en.lz)));
+    float yaw, pitch;
+    equirect_angles(L, yaw, pitch);
+    f3 v = level_bilinear(pyr, level(en.l0), yaw, pitch);
+    if (en.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, level(en.l1), yaw, pitch), en.delta);
+    sr = fmaf(v.x, en.lz, sr);
+    sg = fmaf(v.y, en.lz, sg);
+    sb = fmaf(v.z, en.lz, sb);
+    if (kF64Sums) {
+      if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
+        ar += sr;
+        ag += sg;
+        ab += sb;
+        sr = sg = sb = 0.0f;
+      }
+    } else {
+      sw += en.lz;
+    }
+  }
+  if (kF64Sums) {
+    ar += sr;
+    ag += sg;
+    ab += sb;
+    const double inv = 1.0 / wsum;
+    return make_float4(static_cast<float>(ar * inv), static_cast<float>(ag * inv), static_cast<float>(ab * inv),
+                       1.0f);
+  }
+  return make_float4(sr / sw, sg / sw, sb / sw, 1.0f);
+}
+
+}  // namespace althea

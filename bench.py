@@ -82,7 +82,28 @@ def secondary(torch, aa) -> dict:
     ms = time_events(torch, lambda: aa.prefilter_cube(pyr1, 2048, 1024, 512, 6, 1024), 2)
     res["prefilter_512x6x6_1024spp"] = {"ms": ms, "texels_per_s": texels1 / (ms * 1e-3),
                                         "samples_per_s": texels1 * 1024 / (ms * 1e-3)}
+    res["batched_prefilter_64maps"] = batched(torch, aa)
     return res
+
+
+def batched(torch, aa) -> dict:
+    """64 maps, widths uniform over {256..4096} (seeded by index), faces W/4,
+    5 roughness mips, 1024 samples: one graph launch per run (pyramids, sample
+    tables, persistent prefilter), launch shape from the autotuner."""
+    from althea_amd.batch import PrefilterBatch, batch_widths
+
+    widths = batch_widths(64)
+    maps = [torch.from_numpy(aa.synth.env(seed=i, width=w, height=w // 2)).cuda() for i, w in enumerate(widths)]
+    b = PrefilterBatch(maps, [w // 4 for w in widths], mips=5, samples=1024)
+    t0 = time.perf_counter()
+    tuning = b.autotune()
+    tune_s = time.perf_counter() - t0
+    b.run()  # builds and instantiates the graph
+    ms = time_events(torch, b.run, 2)
+    texels = b.texels()
+    return {"maps": 64, "widths": {str(w): widths.count(w) for w in sorted(set(widths))}, "texels": texels,
+            "ms_per_run": ms, "texels_per_s": texels / (ms * 1e-3), "samples_per_s": texels * 1024 / (ms * 1e-3),
+            "tuning": tuning, "autotune_s": tune_s}
 
 
 def cpu_baseline(aa, scene, ibl, env) -> dict:

@@ -1,0 +1,120 @@
+// C++ host facade over the C ABI, shaped like nithinp7/Althea's
+// Include/Althea/ImageBasedLighting.h and Include/Althea/DeferredRendering.h:
+// the same type and function names, device buffers in place of VkImages,
+// std::runtime_error on failure as the reference throws
+// (Src/ImageBasedLighting.cpp:424).  Header-only; link libalthea_amd.so.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "althea/althea.h"
+
+namespace AltheaEngine {
+
+inline void throwIfFailed(althea_status status, const char* where) {
+  if (status != ALTHEA_OK)
+    throw std::runtime_error(std::string(where) + ": " + althea_status_string(status) + ": " + althea_last_error());
+}
+
+// IBLHandles (ImageBasedLighting.h:17-22): device pointers in the same order
+// (environment, prefiltered, irradiance, BRDF LUT) instead of bindless indices.
+struct IBLHandles {
+  const float* environmentMap;
+  const float* prefilteredMap;
+  const float* irradianceMap;
+  const float* brdfLut;
+};
+
+// IBLResources (ImageBasedLighting.h:24-43): move-only owner of the device
+// images of one environment.
+class IBLResources {
+ public:
+  IBLResources() = default;
+  explicit IBLResources(althea_ibl* ibl) : _ibl(ibl) {}
+
+  IBLHandles getHandles() const {
+    althea_ibl_views v = views();
+    return {v.environment, v.prefiltered, v.irradiance, v.brdf_lut};
+  }
+  althea_ibl_views views() const {
+    althea_ibl_views v{};
+    throwIfFailed(althea_ibl_get_views(_ibl.get(), &v), "IBLResources::views");
+    return v;
+  }
+  // Device bytes the shading pass reads (prefiltered + irradiance + LUT).
+  size_t stateBytes() const { return althea_ibl_state_bytes(_ibl.get()); }
+  althea_ibl* get() const { return _ibl.get(); }
+  explicit operator bool() const { return static_cast<bool>(_ibl); }
+
+ private:
+  struct Deleter {
+    void operator()(althea_ibl* p) const { althea_ibl_destroy(p); }
+  };
+  std::unique_ptr<althea_ibl, Deleter> _ibl;
+};
+
+namespace ImageBasedLighting {
+
+inline althea_ibl_desc defaultDesc() {
+  althea_ibl_desc d{};
+  althea_ibl_desc_default(&d);
+  return d;
+}
+
+// ImageBasedLighting::createResources (ImageBasedLighting.h:46-49): the
+// environment arrives as a device equirect map instead of a file name; the
+// irradiance cube, prefiltered mip chain and BRDF LUT are computed on
+// `stream` (the reference precomputes equirect images, :137-412).
+inline IBLResources createResources(const float* d_equirect, int width, int height,
+                                    const althea_ibl_desc& desc = defaultDesc(), hipStream_t stream = nullptr) {
+  althea_ibl* ibl = nullptr;
+  throwIfFailed(althea_ibl_create(&desc, &ibl), "ImageBasedLighting::createResources");
+  IBLResources res(ibl);
+  throwIfFailed(althea_ibl_precompute(ibl, d_equirect, width, height, stream), "ImageBasedLighting::createResources");
+  return res;
+}
+
+}  // namespace ImageBasedLighting
+
+// GBufferHandles: the four planes in GBufferResources::buildMaterial order
+// (Src/DeferredRendering.cpp:23-35).
+struct GBufferHandles {
+  const void* position;
+  const void* normal;
+  const void* albedo;
+  const void* metallicRoughnessOcclusion;
+};
+
+// GBufferResources (DeferredRendering.h:38-102) over caller-owned device
+// planes of vec4 per pixel (FP32 or FP16).
+class GBufferResources {
+ public:
+  GBufferResources() = default;
+  GBufferResources(const void* position, const void* normal, const void* albedo, const void* mro, int width,
+                   int height, int format = ALTHEA_GBUFFER_FP32)
+      : _view{position, normal, albedo, mro, width, height, format} {}
+
+  GBufferHandles getHandles() const { return {_view.position, _view.normal, _view.albedo, _view.mro}; }
+  const althea_gbuffer_view& view() const { return _view; }
+
+ private:
+  althea_gbuffer_view _view{};
+};
+
+namespace DeferredRendering {
+
+// The deferred lighting resolve of DeferredPass.frag (one frame).
+inline void resolve(const GBufferResources& gbuffer, const IBLResources& ibl, const althea_point_light* d_lights,
+                    int lightCount, const althea_globals& globals, const althea_ssao_desc* ssao, void* d_out,
+                    unsigned flags = 0, float* d_ao = nullptr, hipStream_t stream = nullptr) {
+  althea_ibl_views v = ibl.views();
+  throwIfFailed(althea_deferred_resolve(&gbuffer.view(), &v, d_lights, lightCount, &globals, ssao, flags, d_out, d_ao,
+                                        stream),
+                "DeferredRendering::resolve");
+}
+
+}  // namespace DeferredRendering
+}  // namespace AltheaEngine

@@ -142,6 +142,51 @@ ALTHEA_API althea_status althea_irradiance_cube(const float* d_pyramid, int env_
 ALTHEA_API althea_status althea_brdf_lut(int size, int samples, float* d_rg, hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched prefilter (no counterpart in the reference, which prefilters one  */
+/* map with five dispatches, Src/ImageBasedLighting.cpp:352-402)             */
+/* ------------------------------------------------------------------------ */
+
+/* An equirect source map in device memory (RGBA32F, power-of-two extents). */
+typedef struct althea_env_view {
+  const float* rgba;
+  int width, height;
+} althea_env_view;
+
+/* Launch shape of the persistent prefilter: a wave shades a tile_w x tile_h
+ * texel tile (8x8, 16x4 or 4x16); the grid holds waves_per_cu waves per CU. */
+typedef struct althea_tuning {
+  int tile_w, tile_h;
+  int waves_per_cu;
+  int reserved;
+} althea_tuning;
+
+typedef struct althea_prefilter_batch althea_prefilter_batch;
+
+/* Prefilters `count` maps (any mix of extents) into cubes of face_sizes[i]
+ * with `mips` roughness levels m/(mips-1) and `samples` Hammersley samples,
+ * the same estimator as althea_prefilter_cube.  The batch owns the pyramids
+ * and output cubes; the sources must stay valid while the batch is used. */
+ALTHEA_API althea_status althea_prefilter_batch_create(const althea_env_view* envs, int count, const int* face_sizes,
+                                                       int mips, int samples, unsigned flags,
+                                                       althea_prefilter_batch** out);
+ALTHEA_API althea_status althea_prefilter_batch_destroy(althea_prefilter_batch* batch);
+/* One run: a single hipGraph launch holding every map's pyramid chain, the
+ * sample tables of each distinct source extent, a counter reset and one
+ * persistent prefilter launch over all work tiles (largest first). */
+ALTHEA_API althea_status althea_prefilter_batch_run(althea_prefilter_batch* batch, hipStream_t stream);
+/* Times every candidate launch shape on a fixed subset of the work list and
+ * keeps the fastest (synchronises `stream`). */
+ALTHEA_API althea_status althea_prefilter_batch_autotune(althea_prefilter_batch* batch, althea_tuning* chosen,
+                                                         hipStream_t stream);
+ALTHEA_API althea_status althea_prefilter_batch_set_tuning(althea_prefilter_batch* batch, const althea_tuning* tuning);
+ALTHEA_API althea_status althea_prefilter_batch_get_tuning(const althea_prefilter_batch* batch, althea_tuning* tuning);
+/* Device cube of map `index` (layout of althea_prefilter_cube). */
+ALTHEA_API althea_status althea_prefilter_batch_cube(const althea_prefilter_batch* batch, int index,
+                                                     const float** d_cube, int* size, int* mips);
+/* Output texels of all maps together. */
+ALTHEA_API size_t althea_prefilter_batch_texels(const althea_prefilter_batch* batch);
+
+/* ------------------------------------------------------------------------ */
 /* IBL resources (ImageBasedLighting::createResources / IBLResources)        */
 /* ------------------------------------------------------------------------ */
 

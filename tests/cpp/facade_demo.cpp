@@ -1,0 +1,114 @@
+// End-to-end use of the C++ facade (include/althea/ImageBasedLighting.hpp),
+// written as a consumer of ImageBasedLighting::createResources and
+// DeferredRendering would: constant environment -> closed-form checks on the
+// prefiltered and irradiance cubes, then one resolve of a small G-buffer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "althea/ImageBasedLighting.hpp"
+
+using namespace AltheaEngine;
+
+#define HIP_OK(x)                                                     \
+  do {                                                                \
+    if ((x) != hipSuccess) {                                          \
+      std::fprintf(stderr, "HIP failure %s at %d\n", #x, __LINE__); \
+      return 1;                                                       \
+    }                                                                 \
+  } while (0)
+
+int main() {
+  const int W = 64, H = 32;
+  std::vector<float> env(4 * W * H);
+  for (int i = 0; i < W * H; ++i) {
+    env[4 * i] = 2.0f;
+    env[4 * i + 1] = 3.0f;
+    env[4 * i + 2] = 5.0f;
+    env[4 * i + 3] = 1.0f;
+  }
+  float* d_env = nullptr;
+  HIP_OK(hipMalloc(&d_env, env.size() * 4));
+  HIP_OK(hipMemcpy(d_env, env.data(), env.size() * 4, hipMemcpyHostToDevice));
+  althea_ibl_desc desc = ImageBasedLighting::defaultDesc();
+  desc.irradiance_size = 4;
+  desc.prefiltered_size = 16;
+  desc.prefiltered_mips = 5;
+  desc.prefilter_samples = 64;
+  desc.brdf_lut_size = 16;
+  desc.brdf_lut_samples = 64;
+  try {
+    IBLResources ibl = ImageBasedLighting::createResources(d_env, W, H, desc);
+    HIP_OK(hipDeviceSynchronize());
+    IBLHandles h = ibl.getHandles();
+    althea_ibl_views v = ibl.views();
+    std::vector<float> pref(4 * althea_cube_texels(v.prefiltered_size, v.prefiltered_mips));
+    std::vector<float> irr(4 * 6 * v.irradiance_size * v.irradiance_size);
+    HIP_OK(hipMemcpy(pref.data(), h.prefilteredMap, pref.size() * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(irr.data(), h.irradianceMap, irr.size() * 4, hipMemcpyDeviceToHost));
+    double max_pref = 0.0, max_irr = 0.0;
+    // Constant radiance: prefilter is the constant; irradiance is the
+    // left Riemann sum (pi / 2P) cot(pi / 2P) of the constant, P = 150.
+    const double P = 150.0, k = (M_PI / (2 * P)) / std::tan(M_PI / (2 * P));
+    for (size_t t = 0; t < pref.size() / 4; ++t)
+      for (int c = 0; c < 3; ++c)
+        max_pref = std::fmax(max_pref, std::fabs(pref[4 * t + c] - env[c]) / env[c]);
+    for (size_t t = 0; t < irr.size() / 4; ++t)
+      for (int c = 0; c < 3; ++c)
+        max_irr = std::fmax(max_irr, std::fabs(irr[4 * t + c] - env[c] * k) / (env[c] * k));
+    // A 4x4 G-buffer facing the camera, one light.
+    const int GW = 4, GH = 4;
+    std::vector<float> pos(4 * GW * GH), nrm(4 * GW * GH), alb(4 * GW * GH), mro(4 * GW * GH);
+    for (int i = 0; i < GW * GH; ++i) {
+      pos[4 * i + 2] = -3.0f;
+      pos[4 * i + 3] = 1.0f;
+      nrm[4 * i + 2] = 1.0f;
+      alb[4 * i] = alb[4 * i + 1] = alb[4 * i + 2] = 0.5f;
+      mro[4 * i + 1] = 0.5f;
+      mro[4 * i + 2] = 1.0f;
+    }
+    float *d_pos, *d_nrm, *d_alb, *d_mro, *d_out;
+    althea_point_light light{{0.0f, 0.0f, 1.0f}, 0.0f, {10.0f, 10.0f, 10.0f}, 0.0f};
+    althea_point_light* d_light;
+    HIP_OK(hipMalloc(&d_pos, pos.size() * 4));
+    HIP_OK(hipMalloc(&d_nrm, pos.size() * 4));
+    HIP_OK(hipMalloc(&d_alb, pos.size() * 4));
+    HIP_OK(hipMalloc(&d_mro, pos.size() * 4));
+    HIP_OK(hipMalloc(&d_out, pos.size() * 4));
+    HIP_OK(hipMalloc(&d_light, sizeof(light)));
+    HIP_OK(hipMemcpy(d_pos, pos.data(), pos.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_nrm, nrm.data(), pos.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_alb, alb.data(), pos.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_mro, mro.data(), pos.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_light, &light, sizeof(light), hipMemcpyHostToDevice));
+    althea_globals g{};
+    for (int i = 0; i < 16; i += 5) g.projection[i] = g.inverse_projection[i] = g.view[i] = g.inverse_view[i] = 1.0f;
+    g.inverse_projection[10] = 0.0f;
+    g.inverse_projection[14] = -1.0f;  // view ray (ndc.x, ndc.y, -1)
+    g.exposure = 1.0f;
+    GBufferResources gbuf(d_pos, d_nrm, d_alb, d_mro, GW, GH);
+    DeferredRendering::resolve(gbuf, ibl, d_light, 1, g, nullptr, d_out, ALTHEA_RESOLVE_TONEMAP);
+    std::vector<float> out(pos.size());
+    HIP_OK(hipMemcpy(out.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost));
+    bool finite = true;
+    for (float f : out) finite = finite && std::isfinite(f) && f >= 0.0f && f <= 1.0f;
+    std::printf("prefiltered max rel %.3g irradiance max rel %.3g resolve finite %d state bytes %zu\n", max_pref,
+                max_irr, finite ? 1 : 0, ibl.stateBytes());
+    // Errors come back as exceptions, as the reference throws.
+    bool threw = false;
+    try {
+      ImageBasedLighting::createResources(d_env, 60, 32, desc);
+    } catch (const std::runtime_error& e) {
+      threw = true;
+      std::printf("rejected non-power-of-two source: %s\n", e.what());
+    }
+    const bool ok = max_pref < 1e-5 && max_irr < 1e-5 && finite && threw;
+    std::printf("facade %s\n", ok ? "ok" : "FAILED");
+    return ok ? 0 : 2;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "exception: %s\n", e.what());
+    return 3;
+  }
+}
