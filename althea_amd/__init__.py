@@ -298,3 +298,28 @@ class DeferredRendering:
                                             flags, _ptr(out), _ptr(ao_out), _stream(stream)),
               "althea_deferred_resolve")
         return out
+
+    @staticmethod
+    def resolve_mlp(gbuffer, ibl, mlp, lights, globals_, ssao=None, tonemap: bool = False, out_fp16: bool = False,
+                    ao_out=None, out=None, mlp_used=None, stream=None):
+        """The resolve with the prefiltered term from `mlp` (a PrefilterMLP
+        whose error monitor was built for this IBL's cube) where the monitor
+        accepts, the cube elsewhere.  `mlp_used` (optional uint8 device tensor
+        of width*height) receives 1 where the MLP supplied the term."""
+        torch = _torch()
+        if not isinstance(gbuffer, GBufferResources):
+            gbuffer = GBufferResources(*gbuffer)
+        views = ibl.views() if isinstance(ibl, IBLResources) else ibl
+        gv = gbuffer.getHandles()
+        n = int(lights.shape[0]) if lights is not None else 0
+        if out is None:
+            dt = torch.float16 if out_fp16 else torch.float32
+            out = torch.empty((gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
+        if mlp_used is not None:
+            assert mlp_used.dtype == torch.uint8 and mlp_used.numel() >= gbuffer.height * gbuffer.width
+        flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)
+        check(lib().althea_deferred_resolve_mlp(C.byref(gv), C.byref(views), mlp.handle, _ptr(lights) if n else None,
+                                                n, C.byref(globals_), C.byref(ssao) if ssao is not None else None,
+                                                flags, _ptr(out), _ptr(ao_out), _ptr(mlp_used), _stream(stream)),
+              "althea_deferred_resolve_mlp")
+        return out

@@ -70,6 +70,9 @@ class TuningC(C.Structure):
 assert C.sizeof(GlobalsC) == 416, C.sizeof(GlobalsC)
 
 # name -> (restype, argtypes)
+MONITOR_CELLS = 64      # ALTHEA_MLP_MONITOR_CELLS
+MONITOR_FLOOR = 1e-2    # ALTHEA_MLP_MONITOR_FLOOR
+
 PROTOTYPES = {
     "althea_status_string": (C.c_char_p, [i32]),
     "althea_last_error": (C.c_char_p, []),
@@ -93,6 +96,9 @@ PROTOTYPES = {
     "althea_ibl_state_bytes": (sz, [vp]),
     "althea_deferred_resolve": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
                                       C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),
+    "althea_hdr_info": (i32, [C.c_char_p, C.POINTER(i32), C.POINTER(i32)]),
+    "althea_hdr_read": (i32, [C.c_char_p, vp, sz]),
+    "althea_hdr_write": (i32, [C.c_char_p, vp, i32, i32]),
     "althea_mlp_desc_default": (None, [C.POINTER(MlpDescC)]),
     "althea_mlp_param_count": (i32, []),
     "althea_mlp_create": (i32, [C.POINTER(MlpDescC), C.POINTER(vp)]),
@@ -103,6 +109,10 @@ PROTOTYPES = {
     "althea_mlp_gradients": (i32, [vp, vp, vp, i32, vp, vp, vp]),
     "althea_mlp_infer": (i32, [vp, vp, i32, vp, vp]),
     "althea_mlp_state_bytes": (sz, [vp]),
+    "althea_mlp_set_learning_rate": (i32, [vp, C.c_float]),
+    "althea_mlp_build_monitor": (i32, [vp, vp, i32, i32, C.c_float, vp, C.POINTER(i32), vp]),
+    "althea_deferred_resolve_mlp": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, vp, i32,
+                                          C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp, vp]),
     "althea_mfma_bf16_probe": (i32, [vp, vp, vp, vp]),
     "althea_prefilter_batch_create": (i32, [C.POINTER(EnvViewC), i32, C.POINTER(i32), i32, i32, C.c_uint,
                                             C.POINTER(vp)]),

@@ -11,6 +11,7 @@
 
 #include "althea/althea.h"
 #include "internal.hpp"
+#include "mlp_view.hpp"
 
 namespace althea {
 
@@ -80,6 +81,10 @@ class Workspace {
   hipError_t e_ = hipSuccess;
 };
 
+static bool cube_ok(int size, int mips) {
+  return is_pow2(size) && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1;
+}
+
 static bool env_dims_ok(int w, int h) {
   return is_pow2(w) && is_pow2(h) && w <= (1 << 15) && h <= (1 << 15);
 }
@@ -100,6 +105,29 @@ struct althea_ibl {
   void* workspace = nullptr;
   size_t workspace_bytes = 0;
 };
+
+static althea_status resolve_args_ok(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                     const althea_point_light* d_lights, int light_count,
+                                     const althea_globals* globals, const althea_ssao_desc* ssao, void* d_out) {
+  ALTHEA_REQUIRE(g && ibl && globals && d_out, "deferred_resolve: null argument");
+  ALTHEA_REQUIRE(g->position && g->normal && g->albedo && g->mro, "deferred_resolve: null G-buffer plane");
+  ALTHEA_REQUIRE(g->width >= 1 && g->height >= 1 && g->width <= 32768 && g->height <= 32768,
+                 "deferred_resolve: extent %dx%d", g->width, g->height);
+  ALTHEA_REQUIRE(g->format == ALTHEA_GBUFFER_FP32 || g->format == ALTHEA_GBUFFER_FP16,
+                 "deferred_resolve: G-buffer format %d", g->format);
+  ALTHEA_REQUIRE(ibl->environment && ibl->prefiltered && ibl->irradiance && ibl->brdf_lut,
+                 "deferred_resolve: incomplete IBL views");
+  ALTHEA_REQUIRE(env_dims_ok(ibl->env_width, ibl->env_height), "deferred_resolve: environment %dx%d",
+                 ibl->env_width, ibl->env_height);
+  ALTHEA_REQUIRE(cube_ok(ibl->prefiltered_size, ibl->prefiltered_mips), "deferred_resolve: prefiltered %d / %d",
+                 ibl->prefiltered_size, ibl->prefiltered_mips);
+  ALTHEA_REQUIRE(ibl->irradiance_size >= 1 && ibl->brdf_lut_size >= 1, "deferred_resolve: IBL sizes");
+  ALTHEA_REQUIRE(light_count >= 0 && (light_count == 0 || d_lights), "deferred_resolve: lights");
+  if (ssao && ssao->enabled)
+    ALTHEA_REQUIRE(ssao->samples >= 1 && ssao->samples <= ALTHEA_SSAO_MAX_SAMPLES && ssao->radius > 0.0f,
+                   "deferred_resolve: ssao samples %d radius %g", ssao->samples, ssao->radius);
+  return ALTHEA_OK;
+}
 
 extern "C" {
 
@@ -147,10 +175,6 @@ althea_status althea_env_pyramid_build(const float* d_level0, int width, int hei
 size_t althea_cube_texels(int size, int mips) {
   if (size <= 0 || mips <= 0) return 0;
   return cube_mip_offset(size, mips);
-}
-
-static bool cube_ok(int size, int mips) {
-  return is_pow2(size) && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1;
 }
 
 althea_status althea_cube_sample(const float* d_cube, int size, int mips, const float* d_dir_lod, int count,
@@ -203,7 +227,7 @@ althea_status althea_irradiance_cube(const float* d_pyramid, int env_width, int 
   ALTHEA_REQUIRE(size >= 1 && size <= 4096, "irradiance_cube: size %d out of range", size);
   ALTHEA_REQUIRE(theta_samples >= 1 && theta_samples <= 100000, "irradiance_cube: theta_samples %d out of range",
                  theta_samples);
-  Workspace ws(irradiance_workspace_bytes(env_width, env_height, theta_samples), stream);
+  Workspace ws(irradiance_workspace_bytes(env_width, env_height, size, theta_samples), stream);
   if (!ws.ok()) return ALTHEA_ERROR_OUT_OF_MEMORY;
   return launch_irradiance(d_pyramid, env_width, env_height, size, theta_samples, flags, d_cube, ws.get(), stream);
 }
@@ -287,7 +311,7 @@ althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, in
   ibl->env_h = height;
   size_t ws_bytes = std::max(prefilter_workspace_bytes(width, height, d.prefiltered_size, d.prefiltered_mips,
                                                        d.prefilter_samples),
-                             irradiance_workspace_bytes(width, height, d.irradiance_theta_samples));
+                             irradiance_workspace_bytes(width, height, d.irradiance_size, d.irradiance_theta_samples));
   if (ws_bytes > ibl->workspace_bytes) {
     ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
     (void)hipFree(ibl->workspace);
@@ -336,24 +360,29 @@ althea_status althea_deferred_resolve(const althea_gbuffer_view* g, const althea
                                       const althea_point_light* d_lights, int light_count,
                                       const althea_globals* globals, const althea_ssao_desc* ssao, unsigned flags,
                                       void* d_out, float* d_ao_out, hipStream_t stream) {
-  ALTHEA_REQUIRE(g && ibl && globals && d_out, "deferred_resolve: null argument");
-  ALTHEA_REQUIRE(g->position && g->normal && g->albedo && g->mro, "deferred_resolve: null G-buffer plane");
-  ALTHEA_REQUIRE(g->width >= 1 && g->height >= 1 && g->width <= 32768 && g->height <= 32768,
-                 "deferred_resolve: extent %dx%d", g->width, g->height);
-  ALTHEA_REQUIRE(g->format == ALTHEA_GBUFFER_FP32 || g->format == ALTHEA_GBUFFER_FP16,
-                 "deferred_resolve: G-buffer format %d", g->format);
-  ALTHEA_REQUIRE(ibl->environment && ibl->prefiltered && ibl->irradiance && ibl->brdf_lut,
-                 "deferred_resolve: incomplete IBL views");
-  ALTHEA_REQUIRE(env_dims_ok(ibl->env_width, ibl->env_height), "deferred_resolve: environment %dx%d",
-                 ibl->env_width, ibl->env_height);
-  ALTHEA_REQUIRE(cube_ok(ibl->prefiltered_size, ibl->prefiltered_mips), "deferred_resolve: prefiltered %d / %d",
+  althea_status st = resolve_args_ok(g, ibl, d_lights, light_count, globals, ssao, d_out);
+  if (st != ALTHEA_OK) return st;
+  return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, nullptr, nullptr,
+                        stream);
+}
+
+althea_status althea_deferred_resolve_mlp(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                          const althea_mlp* mlp, const althea_point_light* d_lights,
+                                          int light_count, const althea_globals* globals,
+                                          const althea_ssao_desc* ssao, unsigned flags, void* d_out,
+                                          float* d_ao_out, uint8_t* d_mlp_used, hipStream_t stream) {
+  althea_status st = resolve_args_ok(g, ibl, d_lights, light_count, globals, ssao, d_out);
+  if (st != ALTHEA_OK) return st;
+  ALTHEA_REQUIRE(mlp, "deferred_resolve_mlp: null MLP");
+  MlpInferenceView v{};
+  st = mlp_inference_view(mlp, &v);
+  if (st != ALTHEA_OK) return st;
+  ALTHEA_REQUIRE(v.mon_mips > 0 && v.cell_ok, "deferred_resolve_mlp: the MLP's error monitor is not built");
+  ALTHEA_REQUIRE(v.mon_size == ibl->prefiltered_size && v.mon_mips == ibl->prefiltered_mips,
+                 "deferred_resolve_mlp: monitor built for a %d / %d cube, IBL has %d / %d", v.mon_size, v.mon_mips,
                  ibl->prefiltered_size, ibl->prefiltered_mips);
-  ALTHEA_REQUIRE(ibl->irradiance_size >= 1 && ibl->brdf_lut_size >= 1, "deferred_resolve: IBL sizes");
-  ALTHEA_REQUIRE(light_count >= 0 && (light_count == 0 || d_lights), "deferred_resolve: lights");
-  if (ssao && ssao->enabled)
-    ALTHEA_REQUIRE(ssao->samples >= 1 && ssao->samples <= ALTHEA_SSAO_MAX_SAMPLES && ssao->radius > 0.0f,
-                   "deferred_resolve: ssao samples %d radius %g", ssao->samples, ssao->radius);
-  return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, stream);
+  return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, &v, d_mlp_used,
+                        stream);
 }
 
 }  // extern "C"

@@ -411,7 +411,8 @@ struct IrradianceParams {
   int phi_samples;
   int l0, l1;
   float delta;
-  double scale;  // pi / theta_samples / phi_samples
+  int rows_per_chunk;  // theta rows per grid chunk
+  double scale;        // pi / theta_samples / phi_samples
 };
 
 __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float2* __restrict__ theta_tab,
@@ -429,28 +430,35 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
   }
 }
 
-// kT x kT texels per 256-lane block, 256 / kT^2 theta slices; lane = slice *
-// kT^2 + texel, so a wave holds 64 / kT^2 slices of every texel of the tile.
-template <bool kF64Sums, int kT>
-__global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ pyr,
-                                                              const float2* __restrict__ theta_tab,
-                                                              const float4* __restrict__ phi_tab,
-                                                              const IrradianceParams p, float4* __restrict__ cube) {
-  constexpr int kTexels = kT * kT;
-  constexpr int kSlices = 256 / kTexels;
+// One wave per 8 x 8 texel tile and theta row: every lane of a wave looks up
+// the same (theta, phi) in its own texel frame, so the 64 lookups of a sample
+// land on neighbouring source texels.  Grid (tiles, chunks): chunk c owns a
+// contiguous run of theta rows, the block's 4 waves take every 4th row of it.
+// Per row the phi sum is FP32; rows are summed in FP64 (kF64Sums) or FP32;
+// the block writes one partial per texel and chunk, which
+// irradiance_reduce_kernel sums in chunk order.
+constexpr int kIrrTile = 8;
+
+template <bool kF64Sums>
+__global__ __launch_bounds__(256) void irradiance_partial_kernel(const char* __restrict__ pyr,
+                                                                 const float2* __restrict__ theta_tab,
+                                                                 const float4* __restrict__ phi_tab,
+                                                                 const IrradianceParams p,
+                                                                 double* __restrict__ partial) {
   const int S = p.size;
-  const int tpf = (S + kT - 1) / kT;
+  const int tpf = (S + kIrrTile - 1) / kIrrTile;
   const int face = blockIdx.x / (tpf * tpf);
   const int t = blockIdx.x - face * tpf * tpf;
-  const int lane_texel = threadIdx.x % kTexels;
-  const int slice = threadIdx.x / kTexels;
-  const int x = (t % tpf) * kT + (lane_texel % kT);
-  const int y = (t / tpf) * kT + (lane_texel / kT);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = (t % tpf) * kIrrTile + (lane & 7);
+  const int y = (t / tpf) * kIrrTile + (lane >> 3);
   const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+  const int lo = blockIdx.y * p.rows_per_chunk;
+  const int hi = min(p.theta_samples, lo + p.rows_per_chunk);
 
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float fr32 = 0.0f, fg32 = 0.0f, fb32 = 0.0f;
-  for (int i = slice; i < p.theta_samples; i += kSlices) {
+  for (int i = lo + wave; i < hi; i += 4) {
     const float2 th = theta_tab[i];
     const f3 a = fr.t * th.x + fr.b * th.y;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f;
@@ -467,41 +475,71 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __rest
       sb = fmaf(v.z, ph.z, sb);
     }
     if (kF64Sums) {
-      ar += sr; ag += sg; ab += sb;
+      ar += sr;
+      ag += sg;
+      ab += sb;
     } else {
-      fr32 += sr; fg32 += sg; fb32 += sb;
+      fr32 += sr;
+      fg32 += sg;
+      fb32 += sb;
     }
   }
-  if (!kF64Sums) {
-    ar = fr32; ag = fg32; ab = fb32;
-  }
-  // Reduce the theta slices of this wave (lanes l, l ^ kTexels, ...) with
-  // shuffles, then the four waves in LDS, in a fixed order.
-#pragma unroll
-  for (int o = kTexels; o < 64; o <<= 1) {
-    ar += __shfl_xor(ar, o, 64);
-    ag += __shfl_xor(ag, o, 64);
-    ab += __shfl_xor(ab, o, 64);
-  }
-  __shared__ double s_red[4][kTexels][3];
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) < kTexels) {
-    s_red[wave][lane_texel][0] = ar;
-    s_red[wave][lane_texel][1] = ag;
-    s_red[wave][lane_texel][2] = ab;
-  }
+  __shared__ double s_red[4][64][3];
+  s_red[wave][lane][0] = kF64Sums ? ar : static_cast<double>(fr32);
+  s_red[wave][lane][1] = kF64Sums ? ag : static_cast<double>(fg32);
+  s_red[wave][lane][2] = kF64Sums ? ab : static_cast<double>(fb32);
   __syncthreads();
-  if (threadIdx.x < kTexels && x < S && y < S) {
-    double r = 0.0, g = 0.0, bl = 0.0;
-    for (int w = 0; w < 4; ++w) {
-      r += s_red[w][lane_texel][0];
-      g += s_red[w][lane_texel][1];
-      bl += s_red[w][lane_texel][2];
+  if (wave != 0 || x >= S || y >= S) return;
+  double* out = partial + (static_cast<size_t>(blockIdx.y) * 6 * S * S + (static_cast<size_t>(face) * S + y) * S + x) * 3;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (kF64Sums) {
+      out[c] = ((s_red[0][lane][c] + s_red[1][lane][c]) + s_red[2][lane][c]) + s_red[3][lane][c];
+    } else {
+      float v = static_cast<float>(s_red[0][lane][c]);
+      v += static_cast<float>(s_red[1][lane][c]);
+      v += static_cast<float>(s_red[2][lane][c]);
+      v += static_cast<float>(s_red[3][lane][c]);
+      out[c] = v;
     }
-    cube[(static_cast<size_t>(face) * S + y) * S + x] =
-        make_float4(static_cast<float>(r * p.scale), static_cast<float>(g * p.scale),
-                    static_cast<float>(bl * p.scale), 1.0f);
   }
+}
+
+// Sum of the chunk partials of each texel in chunk order, times pi / (theta phi).
+template <bool kF64Sums>
+__global__ void irradiance_reduce_kernel(const double* __restrict__ partial, int texels, int chunks, double scale,
+                                         float4* __restrict__ cube) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= texels) return;
+  double r = 0.0, g = 0.0, b = 0.0;
+  float rf = 0.0f, gf = 0.0f, bf = 0.0f;
+  for (int c = 0; c < chunks; ++c) {
+    const double* q = partial + (static_cast<size_t>(c) * texels + i) * 3;
+    if (kF64Sums) {
+      r += q[0];
+      g += q[1];
+      b += q[2];
+    } else {
+      rf += static_cast<float>(q[0]);
+      gf += static_cast<float>(q[1]);
+      bf += static_cast<float>(q[2]);
+    }
+  }
+  if (kF64Sums)
+    cube[i] = make_float4(static_cast<float>(r * scale), static_cast<float>(g * scale), static_cast<float>(b * scale),
+                          1.0f);
+  else
+    cube[i] = make_float4(rf * static_cast<float>(scale), gf * static_cast<float>(scale), bf * static_cast<float>(scale),
+                          1.0f);
+}
+
+// Chunks of theta rows: enough blocks for about 8192 waves, at least 4 rows
+// (one per wave) per chunk.
+static int irradiance_chunks(int size, int theta) {
+  const int tpf = (size + kIrrTile - 1) / kIrrTile;
+  const int tiles = 6 * tpf * tpf;
+  const int want = (8192 / 4 + tiles - 1) / tiles;
+  return std::max(1, std::min(want, theta / 4));
 }
 
 static int irradiance_phi_samples(int w, int h, int theta) {
@@ -510,9 +548,10 @@ static int irradiance_phi_samples(int w, int h, int theta) {
   return static_cast<int>(static_cast<float>(h) * static_cast<float>(theta) / static_cast<float>(w));
 }
 
-size_t irradiance_workspace_bytes(int w, int h, int theta) {
+size_t irradiance_workspace_bytes(int w, int h, int size, int theta) {
   int phi = irradiance_phi_samples(w, h, theta);
-  return align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
+  return align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1)) +
+         align256(sizeof(double) * 3 * 6 * static_cast<size_t>(size) * size * irradiance_chunks(size, theta));
 }
 
 althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
@@ -544,28 +583,47 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.lv0 = make_level(g.w[p.l0], g.h[p.l0], g.off[p.l0]);
   p.lv1 = make_level(g.w[p.l1], g.h[p.l1], g.off[p.l1]);
   p.scale = M_PI / theta_samples / phi;
-  // 2x2-texel tiles (64 theta slices per texel) keep a 32^2 cube at 1536
-  // blocks, enough waves to hide the gather latency.
-  constexpr int kT = 2;
-  int tpf = (size + kT - 1) / kT;
-  dim3 grid(6 * tpf * tpf);
+  const int chunks = irradiance_chunks(size, theta_samples);
+  p.rows_per_chunk = (theta_samples + chunks - 1) / chunks;
+  const int used_chunks = (theta_samples + p.rows_per_chunk - 1) / p.rows_per_chunk;
+  double* partial = reinterpret_cast<double*>(ws + align256(sizeof(float2) * theta_samples) +
+                                              align256(sizeof(float4) * std::max(phi, 1)));
+  const int tpf = (size + kIrrTile - 1) / kIrrTile;
+  const dim3 grid(6 * tpf * tpf, used_chunks);
   const char* pyr4 = reinterpret_cast<const char*>(pyr);
   float4* cube4 = reinterpret_cast<float4*>(cube);
-  if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
-    hipLaunchKernelGGL((irradiance_cube_kernel<false, kT>), grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p,
+  const int texels = 6 * size * size;
+  const bool f64 = (flags & ALTHEA_PRECOMPUTE_FP32_SUMS) == 0;
+  if (f64)
+    hipLaunchKernelGGL(irradiance_partial_kernel<true>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, partial);
+  else
+    hipLaunchKernelGGL(irradiance_partial_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, partial);
+  st = check_launch("irradiance_partial_kernel");
+  if (st != ALTHEA_OK) return st;
+  const dim3 rgrid((texels + 255) / 256);
+  if (f64)
+    hipLaunchKernelGGL(irradiance_reduce_kernel<true>, rgrid, dim3(256), 0, s, partial, texels, used_chunks, p.scale,
                        cube4);
   else
-    hipLaunchKernelGGL((irradiance_cube_kernel<true, kT>), grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p,
-                       cube4);
-  return check_launch("irradiance_cube_kernel");
+    hipLaunchKernelGGL(irradiance_reduce_kernel<false>, rgrid, dim3(256), 0, s, partial, texels, used_chunks,
+                       p.scale, cube4);
+  return check_launch("irradiance_reduce_kernel");
 }
 
 // --------------------------------------------------------------- BRDF LUT
-// Karis split-sum (A, B) at texel centre (NdotV, roughness); a 64-lane wave
-// covers 64 NdotV columns of one roughness row so the sample quantities are
-// wave-uniform.  FP32 per-sample arithmetic, FP64 sums and normalisation.
-__global__ __launch_bounds__(64) void brdf_lut_kernel(int size, int samples, float2* __restrict__ rg) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
+// Karis split-sum (A, B) at texel centre (NdotV, roughness).  A block covers
+// 64 NdotV columns of one roughness row (the sample quantities are then
+// wave-uniform) with kLutWaves waves, each owning a contiguous range of
+// samples that is a whole number of 32-sample blocks.  FP32 per-sample
+// arithmetic, 32-sample FP32 block sums flushed into FP64, the waves' FP64
+// partials combined in wave order, FP64 normalisation.
+constexpr int kLutWaves = 16;
+
+__global__ __launch_bounds__(64 * kLutWaves) void brdf_lut_kernel(int size, int samples, int chunk,
+                                                                  float2* __restrict__ rg) {
+  __shared__ double s_a[kLutWaves][64], s_b[kLutWaves][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = blockIdx.x * 64 + lane;
   const int y = blockIdx.y;
   const float nv = (static_cast<float>(x) + 0.5f) / static_cast<float>(size);
   const float r = (static_cast<float>(y) + 0.5f) / static_cast<float>(size);
@@ -576,7 +634,8 @@ __global__ __launch_bounds__(64) void brdf_lut_kernel(int size, int samples, flo
   const float gv = nv / fmaf(nv, 1.0f - k, k);
   double sa = 0.0, sb = 0.0;
   float fa = 0.0f, fb = 0.0f;
-  for (int s = 0; s < samples; ++s) {
+  const int s0 = wave * chunk, s1 = min(samples, s0 + chunk);
+  for (int s = s0; s < s1; ++s) {
     float x1, x2;
     hammersley(static_cast<uint32_t>(s), static_cast<uint32_t>(samples), x1, x2);
     const float den = fmaf(a2 - 1.0f, x2, 1.0f);
@@ -604,12 +663,25 @@ __global__ __launch_bounds__(64) void brdf_lut_kernel(int size, int samples, flo
   }
   sa += fa;
   sb += fb;
-  if (x < size) rg[y * size + x] = make_float2(static_cast<float>(sa / samples), static_cast<float>(sb / samples));
+  s_a[wave][lane] = sa;
+  s_b[wave][lane] = sb;
+  __syncthreads();
+  if (wave != 0 || x >= size) return;
+  double ta = 0.0, tb = 0.0;
+#pragma unroll
+  for (int w = 0; w < kLutWaves; ++w) {
+    ta += s_a[w][lane];
+    tb += s_b[w][lane];
+  }
+  rg[y * size + x] = make_float2(static_cast<float>(ta / samples), static_cast<float>(tb / samples));
 }
 
 althea_status launch_brdf_lut(int size, int samples, float* rg, hipStream_t s) {
   dim3 grid((size + 63) / 64, size);
-  hipLaunchKernelGGL(brdf_lut_kernel, grid, dim3(64), 0, s, size, samples, reinterpret_cast<float2*>(rg));
+  const int per_wave = (samples + kLutWaves - 1) / kLutWaves;
+  const int chunk = (per_wave + kBlockSums - 1) / kBlockSums * kBlockSums;
+  hipLaunchKernelGGL(brdf_lut_kernel, grid, dim3(64 * kLutWaves), 0, s, size, samples, chunk,
+                     reinterpret_cast<float2*>(rg));
   return check_launch("brdf_lut_kernel");
 }
 

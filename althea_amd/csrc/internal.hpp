@@ -65,17 +65,19 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
                                       hipStream_t s);
 althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
                                 unsigned flags, float* cube, void* workspace, hipStream_t s);
-size_t irradiance_workspace_bytes(int env_w, int env_h, int theta_samples);
+size_t irradiance_workspace_bytes(int env_w, int env_h, int size, int theta_samples);
 althea_status launch_brdf_lut(int size, int samples, float* rg, hipStream_t s);
 althea_status launch_cube_sample(const float* cube, int size, int mips, const float* dir_lod, int count,
                                  float* out, void* workspace, hipStream_t s);
 althea_status launch_env_sample(const float* pyr, int w, int h, const float* dir_lod, int count, float* out,
                                 void* workspace, hipStream_t s);
 
-// Deferred resolve (resolve_kernels.hip).
+// Deferred resolve (resolve_kernels.hip); with `mlp` the prefiltered term
+// comes from the MLP where its monitor accepts.
+struct MlpInferenceView;
 althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
-                             hipStream_t s);
+                             const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s);
 
 }  // namespace althea

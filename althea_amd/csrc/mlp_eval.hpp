@@ -1,0 +1,173 @@
+// Device building blocks of the prefiltered-radiance MLP shared by the
+// training kernels, the error monitor and the fused MLP-IBL resolve.
+//
+// MFMA lane layout (v_mfma_f32_16x16x32_bf16, lane l = 16 g + n):
+//   A (16 x 32): lane holds W[row = n][k = 8 g + j], j = 0..7
+//   B (32 x 16): lane holds X^T[k = 8 g + j][sample = n]
+//   D (16 x 16): lane holds D[row = 4 g + i][sample = n], i = 0..3
+// Hidden layers feed the next layer in registers through the K permutation
+// baked into the packed weights (mlp_layout.hpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "althea/althea.h"
+#include "device_math.hpp"
+#include "mlp_layout.hpp"
+#include "mlp_view.hpp"
+
+namespace althea {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 load8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ __forceinline__ bf16x8 pack8(f32x4 lo, f32x4 hi) {
+  bf16x8 r;
+  r[0] = static_cast<__bf16>(lo[0]);
+  r[1] = static_cast<__bf16>(lo[1]);
+  r[2] = static_cast<__bf16>(lo[2]);
+  r[3] = static_cast<__bf16>(lo[3]);
+  r[4] = static_cast<__bf16>(hi[0]);
+  r[5] = static_cast<__bf16>(hi[1]);
+  r[6] = static_cast<__bf16>(hi[2]);
+  r[7] = static_cast<__bf16>(hi[3]);
+  return r;
+}
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+// Input encoding: feature f of (dir, roughness): dir (0..2), sin/cos(2^k pi
+// dir_c) for k = 0..3 (3 + 6k + c sin, 3 + 6k + 3 + c cos), roughness (27),
+// zero padding (28..31).
+__device__ __forceinline__ float encode_feature(int f, float dx, float dy, float dz, float rough) {
+  if (f < 3) return f == 0 ? dx : (f == 1 ? dy : dz);
+  if (f == 27) return rough;
+  if (f > 27) return 0.0f;
+  int idx = f - 3;
+  int k = idx / 6, r = idx - 6 * k;
+  int c = r < 3 ? r : r - 3;
+  float d = c == 0 ? dx : (c == 1 ? dy : dz);
+  float s, co;
+  sincospif(ldexpf(d, k), &s, &co);
+  return r < 3 ? s : co;
+}
+
+// ------------------------------------------------- LDS-resident inference
+// The packed forward operands in LDS, 16-byte groups XOR-swizzled so that
+// each aligned group of 8 lanes of an A-operand read hits distinct banks:
+// 64-wide rows (8 groups) swap group G with G ^ (row & 7), 32-wide rows
+// (4 groups) with G ^ ((row >> 1) & 3).
+struct MlpSmem {
+  bf16x8 w1[64 * 4];
+  bf16x8 w2[64 * 8];
+  bf16x8 w3[64 * 8];
+  bf16x8 w4[16 * 8];
+  float b[3 * 64 + 16];  // b1, b2, b3, b4 (zero-padded to 16)
+};
+// Per-wave staging of the 64 encoded queries (32 BF16 features each).
+struct MlpWaveStage {
+  bf16x8 feat[64 * 4];
+};
+
+__device__ __forceinline__ int mlp_swz64(int row, int grp) { return row * 8 + (grp ^ (row & 7)); }
+__device__ __forceinline__ int mlp_swz32(int row, int grp) { return row * 4 + (grp ^ ((row >> 1) & 3)); }
+
+// Block-cooperative copy of the packed weights into LDS; the caller
+// synchronises the block before the first evaluation.
+__device__ inline void mlp_stage_weights(MlpSmem& s, const MlpDeviceWeights& w, int tid, int nthreads) {
+  for (int i = tid; i < 64 * 4; i += nthreads) s.w1[mlp_swz32(i >> 2, i & 3)] = load8(w.w1 + 8 * i);
+  for (int i = tid; i < 64 * 8; i += nthreads) {
+    s.w2[mlp_swz64(i >> 3, i & 7)] = load8(w.w2 + 8 * i);
+    s.w3[mlp_swz64(i >> 3, i & 7)] = load8(w.w3 + 8 * i);
+  }
+  for (int i = tid; i < 16 * 8; i += nthreads) s.w4[mlp_swz64(i >> 3, i & 7)] = load8(w.w4 + 8 * i);
+  for (int i = tid; i < 64; i += nthreads) {
+    s.b[i] = w.b1[i];
+    s.b[64 + i] = w.b2[i];
+    s.b[128 + i] = w.b3[i];
+    if (i < 16) s.b[192 + i] = w.b4[i];
+  }
+}
+
+// Orders this wave's LDS writes before its later LDS reads from other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Log-radiance of this lane's query (d, rough), evaluated for all 64 queries
+// of the wave as four 16-sample MFMA tiles.  Every lane of the wave must be
+// active.  Arithmetic identical to the streaming inference kernel: same
+// features, same MFMA order, same bias and SiLU.
+__device__ inline f3 mlp_wave_eval(const MlpSmem& s, MlpWaveStage& st, float dx, float dy, float dz, float rough) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, n = lane & 15;
+  wave_lds_sync();  // the previous evaluation's reads are done
+#pragma unroll
+  for (int G = 0; G < 4; ++G) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = static_cast<__bf16>(encode_feature(8 * G + j, dx, dy, dz, rough));
+    st.feat[mlp_swz32(lane, G)] = v;
+  }
+  wave_lds_sync();
+  f3 out = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+  for (int t = 0; t < 4; ++t) {
+    // Reload the weights per tile: hoisting the 88 loop-invariant operand
+    // reads out of this loop would hold 352 VGPRs.
+    asm volatile("" ::: "memory");
+    const bf16x8 x = st.feat[mlp_swz32(16 * t + n, g)];
+    f32x4 h[4];
+    auto act = [&](f32x4 acc, int bias0, int b) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float z = acc[i] + s.b[bias0 + 16 * b + 4 * g + i];
+        h[b][i] = z * sigmoidf(z);
+      }
+    };
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(s.w1[mlp_swz32(16 * b + n, g)], x, acc);
+      act(acc, 0, b);
+    }
+    bf16x8 hb[2] = {pack8(h[0], h[1]), pack8(h[2], h[3])};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) acc = mfma(s.w2[mlp_swz64(16 * b + n, 4 * kb + g)], hb[kb], acc);
+      act(acc, 64, b);
+    }
+    hb[0] = pack8(h[0], h[1]);
+    hb[1] = pack8(h[2], h[3]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) acc = mfma(s.w3[mlp_swz64(16 * b + n, 4 * kb + g)], hb[kb], acc);
+      act(acc, 128, b);
+    }
+    hb[0] = pack8(h[0], h[1]);
+    hb[1] = pack8(h[2], h[3]);
+    f32x4 y = {0.f, 0.f, 0.f, 0.f};
+    y = mfma(s.w4[mlp_swz64(n, g)], hb[0], y);
+    y = mfma(s.w4[mlp_swz64(n, 4 + g)], hb[1], y);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] += s.b[192 + 4 * g + i];
+    // Sample 16 t + n's outputs sit in lane n (g == 0), rows 0..2.
+    const float y0 = __shfl(y[0], n, 64), y1 = __shfl(y[1], n, 64), y2 = __shfl(y[2], n, 64);
+    if (g == t) out = mk(y0, y1, y2);
+  }
+  return out;
+}
+
+}  // namespace althea

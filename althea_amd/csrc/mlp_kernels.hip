@@ -29,49 +29,10 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "mlp_eval.hpp"
 #include "mlp_layout.hpp"
 
 namespace althea {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ bf16x8 load8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
-
-__device__ __forceinline__ bf16x8 pack8(f32x4 lo, f32x4 hi) {
-  bf16x8 r;
-  r[0] = static_cast<__bf16>(lo[0]);
-  r[1] = static_cast<__bf16>(lo[1]);
-  r[2] = static_cast<__bf16>(lo[2]);
-  r[3] = static_cast<__bf16>(lo[3]);
-  r[4] = static_cast<__bf16>(hi[0]);
-  r[5] = static_cast<__bf16>(hi[1]);
-  r[6] = static_cast<__bf16>(hi[2]);
-  r[7] = static_cast<__bf16>(hi[3]);
-  return r;
-}
-
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
-
-// Input encoding: feature f of (dir, roughness): dir (0..2), sin/cos(2^k pi
-// dir_c) for k = 0..3 (3 + 6k + c sin, 3 + 6k + 3 + c cos), roughness (27),
-// zero padding (28..31).
-__device__ __forceinline__ float encode_feature(int f, float dx, float dy, float dz, float rough) {
-  if (f < 3) return f == 0 ? dx : (f == 1 ? dy : dz);
-  if (f == 27) return rough;
-  if (f > 27) return 0.0f;
-  int idx = f - 3;
-  int k = idx / 6, r = idx - 6 * k;
-  int c = r < 3 ? r : r - 3;
-  float d = c == 0 ? dx : (c == 1 ? dy : dz);
-  float s, co;
-  sincospif(ldexpf(d, k), &s, &co);
-  return r < 3 ? s : co;
-}
 
 // ------------------------------------------------------------ pack weights
 // Builds the BF16 operand layouts from the FP32 master copy (mlp_layout.hpp).
@@ -420,6 +381,77 @@ __global__ void mfma_probe_kernel(const float* __restrict__ a, const float* __re
   for (int i = 0; i < 4; ++i) d[(4 * g + i) * 16 + n] = acc[i];  // D[row 4g+i][col n]
 }
 
+// ------------------------------------------------------------ error monitor
+// One block per cell (mip m, face, 8 x 8 grid over the face).  Each texel
+// centre of the cell is queried through the MLP (unit direction of the texel
+// centre, roughness m / (mips - 1)) and compared with the cube texel, which
+// is the judge: e = mean over rgb of |mlp - cube| / max(|cube|, floor).  The
+// cell error is the mean of e over its texels; the cell is accepted when
+// that is at most the threshold.  Texel x of a face of size S lies in cell
+// floor((x + 0.5) * 8 / S), the cell a direction through the texel centre
+// selects; a cell with no texel centre (S < 8) is judged at the texel that
+// contains the cell centre.
+struct MonitorParams {
+  MlpDeviceWeights w;
+  const float4* cube;
+  CubeMips cm;
+  float threshold, floor;
+  float* cell_error;  // [mip][face][cy][cx]
+  uint8_t* cell_ok;
+};
+
+// Texels [lo, hi) of one axis whose centres fall in cell c.
+__device__ __forceinline__ void monitor_range(int c, int S, int& lo, int& hi) {
+  lo = (c * S - 4 + 7) >> 3;  // ceil((c S - 4) / 8)
+  hi = ((c + 1) * S - 4 + 7) >> 3;
+  if (hi <= lo) {
+    lo = ((2 * c + 1) * S) >> 4;
+    hi = lo + 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void mlp_monitor_kernel(const MonitorParams p) {
+  __shared__ MlpSmem s;
+  __shared__ MlpWaveStage st[4];
+  __shared__ float s_sum[4];
+  mlp_stage_weights(s, p.w, threadIdx.x, 256);
+  __syncthreads();
+  const int cx = blockIdx.x & 7, cy = blockIdx.x >> 3, face = blockIdx.y, m = blockIdx.z;
+  const int S = max(1, p.cm.size >> m);
+  int x0, x1, y0, y1;
+  monitor_range(cx, S, x0, x1);
+  monitor_range(cy, S, y0, y1);
+  const int w = x1 - x0, area = w * (y1 - y0);
+  const float rough = p.cm.mips > 1 ? static_cast<float>(m) / static_cast<float>(p.cm.mips - 1) : 0.0f;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float4* mip = p.cube + p.cm.off[m] + static_cast<size_t>(face) * S * S;
+  float sum = 0.0f;
+  for (int base = 0; base < area; base += 256) {
+    const int i = base + static_cast<int>(threadIdx.x);
+    const bool valid = i < area;
+    const int x = x0 + (valid ? i % w : 0), y = y0 + (valid ? i / w : 0);
+    const f3 d = normalize_fast(cube_texel_raw_dir(face, x, y, S));
+    const f3 ly = mlp_wave_eval(s, st[wave], d.x, d.y, d.z, rough);
+    if (valid) {
+      const float4 c = mip[y * S + x];
+      const float e = fabsf(expf(ly.x) - c.x) / fmaxf(fabsf(c.x), p.floor) +
+                      fabsf(expf(ly.y) - c.y) / fmaxf(fabsf(c.y), p.floor) +
+                      fabsf(expf(ly.z) - c.z) / fmaxf(fabsf(c.z), p.floor);
+      sum += e * (1.0f / 3.0f);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if (lane == 0) s_sum[wave] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float err = (s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3]) / static_cast<float>(area);
+    const int cell = (m * 6 + face) * 64 + blockIdx.x;
+    p.cell_error[cell] = err;
+    p.cell_ok[cell] = err <= p.threshold ? 1 : 0;
+  }
+}
+
 // ------------------------------------------------------------ host side
 struct MlpState {
   althea_mlp_desc desc{};
@@ -434,6 +466,10 @@ struct MlpState {
   float* wpartial = nullptr;
   float* bpartial = nullptr;
   int step = 0;
+  // Error monitor of the last althea_mlp_build_monitor call.
+  float* cell_error = nullptr;
+  uint8_t* cell_ok = nullptr;
+  int mon_size = 0, mon_mips = 0;
 };
 
 static constexpr int kFwdBlocks = 256;    // training forward grid
@@ -552,6 +588,20 @@ struct althea_mlp {
   MlpState s;
 };
 
+namespace althea {
+althea_status mlp_inference_view(const althea_mlp* m, MlpInferenceView* v) {
+  if (!m || !v) {
+    set_error("mlp: null handle");
+    return ALTHEA_ERROR_INVALID_ARGUMENT;
+  }
+  v->w = m->s.w;
+  v->cell_ok = m->s.cell_ok;
+  v->mon_size = m->s.mon_size;
+  v->mon_mips = m->s.mon_mips;
+  return ALTHEA_OK;
+}
+}  // namespace althea
+
 extern "C" {
 
 void althea_mlp_desc_default(althea_mlp_desc* d) {
@@ -622,6 +672,8 @@ althea_status althea_mlp_destroy(althea_mlp* m) {
   (void)hipFree(s->act_mem);
   (void)hipFree(s->wpartial);
   (void)hipFree(s->bpartial);
+  (void)hipFree(s->cell_error);
+  (void)hipFree(s->cell_ok);
   delete m;
   return ALTHEA_OK;
 }
@@ -668,6 +720,12 @@ althea_status althea_mlp_gradients(althea_mlp* m, const float* d_dir_rough, cons
   return run_train(&m->s, d_dir_rough, d_targets, count, d_grad, d_loss, false, stream);
 }
 
+althea_status althea_mlp_set_learning_rate(althea_mlp* m, float learning_rate) {
+  ALTHEA_REQUIRE(m && learning_rate > 0.0f, "mlp_set_learning_rate: invalid argument");
+  m->s.desc.learning_rate = learning_rate;
+  return ALTHEA_OK;
+}
+
 althea_status althea_mlp_train_step(althea_mlp* m, const float* d_dir_rough, const float* d_targets, int count,
                                     float* d_loss, hipStream_t stream) {
   ALTHEA_REQUIRE(m && d_dir_rough && d_targets && count >= 1, "mlp_train_step: invalid argument");
@@ -679,6 +737,53 @@ althea_status althea_mfma_bf16_probe(const float* d_a, const float* d_b, float* 
   ALTHEA_REQUIRE(d_a && d_b && d_d, "mfma_bf16_probe: null buffer");
   hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, stream, d_a, d_b, d_d);
   return check_launch("mfma_probe_kernel");
+}
+
+althea_status althea_mlp_build_monitor(althea_mlp* m, const float* d_prefiltered, int size, int mips,
+                                      float threshold, float* d_cell_error, int* accepted, hipStream_t stream) {
+  ALTHEA_REQUIRE(m && d_prefiltered, "mlp_build_monitor: null argument");
+  ALTHEA_REQUIRE(is_pow2(size) && mips >= 1 && mips <= kMaxLevels && (size >> (mips - 1)) >= 1,
+                 "mlp_build_monitor: cube %d / %d mips", size, mips);
+  ALTHEA_REQUIRE(!(threshold != threshold), "mlp_build_monitor: threshold is NaN");
+  MlpState* s = &m->s;
+  const int cells = mips * 6 * ALTHEA_MLP_MONITOR_CELLS;
+  if (s->mon_mips != mips || !s->cell_error) {
+    ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(s->cell_error);
+    (void)hipFree(s->cell_ok);
+    s->cell_error = nullptr;
+    s->cell_ok = nullptr;
+    s->mon_size = s->mon_mips = 0;
+    ALTHEA_TRY_HIP(hipMalloc(&s->cell_error, cells * sizeof(float)));
+    ALTHEA_TRY_HIP(hipMalloc(&s->cell_ok, cells));
+  }
+  MonitorParams p{};
+  p.w = s->w;
+  p.cube = reinterpret_cast<const float4*>(d_prefiltered);
+  p.cm.size = size;
+  p.cm.mips = mips;
+  for (int l = 0; l < mips; ++l) p.cm.off[l] = static_cast<uint32_t>(cube_mip_offset(size, l));
+  p.threshold = threshold;
+  p.floor = ALTHEA_MLP_MONITOR_FLOOR;
+  p.cell_error = s->cell_error;
+  p.cell_ok = s->cell_ok;
+  hipLaunchKernelGGL(mlp_monitor_kernel, dim3(ALTHEA_MLP_MONITOR_CELLS, 6, mips), dim3(256), 0, stream, p);
+  althea_status st = check_launch("mlp_monitor_kernel");
+  if (st != ALTHEA_OK) return st;
+  s->mon_size = size;
+  s->mon_mips = mips;
+  if (d_cell_error)
+    ALTHEA_TRY_HIP(hipMemcpyAsync(d_cell_error, s->cell_error, cells * sizeof(float), hipMemcpyDeviceToDevice,
+                                  stream));
+  if (accepted) {
+    std::vector<uint8_t> ok(cells);
+    ALTHEA_TRY_HIP(hipMemcpyAsync(ok.data(), s->cell_ok, cells, hipMemcpyDeviceToHost, stream));
+    ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+    int n = 0;
+    for (uint8_t v : ok) n += v;
+    *accepted = n;
+  }
+  return ALTHEA_OK;
 }
 
 size_t althea_mlp_state_bytes(const althea_mlp* m) {

@@ -15,10 +15,12 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "device_math.hpp"
 #include "internal.hpp"
+#include "mlp_eval.hpp"
 
 namespace althea {
 
@@ -36,8 +38,8 @@ struct ResolveParams {
   int irr_size;
   const float2* lut;
   int lut_size;
-  const althea_point_light* lights;
   int light_count;
+  int xcd_run;  // tiles per run dealt to one XCD (xcd_tile)
   float ray0[3], rayx[3], rayy[3];  // view ray = ray0 + rayx*(px+0.5) + rayy*(py+0.5)
   float view_z[4];                  // row 2 of the view matrix
   float vp[16];                     // projection * view, column-major
@@ -51,6 +53,8 @@ struct ResolveParams {
   float noise[16][4];
   void* out;
   float* ao_out;
+  MlpInferenceView mlp;  // MLP variant only
+  uint8_t* mlp_used;
 };
 
 template <int kFmt>
@@ -161,18 +165,21 @@ __device__ __forceinline__ void lut_sample(const ResolveParams& p, float u, floa
   B = fmaf(bb - tb, ay, tb);
 }
 
-constexpr int kLightChunk = 256;  // lights staged in LDS per pass
+// LDS of the MLP variant: the per-wave query staging, then the weights.
+struct MlpResolveSmem {
+  MlpWaveStage stage[4];
+  MlpSmem mlp;
+};
 
-template <int kFmt, bool kOutF16>
-__global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
-  __shared__ float4 s_noise[16];
-  __shared__ float4 s_light_pos[kLightChunk];  // xyz position, w = 1 if the emission is grey
-  __shared__ float4 s_light_em[kLightChunk];
-  if (threadIdx.x < 16)
-    s_noise[threadIdx.x] =
-        make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
-  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int py = blockIdx.y * 4 + (threadIdx.x >> 6);
+// One 64 x 4 tile (linear id vt before the XCD mapping).  Every lane of
+// the block takes part in the block barriers of the light staging.
+template <int kFmt, bool kOutF16, bool kMlp>
+__device__ __forceinline__ void resolve_tile(const ResolveParams& p, int vt, int tiles, int tiles_x,
+                                             const float4* __restrict__ lights, const float4* s_noise,
+                                             MlpWaveStage* s_stage, const MlpSmem* s_mlp) {
+  const int tile = xcd_tile(vt, tiles, p.xcd_run);
+  const int px = (tile % tiles_x) * 64 + (threadIdx.x & 63);
+  const int py = (tile / tiles_x) * 4 + (threadIdx.x >> 6);
   const bool inside = px < p.width && py < p.height;
   const size_t idx = static_cast<size_t>(min(py, p.height - 1)) * p.width + min(px, p.width - 1);
   const float cxp = static_cast<float>(px) + 0.5f, cyp = static_cast<float>(py) + 0.5f;
@@ -192,23 +199,50 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
       store_out<kOutF16>(p.out, idx, c);
     }
   }
-  // Every lane stays for the light staging barriers; background lanes and
-  // lanes outside the image shade nothing, store nothing and skip the other
+  // Every lane stays for the block barriers; background lanes and lanes
+  // outside the image shade nothing, store nothing and skip the other
   // G-buffer planes.
   const bool shade = inside && !background;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 nrm = shade ? load_gbuf<kFmt>(p.normal, idx) : zero4;
-  const float4 alb = shade ? load_gbuf<kFmt>(p.albedo, idx) : zero4;
   const float4 mro = shade ? load_gbuf<kFmt>(p.mro, idx) : zero4;
   const f3 P = mk(pos.x, pos.y, pos.z);
-  const f3 N = shade ? normalize_fast(mk(nrm.x, nrm.y, nrm.z)) : mk(0.0f, 0.0f, 1.0f);
-  const f3 base = mk(alb.x, alb.y, alb.z);
+  // Lanes that shade nothing get N = 0, so no light passes their horizon test.
+  const f3 N = shade ? normalize_fast(mk(nrm.x, nrm.y, nrm.z)) : mk(0.0f, 0.0f, 0.0f);
   const float metallic = mro.x, roughness = shade ? mro.y : 1.0f;
   float ao = mro.z;
-
   const f3 I = normalize_fast(dir);
-  f3 color = mk(0.0f, 0.0f, 0.0f);
   const float NI = dot(N, I);
+  const float nd = 2.0f * NI;
+  const f3 R = mk(fmaf(-nd, N.x, I.x), fmaf(-nd, N.y, I.y), fmaf(-nd, N.z, I.z));
+  const float lod = roughness * static_cast<float>(p.pref.mips - 1);
+  f3 refl = mk(0.0f, 0.0f, 0.0f);
+  bool from_mlp = false;
+  if (kMlp) {
+    // Monitor cell of R at the two mips the cube lookup would blend.
+    int rface;
+    float rpc, rpt;
+    cube_select(R, rface, rpc, rpt);
+    const float d = fminf(fmaxf(lod, 0.0f), static_cast<float>(p.pref.mips - 1));
+    const float fl = floorf(d);
+    const int m0 = static_cast<int>(fl), m1 = min(m0 + 1, p.pref.mips - 1);
+    const int cx = min(max(static_cast<int>((rpc + 1.0f) * 4.0f), 0), 7);
+    const int cy = min(max(static_cast<int>((rpt + 1.0f) * 4.0f), 0), 7);
+    const int cell = rface * ALTHEA_MLP_MONITOR_CELLS + cy * 8 + cx;
+    const int per_mip = 6 * ALTHEA_MLP_MONITOR_CELLS;
+    from_mlp = shade && p.mlp.cell_ok[m0 * per_mip + cell] != 0 &&
+               (d == fl || p.mlp.cell_ok[m1 * per_mip + cell] != 0);
+    if (__any(from_mlp)) {
+      const f3 ly = mlp_wave_eval(*s_mlp, s_stage[threadIdx.x >> 6], R.x, R.y, R.z, roughness);
+      refl = mk(expf(ly.x), expf(ly.y), expf(ly.z));
+    }
+    if (inside && p.mlp_used) p.mlp_used[idx] = from_mlp ? 1 : 0;
+    // Keep the remaining G-buffer reads after the evaluation (register pressure).
+    asm volatile("" ::: "memory");
+  }
+  const float4 alb = shade ? load_gbuf<kFmt>(p.albedo, idx) : zero4;
+  const f3 base = mk(alb.x, alb.y, alb.z);
+  f3 color = mk(0.0f, 0.0f, 0.0f);
   const float n_dot_v = fmaxf(-NI, 0.0f);
   const f3 F0 = mk(fmaf(metallic, base.x - 0.04f, 0.04f), fmaf(metallic, base.y - 0.04f, 0.04f),
                    fmaf(metallic, base.z - 0.04f, 0.04f));
@@ -219,9 +253,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
   const f3 dF = mk(fmaxf(om_r, F0.x) - F0.x, fmaxf(om_r, F0.y) - F0.y, fmaxf(om_r, F0.z) - F0.z);
   const f3 kd = base * (1.0f - metallic);
   if (shade) {
-    const float nd = 2.0f * NI;
-    const f3 R = mk(fmaf(-nd, N.x, I.x), fmaf(-nd, N.y, I.y), fmaf(-nd, N.z, I.z));
-    const f3 refl = cube_sample(p.prefiltered, p.pref, R, roughness * static_cast<float>(p.pref.mips - 1));
+    if (!from_mlp) refl = cube_sample(p.prefiltered, p.pref, R, lod);
     int face;
     float pc, pt;
     cube_select(N, face, pc, pt);
@@ -257,22 +289,15 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
   const float a2m1 = a2 - 1.0f;
   const f3 kdp = kd * kInvPi;
   float A1 = 0.0f, A2 = 0.0f, A3 = 0.0f, A4 = 0.0f;
-  for (int base_i = 0; base_i < p.light_count; base_i += kLightChunk) {
-    const int n = min(kLightChunk, p.light_count - base_i);
-    __syncthreads();
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      const althea_point_light lt = p.lights[base_i + k];
-      const bool grey = lt.emission[0] == lt.emission[1] && lt.emission[1] == lt.emission[2];
-      s_light_pos[k] = make_float4(lt.position[0], lt.position[1], lt.position[2], grey ? 1.0f : 0.0f);
-      s_light_em[k] = make_float4(lt.emission[0], lt.emission[1], lt.emission[2], 0.0f);
-    }
-    __syncthreads();
-    if (!__any(shade)) continue;
-    for (int i = 0; i < n; ++i) {
-      const float4 lp = s_light_pos[i];
+  // Lights are wave-uniform: scalar loads (constant cache) into SGPRs, and
+  // the grey test is a scalar compare of the emission bits.
+  if (__any(shade)) {
+    for (int i = 0; i < p.light_count; ++i) {
+      const float4 lp = lights[2 * i];
+      const float4 em = lights[2 * i + 1];
       const f3 L = mk(lp.x - P.x, lp.y - P.y, lp.z - P.z);
       const float nl_raw = dot(N, L);
-      if (!__any(shade && nl_raw > 0.0f)) continue;
+      if (__builtin_amdgcn_ballot_w64(nl_raw > 0.0f) == 0) continue;
       const float d2 = dot(L, L);
       const float inv_d = rsq_fast(d2);
       const float vl = dot(I, L);
@@ -285,8 +310,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
       const float den = (t * t) * fmaf(n_dot_l, om_k, kdir) * fmaf(n_dot_l, four_nv, 0.0001f);
       const float s = c1 * n_dot_l * rcp_fast(den);
       const float w = (inv_d * inv_d) * n_dot_l;
-      const float4 em = s_light_em[i];
-      if (lp.w != 0.0f) {
+      const bool grey = __float_as_uint(em.x) == __float_as_uint(em.y) && __float_as_uint(em.y) == __float_as_uint(em.z);
+      if (grey) {
         const float ew = em.x * w;
         const float S = s * ew;
         A1 += ew;
@@ -301,12 +326,51 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p) {
       }
     }
   }
-  if (!shade) return;
-  color.x += fmaf(F0.x, fmaf(-kdp.x, A1, A2), fmaf(dF.x, fmaf(-kdp.x, A3, A4), kdp.x * A1));
-  color.y += fmaf(F0.y, fmaf(-kdp.y, A1, A2), fmaf(dF.y, fmaf(-kdp.y, A3, A4), kdp.y * A1));
-  color.z += fmaf(F0.z, fmaf(-kdp.z, A1, A2), fmaf(dF.z, fmaf(-kdp.z, A3, A4), kdp.z * A1));
-  if (p.tonemap) color = tonemap(color, p.exposure);
-  store_out<kOutF16>(p.out, idx, color);
+  if (shade) {
+    color.x += fmaf(F0.x, fmaf(-kdp.x, A1, A2), fmaf(dF.x, fmaf(-kdp.x, A3, A4), kdp.x * A1));
+    color.y += fmaf(F0.y, fmaf(-kdp.y, A1, A2), fmaf(dF.y, fmaf(-kdp.y, A3, A4), kdp.y * A1));
+    color.z += fmaf(F0.z, fmaf(-kdp.z, A1, A2), fmaf(dF.z, fmaf(-kdp.z, A3, A4), kdp.z * A1));
+    if (p.tonemap) color = tonemap(color, p.exposure);
+    store_out<kOutF16>(p.out, idx, color);
+  }
+}
+
+// Shared prologue of both kernels: the SSAO noise tile (and the MLP weights)
+// in LDS, visible to every wave.
+template <bool kMlp>
+__device__ __forceinline__ void resolve_prologue(const ResolveParams& p, float4* s_noise, MlpSmem* s_mlp) {
+  if (threadIdx.x < 16)
+    s_noise[threadIdx.x] =
+        make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
+  if (kMlp) mlp_stage_weights(*s_mlp, p.mlp.w, threadIdx.x, 256);
+  __syncthreads();
+}
+
+// Plain resolve: one block per tile.
+template <int kFmt, bool kOutF16>
+__global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, const float4* __restrict__ lights) {
+  __shared__ float4 s_noise[16];
+  resolve_prologue<false>(p, s_noise, nullptr);
+  const int tiles_x = (p.width + 63) / 64;
+  const int tiles = tiles_x * ((p.height + 3) / 4);
+  resolve_tile<kFmt, kOutF16, false>(p, blockIdx.x, tiles, tiles_x, lights, s_noise, nullptr, nullptr);
+}
+
+// MLP-IBL resolve: persistent blocks keep the MLP weights in LDS for several
+// tiles; with a grid that is a multiple of 8, xcd_tile keeps each XCD's tiles
+// contiguous.  The kernel needs about 150 VGPRs (3 waves per SIMD, 3 blocks
+// per CU); a budget of 128 for 4 waves per SIMD would spill.
+constexpr int kMlpBlocksPerCu = 3;
+template <int kFmt, bool kOutF16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMlpBlocksPerCu))) void resolve_mlp_kernel(
+    const ResolveParams p, const float4* __restrict__ lights) {
+  __shared__ float4 s_noise[16];
+  __shared__ MlpResolveSmem s_m;
+  resolve_prologue<true>(p, s_noise, &s_m.mlp);
+  const int tiles_x = (p.width + 63) / 64;
+  const int tiles = tiles_x * ((p.height + 3) / 4);
+  for (int vt = blockIdx.x; vt < tiles; vt += gridDim.x)
+    resolve_tile<kFmt, kOutF16, true>(p, vt, tiles, tiles_x, lights, s_noise, s_m.stage, &s_m.mlp);
 }
 
 // Host: the view ray of DeferredPass.vert is linear in NDC, so it is affine in
@@ -330,10 +394,12 @@ static void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3]
   }
 }
 
+constexpr int kXcdRunRows = 8;
+
 althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
-                             hipStream_t s) {
+                             const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s) {
   ResolveParams p{};
   p.position = g->position;
   p.normal = g->normal;
@@ -353,7 +419,6 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   p.irr_size = ibl->irradiance_size;
   p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
   p.lut_size = ibl->brdf_lut_size;
-  p.lights = lights;
   p.light_count = light_count;
   view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy);
   for (int c = 0; c < 4; ++c) p.view_z[c] = globals->view[c * 4 + 2];
@@ -379,19 +444,42 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   }
   p.out = out;
   p.ao_out = ao_out;
-  dim3 grid((g->width + 63) / 64, (g->height + 3) / 4);
+  const int tiles = ((g->width + 63) / 64) * ((g->height + 3) / 4);
+  // Runs of kXcdRunRows tile rows per XCD.
+  p.xcd_run = kXcdRunRows * ((g->width + 63) / 64);
+  const float4* lights4 = reinterpret_cast<const float4*>(lights);
   const bool f16_out = (flags & ALTHEA_RESOLVE_OUT_FP16) != 0;
-  if (g->format == ALTHEA_GBUFFER_FP32) {
-    if (f16_out)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p);
+  const bool fp32 = g->format == ALTHEA_GBUFFER_FP32;
+  if (mlp) {
+    p.mlp = *mlp;
+    p.mlp_used = mlp_used;
+    // Persistent blocks, as many as are resident at once; a multiple of 8
+    // keeps each XCD's tiles contiguous.
+    int dev = 0, cus = 256;
+    ALTHEA_TRY_HIP(hipGetDevice(&dev));
+    ALTHEA_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int blocks = std::min(tiles, kMlpBlocksPerCu * cus);
+    if (blocks >= kXcds) blocks -= blocks % kXcds;
+    const dim3 grid(blocks);
+    if (fp32 && f16_out)
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);
+    else if (fp32)
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p, lights4);
+    else if (f16_out)
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p, lights4);
     else
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p);
-  } else {
-    if (f16_out)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p);
-    else
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p);
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p, lights4);
+    return check_launch("resolve_mlp_kernel");
   }
+  const dim3 grid(tiles);
+  if (fp32 && f16_out)
+    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);
+  else if (fp32)
+    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p, lights4);
+  else if (f16_out)
+    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p, lights4);
+  else
+    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p, lights4);
   return check_launch("resolve_kernel");
 }
 

@@ -65,6 +65,9 @@ class PrefilterMLP:
         assert p.size == param_count()
         check(lib().althea_mlp_set_params(self._h, p.ctypes.data, _stream()), "althea_mlp_set_params")
 
+    def set_learning_rate(self, lr: float) -> None:
+        check(lib().althea_mlp_set_learning_rate(self._h, float(lr)), "althea_mlp_set_learning_rate")
+
     def train_step(self, dir_rough, targets, loss_out=None, stream=None) -> None:
         check(lib().althea_mlp_train_step(self._h, _ptr(dir_rough), _ptr(targets), int(dir_rough.shape[0]),
                                           _ptr(loss_out), _stream(stream)), "althea_mlp_train_step")
@@ -88,6 +91,18 @@ class PrefilterMLP:
     def state_bytes(self) -> int:
         return lib().althea_mlp_state_bytes(self._h)
 
+    def build_monitor(self, prefiltered, size: int, mips: int, threshold: float = 1e-3, stream=None):
+        """Judge the MLP against every texel of the prefiltered cube (device
+        tensor of cube texels x 4); returns (cell errors as a device tensor
+        [mips, 6, 8, 8], number of accepted cells).  The handle keeps the
+        acceptance for DeferredRendering.resolve_mlp."""
+        torch = _torch()
+        err = torch.empty((mips, 6, 8, 8), dtype=torch.float32, device="cuda")
+        acc = C.c_int(0)
+        check(lib().althea_mlp_build_monitor(self._h, _ptr(prefiltered), size, mips, float(threshold), _ptr(err),
+                                             C.byref(acc), _stream(stream)), "althea_mlp_build_monitor")
+        return err, acc.value
+
 
 def mfma_probe(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     """D = A . B through one BF16 MFMA with the MLP kernels' lane layout."""
@@ -97,3 +112,41 @@ def mfma_probe(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     td = torch.empty((16, 16), dtype=torch.float32, device="cuda")
     check(lib().althea_mfma_bf16_probe(_ptr(ta), _ptr(tb), _ptr(td), _stream()), "althea_mfma_bf16_probe")
     return td.cpu().numpy()
+
+
+def training_set(pyramid, env_w: int, env_h: int, count: int, seed: int, samples: int = 1024, rmin: float = 0.0,
+                 rmax: float = 1.0, stream=None):
+    """(direction, roughness) queries, uniform on the sphere and in
+    [rmin, rmax], labelled by the plain GPU prefilter with `samples`
+    Hammersley samples (float4 radiance per query)."""
+    torch = _torch()
+    from . import prefilter_points, synth
+
+    dr = torch.from_numpy(synth.dir_rough(seed=seed, n=count, rmin=rmin, rmax=rmax)).cuda()
+    return dr, prefilter_points(pyramid, env_w, env_h, samples, dr, stream=stream)
+
+
+def fit(net: PrefilterMLP, dir_rough, labels, batch: int = 65536, epochs: int = 8, lr: float | None = None,
+        final_lr: float | None = None, stream=None) -> list:
+    """Adam over the training set in contiguous batches (the queries are
+    i.i.d., so a fixed order is a random order), `epochs` passes.  With `lr`
+    and `final_lr` the step size decays geometrically from one to the other
+    across the epochs.  Returns the loss of each epoch's first batch (mean
+    squared log error before its step)."""
+    torch = _torch()
+    n = int(dir_rough.shape[0])
+    steps = max(1, n // batch)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    history = []
+    for ep in range(epochs):
+        if lr is not None:
+            f = ep / max(1, epochs - 1)
+            net.set_learning_rate(lr * ((final_lr / lr) ** f if final_lr else 1.0))
+        for k in range(steps):
+            lo = k * batch
+            hi = n if k == steps - 1 else lo + batch
+            net.train_step(dir_rough[lo:hi], labels[lo:hi], loss_out=loss if k == 0 else None, stream=stream)
+            if k == 0:
+                history.append(loss)
+                loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    return [float(h.item()) for h in history]

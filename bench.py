@@ -38,6 +38,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
     p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
     p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
+    p.add_argument("--only", choices=["resolve", "prefilter", "copy"],
+                   help="profiling runs: time one kernel family and print its line only "
+                        "(copy = a plain 1 GiB device copy, the calibration for HBM byte counters)")
     return p.parse_args()
 
 
@@ -106,6 +109,57 @@ def batched(torch, aa) -> dict:
             "tuning": tuning, "autotune_s": tune_s}
 
 
+def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
+    """The learned stand-in on the same frame: 2^22 (direction, roughness)
+    queries labelled by the 1024-sample prefilter of the frame's environment,
+    Adam (lr 1e-3, batch 65536), error monitor at 1e-3 against the 512^2 x6
+    cube, then the fused MLP-IBL resolve.  Its frame is compared with the
+    cube-IBL frame over every pixel (mean relative difference, floor 1e-2)."""
+    from althea_amd import mlp as M
+
+    t0 = time.perf_counter()
+    pyr = ibl.environment_tensor()
+    dr, labels = M.training_set(pyr, env.shape[1], env.shape[0], 1 << 22, seed=SEED0 + 4, samples=1024)
+    torch.cuda.synchronize()
+    label_s = time.perf_counter() - t0
+    net = M.PrefilterMLP(seed=SEED0 + 5, lr=1e-3, init_output_bias=float(np.log(labels[:, :3].mean().item())))
+    epochs = 16
+    t0 = time.perf_counter()
+    hist = M.fit(net, dr, labels, batch=65536, epochs=epochs)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    train_steps = epochs * ((1 << 22) // 65536)
+    v = ibl.views()
+    t0 = time.perf_counter()
+    cell_err, accepted = net.build_monitor(ibl.prefiltered_tensor(), v.prefiltered_size, v.prefiltered_mips,
+                                           threshold=1e-3)
+    monitor_s = time.perf_counter() - t0
+    cells = cell_err.numel()
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    used = torch.zeros(WIDTH * HEIGHT, dtype=torch.uint8, device="cuda")
+
+    def step():
+        aa.DeferredRendering.resolve_mlp(gbuf, ibl, net, lights, scene.globals, scene.ssao, out=out, mlp_used=used)
+
+    step()
+    ms = time_events(torch, step, steps)
+    plain = aa.DeferredRendering.resolve(gbuf, ibl, lights, scene.globals, scene.ssao)
+    d = ((out[:, :3] - plain[:, :3]).abs() / plain[:, :3].abs().clamp_min(1e-2)).double()
+    diff = {"mean": float(d.mean().item()), "max": float(d.max().item())}
+    e = cell_err.cpu().numpy()
+    return {"mpix_per_s": WIDTH * HEIGHT / (ms * 1e-3) / 1e6, "ms": ms,
+            "mlp_pixel_fraction": float(used.float().mean().item()),
+            "monitor": {"threshold": 1e-3, "accepted_cells": accepted, "cells": cells,
+                        "cell_error_quantiles": {q: float(np.quantile(e, float(q))) for q in ("0.1", "0.5", "0.9")},
+                        "build_s": monitor_s},
+            "rel_diff_vs_cube_ibl_frame": diff,
+            "ibl_state_bytes": {"cube_ibl": ibl.state_bytes(), "mlp_weights": net.state_bytes(),
+                                "monitor_flags": cells},
+            "training": {"queries": 1 << 22, "label_spp": 1024, "label_s": label_s, "batch": 65536,
+                         "steps": train_steps, "train_s": train_s, "ms_per_step": 1e3 * train_s / train_steps,
+                         "loss_first": hist[0], "loss_last": hist[-1]}}
+
+
 def cpu_baseline(aa, scene, ibl, env) -> dict:
     """The FP64 oracle (a from-source restatement of the reference shaders) on
     the host cores: the whole 3840x2160 frame, 64 lights, SSAO."""
@@ -124,6 +178,38 @@ def cpu_baseline(aa, scene, ibl, env) -> dict:
             "sample": f"full {WIDTH}x{HEIGHT} frame, {LIGHTS} lights, SSAO 16 (FP64 oracle, {dt:.1f} s)"}
 
 
+def timed_steps(torch, fn, args) -> float:
+    for _ in range(args.warmup):
+        fn()
+    return time_events(torch, fn, args.steps)
+
+
+def copy_calibration(torch, args) -> int:
+    """A plain elementwise copy of 1 GiB (read 1 GiB, write 1 GiB): what the HBM
+    byte counters read for a kernel whose traffic is known."""
+    n = 1 << 28
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    ms = timed_steps(torch, lambda: torch.neg(src, out=dst), args)
+    nbytes = 2 * n * 4
+    print(json.dumps({"kind": "copy", "bytes_per_launch": nbytes, "ms": ms, "gbs": nbytes / (ms * 1e-3) / 1e9}),
+          flush=True)
+    return 0
+
+
+def prefilter_only(torch, aa, args) -> int:
+    """The largest single prefilter: 4096x2048 source (8 sun lobes up to 1e4)
+    -> 1024^2 x6 cube, 5 roughness mips, 4096 samples per texel."""
+    env = aa.synth.env(seed=SEED0 + 2, width=4096, height=2048, suns=8, peak_max=1e4)
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    texels = aa.cube_texels(1024, 5)
+    ms = timed_steps(torch, lambda: aa.prefilter_cube(pyr, 4096, 2048, 1024, 5, 4096), args)
+    print(json.dumps({"kind": "prefilter", "workload": "4096x2048 -> 1024^2x6x5 mips, 4096 spp", "ms": ms,
+                      "texels_per_s": texels / (ms * 1e-3), "samples_per_s": texels * 4096 / (ms * 1e-3)}),
+          flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -136,6 +222,10 @@ def main() -> int:
     import althea_amd as aa
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.only == "copy":
+        return copy_calibration(torch, args)
+    if args.only == "prefilter":
+        return prefilter_only(torch, aa, args)
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
     desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
                       brdf_lut_size=128, brdf_lut_samples=1024)
@@ -177,9 +267,10 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": nbytes, "kernel": "resolve_kernel"},
     }
-    if not args.no_secondary:
+    if not args.no_secondary and args.only is None:
         line["secondary"] = secondary(torch, aa)
-    if not args.no_cpu_baseline:
+        line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
+    if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)
     print(json.dumps(line), flush=True)
     return 0

@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "althea/althea.h"
 
@@ -74,6 +75,34 @@ inline IBLResources createResources(const float* d_equirect, int width, int heig
   throwIfFailed(althea_ibl_create(&desc, &ibl), "ImageBasedLighting::createResources");
   IBLResources res(ibl);
   throwIfFailed(althea_ibl_precompute(ibl, d_equirect, width, height, stream), "ImageBasedLighting::createResources");
+  return res;
+}
+
+// createResources from an RGBE file, the reference's form (it loads
+// Content/HDRI_Skybox/<name>.hdr and throws "Specified environment map does
+// not exist!" when the file is missing, ImageBasedLighting.cpp:420-425).  The
+// map must have power-of-two extents.  Synchronises `stream` before the
+// staging buffer is released.
+inline IBLResources createResources(const std::string& hdrPath, const althea_ibl_desc& desc = defaultDesc(),
+                                    hipStream_t stream = nullptr) {
+  int w = 0, h = 0;
+  althea_status st = althea_hdr_info(hdrPath.c_str(), &w, &h);
+  if (st == ALTHEA_ERROR_NOT_FOUND) throw std::runtime_error("Specified environment map does not exist!");
+  throwIfFailed(st, "ImageBasedLighting::createResources");
+  std::vector<float> host(static_cast<size_t>(w) * h * 4);
+  throwIfFailed(althea_hdr_read(hdrPath.c_str(), host.data(), host.size()), "ImageBasedLighting::createResources");
+  float* d_env = nullptr;
+  if (hipMalloc(&d_env, host.size() * sizeof(float)) != hipSuccess)
+    throw std::runtime_error("ImageBasedLighting::createResources: device allocation failed");
+  struct Free {
+    float* p;
+    ~Free() { (void)hipFree(p); }
+  } guard{d_env};
+  if (hipMemcpyAsync(d_env, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream) != hipSuccess)
+    throw std::runtime_error("ImageBasedLighting::createResources: upload failed");
+  IBLResources res = createResources(d_env, w, h, desc, stream);
+  if (hipStreamSynchronize(stream) != hipSuccess)
+    throw std::runtime_error("ImageBasedLighting::createResources: precompute failed");
   return res;
 }
 

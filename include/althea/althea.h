@@ -299,6 +299,22 @@ ALTHEA_API althea_status althea_deferred_resolve(const althea_gbuffer_view* gbuf
                                                  hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* RGBE (.hdr) files                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* The format of the reference's environment maps (Content/HDRI_Skybox) and of
+ * its prefiltered-map cache (Content/PrecomputedMaps), read and written there
+ * through stb_image (Src/Utilities.cpp:216-255): Radiance header, flat or
+ * run-length coded RGBE scanlines, value = byte * 2^(E - 136).  Host memory,
+ * RGBA32F rows top-down (alpha 1 on read, ignored on write).  A missing file
+ * is ALTHEA_ERROR_NOT_FOUND (ImageBasedLighting.cpp:423-425 throws there), a
+ * malformed one ALTHEA_ERROR_IO, another pixel format or orientation
+ * ALTHEA_ERROR_UNSUPPORTED. */
+ALTHEA_API althea_status althea_hdr_info(const char* path, int* width, int* height);
+ALTHEA_API althea_status althea_hdr_read(const char* path, float* host_rgba, size_t capacity_floats);
+ALTHEA_API althea_status althea_hdr_write(const char* path, const float* host_rgba, int width, int height);
+
+/* ------------------------------------------------------------------------ */
 /* Learned prefiltered radiance (no counterpart in the reference)            */
 /* ------------------------------------------------------------------------ */
 
@@ -330,6 +346,9 @@ ALTHEA_API althea_status althea_mlp_get_params(const althea_mlp* mlp, float* hos
  * receives the batch's mean squared log error before the step. */
 ALTHEA_API althea_status althea_mlp_train_step(althea_mlp* mlp, const float* d_dir_rough, const float* d_targets,
                                                int count, float* d_loss, hipStream_t stream);
+/* Adam step size for the following train steps (a schedule sets it between
+ * steps; the moments and step count are kept). */
+ALTHEA_API althea_status althea_mlp_set_learning_rate(althea_mlp* mlp, float learning_rate);
 /* The batch gradient (10371 floats, parameter order) without an update. */
 ALTHEA_API althea_status althea_mlp_gradients(althea_mlp* mlp, const float* d_dir_rough, const float* d_targets,
                                               int count, float* d_grad, float* d_loss, hipStream_t stream);
@@ -338,6 +357,36 @@ ALTHEA_API althea_status althea_mlp_infer(const althea_mlp* mlp, const float* d_
                                           hipStream_t stream);
 /* Bytes of inference state (BF16 weights + FP32 biases). */
 ALTHEA_API size_t althea_mlp_state_bytes(const althea_mlp* mlp);
+
+/* Error monitor.  The prefiltered cube is the judge: the MLP is evaluated at
+ * every texel centre of every mip m of d_prefiltered (size, mips; roughness
+ * m/(mips-1), the roughness the cube was prefiltered at) and per texel
+ *   e = mean over rgb of |mlp - cube| / max(|cube|, ALTHEA_MLP_MONITOR_FLOOR).
+ * Each face is split into an 8 x 8 grid of cells (texel x of a size-S face
+ * is in cell floor((x + 0.5) * 8 / S)); a cell's error is the mean e of its
+ * texels and the cell is accepted when that is <= threshold.  The handle
+ * keeps the acceptance for althea_deferred_resolve_mlp.  d_cell_error
+ * (optional, device) receives mips * 6 * 64 floats [mip][face][cy][cx];
+ * accepted (optional, host) the number of accepted cells, which synchronises
+ * `stream`. */
+#define ALTHEA_MLP_MONITOR_CELLS 64
+#define ALTHEA_MLP_MONITOR_FLOOR 1e-2f
+ALTHEA_API althea_status althea_mlp_build_monitor(althea_mlp* mlp, const float* d_prefiltered, int size, int mips,
+                                                  float threshold, float* d_cell_error, int* accepted,
+                                                  hipStream_t stream);
+
+/* althea_deferred_resolve with the prefiltered-radiance term taken from the
+ * MLP (BF16 MFMA inside the resolve, weights in LDS) where the monitor
+ * accepted the cell of the reflection direction at both mips bracketing the
+ * pixel's LOD roughness*(mips-1), and from the cube everywhere else.  The
+ * monitor must have been built for ibl->prefiltered_size / prefiltered_mips.
+ * d_mlp_used (optional): one byte per pixel, 1 where the MLP supplied the
+ * term. */
+ALTHEA_API althea_status althea_deferred_resolve_mlp(const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl,
+                                                     const althea_mlp* mlp, const althea_point_light* d_lights,
+                                                     int light_count, const althea_globals* globals,
+                                                     const althea_ssao_desc* ssao, unsigned flags, void* d_out,
+                                                     float* d_ao_out, uint8_t* d_mlp_used, hipStream_t stream);
 
 /* Diagnostic: one 16x16x32 BF16 MFMA through the lane layout the MLP kernels
  * assume.  d_a 16x32, d_b 32x16 (row-major FP32, rounded to BF16 on load),

@@ -714,11 +714,17 @@ inline dvec3 view_ray(const ResolveInputs& in, int px, int py) {
 inline dvec3 tonemap(dvec3 c, double exposure) {
   return {1.0 - std::exp(-c.x * exposure), 1.0 - std::exp(-c.y * exposure), 1.0 - std::exp(-c.z * exposure)};
 }
-inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_out, double* margin) {
+// refl_in (optional, 3 doubles): the prefiltered radiance to use instead of
+// the cube lookup (the learned stand-in's value), ignored when refl_in[0] is
+// NaN.  query_out (optional, 4 doubles): the reflection direction and
+// roughness the prefiltered term is looked up with (NaN for background).
+inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_out, double* margin,
+                           const double* refl_in = nullptr, double* query_out = nullptr) {
   size_t idx = static_cast<size_t>(py) * in.W + px;
   dvec3 dir = view_ray(in, px, py);
   const float* p = in.position + 4 * idx;
   if (margin) *margin = 1e30;
+  if (query_out) query_out[0] = query_out[1] = query_out[2] = query_out[3] = std::nan("");
   if (p[3] == 0.0f) {
     dvec3 bg = sample_env_background(*in.env, dir);
     if (in.tonemap) bg = tonemap(bg, in.exposure);
@@ -734,7 +740,15 @@ inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_o
   double metallic = m[0], roughness = m[1], ao = m[2];
   dvec3 I = normalize(dir);
   dvec3 R = I - N * (2.0 * dot(N, I));
-  dvec3 refl = cube_sample(in.prefiltered, R, roughness * (in.prefiltered.mips - 1));
+  if (query_out) {
+    query_out[0] = R.x;
+    query_out[1] = R.y;
+    query_out[2] = R.z;
+    query_out[3] = roughness;
+  }
+  dvec3 refl = (refl_in && !std::isnan(refl_in[0]))
+                   ? dvec3{refl_in[0], refl_in[1], refl_in[2]}
+                   : cube_sample(in.prefiltered, R, roughness * (in.prefiltered.mips - 1));
   dvec3 irr = cube_sample(in.irradiance, N, 0.0);
   if (in.ssao_on) ao = ssao(in.position, in.W, in.H, px, py, P, N, in.view, in.proj, in.ssao, margin);
   if (ao_out) *ao_out = ao;

@@ -224,8 +224,11 @@ struct oracle_resolve_args {
   float ssao_bias;
 };
 
-void oracle_resolve(const oracle_resolve_args* a, const int* pixels, int count, double* out3,
-                    double* out_ao, double* out_margin) {
+// refl_in (optional, 3 per pixel): prefiltered radiance replacing the cube
+// lookup where refl_in[3 i] is not NaN; out_query (optional, 4 per pixel):
+// reflection direction and roughness of each pixel's lookup.
+void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int count, const double* refl_in,
+                       double* out3, double* out_ao, double* out_margin, double* out_query) {
   std::vector<Light> lights(a->light_count);
   for (int i = 0; i < a->light_count; ++i) {
     const float* l = a->lights + 8 * i;
@@ -256,11 +259,17 @@ void oracle_resolve(const oracle_resolve_args* a, const int* pixels, int count, 
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < count; ++i) {
     double ao, margin;
-    dvec3 c = resolve_pixel(in, pixels[2 * i], pixels[2 * i + 1], &ao, &margin);
+    dvec3 c = resolve_pixel(in, pixels[2 * i], pixels[2 * i + 1], &ao, &margin, refl_in ? refl_in + 3 * i : nullptr,
+                            out_query ? out_query + 4 * i : nullptr);
     out3[3 * i] = c.x; out3[3 * i + 1] = c.y; out3[3 * i + 2] = c.z;
     if (out_ao) out_ao[i] = ao;
     if (out_margin) out_margin[i] = margin;
   }
+}
+
+void oracle_resolve(const oracle_resolve_args* a, const int* pixels, int count, double* out3, double* out_ao,
+                    double* out_margin) {
+  oracle_resolve_ex(a, pixels, count, nullptr, out3, out_ao, out_margin, nullptr);
 }
 
 // pbrMaterial for one shading point.  vec: pos, V, N, base, refl, irr (18

@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
         L.oracle_brdf_lut.argtypes = [i32, i32, vp, i32, vp]
         L.oracle_brdf_lut_point.argtypes = [dbl, dbl, i32, vp]
         L.oracle_resolve.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp]
+        L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
         L.oracle_mlp_param_count.restype = i32
         L.oracle_mlp_encode.argtypes = [vp, i32, vp]
@@ -237,10 +238,13 @@ class Env:
 
 
 # ----------------------------------------------------------------- resolve
-def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer=None):
+def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer=None, refl=None,
+            query: bool = False):
     """Oracle resolve of pixels[:, (x, y)] for a synth.Scene and IBL views
     (numpy dict with prefiltered / irradiance / lut arrays and sizes).
-    Returns (rgb, ao, decision margin)."""
+    Returns (rgb, ao, decision margin), plus the (R, roughness) query of each
+    pixel's prefiltered lookup when `query`.  `refl` (n, 3): prefiltered
+    radiance replacing the cube lookup where refl[i, 0] is not NaN."""
     planes = gbuffer if gbuffer is not None else (scene.position, scene.normal, scene.albedo, scene.material)
     planes = [_c(p, np.float32) for p in planes]
     pref = _c(views["prefiltered"], np.float32)
@@ -260,8 +264,11 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
     rgb = np.empty((px.shape[0], 3), np.float64)
     ao = np.empty(px.shape[0], np.float64)
     margin = np.empty(px.shape[0], np.float64)
-    lib().oracle_resolve(C.byref(a), _p(px), px.shape[0], _p(rgb), _p(ao), _p(margin))
-    return rgb, ao, margin
+    r_in = _c(refl, np.float64) if refl is not None else None
+    q = np.empty((px.shape[0], 4), np.float64) if query else None
+    lib().oracle_resolve_ex(C.byref(a), _p(px), px.shape[0], _p(r_in) if r_in is not None else None, _p(rgb),
+                            _p(ao), _p(margin), _p(q) if q is not None else None)
+    return (rgb, ao, margin, q) if query else (rgb, ao, margin)
 
 
 def pbr_material(pos, V, N, base, refl, irr, metallic: float, roughness: float, ao: float, lut, lut_size: int,

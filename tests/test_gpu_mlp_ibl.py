@@ -1,0 +1,186 @@
+"""The learned prefiltered-radiance stand-in in use: error monitor, fused
+MLP-IBL resolve and what the trained network gains, through the C ABI.
+
+- Monitor: every cell error the GPU reports equals a host recomputation.
+  That recomputation queries the MLP (streaming inference kernel) at each
+  texel centre and applies the cell rule. The tolerance covers BF16 feature
+  rounding flips from FP32 vs FP64 texel directions.
+- Fused resolve: rejecting every cell reproduces the plain resolve exactly.
+  Accepting every cell matches the oracle resolve that takes its
+  prefiltered term from the same MLP at the oracle's reflection query.
+- Gain: on rough lobes, the trained 21 KB network is closer to the exact
+  prefilter than a cube holding more bytes.
+"""
+import numpy as np
+import pytest
+
+import althea_amd as aa
+from althea_amd import mlp as M
+from tests import oracle_lib as ol
+from tests.parity import RADIANCE_FLOOR, rel_err, summary
+
+pytestmark = pytest.mark.gpu
+
+SEED0 = 0x414C0000
+FLOOR = 1e-2  # ALTHEA_MLP_MONITOR_FLOOR
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    assert t.cuda.is_available()
+    return t
+
+
+def _smooth_env():
+    return aa.synth.env(seed=SEED0 + 100, width=512, height=256, sigma=0.0, suns=4, peak_min=100.0, peak_max=100.0,
+                        sun_sigma_deg=8.0)
+
+
+def _ibl(torch, env, size, mips, samples=256):
+    desc = aa.IBLDesc(prefiltered_size=size, prefiltered_mips=mips, prefilter_samples=samples)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    torch.cuda.synchronize()
+    return ibl
+
+
+def _trained(torch, env, steps_epochs=6, count=1 << 17, rmin=0.0, seed=7, batch=8192, samples=256, final_lr=None):
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    w, h = env.shape[1], env.shape[0]
+    dr, labels = M.training_set(pyr, w, h, count, seed=seed, samples=samples, rmin=rmin)
+    bias = float(np.log(labels[:, :3].mean().item()))
+    net = M.PrefilterMLP(seed=seed + 1, lr=3e-3, init_output_bias=bias)
+    hist = M.fit(net, dr, labels, batch=batch, epochs=steps_epochs, lr=3e-3, final_lr=final_lr)
+    return net, pyr, hist
+
+
+def _cell_range(c, S):
+    lo = -((4 - c * S) // 8)  # ceil((c S - 4) / 8)
+    hi = -((4 - (c + 1) * S) // 8)
+    if hi <= lo:
+        lo = ((2 * c + 1) * S) // 16
+        hi = lo + 1
+    return lo, hi
+
+
+def host_monitor(torch, net, cube, size, mips):
+    """Cell errors [mips, 6, 8, 8] from the streaming inference kernel at
+    every texel centre, with the monitor's cell rule and floor."""
+    out = np.zeros((mips, 6, 8, 8))
+    off = 0
+    for m in range(mips):
+        S = max(1, size >> m)
+        r = m / (mips - 1) if mips > 1 else 0.0
+        f, y, x = np.meshgrid(np.arange(6), np.arange(S), np.arange(S), indexing="ij")
+        ids = np.stack([f.ravel(), x.ravel(), y.ravel()], 1).astype(np.int32)
+        d = ol.cube_texel_dirs(S, ids).astype(np.float32)
+        q = np.concatenate([d, np.full((d.shape[0], 1), r, np.float32)], 1)
+        pred = net.infer(torch.from_numpy(q).cuda()).cpu().numpy()[:, :3].astype(np.float64)
+        ref = cube[off:off + 6 * S * S, :3].astype(np.float64)
+        e = (np.abs(pred - ref) / np.maximum(np.abs(ref), FLOOR)).mean(1).reshape(6, S, S)
+        for cy in range(8):
+            y0, y1 = _cell_range(cy, S)
+            for cx in range(8):
+                x0, x1 = _cell_range(cx, S)
+                out[m, :, cy, cx] = e[:, y0:y1, x0:x1].mean((1, 2))
+        off += 6 * S * S
+    return out
+
+
+def test_monitor_matches_host_recompute(torch):
+    env = _smooth_env()
+    ibl = _ibl(torch, env, size=32, mips=4)  # mip 3 has 4^2 faces: cells without texel centres
+    net, _, _ = _trained(torch, env, steps_epochs=2, count=1 << 15)
+    cube_t = ibl.prefiltered_tensor()
+    err, accepted = net.build_monitor(cube_t, 32, 4, threshold=0.05)
+    gpu = err.cpu().numpy()
+    host = host_monitor(torch, net, cube_t.cpu().numpy(), 32, 4)
+    assert np.isfinite(gpu).all()
+    np.testing.assert_allclose(gpu, host, rtol=2e-2, atol=1e-4)
+    assert accepted == int((gpu <= 0.05).sum())
+    # Cells far from the threshold agree on acceptance with the host values.
+    far = np.abs(host - 0.05) > 1e-3
+    assert np.array_equal((gpu <= 0.05)[far], (host <= 0.05)[far])
+
+
+def _scene_run(torch, scene, ibl, net=None, used=False):
+    g = scene.gbuffer_tensors("cuda")
+    lights = scene.lights_tensor("cuda")
+    if net is None:
+        out = aa.DeferredRendering.resolve(g, ibl, lights, scene.globals, scene.ssao)
+        torch.cuda.synchronize()
+        return out.cpu().numpy().reshape(scene.height, scene.width, 4), None
+    mask = torch.zeros(scene.pixels, dtype=torch.uint8, device="cuda")
+    out = aa.DeferredRendering.resolve_mlp(g, ibl, net, lights, scene.globals, scene.ssao, mlp_used=mask)
+    torch.cuda.synchronize()
+    return (out.cpu().numpy().reshape(scene.height, scene.width, 4),
+            mask.cpu().numpy().reshape(scene.height, scene.width))
+
+
+def test_resolve_mlp_rejecting_every_cell_is_the_plain_resolve(torch):
+    env = _smooth_env()
+    ibl = _ibl(torch, env, size=64, mips=5)
+    net = M.PrefilterMLP(seed=11)
+    _, accepted = net.build_monitor(ibl.prefiltered_tensor(), 64, 5, threshold=-1.0)
+    assert accepted == 0
+    scene = aa.synth.scene(seed=SEED0 + 5, width=333, height=187, lights=8)
+    plain, _ = _scene_run(torch, scene, ibl)
+    fused, mask = _scene_run(torch, scene, ibl, net)
+    assert mask.sum() == 0
+    np.testing.assert_array_equal(fused, plain)
+
+
+def test_resolve_mlp_accepting_every_cell_matches_oracle(torch):
+    env = _smooth_env()
+    ibl = _ibl(torch, env, size=64, mips=5)
+    net, _, _ = _trained(torch, env, steps_epochs=2, count=1 << 15)
+    err, accepted = net.build_monitor(ibl.prefiltered_tensor(), 64, 5, threshold=1e30)
+    assert accepted == 5 * 6 * 64
+    scene = aa.synth.scene(seed=SEED0 + 6, width=320, height=180, lights=8)
+    img, mask = _scene_run(torch, scene, ibl, net)
+    shaded = scene.position[..., 3].reshape(scene.height, scene.width) != 0
+    assert np.array_equal(mask.astype(bool), shaded)
+    rng = np.random.default_rng(6)
+    pix = np.stack([rng.integers(0, scene.width, 4000), rng.integers(0, scene.height, 4000)], 1).astype(np.int32)
+    views, oenv = ibl.views_numpy(), ol.Env(env)
+    _, _, _, q = ol.resolve(scene, views, oenv, pix, query=True)
+    refl = np.full((pix.shape[0], 3), np.nan)
+    fg = ~np.isnan(q[:, 0])
+    pred = net.infer(torch.from_numpy(q[fg].astype(np.float32)).cuda()).cpu().numpy()[:, :3]
+    refl[fg] = pred
+    ref, _, margin = ol.resolve(scene, views, oenv, pix, refl=refl)
+    got = img[pix[:, 1], pix[:, 0], :3]
+    clear = margin > 2e-3
+    e = rel_err(got, ref, RADIANCE_FLOOR).max(1)[clear]
+    s = summary(e)
+    # Same bar as the plain resolve on a smooth source, plus the rare BF16
+    # rounding flip of an MLP feature between the FP32 and FP64 reflection
+    # directions.
+    assert s["mean"] < 1e-5 and s["p99"] < 1e-4, s
+
+
+def test_mlp_gain_over_cube_of_more_bytes(torch):
+    """Rough lobes (roughness 0.5..1): the trained MLP (BF16 weights, 21 KB)
+    against a 16^2 x6 cube with 5 mips (RGBA32F, 32.7 KB) prefiltered from
+    the same source, both judged against the 1024-sample prefilter at the
+    exact query."""
+    env = _smooth_env()
+    net, pyr, hist = _trained(torch, env, steps_epochs=200, count=1 << 20, rmin=0.5, batch=65536, samples=1024,
+                              final_lr=1e-5)
+    assert hist[-1] < hist[0] / 10, hist
+    w, h = env.shape[1], env.shape[0]
+    q, truth = M.training_set(pyr, w, h, 1 << 14, seed=99, samples=1024, rmin=0.5)
+    pred = net.infer(q).cpu().numpy()[:, :3]
+    small = aa.prefilter_cube(pyr, w, h, 16, 5, 1024)
+    qn = q.cpu().numpy()
+    # Cube mips hold roughness m/4; the cube path reads LOD roughness*4.
+    dir_lod = np.concatenate([qn[:, :3], qn[:, 3:4] * 4.0], 1).astype(np.float32)
+    cube_pred = aa.cube_sample(small, 16, 5, torch.from_numpy(dir_lod).cuda()).cpu().numpy()[:, :3]
+    t = truth.cpu().numpy()[:, :3]
+    e_mlp = rel_err(pred, t, RADIANCE_FLOOR).mean()
+    e_cube = rel_err(cube_pred, t, RADIANCE_FLOOR).mean()
+    assert net.state_bytes() < 16 * aa.cube_texels(16, 5)
+    print(f"mean rel error: mlp {e_mlp:.3e} ({net.state_bytes()} B), cube16 {e_cube:.3e} "
+          f"({16 * aa.cube_texels(16, 5)} B)")
+    assert e_mlp < e_cube, (e_mlp, e_cube)

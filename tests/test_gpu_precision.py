@@ -1,0 +1,118 @@
+"""Precision study: every output of the path against the FP64 oracle, for the
+FP64-reduction build and its all-FP32 variant, and for the FP16 G-buffer.
+
+Source: 4096x2048 synthetic HDR map (log-normal sigma 1.5 per texel, 8 sun
+lobes with peaks in [1e3, 1e4], angular sigma 2 degrees).
+- Prefiltered: 1024^2 x6, 5 roughness mips, 4096 samples per texel. That is
+  34 G texel-samples, with FP64 weight sums against all-FP32 sums.
+- Irradiance: 32^2 cube, 45 000 samples per texel.
+- BRDF LUT: 128^2, 1024 samples.
+- Shading: the 3840x2160 scene with 64 lights and SSAO, from an FP32 and
+  from an FP16 G-buffer; both are judged against the oracle shading the FP32
+  G-buffer.
+
+Relative error is |x - ref| / max(|ref|, 1e-2), the max over rgb per output.
+The test prints max / mean / p99 per output (pytest -s shows the table).
+Its assertions are the regression bounds of the measured figures.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import althea_amd as aa
+from tests import oracle_lib as ol
+from tests.parity import LUT_FLOOR, RADIANCE_FLOOR, cube_gather, cube_texel_ids, rel_err, summary
+
+pytestmark = pytest.mark.gpu
+
+SEED0 = 0x414C0000
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.fixture(scope="module")
+def source(torch):
+    env = aa.synth.env(seed=SEED0 + 2, width=4096, height=2048, suns=8, peak_min=1e3, peak_max=1e4)
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    return env, pyr, ol.Env(env)
+
+
+REPORT = {}
+
+
+def _row(name, err):
+    s = summary(err)
+    REPORT[name] = {k: float(v) for k, v in s.items()}
+    print(f"{name:44s} max {s['max']:.3e}  mean {s['mean']:.3e}  p99 {s['p99']:.3e}")
+    return s
+
+
+def test_prefilter_fp64_vs_fp32_sums(torch, source):
+    env, pyr, oenv = source
+    ids = cube_texel_ids(1024, 5, per_face=24, seed=21)
+    ref = oenv.prefilter_texels(1024, 5, 4096, ids)[:, :3]
+    errs = {}
+    for fp32 in (False, True):
+        cube = aa.prefilter_cube(pyr, 4096, 2048, 1024, 5, 4096, fp32_sums=fp32)
+        got = cube_gather(cube.cpu().numpy(), 1024, ids)
+        del cube
+        errs[fp32] = rel_err(got, ref, RADIANCE_FLOOR)
+        tag = "all-FP32 sums" if fp32 else "FP64 sums"
+        _row(f"prefiltered 1024^2x6x5 4096spp, {tag}", errs[fp32])
+        for m in range(5):
+            sel = ids[:, 0] == m
+            _row(f"  mip {m} (roughness {m / 4:.2f}), {tag}", errs[fp32][sel])
+    s64 = summary(errs[False])
+    rough = ids[:, 0] >= 1
+    # Rough mips average thousands of lookups; mip 0 is a mirror lookup of
+    # per-texel noise (FP32 addressing x neighbour contrast).
+    assert summary(errs[False][rough])["max"] < 1e-4, summary(errs[False][rough])
+    assert s64["mean"] < 1e-4, s64
+    assert errs[False][rough].mean() <= errs[True][rough].mean() * 1.5 + 1e-8
+
+
+def test_irradiance_fp64_vs_fp32_sums(torch, source):
+    env, pyr, oenv = source
+    ids = cube_texel_ids(32, 1, per_face=30, seed=22)
+    ref = oenv.irradiance_texels(32, ids[:, 1:])
+    for fp32 in (False, True):
+        cube = aa.irradiance_cube(pyr, 4096, 2048, 32, fp32_sums=fp32).cpu().numpy()
+        e = rel_err(cube_gather(cube, 32, ids), ref, RADIANCE_FLOOR)
+        s = _row(f"irradiance 32^2x6 45000spp, {'all-FP32' if fp32 else 'FP64'} sums", e)
+        if not fp32:
+            assert s["max"] < 1e-4, s
+
+
+def test_brdf_lut_precision(torch):
+    lut = aa.brdf_lut(128, 1024).cpu().numpy().reshape(-1, 2)
+    ref = ol.brdf_lut(128, 1024)
+    s = _row("BRDF LUT 128^2 1024spp (FP64 sums + normalisation)", rel_err(lut, ref, LUT_FLOOR))
+    assert s["max"] < 1e-4, s
+
+
+def test_resolve_fp16_vs_fp32_gbuffer(torch, source):
+    env1 = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    desc = aa.IBLDesc(prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env1).cuda(), desc)
+    scene = aa.synth.scene(seed=SEED0 + 1, width=3840, height=2160, lights=64)
+    rng = np.random.default_rng(23)
+    pix = np.stack([rng.integers(0, 3840, 3000), rng.integers(0, 2160, 3000)], 1).astype(np.int32)
+    ref, _, margin = ol.resolve(scene, ibl.views_numpy(), ol.Env(env1), pix)
+    lin = pix[:, 1] * 3840 + pix[:, 0]
+    res = {}
+    for fp16 in (False, True):
+        g = scene.gbuffer_tensors("cuda", fp16=fp16)
+        out = aa.DeferredRendering.resolve(g, ibl, scene.lights_tensor("cuda"), scene.globals, scene.ssao)
+        got = out.cpu().numpy()[lin, :3]
+        res[fp16] = rel_err(got, ref, RADIANCE_FLOOR)
+        _row(f"shaded 3840x2160 from {'FP16' if fp16 else 'FP32'} G-buffer", res[fp16])
+    clear = margin > 2e-3
+    assert summary(res[False][clear])["mean"] < 1e-4
+    print("PRECISION_REPORT " + json.dumps(REPORT))
