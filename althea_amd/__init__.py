@@ -19,6 +19,7 @@ GPU these calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -173,11 +174,14 @@ class IBLResources:
         h = C.c_void_p()
         check(lib().althea_ibl_create(C.byref(desc.c()), C.byref(h)), "althea_ibl_create")
         self._h = h
+        self._destroy = lib().althea_ibl_destroy
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().althea_ibl_destroy(h)
+        # The destroy function is bound at creation: module globals may be
+        # gone when this runs at interpreter exit.
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self._h = None
 
     @property
@@ -246,15 +250,39 @@ class IBLResources:
 
 class ImageBasedLighting:
     """ImageBasedLighting::createResources (ImageBasedLighting.h:45-49): the
-    environment comes in as a device tensor instead of a file name."""
+    environment comes in as a device tensor (h, w, 4) or as the path of an
+    RGBE .hdr file, the reference's form (a missing file raises, as
+    ImageBasedLighting.cpp:423-425 does)."""
 
     @staticmethod
     def createResources(env, desc: IBLDesc | None = None, stream=None) -> IBLResources:
+        if isinstance(env, (str, os.PathLike)):
+            from . import hdr
+
+            env = _torch().from_numpy(hdr.read(env)).cuda()
         res = IBLResources(desc or IBLDesc())
         res.precompute(env, stream)
         return res
 
     create_resources = createResources
+
+    @staticmethod
+    def createResourcesBatched(envs, descs, tuning=None, stream=None):
+        """IBL resources for many environment maps (device tensors (h, w, 4))
+        with one batched prefilter launch (althea_ibl_precompute_batched).
+        `descs`: one IBLDesc per map or a single one for all; they must share
+        mips, samples and precision.  `tuning` (tile_w, tile_h, waves_per_cu)
+        or None for the autotuner.  Returns (resources, tuning used)."""
+        if isinstance(descs, IBLDesc):
+            descs = [descs] * len(envs)
+        res = [IBLResources(d) for d in descs]
+        views = (_lib.EnvViewC * len(envs))(*[_lib.EnvViewC(_ptr(e), int(e.shape[1]), int(e.shape[0])) for e in envs])
+        handles = (C.c_void_p * len(envs))(*[r.handle.value for r in res])
+        t_in = _lib.TuningC(*tuning, 0) if tuning is not None else None
+        used = _lib.TuningC()
+        check(lib().althea_ibl_precompute_batched(handles, views, len(envs), C.byref(t_in) if t_in else None,
+                                                  C.byref(used), _stream(stream)), "althea_ibl_precompute_batched")
+        return res, (used.tile_w, used.tile_h, used.waves_per_cu)
 
 
 # ------------------------------------------------------ G-buffer / resolve

@@ -93,6 +93,8 @@ PROTOTYPES = {
     "althea_ibl_destroy": (i32, [vp]),
     "althea_ibl_precompute": (i32, [vp, vp, i32, i32, vp]),
     "althea_ibl_get_views": (i32, [vp, C.POINTER(IBLViewsC)]),
+    "althea_ibl_precompute_batched": (i32, [C.POINTER(vp), C.POINTER(EnvViewC), i32, C.POINTER(TuningC),
+                                            C.POINTER(TuningC), vp]),
     "althea_ibl_state_bytes": (sz, [vp]),
     "althea_deferred_resolve": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
                                       C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),

@@ -52,14 +52,17 @@ class PrefilterBatch:
         check(lib().althea_prefilter_batch_create(views, n, sizes, mips, samples, flags, C.byref(h)),
               "althea_prefilter_batch_create")
         self._h = h
+        self._destroy = lib().althea_prefilter_batch_destroy
         self.count = n
         self.mips = mips
         self.samples = samples
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().althea_prefilter_batch_destroy(h)
+        # The destroy function is bound at creation: module globals may be
+        # gone when this runs at interpreter exit.
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self._h = None
 
     def run(self, stream=None) -> None:

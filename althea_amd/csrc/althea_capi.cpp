@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "althea/althea.h"
 #include "internal.hpp"
@@ -292,10 +293,10 @@ althea_status althea_ibl_destroy(althea_ibl* ibl) {
   return ALTHEA_OK;
 }
 
-althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, int width, int height,
-                                    hipStream_t stream) {
-  ALTHEA_REQUIRE(ibl && d_equirect, "ibl_precompute: null argument");
-  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_precompute: %dx%d is not a power-of-two extent", width, height);
+}  // extern "C"
+
+// Sizes the handle's pyramid and workspace for a width x height source.
+static althea_status ibl_prepare(althea_ibl* ibl, int width, int height, hipStream_t stream) {
   const althea_ibl_desc& d = ibl->desc;
   size_t texels = althea_env_pyramid_texels(width, height);
   if (texels != ibl->env_texels) {
@@ -320,7 +321,19 @@ althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, in
     ALTHEA_TRY_HIP(hipMalloc(&ibl->workspace, ws_bytes));
     ibl->workspace_bytes = ws_bytes;
   }
-  althea_status st = launch_pyramid(d_equirect, width, height, ibl->env, stream);
+  return ALTHEA_OK;
+}
+
+extern "C" {
+
+althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, int width, int height,
+                                    hipStream_t stream) {
+  ALTHEA_REQUIRE(ibl && d_equirect, "ibl_precompute: null argument");
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_precompute: %dx%d is not a power-of-two extent", width, height);
+  const althea_ibl_desc& d = ibl->desc;
+  althea_status st = ibl_prepare(ibl, width, height, stream);
+  if (st != ALTHEA_OK) return st;
+  st = launch_pyramid(d_equirect, width, height, ibl->env, stream);
   if (st != ALTHEA_OK) return st;
   st = launch_irradiance(ibl->env, width, height, d.irradiance_size, d.irradiance_theta_samples, d.flags,
                          ibl->irradiance, ibl->workspace, stream);
@@ -329,6 +342,45 @@ althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, in
                              d.flags, ibl->prefiltered, ibl->workspace, stream);
   if (st != ALTHEA_OK) return st;
   return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
+}
+
+althea_status althea_ibl_precompute_batched(althea_ibl* const* ibls, const althea_env_view* envs, int count,
+                                            const althea_tuning* tuning, althea_tuning* used, hipStream_t stream) {
+  ALTHEA_REQUIRE(ibls && envs && count >= 1, "ibl_precompute_batched: invalid argument");
+  for (int i = 0; i < count; ++i) {
+    ALTHEA_REQUIRE(ibls[i] && envs[i].rgba, "ibl_precompute_batched: null handle or map %d", i);
+    ALTHEA_REQUIRE(env_dims_ok(envs[i].width, envs[i].height), "ibl_precompute_batched: map %d is %dx%d", i,
+                   envs[i].width, envs[i].height);
+    const althea_ibl_desc& a = ibls[0]->desc;
+    const althea_ibl_desc& b = ibls[i]->desc;
+    ALTHEA_REQUIRE(b.prefiltered_mips == a.prefiltered_mips && b.prefilter_samples == a.prefilter_samples &&
+                       (b.flags & ALTHEA_PRECOMPUTE_FP32_SUMS) == (a.flags & ALTHEA_PRECOMPUTE_FP32_SUMS),
+                   "ibl_precompute_batched: handle %d differs in mips, samples or precision from handle 0", i);
+    for (int j = 0; j < i; ++j)
+      ALTHEA_REQUIRE(ibls[j] != ibls[i], "ibl_precompute_batched: handle %d repeats handle %d", i, j);
+  }
+  std::vector<float*> pyr(count), cubes(count);
+  std::vector<int> sizes(count);
+  for (int i = 0; i < count; ++i) {
+    althea_status st = ibl_prepare(ibls[i], envs[i].width, envs[i].height, stream);
+    if (st != ALTHEA_OK) return st;
+    pyr[i] = ibls[i]->env;
+    cubes[i] = ibls[i]->prefiltered;
+    sizes[i] = ibls[i]->desc.prefiltered_size;
+  }
+  const althea_ibl_desc& d0 = ibls[0]->desc;
+  althea_status st = prefilter_batch_into(envs, count, sizes.data(), d0.prefiltered_mips, d0.prefilter_samples,
+                                          d0.flags, pyr.data(), cubes.data(), tuning, used, stream);
+  if (st != ALTHEA_OK) return st;
+  for (int i = 0; i < count; ++i) {
+    const althea_ibl_desc& d = ibls[i]->desc;
+    st = launch_irradiance(ibls[i]->env, envs[i].width, envs[i].height, d.irradiance_size, d.irradiance_theta_samples,
+                           d.flags, ibls[i]->irradiance, ibls[i]->workspace, stream);
+    if (st != ALTHEA_OK) return st;
+    st = launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibls[i]->lut, stream);
+    if (st != ALTHEA_OK) return st;
+  }
+  return ALTHEA_OK;
 }
 
 althea_status althea_ibl_get_views(const althea_ibl* ibl, althea_ibl_views* v) {

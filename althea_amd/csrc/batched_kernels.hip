@@ -125,6 +125,7 @@ struct althea_prefilter_batch {
   // Device state.
   std::vector<float*> pyramids;
   std::vector<float*> cubes;
+  bool owns_buffers = true;  // false when the pyramids and cubes belong to IBL handles
   BatchMap* d_maps = nullptr;
   LevelC* d_levels = nullptr;
   GgxEntry* d_tables = nullptr;
@@ -269,10 +270,11 @@ althea_status build_graph(althea_prefilter_batch* b) {
 
 }  // namespace
 
-extern "C" {
-
-althea_status althea_prefilter_batch_create(const althea_env_view* envs, int count, const int* face_sizes, int mips,
-                                            int samples, unsigned flags, althea_prefilter_batch** out) {
+// ext_pyramids / ext_cubes (optional): caller buffers the batch writes into
+// instead of its own.
+static althea_status batch_create(const althea_env_view* envs, int count, const int* face_sizes, int mips,
+                                  int samples, unsigned flags, float* const* ext_pyramids, float* const* ext_cubes,
+                                  althea_prefilter_batch** out) {
   ALTHEA_REQUIRE(envs && face_sizes && out && count >= 1, "prefilter_batch_create: invalid argument");
   *out = nullptr;
   ALTHEA_REQUIRE(samples >= 1 && samples <= (1 << 20), "prefilter_batch_create: samples %d", samples);
@@ -306,6 +308,7 @@ althea_status althea_prefilter_batch_create(const althea_env_view* envs, int cou
   std::vector<LevelC> levels(static_cast<size_t>(count) * kMaxLevels);
   b->pyramids.assign(count, nullptr);
   b->cubes.assign(count, nullptr);
+  b->owns_buffers = ext_pyramids == nullptr;
   for (int i = 0; i < count; ++i) {
     const auto ext = std::make_pair(envs[i].width, envs[i].height);
     auto it = set_of.find(ext);
@@ -313,8 +316,12 @@ althea_status althea_prefilter_batch_create(const althea_env_view* envs, int cou
       it = set_of.emplace(ext, static_cast<int>(b->extents.size())).first;
       b->extents.push_back(ext);
     }
-    if (hipMalloc(&b->pyramids[i], 16 * althea_env_pyramid_texels(envs[i].width, envs[i].height)) != hipSuccess ||
-        hipMalloc(&b->cubes[i], 16 * althea_cube_texels(face_sizes[i], mips)) != hipSuccess) {
+    if (!b->owns_buffers) {
+      b->pyramids[i] = ext_pyramids[i];
+      b->cubes[i] = ext_cubes[i];
+    } else if (hipMalloc(&b->pyramids[i], 16 * althea_env_pyramid_texels(envs[i].width, envs[i].height)) !=
+                   hipSuccess ||
+               hipMalloc(&b->cubes[i], 16 * althea_cube_texels(face_sizes[i], mips)) != hipSuccess) {
       set_error("prefilter_batch_create: out of device memory at map %d", i);
       return fail(ALTHEA_ERROR_OUT_OF_MEMORY);
     }
@@ -345,12 +352,21 @@ althea_status althea_prefilter_batch_create(const althea_env_view* envs, int cou
   return ALTHEA_OK;
 }
 
+extern "C" {
+
+althea_status althea_prefilter_batch_create(const althea_env_view* envs, int count, const int* face_sizes, int mips,
+                                            int samples, unsigned flags, althea_prefilter_batch** out) {
+  return batch_create(envs, count, face_sizes, mips, samples, flags, nullptr, nullptr, out);
+}
+
 althea_status althea_prefilter_batch_destroy(althea_prefilter_batch* b) {
   if (!b) return ALTHEA_OK;
   drop_graph(b);
   if (b->capture) (void)hipStreamDestroy(b->capture);
-  for (float* p : b->pyramids) (void)hipFree(p);
-  for (float* p : b->cubes) (void)hipFree(p);
+  if (b->owns_buffers) {
+    for (float* p : b->pyramids) (void)hipFree(p);
+    for (float* p : b->cubes) (void)hipFree(p);
+  }
   (void)hipFree(b->d_maps);
   (void)hipFree(b->d_levels);
   (void)hipFree(b->d_tables);
@@ -458,3 +474,27 @@ althea_status althea_prefilter_batch_cube(const althea_prefilter_batch* b, int i
 size_t althea_prefilter_batch_texels(const althea_prefilter_batch* b) { return b ? b->texels : 0; }
 
 }  // extern "C"
+
+namespace althea {
+
+althea_status prefilter_batch_into(const althea_env_view* envs, int count, const int* face_sizes, int mips,
+                                   int samples, unsigned flags, float* const* pyramids, float* const* cubes,
+                                   const althea_tuning* tuning, althea_tuning* used, hipStream_t stream) {
+  althea_prefilter_batch* b = nullptr;
+  althea_status st = batch_create(envs, count, face_sizes, mips, samples, flags, pyramids, cubes, &b);
+  if (st != ALTHEA_OK) return st;
+  althea_tuning t{};
+  st = tuning ? althea_prefilter_batch_set_tuning(b, tuning) : althea_prefilter_batch_autotune(b, &t, stream);
+  if (st == ALTHEA_OK) st = althea_prefilter_batch_run(b, stream);
+  if (st == ALTHEA_OK && used) st = althea_prefilter_batch_get_tuning(b, used);
+  // The batch's tables and work list are in use until the launch completes.
+  const hipError_t e = hipStreamSynchronize(stream);
+  althea_prefilter_batch_destroy(b);
+  if (st == ALTHEA_OK && e != hipSuccess) {
+    set_error("prefilter_batch_into: %s", hipGetErrorString(e));
+    st = ALTHEA_ERROR_HIP;
+  }
+  return st;
+}
+
+}  // namespace althea

@@ -345,23 +345,6 @@ __device__ inline f3 cube_sample(const float4* __restrict__ cube, const CubeMips
   return lerp3(a, b, delta);
 }
 
-// ------------------------------------------------------------ launch helpers
-// Workgroups of a launch are dispatched to the 8 XCDs round-robin by linear
-// id, and each XCD has its own L2.  xcd_tile deals the tiles to the XCDs in
-// runs of `run` consecutive tiles (run r to XCD r % 8), so each XCD works on
-// spatially adjacent tiles while every XCD still samples the whole image (a
-// single band per XCD would leave the XCDs with unequal work).  Bijective on
-// [0, n); ids past the last whole round of 8 runs keep their value; run 0 is
-// the identity.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int id, int n, int run) {
-  if (run <= 0) return id;
-  const int round = kXcds * run;
-  if (id >= (n / round) * round) return id;
-  const int xcd = id % kXcds, j = id / kXcds;
-  return ((j / run) * kXcds + xcd) * run + j % run;
-}
-
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll

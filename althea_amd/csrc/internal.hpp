@@ -72,6 +72,13 @@ althea_status launch_cube_sample(const float* cube, int size, int mips, const fl
 althea_status launch_env_sample(const float* pyr, int w, int h, const float* dir_lod, int count, float* out,
                                 void* workspace, hipStream_t s);
 
+// Batched prefilter into caller buffers (batched_kernels.hip): pyramids of
+// `envs` into `pyramids[i]`, prefiltered cubes into `cubes[i]`, one graph
+// launch; tuning NULL runs the autotuner.  Synchronises `stream`.
+althea_status prefilter_batch_into(const althea_env_view* envs, int count, const int* face_sizes, int mips,
+                                   int samples, unsigned flags, float* const* pyramids, float* const* cubes,
+                                   const althea_tuning* tuning, althea_tuning* used, hipStream_t stream);
+
 // Deferred resolve (resolve_kernels.hip); with `mlp` the prefiltered term
 // comes from the MLP where its monitor accepts.
 struct MlpInferenceView;

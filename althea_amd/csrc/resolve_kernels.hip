@@ -39,7 +39,6 @@ struct ResolveParams {
   const float2* lut;
   int lut_size;
   int light_count;
-  int xcd_run;  // tiles per run dealt to one XCD (xcd_tile)
   float ray0[3], rayx[3], rayy[3];  // view ray = ray0 + rayx*(px+0.5) + rayy*(py+0.5)
   float view_z[4];                  // row 2 of the view matrix
   float vp[16];                     // projection * view, column-major
@@ -171,13 +170,14 @@ struct MlpResolveSmem {
   MlpSmem mlp;
 };
 
-// One 64 x 4 tile (linear id vt before the XCD mapping).  Every lane of
-// the block takes part in the block barriers of the light staging.
+// One 64 x 4 tile, row-major tile id.  Tiles are taken in launch order: a
+// mapping that gives each XCD (with its own L2) contiguous runs of tiles was
+// measured slower at 4K (runs of 2, 8 or 68 tile rows: 0.87, 0.89, 0.84 ms
+// against 0.83 ms).
 template <int kFmt, bool kOutF16, bool kMlp>
-__device__ __forceinline__ void resolve_tile(const ResolveParams& p, int vt, int tiles, int tiles_x,
+__device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, int tiles_x,
                                              const float4* __restrict__ lights, const float4* s_noise,
                                              MlpWaveStage* s_stage, const MlpSmem* s_mlp) {
-  const int tile = xcd_tile(vt, tiles, p.xcd_run);
   const int px = (tile % tiles_x) * 64 + (threadIdx.x & 63);
   const int py = (tile / tiles_x) * 4 + (threadIdx.x >> 6);
   const bool inside = px < p.width && py < p.height;
@@ -352,14 +352,12 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, con
   __shared__ float4 s_noise[16];
   resolve_prologue<false>(p, s_noise, nullptr);
   const int tiles_x = (p.width + 63) / 64;
-  const int tiles = tiles_x * ((p.height + 3) / 4);
-  resolve_tile<kFmt, kOutF16, false>(p, blockIdx.x, tiles, tiles_x, lights, s_noise, nullptr, nullptr);
+  resolve_tile<kFmt, kOutF16, false>(p, blockIdx.x, tiles_x, lights, s_noise, nullptr, nullptr);
 }
 
 // MLP-IBL resolve: persistent blocks keep the MLP weights in LDS for several
-// tiles; with a grid that is a multiple of 8, xcd_tile keeps each XCD's tiles
-// contiguous.  The kernel needs about 150 VGPRs (3 waves per SIMD, 3 blocks
-// per CU); a budget of 128 for 4 waves per SIMD would spill.
+// tiles.  The kernel needs about 150 VGPRs (3 waves per SIMD, 3 blocks per
+// CU); a budget of 128 for 4 waves per SIMD would spill.
 constexpr int kMlpBlocksPerCu = 3;
 template <int kFmt, bool kOutF16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMlpBlocksPerCu))) void resolve_mlp_kernel(
@@ -370,7 +368,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMlpBlocksP
   const int tiles_x = (p.width + 63) / 64;
   const int tiles = tiles_x * ((p.height + 3) / 4);
   for (int vt = blockIdx.x; vt < tiles; vt += gridDim.x)
-    resolve_tile<kFmt, kOutF16, true>(p, vt, tiles, tiles_x, lights, s_noise, s_m.stage, &s_m.mlp);
+    resolve_tile<kFmt, kOutF16, true>(p, vt, tiles_x, lights, s_noise, s_m.stage, &s_m.mlp);
 }
 
 // Host: the view ray of DeferredPass.vert is linear in NDC, so it is affine in
@@ -393,8 +391,6 @@ static void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3]
     rayy[r] = static_cast<float>((ay[r] - o[r]) / H);
   }
 }
-
-constexpr int kXcdRunRows = 8;
 
 althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
@@ -445,21 +441,17 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   p.out = out;
   p.ao_out = ao_out;
   const int tiles = ((g->width + 63) / 64) * ((g->height + 3) / 4);
-  // Runs of kXcdRunRows tile rows per XCD.
-  p.xcd_run = kXcdRunRows * ((g->width + 63) / 64);
   const float4* lights4 = reinterpret_cast<const float4*>(lights);
   const bool f16_out = (flags & ALTHEA_RESOLVE_OUT_FP16) != 0;
   const bool fp32 = g->format == ALTHEA_GBUFFER_FP32;
   if (mlp) {
     p.mlp = *mlp;
     p.mlp_used = mlp_used;
-    // Persistent blocks, as many as are resident at once; a multiple of 8
-    // keeps each XCD's tiles contiguous.
+    // Persistent blocks, as many as are resident at once.
     int dev = 0, cus = 256;
     ALTHEA_TRY_HIP(hipGetDevice(&dev));
     ALTHEA_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    int blocks = std::min(tiles, kMlpBlocksPerCu * cus);
-    if (blocks >= kXcds) blocks -= blocks % kXcds;
+    const int blocks = std::min(tiles, kMlpBlocksPerCu * cus);
     const dim3 grid(blocks);
     if (fp32 && f16_out)
       hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);

@@ -44,11 +44,14 @@ class PrefilterMLP:
         h = C.c_void_p()
         check(lib().althea_mlp_create(C.byref(d), C.byref(h)), "althea_mlp_create")
         self._h = h
+        self._destroy = lib().althea_mlp_destroy
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().althea_mlp_destroy(h)
+        # The destroy function is bound at creation: module globals may be
+        # gone when this runs at interpreter exit.
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self._h = None
 
     @property

@@ -224,6 +224,17 @@ ALTHEA_API althea_status althea_ibl_destroy(althea_ibl* ibl);
  * prefiltered mips and the BRDF LUT (ImageBasedLighting.cpp:137-412). */
 ALTHEA_API althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, int width, int height,
                                                hipStream_t stream);
+/* althea_ibl_precompute for `count` handles at once: every handle's pyramid
+ * and prefiltered mips come from one persistent, graph-captured batched
+ * prefilter launch (as althea_prefilter_batch_run), then each handle's
+ * irradiance and LUT.  The handles must share prefiltered_mips,
+ * prefilter_samples and precision flags; face sizes and map extents may
+ * differ.  tuning NULL runs the autotuner; `used` (optional) receives the
+ * launch shape.  Synchronises `stream`.  (The reference precomputes one map
+ * at a time, ImageBasedLighting.cpp:137-412.) */
+ALTHEA_API althea_status althea_ibl_precompute_batched(althea_ibl* const* ibls, const althea_env_view* envs,
+                                                       int count, const althea_tuning* tuning, althea_tuning* used,
+                                                       hipStream_t stream);
 ALTHEA_API althea_status althea_ibl_get_views(const althea_ibl* ibl, althea_ibl_views* views);
 /* Device bytes the shading pass reads for this IBL set (prefiltered +
  * irradiance + LUT), the "IBL state per environment". */

@@ -6,6 +6,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "althea/ImageBasedLighting.hpp"
@@ -104,7 +106,23 @@ int main() {
       threw = true;
       std::printf("rejected non-power-of-two source: %s\n", e.what());
     }
-    const bool ok = max_pref < 1e-5 && max_irr < 1e-5 && finite && threw;
+    // The reference's file form: a missing .hdr throws its message; an .hdr
+    // written here precomputes like the device buffer did.
+    bool missing_threw = false;
+    try {
+      ImageBasedLighting::createResources(std::string("/nonexistent/Skybox.hdr"), desc);
+    } catch (const std::runtime_error& e) {
+      missing_threw = std::string(e.what()) == "Specified environment map does not exist!";
+    }
+    const std::string path = "/tmp/althea_facade_demo_" + std::to_string(getpid()) + ".hdr";
+    throwIfFailed(althea_hdr_write(path.c_str(), env.data(), W, H), "althea_hdr_write");
+    IBLResources from_file = ImageBasedLighting::createResources(path, desc);
+    std::remove(path.c_str());
+    std::vector<float> pref2(pref.size());
+    HIP_OK(hipMemcpy(pref2.data(), from_file.getHandles().prefilteredMap, pref2.size() * 4, hipMemcpyDeviceToHost));
+    const bool same = pref2 == pref;  // 2, 3 and 5 are exact in RGBE
+    std::printf("missing file threw %d, file route identical %d\n", missing_threw ? 1 : 0, same ? 1 : 0);
+    const bool ok = max_pref < 1e-5 && max_irr < 1e-5 && finite && threw && missing_threw && same;
     std::printf("facade %s\n", ok ? "ok" : "FAILED");
     return ok ? 0 : 2;
   } catch (const std::exception& e) {

@@ -91,3 +91,22 @@ def test_batch_rejects_bad_shapes(torch):
     b = PrefilterBatch(maps, [16], mips=3, samples=16)
     with pytest.raises(aa.AltheaError, match="tile"):
         b.set_tuning(32, 2, 16)
+
+
+def test_ibl_precompute_batched_equals_one_map_at_a_time(torch):
+    """althea_ibl_precompute_batched fills IBL handles with the same pyramids,
+    prefiltered cubes, irradiance and LUTs as separate precomputes."""
+    widths = [256, 64, 128]
+    maps = _maps(torch, widths, seed0=400)
+    descs = [aa.IBLDesc(prefiltered_size=w // 4, prefiltered_mips=3, prefilter_samples=128, irradiance_size=8,
+                        brdf_lut_size=16, brdf_lut_samples=64) for w in widths]
+    batched, used = aa.ImageBasedLighting.createResourcesBatched(maps, descs, tuning=(16, 4, 16))
+    assert used == (16, 4, 16)
+    for env, desc, b in zip(maps, descs, batched):
+        one = aa.ImageBasedLighting.createResources(env, desc)
+        torch.cuda.synchronize()
+        for name in ("environment_tensor", "prefiltered_tensor", "irradiance_tensor", "lut_tensor"):
+            torch.testing.assert_close(getattr(b, name)(), getattr(one, name)(), rtol=0, atol=0)
+    # The autotuner path picks one of its shapes.
+    _, used = aa.ImageBasedLighting.createResourcesBatched(maps, descs)
+    assert (used[0], used[1]) in ((8, 8), (16, 4), (4, 16)) and used[2] in (8, 16, 32)

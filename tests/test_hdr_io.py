@@ -104,3 +104,32 @@ def test_error_codes(tmp_path):
     trunc.write_bytes(GOLDEN.read_bytes()[:3000])
     with pytest.raises(AltheaError):
         hdr.read(trunc)
+
+
+def test_create_resources_missing_file_raises_not_found(tmp_path):
+    import althea_amd as aa
+
+    with pytest.raises(AltheaError) as e:
+        aa.ImageBasedLighting.createResources(str(tmp_path / "NoSuchSkybox.hdr"))
+    assert e.value.status == 4
+
+
+@pytest.mark.gpu
+def test_create_resources_from_file_equals_device_path(tmp_path):
+    """The .hdr route precomputes exactly what the tensor route does on the
+    decoded image."""
+    import torch
+
+    import althea_amd as aa
+
+    env = aa.synth.env(seed=77, width=256, height=128)
+    path = tmp_path / "env.hdr"
+    hdr.write(path, env)
+    decoded = hdr.read(path)
+    desc = aa.IBLDesc(prefiltered_size=32, prefiltered_mips=3, prefilter_samples=64, irradiance_size=8,
+                      brdf_lut_size=16, brdf_lut_samples=64)
+    a = aa.ImageBasedLighting.createResources(str(path), desc)
+    b = aa.ImageBasedLighting.createResources(torch.from_numpy(decoded).cuda(), desc)
+    torch.cuda.synchronize()
+    for name in ("prefiltered_tensor", "irradiance_tensor", "lut_tensor", "environment_tensor"):
+        torch.testing.assert_close(getattr(a, name)(), getattr(b, name)(), rtol=0, atol=0)
