@@ -40,22 +40,38 @@ __device__ __forceinline__ bf16x8 pack8(f32x4 lo, f32x4 hi) {
   return r;
 }
 
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
+// Logistic sigmoid with the hardware exp2 and reciprocal (about 2 ulp); the
+// only activation path, shared by training and inference.
+__device__ __forceinline__ float sigmoidf(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
 
-// Input encoding: feature f of (dir, roughness): dir (0..2), sin/cos(2^k pi
-// dir_c) for k = 0..3 (3 + 6k + c sin, 3 + 6k + 3 + c cos), roughness (27),
-// zero padding (28..31).
-__device__ __forceinline__ float encode_feature(int f, float dx, float dy, float dz, float rough) {
-  if (f < 3) return f == 0 ? dx : (f == 1 ? dy : dz);
-  if (f == 27) return rough;
-  if (f > 27) return 0.0f;
-  int idx = f - 3;
-  int k = idx / 6, r = idx - 6 * k;
-  int c = r < 3 ? r : r - 3;
-  float d = c == 0 ? dx : (c == 1 ? dy : dz);
-  float s, co;
-  sincospif(ldexpf(d, k), &s, &co);
-  return r < 3 ? s : co;
+// The input features of one query (28, zero-padded to 32): dir (0..2), then
+// for k = 0..3 sin(2^k pi d_c) at 3 + 6k + c and cos(2^k pi d_c) at
+// 3 + 6k + 3 + c (c = x, y, z), then roughness (27).  sin/cos(pi d) come
+// from sincospif, the higher octaves from the double-angle identities
+// sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a) (about
+// 5e-7 absolute after three doublings, far below the BF16 input rounding).
+__device__ __forceinline__ void encode_all(float dx, float dy, float dz, float rough, float f[32]) {
+  f[0] = dx;
+  f[1] = dy;
+  f[2] = dz;
+  const float d[3] = {dx, dy, dz};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float s, co;
+    sincospif(d[c], &s, &co);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      f[3 + 6 * k + c] = s;
+      f[3 + 6 * k + 3 + c] = co;
+      const float s2 = 2.0f * s * co;
+      co = (co - s) * (co + s);
+      s = s2;
+    }
+  }
+  f[27] = rough;
+  f[28] = f[29] = f[30] = f[31] = 0.0f;
 }
 
 // ------------------------------------------------- LDS-resident inference
@@ -110,11 +126,13 @@ __device__ inline f3 mlp_wave_eval(const MlpSmem& s, MlpWaveStage& st, float dx,
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, n = lane & 15;
   wave_lds_sync();  // the previous evaluation's reads are done
+  float f[32];
+  encode_all(dx, dy, dz, rough, f);
 #pragma unroll
   for (int G = 0; G < 4; ++G) {
     bf16x8 v;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = static_cast<__bf16>(encode_feature(8 * G + j, dx, dy, dz, rough));
+    for (int j = 0; j < 8; ++j) v[j] = static_cast<__bf16>(f[8 * G + j]);
     st.feat[mlp_swz32(lane, G)] = v;
   }
   wave_lds_sync();

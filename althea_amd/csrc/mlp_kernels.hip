@@ -124,9 +124,12 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const FwdParams p) {
     const bool valid = s < p.count;
     const float4 q = valid ? p.dir_rough[s] : make_float4(0.f, 0.f, 1.f, 0.f);
     // Layer 1 input (B operand): this lane's 8 features.
+    float f[32];
+    encode_all(q.x, q.y, q.z, q.w, f);
     bf16x8 x;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = static_cast<__bf16>(encode_feature(8 * g + j, q.x, q.y, q.z, q.w));
+    for (int j = 0; j < 8; ++j)
+      x[j] = static_cast<__bf16>(g == 0 ? f[j] : (g == 1 ? f[8 + j] : (g == 2 ? f[16 + j] : f[24 + j])));
 
     // Hidden layer: D = W . H^T (+ bias), SiLU; `sp` keeps SiLU'(z) for training.
     auto hidden = [&](auto&& wtile, const bf16x8* in, int nk, const float* bias, f32x4* h, f32x4* sp) {
@@ -337,22 +340,44 @@ struct AdamParams {
   float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
 };
 
-__global__ void mlp_adam_kernel(const AdamParams p) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0 && p.loss_out) {
+// kAdamLanes lanes per parameter each sum every kAdamLanes-th partial row,
+// then a fixed xor tree over those lanes (every lane ends with the same
+// total); the last block reduces the loss partials.  Deterministic.
+constexpr int kAdamLanes = 16;
+constexpr int kAdamParamsPerBlock = 256 / kAdamLanes;
+
+__global__ __launch_bounds__(256) void mlp_adam_kernel(const AdamParams p) {
+  if (blockIdx.x == gridDim.x - 1) {
+    if (!p.loss_out) return;
+    __shared__ float s_loss[256];
     float l = 0.0f;
-    for (int b = 0; b < p.bblocks; ++b) l += p.bpartial[static_cast<size_t>(b) * kMlpBiasPartial + kBiasLoss];
-    *p.loss_out = l * p.loss_scale;
+    for (int b = threadIdx.x; b < p.bblocks; b += 256)
+      l += p.bpartial[static_cast<size_t>(b) * kMlpBiasPartial + kBiasLoss];
+    s_loss[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (static_cast<int>(threadIdx.x) < o) s_loss[threadIdx.x] += s_loss[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *p.loss_out = s_loss[0] * p.loss_scale;
+    return;
   }
-  if (t >= kMlpParams) return;
+  const int lane = threadIdx.x & (kAdamLanes - 1);
+  const int t = blockIdx.x * kAdamParamsPerBlock + static_cast<int>(threadIdx.x) / kAdamLanes;
   float gsum = 0.0f;
-  int wi = mlp_param_to_wgrad(t);
-  if (wi >= 0) {
-    for (int b = 0; b < p.wblocks; ++b) gsum += p.wpartial[static_cast<size_t>(b) * kMlpWeights + wi];
-  } else {
-    int bi = mlp_param_to_bias(t);
-    for (int b = 0; b < p.bblocks; ++b) gsum += p.bpartial[static_cast<size_t>(b) * kMlpBiasPartial + bi];
+  if (t < kMlpParams) {
+    const int wi = mlp_param_to_wgrad(t);
+    if (wi >= 0) {
+      for (int b = lane; b < p.wblocks; b += kAdamLanes) gsum += p.wpartial[static_cast<size_t>(b) * kMlpWeights + wi];
+    } else {
+      const int bi = mlp_param_to_bias(t);
+      for (int b = lane; b < p.bblocks; b += kAdamLanes)
+        gsum += p.bpartial[static_cast<size_t>(b) * kMlpBiasPartial + bi];
+    }
   }
+#pragma unroll
+  for (int o = kAdamLanes / 2; o > 0; o >>= 1) gsum += __shfl_xor(gsum, o, kAdamLanes);
+  if (t >= kMlpParams || lane != 0) return;
   if (p.grad_out) p.grad_out[t] = gsum;
   if (!p.update) return;
   float m = p.b1 * p.m[t] + (1.0f - p.b1) * gsum;
@@ -469,7 +494,7 @@ struct MlpState {
   // Error monitor of the last althea_mlp_build_monitor call.
   float* cell_error = nullptr;
   uint8_t* cell_ok = nullptr;
-  int mon_size = 0, mon_mips = 0;
+  int mon_size = 0, mon_mips = 0, mon_accepted = 0;
 };
 
 static constexpr int kFwdBlocks = 256;    // training forward grid
@@ -574,7 +599,8 @@ static althea_status run_train(MlpState* s, const float* dir_rough, const float*
     ap.c1 = static_cast<float>(1.0 - std::pow(static_cast<double>(s->desc.beta1), s->step));
     ap.c2 = static_cast<float>(1.0 - std::pow(static_cast<double>(s->desc.beta2), s->step));
   }
-  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, ap);
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + kAdamParamsPerBlock - 1) / kAdamParamsPerBlock + 1),
+                     dim3(256), 0, stream, ap);
   st = check_launch("mlp_adam_kernel");
   if (st != ALTHEA_OK) return st;
   return update ? launch_pack(s, stream) : ALTHEA_OK;
@@ -598,6 +624,7 @@ althea_status mlp_inference_view(const althea_mlp* m, MlpInferenceView* v) {
   v->cell_ok = m->s.cell_ok;
   v->mon_size = m->s.mon_size;
   v->mon_mips = m->s.mon_mips;
+  v->mon_accepted = m->s.mon_accepted;
   return ALTHEA_OK;
 }
 }  // namespace althea
@@ -775,14 +802,15 @@ althea_status althea_mlp_build_monitor(althea_mlp* m, const float* d_prefiltered
   if (d_cell_error)
     ALTHEA_TRY_HIP(hipMemcpyAsync(d_cell_error, s->cell_error, cells * sizeof(float), hipMemcpyDeviceToDevice,
                                   stream));
-  if (accepted) {
-    std::vector<uint8_t> ok(cells);
-    ALTHEA_TRY_HIP(hipMemcpyAsync(ok.data(), s->cell_ok, cells, hipMemcpyDeviceToHost, stream));
-    ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
-    int n = 0;
-    for (uint8_t v : ok) n += v;
-    *accepted = n;
-  }
+  // The accepted count is kept so the resolve can skip the MLP variant when
+  // no cell is accepted (built once per environment; the copy synchronises).
+  std::vector<uint8_t> ok(cells);
+  ALTHEA_TRY_HIP(hipMemcpyAsync(ok.data(), s->cell_ok, cells, hipMemcpyDeviceToHost, stream));
+  ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+  int n = 0;
+  for (uint8_t v : ok) n += v;
+  s->mon_accepted = n;
+  if (accepted) *accepted = n;
   return ALTHEA_OK;
 }
 

@@ -15,6 +15,7 @@ struct MlpInferenceView {
   MlpDeviceWeights w;
   const uint8_t* cell_ok;  // [mip][face][cy][cx], ALTHEA_MLP_MONITOR_CELLS per face
   int mon_size, mon_mips;
+  int mon_accepted;  // accepted cells of that build
 };
 althea_status mlp_inference_view(const althea_mlp* m, MlpInferenceView* v);
 

@@ -104,6 +104,8 @@ __device__ __forceinline__ float view_depth(const ResolveParams& p, f3 q) {
   return -fmaf(p.view_z[0], q.x, fmaf(p.view_z[1], q.y, fmaf(p.view_z[2], q.z, p.view_z[3])));
 }
 
+constexpr int kSsaoBatch = 4;  // position gathers issued together
+
 template <int kFmt>
 __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int py, f3 P, f3 N) {
   const float4 nz = s_noise[(px & 3) + 4 * (py & 3)];
@@ -127,23 +129,40 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
   clip3(Nr, 0.0f, nx, ny, nw);
   auto vz = [&](f3 v) { return -fmaf(p.view_z[0], v.x, fmaf(p.view_z[1], v.y, p.view_z[2] * v.z)); };
   const float dT = vz(Tr), dB = vz(Br), dN = vz(Nr);
+  // Taps in groups of kSsaoBatch: their position loads are issued together
+  // (an invalid tap loads texel 0 and is ignored), then evaluated in tap
+  // order, so the per-tap arithmetic and the sum order are those of a
+  // one-tap-at-a-time loop.
   float occ = 0.0f;
-  for (int k = 0; k < p.ssao_samples; ++k) {
-    const float kx = p.kernel[k][0], ky = p.kernel[k][1], kz = p.kernel[k][2];
-    const float sd = fmaf(dT, kx, fmaf(dB, ky, fmaf(dN, kz, frag_depth)));
-    const float cx = fmaf(tx, kx, fmaf(bx, ky, fmaf(nx, kz, px0)));
-    const float cy = fmaf(ty, kx, fmaf(by, ky, fmaf(ny, kz, py0)));
-    const float cw = fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
-    if (!(cw > 0.0f)) continue;
-    const float inv_w = rcp_fast(cw);
-    const float u = fmaf(0.5f * cx, inv_w, 0.5f), v = fmaf(0.5f * cy, inv_w, 0.5f);
-    if (!(u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f)) continue;
-    const int qx = min(static_cast<int>(u * fw), p.width - 1);
-    const int qy = min(static_cast<int>(v * fh), p.height - 1);
-    const float4 q = load_gbuf<kFmt>(p.position, static_cast<size_t>(qy) * p.width + qx);
-    if (q.w == 0.0f) continue;
-    const float qd = view_depth(p, mk(q.x, q.y, q.z));
-    if (qd <= sd - p.ssao_bias) occ += smoothstep01(p.ssao_radius * rcp_fast(fabsf(frag_depth - qd)));
+  const int n = p.ssao_samples;
+  for (int k0 = 0; k0 < n; k0 += kSsaoBatch) {
+    uint32_t idx[kSsaoBatch];
+    bool ok[kSsaoBatch];
+    float sd[kSsaoBatch];
+#pragma unroll
+    for (int j = 0; j < kSsaoBatch; ++j) {
+      const int k = min(k0 + j, n - 1);
+      const float kx = p.kernel[k][0], ky = p.kernel[k][1], kz = p.kernel[k][2];
+      sd[j] = fmaf(dT, kx, fmaf(dB, ky, fmaf(dN, kz, frag_depth)));
+      const float cx = fmaf(tx, kx, fmaf(bx, ky, fmaf(nx, kz, px0)));
+      const float cy = fmaf(ty, kx, fmaf(by, ky, fmaf(ny, kz, py0)));
+      const float cw = fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
+      const float inv_w = rcp_fast(cw);
+      const float u = fmaf(0.5f * cx, inv_w, 0.5f), v = fmaf(0.5f * cy, inv_w, 0.5f);
+      ok[j] = k0 + j < n && cw > 0.0f && u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f;
+      const int qx = min(static_cast<int>(u * fw), p.width - 1);
+      const int qy = min(static_cast<int>(v * fh), p.height - 1);
+      idx[j] = ok[j] ? static_cast<uint32_t>(qy) * static_cast<uint32_t>(p.width) + static_cast<uint32_t>(qx) : 0u;
+    }
+    float4 q[kSsaoBatch];
+#pragma unroll
+    for (int j = 0; j < kSsaoBatch; ++j) q[j] = load_gbuf<kFmt>(p.position, idx[j]);
+#pragma unroll
+    for (int j = 0; j < kSsaoBatch; ++j) {
+      if (!ok[j] || q[j].w == 0.0f) continue;
+      const float qd = view_depth(p, mk(q[j].x, q[j].y, q[j].z));
+      if (qd <= sd[j] - p.ssao_bias) occ += smoothstep01(p.ssao_radius * rcp_fast(fabsf(frag_depth - qd)));
+    }
   }
   return fmaf(-occ, p.ssao_inv_samples, 1.0f);
 }
@@ -371,6 +390,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMlpBlocksP
     resolve_tile<kFmt, kOutF16, true>(p, vt, tiles_x, lights, s_noise, s_m.stage, &s_m.mlp);
 }
 
+__global__ void fill_u8_kernel(uint8_t* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // Host: the view ray of DeferredPass.vert is linear in NDC, so it is affine in
 // the pixel centre; fold mat3(inverseView) * inverseProjection into it.
 static void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]) {
@@ -444,6 +468,17 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   const float4* lights4 = reinterpret_cast<const float4*>(lights);
   const bool f16_out = (flags & ALTHEA_RESOLVE_OUT_FP16) != 0;
   const bool fp32 = g->format == ALTHEA_GBUFFER_FP32;
+  if (mlp && mlp->mon_accepted == 0) {
+    // No cell accepted: the MLP variant would shade exactly like the plain
+    // kernel at lower occupancy.  Run the plain kernel and report no MLP use.
+    if (mlp_used) {
+      hipLaunchKernelGGL(fill_u8_kernel, dim3((g->width * g->height + 255) / 256), dim3(256), 0, s, mlp_used,
+                         g->width * g->height);
+      althea_status st = check_launch("fill_u8_kernel");
+      if (st != ALTHEA_OK) return st;
+    }
+    mlp = nullptr;
+  }
   if (mlp) {
     p.mlp = *mlp;
     p.mlp_used = mlp_used;

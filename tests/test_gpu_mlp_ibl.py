@@ -118,17 +118,29 @@ def _scene_run(torch, scene, ibl, net=None, used=False):
             mask.cpu().numpy().reshape(scene.height, scene.width))
 
 
-def test_resolve_mlp_rejecting_every_cell_is_the_plain_resolve(torch):
+def test_resolve_mlp_falls_back_to_the_plain_resolve(torch):
+    """Pixels the MLP does not serve are bit-identical to the plain resolve,
+    with no cell accepted (the plain kernel runs) and with some accepted (the
+    MLP kernel's own fallback path)."""
     env = _smooth_env()
     ibl = _ibl(torch, env, size=64, mips=5)
+    scene = aa.synth.scene(seed=SEED0 + 5, width=333, height=187, lights=8)
+    plain, _ = _scene_run(torch, scene, ibl)
     net = M.PrefilterMLP(seed=11)
     _, accepted = net.build_monitor(ibl.prefiltered_tensor(), 64, 5, threshold=-1.0)
     assert accepted == 0
-    scene = aa.synth.scene(seed=SEED0 + 5, width=333, height=187, lights=8)
-    plain, _ = _scene_run(torch, scene, ibl)
     fused, mask = _scene_run(torch, scene, ibl, net)
     assert mask.sum() == 0
     np.testing.assert_array_equal(fused, plain)
+    trained, _, _ = _trained(torch, env, steps_epochs=2, count=1 << 15)
+    err, _ = trained.build_monitor(ibl.prefiltered_tensor(), 64, 5, threshold=1e30)
+    thr = float(np.median(err.cpu().numpy()))
+    _, accepted = trained.build_monitor(ibl.prefiltered_tensor(), 64, 5, threshold=thr)
+    assert 0 < accepted < 5 * 6 * 64
+    fused, mask = _scene_run(torch, scene, ibl, trained)
+    off = mask == 0
+    assert 0 < mask.sum() < mask.size
+    np.testing.assert_array_equal(fused[off], plain[off])
 
 
 def test_resolve_mlp_accepting_every_cell_matches_oracle(torch):
