@@ -150,9 +150,11 @@ __device__ __forceinline__ f3 level_bilinear(const char* __restrict__ pyr_bytes,
   const float ax = x - fx, ay = y - fy;
   const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
   const uint32_t x0 = static_cast<uint32_t>(ix) & c.wmask, x1 = static_cast<uint32_t>(ix + 1) & c.wmask;
-  const uint32_t r0 = ((static_cast<uint32_t>(iy) & c.hmask) << c.wshift) + c.off;
-  const uint32_t r1 = ((static_cast<uint32_t>(iy + 1) & c.hmask) << c.wshift) + c.off;
-  auto ld = [&](uint32_t t) { return xyz(*reinterpret_cast<const float4*>(pyr_bytes + (t << 4))); };
+  const uint32_t r0 = (static_cast<uint32_t>(iy) & c.hmask) << c.wshift;
+  const uint32_t r1 = (static_cast<uint32_t>(iy + 1) & c.hmask) << c.wshift;
+  // The level offset goes into the (wave-uniform, scalar) base address.
+  const char* lvl = pyr_bytes + (static_cast<size_t>(c.off) << 4);
+  auto ld = [&](uint32_t t) { return xyz(*reinterpret_cast<const float4*>(lvl + (t << 4))); };
   const f3 t00 = ld(r0 + x0), t10 = ld(r0 + x1), t01 = ld(r1 + x0), t11 = ld(r1 + x1);
   const f3 top = lerp3(t00, t10, ax), bot = lerp3(t01, t11, ax);
   return lerp3(top, bot, ay);

@@ -32,7 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters on a
 # 1 GiB copy (bench.py --only copy); commands and figures in
 # profiles/resolve_kernel_4k_fp32_64lights_ssao16.md.
-RESOLVE_TRAFFIC_BYTES = {"fp32": 3.391e9}
+RESOLVE_TRAFFIC_BYTES = {"fp32": 3.362e9}
 
 
 def parse():
@@ -174,6 +174,18 @@ def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
                          "loss_first": hist[0], "loss_last": hist[-1]}}
 
 
+def fp16_resolve(torch, aa, scene, ibl, lights, steps: int) -> dict:
+    """The same frame shaded from an FP16 G-buffer (32 B/px)."""
+    g16 = aa.GBufferResources(*scene.gbuffer_tensors("cuda", fp16=True))
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    fn = lambda: aa.DeferredRendering.resolve(g16, ibl, lights, scene.globals, scene.ssao, out=out)
+    fn()
+    ms = time_events(torch, fn, steps)
+    nbytes = algorithmic_bytes(ibl, True)
+    return {"ms": ms, "mpix_per_s": WIDTH * HEIGHT / (ms * 1e-3) / 1e6, "algorithmic_bytes": nbytes,
+            "achieved_gbs": nbytes / (ms * 1e-3) / 1e9}
+
+
 def cpu_baseline(aa, scene, ibl, env) -> dict:
     """The FP64 oracle (a from-source restatement of the reference shaders) on
     the host cores: the whole 3840x2160 frame, 64 lights, SSAO."""
@@ -284,6 +296,7 @@ def main() -> int:
     }
     if not args.no_secondary and args.only is None:
         line["secondary"] = secondary(torch, aa)
+        line["secondary"]["resolve_fp16_gbuffer"] = fp16_resolve(torch, aa, scene, ibl, lights, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)
