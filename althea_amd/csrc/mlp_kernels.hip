@@ -409,12 +409,13 @@ __global__ void mfma_probe_kernel(const float* __restrict__ a, const float* __re
 // ------------------------------------------------------------ error monitor
 // One block per cell (mip m, face, 8 x 8 grid over the face).  Each texel
 // centre of the cell is queried through the MLP (unit direction of the texel
-// centre, roughness m / (mips - 1)) and compared with the cube texel, which
-// is the judge: e = mean over rgb of |mlp - cube| / max(|cube|, floor).  The
+// centre, roughness m / (mips - 1)) and compared with the cube texel, the
+// reference it must match: e = mean over rgb of |mlp - cube| / max(|cube|,
+// floor).  The
 // cell error is the mean of e over its texels; the cell is accepted when
 // that is at most the threshold.  Texel x of a face of size S lies in cell
 // floor((x + 0.5) * 8 / S), the cell a direction through the texel centre
-// selects; a cell with no texel centre (S < 8) is judged at the texel that
+// selects; a cell with no texel centre (S < 8) is measured at the texel that
 // contains the cell centre.
 struct MonitorParams {
   MlpDeviceWeights w;

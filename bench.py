@@ -240,7 +240,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the resolve and precompute are single-GPU kernels here "
-              "(see DESIGN.md for how the path would shard); use --gpus 1", file=sys.stderr)
+              "(the resolve would shard by screen bands, the precompute by map, face and tile); use --gpus 1", file=sys.stderr)
         return 2
     import torch
 

@@ -369,7 +369,7 @@ ALTHEA_API althea_status althea_mlp_infer(const althea_mlp* mlp, const float* d_
 /* Bytes of inference state (BF16 weights + FP32 biases). */
 ALTHEA_API size_t althea_mlp_state_bytes(const althea_mlp* mlp);
 
-/* Error monitor.  The prefiltered cube is the judge: the MLP is evaluated at
+/* Error monitor.  The prefiltered cube is the reference: the MLP is evaluated at
  * every texel centre of every mip m of d_prefiltered (size, mips; roughness
  * m/(mips-1), the roughness the cube was prefiltered at) and per texel
  *   e = mean over rgb of |mlp - cube| / max(|cube|, ALTHEA_MLP_MONITOR_FLOOR).

@@ -175,7 +175,7 @@ def test_resolve_mlp_accepting_every_cell_matches_oracle(torch):
 def test_mlp_gain_over_cube_of_more_bytes(torch):
     """Rough lobes (roughness 0.5..1): the trained MLP (BF16 weights, 21 KB)
     against a 16^2 x6 cube with 5 mips (RGBA32F, 32.7 KB) prefiltered from
-    the same source, both judged against the 1024-sample prefilter at the
+    the same source, both measured against the 1024-sample prefilter at the
     exact query."""
     env = _smooth_env()
     net, pyr, hist = _trained(torch, env, steps_epochs=200, count=1 << 20, rmin=0.5, batch=65536, samples=1024,

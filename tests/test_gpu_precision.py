@@ -8,7 +8,7 @@ lobes with peaks in [1e3, 1e4], angular sigma 2 degrees).
 - Irradiance: 32^2 cube, 45 000 samples per texel.
 - BRDF LUT: 128^2, 1024 samples.
 - Shading: the 3840x2160 scene with 64 lights and SSAO, from an FP32 and
-  from an FP16 G-buffer; both are judged against the oracle shading the FP32
+  from an FP16 G-buffer; both are measured against the oracle shading the FP32
   G-buffer.
 
 Relative error is |x - ref| / max(|ref|, 1e-2), the max over rgb per output.
