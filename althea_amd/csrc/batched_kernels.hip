@@ -159,7 +159,9 @@ std::vector<int4> build_work(const althea_prefilter_batch* b, int tw, int th, si
   std::vector<double> mip_cost(b->mips);
   for (int m = 0; m < b->mips; ++m) {
     const double r = b->mips > 1 ? static_cast<double>(m) / (b->mips - 1) : 0.0;
-    mip_cost[m] = kept_samples(b->samples, r) * (m == 0 ? 1.0 : 1.45);  // trilinear above roughness 0
+    // Roughness 0 is one table entry (coinciding samples); rougher mips blend
+    // two source levels for most samples.
+    mip_cost[m] = r == 0.0 ? 1.0 : kept_samples(b->samples, r) * 1.45;
   }
   struct Item {
     double cost;

@@ -160,6 +160,7 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
         entry.delta = lodf - fl;
         entry.l0 = static_cast<int>(fl);
         entry.l1 = min(entry.l0 + 1, p.env_levels - 1);
+        entry.weight = static_cast<float>(lz);
         my_sum += lz;
         w32 = static_cast<float>(lz);
       }
@@ -196,6 +197,14 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
     counts[m] = s_base;
     wsum[m] = s_sum[0];
     wsum32[m] = s_w32;
+    // Roughness 0 (alpha = 0): every half vector is N, so every kept sample
+    // is L = N at LOD 0 with NdotL = 1 and the entries are bit-identical.  One
+    // entry carrying their summed weight gives the same weighted mean (v * n /
+    // n, exact for n <= 2^24) with one lookup instead of n.
+    if (rough == 0.0 && s_base > 1) {
+      tab[0].weight = static_cast<float>(s_base);
+      counts[m] = 1;
+    }
   }
 }
 

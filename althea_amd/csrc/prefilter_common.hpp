@@ -7,11 +7,14 @@
 namespace althea {
 
 // Sample-table entry (one per Hammersley sample with NdotL > 0, in sample
-// order): L in the texel frame (T, B, N) and the source LOD split into
-// (l0, l1, delta).  32 bytes, one scalar load.
+// order): L in the texel frame (T, B, N), the source LOD split into
+// (l0, l1, delta) and the sample's weight in the NdotL-weighted mean (NdotL =
+// lz, or the summed weight of coinciding samples).  32 bytes, one scalar load.
 struct GgxEntry {
   float lx, ly, lz, delta;
-  int l0, l1, pad0, pad1;
+  int l0, l1;
+  float weight;
+  int pad;
 };
 static_assert(sizeof(GgxEntry) == 32, "one scalar load per entry");
 
@@ -35,9 +38,9 @@ __device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, 
     equirect_angles(L, yaw, pitch);
     f3 v = level_bilinear(pyr, level(en.l0), yaw, pitch);
     if (en.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, level(en.l1), yaw, pitch), en.delta);
-    sr = fmaf(v.x, en.lz, sr);
-    sg = fmaf(v.y, en.lz, sg);
-    sb = fmaf(v.z, en.lz, sb);
+    sr = fmaf(v.x, en.weight, sr);
+    sg = fmaf(v.y, en.weight, sg);
+    sb = fmaf(v.z, en.weight, sb);
     if (kF64Sums) {
       if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
         ar += sr;
@@ -46,7 +49,7 @@ __device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, 
         sr = sg = sb = 0.0f;
       }
     } else {
-      sw += en.lz;
+      sw += en.weight;
     }
   }
   if (kF64Sums) {
