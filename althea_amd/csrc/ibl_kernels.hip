@@ -420,8 +420,7 @@ struct IrradianceParams {
   int phi_samples;
   int l0, l1;
   float delta;
-  int rows_per_chunk;  // theta rows per grid chunk
-  double scale;        // pi / theta_samples / phi_samples
+  double scale;  // pi / theta_samples / phi_samples
 };
 
 __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float2* __restrict__ theta_tab,
@@ -439,35 +438,27 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
   }
 }
 
-// One wave per 8 x 8 texel tile and theta row: every lane of a wave looks up
-// the same (theta, phi) in its own texel frame, so the 64 lookups of a sample
-// land on neighbouring source texels.  Grid (tiles, chunks): chunk c owns a
-// contiguous run of theta rows, the block's 4 waves take every 4th row of it.
-// Per row the phi sum is FP32; rows are summed in FP64 (kF64Sums) or FP32;
-// the block writes one partial per texel and chunk, which
-// irradiance_reduce_kernel sums in chunk order.
-constexpr int kIrrTile = 8;
-
+// One wave per output texel, lane = theta row (rows lane, lane + 64, ...):
+// for a fixed phi the 64 lanes look up 64 consecutive directions on one ring
+// around the texel normal, so a wave's lookups are neighbours.  Each lane
+// sums its row over phi in FP32 in phi order; the rows are combined in FP64
+// (kF64Sums) or FP32 by a fixed xor tree across the wave.
 template <bool kF64Sums>
-__global__ __launch_bounds__(256) void irradiance_partial_kernel(const char* __restrict__ pyr,
-                                                                 const float2* __restrict__ theta_tab,
-                                                                 const float4* __restrict__ phi_tab,
-                                                                 const IrradianceParams p,
-                                                                 double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ pyr,
+                                                              const float2* __restrict__ theta_tab,
+                                                              const float4* __restrict__ phi_tab,
+                                                              const IrradianceParams p, float4* __restrict__ cube) {
   const int S = p.size;
-  const int tpf = (S + kIrrTile - 1) / kIrrTile;
-  const int face = blockIdx.x / (tpf * tpf);
-  const int t = blockIdx.x - face * tpf * tpf;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x = (t % tpf) * kIrrTile + (lane & 7);
-  const int y = (t / tpf) * kIrrTile + (lane >> 3);
-  const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
-  const int lo = blockIdx.y * p.rows_per_chunk;
-  const int hi = min(p.theta_samples, lo + p.rows_per_chunk);
-
+  const int texel = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
+  if (texel >= 6 * S * S) return;  // whole waves; no block barrier follows
+  const int lane = threadIdx.x & 63;
+  const int face = texel / (S * S);
+  const int rem = texel - face * S * S;
+  const int y = rem / S, x = rem - y * S;
+  const Frame fr = local_to_world(cube_texel_raw_dir(face, x, y, S));
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float fr32 = 0.0f, fg32 = 0.0f, fb32 = 0.0f;
-  for (int i = lo + wave; i < hi; i += 4) {
+  for (int i = lane; i < p.theta_samples; i += 64) {
     const float2 th = theta_tab[i];
     const f3 a = fr.t * th.x + fr.b * th.y;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f;
@@ -493,62 +484,26 @@ __global__ __launch_bounds__(256) void irradiance_partial_kernel(const char* __r
       fb32 += sb;
     }
   }
-  __shared__ double s_red[4][64][3];
-  s_red[wave][lane][0] = kF64Sums ? ar : static_cast<double>(fr32);
-  s_red[wave][lane][1] = kF64Sums ? ag : static_cast<double>(fg32);
-  s_red[wave][lane][2] = kF64Sums ? ab : static_cast<double>(fb32);
-  __syncthreads();
-  if (wave != 0 || x >= S || y >= S) return;
-  double* out = partial + (static_cast<size_t>(blockIdx.y) * 6 * S * S + (static_cast<size_t>(face) * S + y) * S + x) * 3;
+  if (kF64Sums) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (kF64Sums) {
-      out[c] = ((s_red[0][lane][c] + s_red[1][lane][c]) + s_red[2][lane][c]) + s_red[3][lane][c];
-    } else {
-      float v = static_cast<float>(s_red[0][lane][c]);
-      v += static_cast<float>(s_red[1][lane][c]);
-      v += static_cast<float>(s_red[2][lane][c]);
-      v += static_cast<float>(s_red[3][lane][c]);
-      out[c] = v;
+    for (int o = 32; o > 0; o >>= 1) {
+      ar += __shfl_xor(ar, o, 64);
+      ag += __shfl_xor(ag, o, 64);
+      ab += __shfl_xor(ab, o, 64);
     }
-  }
-}
-
-// Sum of the chunk partials of each texel in chunk order, times pi / (theta phi).
-template <bool kF64Sums>
-__global__ void irradiance_reduce_kernel(const double* __restrict__ partial, int texels, int chunks, double scale,
-                                         float4* __restrict__ cube) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= texels) return;
-  double r = 0.0, g = 0.0, b = 0.0;
-  float rf = 0.0f, gf = 0.0f, bf = 0.0f;
-  for (int c = 0; c < chunks; ++c) {
-    const double* q = partial + (static_cast<size_t>(c) * texels + i) * 3;
-    if (kF64Sums) {
-      r += q[0];
-      g += q[1];
-      b += q[2];
-    } else {
-      rf += static_cast<float>(q[0]);
-      gf += static_cast<float>(q[1]);
-      bf += static_cast<float>(q[2]);
+    if (lane == 0)
+      cube[texel] = make_float4(static_cast<float>(ar * p.scale), static_cast<float>(ag * p.scale),
+                                static_cast<float>(ab * p.scale), 1.0f);
+  } else {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      fr32 += __shfl_xor(fr32, o, 64);
+      fg32 += __shfl_xor(fg32, o, 64);
+      fb32 += __shfl_xor(fb32, o, 64);
     }
+    const float sc = static_cast<float>(p.scale);
+    if (lane == 0) cube[texel] = make_float4(fr32 * sc, fg32 * sc, fb32 * sc, 1.0f);
   }
-  if (kF64Sums)
-    cube[i] = make_float4(static_cast<float>(r * scale), static_cast<float>(g * scale), static_cast<float>(b * scale),
-                          1.0f);
-  else
-    cube[i] = make_float4(rf * static_cast<float>(scale), gf * static_cast<float>(scale), bf * static_cast<float>(scale),
-                          1.0f);
-}
-
-// Chunks of theta rows: enough blocks for about 8192 waves, at least 4 rows
-// (one per wave) per chunk.
-static int irradiance_chunks(int size, int theta) {
-  const int tpf = (size + kIrrTile - 1) / kIrrTile;
-  const int tiles = 6 * tpf * tpf;
-  const int want = (8192 / 4 + tiles - 1) / tiles;
-  return std::max(1, std::min(want, theta / 4));
 }
 
 static int irradiance_phi_samples(int w, int h, int theta) {
@@ -557,10 +512,9 @@ static int irradiance_phi_samples(int w, int h, int theta) {
   return static_cast<int>(static_cast<float>(h) * static_cast<float>(theta) / static_cast<float>(w));
 }
 
-size_t irradiance_workspace_bytes(int w, int h, int size, int theta) {
+size_t irradiance_workspace_bytes(int w, int h, int, int theta) {
   int phi = irradiance_phi_samples(w, h, theta);
-  return align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1)) +
-         align256(sizeof(double) * 3 * 6 * static_cast<size_t>(size) * size * irradiance_chunks(size, theta));
+  return align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
 }
 
 althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
@@ -592,31 +546,14 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.lv0 = make_level(g.w[p.l0], g.h[p.l0], g.off[p.l0]);
   p.lv1 = make_level(g.w[p.l1], g.h[p.l1], g.off[p.l1]);
   p.scale = M_PI / theta_samples / phi;
-  const int chunks = irradiance_chunks(size, theta_samples);
-  p.rows_per_chunk = (theta_samples + chunks - 1) / chunks;
-  const int used_chunks = (theta_samples + p.rows_per_chunk - 1) / p.rows_per_chunk;
-  double* partial = reinterpret_cast<double*>(ws + align256(sizeof(float2) * theta_samples) +
-                                              align256(sizeof(float4) * std::max(phi, 1)));
-  const int tpf = (size + kIrrTile - 1) / kIrrTile;
-  const dim3 grid(6 * tpf * tpf, used_chunks);
+  const dim3 grid((6 * size * size + 3) / 4);
   const char* pyr4 = reinterpret_cast<const char*>(pyr);
   float4* cube4 = reinterpret_cast<float4*>(cube);
-  const int texels = 6 * size * size;
-  const bool f64 = (flags & ALTHEA_PRECOMPUTE_FP32_SUMS) == 0;
-  if (f64)
-    hipLaunchKernelGGL(irradiance_partial_kernel<true>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, partial);
+  if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
+    hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
   else
-    hipLaunchKernelGGL(irradiance_partial_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, partial);
-  st = check_launch("irradiance_partial_kernel");
-  if (st != ALTHEA_OK) return st;
-  const dim3 rgrid((texels + 255) / 256);
-  if (f64)
-    hipLaunchKernelGGL(irradiance_reduce_kernel<true>, rgrid, dim3(256), 0, s, partial, texels, used_chunks, p.scale,
-                       cube4);
-  else
-    hipLaunchKernelGGL(irradiance_reduce_kernel<false>, rgrid, dim3(256), 0, s, partial, texels, used_chunks,
-                       p.scale, cube4);
-  return check_launch("irradiance_reduce_kernel");
+    hipLaunchKernelGGL(irradiance_cube_kernel<true>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+  return check_launch("irradiance_cube_kernel");
 }
 
 // --------------------------------------------------------------- BRDF LUT
