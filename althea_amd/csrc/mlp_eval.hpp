@@ -118,11 +118,14 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Log-radiance of this lane's query (d, rough), evaluated for all 64 queries
-// of the wave as four 16-sample MFMA tiles.  Every lane of the wave must be
+// Log-radiance of this lane's query (d, rough), evaluated for the wave as
+// four 16-sample MFMA tiles (lanes 16t .. 16t+15); a tile none of whose lanes
+// sets `want` is skipped and its lanes get 0.  Every lane of the wave must be
 // active.  Arithmetic identical to the streaming inference kernel: same
 // features, same MFMA order, same bias and SiLU.
-__device__ inline f3 mlp_wave_eval(const MlpSmem& s, MlpWaveStage& st, float dx, float dy, float dz, float rough) {
+__device__ inline f3 mlp_wave_eval(const MlpSmem& s, MlpWaveStage& st, float dx, float dy, float dz, float rough,
+                                   bool want = true) {
+  const uint64_t wanted = __builtin_amdgcn_ballot_w64(want);
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, n = lane & 15;
   wave_lds_sync();  // the previous evaluation's reads are done
@@ -139,6 +142,7 @@ __device__ inline f3 mlp_wave_eval(const MlpSmem& s, MlpWaveStage& st, float dx,
   f3 out = mk(0.0f, 0.0f, 0.0f);
 #pragma unroll 1
   for (int t = 0; t < 4; ++t) {
+    if (((wanted >> (16 * t)) & 0xFFFFull) == 0) continue;  // wave-uniform
     // Reload the weights per tile: hoisting the 88 loop-invariant operand
     // reads out of this loop would hold 352 VGPRs.
     asm volatile("" ::: "memory");

@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void mlp_monitor_kernel(const MonitorParams p)
     const bool valid = i < area;
     const int x = x0 + (valid ? i % w : 0), y = y0 + (valid ? i / w : 0);
     const f3 d = normalize_fast(cube_texel_raw_dir(face, x, y, S));
-    const f3 ly = mlp_wave_eval(s, st[wave], d.x, d.y, d.z, rough);
+    const f3 ly = mlp_wave_eval(s, st[wave], d.x, d.y, d.z, rough, valid);
     if (valid) {
       const float4 c = mip[y * S + x];
       const float e = fabsf(expf(ly.x) - c.x) / fmaxf(fabsf(c.x), p.floor) +

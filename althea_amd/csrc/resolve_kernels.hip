@@ -127,6 +127,11 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
   clip3(Tr, 0.0f, tx, ty, tw);
   clip3(Br, 0.0f, bx, by, bw);
   clip3(Nr, 0.0f, nx, ny, nw);
+  // Screen position in pixels, (0.5 x / w + 0.5) * W: fold W / 2 into the
+  // clip x and y coefficients so each tap needs one FMA per axis.
+  const float hw = 0.5f * fw, hh = 0.5f * fh;
+  px0 *= hw, tx *= hw, bx *= hw, nx *= hw;
+  py0 *= hh, ty *= hh, by *= hh, ny *= hh;
   auto vz = [&](f3 v) { return -fmaf(p.view_z[0], v.x, fmaf(p.view_z[1], v.y, p.view_z[2] * v.z)); };
   const float dT = vz(Tr), dB = vz(Br), dN = vz(Nr);
   // Taps in groups of kSsaoBatch: their position loads are issued together
@@ -148,10 +153,10 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
       const float cy = fmaf(ty, kx, fmaf(by, ky, fmaf(ny, kz, py0)));
       const float cw = fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
       const float inv_w = rcp_fast(cw);
-      const float u = fmaf(0.5f * cx, inv_w, 0.5f), v = fmaf(0.5f * cy, inv_w, 0.5f);
-      ok[j] = k0 + j < n && cw > 0.0f && u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f;
-      const int qx = min(static_cast<int>(u * fw), p.width - 1);
-      const int qy = min(static_cast<int>(v * fh), p.height - 1);
+      const float sx = fmaf(cx, inv_w, hw), sy = fmaf(cy, inv_w, hh);  // pixels
+      ok[j] = k0 + j < n && cw > 0.0f && sx >= 0.0f && sx < fw && sy >= 0.0f && sy < fh;
+      const int qx = min(static_cast<int>(sx), p.width - 1);
+      const int qy = min(static_cast<int>(sy), p.height - 1);
       idx[j] = ok[j] ? static_cast<uint32_t>(qy) * static_cast<uint32_t>(p.width) + static_cast<uint32_t>(qx) : 0u;
     }
     float4 q[kSsaoBatch];
@@ -252,7 +257,7 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     from_mlp = shade && p.mlp.cell_ok[m0 * per_mip + cell] != 0 &&
                (d == fl || p.mlp.cell_ok[m1 * per_mip + cell] != 0);
     if (__any(from_mlp)) {
-      const f3 ly = mlp_wave_eval(*s_mlp, s_stage[threadIdx.x >> 6], R.x, R.y, R.z, roughness);
+      const f3 ly = mlp_wave_eval(*s_mlp, s_stage[threadIdx.x >> 6], R.x, R.y, R.z, roughness, from_mlp);
       refl = mk(expf(ly.x), expf(ly.y), expf(ly.z));
     }
     if (inside && p.mlp_used) p.mlp_used[idx] = from_mlp ? 1 : 0;
