@@ -29,7 +29,7 @@ from . import synth
 from ._lib import AltheaError, check, lib
 
 __all__ = ["AltheaError", "IBLDesc", "IBLResources", "ImageBasedLighting", "GBufferResources",
-           "DeferredRendering", "synth", "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
+           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "synth", "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
            "brdf_lut", "cube_sample", "env_sample", "prefilter_points", "env_levels", "cube_texels"]
 
 
@@ -304,13 +304,66 @@ class GBufferResources:
                                  self.format)
 
 
+class ReflectionBuffer:
+    """The frame-sized reflection buffer of Src/ReflectionBuffer.cpp:20-94:
+    `mips` levels (5 in the reference) stored back to back in one device
+    tensor of (texels, 4), FP16 (the reference's R16G16B16A16_SFLOAT) or
+    FP32."""
+
+    def __init__(self, width: int, height: int, mips: int = 5, fp16: bool = True):
+        torch = _torch()
+        if not 1 <= mips <= _lib.REFLECTION_MAX_MIPS:
+            raise ValueError(f"mips {mips} outside 1..{_lib.REFLECTION_MAX_MIPS}")
+        self.width, self.height, self.mips, self.fp16 = width, height, mips, fp16
+        n = lib().althea_reflection_texels(width, height, mips)
+        self.data = torch.zeros((n, 4), dtype=torch.float16 if fp16 else torch.float32, device="cuda")
+
+    def view(self) -> _lib.ReflectionViewC:
+        return _lib.ReflectionViewC(_ptr(self.data), self.width, self.height, self.mips,
+                                    _lib.REFLECTION_FP16 if self.fp16 else _lib.REFLECTION_FP32)
+
+    def extent(self, level: int) -> tuple:
+        return max(1, self.width >> level), max(1, self.height >> level)
+
+    def level(self, level: int):
+        """Level `level` as a (h, w, 4) view of the device tensor."""
+        w, h = self.extent(level)
+        off = lib().althea_reflection_mip_offset(self.width, self.height, level)
+        return self.data[off:off + w * h].view(h, w, 4)
+
+
+class ScreenSpaceReflection:
+    """Src/ScreenSpaceReflection.cpp: captureReflection fills level 0 of a
+    ReflectionBuffer (Shaders/SSR.frag), convolveReflectionBuffer the glossy
+    levels (Shaders/SSRGlossyConvolve.comp)."""
+
+    @staticmethod
+    def captureReflection(gbuffer, ibl, lights, globals_, reflection: ReflectionBuffer, stream=None):
+        if not isinstance(gbuffer, GBufferResources):
+            gbuffer = GBufferResources(*gbuffer)
+        views = ibl.views() if isinstance(ibl, IBLResources) else ibl
+        gv = gbuffer.getHandles()
+        rv = reflection.view()
+        n = int(lights.shape[0]) if lights is not None else 0
+        check(lib().althea_ssr_capture(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
+                                       C.byref(globals_), C.byref(rv), _stream(stream)), "althea_ssr_capture")
+        return reflection
+
+    @staticmethod
+    def convolveReflectionBuffer(reflection: ReflectionBuffer, stream=None):
+        rv = reflection.view()
+        check(lib().althea_reflection_convolve(C.byref(rv), _stream(stream)), "althea_reflection_convolve")
+        return reflection
+
+
 class DeferredRendering:
     @staticmethod
     def resolve(gbuffer, ibl, lights, globals_, ssao=None, tonemap: bool = False, out_fp16: bool = False,
-                ao_out=None, out=None, stream=None):
+                ao_out=None, out=None, reflection: ReflectionBuffer | None = None, stream=None):
         """One frame of the deferred resolve.  `gbuffer` is GBufferResources
         or a list of the four planes; `ibl` an IBLResources or IBLViewsC;
-        `lights` an (n, 8) float32 device tensor."""
+        `lights` an (n, 8) float32 device tensor; `reflection` (optional) a
+        ReflectionBuffer blended into the prefiltered term."""
         torch = _torch()
         if not isinstance(gbuffer, GBufferResources):
             gbuffer = GBufferResources(*gbuffer)
@@ -321,9 +374,17 @@ class DeferredRendering:
             dt = torch.float16 if out_fp16 else torch.float32
             out = torch.empty((gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
         flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)
+        ssao_p = C.byref(ssao) if ssao is not None else None
+        if reflection is not None:
+            rv = reflection.view()
+            check(lib().althea_deferred_resolve_reflect(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
+                                                        C.byref(globals_), ssao_p, C.byref(rv), flags, _ptr(out),
+                                                        _ptr(ao_out), _stream(stream)),
+                  "althea_deferred_resolve_reflect")
+            return out
         check(lib().althea_deferred_resolve(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
-                                            C.byref(globals_), C.byref(ssao) if ssao is not None else None,
-                                            flags, _ptr(out), _ptr(ao_out), _stream(stream)),
+                                            C.byref(globals_), ssao_p, flags, _ptr(out), _ptr(ao_out),
+                                            _stream(stream)),
               "althea_deferred_resolve")
         return out
 

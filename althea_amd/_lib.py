@@ -67,6 +67,15 @@ class TuningC(C.Structure):
     _fields_ = [("tile_w", i32), ("tile_h", i32), ("waves_per_cu", i32), ("reserved", i32)]
 
 
+class ReflectionViewC(C.Structure):
+    _fields_ = [("data", vp), ("width", i32), ("height", i32), ("mips", i32), ("format", i32)]
+
+
+REFLECTION_MAX_MIPS = 8  # ALTHEA_REFLECTION_MAX_MIPS
+REFLECTION_FP16 = 0      # ALTHEA_REFLECTION_FP16
+REFLECTION_FP32 = 1      # ALTHEA_REFLECTION_FP32
+
+
 assert C.sizeof(GlobalsC) == 416, C.sizeof(GlobalsC)
 
 # name -> (restype, argtypes)
@@ -98,6 +107,14 @@ PROTOTYPES = {
     "althea_ibl_state_bytes": (sz, [vp]),
     "althea_deferred_resolve": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
                                       C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),
+    "althea_reflection_texels": (sz, [i32, i32, i32]),
+    "althea_reflection_mip_offset": (sz, [i32, i32, i32]),
+    "althea_ssr_capture": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
+                                 C.POINTER(ReflectionViewC), vp]),
+    "althea_reflection_convolve": (i32, [C.POINTER(ReflectionViewC), vp]),
+    "althea_deferred_resolve_reflect": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
+                                              C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),
+                                              C.c_uint, vp, vp, vp]),
     "althea_hdr_info": (i32, [C.c_char_p, C.POINTER(i32), C.POINTER(i32)]),
     "althea_hdr_read": (i32, [C.c_char_p, vp, sz]),
     "althea_hdr_write": (i32, [C.c_char_p, vp, i32, i32]),

@@ -130,6 +130,20 @@ static althea_status resolve_args_ok(const althea_gbuffer_view* g, const althea_
   return ALTHEA_OK;
 }
 
+static althea_status reflection_ok(const althea_reflection_view* r, const althea_gbuffer_view* g,
+                                   const char* what) {
+  ALTHEA_REQUIRE(r && r->data, "%s: null reflection buffer", what);
+  ALTHEA_REQUIRE(r->format == ALTHEA_REFLECTION_FP16 || r->format == ALTHEA_REFLECTION_FP32,
+                 "%s: reflection format %d", what, r->format);
+  ALTHEA_REQUIRE(r->mips >= 1 && r->mips <= ALTHEA_REFLECTION_MAX_MIPS, "%s: reflection mips %d", what, r->mips);
+  ALTHEA_REQUIRE(r->width >= 1 && r->height >= 1 && r->width <= 32768 && r->height <= 32768,
+                 "%s: reflection extent %dx%d", what, r->width, r->height);
+  if (g)
+    ALTHEA_REQUIRE(r->width == g->width && r->height == g->height,
+                   "%s: reflection buffer %dx%d, G-buffer %dx%d", what, r->width, r->height, g->width, g->height);
+  return ALTHEA_OK;
+}
+
 extern "C" {
 
 const char* althea_status_string(althea_status status) {
@@ -435,6 +449,49 @@ althea_status althea_deferred_resolve_mlp(const althea_gbuffer_view* g, const al
                  ibl->prefiltered_size, ibl->prefiltered_mips);
   return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, &v, d_mlp_used,
                         stream);
+}
+
+size_t althea_reflection_mip_offset(int width, int height, int level) {
+  if (width < 1 || height < 1 || level < 0) return 0;
+  size_t off = 0;
+  for (int m = 0; m < level; ++m)
+    off += static_cast<size_t>(std::max(1, width >> m)) * static_cast<size_t>(std::max(1, height >> m));
+  return off;
+}
+
+size_t althea_reflection_texels(int width, int height, int mips) {
+  if (mips < 1 || mips > ALTHEA_REFLECTION_MAX_MIPS) return 0;
+  return althea_reflection_mip_offset(width, height, mips);
+}
+
+althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                 const althea_point_light* d_lights, int light_count, const althea_globals* globals,
+                                 const althea_reflection_view* reflection, hipStream_t stream) {
+  althea_status st =
+      resolve_args_ok(g, ibl, d_lights, light_count, globals, nullptr, reflection ? reflection->data : nullptr);
+  if (st != ALTHEA_OK) return st;
+  st = reflection_ok(reflection, g, "ssr_capture");
+  if (st != ALTHEA_OK) return st;
+  return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, stream);
+}
+
+althea_status althea_reflection_convolve(const althea_reflection_view* reflection, hipStream_t stream) {
+  althea_status st = reflection_ok(reflection, nullptr, "reflection_convolve");
+  if (st != ALTHEA_OK) return st;
+  return launch_reflection_convolve(reflection, stream);
+}
+
+althea_status althea_deferred_resolve_reflect(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                              const althea_point_light* d_lights, int light_count,
+                                              const althea_globals* globals, const althea_ssao_desc* ssao,
+                                              const althea_reflection_view* reflection, unsigned flags, void* d_out,
+                                              float* d_ao_out, hipStream_t stream) {
+  althea_status st = resolve_args_ok(g, ibl, d_lights, light_count, globals, ssao, d_out);
+  if (st != ALTHEA_OK) return st;
+  st = reflection_ok(reflection, g, "deferred_resolve_reflect");
+  if (st != ALTHEA_OK) return st;
+  return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, nullptr, nullptr,
+                        stream, reflection);
 }
 
 }  // extern "C"

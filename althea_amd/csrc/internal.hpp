@@ -79,12 +79,28 @@ althea_status prefilter_batch_into(const althea_env_view* envs, int count, const
                                    int samples, unsigned flags, float* const* pyramids, float* const* cubes,
                                    const althea_tuning* tuning, althea_tuning* used, hipStream_t stream);
 
+// Host: the view ray of DeferredPass.vert:10-22 / SSR.vert:15-22 is linear
+// in NDC, so affine in the pixel centre (p + 0.5): ray0 + rayx x + rayy y;
+// and projection * view, column-major (resolve_kernels.hip).
+void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]);
+void view_projection(const althea_globals& g, float vp[16]);
+
 // Deferred resolve (resolve_kernels.hip); with `mlp` the prefiltered term
 // comes from the MLP where its monitor accepts.
 struct MlpInferenceView;
 althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
-                             const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s);
+                             const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s,
+                             const althea_reflection_view* reflection = nullptr);
+
+// Screen-space reflection (ssr_kernels.hip): level 0 of the reflection
+// buffer, then the glossy convolution of levels 1 .. mips-1.
+struct ReflChain;
+void refl_chain(const althea_reflection_view& v, ReflChain& c);
+althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                 const althea_point_light* lights, int light_count, const althea_globals* globals,
+                                 const althea_reflection_view* reflection, hipStream_t s);
+althea_status launch_reflection_convolve(const althea_reflection_view* reflection, hipStream_t s);
 
 }  // namespace althea

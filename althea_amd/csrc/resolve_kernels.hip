@@ -21,6 +21,8 @@
 #include "device_math.hpp"
 #include "internal.hpp"
 #include "mlp_eval.hpp"
+#include "reflection.hpp"
+#include "shading.hpp"
 
 namespace althea {
 
@@ -54,6 +56,7 @@ struct ResolveParams {
   float* ao_out;
   MlpInferenceView mlp;  // MLP variant only
   uint8_t* mlp_used;
+  ReflChain refl;  // reflection-buffer variant only
 };
 
 template <int kFmt>
@@ -79,16 +82,6 @@ __device__ __forceinline__ void store_out(void* out, size_t idx, f3 c) {
   } else {
     static_cast<float4*>(out)[idx] = make_float4(c.x, c.y, c.z, 1.0f);
   }
-}
-
-// Smith-Schlick view term gv = NdotV / (NdotV (1 - k) + k) (PBRMaterial.glsl:55-57).
-__device__ __forceinline__ float gv_of(float n_dot_v, float k) {
-  return n_dot_v * rcp_fast(fmaf(n_dot_v, 1.0f - k, k));
-}
-
-__device__ __forceinline__ float pow5(float x) {
-  float x2 = x * x;
-  return x2 * x2 * x;
 }
 
 __device__ __forceinline__ f3 tonemap(f3 c, float exposure) {
@@ -172,22 +165,6 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
   return fmaf(-occ, p.ssao_inv_samples, 1.0f);
 }
 
-__device__ __forceinline__ void lut_sample(const ResolveParams& p, float u, float v, float& A, float& B) {
-  const int n = p.lut_size;
-  const float x = fmaf(u, static_cast<float>(n), -0.5f), y = fmaf(v, static_cast<float>(n), -0.5f);
-  const float fx = floorf(x), fy = floorf(y);
-  const float ax = x - fx, ay = y - fy;
-  const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
-  const int x0 = min(max(ix, 0), n - 1), x1 = min(max(ix + 1, 0), n - 1);
-  const int y0 = min(max(iy, 0), n - 1), y1 = min(max(iy + 1, 0), n - 1);
-  const float2 t00 = p.lut[y0 * n + x0], t10 = p.lut[y0 * n + x1];
-  const float2 t01 = p.lut[y1 * n + x0], t11 = p.lut[y1 * n + x1];
-  const float ta = fmaf(t10.x - t00.x, ax, t00.x), ba = fmaf(t11.x - t01.x, ax, t01.x);
-  const float tb = fmaf(t10.y - t00.y, ax, t00.y), bb = fmaf(t11.y - t01.y, ax, t01.y);
-  A = fmaf(ba - ta, ay, ta);
-  B = fmaf(bb - tb, ay, tb);
-}
-
 // LDS of the MLP variant: the per-wave query staging, then the weights.
 struct MlpResolveSmem {
   MlpWaveStage stage[4];
@@ -198,7 +175,7 @@ struct MlpResolveSmem {
 // mapping that gives each XCD (with its own L2) contiguous runs of tiles was
 // measured slower at 4K (runs of 2, 8 or 68 tile rows: 0.87, 0.89, 0.84 ms
 // against 0.83 ms).
-template <int kFmt, bool kOutF16, bool kMlp>
+template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false>
 __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, int tiles_x,
                                              const float4* __restrict__ lights, const float4* s_noise,
                                              MlpWaveStage* s_stage, const MlpSmem* s_mlp) {
@@ -265,95 +242,32 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     asm volatile("" ::: "memory");
   }
   const float4 alb = shade ? load_gbuf<kFmt>(p.albedo, idx) : zero4;
-  const f3 base = mk(alb.x, alb.y, alb.z);
+  const PbrPoint m = pbr_point(mk(alb.x, alb.y, alb.z), metallic, roughness, NI);
   f3 color = mk(0.0f, 0.0f, 0.0f);
-  const float n_dot_v = fmaxf(-NI, 0.0f);
-  const f3 F0 = mk(fmaf(metallic, base.x - 0.04f, 0.04f), fmaf(metallic, base.y - 0.04f, 0.04f),
-                   fmaf(metallic, base.z - 0.04f, 0.04f));
-  const float a = roughness * roughness;
-  const float a2 = a * a;
-  const float kdir = (a + 1.0f) * (a + 1.0f) * 0.125f;
-  const float om_r = 1.0f - roughness;
-  const f3 dF = mk(fmaxf(om_r, F0.x) - F0.x, fmaxf(om_r, F0.y) - F0.y, fmaxf(om_r, F0.z) - F0.z);
-  const f3 kd = base * (1.0f - metallic);
   if (shade) {
     if (!from_mlp) refl = cube_sample(p.prefiltered, p.pref, R, lod);
+    if (kRefl) {
+      // Reflection buffer at LOD roughness * (mips - 1); where its alpha a
+      // reaches 0.01, mix(prefiltered, rgb / a, a) (DeferredPass.frag:59-68).
+      const float4 rb = refl_trilinear_pixel(p.refl, px, py, roughness * static_cast<float>(p.refl.mips - 1));
+      if (rb.w >= 0.01f) {
+        const float ia = 1.0f / rb.w;
+        refl = mk(fmaf(fmaf(rb.x, ia, -refl.x), rb.w, refl.x), fmaf(fmaf(rb.y, ia, -refl.y), rb.w, refl.y),
+                  fmaf(fmaf(rb.z, ia, -refl.z), rb.w, refl.z));
+      }
+    }
     int face;
     float pc, pt;
     cube_select(N, face, pc, pt);
     const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
     if (p.ssao_on) ao = ssao<kFmt>(p, s_noise, px, py, P, N);
     if (p.ao_out) p.ao_out[idx] = ao;
-    // Environment term of pbrMaterial (PBRMaterial.glsl:101-114).
-    const float fx = pow5(1.0f - n_dot_v);
-    const f3 F = mk(fmaf(dF.x, fx, F0.x), fmaf(dF.y, fx, F0.y), fmaf(dF.z, fx, F0.z));
     float A, B;
-    lut_sample(p, n_dot_v, roughness, A, B);
-    color = mk((irr.x * ((1.0f - F.x) * kd.x) + refl.x * fmaf(F.x, A, B)) * ao,
-               (irr.y * ((1.0f - F.y) * kd.y) + refl.y * fmaf(F.y, A, B)) * ao,
-               (irr.z * ((1.0f - F.z) * kd.z) + refl.z * fmaf(F.z, A, B)) * ao);
+    lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
+    color = pbr_environment(m, irr, refl, A, B, ao);
   }
-
-  // Point lights (PBRMaterial.glsl:117-159), per light:
-  //   d = |Lp - P|, L = (Lp - P)/d, NdotL = max(N.L, 0),
-  //   |V + L|^2 = 2 + 2 V.L (unit V, L), NdotH = max((N.V + N.L)/|V + L|, 0),
-  //   F = F0 + dF (1 - NdotH)^5, D = a2 / (pi t^2), t = NdotH^2 (a2 - 1) + 1,
-  //   spec = D F G / (4 NdotL NdotV + 1e-4), G = gv gl, gl = NdotL / (NdotL (1-k) + k),
-  //   diffuse = (1 - F) kd / pi, color += (diffuse + spec) E NdotL / d^2.
-  // The three denominators share one reciprocal.  Grey lights (E.x = E.y =
-  // E.z) accumulate four per-pixel scalars instead of three channels:
-  //   sum (F_c s + (1 - F_c) kdp_c) e w
-  //     = F0_c (A2 - kdp_c A1) + dF_c (A4 - kdp_c A3) + kdp_c A1
-  //   with ew = e w, S = s ew, A1 = sum ew, A2 = sum S, A3 = sum x5 ew, A4 = sum x5 S.
-  // A light below the horizon of every lane of the wave adds exactly zero and
-  // is skipped with a wave-uniform branch.
-  const float c1 = a2 * gv_of(n_dot_v, kdir) * kInvPi;
-  const float four_nv = 4.0f * n_dot_v;
-  const float om_k = 1.0f - kdir;
-  const float a2m1 = a2 - 1.0f;
-  const f3 kdp = kd * kInvPi;
-  float A1 = 0.0f, A2 = 0.0f, A3 = 0.0f, A4 = 0.0f;
-  // Lights are wave-uniform: scalar loads (constant cache) into SGPRs, and
-  // the grey test is a scalar compare of the emission bits.
-  if (__any(shade)) {
-    for (int i = 0; i < p.light_count; ++i) {
-      const float4 lp = lights[2 * i];
-      const float4 em = lights[2 * i + 1];
-      const f3 L = mk(lp.x - P.x, lp.y - P.y, lp.z - P.z);
-      const float nl_raw = dot(N, L);
-      if (__builtin_amdgcn_ballot_w64(nl_raw > 0.0f) == 0) continue;
-      const float d2 = dot(L, L);
-      const float inv_d = rsq_fast(d2);
-      const float vl = dot(I, L);
-      const float h2 = fmaf(2.0f * vl, inv_d, 2.0f);
-      const float n_dot_l = fmaxf(nl_raw * inv_d, 0.0f);
-      const float n_dot_h = fmaxf(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2), 0.0f);
-      const float x = 1.0f - n_dot_h;
-      const float x5 = pow5(x);
-      const float t = fmaf(n_dot_h * n_dot_h, a2m1, 1.0f);
-      const float den = (t * t) * fmaf(n_dot_l, om_k, kdir) * fmaf(n_dot_l, four_nv, 0.0001f);
-      const float s = c1 * n_dot_l * rcp_fast(den);
-      const float w = (inv_d * inv_d) * n_dot_l;
-      const bool grey = __float_as_uint(em.x) == __float_as_uint(em.y) && __float_as_uint(em.y) == __float_as_uint(em.z);
-      if (grey) {
-        const float ew = em.x * w;
-        const float S = s * ew;
-        A1 += ew;
-        A2 += S;
-        A3 = fmaf(x5, ew, A3);
-        A4 = fmaf(x5, S, A4);
-      } else {
-        const f3 F = mk(fmaf(dF.x, x5, F0.x), fmaf(dF.y, x5, F0.y), fmaf(dF.z, x5, F0.z));
-        color.x = fmaf(fmaf(F.x, s, fmaf(-F.x, kdp.x, kdp.x)) * em.x, w, color.x);
-        color.y = fmaf(fmaf(F.y, s, fmaf(-F.y, kdp.y, kdp.y)) * em.y, w, color.y);
-        color.z = fmaf(fmaf(F.z, s, fmaf(-F.z, kdp.z, kdp.z)) * em.z, w, color.z);
-      }
-    }
-  }
+  pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color);
   if (shade) {
-    color.x += fmaf(F0.x, fmaf(-kdp.x, A1, A2), fmaf(dF.x, fmaf(-kdp.x, A3, A4), kdp.x * A1));
-    color.y += fmaf(F0.y, fmaf(-kdp.y, A1, A2), fmaf(dF.y, fmaf(-kdp.y, A3, A4), kdp.y * A1));
-    color.z += fmaf(F0.z, fmaf(-kdp.z, A1, A2), fmaf(dF.z, fmaf(-kdp.z, A3, A4), kdp.z * A1));
     if (p.tonemap) color = tonemap(color, p.exposure);
     store_out<kOutF16>(p.out, idx, color);
   }
@@ -370,13 +284,13 @@ __device__ __forceinline__ void resolve_prologue(const ResolveParams& p, float4*
   __syncthreads();
 }
 
-// Plain resolve: one block per tile.
-template <int kFmt, bool kOutF16>
+// Plain resolve: one block per tile; kRefl blends the reflection buffer.
+template <int kFmt, bool kOutF16, bool kRefl = false>
 __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, const float4* __restrict__ lights) {
   __shared__ float4 s_noise[16];
   resolve_prologue<false>(p, s_noise, nullptr);
   const int tiles_x = (p.width + 63) / 64;
-  resolve_tile<kFmt, kOutF16, false>(p, blockIdx.x, tiles_x, lights, s_noise, nullptr, nullptr);
+  resolve_tile<kFmt, kOutF16, false, kRefl>(p, blockIdx.x, tiles_x, lights, s_noise, nullptr, nullptr);
 }
 
 // MLP-IBL resolve: persistent blocks keep the MLP weights in LDS for several
@@ -402,7 +316,7 @@ __global__ void fill_u8_kernel(uint8_t* __restrict__ p, int n) {
 
 // Host: the view ray of DeferredPass.vert is linear in NDC, so it is affine in
 // the pixel centre; fold mat3(inverseView) * inverseProjection into it.
-static void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]) {
+void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]) {
   const float* ip = g.inverse_projection;
   const float* iv = g.inverse_view;
   auto e_of = [&](double nx, double ny, double out[3]) {
@@ -421,10 +335,20 @@ static void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3]
   }
 }
 
+void view_projection(const althea_globals& g, float vp[16]) {
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double acc = 0.0;
+      for (int k = 0; k < 4; ++k) acc += static_cast<double>(g.projection[k * 4 + r]) * g.view[c * 4 + k];
+      vp[c * 4 + r] = static_cast<float>(acc);
+    }
+}
+
 althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
-                             const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s) {
+                             const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s,
+                             const althea_reflection_view* reflection) {
   ResolveParams p{};
   p.position = g->position;
   p.normal = g->normal;
@@ -447,13 +371,7 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   p.light_count = light_count;
   view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy);
   for (int c = 0; c < 4; ++c) p.view_z[c] = globals->view[c * 4 + 2];
-  for (int c = 0; c < 4; ++c)
-    for (int r = 0; r < 4; ++r) {
-      double acc = 0.0;
-      for (int k = 0; k < 4; ++k)
-        acc += static_cast<double>(globals->projection[k * 4 + r]) * globals->view[c * 4 + k];
-      p.vp[c * 4 + r] = static_cast<float>(acc);
-    }
+  view_projection(*globals, p.vp);
   p.exposure = globals->exposure;
   p.tonemap = (flags & ALTHEA_RESOLVE_TONEMAP) ? 1 : 0;
   p.ssao_on = (ssao && ssao->enabled) ? 1 : 0;
@@ -504,6 +422,18 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
     return check_launch("resolve_mlp_kernel");
   }
   const dim3 grid(tiles);
+  if (reflection) {
+    refl_chain(*reflection, p.refl);
+    if (fp32 && f16_out)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true, true>), grid, dim3(256), 0, s, p, lights4);
+    else if (fp32)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false, true>), grid, dim3(256), 0, s, p, lights4);
+    else if (f16_out)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true, true>), grid, dim3(256), 0, s, p, lights4);
+    else
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false, true>), grid, dim3(256), 0, s, p, lights4);
+    return check_launch("resolve_kernel");
+  }
   if (fp32 && f16_out)
     hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);
   else if (fp32)

@@ -1,0 +1,278 @@
+// Screen-space reflection and the glossy reflection buffer for gfx950.
+//
+// ssr_capture_kernel (Shaders/SSR.frag:80-149 with SSR.vert:15-22): one lane
+// per pixel in 64 x 4 pixel tiles, so the 64 lanes of a wave march parallel
+// rays one pixel apart and their bilinear position reads at each step share
+// cache lines.  A lane leaves the march at its hit, its exit from the screen
+// or step 128; the wave leaves when no lane is marching.  The hit shading
+// (SSR.frag:46-78, pbrMaterial with the IBL state and the point lights) runs
+// once after the march for all hit lanes of the wave together.
+//
+// reflection_convolve_kernel (Shaders/SSRGlossyConvolve.comp:22-55, scheduled
+// by Src/ReflectionBuffer.cpp:163-277): one lane per texel of level m, 7
+// bilinear taps into level m-1 along one axis.
+//
+// G-buffer coverage is position.w != 0 (the flag the resolve uses,
+// DeferredPass.frag:44); the reference's SSR tests normal.a (SSR.frag:131)
+// and rebuilds positions from depth (SSR.frag:38-41,
+// Misc/ReconstructPosition.glsl); this G-buffer stores world positions
+// (DeferredRendering.cpp:23-35), which the march filters bilinearly as the
+// reference's sampler filters depth.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "device_math.hpp"
+#include "internal.hpp"
+#include "reflection.hpp"
+#include "shading.hpp"
+
+namespace althea {
+
+constexpr int kSsrSteps = 128;        // RAYMARCH_STEPS, SSR.frag:80
+constexpr float kSsrStep = 0.005f;    // uv per step, SSR.frag:89
+constexpr float kSsrCone = 0.999f;    // cos of the accepted angle, SSR.frag:112
+constexpr float kSsrFar = 10000.0f;   // march target distance, SSR.frag:83
+
+struct SsrParams {
+  const void* position;
+  const void* normal;
+  const void* albedo;
+  const void* mro;
+  int width, height;
+  const float4* prefiltered;
+  CubeMips pref;
+  const float4* irradiance;
+  int irr_size;
+  const float2* lut;
+  int lut_size;
+  int light_count;
+  float ray0[3], rayx[3], rayy[3];  // view ray = ray0 + rayx (px+0.5) + rayy (py+0.5)
+  float vp[16];                     // projection * view, column-major
+  ReflChain out;                    // level 0 is written
+};
+
+template <int kFmt>
+__device__ __forceinline__ float4 gb_load(const void* plane, int i) {
+  if (kFmt == ALTHEA_GBUFFER_FP32) return static_cast<const float4*>(plane)[i];
+  const ushort4 h = static_cast<const ushort4*>(plane)[i];
+  return make_float4(__half2float(__ushort_as_half(h.x)), __half2float(__ushort_as_half(h.y)),
+                     __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
+}
+
+// Bilinear CLAMP_TO_EDGE read of a G-buffer plane at uv in [0, 1]^2.
+template <int kFmt>
+__device__ __forceinline__ float4 gb_bilinear(const void* plane, int W, int H, float u, float v) {
+  const float x = fmaf(u, static_cast<float>(W), -0.5f), y = fmaf(v, static_cast<float>(H), -0.5f);
+  const float fx = floorf(x), fy = floorf(y);
+  const float ax = x - fx, ay = y - fy;
+  const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  const int x0 = min(max(ix, 0), W - 1), x1 = min(max(ix + 1, 0), W - 1);
+  const int y0 = min(max(iy, 0), H - 1), y1 = min(max(iy + 1, 0), H - 1);
+  const float4 t00 = gb_load<kFmt>(plane, y0 * W + x0), t10 = gb_load<kFmt>(plane, y0 * W + x1);
+  const float4 t01 = gb_load<kFmt>(plane, y1 * W + x0), t11 = gb_load<kFmt>(plane, y1 * W + x1);
+  return lerp4(lerp4(t00, t10, ax), lerp4(t01, t11, ax), ay);
+}
+
+template <int kFmt>
+__global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, const float4* __restrict__ lights) {
+  const int tiles_x = (p.width + 63) / 64;
+  const int px = (blockIdx.x % tiles_x) * 64 + (threadIdx.x & 63);
+  const int py = (blockIdx.x / tiles_x) * 4 + (threadIdx.x >> 6);
+  const bool inside = px < p.width && py < p.height;
+  const int idx = min(py, p.height - 1) * p.width + min(px, p.width - 1);
+  const float4 pos = gb_load<kFmt>(p.position, idx);
+  const bool covered = inside && pos.w != 0.0f;
+  const float4 nrm = covered ? gb_load<kFmt>(p.normal, idx) : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+  const f3 P = mk(pos.x, pos.y, pos.z);
+  const f3 N = normalize_fast(mk(nrm.x, nrm.y, nrm.z));
+  // reflect(normalize(view ray), N) (SSR.frag:137).
+  const float cxp = static_cast<float>(px) + 0.5f, cyp = static_cast<float>(py) + 0.5f;
+  const f3 I = normalize_fast(mk(fmaf(p.rayy[0], cyp, fmaf(p.rayx[0], cxp, p.ray0[0])),
+                                 fmaf(p.rayy[1], cyp, fmaf(p.rayx[1], cxp, p.ray0[1])),
+                                 fmaf(p.rayy[2], cyp, fmaf(p.rayx[2], cxp, p.ray0[2]))));
+  const float ni = 2.0f * dot(N, I);
+  const f3 R = mk(fmaf(-ni, N.x, I.x), fmaf(-ni, N.y, I.y), fmaf(-ni, N.z, I.z));
+  // March direction in uv towards the projection of P + 10000 R
+  // (SSR.frag:82-90), and the reference plane normal (SSR.frag:92-93).
+  const f3 E = mk(fmaf(R.x, kSsrFar, P.x), fmaf(R.y, kSsrFar, P.y), fmaf(R.z, kSsrFar, P.z));
+  const float ex = fmaf(p.vp[0], E.x, fmaf(p.vp[4], E.y, fmaf(p.vp[8], E.z, p.vp[12])));
+  const float ey = fmaf(p.vp[1], E.x, fmaf(p.vp[5], E.y, fmaf(p.vp[9], E.z, p.vp[13])));
+  const float ew = fmaf(p.vp[3], E.x, fmaf(p.vp[7], E.y, fmaf(p.vp[11], E.z, p.vp[15])));
+  const float u0 = cxp / static_cast<float>(p.width), v0 = cyp / static_cast<float>(p.height);
+  const float du = fmaf(0.5f, ex / ew, 0.5f) - u0, dv = fmaf(0.5f, ey / ew, 0.5f) - v0;
+  const float sl = kSsrStep * rsq_fast(fmaf(du, du, dv * dv));
+  const float su = du * sl, sv = dv * sl;
+  const f3 perp = normalize_fast(cross(cross(R, N), R));
+  // A non-finite march direction (the target projects onto the pixel, or to
+  // w = 0) gives no reflection.
+  bool active = covered && isfinite(su) && isfinite(sv);
+  bool hit = false;
+  float hu = 0.0f, hv = 0.0f;
+  f3 Q = mk(0.0f, 0.0f, 0.0f), Nh = mk(0.0f, 0.0f, 0.0f);
+  float prev = 0.0f;
+  for (int i = 0; i < kSsrSteps; ++i) {
+    if (!__any(active)) break;
+    if (active) {
+      // currentUV += uvStep * stepSize, as uv0 + (i + 1) step (same value in
+      // exact arithmetic, one rounding instead of i + 1).
+      const float s = static_cast<float>(i + 1);
+      const float u = fmaf(s, su, u0), v = fmaf(s, sv, v0);
+      if (!(u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f)) {
+        active = false;  // off screen: SSR.frag:97-99
+      } else {
+        const float4 q4 = gb_bilinear<kFmt>(p.position, p.width, p.height, u, v);
+        const f3 dq = mk(q4.x - P.x, q4.y - P.y, q4.z - P.z);
+        const float rl = rsq_fast(dot(dq, dq));
+        const float proj = dot(dq, perp) * rl;
+        const float f = dot(dq, R) * rl;
+        if (i > 0 && proj * prev <= 0.0f && f > kSsrCone) {
+          const float4 n4 = gb_bilinear<kFmt>(p.normal, p.width, p.height, u, v);
+          const f3 nh = normalize_fast(mk(n4.x, n4.y, n4.z));
+          if (dot(nh, R) < 0.0f) {
+            hit = true;
+            active = false;
+            hu = u;
+            hv = v;
+            Q = mk(q4.x, q4.y, q4.z);
+            Nh = nh;
+          }
+        }
+        prev = proj;
+      }
+    }
+  }
+  // environmentLitSample (SSR.frag:46-78): V = the reflected ray, occlusion 1.
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 alb = hit ? gb_bilinear<kFmt>(p.albedo, p.width, p.height, hu, hv) : zero4;
+  const float4 mro = hit ? gb_bilinear<kFmt>(p.mro, p.width, p.height, hu, hv) : zero4;
+  const float roughness = hit ? mro.y : 1.0f;
+  const float NI = dot(Nh, R);
+  const PbrPoint m = pbr_point(mk(alb.x, alb.y, alb.z), mro.x, roughness, NI);
+  f3 color = mk(0.0f, 0.0f, 0.0f);
+  if (hit) {
+    const float nd = 2.0f * NI;
+    const f3 Rr = mk(fmaf(-nd, Nh.x, R.x), fmaf(-nd, Nh.y, R.y), fmaf(-nd, Nh.z, R.z));
+    const f3 refl = cube_sample(p.prefiltered, p.pref, Rr, roughness * static_cast<float>(p.pref.mips - 1));
+    int face;
+    float pc, pt;
+    cube_select(Nh, face, pc, pt);
+    const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
+    float A, B;
+    lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
+    color = pbr_environment(m, irr, refl, A, B, 1.0f);
+  }
+  pbr_lights(m, Q, Nh, R, NI, __any(hit), lights, p.light_count, color);
+  if (inside) refl_store(p.out, static_cast<uint32_t>(idx), hit ? make_float4(color.x, color.y, color.z, 1.0f) : zero4);
+}
+
+// Level m from level m-1.  The texel's uv is texel / extent(m) with no
+// half-texel offset (SSRGlossyConvolve.comp:34-35); a tap at uv + o d / W_m
+// lands at (q + c) source texels, q integer, c = frac + o s - 0.5 with
+// s = W_{m-1} / W_m (along x) or H_{m-1} / W_m (along y).
+struct ConvolveArgs {
+  ReflChain c;
+  int level;
+  int vertical;  // blur along y (odd levels, ReflectionBuffer.cpp:254-255)
+  float scale;   // s above
+};
+
+__global__ __launch_bounds__(256) void reflection_convolve_kernel(const ConvolveArgs a) {
+  const int l = a.level;
+  const int dw = a.c.w[l], dh = a.c.h[l];
+  const int tiles_x = (dw + 63) / 64;
+  const int x = (blockIdx.x % tiles_x) * 64 + (threadIdx.x & 63);
+  const int y = (blockIdx.x / tiles_x) * 4 + (threadIdx.x >> 6);
+  if (x >= dw || y >= dh) return;
+  int qx, qy;
+  float fx, fy;
+  rational_split(static_cast<uint32_t>(x) * static_cast<uint32_t>(a.c.w[l - 1]), static_cast<uint32_t>(dw), qx, fx);
+  rational_split(static_cast<uint32_t>(y) * static_cast<uint32_t>(a.c.h[l - 1]), static_cast<uint32_t>(dh), qy, fy);
+  fx -= 0.5f;
+  fy -= 0.5f;
+  // 13-tap binomial kernel (row 16 of Pascal's triangle without its two
+  // outer taps per side) folded into 7 linear taps (SSRGlossyConvolve.comp:42-52).
+  constexpr float kOff[3] = {1.411764705882353f, 3.2941176470588234f, 5.176470588235294f};
+  constexpr float kW[3] = {0.2969069646728344f, 0.09447039785044732f, 0.010381362401148057f};
+  auto tap = [&](float o) {
+    const float d = o * a.scale;
+    return a.vertical ? refl_bilinear(a.c, l - 1, qx, fx, qy, fy + d) : refl_bilinear(a.c, l - 1, qx, fx + d, qy, fy);
+  };
+  const float4 c0 = tap(0.0f);
+  float4 acc = make_float4(c0.x * 0.1964825501511404f, c0.y * 0.1964825501511404f, c0.z * 0.1964825501511404f,
+                           c0.w * 0.1964825501511404f);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float4 t0 = tap(kOff[k]);
+    acc = make_float4(fmaf(t0.x, kW[k], acc.x), fmaf(t0.y, kW[k], acc.y), fmaf(t0.z, kW[k], acc.z),
+                      fmaf(t0.w, kW[k], acc.w));
+    const float4 t1 = tap(-kOff[k]);
+    acc = make_float4(fmaf(t1.x, kW[k], acc.x), fmaf(t1.y, kW[k], acc.y), fmaf(t1.z, kW[k], acc.z),
+                      fmaf(t1.w, kW[k], acc.w));
+  }
+  refl_store(a.c, a.c.off[l] + static_cast<uint32_t>(y * dw + x), acc);
+}
+
+void refl_chain(const althea_reflection_view& v, ReflChain& c) {
+  c.data = v.data;
+  c.f16 = v.format == ALTHEA_REFLECTION_FP16 ? 1 : 0;
+  c.mips = v.mips;
+  uint32_t off = 0;
+  for (int m = 0; m < ALTHEA_REFLECTION_MAX_MIPS; ++m) {
+    c.w[m] = std::max(1, v.width >> m);
+    c.h[m] = std::max(1, v.height >> m);
+    c.off[m] = off;
+    if (m < v.mips) off += static_cast<uint32_t>(c.w[m]) * static_cast<uint32_t>(c.h[m]);
+  }
+}
+
+althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                 const althea_point_light* lights, int light_count, const althea_globals* globals,
+                                 const althea_reflection_view* reflection, hipStream_t s) {
+  SsrParams p{};
+  p.position = g->position;
+  p.normal = g->normal;
+  p.albedo = g->albedo;
+  p.mro = g->mro;
+  p.width = g->width;
+  p.height = g->height;
+  p.prefiltered = reinterpret_cast<const float4*>(ibl->prefiltered);
+  p.pref.size = ibl->prefiltered_size;
+  p.pref.mips = ibl->prefiltered_mips;
+  for (int m = 0; m < ibl->prefiltered_mips; ++m)
+    p.pref.off[m] = static_cast<uint32_t>(cube_mip_offset(ibl->prefiltered_size, m));
+  p.irradiance = reinterpret_cast<const float4*>(ibl->irradiance);
+  p.irr_size = ibl->irradiance_size;
+  p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
+  p.lut_size = ibl->brdf_lut_size;
+  p.light_count = light_count;
+  view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy);
+  view_projection(*globals, p.vp);
+  refl_chain(*reflection, p.out);
+  const dim3 grid(((g->width + 63) / 64) * ((g->height + 3) / 4));
+  const float4* lights4 = reinterpret_cast<const float4*>(lights);
+  if (g->format == ALTHEA_GBUFFER_FP32)
+    hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP32>), grid, dim3(256), 0, s, p, lights4);
+  else
+    hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP16>), grid, dim3(256), 0, s, p, lights4);
+  return check_launch("ssr_capture_kernel");
+}
+
+althea_status launch_reflection_convolve(const althea_reflection_view* reflection, hipStream_t s) {
+  ConvolveArgs a{};
+  refl_chain(*reflection, a.c);
+  for (int l = 1; l < reflection->mips; ++l) {
+    a.level = l;
+    a.vertical = (l & 1) ? 1 : 0;
+    a.scale = static_cast<float>(static_cast<double>(a.vertical ? a.c.h[l - 1] : a.c.w[l - 1]) / a.c.w[l]);
+    const dim3 grid(((a.c.w[l] + 63) / 64) * ((a.c.h[l] + 3) / 4));
+    hipLaunchKernelGGL(reflection_convolve_kernel, grid, dim3(256), 0, s, a);
+    althea_status st = check_launch("reflection_convolve_kernel");
+    if (st != ALTHEA_OK) return st;
+  }
+  return ALTHEA_OK;
+}
+
+}  // namespace althea

@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
     p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
     p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
-    p.add_argument("--only", choices=["resolve", "prefilter", "copy"],
+    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr"],
                    help="profiling runs: time one kernel family and print its line only "
                         "(copy = a plain 1 GiB device copy, the calibration for HBM byte counters)")
     return p.parse_args()
@@ -120,27 +120,31 @@ def batched(torch, aa) -> dict:
 
 def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
     """The learned stand-in on the same frame.  2^22 (direction, roughness)
-    queries labelled by the 1024-sample prefilter of the frame's
-    environment; Adam, batch 65536, step size 3e-3 decaying to 1e-5 over 256
-    epochs (16384 steps); then the error monitor against the 512^2 x6 cube
+    queries, roughness uniform in [1/(mips-1), 1] (the range whose cube
+    lookups blend two mips the monitor can accept; mip 0 is a mirror of
+    per-texel noise), labelled by the 1024-sample prefilter of the frame's
+    environment; Adam, batch 16384, step size 3e-3 decaying to 1e-5 over 256
+    epochs (65536 steps); then the error monitor against the 512^2 x6 cube
     and the fused MLP-IBL resolve at monitor thresholds 1e-3 (the target),
-    3e-3 and 1e-2.  Each MLP-IBL frame is compared with the cube-IBL frame
+    3e-3, 5e-3 and 1e-2.  Each MLP-IBL frame is compared with the cube-IBL frame
     over every pixel (relative difference, floor 1e-2)."""
     from althea_amd import mlp as M
 
     t0 = time.perf_counter()
     pyr = ibl.environment_tensor()
-    dr, labels = M.training_set(pyr, env.shape[1], env.shape[0], 1 << 22, seed=SEED0 + 4, samples=1024)
+    v = ibl.views()
+    rmin = 1.0 / (v.prefiltered_mips - 1)
+    dr, labels = M.training_set(pyr, env.shape[1], env.shape[0], 1 << 22, seed=SEED0 + 4, samples=1024, rmin=rmin)
     torch.cuda.synchronize()
     label_s = time.perf_counter() - t0
     net = M.PrefilterMLP(seed=SEED0 + 5, lr=3e-3, init_output_bias=float(np.log(labels[:, :3].mean().item())))
     epochs = 256
     t0 = time.perf_counter()
-    hist = M.fit(net, dr, labels, batch=65536, epochs=epochs, lr=3e-3, final_lr=1e-5)
+    batch = 16384
+    hist = M.fit(net, dr, labels, batch=batch, epochs=epochs, lr=3e-3, final_lr=1e-5)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    train_steps = epochs * ((1 << 22) // 65536)
-    v = ibl.views()
+    train_steps = epochs * ((1 << 22) // batch)
     cube = ibl.prefiltered_tensor()
     out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
     used = torch.zeros(WIDTH * HEIGHT, dtype=torch.uint8, device="cuda")
@@ -148,7 +152,7 @@ def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
     plain_ms = time_events(torch, lambda: aa.DeferredRendering.resolve(gbuf, ibl, lights, scene.globals, scene.ssao,
                                                                         out=out), steps)
     per_threshold = {}
-    for thr in (1e-3, 3e-3, 1e-2):
+    for thr in (1e-3, 3e-3, 5e-3, 1e-2):
         t0 = time.perf_counter()
         cell_err, accepted = net.build_monitor(cube, v.prefiltered_size, v.prefiltered_mips, threshold=thr)
         torch.cuda.synchronize()
@@ -173,9 +177,65 @@ def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
                                             for m in range(e.shape[0])},
             "ibl_state_bytes": {"cube_ibl": ibl.state_bytes(), "mlp_weights": net.state_bytes(),
                                 "monitor_flags": int(e.size)},
-            "training": {"queries": 1 << 22, "label_spp": 1024, "label_s": label_s, "batch": 65536,
-                         "steps": train_steps, "train_s": train_s, "ms_per_step": 1e3 * train_s / train_steps,
-                         "loss_first": hist[0], "loss_last": hist[-1]}}
+            "training": {"queries": 1 << 22, "label_spp": 1024, "label_s": label_s, "batch": batch,
+                         "roughness_range": [rmin, 1.0], "steps": train_steps, "train_s": train_s,
+                         "ms_per_step": 1e3 * train_s / train_steps, "loss_first": hist[0], "loss_last": hist[-1]}}
+
+
+def ssr_frame(torch, aa, scene, ibl, gbuf, lights, steps: int) -> dict:
+    """Screen-space reflections on the same frame: capture into a 5-level
+    FP16 reflection buffer (Src/ReflectionBuffer.cpp:30-37), the glossy
+    convolution of levels 1-4, and the resolve reading the buffer.  The
+    capture's compulsory bytes: position and normal planes once, level 0
+    written once, the IBL state and lights; the march's filtered position
+    reads and the hit shading's albedo / MRO reads are gathers on top."""
+    refl = aa.ReflectionBuffer(WIDTH, HEIGHT, mips=5, fp16=True)
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    glob, ssao = scene.globals, scene.ssao
+
+    def capture():
+        aa.ScreenSpaceReflection.captureReflection(gbuf, ibl, lights, glob, refl)
+
+    def convolve():
+        aa.ScreenSpaceReflection.convolveReflectionBuffer(refl)
+
+    def resolve():
+        aa.DeferredRendering.resolve(gbuf, ibl, lights, glob, ssao, out=out, reflection=refl)
+
+    def frame():
+        capture()
+        convolve()
+        resolve()
+
+    frame()
+    cap_ms = time_events(torch, capture, steps)
+    conv_ms = time_events(torch, convolve, steps)
+    res_ms = time_events(torch, resolve, steps)
+    frame_ms = time_events(torch, frame, steps)
+    hits = float((refl.level(0)[..., 3] == 1.0).float().mean().item())
+    px = WIDTH * HEIGHT
+    gb = 16 if gbuf.getHandles().format == 0 else 8
+    cap_bytes = px * (2 * gb + 8) + ibl.state_bytes() + LIGHTS * 32
+    texels = [max(1, WIDTH >> m) * max(1, HEIGHT >> m) for m in range(5)]
+    conv_bytes = sum(8 * (texels[m - 1] + texels[m]) for m in range(1, 5))  # read m-1, write m (FP16 RGBA)
+    return {"capture_ms": cap_ms, "capture_mpix_per_s": px / (cap_ms * 1e-3) / 1e6,
+            "capture_algorithmic_bytes": cap_bytes, "capture_achieved_gbs": cap_bytes / (cap_ms * 1e-3) / 1e9,
+            "hit_fraction": hits, "convolve_ms": conv_ms, "convolve_levels": 4,
+            "convolve_algorithmic_bytes": conv_bytes, "resolve_with_buffer_ms": res_ms,
+            "frame_ms": frame_ms, "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
+
+
+def ssr_only(torch, aa, args) -> int:
+    """Profiling run of the SSR pipeline on the 4K frame."""
+    env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
+                      brdf_lut_size=128, brdf_lut_samples=1024)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
+    gbuf = aa.GBufferResources(*scene.gbuffer_tensors("cuda", fp16=args.fp16_gbuffer))
+    r = ssr_frame(torch, aa, scene, ibl, gbuf, scene.lights_tensor("cuda"), args.steps)
+    print(json.dumps({"kind": "ssr", **r}), flush=True)
+    return 0
 
 
 def fp16_resolve(torch, aa, scene, ibl, lights, steps: int) -> dict:
@@ -257,6 +317,8 @@ def main() -> int:
         return copy_calibration(torch, args)
     if args.only == "prefilter":
         return prefilter_only(torch, aa, args)
+    if args.only == "ssr":
+        return ssr_only(torch, aa, args)
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
     desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
                       brdf_lut_size=128, brdf_lut_samples=1024)
@@ -302,6 +364,7 @@ def main() -> int:
     if not args.no_secondary and args.only is None:
         line["secondary"] = secondary(torch, aa)
         line["secondary"]["resolve_fp16_gbuffer"] = fp16_resolve(torch, aa, scene, ibl, lights, args.steps)
+        line["secondary"]["ssr"] = ssr_frame(torch, aa, scene, ibl, gbuf, lights, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)

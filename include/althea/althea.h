@@ -310,6 +310,67 @@ ALTHEA_API althea_status althea_deferred_resolve(const althea_gbuffer_view* gbuf
                                                  hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Screen-space reflection + glossy reflection buffer                        */
+/* ------------------------------------------------------------------------ */
+
+/* The reflection buffer of Src/ReflectionBuffer.cpp:20-94: one frame-sized
+ * image with `mips` levels (the reference uses 5), level m of extent
+ * max(1, width >> m) x max(1, height >> m), RGBA per texel, stored level
+ * after level ([mip][y][x], no padding) in caller-owned device memory of
+ * althea_reflection_texels() texels.  FP16 is the reference's
+ * R16G16B16A16_SFLOAT (:37); FP32 serves precision studies. */
+#define ALTHEA_REFLECTION_MAX_MIPS 8
+#define ALTHEA_REFLECTION_FP16 0
+#define ALTHEA_REFLECTION_FP32 1
+
+typedef struct althea_reflection_view {
+  void* data;
+  int width, height; /* level 0: the frame */
+  int mips;          /* 1 .. ALTHEA_REFLECTION_MAX_MIPS */
+  int format;        /* ALTHEA_REFLECTION_FP16 or ALTHEA_REFLECTION_FP32 */
+} althea_reflection_view;
+
+ALTHEA_API size_t althea_reflection_texels(int width, int height, int mips);
+ALTHEA_API size_t althea_reflection_mip_offset(int width, int height, int level);
+
+/* Level 0 of the reflection buffer: ScreenSpaceReflection::captureReflection
+ * (Src/ScreenSpaceReflection.cpp:59-83) drawing Shaders/SSR.{vert,frag}.  Per
+ * pixel with G-buffer coverage (position.w != 0): reflect the view ray about
+ * the normal, march 128 steps of 0.005 in uv towards the projection of
+ * P + 10000 R, reading the position plane with a bilinear CLAMP_TO_EDGE
+ * filter; the first step (after the first) where the ray crosses the plane
+ * through R perpendicular to cross(R, N), within acos(0.999) of R, with a
+ * hit normal facing the ray, is shaded by pbrMaterial with the IBL state and
+ * `light_count` point lights (ambient occlusion 1) and stored as (rgb, 1).
+ * Everything else stores (0, 0, 0, 0).  Linear HDR, no tonemap.  The
+ * buffer's extent must equal the G-buffer's. */
+ALTHEA_API althea_status althea_ssr_capture(const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl,
+                                            const althea_point_light* d_lights, int light_count,
+                                            const althea_globals* globals, const althea_reflection_view* reflection,
+                                            hipStream_t stream);
+
+/* Levels 1 .. mips-1 of the reflection buffer:
+ * ReflectionBuffer::convolveReflectionBuffer (Src/ReflectionBuffer.cpp:163-277)
+ * running Shaders/SSRGlossyConvolve.comp per level.  Level m samples level
+ * m-1 bilinearly (CLAMP_TO_EDGE) at uv = texel / extent(m) (no half-texel
+ * offset), blurred along y for odd m and along x for even m by the 7-tap
+ * linear-sampled Gaussian: uv offsets of 1.4118, 3.2941 and 5.1765 divided by
+ * the level's width, along either axis (SSRGlossyConvolve.comp:40-52).
+ * Levels are written in order on `stream`. */
+ALTHEA_API althea_status althea_reflection_convolve(const althea_reflection_view* reflection, hipStream_t stream);
+
+/* althea_deferred_resolve with the reflection buffer blended into the
+ * prefiltered term (DeferredPass.frag:59-68): the buffer is read trilinearly
+ * at the pixel centre at LOD roughness*(mips-1); where its alpha a >= 0.01
+ * the term becomes mix(prefiltered, rgb / a, a). */
+ALTHEA_API althea_status althea_deferred_resolve_reflect(const althea_gbuffer_view* gbuffer,
+                                                         const althea_ibl_views* ibl,
+                                                         const althea_point_light* d_lights, int light_count,
+                                                         const althea_globals* globals, const althea_ssao_desc* ssao,
+                                                         const althea_reflection_view* reflection, unsigned flags,
+                                                         void* d_out, float* d_ao_out, hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* RGBE (.hdr) files                                                         */
 /* ------------------------------------------------------------------------ */
 

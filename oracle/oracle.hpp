@@ -23,6 +23,12 @@
 //   brdf_lut            scipy quadrature of the split-sum integral, PNG fixture
 //   pbr_material        closed-form single light at normal incidence
 //   ssao                flat plane -> 1, fully covered point -> 0
+//   ssr march           flat plane -> no hit; floor + wall scene -> hit at the
+//                       analytic reflection of each floor pixel
+//   glossy convolution  constant -> constant, linear ramp -> closed form,
+//                       same-size impulse -> trimmed binomial row 16
+//   reflection blend    alpha 0 -> plain resolve, alpha 1 / 0.5 -> the
+//                       prefiltered term replaced / halved
 //   mlp                 central finite differences, Adam closed form
 #pragma once
 
@@ -685,7 +691,8 @@ inline double ssao(const float* position, int W, int H, int px, int py, dvec3 P,
 // a14. Deferred resolve of one pixel.  DeferredPass.vert:10-22 (view ray
 // mat3(invView) * (invProj * (ndc, 0, 1)).xyz, linear across the screen),
 // DeferredPass.frag:41-94: background when position.a == 0 (Q13), reflection
-// from the prefiltered map at LOD roughness*(mips-1) (no SSR buffer: Q10),
+// from the prefiltered map at LOD roughness*(mips-1), blended with the
+// reflection buffer when one is given (Q10),
 // irradiance at LOD 0, SSAO into the occlusion channel (Q9), pbrMaterial (Q1),
 // tonemap 1 - exp(-c * exposure) (Q12).  Cube maps replace the equirect
 // lookups of PBRMaterial.glsl:4-19.
@@ -702,6 +709,7 @@ struct ResolveInputs {
   double exposure;
   bool tonemap, ssao_on;
   SsaoDesc ssao;
+  const struct ReflChain* reflection = nullptr;  // DeferredPass.frag:59-68 when set
 };
 inline dvec3 view_ray(const ResolveInputs& in, int px, int py) {
   double u = (px + 0.5) / in.W, v = (py + 0.5) / in.H;
@@ -714,6 +722,173 @@ inline dvec3 view_ray(const ResolveInputs& in, int px, int py) {
 inline dvec3 tonemap(dvec3 c, double exposure) {
   return {1.0 - std::exp(-c.x * exposure), 1.0 - std::exp(-c.y * exposure), 1.0 - std::exp(-c.z * exposure)};
 }
+// ---------------------------------------------------------------------------
+// Reflection buffer (Src/ReflectionBuffer.cpp:20-94): levels of extent
+// max(1, W >> m) x max(1, H >> m), RGBA, stored level after level.  Filtered
+// reads follow the Vulkan LINEAR / CLAMP_TO_EDGE rules (texel coordinate
+// u * extent - 0.5, floor, linear weights, clamped indices).
+// ---------------------------------------------------------------------------
+struct ReflChain {
+  const float* data;  // RGBA32F (FP16 buffers are widened by the caller)
+  int W, H, mips;
+  int w(int m) const { return std::max(1, W >> m); }
+  int h(int m) const { return std::max(1, H >> m); }
+  size_t off(int m) const {
+    size_t o = 0;
+    for (int k = 0; k < m; ++k) o += static_cast<size_t>(w(k)) * h(k);
+    return o;
+  }
+};
+inline void rgba_bilinear(const float* img, int W, int H, double x, double y, double out[4]) {
+  double fx = std::floor(x), fy = std::floor(y);
+  double ax = x - fx, ay = y - fy;
+  int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  int x0 = std::min(std::max(ix, 0), W - 1), x1 = std::min(std::max(ix + 1, 0), W - 1);
+  int y0 = std::min(std::max(iy, 0), H - 1), y1 = std::min(std::max(iy + 1, 0), H - 1);
+  const float* t00 = img + 4 * (static_cast<size_t>(y0) * W + x0);
+  const float* t10 = img + 4 * (static_cast<size_t>(y0) * W + x1);
+  const float* t01 = img + 4 * (static_cast<size_t>(y1) * W + x0);
+  const float* t11 = img + 4 * (static_cast<size_t>(y1) * W + x1);
+  for (int c = 0; c < 4; ++c) {
+    double top = t00[c] + (t10[c] - t00[c]) * ax, bot = t01[c] + (t11[c] - t01[c]) * ax;
+    out[c] = top + (bot - top) * ay;
+  }
+}
+// texture(img, uv) with the LINEAR / CLAMP_TO_EDGE sampler.
+inline void rgba_sample_uv(const float* img, int W, int H, double u, double v, double out[4]) {
+  rgba_bilinear(img, W, H, u * W - 0.5, v * H - 0.5, out);
+}
+
+// Shaders/SSRGlossyConvolve.comp:22-55 for one texel (x, y) of a dw x dh
+// level from the sw x sh level below it: uv = (x, y) / (dw, dh) (no
+// half-texel offset, :34-35), 7 bilinear taps at uv +- o_k d / dw with
+// "resolution" = the level's width for either direction (:38-52), d = (0, 1)
+// for odd levels and (1, 0) for even ones (Src/ReflectionBuffer.cpp:254-255).
+// The weights are the 13-tap row C(16, 2..14) / 65502 folded pairwise into
+// linear taps.
+inline void convolve_texel(const float* src, int sw, int sh, int dw, int dh, int x, int y, bool vertical,
+                           double out[4]) {
+  static const double kOff[3] = {1.411764705882353, 3.2941176470588234, 5.176470588235294};
+  static const double kW[3] = {0.2969069646728344, 0.09447039785044732, 0.010381362401148057};
+  const double u = static_cast<double>(x) / dw, v = static_cast<double>(y) / dh;
+  double acc[4], t[4];
+  rgba_sample_uv(src, sw, sh, u, v, t);
+  for (int c = 0; c < 4; ++c) acc[c] = t[c] * 0.1964825501511404;
+  for (int k = 0; k < 3; ++k)
+    for (int sgn = 1; sgn >= -1; sgn -= 2) {
+      const double o = sgn * kOff[k] / dw;
+      rgba_sample_uv(src, sw, sh, vertical ? u : u + o, vertical ? v + o : v, t);
+      for (int c = 0; c < 4; ++c) acc[c] += t[c] * kW[k];
+    }
+  for (int c = 0; c < 4; ++c) out[c] = acc[c];
+}
+
+// textureLod(reflectionBuffer, uv, lod) at the centre of frame pixel
+// (px, py): LINEAR mipmapping between floor(lod) and floor(lod) + 1, lod
+// clamped to [0, mips - 1] (DeferredPass.frag:29-31).
+inline void reflection_sample(const ReflChain& c, int px, int py, double lod, double out[4]) {
+  double d = clampd(lod, 0.0, c.mips - 1.0);
+  int l0 = static_cast<int>(std::floor(d)), l1 = std::min(l0 + 1, c.mips - 1);
+  double t = d - l0;
+  double u = (px + 0.5) / c.W, v = (py + 0.5) / c.H, a[4], b[4];
+  rgba_sample_uv(c.data + 4 * c.off(l0), c.w(l0), c.h(l0), u, v, a);
+  rgba_sample_uv(c.data + 4 * c.off(l1), c.w(l1), c.h(l1), u, v, b);
+  for (int k = 0; k < 4; ++k) out[k] = a[k] + (b[k] - a[k]) * t;
+}
+
+// ---------------------------------------------------------------------------
+// Screen-space reflection of one pixel: Shaders/SSR.frag:80-149 (march) and
+// :46-78 (hit shading), SSR.vert:15-22 (view ray).  The G-buffer here holds
+// world positions (read with the bilinear filter the reference applies to
+// depth before rebuilding positions, SSR.frag:38-41) and marks coverage in
+// position.w (DeferredPass.frag:44); the reference's SSR tests normal.a.
+// currentUV is uv0 + (i + 1) step (the reference accumulates; equal in exact
+// arithmetic).  A non-finite march step (the target projects onto the pixel
+// or to w = 0) gives no reflection.  out = (rgb, 1) at a hit, else 0.
+// hit_step: the step of the hit, -1 without one, -2 for uncovered pixels.
+// margin: smallest normalised distance of a decision that fixed the outcome
+// from its threshold (off-screen test in uv; sign and cone tests scaled by
+// |Q - P| / max(1, |P|); facing test in cosine), so tests can separate
+// FP32/FP64 decision flips from arithmetic error.
+// ---------------------------------------------------------------------------
+inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], double* margin, int* hit_step,
+                      double* hit_uv = nullptr) {
+  const int W = in.W, H = in.H;
+  for (int c = 0; c < 4; ++c) out[c] = 0.0;
+  double m = 1e30;
+  if (hit_step) *hit_step = -2;
+  if (margin) *margin = m;
+  const size_t idx = static_cast<size_t>(py) * W + px;
+  const float* p = in.position + 4 * idx;
+  if (p[3] == 0.0f) return;
+  if (hit_step) *hit_step = -1;
+  const float* n = in.normal + 4 * idx;
+  dvec3 P{p[0], p[1], p[2]};
+  dvec3 N = normalize({n[0], n[1], n[2]});
+  dvec3 I = normalize(view_ray(in, px, py));
+  dvec3 R = I - N * (2.0 * dot(N, I));
+  dvec3 E = P + R * 10000.0;
+  double e4[4] = {E.x, E.y, E.z, 1.0}, ev[4], clip[4];
+  in.view.apply(e4, ev);
+  in.proj.apply(ev, clip);
+  const double u0 = (px + 0.5) / W, v0 = (py + 0.5) / H;
+  double du = 0.5 * clip[0] / clip[3] + 0.5 - u0, dv = 0.5 * clip[1] / clip[3] + 0.5 - v0;
+  double len = std::sqrt(du * du + dv * dv);
+  double su = du / len * 0.005, sv = dv / len * 0.005;
+  if (!std::isfinite(su) || !std::isfinite(sv)) return;
+  dvec3 perp = normalize(cross(cross(R, N), R));
+  const double scale_p = 1.0 / std::max(1.0, length(P));
+  double prev = 0.0;
+  for (int i = 0; i < 128; ++i) {
+    double u = u0 + (i + 1) * su, v = v0 + (i + 1) * sv;
+    double mb = std::min(std::min(std::fabs(u), std::fabs(1.0 - u)), std::min(std::fabs(v), std::fabs(1.0 - v)));
+    m = std::min(m, mb);
+    if (!(u >= 0.0 && u <= 1.0 && v >= 0.0 && v <= 1.0)) break;
+    double q[4];
+    rgba_sample_uv(in.position, W, H, u, v, q);
+    dvec3 Q{q[0], q[1], q[2]};
+    dvec3 dq = Q - P;
+    double dl = length(dq);
+    dvec3 dir = dq * (1.0 / dl);
+    double proj = dot(dir, perp), f = dot(dir, R);
+    if (i > 0) {
+      double nq[4];
+      rgba_sample_uv(in.normal, W, H, u, v, nq);
+      dvec3 Nh = normalize({nq[0], nq[1], nq[2]});
+      double facing = dot(Nh, R);
+      const double s = dl * scale_p;
+      const bool a = proj * prev <= 0.0, b = f > 0.999, c = facing < 0.0;
+      const double ma = std::min(std::fabs(proj), std::fabs(prev)) * s, mbb = std::fabs(f - 0.999) * s;
+      const double mc = std::fabs(facing);
+      if (a && b && c) {
+        m = std::min(m, std::min(ma, std::min(mbb, mc)));
+        // environmentLitSample (SSR.frag:46-78): V = R, occlusion 1.
+        double al[4], mr[4];
+        rgba_sample_uv(in.albedo, W, H, u, v, al);
+        rgba_sample_uv(in.mro, W, H, u, v, mr);
+        dvec3 V = normalize(R);
+        dvec3 Rr = V - Nh * (2.0 * dot(Nh, V));
+        dvec3 refl = cube_sample(in.prefiltered, Rr, mr[1] * (in.prefiltered.mips - 1));
+        dvec3 irr = cube_sample(in.irradiance, Nh, 0.0);
+        dvec3 col = pbr_material(Q, V, Nh, {al[0], al[1], al[2]}, refl, irr, mr[0], mr[1], 1.0, in.lut, in.lights,
+                                 in.light_count);
+        out[0] = col.x; out[1] = col.y; out[2] = col.z; out[3] = 1.0;
+        if (hit_step) *hit_step = i;
+        if (hit_uv) { hit_uv[0] = u; hit_uv[1] = v; }
+        break;
+      }
+      // The step stays a miss while any failing test stays clear of its threshold.
+      double mf = 0.0;
+      if (!a) mf = std::max(mf, ma);
+      if (!b) mf = std::max(mf, mbb);
+      if (!c) mf = std::max(mf, mc);
+      m = std::min(m, mf);
+    }
+    prev = proj;
+  }
+  if (margin) *margin = m;
+}
+
 // refl_in (optional, 3 doubles): the prefiltered radiance to use instead of
 // the cube lookup (the learned stand-in's value), ignored when refl_in[0] is
 // NaN.  query_out (optional, 4 doubles): the reflection direction and
@@ -749,8 +924,23 @@ inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_o
   dvec3 refl = (refl_in && !std::isnan(refl_in[0]))
                    ? dvec3{refl_in[0], refl_in[1], refl_in[2]}
                    : cube_sample(in.prefiltered, R, roughness * (in.prefiltered.mips - 1));
+  if (in.reflection) {
+    // DeferredPass.frag:59-68: the reflection buffer at LOD roughness*(mips-1)
+    // replaces the term where its alpha a >= 0.01 by mix(term, rgb / a, a).
+    double rb[4];
+    reflection_sample(*in.reflection, px, py, roughness * (in.reflection->mips - 1), rb);
+    if (margin) *margin = std::min(*margin, std::fabs(rb[3] - 0.01));
+    if (!(rb[3] < 0.01)) {
+      const double a = rb[3];
+      refl = refl * (1.0 - a) + dvec3{rb[0] / a, rb[1] / a, rb[2] / a} * a;
+    }
+  }
   dvec3 irr = cube_sample(in.irradiance, N, 0.0);
-  if (in.ssao_on) ao = ssao(in.position, in.W, in.H, px, py, P, N, in.view, in.proj, in.ssao, margin);
+  if (in.ssao_on) {
+    double ms = 1e30;
+    ao = ssao(in.position, in.W, in.H, px, py, P, N, in.view, in.proj, in.ssao, &ms);
+    if (margin) *margin = std::min(*margin, ms);
+  }
   if (ao_out) *ao_out = ao;
   dvec3 c = pbr_material(P, I, N, base, refl, irr, metallic, roughness, ao, in.lut, in.lights,
                          in.light_count);

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "oracle.hpp"
 #include "oracle_mlp.hpp"
@@ -222,19 +223,11 @@ struct oracle_resolve_args {
   int ssao_samples;
   float ssao_radius;
   float ssao_bias;
+  const float* reflection;  // optional RGBA32F reflection buffer, all levels
+  int refl_width, refl_height, refl_mips;
 };
 
-// refl_in (optional, 3 per pixel): prefiltered radiance replacing the cube
-// lookup where refl_in[3 i] is not NaN; out_query (optional, 4 per pixel):
-// reflection direction and roughness of each pixel's lookup.
-void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int count, const double* refl_in,
-                       double* out3, double* out_ao, double* out_margin, double* out_query) {
-  std::vector<Light> lights(a->light_count);
-  for (int i = 0; i < a->light_count; ++i) {
-    const float* l = a->lights + 8 * i;
-    lights[i].position = {l[0], l[1], l[2]};
-    lights[i].emission = {l[4], l[5], l[6]};
-  }
+static ResolveInputs resolve_inputs(const oracle_resolve_args* a, const std::vector<Light>& lights, ReflChain* chain) {
   ResolveInputs in;
   in.position = a->position;
   in.normal = a->normal;
@@ -242,7 +235,7 @@ void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int coun
   in.mro = a->mro;
   in.W = a->width;
   in.H = a->height;
-  in.env = &static_cast<OracleEnv*>(a->env)->pyr;
+  in.env = a->env ? &static_cast<OracleEnv*>(a->env)->pyr : nullptr;
   in.prefiltered = {a->prefiltered, a->pref_size, a->pref_mips};
   in.irradiance = {a->irradiance, a->irr_size, 1};
   in.lut = {a->lut, a->lut_size};
@@ -256,6 +249,31 @@ void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int coun
   in.tonemap = a->tonemap != 0;
   in.ssao_on = a->ssao_on != 0;
   in.ssao = {a->ssao_kernel, a->ssao_noise, a->ssao_samples, a->ssao_radius, a->ssao_bias};
+  if (a->reflection) {
+    *chain = {a->reflection, a->refl_width, a->refl_height, a->refl_mips};
+    in.reflection = chain;
+  }
+  return in;
+}
+
+static std::vector<Light> resolve_lights(const oracle_resolve_args* a) {
+  std::vector<Light> lights(a->light_count);
+  for (int i = 0; i < a->light_count; ++i) {
+    const float* l = a->lights + 8 * i;
+    lights[i].position = {l[0], l[1], l[2]};
+    lights[i].emission = {l[4], l[5], l[6]};
+  }
+  return lights;
+}
+
+// refl_in (optional, 3 per pixel): prefiltered radiance replacing the cube
+// lookup where refl_in[3 i] is not NaN; out_query (optional, 4 per pixel):
+// reflection direction and roughness of each pixel's lookup.
+void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int count, const double* refl_in,
+                       double* out3, double* out_ao, double* out_margin, double* out_query) {
+  std::vector<Light> lights = resolve_lights(a);
+  ReflChain chain;
+  ResolveInputs in = resolve_inputs(a, lights, &chain);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < count; ++i) {
     double ao, margin;
@@ -270,6 +288,40 @@ void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int coun
 void oracle_resolve(const oracle_resolve_args* a, const int* pixels, int count, double* out3, double* out_ao,
                     double* out_margin) {
   oracle_resolve_ex(a, pixels, count, nullptr, out3, out_ao, out_margin, nullptr);
+}
+
+// Screen-space reflection of pixels[:, (x, y)]: out4 rgba, margin, hit step
+// (-1 miss, -2 uncovered), hit uv (optional, 2 per pixel).
+void oracle_ssr(const oracle_resolve_args* a, const int* pixels, int count, double* out4, double* out_margin,
+                int* out_step, double* out_uv) {
+  std::vector<Light> lights = resolve_lights(a);
+  ReflChain chain;
+  ResolveInputs in = resolve_inputs(a, lights, &chain);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < count; ++i) {
+    double margin, uv[2] = {std::nan(""), std::nan("")};
+    int step;
+    ssr_pixel(in, pixels[2 * i], pixels[2 * i + 1], out4 + 4 * i, &margin, &step, uv);
+    if (out_margin) out_margin[i] = margin;
+    if (out_step) out_step[i] = step;
+    if (out_uv) { out_uv[2 * i] = uv[0]; out_uv[2 * i + 1] = uv[1]; }
+  }
+}
+
+// One level of the glossy convolution: dst (dh x dw x 4 doubles) from the
+// RGBA32F level src (sh x sw).
+void oracle_convolve_level(const float* src, int sw, int sh, int dw, int dh, int vertical, double* dst) {
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < dh; ++y)
+    for (int x = 0; x < dw; ++x)
+      convolve_texel(src, sw, sh, dw, dh, x, y, vertical != 0, dst + 4 * (static_cast<size_t>(y) * dw + x));
+}
+
+// textureLod of an RGBA32F reflection buffer at the centres of pixels[:, (x, y)].
+void oracle_reflection_sample(const float* data, int W, int H, int mips, const int* pixels, const double* lods,
+                              int count, double* out4) {
+  ReflChain c{data, W, H, mips};
+  for (int i = 0; i < count; ++i) reflection_sample(c, pixels[2 * i], pixels[2 * i + 1], lods[i], out4 + 4 * i);
 }
 
 // pbrMaterial for one shading point.  vec: pos, V, N, base, refl, irr (18

@@ -22,7 +22,8 @@ class ResolveArgs(C.Structure):
                 ("irr_size", i32), ("lut", vp), ("lut_size", i32), ("lights", vp), ("light_count", i32),
                 ("projection", vp), ("inverse_projection", vp), ("view", vp), ("inverse_view", vp),
                 ("exposure", C.c_float), ("tonemap", i32), ("ssao_on", i32), ("ssao_kernel", vp),
-                ("ssao_noise", vp), ("ssao_samples", i32), ("ssao_radius", C.c_float), ("ssao_bias", C.c_float)]
+                ("ssao_noise", vp), ("ssao_samples", i32), ("ssao_radius", C.c_float), ("ssao_bias", C.c_float),
+                ("reflection", vp), ("refl_width", i32), ("refl_height", i32), ("refl_mips", i32)]
 
 
 _L = None
@@ -62,6 +63,9 @@ def lib() -> C.CDLL:
         L.oracle_resolve.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp]
         L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
+        L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp]
+        L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
+        L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
         L.oracle_mlp_param_count.restype = i32
         L.oracle_mlp_encode.argtypes = [vp, i32, vp]
         L.oracle_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
@@ -238,13 +242,9 @@ class Env:
 
 
 # ----------------------------------------------------------------- resolve
-def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer=None, refl=None,
-            query: bool = False):
-    """Oracle resolve of pixels[:, (x, y)] for a synth.Scene and IBL views
-    (numpy dict with prefiltered / irradiance / lut arrays and sizes).
-    Returns (rgb, ao, decision margin), plus the (R, roughness) query of each
-    pixel's prefiltered lookup when `query`.  `refl` (n, 3): prefiltered
-    radiance replacing the cube lookup where refl[i, 0] is not NaN."""
+def _resolve_args(scene, views: dict, env, tonemap: bool, gbuffer, reflection, keep: list) -> ResolveArgs:
+    """ResolveArgs over numpy copies that `keep` holds alive.  `reflection`:
+    (float32 array of every level, (texels, 4), mips) or None."""
     planes = gbuffer if gbuffer is not None else (scene.position, scene.normal, scene.albedo, scene.material)
     planes = [_c(p, np.float32) for p in planes]
     pref = _c(views["prefiltered"], np.float32)
@@ -255,11 +255,27 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
     mats = [_c(g[16 * k:16 * k + 16], np.float32) for k in (0, 1, 2, 4)]
     kern = _c(scene.ssao_kernel, np.float32)
     noise = _c(scene.ssao_noise, np.float32)
-    a = ResolveArgs(_p(planes[0]), _p(planes[1]), _p(planes[2]), _p(planes[3]), scene.width, scene.height,
-                    env.handle, _p(pref), views["pref_size"], views["pref_mips"], _p(irr), views["irr_size"],
-                    _p(lut), views["lut_size"], _p(lights), lights.shape[0], _p(mats[0]), _p(mats[1]),
-                    _p(mats[2]), _p(mats[3]), float(g[101]), 1 if tonemap else 0, 1 if scene.ssao_enabled else 0,
-                    _p(kern), _p(noise), kern.shape[0], scene.ssao_radius, scene.ssao_bias)
+    chain, mips = (None, 0) if reflection is None else (_c(reflection[0], np.float32), int(reflection[1]))
+    keep += [planes, pref, irr, lut, lights, g, mats, kern, noise, chain]
+    return ResolveArgs(_p(planes[0]), _p(planes[1]), _p(planes[2]), _p(planes[3]), scene.width, scene.height,
+                       env.handle if env is not None else None, _p(pref), views["pref_size"], views["pref_mips"],
+                       _p(irr), views["irr_size"], _p(lut), views["lut_size"], _p(lights), lights.shape[0],
+                       _p(mats[0]), _p(mats[1]), _p(mats[2]), _p(mats[3]), float(g[101]), 1 if tonemap else 0,
+                       1 if scene.ssao_enabled else 0, _p(kern), _p(noise), kern.shape[0], scene.ssao_radius,
+                       scene.ssao_bias, _p(chain) if chain is not None else None, scene.width, scene.height, mips)
+
+
+def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer=None, refl=None,
+            query: bool = False, reflection=None):
+    """Oracle resolve of pixels[:, (x, y)] for a synth.Scene and IBL views
+    (numpy dict with prefiltered / irradiance / lut arrays and sizes).
+    Returns (rgb, ao, decision margin), plus the (R, roughness) query of each
+    pixel's prefiltered lookup when `query`.  `refl` (n, 3): prefiltered
+    radiance replacing the cube lookup where refl[i, 0] is not NaN.
+    `reflection`: (levels as a (texels, 4) float32 array, mips), the
+    reflection buffer blended into the prefiltered term."""
+    keep = []
+    a = _resolve_args(scene, views, env, tonemap, gbuffer, reflection, keep)
     px = _c(pixels, np.int32)
     rgb = np.empty((px.shape[0], 3), np.float64)
     ao = np.empty(px.shape[0], np.float64)
@@ -269,6 +285,40 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
     lib().oracle_resolve_ex(C.byref(a), _p(px), px.shape[0], _p(r_in) if r_in is not None else None, _p(rgb),
                             _p(ao), _p(margin), _p(q) if q is not None else None)
     return (rgb, ao, margin, q) if query else (rgb, ao, margin)
+
+
+def ssr(scene, views: dict, pixels, gbuffer=None):
+    """Oracle screen-space reflection of pixels[:, (x, y)]: (rgba, decision
+    margin, hit step (-1 miss, -2 uncovered), hit uv)."""
+    keep = []
+    a = _resolve_args(scene, views, None, False, gbuffer, None, keep)
+    px = _c(pixels, np.int32)
+    n = px.shape[0]
+    rgba = np.empty((n, 4), np.float64)
+    margin = np.empty(n, np.float64)
+    step = np.empty(n, np.int32)
+    uv = np.empty((n, 2), np.float64)
+    lib().oracle_ssr(C.byref(a), _p(px), n, _p(rgba), _p(margin), _p(step), _p(uv))
+    return rgba, margin, step, uv
+
+
+def convolve_level(src, dw: int, dh: int, vertical: bool) -> np.ndarray:
+    """One glossy-convolution level (dh, dw, 4) from src (sh, sw, 4)."""
+    s = _c(src, np.float32)
+    sh, sw = s.shape[:2]
+    out = np.empty((dh, dw, 4), np.float64)
+    lib().oracle_convolve_level(_p(s), sw, sh, dw, dh, 1 if vertical else 0, _p(out))
+    return out
+
+
+def reflection_sample(chain, width: int, height: int, mips: int, pixels, lods) -> np.ndarray:
+    """textureLod of the reflection buffer (all levels, (texels, 4)) at pixel centres."""
+    c = _c(chain, np.float32)
+    px = _c(pixels, np.int32)
+    lo = _c(lods, np.float64)
+    out = np.empty((px.shape[0], 4), np.float64)
+    lib().oracle_reflection_sample(_p(c), width, height, mips, _p(px), _p(lo), px.shape[0], _p(out))
+    return out
 
 
 def pbr_material(pos, V, N, base, refl, irr, metallic: float, roughness: float, ao: float, lut, lut_size: int,

@@ -50,6 +50,36 @@ def test_host_queries_without_gpu():
     assert lib.althea_status_string(1) == b"invalid argument"
 
 
+def test_reflection_queries_and_argument_checks_without_gpu():
+    """Reflection-buffer sizes, and the argument checks that run before any
+    device work (dummy device pointers are never dereferenced)."""
+    import ctypes as C
+
+    from althea_amd import _lib
+
+    lib = _lib.lib()
+    assert lib.althea_reflection_texels(40, 24, 3) == 40 * 24 + 20 * 12 + 10 * 6
+    assert lib.althea_reflection_texels(37, 5, 5) == 37 * 5 + 18 * 2 + 9 * 1 + 4 * 1 + 2 * 1
+    assert lib.althea_reflection_mip_offset(40, 24, 2) == 40 * 24 + 20 * 12
+    assert lib.althea_reflection_texels(40, 24, 0) == 0
+    dummy = 0x1000
+    g = _lib.GBufferViewC(dummy, dummy, dummy, dummy, 64, 32, 0)
+    v = _lib.IBLViewsC()
+    v.environment, v.env_width, v.env_height = dummy, 64, 32
+    v.prefiltered, v.prefiltered_size, v.prefiltered_mips = dummy, 16, 3
+    v.irradiance, v.irradiance_size, v.brdf_lut, v.brdf_lut_size = dummy, 8, dummy, 16
+    glob = _lib.GlobalsC()
+    bad_extent = _lib.ReflectionViewC(dummy, 64, 31, 5, 0)
+    assert lib.althea_ssr_capture(C.byref(g), C.byref(v), None, 0, C.byref(glob), C.byref(bad_extent), None) == 1
+    assert b"reflection buffer 64x31" in lib.althea_last_error()
+    bad_mips = _lib.ReflectionViewC(dummy, 64, 32, 9, 0)
+    assert lib.althea_reflection_convolve(C.byref(bad_mips), None) == 1
+    bad_fmt = _lib.ReflectionViewC(dummy, 64, 32, 5, 7)
+    assert lib.althea_deferred_resolve_reflect(C.byref(g), C.byref(v), None, 0, C.byref(glob), None,
+                                               C.byref(bad_fmt), 0, dummy, None, None) == 1
+    assert lib.althea_reflection_convolve(None, None) == 1
+
+
 def test_facade_header_compiles():
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
                     "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "facade_demo.cpp")], check=True)

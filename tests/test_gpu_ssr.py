@@ -1,0 +1,192 @@
+"""GPU parity of the screen-space reflection path against the FP64 oracle,
+through the C ABI:
+
+- althea_ssr_capture (Shaders/SSR.frag) on the synthetic heightfield and on
+  the analytic floor + wall scene: the hit decision per pixel and the shaded
+  colour of each hit;
+- althea_reflection_convolve (Shaders/SSRGlossyConvolve.comp): every level
+  against the oracle's convolution of the GPU's own level below it, FP32 and
+  FP16 buffers, a 4K buffer and extents that clamp to 1;
+- althea_deferred_resolve_reflect (DeferredPass.frag:59-68): against the
+  oracle resolve reading the same reflection buffer; a zero buffer reproduces
+  the plain resolve bit for bit.
+
+The march is a chain of discrete decisions (off-screen test, sign change,
+cone, facing).  The oracle reports each pixel's smallest normalised decision
+margin; pixels within MARGIN of a threshold may flip between FP32 and FP64
+and are only counted.
+"""
+import numpy as np
+import pytest
+
+import althea_amd as aa
+from tests import oracle_lib as ol
+from tests.parity import RADIANCE_FLOOR, rel_err, summary
+from tests.test_oracle_ssr import wall_scene
+
+pytestmark = pytest.mark.gpu
+
+SEED0 = 0x414C0000
+MARGIN = 2e-5
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.fixture(scope="module")
+def ibl(torch):
+    env = aa.synth.env(seed=SEED0 + 100, width=512, height=256, sigma=0.0, suns=4, peak_min=100.0,
+                       peak_max=100.0, sun_sigma_deg=8.0)
+    desc = aa.IBLDesc(prefiltered_size=128, prefiltered_mips=5)
+    res = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    torch.cuda.synchronize()
+    return res, ol.Env(env)
+
+
+def _capture(torch, scene, ibl_res, fp16_buffer=False, fp16_gbuffer=False, mips=1):
+    g = scene.gbuffer_tensors("cuda", fp16=fp16_gbuffer)
+    refl = aa.ReflectionBuffer(scene.width, scene.height, mips=mips, fp16=fp16_buffer)
+    aa.ScreenSpaceReflection.captureReflection(g, ibl_res, scene.lights_tensor("cuda"), scene.globals, refl)
+    torch.cuda.synchronize()
+    return refl
+
+
+def _all_pixels(scene):
+    y, x = np.mgrid[0:scene.height, 0:scene.width]
+    return np.stack([x.ravel(), y.ravel()], 1).astype(np.int32)
+
+
+def _compare_capture(scene, views, got, pix):
+    ref, margin, step, _ = ol.ssr(scene, views, pix)
+    clear = margin > MARGIN
+    hit_ref = step >= 0
+    hit_got = got[:, 3] == 1.0
+    assert np.all((got[:, 3] == 0.0) | (got[:, 3] == 1.0))
+    assert np.all(got[~hit_got] == 0.0)
+    disagree = hit_ref != hit_got
+    both = hit_ref & hit_got & clear
+    err = rel_err(got[both, :3], ref[both, :3], RADIANCE_FLOOR).max(1)
+    return {"pixels": len(pix), "hits": int(hit_ref.sum()), "clear": float(clear.mean()),
+            "disagree_clear": int((disagree & clear).sum()), "disagree_near": int((disagree & ~clear).sum()),
+            "err": summary(err) if both.any() else None}
+
+
+def test_ssr_capture_heightfield_matches_oracle(torch, ibl):
+    ibl_res, _ = ibl
+    scene = aa.synth.scene(seed=SEED0 + 21, width=480, height=270, lights=8)
+    refl = _capture(torch, scene, ibl_res)
+    got = refl.level(0).cpu().numpy().reshape(-1, 4).astype(np.float64)
+    pix = _all_pixels(scene)
+    r = _compare_capture(scene, ibl_res.views_numpy(), got, pix)
+    print("SSR heightfield", r)
+    assert r["hits"] > 2000, r
+    assert r["clear"] > 0.95, r
+    # Decisions agree wherever the oracle is clear of every threshold.
+    assert r["disagree_clear"] <= max(2, r["hits"] // 2000), r
+    assert r["disagree_near"] <= 0.02 * r["hits"] + 5, r
+    # A hit is shaded from bilinear reads at the march's FP32 uv (about
+    # 6e-8, i.e. 3e-5 texel at 480 px): per-texel random albedo, metallic and
+    # roughness turn that into up to 1e-4-level colour differences.
+    assert r["err"]["mean"] < 5e-5 and r["err"]["max"] < 2e-3, r
+
+
+def test_ssr_capture_wall_scene_matches_oracle(torch, ibl):
+    ibl_res, _ = ibl
+    scene, use_f, _, _ = wall_scene(W=320, H=180)
+    scene.lights = np.array([[0.0, 2.0, -4.0, 0.0, 20.0, 20.0, 20.0, 0.0]], np.float32)
+    refl = _capture(torch, scene, ibl_res)
+    got = refl.level(0).cpu().numpy().reshape(-1, 4).astype(np.float64)
+    pix = _all_pixels(scene)
+    r = _compare_capture(scene, ibl_res.views_numpy(), got, pix)
+    print("SSR wall", r)
+    assert r["hits"] > 0.5 * use_f.sum(), r
+    assert r["disagree_clear"] == 0, r
+    assert r["err"]["max"] < 1e-4, r
+
+
+def test_ssr_capture_fp16_buffer_and_gbuffer(torch, ibl):
+    """FP16 storage of the buffer rounds the FP32 capture once; an FP16
+    G-buffer marches over FP16 positions (checked for coverage and rate)."""
+    ibl_res, _ = ibl
+    scene = aa.synth.scene(seed=SEED0 + 22, width=320, height=180, lights=4)
+    f32 = _capture(torch, scene, ibl_res).level(0).cpu().numpy()
+    f16 = _capture(torch, scene, ibl_res, fp16_buffer=True).level(0).float().cpu().numpy()
+    want = f32.astype(np.float16).astype(np.float32)
+    np.testing.assert_array_equal(f16, want)
+    g16 = _capture(torch, scene, ibl_res, fp16_gbuffer=True).level(0).cpu().numpy()
+    covered = scene.position[..., 3] != 0
+    assert np.all(g16[~covered] == 0.0)
+    h32, h16 = f32[..., 3] == 1.0, g16[..., 3] == 1.0
+    assert abs(int(h32.sum()) - int(h16.sum())) < 0.05 * h32.sum() + 10
+
+
+def _random_level0(torch, W, H, seed, fp16):
+    rng = np.random.default_rng(seed)
+    rgb = np.exp(rng.normal(0.0, 1.5, (H, W, 3))).astype(np.float32)
+    a = (rng.uniform(size=(H, W)) < 0.6).astype(np.float32)
+    lvl = np.concatenate([rgb * a[..., None], a[..., None]], -1)
+    refl = aa.ReflectionBuffer(W, H, mips=5, fp16=fp16)
+    refl.level(0).copy_(torch.from_numpy(lvl).to(refl.data.dtype).cuda())
+    return refl
+
+
+@pytest.mark.parametrize("W,H,fp16", [(3840, 2160, True), (3840, 2160, False), (333, 97, False), (37, 5, True),
+                                      (37, 5, False)])
+def test_convolve_levels_match_oracle(torch, W, H, fp16):
+    refl = _random_level0(torch, W, H, seed=W + H, fp16=fp16)
+    aa.ScreenSpaceReflection.convolveReflectionBuffer(refl)
+    torch.cuda.synchronize()
+    for m in range(1, refl.mips):
+        src = refl.level(m - 1).float().cpu().numpy()
+        got = refl.level(m).float().cpu().numpy().astype(np.float64)
+        dw, dh = refl.extent(m)
+        ref = ol.convolve_level(src, dw, dh, vertical=bool(m & 1))
+        err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-2)
+        # FP16 storage rounds once (2^-11 relative); FP32 carries the FP32
+        # arithmetic of 7 bilinear taps (tap coordinates to 5e-7 texel times
+        # the neighbour contrast of a log-normal source).
+        bound = 2.0 ** -11 + 1e-5 if fp16 else 1e-5
+        assert err.max() < bound, (m, summary(err))
+
+
+def test_resolve_reflect_zero_buffer_is_the_plain_resolve(torch, ibl):
+    ibl_res, _ = ibl
+    scene = aa.synth.scene(seed=SEED0 + 23, width=320, height=180, lights=8)
+    g = scene.gbuffer_tensors("cuda")
+    lights = scene.lights_tensor("cuda")
+    plain = aa.DeferredRendering.resolve(g, ibl_res, lights, scene.globals, scene.ssao)
+    refl = aa.ReflectionBuffer(scene.width, scene.height, mips=5)
+    got = aa.DeferredRendering.resolve(g, ibl_res, lights, scene.globals, scene.ssao, reflection=refl)
+    torch.cuda.synchronize()
+    assert torch.equal(plain, got)
+
+
+@pytest.mark.parametrize("fp16", [True, False])
+def test_resolve_reflect_full_pipeline_matches_oracle(torch, ibl, fp16):
+    ibl_res, oenv = ibl
+    scene = aa.synth.scene(seed=SEED0 + 24, width=480, height=270, lights=8)
+    g = scene.gbuffer_tensors("cuda")
+    lights = scene.lights_tensor("cuda")
+    refl = aa.ReflectionBuffer(scene.width, scene.height, mips=5, fp16=fp16)
+    aa.ScreenSpaceReflection.captureReflection(g, ibl_res, lights, scene.globals, refl)
+    aa.ScreenSpaceReflection.convolveReflectionBuffer(refl)
+    out = aa.DeferredRendering.resolve(g, ibl_res, lights, scene.globals, scene.ssao, reflection=refl)
+    torch.cuda.synchronize()
+    chain = refl.data.float().cpu().numpy()
+    pix = _all_pixels(scene)[::7]
+    ref, _, margin = ol.resolve(scene, ibl_res.views_numpy(), oenv, pix, reflection=(chain, refl.mips))
+    got = out.cpu().numpy()[pix[:, 1] * scene.width + pix[:, 0], :3]
+    clear = margin > 2e-3
+    err = rel_err(got, ref, RADIANCE_FLOOR).max(1)
+    s = summary(err[clear])
+    print("resolve with reflection buffer", s, "clear", clear.mean())
+    assert clear.mean() > 0.6
+    assert s["mean"] < 1e-5 and s["max"] < 1e-3, s
+    # The buffer changed the frame where it has coverage.
+    plain = aa.DeferredRendering.resolve(g, ibl_res, lights, scene.globals, scene.ssao).cpu().numpy()
+    assert np.abs(plain - out.cpu().numpy()).max() > 0.0
