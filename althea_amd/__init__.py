@@ -317,6 +317,9 @@ class ReflectionBuffer:
         self.width, self.height, self.mips, self.fp16 = width, height, mips, fp16
         n = lib().althea_reflection_texels(width, height, mips)
         self.data = torch.zeros((n, 4), dtype=torch.float16 if fp16 else torch.float32, device="cuda")
+        # The capture's depth-plane workspace (the reference's depth attachment).
+        ws = lib().althea_ssr_workspace_bytes(width, height)
+        self.workspace = torch.empty(max(ws, 4), dtype=torch.uint8, device="cuda")
 
     def view(self) -> _lib.ReflectionViewC:
         return _lib.ReflectionViewC(_ptr(self.data), self.width, self.height, self.mips,
@@ -346,7 +349,8 @@ class ScreenSpaceReflection:
         rv = reflection.view()
         n = int(lights.shape[0]) if lights is not None else 0
         check(lib().althea_ssr_capture(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
-                                       C.byref(globals_), C.byref(rv), _stream(stream)), "althea_ssr_capture")
+                                       C.byref(globals_), C.byref(rv), _ptr(reflection.workspace), _stream(stream)),
+              "althea_ssr_capture")
         return reflection
 
     @staticmethod

@@ -109,8 +109,9 @@ PROTOTYPES = {
                                       C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),
     "althea_reflection_texels": (sz, [i32, i32, i32]),
     "althea_reflection_mip_offset": (sz, [i32, i32, i32]),
+    "althea_ssr_workspace_bytes": (sz, [i32, i32]),
     "althea_ssr_capture": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
-                                 C.POINTER(ReflectionViewC), vp]),
+                                 C.POINTER(ReflectionViewC), vp, vp]),
     "althea_reflection_convolve": (i32, [C.POINTER(ReflectionViewC), vp]),
     "althea_deferred_resolve_reflect": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                               C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),

@@ -464,15 +464,22 @@ size_t althea_reflection_texels(int width, int height, int mips) {
   return althea_reflection_mip_offset(width, height, mips);
 }
 
+size_t althea_ssr_workspace_bytes(int width, int height) {
+  if (width < 2 || height < 1) return 0;
+  return ssr_workspace_bytes(width, height);
+}
+
 althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                  const althea_point_light* d_lights, int light_count, const althea_globals* globals,
-                                 const althea_reflection_view* reflection, hipStream_t stream) {
+                                 const althea_reflection_view* reflection, void* d_workspace, hipStream_t stream) {
   althea_status st =
       resolve_args_ok(g, ibl, d_lights, light_count, globals, nullptr, reflection ? reflection->data : nullptr);
   if (st != ALTHEA_OK) return st;
   st = reflection_ok(reflection, g, "ssr_capture");
   if (st != ALTHEA_OK) return st;
-  return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, stream);
+  ALTHEA_REQUIRE(g->width >= 2, "ssr_capture: width %d (at least 2)", g->width);
+  ALTHEA_REQUIRE(d_workspace, "ssr_capture: null workspace");
+  return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, d_workspace, stream);
 }
 
 althea_status althea_reflection_convolve(const althea_reflection_view* reflection, hipStream_t stream) {

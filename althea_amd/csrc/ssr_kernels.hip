@@ -1,10 +1,8 @@
 // Screen-space reflection and the glossy reflection buffer for gfx950.
 //
 // ssr_capture_kernel (Shaders/SSR.frag:80-149 with SSR.vert:15-22): one lane
-// per pixel in 64 x 4 pixel tiles, so the 64 lanes of a wave march parallel
-// rays one pixel apart and their bilinear position reads at each step share
-// cache lines.  A lane leaves the march at its hit, its exit from the screen
-// or step 128; the wave leaves when no lane is marching.  The hit shading
+// per pixel (tile order below).  A lane leaves the march at its hit, its exit
+// from the screen or step 128; the wave leaves when no lane is marching.  The hit shading
 // (SSR.frag:46-78, pbrMaterial with the IBL state and the point lights) runs
 // once after the march for all hit lanes of the wave together.
 //
@@ -12,12 +10,22 @@
 // by Src/ReflectionBuffer.cpp:163-277): one lane per texel of level m, 7
 // bilinear taps into level m-1 along one axis.
 //
+// Positions along the march are rebuilt from the depth buffer as the
+// reference does (SSR.frag:38-41, Misc/ReconstructPosition.glsl): bilinear
+// depth (the LINEAR / CLAMP_TO_EDGE depth sampler, DeferredRendering.cpp:67-69),
+// then the point at that view depth on the pixel's view ray.  This G-buffer
+// carries world positions instead of a depth attachment, so ssr_depth_kernel
+// first writes the depth plane: 1 / view distance per pixel, which is affine
+// in the [0, 1] depth the rasterizer stores (filtering either gives the same
+// result in exact arithmetic, and 1/z keeps full FP32 relative precision where
+// [0, 1] depth crowds towards 1); uncovered pixels hold the far plane (depth
+// cleared to 1, DeferredRendering.cpp:105-106).  4 bytes per step instead of a
+// 64-byte filtered position read.  The near and far planes come from the
+// projection; ReconstructPosition.glsl hard-codes the default camera's 0.01
+// and 1000.
+//
 // G-buffer coverage is position.w != 0 (the flag the resolve uses,
-// DeferredPass.frag:44); the reference's SSR tests normal.a (SSR.frag:131)
-// and rebuilds positions from depth (SSR.frag:38-41,
-// Misc/ReconstructPosition.glsl); this G-buffer stores world positions
-// (DeferredRendering.cpp:23-35), which the march filters bilinearly as the
-// reference's sampler filters depth.
+// DeferredPass.frag:44); the reference's SSR tests normal.a (SSR.frag:131).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -49,8 +57,12 @@ struct SsrParams {
   const float2* lut;
   int lut_size;
   int light_count;
-  float ray0[3], rayx[3], rayy[3];  // view ray = ray0 + rayx (px+0.5) + rayy (py+0.5)
+  float ray0[3], rayu[3], rayv[3];  // view ray at uv: ray0 + rayu u + rayv v
   float vp[16];                     // projection * view, column-major
+  float view_z[4];                  // row 2 of the view matrix
+  float eye[3];
+  float inv_far;                    // depth-plane value of an uncovered pixel
+  float* depth;                     // W*H + 1 floats: 1 / view distance, then a zero pad
   ReflChain out;                    // level 0 is written
 };
 
@@ -76,23 +88,112 @@ __device__ __forceinline__ float4 gb_bilinear(const void* plane, int W, int H, f
   return lerp4(lerp4(t00, t10, ax), lerp4(t01, t11, ax), ay);
 }
 
+// Depth plane: 1 / view distance of each covered pixel, 1 / far elsewhere,
+// and a zero after the last texel (read with weight 0 by the pair loads).
+template <int kFmt>
+__global__ __launch_bounds__(256) void ssr_depth_kernel(const SsrParams p) {
+  const int n = p.width * p.height;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    p.depth[n] = 0.0f;
+    return;
+  }
+  const float4 q = gb_load<kFmt>(p.position, i);
+  const float vz = fmaf(p.view_z[0], q.x, fmaf(p.view_z[1], q.y, fmaf(p.view_z[2], q.z, p.view_z[3])));
+  p.depth[i] = q.w != 0.0f ? -1.0f / vz : p.inv_far;
+}
+
+// Bilinear CLAMP_TO_EDGE depth at uv: two 8-byte row reads (depth_tap),
+// then the blend (depth_value).  An index clamped at the left or right edge
+// becomes the pair (0, 1) with weight 0 or the pair (W-2, W-1) with weight 1,
+// which reads the same texel (W >= 2).  Row offsets use 24-bit multiplies
+// (W, H < 2^24).
+struct DepthTap {
+  float a0, b0, a1, b1, ax, ay;
+};
+
+__device__ __forceinline__ DepthTap depth_tap(const float* __restrict__ d, int W, int H, float u, float v) {
+  const float x = fmaf(u, static_cast<float>(W), -0.5f), y = fmaf(v, static_cast<float>(H), -0.5f);
+  const float fx = floorf(x), fy = floorf(y);
+  DepthTap t;
+  t.ax = x - fx;
+  t.ay = y - fy;
+  int x0 = static_cast<int>(fx);
+  const int iy = static_cast<int>(fy);
+  if (x0 < 0) x0 = 0, t.ax = 0.0f;
+  if (x0 > W - 2) x0 = W - 2, t.ax = 1.0f;
+  const int y0 = min(max(iy, 0), H - 1), y1 = min(max(iy + 1, 0), H - 1);
+  const float* r0 = d + (__umul24(static_cast<unsigned>(y0), static_cast<unsigned>(W)) + x0);
+  const float* r1 = d + (__umul24(static_cast<unsigned>(y1), static_cast<unsigned>(W)) + x0);
+  t.a0 = r0[0];
+  t.b0 = r0[1];
+  t.a1 = r1[0];
+  t.b1 = r1[1];
+  return t;
+}
+
+__device__ __forceinline__ float depth_value(const DepthTap& t) {
+  const float top = fmaf(t.b0 - t.a0, t.ax, t.a0), bot = fmaf(t.b1 - t.a1, t.ax, t.a1);
+  return fmaf(bot - top, t.ay, top);
+}
+
+// View ray through uv (SSR.vert:20, interpolated across the screen).
+__device__ __forceinline__ f3 view_ray_at(const SsrParams& p, float u, float v) {
+  return mk(fmaf(p.rayv[0], v, fmaf(p.rayu[0], u, p.ray0[0])), fmaf(p.rayv[1], v, fmaf(p.rayu[1], u, p.ray0[1])),
+            fmaf(p.rayv[2], v, fmaf(p.rayu[2], u, p.ray0[2])));
+}
+
+__device__ __forceinline__ float view_z_of(const SsrParams& p, f3 r) {
+  return fmaf(p.view_z[0], r.x, fmaf(p.view_z[1], r.y, p.view_z[2] * r.z));
+}
+
+// reconstructPosition (Misc/ReconstructPosition.glsl:4-22): the point at
+// view distance 1 / s on the view ray through uv.
+__device__ __forceinline__ f3 reconstruct(const SsrParams& p, float u, float v, float s) {
+  const f3 ray = view_ray_at(p, u, v);
+  const float t = -rcp_fast(s * view_z_of(p, ray));
+  return mk(fmaf(ray.x, t, p.eye[0]), fmaf(ray.y, t, p.eye[1]), fmaf(ray.z, t, p.eye[2]));
+}
+
+// Tiles: 16 x 16 pixels per block, each wave an 8 x 8 quadrant, so a wave's
+// 64 rays start close together and their filtered depth reads share cache
+// lines.  The march reads far along the frame (mostly up it), so L2 reuse
+// decides the speed: blocks are dealt to the 8 XCDs in turn (block b runs on
+// XCD b % 8), so XCD x takes the tiles of column band x, walking it column by
+// column.  Each XCD's L2 then holds one band's depth (about 4 MB at 4K)
+// instead of the whole frame's.  Measured at 4K: 64 x 4 tiles in launch order
+// 4.6 ms, column bands 3.5 ms, bands of 16 x 16 tiles 2.5 ms.
+constexpr int kSsrTile = 16;
+constexpr int kXcds = 8;
+
+__host__ __device__ inline int ssr_band_tiles(int tiles_x, int tiles_y) {
+  return ((tiles_x + kXcds - 1) / kXcds) * tiles_y;
+}
+
 template <int kFmt>
 __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, const float4* __restrict__ lights) {
-  const int tiles_x = (p.width + 63) / 64;
-  const int px = (blockIdx.x % tiles_x) * 64 + (threadIdx.x & 63);
-  const int py = (blockIdx.x / tiles_x) * 4 + (threadIdx.x >> 6);
+  const int tiles_x = (p.width + kSsrTile - 1) / kSsrTile, tiles_y = (p.height + kSsrTile - 1) / kSsrTile;
+  const int band = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  const int c0 = band * tiles_x / kXcds, bw = (band + 1) * tiles_x / kXcds - c0;
+  if (j >= bw * tiles_y) return;  // the band has fewer tiles than the widest one
+  const int tx = c0 + j / tiles_y, ty = j % tiles_y;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = tx * kSsrTile + (lane & 7) + 8 * (wave & 1);
+  const int py = ty * kSsrTile + (lane >> 3) + 8 * (wave >> 1);
   const bool inside = px < p.width && py < p.height;
   const int idx = min(py, p.height - 1) * p.width + min(px, p.width - 1);
   const float4 pos = gb_load<kFmt>(p.position, idx);
   const bool covered = inside && pos.w != 0.0f;
   const float4 nrm = covered ? gb_load<kFmt>(p.normal, idx) : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-  const f3 P = mk(pos.x, pos.y, pos.z);
   const f3 N = normalize_fast(mk(nrm.x, nrm.y, nrm.z));
+  const float u0 = (static_cast<float>(px) + 0.5f) / static_cast<float>(p.width);
+  const float v0 = (static_cast<float>(py) + 0.5f) / static_cast<float>(p.height);
+  // The pixel's own position from its depth texel (SSR.frag:136).
+  const f3 P = reconstruct(p, u0, v0, p.depth[idx]);
   // reflect(normalize(view ray), N) (SSR.frag:137).
-  const float cxp = static_cast<float>(px) + 0.5f, cyp = static_cast<float>(py) + 0.5f;
-  const f3 I = normalize_fast(mk(fmaf(p.rayy[0], cyp, fmaf(p.rayx[0], cxp, p.ray0[0])),
-                                 fmaf(p.rayy[1], cyp, fmaf(p.rayx[1], cxp, p.ray0[1])),
-                                 fmaf(p.rayy[2], cyp, fmaf(p.rayx[2], cxp, p.ray0[2]))));
+  const f3 ray0 = view_ray_at(p, u0, v0);
+  const f3 I = normalize_fast(ray0);
   const float ni = 2.0f * dot(N, I);
   const f3 R = mk(fmaf(-ni, N.x, I.x), fmaf(-ni, N.y, I.y), fmaf(-ni, N.z, I.z));
   // March direction in uv towards the projection of P + 10000 R
@@ -101,30 +202,39 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
   const float ex = fmaf(p.vp[0], E.x, fmaf(p.vp[4], E.y, fmaf(p.vp[8], E.z, p.vp[12])));
   const float ey = fmaf(p.vp[1], E.x, fmaf(p.vp[5], E.y, fmaf(p.vp[9], E.z, p.vp[13])));
   const float ew = fmaf(p.vp[3], E.x, fmaf(p.vp[7], E.y, fmaf(p.vp[11], E.z, p.vp[15])));
-  const float u0 = cxp / static_cast<float>(p.width), v0 = cyp / static_cast<float>(p.height);
   const float du = fmaf(0.5f, ex / ew, 0.5f) - u0, dv = fmaf(0.5f, ey / ew, 0.5f) - v0;
   const float sl = kSsrStep * rsq_fast(fmaf(du, du, dv * dv));
-  const float su = du * sl, sv = dv * sl;
-  const f3 perp = normalize_fast(cross(cross(R, N), R));
   // A non-finite march direction (the target projects onto the pixel, or to
   // w = 0) gives no reflection.
-  bool active = covered && isfinite(su) && isfinite(sv);
+  bool active = covered && isfinite(du * sl) && isfinite(dv * sl);
+  const float su = active ? du * sl : 0.0f, sv = active ? dv * sl : 0.0f;
+  const f3 perp = normalize_fast(cross(cross(R, N), R));
+  // Along the march uv, the view ray and its view-space z are affine in the
+  // step number k: uv0 + k step (currentUV += uvStep * stepSize in the
+  // reference, equal in exact arithmetic, one rounding instead of k).
+  const f3 rk = mk(fmaf(p.rayv[0], sv, p.rayu[0] * su), fmaf(p.rayv[1], sv, p.rayu[1] * su),
+                   fmaf(p.rayv[2], sv, p.rayu[2] * su));
+  const float vz0 = view_z_of(p, ray0), vzk = view_z_of(p, rk);
+  const f3 EP = mk(p.eye[0] - P.x, p.eye[1] - P.y, p.eye[2] - P.z);
   bool hit = false;
-  float hu = 0.0f, hv = 0.0f;
+  float hk = 0.0f;
   f3 Q = mk(0.0f, 0.0f, 0.0f), Nh = mk(0.0f, 0.0f, 0.0f);
   float prev = 0.0f;
+  // Lanes that left the march issue no loads (exec-masked): measured faster
+  // than marching every lane to the wave's end, with or without issuing the
+  // next step's loads ahead.
   for (int i = 0; i < kSsrSteps; ++i) {
     if (!__any(active)) break;
     if (active) {
-      // currentUV += uvStep * stepSize, as uv0 + (i + 1) step (same value in
-      // exact arithmetic, one rounding instead of i + 1).
-      const float s = static_cast<float>(i + 1);
-      const float u = fmaf(s, su, u0), v = fmaf(s, sv, v0);
+      const float k = static_cast<float>(i + 1);
+      const float u = fmaf(k, su, u0), v = fmaf(k, sv, v0);
       if (!(u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f)) {
         active = false;  // off screen: SSR.frag:97-99
       } else {
-        const float4 q4 = gb_bilinear<kFmt>(p.position, p.width, p.height, u, v);
-        const f3 dq = mk(q4.x - P.x, q4.y - P.y, q4.z - P.z);
+        const DepthTap tap = depth_tap(p.depth, p.width, p.height, u, v);
+        const f3 ray = mk(fmaf(k, rk.x, ray0.x), fmaf(k, rk.y, ray0.y), fmaf(k, rk.z, ray0.z));
+        const float t = -rcp_fast(depth_value(tap) * fmaf(k, vzk, vz0));
+        const f3 dq = mk(fmaf(ray.x, t, EP.x), fmaf(ray.y, t, EP.y), fmaf(ray.z, t, EP.z));
         const float rl = rsq_fast(dot(dq, dq));
         const float proj = dot(dq, perp) * rl;
         const float f = dot(dq, R) * rl;
@@ -134,16 +244,16 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
           if (dot(nh, R) < 0.0f) {
             hit = true;
             active = false;
-            hu = u;
-            hv = v;
-            Q = mk(q4.x, q4.y, q4.z);
+            hk = k;
             Nh = nh;
+            Q = mk(fmaf(ray.x, t, p.eye[0]), fmaf(ray.y, t, p.eye[1]), fmaf(ray.z, t, p.eye[2]));
           }
         }
         prev = proj;
       }
     }
   }
+  const float hu = fmaf(hk, su, u0), hv = fmaf(hk, sv, v0);
   // environmentLitSample (SSR.frag:46-78): V = the reflected ray, occlusion 1.
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 alb = hit ? gb_bilinear<kFmt>(p.albedo, p.width, p.height, hu, hv) : zero4;
@@ -228,9 +338,13 @@ void refl_chain(const althea_reflection_view& v, ReflChain& c) {
   }
 }
 
+size_t ssr_workspace_bytes(int width, int height) {
+  return (static_cast<size_t>(width) * static_cast<size_t>(height) + 1) * sizeof(float);
+}
+
 althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                  const althea_point_light* lights, int light_count, const althea_globals* globals,
-                                 const althea_reflection_view* reflection, hipStream_t s) {
+                                 const althea_reflection_view* reflection, void* workspace, hipStream_t s) {
   SsrParams p{};
   p.position = g->position;
   p.normal = g->normal;
@@ -248,15 +362,34 @@ althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
   p.lut_size = ibl->brdf_lut_size;
   p.light_count = light_count;
-  view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy);
+  // View ray per unit uv: the pixel-affine form scaled by the extent.
+  float rx[3], ry[3];
+  view_ray_affine(*globals, g->width, g->height, p.ray0, rx, ry);
+  for (int c = 0; c < 3; ++c) {
+    p.rayu[c] = static_cast<float>(static_cast<double>(rx[c]) * g->width);
+    p.rayv[c] = static_cast<float>(static_cast<double>(ry[c]) * g->height);
+  }
   view_projection(*globals, p.vp);
+  for (int c = 0; c < 4; ++c) p.view_z[c] = globals->view[c * 4 + 2];
+  for (int c = 0; c < 3; ++c) p.eye[c] = globals->inverse_view[12 + c];
+  // Far plane of a [0, 1]-depth perspective projection: P[3][2] / (P[2][2] + 1).
+  const double far =
+      static_cast<double>(globals->projection[14]) / (static_cast<double>(globals->projection[10]) + 1.0);
+  p.inv_far = static_cast<float>(1.0 / far);
+  p.depth = static_cast<float*>(workspace);
   refl_chain(*reflection, p.out);
-  const dim3 grid(((g->width + 63) / 64) * ((g->height + 3) / 4));
+  const int n = g->width * g->height;
+  const dim3 dgrid((n + 1 + 255) / 256);
+  const int tiles_x = (g->width + kSsrTile - 1) / kSsrTile, tiles_y = (g->height + kSsrTile - 1) / kSsrTile;
+  const dim3 grid(kXcds * ssr_band_tiles(tiles_x, tiles_y));
   const float4* lights4 = reinterpret_cast<const float4*>(lights);
-  if (g->format == ALTHEA_GBUFFER_FP32)
+  if (g->format == ALTHEA_GBUFFER_FP32) {
+    hipLaunchKernelGGL((ssr_depth_kernel<ALTHEA_GBUFFER_FP32>), dgrid, dim3(256), 0, s, p);
     hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP32>), grid, dim3(256), 0, s, p, lights4);
-  else
+  } else {
+    hipLaunchKernelGGL((ssr_depth_kernel<ALTHEA_GBUFFER_FP16>), dgrid, dim3(256), 0, s, p);
     hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP16>), grid, dim3(256), 0, s, p, lights4);
+  }
   return check_launch("ssr_capture_kernel");
 }
 

@@ -337,17 +337,21 @@ ALTHEA_API size_t althea_reflection_mip_offset(int width, int height, int level)
  * (Src/ScreenSpaceReflection.cpp:59-83) drawing Shaders/SSR.{vert,frag}.  Per
  * pixel with G-buffer coverage (position.w != 0): reflect the view ray about
  * the normal, march 128 steps of 0.005 in uv towards the projection of
- * P + 10000 R, reading the position plane with a bilinear CLAMP_TO_EDGE
- * filter; the first step (after the first) where the ray crosses the plane
- * through R perpendicular to cross(R, N), within acos(0.999) of R, with a
- * hit normal facing the ray, is shaded by pbrMaterial with the IBL state and
+ * P + 10000 R, rebuilding each step's position from the bilinearly filtered
+ * (CLAMP_TO_EDGE) depth of the G-buffer (far plane where uncovered); the
+ * first step (after the first) where the ray crosses the plane through R
+ * perpendicular to cross(R, N), within acos(0.999) of R, with a hit normal
+ * facing the ray, is shaded by pbrMaterial with the IBL state and
  * `light_count` point lights (ambient occlusion 1) and stored as (rgb, 1).
  * Everything else stores (0, 0, 0, 0).  Linear HDR, no tonemap.  The
- * buffer's extent must equal the G-buffer's. */
+ * buffer's extent must equal the G-buffer's (width >= 2).  d_workspace:
+ * althea_ssr_workspace_bytes(width, height) bytes of device memory for the
+ * depth plane, overwritten. */
+ALTHEA_API size_t althea_ssr_workspace_bytes(int width, int height);
 ALTHEA_API althea_status althea_ssr_capture(const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl,
                                             const althea_point_light* d_lights, int light_count,
                                             const althea_globals* globals, const althea_reflection_view* reflection,
-                                            hipStream_t stream);
+                                            void* d_workspace, hipStream_t stream);
 
 /* Levels 1 .. mips-1 of the reflection buffer:
  * ReflectionBuffer::convolveReflectionBuffer (Src/ReflectionBuffer.cpp:163-277)

@@ -711,13 +711,15 @@ struct ResolveInputs {
   SsaoDesc ssao;
   const struct ReflChain* reflection = nullptr;  // DeferredPass.frag:59-68 when set
 };
-inline dvec3 view_ray(const ResolveInputs& in, int px, int py) {
-  double u = (px + 0.5) / in.W, v = (py + 0.5) / in.H;
+inline dvec3 view_ray_uv(const ResolveInputs& in, double u, double v) {
   double ndc[4] = {u * 2.0 - 1.0, v * 2.0 - 1.0, 0.0, 1.0}, e[4];
   in.inv_proj.apply(ndc, e);
   const double* m = in.inv_view.m;
   return {m[0] * e[0] + m[4] * e[1] + m[8] * e[2], m[1] * e[0] + m[5] * e[1] + m[9] * e[2],
           m[2] * e[0] + m[6] * e[1] + m[10] * e[2]};
+}
+inline dvec3 view_ray(const ResolveInputs& in, int px, int py) {
+  return view_ray_uv(in, (px + 0.5) / in.W, (py + 0.5) / in.H);
 }
 inline dvec3 tonemap(dvec3 c, double exposure) {
   return {1.0 - std::exp(-c.x * exposure), 1.0 - std::exp(-c.y * exposure), 1.0 - std::exp(-c.z * exposure)};
@@ -798,10 +800,15 @@ inline void reflection_sample(const ReflChain& c, int px, int py, double lod, do
 
 // ---------------------------------------------------------------------------
 // Screen-space reflection of one pixel: Shaders/SSR.frag:80-149 (march) and
-// :46-78 (hit shading), SSR.vert:15-22 (view ray).  The G-buffer here holds
-// world positions (read with the bilinear filter the reference applies to
-// depth before rebuilding positions, SSR.frag:38-41) and marks coverage in
-// position.w (DeferredPass.frag:44); the reference's SSR tests normal.a.
+// :46-78 (hit shading), SSR.vert:15-22 (view ray).  Positions come from the
+// depth buffer as in SSR.frag:38-41 and Misc/ReconstructPosition.glsl: the
+// depth of a texel is taken here as 1 / view distance (affine in the [0, 1]
+// depth the rasterizer stores, so the bilinear filter gives the same value),
+// the far plane where the G-buffer is uncovered (depth cleared to 1,
+// DeferredRendering.cpp:105-106), near / far from the projection; the point is
+// the one at the filtered view distance on the view ray through uv.  Coverage
+// of the pixel itself is position.w (DeferredPass.frag:44); the reference's
+// SSR tests normal.a.
 // currentUV is uv0 + (i + 1) step (the reference accumulates; equal in exact
 // arithmetic).  A non-finite march step (the target projects onto the pixel
 // or to w = 0) gives no reflection.  out = (rgb, 1) at a hit, else 0.
@@ -812,7 +819,7 @@ inline void reflection_sample(const ReflChain& c, int px, int py, double lod, do
 // FP32/FP64 decision flips from arithmetic error.
 // ---------------------------------------------------------------------------
 inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], double* margin, int* hit_step,
-                      double* hit_uv = nullptr) {
+                      double* hit_uv = nullptr, double* hit_pos = nullptr) {
   const int W = in.W, H = in.H;
   for (int c = 0; c < 4; ++c) out[c] = 0.0;
   double m = 1e30;
@@ -823,7 +830,32 @@ inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], do
   if (p[3] == 0.0f) return;
   if (hit_step) *hit_step = -1;
   const float* n = in.normal + 4 * idx;
-  dvec3 P{p[0], p[1], p[2]};
+  const double far = in.proj.m[14] / (in.proj.m[10] + 1.0);
+  const double* vm = in.view.m;
+  auto depth_at = [&](int x, int y) {
+    const float* q = in.position + 4 * (static_cast<size_t>(y) * W + x);
+    if (q[3] == 0.0f) return 1.0 / far;
+    return -1.0 / (vm[2] * q[0] + vm[6] * q[1] + vm[10] * q[2] + vm[14]);
+  };
+  auto depth_bilinear = [&](double u, double v) {
+    double x = u * W - 0.5, y = v * H - 0.5;
+    double fx = std::floor(x), fy = std::floor(y);
+    double ax = x - fx, ay = y - fy;
+    int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+    int x0 = std::min(std::max(ix, 0), W - 1), x1 = std::min(std::max(ix + 1, 0), W - 1);
+    int y0 = std::min(std::max(iy, 0), H - 1), y1 = std::min(std::max(iy + 1, 0), H - 1);
+    double top = depth_at(x0, y0) + (depth_at(x1, y0) - depth_at(x0, y0)) * ax;
+    double bot = depth_at(x0, y1) + (depth_at(x1, y1) - depth_at(x0, y1)) * ax;
+    return top + (bot - top) * ay;
+  };
+  auto reconstruct = [&](double u, double v, double s) {
+    dvec3 ray = view_ray_uv(in, u, v);
+    double vz = vm[2] * ray.x + vm[6] * ray.y + vm[10] * ray.z;
+    const double* iv = in.inv_view.m;
+    return dvec3{iv[12], iv[13], iv[14]} + ray * (-1.0 / (s * vz));
+  };
+  const double u0 = (px + 0.5) / W, v0 = (py + 0.5) / H;
+  dvec3 P = reconstruct(u0, v0, depth_at(px, py));
   dvec3 N = normalize({n[0], n[1], n[2]});
   dvec3 I = normalize(view_ray(in, px, py));
   dvec3 R = I - N * (2.0 * dot(N, I));
@@ -831,7 +863,6 @@ inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], do
   double e4[4] = {E.x, E.y, E.z, 1.0}, ev[4], clip[4];
   in.view.apply(e4, ev);
   in.proj.apply(ev, clip);
-  const double u0 = (px + 0.5) / W, v0 = (py + 0.5) / H;
   double du = 0.5 * clip[0] / clip[3] + 0.5 - u0, dv = 0.5 * clip[1] / clip[3] + 0.5 - v0;
   double len = std::sqrt(du * du + dv * dv);
   double su = du / len * 0.005, sv = dv / len * 0.005;
@@ -844,9 +875,7 @@ inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], do
     double mb = std::min(std::min(std::fabs(u), std::fabs(1.0 - u)), std::min(std::fabs(v), std::fabs(1.0 - v)));
     m = std::min(m, mb);
     if (!(u >= 0.0 && u <= 1.0 && v >= 0.0 && v <= 1.0)) break;
-    double q[4];
-    rgba_sample_uv(in.position, W, H, u, v, q);
-    dvec3 Q{q[0], q[1], q[2]};
+    dvec3 Q = reconstruct(u, v, depth_bilinear(u, v));
     dvec3 dq = Q - P;
     double dl = length(dq);
     dvec3 dir = dq * (1.0 / dl);
@@ -875,6 +904,7 @@ inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], do
         out[0] = col.x; out[1] = col.y; out[2] = col.z; out[3] = 1.0;
         if (hit_step) *hit_step = i;
         if (hit_uv) { hit_uv[0] = u; hit_uv[1] = v; }
+        if (hit_pos) { hit_pos[0] = Q.x; hit_pos[1] = Q.y; hit_pos[2] = Q.z; }
         break;
       }
       // The step stays a miss while any failing test stays clear of its threshold.

@@ -291,17 +291,19 @@ void oracle_resolve(const oracle_resolve_args* a, const int* pixels, int count, 
 }
 
 // Screen-space reflection of pixels[:, (x, y)]: out4 rgba, margin, hit step
-// (-1 miss, -2 uncovered), hit uv (optional, 2 per pixel).
+// (-1 miss, -2 uncovered), hit uv and hit position (optional, 2 and 3 per
+// pixel, NaN without a hit).
 void oracle_ssr(const oracle_resolve_args* a, const int* pixels, int count, double* out4, double* out_margin,
-                int* out_step, double* out_uv) {
+                int* out_step, double* out_uv, double* out_pos) {
   std::vector<Light> lights = resolve_lights(a);
   ReflChain chain;
   ResolveInputs in = resolve_inputs(a, lights, &chain);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < count; ++i) {
-    double margin, uv[2] = {std::nan(""), std::nan("")};
+    double margin, uv[2] = {std::nan(""), std::nan("")}, pos[3] = {std::nan(""), std::nan(""), std::nan("")};
     int step;
-    ssr_pixel(in, pixels[2 * i], pixels[2 * i + 1], out4 + 4 * i, &margin, &step, uv);
+    ssr_pixel(in, pixels[2 * i], pixels[2 * i + 1], out4 + 4 * i, &margin, &step, uv, pos);
+    if (out_pos) for (int c = 0; c < 3; ++c) out_pos[3 * i + c] = pos[c];
     if (out_margin) out_margin[i] = margin;
     if (out_step) out_step[i] = step;
     if (out_uv) { out_uv[2 * i] = uv[0]; out_uv[2 * i + 1] = uv[1]; }

@@ -63,7 +63,7 @@ def lib() -> C.CDLL:
         L.oracle_resolve.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp]
         L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
-        L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp]
+        L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
         L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
         L.oracle_mlp_param_count.restype = i32
@@ -289,7 +289,7 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
 
 def ssr(scene, views: dict, pixels, gbuffer=None):
     """Oracle screen-space reflection of pixels[:, (x, y)]: (rgba, decision
-    margin, hit step (-1 miss, -2 uncovered), hit uv)."""
+    margin, hit step (-1 miss, -2 uncovered), hit uv, hit position)."""
     keep = []
     a = _resolve_args(scene, views, None, False, gbuffer, None, keep)
     px = _c(pixels, np.int32)
@@ -298,8 +298,9 @@ def ssr(scene, views: dict, pixels, gbuffer=None):
     margin = np.empty(n, np.float64)
     step = np.empty(n, np.int32)
     uv = np.empty((n, 2), np.float64)
-    lib().oracle_ssr(C.byref(a), _p(px), n, _p(rgba), _p(margin), _p(step), _p(uv))
-    return rgba, margin, step, uv
+    pos = np.empty((n, 3), np.float64)
+    lib().oracle_ssr(C.byref(a), _p(px), n, _p(rgba), _p(margin), _p(step), _p(uv), _p(pos))
+    return rgba, margin, step, uv, pos
 
 
 def convolve_level(src, dw: int, dh: int, vertical: bool) -> np.ndarray:

@@ -62,7 +62,7 @@ def _all_pixels(scene):
 
 
 def _compare_capture(scene, views, got, pix):
-    ref, margin, step, _ = ol.ssr(scene, views, pix)
+    ref, margin, step, _, _ = ol.ssr(scene, views, pix)
     clear = margin > MARGIN
     hit_ref = step >= 0
     hit_got = got[:, 3] == 1.0

@@ -15,7 +15,8 @@ reflection-buffer blend (no GPU).
   (term + c) / 2.
 - March (Shaders/SSR.frag:80-149): the plane y = 0 reflects every ray upward,
   so nothing is hit; on a floor facing a wall, a floor pixel's hit lies on the
-  wall at the analytic mirror intersection, within the march's step and cone.
+  wall at the analytic mirror intersection, within the march's step and cone,
+  and the position rebuilt from the filtered depth is on the wall plane.
 """
 import math
 
@@ -201,7 +202,7 @@ def test_ssr_plane_hits_nothing():
     views = _const_views()
     y, x = np.mgrid[0:sc.height, 0:sc.width]
     pix = np.stack([x.ravel(), y.ravel()], 1).astype(np.int32)
-    rgba, _, step, _ = ol.ssr(sc, views, pix)
+    rgba, _, step, _, _ = ol.ssr(sc, views, pix)
     covered = sc.position[..., 3].ravel() != 0
     assert covered.sum() > 1000
     assert np.all(step[covered] == -1) and np.all(step[~covered] == -2)
@@ -254,7 +255,7 @@ def test_ssr_floor_pixels_hit_the_wall_at_their_mirror_point():
     y, x = np.mgrid[0:H, 0:W]
     sel = use_f.ravel()
     pix = np.stack([x.ravel()[sel], y.ravel()[sel]], 1).astype(np.int32)
-    rgba, _, step, uv = ol.ssr(sc, views, pix)
+    rgba, _, step, uv, hpos = ol.ssr(sc, views, pix)
     P = sc.position.reshape(-1, 4)[sel, :3].astype(np.float64)
     dd = d.reshape(-1, 3)[sel]
     R = dd * np.array([1.0, -1.0, 1.0])  # mirror about the floor normal
@@ -271,14 +272,9 @@ def test_ssr_floor_pixels_hit_the_wall_at_their_mirror_point():
     # ... within about one march step of the analytic point, on the wall ...
     du = np.linalg.norm(uv[hit & on_wall] - uvw[hit & on_wall], axis=1)
     assert np.quantile(du, 0.99) < 0.0075 and du.max() < 0.012, (np.quantile(du, 0.99), du.max())
-    q = np.empty((hit.sum(), 4))
-    for k, (u, v) in enumerate(uv[hit]):
-        q[k] = _bilinear(sc.position, u, v)
-    # Hits read from four covered texels lie on the wall plane; the rest sit on
-    # its top edge, where the position filter blends in sky texels (w = 0).
-    full = q[:, 3] > 0.999
-    assert full.mean() > 0.9 and np.all(np.abs(q[full, 2] - zw) < 1e-9)
-    assert np.all(np.abs(q[~full, 1] / q[~full, 3] - hw) < 0.1)
+    # The hit point, rebuilt from the filtered depth, lies on the wall plane.
+    assert np.all(np.abs(hpos[hit, 2] - zw) < 1e-6)
+    assert np.all((hpos[hit, 1] > -1e-6) & (hpos[hit, 1] < hw + 1e-6))
     # ... and a hit is shaded (alpha 1, lit by the constant environment).
     assert np.all(rgba[hit, 3] == 1.0) and np.all(rgba[hit, :3] > 0.0)
     assert np.all(rgba[~hit] == 0.0)
@@ -286,13 +282,3 @@ def test_ssr_floor_pixels_hit_the_wall_at_their_mirror_point():
     over = Wst[:, 1] > hw + 0.5
     assert over.sum() > 50 and not hit[over].any()
 
-
-def _bilinear(img, u, v):
-    H, W = img.shape[:2]
-    xf, yf = u * W - 0.5, v * H - 0.5
-    x0, y0 = int(np.floor(xf)), int(np.floor(yf))
-    ax, ay = xf - x0, yf - y0
-    c = lambda i, j: img[min(max(j, 0), H - 1), min(max(i, 0), W - 1)].astype(np.float64)
-    top = c(x0, y0) * (1 - ax) + c(x0 + 1, y0) * ax
-    bot = c(x0, y0 + 1) * (1 - ax) + c(x0 + 1, y0 + 1) * ax
-    return top * (1 - ay) + bot * ay
