@@ -465,7 +465,7 @@ size_t althea_reflection_texels(int width, int height, int mips) {
 }
 
 size_t althea_ssr_workspace_bytes(int width, int height) {
-  if (width < 2 || height < 1) return 0;
+  if (width < 1 || height < 1) return 0;
   return ssr_workspace_bytes(width, height);
 }
 
@@ -477,7 +477,6 @@ althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   if (st != ALTHEA_OK) return st;
   st = reflection_ok(reflection, g, "ssr_capture");
   if (st != ALTHEA_OK) return st;
-  ALTHEA_REQUIRE(g->width >= 2, "ssr_capture: width %d (at least 2)", g->width);
   ALTHEA_REQUIRE(d_workspace, "ssr_capture: null workspace");
   return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, d_workspace, stream);
 }

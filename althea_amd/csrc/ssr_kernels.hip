@@ -62,7 +62,8 @@ struct SsrParams {
   float view_z[4];                  // row 2 of the view matrix
   float eye[3];
   float inv_far;                    // depth-plane value of an uncovered pixel
-  float* depth;                     // W*H + 1 floats: 1 / view distance, then a zero pad
+  float* depth;                     // depth plane in 8 x 4 texel blocks (taddr)
+  int depth_tw;                     // blocks per block row, ceil(W / 8)
   ReflChain out;                    // level 0 is written
 };
 
@@ -88,48 +89,46 @@ __device__ __forceinline__ float4 gb_bilinear(const void* plane, int W, int H, f
   return lerp4(lerp4(t00, t10, ax), lerp4(t01, t11, ax), ay);
 }
 
-// Depth plane: 1 / view distance of each covered pixel, 1 / far elsewhere,
-// and a zero after the last texel (read with weight 0 by the pair loads).
+// Depth-plane layout: blocks of 8 x 4 texels, one 128-byte cache line each,
+// block rows of ceil(W / 8) blocks.  A bilinear footprint then falls in one
+// line 2 times in 3 (row-major rows always take two), and a wave's nearby
+// march points share lines in both directions.  Measured at 4K: 2.07 ms
+// against 2.43 ms row-major (4 x 8 blocks 2.24, 16 x 2 blocks 2.23).
+__device__ __forceinline__ int taddr(int tw, int x, int y) {
+  return (((y >> 2) * tw + (x >> 3)) << 5) + ((y & 3) << 3) + (x & 7);
+}
+
+// Depth plane: 1 / view distance of each covered pixel, 1 / far elsewhere.
 template <int kFmt>
 __global__ __launch_bounds__(256) void ssr_depth_kernel(const SsrParams p) {
   const int n = p.width * p.height;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i > n) return;
-  if (i == n) {
-    p.depth[n] = 0.0f;
-    return;
-  }
+  if (i >= n) return;
   const float4 q = gb_load<kFmt>(p.position, i);
   const float vz = fmaf(p.view_z[0], q.x, fmaf(p.view_z[1], q.y, fmaf(p.view_z[2], q.z, p.view_z[3])));
-  p.depth[i] = q.w != 0.0f ? -1.0f / vz : p.inv_far;
+  const int x = i % p.width, y = i / p.width;
+  p.depth[taddr(p.depth_tw, x, y)] = q.w != 0.0f ? -1.0f / vz : p.inv_far;
 }
 
-// Bilinear CLAMP_TO_EDGE depth at uv: two 8-byte row reads (depth_tap),
-// then the blend (depth_value).  An index clamped at the left or right edge
-// becomes the pair (0, 1) with weight 0 or the pair (W-2, W-1) with weight 1,
-// which reads the same texel (W >= 2).  Row offsets use 24-bit multiplies
-// (W, H < 2^24).
+// Bilinear CLAMP_TO_EDGE depth at uv: the four texel reads (depth_tap), then
+// the blend (depth_value).
 struct DepthTap {
   float a0, b0, a1, b1, ax, ay;
 };
 
-__device__ __forceinline__ DepthTap depth_tap(const float* __restrict__ d, int W, int H, float u, float v) {
+__device__ __forceinline__ DepthTap depth_tap(const float* __restrict__ d, int tw, int W, int H, float u, float v) {
   const float x = fmaf(u, static_cast<float>(W), -0.5f), y = fmaf(v, static_cast<float>(H), -0.5f);
   const float fx = floorf(x), fy = floorf(y);
   DepthTap t;
   t.ax = x - fx;
   t.ay = y - fy;
-  int x0 = static_cast<int>(fx);
-  const int iy = static_cast<int>(fy);
-  if (x0 < 0) x0 = 0, t.ax = 0.0f;
-  if (x0 > W - 2) x0 = W - 2, t.ax = 1.0f;
+  const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  const int x0 = min(max(ix, 0), W - 1), x1 = min(max(ix + 1, 0), W - 1);
   const int y0 = min(max(iy, 0), H - 1), y1 = min(max(iy + 1, 0), H - 1);
-  const float* r0 = d + (__umul24(static_cast<unsigned>(y0), static_cast<unsigned>(W)) + x0);
-  const float* r1 = d + (__umul24(static_cast<unsigned>(y1), static_cast<unsigned>(W)) + x0);
-  t.a0 = r0[0];
-  t.b0 = r0[1];
-  t.a1 = r1[0];
-  t.b1 = r1[1];
+  t.a0 = d[taddr(tw, x0, y0)];
+  t.b0 = d[taddr(tw, x1, y0)];
+  t.a1 = d[taddr(tw, x0, y1)];
+  t.b1 = d[taddr(tw, x1, y1)];
   return t;
 }
 
@@ -190,7 +189,8 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
   const float u0 = (static_cast<float>(px) + 0.5f) / static_cast<float>(p.width);
   const float v0 = (static_cast<float>(py) + 0.5f) / static_cast<float>(p.height);
   // The pixel's own position from its depth texel (SSR.frag:136).
-  const f3 P = reconstruct(p, u0, v0, p.depth[idx]);
+  const float s0 = p.depth[taddr(p.depth_tw, min(px, p.width - 1), min(py, p.height - 1))];
+  const f3 P = reconstruct(p, u0, v0, s0);
   // reflect(normalize(view ray), N) (SSR.frag:137).
   const f3 ray0 = view_ray_at(p, u0, v0);
   const f3 I = normalize_fast(ray0);
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
       if (!(u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f)) {
         active = false;  // off screen: SSR.frag:97-99
       } else {
-        const DepthTap tap = depth_tap(p.depth, p.width, p.height, u, v);
+        const DepthTap tap = depth_tap(p.depth, p.depth_tw, p.width, p.height, u, v);
         const f3 ray = mk(fmaf(k, rk.x, ray0.x), fmaf(k, rk.y, ray0.y), fmaf(k, rk.z, ray0.z));
         const float t = -rcp_fast(depth_value(tap) * fmaf(k, vzk, vz0));
         const f3 dq = mk(fmaf(ray.x, t, EP.x), fmaf(ray.y, t, EP.y), fmaf(ray.z, t, EP.z));
@@ -339,7 +339,7 @@ void refl_chain(const althea_reflection_view& v, ReflChain& c) {
 }
 
 size_t ssr_workspace_bytes(int width, int height) {
-  return (static_cast<size_t>(width) * static_cast<size_t>(height) + 1) * sizeof(float);
+  return static_cast<size_t>((width + 7) / 8) * static_cast<size_t>((height + 3) / 4) * 32 * sizeof(float);
 }
 
 althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
@@ -377,9 +377,10 @@ althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
       static_cast<double>(globals->projection[14]) / (static_cast<double>(globals->projection[10]) + 1.0);
   p.inv_far = static_cast<float>(1.0 / far);
   p.depth = static_cast<float*>(workspace);
+  p.depth_tw = (g->width + 7) / 8;
   refl_chain(*reflection, p.out);
   const int n = g->width * g->height;
-  const dim3 dgrid((n + 1 + 255) / 256);
+  const dim3 dgrid((n + 255) / 256);
   const int tiles_x = (g->width + kSsrTile - 1) / kSsrTile, tiles_y = (g->height + kSsrTile - 1) / kSsrTile;
   const dim3 grid(kXcds * ssr_band_tiles(tiles_x, tiles_y));
   const float4* lights4 = reinterpret_cast<const float4*>(lights);

@@ -344,7 +344,7 @@ ALTHEA_API size_t althea_reflection_mip_offset(int width, int height, int level)
  * facing the ray, is shaded by pbrMaterial with the IBL state and
  * `light_count` point lights (ambient occlusion 1) and stored as (rgb, 1).
  * Everything else stores (0, 0, 0, 0).  Linear HDR, no tonemap.  The
- * buffer's extent must equal the G-buffer's (width >= 2).  d_workspace:
+ * buffer's extent must equal the G-buffer's.  d_workspace:
  * althea_ssr_workspace_bytes(width, height) bytes of device memory for the
  * depth plane, overwritten. */
 ALTHEA_API size_t althea_ssr_workspace_bytes(int width, int height);

@@ -70,8 +70,10 @@ def test_reflection_queries_and_argument_checks_without_gpu():
     v.irradiance, v.irradiance_size, v.brdf_lut, v.brdf_lut_size = dummy, 8, dummy, 16
     glob = _lib.GlobalsC()
     bad_extent = _lib.ReflectionViewC(dummy, 64, 31, 5, 0)
-    assert lib.althea_ssr_workspace_bytes(40, 24) == 4 * (40 * 24 + 1)
-    assert lib.althea_ssr_workspace_bytes(1, 24) == 0
+    # Depth plane in 8 x 4 texel blocks.
+    assert lib.althea_ssr_workspace_bytes(40, 24) == 4 * 40 * 24
+    assert lib.althea_ssr_workspace_bytes(37, 5) == 4 * (5 * 8) * (2 * 4)
+    assert lib.althea_ssr_workspace_bytes(0, 24) == 0
     assert lib.althea_ssr_capture(C.byref(g), C.byref(v), None, 0, C.byref(glob), C.byref(bad_extent), dummy,
                                   None) == 1
     assert b"reflection buffer 64x31" in lib.althea_last_error()

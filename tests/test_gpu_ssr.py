@@ -95,6 +95,21 @@ def test_ssr_capture_heightfield_matches_oracle(torch, ibl):
     assert r["err"]["mean"] < 5e-5 and r["err"]["max"] < 2e-3, r
 
 
+@pytest.mark.parametrize("W,H", [(37, 23), (1, 16), (130, 3)])
+def test_ssr_capture_odd_extents_match_oracle(torch, ibl, W, H):
+    """Extents that are not multiples of the 16 x 16 tiles or of the depth
+    plane's 8 x 4 blocks, down to one column."""
+    ibl_res, _ = ibl
+    scene = aa.synth.scene(seed=SEED0 + 25, width=W, height=H, lights=4)
+    refl = _capture(torch, scene, ibl_res)
+    got = refl.level(0).cpu().numpy().reshape(-1, 4).astype(np.float64)
+    r = _compare_capture(scene, ibl_res.views_numpy(), got, _all_pixels(scene))
+    print("SSR", W, H, r)
+    assert r["disagree_clear"] == 0, r
+    if r["err"] is not None:
+        assert r["err"]["max"] < 2e-3, r
+
+
 def test_ssr_capture_wall_scene_matches_oracle(torch, ibl):
     ibl_res, _ = ibl
     scene, use_f, _, _ = wall_scene(W=320, H=180)
