@@ -1,11 +1,12 @@
 // pbrMaterial (Shaders/PBR/PBRMaterial.glsl:72-162) as device functions,
 // shared by the deferred resolve and the SSR hit shading
-// (SSR.frag:46-78 calls the same pbrMaterial).
+// (SSR.frag:55-78 calls the same pbrMaterial).
 //
-// Conventions (live call of SSR.frag:65-75): V = unit view ray,
-// camera -> surface, so NdotV = max(dot(N, -V), 0) (:82); H = normalize(V + L)
-// (:145); alpha = roughness^2, kDirect = (alpha + 1)^2 / 8 (:93-96);
-// Schlick-roughness Fresnel F0 + (max(1 - r, F0) - F0) (1 - x)^5 (:51-53).
+// Conventions (the live call of SSR.frag:65-75): V = unit view ray,
+// camera -> surface, so NdotV = max(dot(N, -V), 0) (PBRMaterial.glsl:82);
+// H = normalize(V + L) (PBRMaterial.glsl:145); alpha = roughness^2,
+// kDirect = (alpha + 1)^2 / 8 (PBRMaterial.glsl:93-96); Schlick-roughness
+// Fresnel F0 + (max(1 - r, F0) - F0) (1 - x)^5 (PBRMaterial.glsl:51-53).
 #pragma once
 
 #include <hip/hip_runtime.h>

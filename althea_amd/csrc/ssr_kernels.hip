@@ -3,7 +3,7 @@
 // ssr_capture_kernel (Shaders/SSR.frag:80-149 with SSR.vert:15-22): one lane
 // per pixel (tile order below).  A lane leaves the march at its hit, its exit
 // from the screen or step 128; the wave leaves when no lane is marching.  The hit shading
-// (SSR.frag:46-78, pbrMaterial with the IBL state and the point lights) runs
+// (SSR.frag:55-78, pbrMaterial with the IBL state and the point lights) runs
 // once after the march for all hit lanes of the wave together.
 //
 // reflection_convolve_kernel (Shaders/SSRGlossyConvolve.comp:22-55, scheduled
@@ -11,7 +11,7 @@
 // bilinear taps into level m-1 along one axis.
 //
 // Positions along the march are rebuilt from the depth buffer as the
-// reference does (SSR.frag:38-41, Misc/ReconstructPosition.glsl): bilinear
+// reference does (SSR.frag:42-45, Misc/ReconstructPosition.glsl): bilinear
 // depth (the LINEAR / CLAMP_TO_EDGE depth sampler, DeferredRendering.cpp:67-69),
 // then the point at that view depth on the pixel's view ray.  This G-buffer
 // carries world positions instead of a depth attachment, so ssr_depth_kernel
@@ -25,7 +25,7 @@
 // and 1000.
 //
 // G-buffer coverage is position.w != 0 (the flag the resolve uses,
-// DeferredPass.frag:44); the reference's SSR tests normal.a (SSR.frag:131).
+// DeferredPass.frag:44); the reference's SSR tests normal.a (SSR.frag:137).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -41,7 +41,7 @@ namespace althea {
 
 constexpr int kSsrSteps = 128;        // RAYMARCH_STEPS, SSR.frag:80
 constexpr float kSsrStep = 0.005f;    // uv per step, SSR.frag:89
-constexpr float kSsrCone = 0.999f;    // cos of the accepted angle, SSR.frag:112
+constexpr float kSsrCone = 0.999f;    // cos of the accepted angle, SSR.frag:116-117
 constexpr float kSsrFar = 10000.0f;   // march target distance, SSR.frag:83
 
 struct SsrParams {
@@ -125,10 +125,15 @@ __device__ __forceinline__ DepthTap depth_tap(const float* __restrict__ d, int t
   const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
   const int x0 = min(max(ix, 0), W - 1), x1 = min(max(ix + 1, 0), W - 1);
   const int y0 = min(max(iy, 0), H - 1), y1 = min(max(iy + 1, 0), H - 1);
-  t.a0 = d[taddr(tw, x0, y0)];
-  t.b0 = d[taddr(tw, x1, y0)];
-  t.a1 = d[taddr(tw, x0, y1)];
-  t.b1 = d[taddr(tw, x1, y1)];
+  // The other three texels from the first: +1 in a block, +25 into the next
+  // block; +8 in a block, one block row on (32 tw) less 3 rows of 8.
+  const int i00 = taddr(tw, x0, y0);
+  const int dx = x1 == x0 ? 0 : ((x0 & 7) == 7 ? 25 : 1);
+  const int dy = y1 == y0 ? 0 : ((y0 & 3) == 3 ? (tw << 5) - 24 : 8);
+  t.a0 = d[i00];
+  t.b0 = d[i00 + dx];
+  t.a1 = d[i00 + dy];
+  t.b1 = d[i00 + dy + dx];
   return t;
 }
 
@@ -188,16 +193,16 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
   const f3 N = normalize_fast(mk(nrm.x, nrm.y, nrm.z));
   const float u0 = (static_cast<float>(px) + 0.5f) / static_cast<float>(p.width);
   const float v0 = (static_cast<float>(py) + 0.5f) / static_cast<float>(p.height);
-  // The pixel's own position from its depth texel (SSR.frag:136).
+  // The pixel's own position from its depth texel (SSR.frag:143).
   const float s0 = p.depth[taddr(p.depth_tw, min(px, p.width - 1), min(py, p.height - 1))];
   const f3 P = reconstruct(p, u0, v0, s0);
-  // reflect(normalize(view ray), N) (SSR.frag:137).
+  // reflect(normalize(view ray), N) (SSR.frag:145).
   const f3 ray0 = view_ray_at(p, u0, v0);
   const f3 I = normalize_fast(ray0);
   const float ni = 2.0f * dot(N, I);
   const f3 R = mk(fmaf(-ni, N.x, I.x), fmaf(-ni, N.y, I.y), fmaf(-ni, N.z, I.z));
   // March direction in uv towards the projection of P + 10000 R
-  // (SSR.frag:82-90), and the reference plane normal (SSR.frag:92-93).
+  // (SSR.frag:82-90), and the reference plane normal (SSR.frag:91-92).
   const f3 E = mk(fmaf(R.x, kSsrFar, P.x), fmaf(R.y, kSsrFar, P.y), fmaf(R.z, kSsrFar, P.z));
   const float ex = fmaf(p.vp[0], E.x, fmaf(p.vp[4], E.y, fmaf(p.vp[8], E.z, p.vp[12])));
   const float ey = fmaf(p.vp[1], E.x, fmaf(p.vp[5], E.y, fmaf(p.vp[9], E.z, p.vp[13])));
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
       const float k = static_cast<float>(i + 1);
       const float u = fmaf(k, su, u0), v = fmaf(k, sv, v0);
       if (!(u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f)) {
-        active = false;  // off screen: SSR.frag:97-99
+        active = false;  // off screen: SSR.frag:102-104
       } else {
         const DepthTap tap = depth_tap(p.depth, p.depth_tw, p.width, p.height, u, v);
         const f3 ray = mk(fmaf(k, rk.x, ray0.x), fmaf(k, rk.y, ray0.y), fmaf(k, rk.z, ray0.z));
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
     }
   }
   const float hu = fmaf(hk, su, u0), hv = fmaf(hk, sv, v0);
-  // environmentLitSample (SSR.frag:46-78): V = the reflected ray, occlusion 1.
+  // environmentLitSample (SSR.frag:55-78): V = the reflected ray, occlusion 1.
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 alb = hit ? gb_bilinear<kFmt>(p.albedo, p.width, p.height, hu, hv) : zero4;
   const float4 mro = hit ? gb_bilinear<kFmt>(p.mro, p.width, p.height, hu, hv) : zero4;
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
 }
 
 // Level m from level m-1.  The texel's uv is texel / extent(m) with no
-// half-texel offset (SSRGlossyConvolve.comp:34-35); a tap at uv + o d / W_m
+// half-texel offset (SSRGlossyConvolve.comp:35-36); a tap at uv + o d / W_m
 // lands at (q + c) source texels, q integer, c = frac + o s - 0.5 with
 // s = W_{m-1} / W_m (along x) or H_{m-1} / W_m (along y).
 struct ConvolveArgs {

@@ -133,6 +133,59 @@ class GBufferResources {
   althea_gbuffer_view _view{};
 };
 
+// ReflectionBuffer (Src/ReflectionBuffer.cpp:20-94): the frame-sized
+// reflection image with `mips` levels (5 in the reference), RGBA16F by
+// default, plus the capture's depth-plane workspace.  Owns its device memory.
+class ReflectionBuffer {
+ public:
+  ReflectionBuffer() = default;
+  ReflectionBuffer(int width, int height, int mips = 5, int format = ALTHEA_REFLECTION_FP16) {
+    const size_t texels = althea_reflection_texels(width, height, mips);
+    if (texels == 0) throw std::runtime_error("ReflectionBuffer: invalid extent or level count");
+    const size_t bytes = texels * (format == ALTHEA_REFLECTION_FP16 ? 8 : 16);
+    void* data = nullptr;
+    void* ws = nullptr;
+    if (hipMalloc(&data, bytes) != hipSuccess) throw std::runtime_error("ReflectionBuffer: hipMalloc failed");
+    _data.reset(data);
+    if (hipMalloc(&ws, althea_ssr_workspace_bytes(width, height)) != hipSuccess)
+      throw std::runtime_error("ReflectionBuffer: hipMalloc failed");
+    _workspace.reset(ws);
+    if (hipMemset(data, 0, bytes) != hipSuccess) throw std::runtime_error("ReflectionBuffer: hipMemset failed");
+    _view = {data, width, height, mips, format};
+  }
+
+  const althea_reflection_view& view() const { return _view; }
+  void* workspace() const { return _workspace.get(); }
+
+ private:
+  struct Free {
+    void operator()(void* p) const { (void)hipFree(p); }
+  };
+  std::unique_ptr<void, Free> _data, _workspace;
+  althea_reflection_view _view{};
+};
+
+// ScreenSpaceReflection (Src/ScreenSpaceReflection.cpp): captureReflection
+// draws Shaders/SSR.frag into level 0, convolveReflectionBuffer fills the
+// glossy levels (Src/ReflectionBuffer.cpp:163-277).
+namespace ScreenSpaceReflection {
+
+inline void captureReflection(const GBufferResources& gbuffer, const IBLResources& ibl,
+                              const althea_point_light* d_lights, int lightCount, const althea_globals& globals,
+                              ReflectionBuffer& reflection, hipStream_t stream = nullptr) {
+  althea_ibl_views v = ibl.views();
+  throwIfFailed(althea_ssr_capture(&gbuffer.view(), &v, d_lights, lightCount, &globals, &reflection.view(),
+                                   reflection.workspace(), stream),
+                "ScreenSpaceReflection::captureReflection");
+}
+
+inline void convolveReflectionBuffer(ReflectionBuffer& reflection, hipStream_t stream = nullptr) {
+  throwIfFailed(althea_reflection_convolve(&reflection.view(), stream),
+                "ScreenSpaceReflection::convolveReflectionBuffer");
+}
+
+}  // namespace ScreenSpaceReflection
+
 namespace DeferredRendering {
 
 // The deferred lighting resolve of DeferredPass.frag (one frame).
@@ -142,6 +195,17 @@ inline void resolve(const GBufferResources& gbuffer, const IBLResources& ibl, co
   althea_ibl_views v = ibl.views();
   throwIfFailed(althea_deferred_resolve(&gbuffer.view(), &v, d_lights, lightCount, &globals, ssao, flags, d_out, d_ao,
                                         stream),
+                "DeferredRendering::resolve");
+}
+
+// The same resolve reading a reflection buffer (DeferredPass.frag:59-68).
+inline void resolve(const GBufferResources& gbuffer, const IBLResources& ibl, const althea_point_light* d_lights,
+                    int lightCount, const althea_globals& globals, const althea_ssao_desc* ssao,
+                    const ReflectionBuffer& reflection, void* d_out, unsigned flags = 0, float* d_ao = nullptr,
+                    hipStream_t stream = nullptr) {
+  althea_ibl_views v = ibl.views();
+  throwIfFailed(althea_deferred_resolve_reflect(&gbuffer.view(), &v, d_lights, lightCount, &globals, ssao,
+                                                &reflection.view(), flags, d_out, d_ao, stream),
                 "DeferredRendering::resolve");
 }
 

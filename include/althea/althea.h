@@ -359,7 +359,7 @@ ALTHEA_API althea_status althea_ssr_capture(const althea_gbuffer_view* gbuffer, 
  * m-1 bilinearly (CLAMP_TO_EDGE) at uv = texel / extent(m) (no half-texel
  * offset), blurred along y for odd m and along x for even m by the 7-tap
  * linear-sampled Gaussian: uv offsets of 1.4118, 3.2941 and 5.1765 divided by
- * the level's width, along either axis (SSRGlossyConvolve.comp:40-52).
+ * the level's width, along either axis (SSRGlossyConvolve.comp:42-52).
  * Levels are written in order on `stream`. */
 ALTHEA_API althea_status althea_reflection_convolve(const althea_reflection_view* reflection, hipStream_t stream);
 

@@ -801,7 +801,7 @@ inline void reflection_sample(const ReflChain& c, int px, int py, double lod, do
 // ---------------------------------------------------------------------------
 // Screen-space reflection of one pixel: Shaders/SSR.frag:80-149 (march) and
 // :46-78 (hit shading), SSR.vert:15-22 (view ray).  Positions come from the
-// depth buffer as in SSR.frag:38-41 and Misc/ReconstructPosition.glsl: the
+// depth buffer as in SSR.frag:42-45 and Misc/ReconstructPosition.glsl: the
 // depth of a texel is taken here as 1 / view distance (affine in the [0, 1]
 // depth the rasterizer stores, so the bilinear filter gives the same value),
 // the far plane where the G-buffer is uncovered (depth cleared to 1,
@@ -891,7 +891,7 @@ inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], do
       const double mc = std::fabs(facing);
       if (a && b && c) {
         m = std::min(m, std::min(ma, std::min(mbb, mc)));
-        // environmentLitSample (SSR.frag:46-78): V = R, occlusion 1.
+        // environmentLitSample (SSR.frag:55-78): V = R, occlusion 1.
         double al[4], mr[4];
         rgba_sample_uv(in.albedo, W, H, u, v, al);
         rgba_sample_uv(in.mro, W, H, u, v, mr);

@@ -98,6 +98,23 @@ int main() {
     for (float f : out) finite = finite && std::isfinite(f) && f >= 0.0f && f <= 1.0f;
     std::printf("prefiltered max rel %.3g irradiance max rel %.3g resolve finite %d state bytes %zu\n", max_pref,
                 max_irr, finite ? 1 : 0, ibl.stateBytes());
+    // Screen-space reflections: capture, glossy levels, the resolve reading them.
+    ReflectionBuffer refl(GW, GH, 5);
+    ScreenSpaceReflection::captureReflection(gbuf, ibl, d_light, 1, g, refl);
+    ScreenSpaceReflection::convolveReflectionBuffer(refl);
+    DeferredRendering::resolve(gbuf, ibl, d_light, 1, g, nullptr, refl, d_out, ALTHEA_RESOLVE_TONEMAP);
+    HIP_OK(hipMemcpy(out.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost));
+    bool refl_finite = true;
+    for (float f : out) refl_finite = refl_finite && std::isfinite(f) && f >= 0.0f && f <= 1.0f;
+    bool extent_threw = false;
+    try {
+      ReflectionBuffer wrong(GW + 1, GH, 5);
+      ScreenSpaceReflection::captureReflection(gbuf, ibl, d_light, 1, g, wrong);
+    } catch (const std::runtime_error& e) {
+      extent_threw = true;
+    }
+    std::printf("reflections finite %d, extent mismatch threw %d\n", refl_finite ? 1 : 0, extent_threw ? 1 : 0);
+    finite = finite && refl_finite && extent_threw;
     // Errors come back as exceptions, as the reference throws.
     bool threw = false;
     try {

@@ -6,7 +6,7 @@ reflection-buffer blend (no GPU).
   no half-texel offset); a unit impulse on a same-size level comes out as the
   7 linear taps built here from the trimmed binomial row C(16, 2..14) / 65502
   and the tent weights of the bilinear filter, including the vertical pass
-  whose offsets are divided by the width (SSRGlossyConvolve.comp:38-52).
+  whose offsets are divided by the width (SSRGlossyConvolve.comp:42-52).
 - Reflection-buffer read (DeferredPass.frag:29-31): level-constant buffers
   give the LINEAR-mipmap blend of the two levels; LOD 0 at a pixel centre is
   that texel.
