@@ -193,6 +193,22 @@ class IBLResources:
         self._env_keepalive = env
         check(lib().althea_ibl_precompute(self._h, _ptr(env), w, h, _stream(stream)), "althea_ibl_precompute")
 
+    def save_cache(self, directory, stream=None) -> None:
+        """Write the precomputed maps in the reference's cache layout
+        (<dir>/Prefiltered<m+1>.hdr, <dir>/IrradianceMap.hdr)."""
+        check(lib().althea_ibl_save_cache(self._h, os.fsencode(directory), _stream(stream)), "althea_ibl_save_cache")
+
+    def load_cache(self, env, directory, stream=None) -> bool:
+        """Fill the handle from a cache written by save_cache and the source
+        map `env` (device tensor (h, w, 4)); False when a file is missing."""
+        h, w = int(env.shape[0]), int(env.shape[1])
+        self._env_keepalive = env
+        st = lib().althea_ibl_load_cache(self._h, _ptr(env), w, h, os.fsencode(directory), _stream(stream))
+        if st == _lib.STATUS_NOT_FOUND:
+            return False
+        check(st, "althea_ibl_load_cache")
+        return True
+
     def getHandles(self) -> _lib.IBLViewsC:
         v = _lib.IBLViewsC()
         check(lib().althea_ibl_get_views(self._h, C.byref(v)), "althea_ibl_get_views")
@@ -265,6 +281,25 @@ class ImageBasedLighting:
         return res
 
     create_resources = createResources
+
+    @staticmethod
+    def createResourcesFromContent(content_dir, env_map_name: str, desc: IBLDesc | None = None,
+                                   stream=None) -> IBLResources:
+        """The reference's createResources(app, cmd, envMapName)
+        (ImageBasedLighting.cpp:415-446): the source is
+        <content>/HDRI_Skybox/<name>.hdr (missing: raises); the maps come from
+        the cache <content>/PrecomputedMaps/<name>/ when it is complete, else
+        they are precomputed and the cache is written."""
+        from . import hdr
+
+        src = os.path.join(content_dir, "HDRI_Skybox", env_map_name + ".hdr")
+        env = _torch().from_numpy(hdr.read(src)).cuda()
+        cache = os.path.join(content_dir, "PrecomputedMaps", env_map_name)
+        res = IBLResources(desc or IBLDesc())
+        if not res.load_cache(env, cache, stream):
+            res.precompute(env, stream)
+            res.save_cache(cache, stream)
+        return res
 
     @staticmethod
     def createResourcesBatched(envs, descs, tuning=None, stream=None):

@@ -72,6 +72,7 @@ class ReflectionViewC(C.Structure):
 
 
 REFLECTION_MAX_MIPS = 8  # ALTHEA_REFLECTION_MAX_MIPS
+STATUS_NOT_FOUND = 4     # ALTHEA_ERROR_NOT_FOUND
 REFLECTION_FP16 = 0      # ALTHEA_REFLECTION_FP16
 REFLECTION_FP32 = 1      # ALTHEA_REFLECTION_FP32
 
@@ -105,6 +106,8 @@ PROTOTYPES = {
     "althea_ibl_precompute_batched": (i32, [C.POINTER(vp), C.POINTER(EnvViewC), i32, C.POINTER(TuningC),
                                             C.POINTER(TuningC), vp]),
     "althea_ibl_state_bytes": (sz, [vp]),
+    "althea_ibl_save_cache": (i32, [vp, C.c_char_p, vp]),
+    "althea_ibl_load_cache": (i32, [vp, vp, i32, i32, C.c_char_p, vp]),
     "althea_deferred_resolve": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
                                       C.POINTER(SsaoDescC), C.c_uint, vp, vp, vp]),
     "althea_reflection_texels": (sz, [i32, i32, i32]),

@@ -7,7 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "althea/althea.h"
@@ -451,6 +453,18 @@ althea_status althea_deferred_resolve_mlp(const althea_gbuffer_view* g, const al
                         stream);
 }
 
+// ------------------------------------------------------------ IBL cache
+// The reference's cache layout (Src/ImageBasedLighting.cpp:428-446):
+// <dir>/Prefiltered<m+1>.hdr per roughness level and <dir>/IrradianceMap.hdr,
+// RGBE through stb (Src/Utilities.cpp:216-255).  The images here are the
+// cubes: each level's six faces stacked top to bottom in face order
+// (+X, -X, +Y, -Y, +Z, -Z), S wide and 6 S tall, as [face][y][x] stores them.
+static std::string cache_file(const char* dir, int prefiltered_level) {
+  std::string d(dir);
+  if (prefiltered_level < 0) return d + "/IrradianceMap.hdr";
+  return d + "/Prefiltered" + std::to_string(prefiltered_level + 1) + ".hdr";
+}
+
 size_t althea_reflection_mip_offset(int width, int height, int level) {
   if (width < 1 || height < 1 || level < 0) return 0;
   size_t off = 0;
@@ -467,6 +481,69 @@ size_t althea_reflection_texels(int width, int height, int mips) {
 size_t althea_ssr_workspace_bytes(int width, int height) {
   if (width < 1 || height < 1) return 0;
   return ssr_workspace_bytes(width, height);
+}
+
+althea_status althea_ibl_save_cache(const althea_ibl* ibl, const char* dir, hipStream_t stream) {
+  ALTHEA_REQUIRE(ibl && dir && dir[0], "ibl_save_cache: null argument");
+  ALTHEA_REQUIRE(ibl->env_w > 0, "ibl_save_cache: the handle holds no precomputed maps");
+  std::error_code ec;
+  std::filesystem::create_directories(dir, ec);
+  if (ec) {
+    set_error("ibl_save_cache: cannot create %s: %s", dir, ec.message().c_str());
+    return ALTHEA_ERROR_IO;
+  }
+  ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+  const althea_ibl_desc& d = ibl->desc;
+  std::vector<float> host;
+  auto save = [&](const float* dev, int S, int level) -> althea_status {
+    host.resize(static_cast<size_t>(6) * S * S * 4);
+    ALTHEA_TRY_HIP(hipMemcpy(host.data(), dev, host.size() * sizeof(float), hipMemcpyDeviceToHost));
+    return althea_hdr_write(cache_file(dir, level).c_str(), host.data(), S, 6 * S);
+  };
+  for (int m = 0; m < d.prefiltered_mips; ++m) {
+    althea_status st = save(ibl->prefiltered + 4 * cube_mip_offset(d.prefiltered_size, m),
+                            std::max(1, d.prefiltered_size >> m), m);
+    if (st != ALTHEA_OK) return st;
+  }
+  return save(ibl->irradiance, d.irradiance_size, -1);
+}
+
+althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, int width, int height, const char* dir,
+                                    hipStream_t stream) {
+  ALTHEA_REQUIRE(ibl && d_equirect && dir && dir[0], "ibl_load_cache: null argument");
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_load_cache: %dx%d is not a power-of-two extent", width, height);
+  const althea_ibl_desc& d = ibl->desc;
+  // Every file present, each with the handle's extent, before any device work.
+  for (int m = -1; m < d.prefiltered_mips; ++m) {
+    const int S = m < 0 ? d.irradiance_size : std::max(1, d.prefiltered_size >> m);
+    const std::string f = cache_file(dir, m);
+    int w = 0, h = 0;
+    althea_status st = althea_hdr_info(f.c_str(), &w, &h);
+    if (st != ALTHEA_OK) return st;
+    ALTHEA_REQUIRE(w == S && h == 6 * S, "ibl_load_cache: %s is %dx%d, the handle needs %dx%d", f.c_str(), w, h, S,
+                   6 * S);
+  }
+  althea_status st = ibl_prepare(ibl, width, height, stream);
+  if (st != ALTHEA_OK) return st;
+  st = launch_pyramid(d_equirect, width, height, ibl->env, stream);
+  if (st != ALTHEA_OK) return st;
+  std::vector<float> host;
+  auto load = [&](float* dev, int S, int level) -> althea_status {
+    host.resize(static_cast<size_t>(6) * S * S * 4);
+    althea_status s2 = althea_hdr_read(cache_file(dir, level).c_str(), host.data(), host.size());
+    if (s2 != ALTHEA_OK) return s2;
+    // Synchronous copy: `host` is reused for the next file.
+    ALTHEA_TRY_HIP(hipMemcpyAsync(dev, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    ALTHEA_TRY_HIP(hipStreamSynchronize(stream));
+    return ALTHEA_OK;
+  };
+  for (int m = 0; m < d.prefiltered_mips; ++m) {
+    st = load(ibl->prefiltered + 4 * cube_mip_offset(d.prefiltered_size, m), std::max(1, d.prefiltered_size >> m), m);
+    if (st != ALTHEA_OK) return st;
+  }
+  st = load(ibl->irradiance, d.irradiance_size, -1);
+  if (st != ALTHEA_OK) return st;
+  return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
 }
 
 althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,

@@ -83,24 +83,60 @@ inline IBLResources createResources(const float* d_equirect, int width, int heig
 // not exist!" when the file is missing, ImageBasedLighting.cpp:420-425).  The
 // map must have power-of-two extents.  Synchronises `stream` before the
 // staging buffer is released.
+namespace detail {
+// An RGBE file in device memory, freed with the object.
+struct DeviceHdr {
+  float* data = nullptr;
+  int width = 0, height = 0;
+  DeviceHdr(const std::string& path, hipStream_t stream) {
+    althea_status st = althea_hdr_info(path.c_str(), &width, &height);
+    if (st == ALTHEA_ERROR_NOT_FOUND) throw std::runtime_error("Specified environment map does not exist!");
+    throwIfFailed(st, "ImageBasedLighting::createResources");
+    std::vector<float> host(static_cast<size_t>(width) * height * 4);
+    throwIfFailed(althea_hdr_read(path.c_str(), host.data(), host.size()), "ImageBasedLighting::createResources");
+    if (hipMalloc(&data, host.size() * sizeof(float)) != hipSuccess)
+      throw std::runtime_error("ImageBasedLighting::createResources: device allocation failed");
+    if (hipMemcpyAsync(data, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      (void)hipFree(data);
+      throw std::runtime_error("ImageBasedLighting::createResources: upload failed");
+    }
+  }
+  ~DeviceHdr() { (void)hipFree(data); }
+  DeviceHdr(const DeviceHdr&) = delete;
+  DeviceHdr& operator=(const DeviceHdr&) = delete;
+};
+}  // namespace detail
+
 inline IBLResources createResources(const std::string& hdrPath, const althea_ibl_desc& desc = defaultDesc(),
                                     hipStream_t stream = nullptr) {
-  int w = 0, h = 0;
-  althea_status st = althea_hdr_info(hdrPath.c_str(), &w, &h);
-  if (st == ALTHEA_ERROR_NOT_FOUND) throw std::runtime_error("Specified environment map does not exist!");
+  detail::DeviceHdr env(hdrPath, stream);
+  IBLResources res = createResources(env.data, env.width, env.height, desc, stream);
+  if (hipStreamSynchronize(stream) != hipSuccess)
+    throw std::runtime_error("ImageBasedLighting::createResources: precompute failed");
+  return res;
+}
+
+// The reference's whole createResources(app, cmd, envMapName)
+// (ImageBasedLighting.cpp:415-446) over a content directory: the source is
+// <content>/HDRI_Skybox/<name>.hdr (missing: throws the reference's
+// message); the maps are read from the cache <content>/PrecomputedMaps/<name>/
+// when every file is there, else precomputed and written to it
+// (althea_ibl_load_cache / althea_ibl_save_cache).
+inline IBLResources createResources(const std::string& contentDir, const std::string& envMapName,
+                                    const althea_ibl_desc& desc = defaultDesc(), hipStream_t stream = nullptr) {
+  detail::DeviceHdr env(contentDir + "/HDRI_Skybox/" + envMapName + ".hdr", stream);
+  const std::string cache = contentDir + "/PrecomputedMaps/" + envMapName;
+  althea_ibl* ibl = nullptr;
+  throwIfFailed(althea_ibl_create(&desc, &ibl), "ImageBasedLighting::createResources");
+  IBLResources res(ibl);
+  althea_status st = althea_ibl_load_cache(ibl, env.data, env.width, env.height, cache.c_str(), stream);
+  if (st == ALTHEA_ERROR_NOT_FOUND) {
+    throwIfFailed(althea_ibl_precompute(ibl, env.data, env.width, env.height, stream),
+                  "ImageBasedLighting::createResources");
+    st = althea_ibl_save_cache(ibl, cache.c_str(), stream);
+  }
   throwIfFailed(st, "ImageBasedLighting::createResources");
-  std::vector<float> host(static_cast<size_t>(w) * h * 4);
-  throwIfFailed(althea_hdr_read(hdrPath.c_str(), host.data(), host.size()), "ImageBasedLighting::createResources");
-  float* d_env = nullptr;
-  if (hipMalloc(&d_env, host.size() * sizeof(float)) != hipSuccess)
-    throw std::runtime_error("ImageBasedLighting::createResources: device allocation failed");
-  struct Free {
-    float* p;
-    ~Free() { (void)hipFree(p); }
-  } guard{d_env};
-  if (hipMemcpyAsync(d_env, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, stream) != hipSuccess)
-    throw std::runtime_error("ImageBasedLighting::createResources: upload failed");
-  IBLResources res = createResources(d_env, w, h, desc, stream);
   if (hipStreamSynchronize(stream) != hipSuccess)
     throw std::runtime_error("ImageBasedLighting::createResources: precompute failed");
   return res;

@@ -240,6 +240,21 @@ ALTHEA_API althea_status althea_ibl_get_views(const althea_ibl* ibl, althea_ibl_
  * irradiance + LUT), the "IBL state per environment". */
 ALTHEA_API size_t althea_ibl_state_bytes(const althea_ibl* ibl);
 
+/* On-disk cache of the precomputed maps in the reference's layout
+ * (Src/ImageBasedLighting.cpp:428-446, written at :345-349 and :397-401):
+ * <dir>/Prefiltered<m+1>.hdr for each roughness level and
+ * <dir>/IrradianceMap.hdr, RGBE files.  The images are this build's cubes,
+ * each level's six faces stacked top to bottom in face order (S x 6S).
+ * Save: after a precompute; creates `dir`; synchronises `stream`.  Load:
+ * builds the environment pyramid from d_equirect, reads every level and the
+ * irradiance cube (RGBE precision, as the reference's own reload), computes
+ * the LUT; ALTHEA_ERROR_NOT_FOUND when a file is missing (then precompute and
+ * save, as the reference does), ALTHEA_ERROR_INVALID_ARGUMENT when a file's
+ * extent does not match the handle.  Synchronises `stream`. */
+ALTHEA_API althea_status althea_ibl_save_cache(const althea_ibl* ibl, const char* dir, hipStream_t stream);
+ALTHEA_API althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, int width, int height,
+                                               const char* dir, hipStream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Deferred resolve (DeferredPass.frag + pbrMaterial + SSAO)                 */
 /* ------------------------------------------------------------------------ */

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <filesystem>
 #include <cstdio>
 #include <string>
 #include <unistd.h>
@@ -139,7 +140,26 @@ int main() {
     HIP_OK(hipMemcpy(pref2.data(), from_file.getHandles().prefilteredMap, pref2.size() * 4, hipMemcpyDeviceToHost));
     const bool same = pref2 == pref;  // 2, 3 and 5 are exact in RGBE
     std::printf("missing file threw %d, file route identical %d\n", missing_threw ? 1 : 0, same ? 1 : 0);
-    const bool ok = max_pref < 1e-5 && max_irr < 1e-5 && finite && threw && missing_threw && same;
+    // The reference's content-directory form with its cache: the first call
+    // precomputes and writes PrecomputedMaps/Demo/, the second reads it.
+    const std::string content = "/tmp/althea_facade_content_" + std::to_string(getpid());
+    std::filesystem::create_directories(content + "/HDRI_Skybox");
+    throwIfFailed(althea_hdr_write((content + "/HDRI_Skybox/Demo.hdr").c_str(), env.data(), W, H), "althea_hdr_write");
+    IBLResources computed = ImageBasedLighting::createResources(content, std::string("Demo"), desc);
+    const bool cached = std::filesystem::exists(content + "/PrecomputedMaps/Demo/Prefiltered1.hdr") &&
+                        std::filesystem::exists(content + "/PrecomputedMaps/Demo/IrradianceMap.hdr");
+    IBLResources loaded = ImageBasedLighting::createResources(content, std::string("Demo"), desc);
+    std::vector<float> pref3(pref.size());
+    HIP_OK(hipMemcpy(pref3.data(), loaded.getHandles().prefilteredMap, pref3.size() * 4, hipMemcpyDeviceToHost));
+    double max_cache = 0.0;  // RGBE: 8 bits under the pixel's shared exponent
+    for (size_t i = 0; i < pref.size(); i += 4) {
+      const double big = std::max(pref[i], std::max(pref[i + 1], pref[i + 2]));
+      for (int c = 0; c < 3; ++c) max_cache = std::max(max_cache, std::fabs(pref3[i + c] - pref[i + c]) / big);
+    }
+    std::filesystem::remove_all(content);
+    std::printf("cache written %d, cache reload max rel %.3g\n", cached ? 1 : 0, max_cache);
+    const bool ok = max_pref < 1e-5 && max_irr < 1e-5 && finite && threw && missing_threw && same && cached &&
+                    max_cache < 1.0 / 128;
     std::printf("facade %s\n", ok ? "ok" : "FAILED");
     return ok ? 0 : 2;
   } catch (const std::exception& e) {
