@@ -29,7 +29,8 @@ from . import synth
 from ._lib import AltheaError, check, lib
 
 __all__ = ["AltheaError", "IBLDesc", "IBLResources", "ImageBasedLighting", "GBufferResources",
-           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "synth", "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
+           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "PointLightCollection", "synth",
+           "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
            "brdf_lut", "cube_sample", "env_sample", "prefilter_points", "env_levels", "cube_texels"]
 
 
@@ -395,14 +396,72 @@ class ScreenSpaceReflection:
         return reflection
 
 
+class PointLightCollection:
+    """The point lights of Src/PointLight.cpp with their shadow cubes:
+    `lights` an (n, 8) float32 device tensor (position xyz, pad, emission
+    rgb, pad, as PointLight in Include/Althea/PointLight.h), `shadow_size`
+    the cube extent (256 in the reference, PointLight.cpp:35).  drawShadowMaps
+    renders the cubes of every light from a triangle mesh (world-space float32
+    positions (v, 3), uint32 indices (t, 3), device tensors)."""
+
+    def __init__(self, light_count: int, create_shadow_map: bool = True, shadow_size: int = 256):
+        torch = _torch()
+        self.lights = torch.zeros((light_count, 8), dtype=torch.float32, device="cuda")
+        self.shadow_size = shadow_size
+        self.shadow_maps = (torch.ones((light_count, 6, shadow_size, shadow_size), dtype=torch.float32,
+                                       device="cuda") if create_shadow_map else None)
+
+    @classmethod
+    def from_lights(cls, lights, create_shadow_map: bool = True, shadow_size: int = 256):
+        c = cls(int(lights.shape[0]), create_shadow_map, shadow_size)
+        c.lights.copy_(lights.reshape(-1, 8))
+        return c
+
+    def getCount(self) -> int:
+        return int(self.lights.shape[0])
+
+    def setLight(self, light_id: int, position, emission) -> None:
+        torch = _torch()
+        row = torch.tensor([*position, 0.0, *emission, 0.0], dtype=torch.float32)
+        self.lights[light_id].copy_(row)
+
+    def getShadowMapExtent(self) -> tuple:
+        return self.shadow_size, self.shadow_size
+
+    def getShadowMapImage(self):
+        return self.shadow_maps
+
+    def drawShadowMaps(self, positions, indices, stream=None):
+        if self.shadow_maps is None:
+            raise ValueError("PointLightCollection was created without shadow maps")
+        torch = _torch()
+        assert positions.dtype == torch.float32 and positions.is_contiguous() and positions.shape[-1] == 3
+        assert indices.dtype in (torch.int32, torch.uint32) and indices.is_contiguous()
+        tris = indices.numel() // 3
+        mesh = _lib.MeshViewC(_ptr(positions) if tris else None, _ptr(indices) if tris else None,
+                              int(positions.numel() // 3), tris)
+        check(lib().althea_shadow_cubes(C.byref(mesh), _ptr(self.lights), self.getCount(), self.shadow_size,
+                                        _ptr(self.shadow_maps), _stream(stream)),
+              "althea_shadow_cubes")
+        return self.shadow_maps
+
+
 class DeferredRendering:
     @staticmethod
     def resolve(gbuffer, ibl, lights, globals_, ssao=None, tonemap: bool = False, out_fp16: bool = False,
-                ao_out=None, out=None, reflection: ReflectionBuffer | None = None, stream=None):
+                ao_out=None, out=None, reflection: ReflectionBuffer | None = None,
+                shadows: PointLightCollection | None = None, stream=None):
         """One frame of the deferred resolve.  `gbuffer` is GBufferResources
         or a list of the four planes; `ibl` an IBLResources or IBLViewsC;
-        `lights` an (n, 8) float32 device tensor; `reflection` (optional) a
-        ReflectionBuffer blended into the prefiltered term."""
+        `lights` an (n, 8) float32 device tensor, or a PointLightCollection
+        (whose shadow cubes are then read when it has them); `reflection`
+        (optional) a ReflectionBuffer blended into the prefiltered term;
+        `shadows` (optional) the PointLightCollection whose cubes shadow
+        `lights`."""
+        if isinstance(lights, PointLightCollection):
+            if shadows is None and lights.shadow_maps is not None:
+                shadows = lights
+            lights = lights.lights
         torch = _torch()
         if not isinstance(gbuffer, GBufferResources):
             gbuffer = GBufferResources(*gbuffer)
@@ -414,6 +473,17 @@ class DeferredRendering:
             out = torch.empty((gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
         flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)
         ssao_p = C.byref(ssao) if ssao is not None else None
+        if shadows is not None:
+            if shadows.shadow_maps is None or shadows.getCount() < n:
+                raise ValueError("shadows: need one shadow cube per light")
+            rv = reflection.view() if reflection is not None else None
+            check(lib().althea_deferred_resolve_shadowed(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
+                                                         C.byref(globals_), ssao_p,
+                                                         C.byref(rv) if rv is not None else None,
+                                                         _ptr(shadows.shadow_maps), shadows.shadow_size, flags,
+                                                         _ptr(out), _ptr(ao_out), _stream(stream)),
+                  "althea_deferred_resolve_shadowed")
+            return out
         if reflection is not None:
             rv = reflection.view()
             check(lib().althea_deferred_resolve_reflect(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,

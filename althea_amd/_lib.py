@@ -71,6 +71,12 @@ class ReflectionViewC(C.Structure):
     _fields_ = [("data", vp), ("width", i32), ("height", i32), ("mips", i32), ("format", i32)]
 
 
+class MeshViewC(C.Structure):
+    _fields_ = [("positions", vp), ("indices", vp), ("vertex_count", i32), ("triangle_count", i32)]
+
+
+SHADOW_NEAR = 0.01       # ALTHEA_SHADOW_NEAR
+SHADOW_FAR = 1000.0      # ALTHEA_SHADOW_FAR
 REFLECTION_MAX_MIPS = 8  # ALTHEA_REFLECTION_MAX_MIPS
 STATUS_NOT_FOUND = 4     # ALTHEA_ERROR_NOT_FOUND
 REFLECTION_FP16 = 0      # ALTHEA_REFLECTION_FP16
@@ -119,6 +125,10 @@ PROTOTYPES = {
     "althea_deferred_resolve_reflect": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                               C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),
                                               C.c_uint, vp, vp, vp]),
+    "althea_shadow_cubes": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp]),
+    "althea_deferred_resolve_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
+                                               C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),
+                                               vp, i32, C.c_uint, vp, vp, vp]),
     "althea_hdr_info": (i32, [C.c_char_p, C.POINTER(i32), C.POINTER(i32)]),
     "althea_hdr_read": (i32, [C.c_char_p, vp, sz]),
     "althea_hdr_write": (i32, [C.c_char_p, vp, i32, i32]),

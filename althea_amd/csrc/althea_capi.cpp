@@ -546,6 +546,37 @@ althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, in
   return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
 }
 
+althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights, int light_count,
+                                  int size, float* d_cubes, hipStream_t stream) {
+  ALTHEA_REQUIRE(mesh && d_cubes && light_count >= 1 && d_lights, "shadow_cubes: null argument or no light");
+  ALTHEA_REQUIRE(size >= 1 && size <= 4096, "shadow_cubes: size %d", size);
+  ALTHEA_REQUIRE(mesh->triangle_count >= 0 && (mesh->triangle_count == 0 || (mesh->positions && mesh->indices &&
+                                                                               mesh->vertex_count >= 1)),
+                 "shadow_cubes: mesh with %d triangles, %d vertices", mesh->triangle_count, mesh->vertex_count);
+  ALTHEA_REQUIRE(static_cast<long long>(light_count) * mesh->triangle_count < (1ll << 31),
+                 "shadow_cubes: %d lights x %d triangles", light_count, mesh->triangle_count);
+  return launch_shadow_cubes(mesh, d_lights, light_count, size, d_cubes, stream);
+}
+
+althea_status althea_deferred_resolve_shadowed(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                               const althea_point_light* d_lights, int light_count,
+                                               const althea_globals* globals, const althea_ssao_desc* ssao,
+                                               const althea_reflection_view* reflection, const float* d_shadow_cubes,
+                                               int shadow_size, unsigned flags, void* d_out, float* d_ao_out,
+                                               hipStream_t stream) {
+  althea_status st = resolve_args_ok(g, ibl, d_lights, light_count, globals, ssao, d_out);
+  if (st != ALTHEA_OK) return st;
+  if (reflection) {
+    st = reflection_ok(reflection, g, "deferred_resolve_shadowed");
+    if (st != ALTHEA_OK) return st;
+  }
+  ALTHEA_REQUIRE(d_shadow_cubes && shadow_size >= 1 && shadow_size <= 4096,
+                 "deferred_resolve_shadowed: shadow cubes %p, size %d", static_cast<const void*>(d_shadow_cubes),
+                 shadow_size);
+  return launch_resolve(g, ibl, d_lights, light_count, globals, ssao, flags, d_out, d_ao_out, nullptr, nullptr,
+                        stream, reflection, d_shadow_cubes, shadow_size);
+}
+
 althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                  const althea_point_light* d_lights, int light_count, const althea_globals* globals,
                                  const althea_reflection_view* reflection, void* d_workspace, hipStream_t stream) {

@@ -92,7 +92,12 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
                              const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s,
-                             const althea_reflection_view* reflection = nullptr);
+                             const althea_reflection_view* reflection = nullptr,
+                             const float* shadow_cubes = nullptr, int shadow_size = 0);
+
+// Point-light shadow cubes (shadow_kernels.hip).
+althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
+                                  int size, float* cubes, hipStream_t s);
 
 // Screen-space reflection (ssr_kernels.hip): level 0 of the reflection
 // buffer, then the glossy convolution of levels 1 .. mips-1.

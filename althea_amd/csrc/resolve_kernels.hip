@@ -23,6 +23,7 @@
 #include "mlp_eval.hpp"
 #include "reflection.hpp"
 #include "shading.hpp"
+#include "shadow.hpp"
 
 namespace althea {
 
@@ -56,7 +57,8 @@ struct ResolveParams {
   float* ao_out;
   MlpInferenceView mlp;  // MLP variant only
   uint8_t* mlp_used;
-  ReflChain refl;  // reflection-buffer variant only
+  ReflChain refl;      // reflection-buffer variant only
+  ShadowCubes shadow;  // shadowed variant only
 };
 
 template <int kFmt>
@@ -175,7 +177,7 @@ struct MlpResolveSmem {
 // mapping that gives each XCD (with its own L2) contiguous runs of tiles was
 // measured slower at 4K (runs of 2, 8 or 68 tile rows: 0.87, 0.89, 0.84 ms
 // against 0.83 ms).
-template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false>
+template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false, bool kShadow = false>
 __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, int tiles_x,
                                              const float4* __restrict__ lights, const float4* s_noise,
                                              MlpWaveStage* s_stage, const MlpSmem* s_mlp) {
@@ -266,7 +268,10 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
     color = pbr_environment(m, irr, refl, A, B, ao);
   }
-  pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color);
+  if (kShadow)
+    pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color, p.shadow);
+  else
+    pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color);
   if (shade) {
     if (p.tonemap) color = tonemap(color, p.exposure);
     store_out<kOutF16>(p.out, idx, color);
@@ -284,13 +289,14 @@ __device__ __forceinline__ void resolve_prologue(const ResolveParams& p, float4*
   __syncthreads();
 }
 
-// Plain resolve: one block per tile; kRefl blends the reflection buffer.
-template <int kFmt, bool kOutF16, bool kRefl = false>
+// Plain resolve: one block per tile; kRefl blends the reflection buffer,
+// kShadow applies the point-light shadow test.
+template <int kFmt, bool kOutF16, bool kRefl = false, bool kShadow = false>
 __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, const float4* __restrict__ lights) {
   __shared__ float4 s_noise[16];
   resolve_prologue<false>(p, s_noise, nullptr);
   const int tiles_x = (p.width + 63) / 64;
-  resolve_tile<kFmt, kOutF16, false, kRefl>(p, blockIdx.x, tiles_x, lights, s_noise, nullptr, nullptr);
+  resolve_tile<kFmt, kOutF16, false, kRefl, kShadow>(p, blockIdx.x, tiles_x, lights, s_noise, nullptr, nullptr);
 }
 
 // MLP-IBL resolve: persistent blocks keep the MLP weights in LDS for several
@@ -348,7 +354,8 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
                              const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s,
-                             const althea_reflection_view* reflection) {
+                             const althea_reflection_view* reflection, const float* shadow_cubes,
+                             int shadow_size) {
   ResolveParams p{};
   p.position = g->position;
   p.normal = g->normal;
@@ -422,16 +429,31 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
     return check_launch("resolve_mlp_kernel");
   }
   const dim3 grid(tiles);
-  if (reflection) {
-    refl_chain(*reflection, p.refl);
-    if (fp32 && f16_out)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true, true>), grid, dim3(256), 0, s, p, lights4);
-    else if (fp32)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false, true>), grid, dim3(256), 0, s, p, lights4);
-    else if (f16_out)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true, true>), grid, dim3(256), 0, s, p, lights4);
-    else
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false, true>), grid, dim3(256), 0, s, p, lights4);
+  if (reflection || shadow_cubes) {
+    if (reflection) refl_chain(*reflection, p.refl);
+    p.shadow.cubes = shadow_cubes;
+    p.shadow.size = shadow_size;
+    const int variant = (reflection ? 1 : 0) | (shadow_cubes ? 2 : 0) | (f16_out ? 4 : 0) | (fp32 ? 0 : 8);
+    switch (variant) {
+#define ALTHEA_RESOLVE_CASE(v, fmt, f16, refl, shadow)                                                            \
+  case v:                                                                                                         \
+    hipLaunchKernelGGL((resolve_kernel<fmt, f16, refl, shadow>), grid, dim3(256), 0, s, p, lights4);              \
+    break;
+      ALTHEA_RESOLVE_CASE(1, ALTHEA_GBUFFER_FP32, false, true, false)
+      ALTHEA_RESOLVE_CASE(2, ALTHEA_GBUFFER_FP32, false, false, true)
+      ALTHEA_RESOLVE_CASE(3, ALTHEA_GBUFFER_FP32, false, true, true)
+      ALTHEA_RESOLVE_CASE(5, ALTHEA_GBUFFER_FP32, true, true, false)
+      ALTHEA_RESOLVE_CASE(6, ALTHEA_GBUFFER_FP32, true, false, true)
+      ALTHEA_RESOLVE_CASE(7, ALTHEA_GBUFFER_FP32, true, true, true)
+      ALTHEA_RESOLVE_CASE(9, ALTHEA_GBUFFER_FP16, false, true, false)
+      ALTHEA_RESOLVE_CASE(10, ALTHEA_GBUFFER_FP16, false, false, true)
+      ALTHEA_RESOLVE_CASE(11, ALTHEA_GBUFFER_FP16, false, true, true)
+      ALTHEA_RESOLVE_CASE(13, ALTHEA_GBUFFER_FP16, true, true, false)
+      ALTHEA_RESOLVE_CASE(14, ALTHEA_GBUFFER_FP16, true, false, true)
+      ALTHEA_RESOLVE_CASE(15, ALTHEA_GBUFFER_FP16, true, true, true)
+#undef ALTHEA_RESOLVE_CASE
+      default: break;
+    }
     return check_launch("resolve_kernel");
   }
   if (fp32 && f16_out)

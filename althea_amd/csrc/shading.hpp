@@ -88,9 +88,18 @@ __device__ __forceinline__ f3 pbr_environment(const PbrPoint& m, f3 irr, f3 refl
 // is skipped with a wave-uniform branch; lanes with N = 0 pass no horizon.
 // Lights are wave-uniform: scalar loads into SGPRs, and the grey test is a
 // scalar compare of the emission bits.  `any_lane`: some lane of the wave
-// shades (wave-uniform).
+// shades (wave-uniform).  `shadow` (ShadowCubes or NoShadow): a light it
+// hides at a lane adds nothing there (PBRMaterial.glsl:138-140); it is asked
+// only where the light is above the horizon.
+struct NoShadow {
+  static constexpr bool kActive = false;
+  __device__ __forceinline__ bool hidden(int, f3, float) const { return false; }
+};
+
+template <class Shadow = NoShadow>
 __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, float NI, bool any_lane,
-                                           const float4* __restrict__ lights, int light_count, f3& color) {
+                                           const float4* __restrict__ lights, int light_count, f3& color,
+                                           const Shadow& shadow = Shadow()) {
   const float c1 = m.a2 * gv_of(m.n_dot_v, m.kdir) * kInvPi;
   const float four_nv = 4.0f * m.n_dot_v;
   const float om_k = 1.0f - m.kdir;
@@ -115,7 +124,8 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
       const float t = fmaf(n_dot_h * n_dot_h, a2m1, 1.0f);
       const float den = (t * t) * fmaf(n_dot_l, om_k, m.kdir) * fmaf(n_dot_l, four_nv, 0.0001f);
       const float s = c1 * n_dot_l * rcp_fast(den);
-      const float w = (inv_d * inv_d) * n_dot_l;
+      float w = (inv_d * inv_d) * n_dot_l;
+      if (Shadow::kActive && n_dot_l > 0.0f && shadow.hidden(i, L, d2 * inv_d)) w = 0.0f;
       const bool grey =
           __float_as_uint(em.x) == __float_as_uint(em.y) && __float_as_uint(em.y) == __float_as_uint(em.z);
       if (grey) {

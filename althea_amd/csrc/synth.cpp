@@ -265,6 +265,32 @@ void althea_synth_gbuffer(uint64_t seed, const float* globals, int width, int he
   }
 }
 
+// The heightfield of althea_synth_gbuffer as a triangle mesh: an n x n grid
+// of cells over [-half, half]^2 in x and z, vertices (n+1)^2 at the terrain
+// height, two triangles per cell (indices 6 n^2).
+void althea_synth_heightfield_mesh(uint64_t seed, double amplitude, int n, double half, float* positions,
+                                   uint32_t* indices) {
+  Terrain terrain{seed};
+  terrain.amplitude = amplitude;
+  for (int j = 0; j <= n; ++j)
+    for (int i = 0; i <= n; ++i) {
+      double x = -half + 2.0 * half * i / n, z = -half + 2.0 * half * j / n;
+      double h, hx, hz;
+      terrain.height(x, z, h, hx, hz);
+      float* p = positions + 3 * (static_cast<size_t>(j) * (n + 1) + i);
+      p[0] = static_cast<float>(x);
+      p[1] = static_cast<float>(h);
+      p[2] = static_cast<float>(z);
+    }
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      uint32_t a = static_cast<uint32_t>(j * (n + 1) + i), b = a + 1, c = a + (n + 1), d = c + 1;
+      uint32_t* t = indices + 6 * (static_cast<size_t>(j) * n + i);
+      t[0] = a; t[1] = c; t[2] = b;
+      t[3] = b; t[4] = c; t[5] = d;
+    }
+}
+
 // Point lights [n][8] (position xyz, 0, emission xyz, 0) in the box
 // [-half, half] x [ymin, ymax] x [-half, half].
 void althea_synth_lights(uint64_t seed, int n, double half, double ymin, double ymax, double imin, double imax,

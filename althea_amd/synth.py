@@ -26,6 +26,7 @@ def _s() -> C.CDLL:
         s.althea_synth_camera.argtypes = [vp, vp, d, i32, i32, i32, C.c_float, vp]
         s.althea_synth_gbuffer.argtypes = [u64, vp, i32, i32, d, vp, vp, vp, vp]
         s.althea_synth_lights.argtypes = [u64, i32, d, d, d, d, d, vp]
+        s.althea_synth_heightfield_mesh.argtypes = [u64, d, i32, d, vp, vp]
         s.althea_synth_ssao.argtypes = [u64, i32, vp, vp]
         s.althea_synth_dir_rough.argtypes = [u64, C.c_int64, d, d, vp]
         _synth = s
@@ -132,3 +133,13 @@ def scene(seed: int, width: int, height: int, lights: int, ssao_samples: int = 1
     n = np.empty((16, 4), np.float32)
     s.althea_synth_ssao(seed, ssao_samples, _p(k), _p(n))
     return Scene(width, height, *planes, lt, g, k, n)
+
+
+def heightfield_mesh(seed: int, n: int = 128, half: float = 10.0, amplitude: float = 1.5):
+    """The scene's heightfield as a triangle mesh: (positions (V, 3) float32,
+    indices (T, 3) uint32), an n x n grid over [-half, half]^2, two triangles
+    per cell, vertices at the terrain height of scene(seed, ...)."""
+    pos = np.empty(((n + 1) * (n + 1), 3), np.float32)
+    idx = np.empty((2 * n * n, 3), np.uint32)
+    _s().althea_synth_heightfield_mesh(seed, amplitude, n, half, _p(pos), _p(idx))
+    return pos, idx

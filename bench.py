@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
     p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
     p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
-    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr"],
+    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr", "shadow"],
                    help="profiling runs: time one kernel family and print its line only "
                         "(copy = a plain 1 GiB device copy, the calibration for HBM byte counters)")
     return p.parse_args()
@@ -238,6 +238,61 @@ def ssr_only(torch, aa, args) -> int:
     return 0
 
 
+SHADOW_SIZE = 256  # the reference's cube extent (Src/PointLight.cpp:35)
+
+
+def shadow_frame(torch, aa, scene, ibl, gbuf, steps: int) -> dict:
+    """Point-light shadows on the same frame: the cubes of all 64 lights
+    (256^2 x 6 each, the reference's extent) from the scene's heightfield as
+    a 128 x 128-cell mesh (32 768 triangles), and the resolve reading them.
+    The producer's compulsory bytes are the cubes written once (clear and
+    depth) and the mesh read once; the resolve's are the plain resolve's plus
+    the cubes read once."""
+    pos, idx = aa.synth.heightfield_mesh(SEED0 + 1, n=128)
+    pos_t = torch.from_numpy(pos).cuda()
+    idx_t = torch.from_numpy(idx.view(np.int32)).cuda()
+    coll = aa.PointLightCollection.from_lights(scene.lights_tensor("cuda"), shadow_size=SHADOW_SIZE)
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    glob, ssao = scene.globals, scene.ssao
+
+    def draw():
+        coll.drawShadowMaps(pos_t, idx_t)
+
+    def resolve():
+        aa.DeferredRendering.resolve(gbuf, ibl, coll, glob, ssao, out=out)
+
+    def frame():
+        draw()
+        resolve()
+
+    frame()
+    draw_ms = time_events(torch, draw, steps)
+    res_ms = time_events(torch, resolve, steps)
+    frame_ms = time_events(torch, frame, steps)
+    cubes = coll.getShadowMapImage()
+    covered = float((cubes < 1.0).float().mean().item())
+    cube_bytes = cubes.numel() * 4
+    px = WIDTH * HEIGHT
+    return {"lights": LIGHTS, "cube_size": SHADOW_SIZE, "triangles": int(idx.shape[0]),
+            "draw_ms": draw_ms, "draw_texels_per_s": cubes.numel() / (draw_ms * 1e-3),
+            "draw_algorithmic_bytes": 2 * cube_bytes + pos.nbytes + idx.nbytes, "texels_covered": covered,
+            "resolve_shadowed_ms": res_ms, "resolve_shadowed_mpix_per_s": px / (res_ms * 1e-3) / 1e6,
+            "frame_ms": frame_ms, "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
+
+
+def shadow_only(torch, aa, args) -> int:
+    """Profiling run of the shadow pipeline on the 4K frame."""
+    env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
+                      brdf_lut_size=128, brdf_lut_samples=1024)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
+    gbuf = aa.GBufferResources(*scene.gbuffer_tensors("cuda", fp16=args.fp16_gbuffer))
+    r = shadow_frame(torch, aa, scene, ibl, gbuf, args.steps)
+    print(json.dumps({"kind": "shadow", **r}), flush=True)
+    return 0
+
+
 def fp16_resolve(torch, aa, scene, ibl, lights, steps: int) -> dict:
     """The same frame shaded from an FP16 G-buffer (32 B/px)."""
     g16 = aa.GBufferResources(*scene.gbuffer_tensors("cuda", fp16=True))
@@ -319,6 +374,8 @@ def main() -> int:
         return prefilter_only(torch, aa, args)
     if args.only == "ssr":
         return ssr_only(torch, aa, args)
+    if args.only == "shadow":
+        return shadow_only(torch, aa, args)
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
     desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
                       brdf_lut_size=128, brdf_lut_samples=1024)
@@ -365,6 +422,7 @@ def main() -> int:
         line["secondary"] = secondary(torch, aa)
         line["secondary"]["resolve_fp16_gbuffer"] = fp16_resolve(torch, aa, scene, ibl, lights, args.steps)
         line["secondary"]["ssr"] = ssr_frame(torch, aa, scene, ibl, gbuf, lights, args.steps)
+        line["secondary"]["shadows"] = shadow_frame(torch, aa, scene, ibl, gbuf, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)

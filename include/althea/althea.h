@@ -390,6 +390,49 @@ ALTHEA_API althea_status althea_deferred_resolve_reflect(const althea_gbuffer_vi
                                                          void* d_out, float* d_ao_out, hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Point-light shadow cubes                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* A triangle mesh in world space: float3 positions, 3 uint32 indices per
+ * triangle (the geometry the reference's shadow pass draws). */
+typedef struct althea_mesh_view {
+  const float* positions;
+  const uint32_t* indices;
+  int vertex_count, triangle_count;
+} althea_mesh_view;
+
+#define ALTHEA_SHADOW_NEAR 0.01f  /* PointLight.cpp:84, ShadowMap.frag */
+#define ALTHEA_SHADOW_FAR 1000.0f
+
+/* The shadow cubes of PointLightCollection::drawShadowMaps
+ * (Src/PointLight.cpp:52-140, :235-282; Shaders/ShadowMapBindless.{vert,frag}):
+ * per light a size x size x 6 cube (256 in the reference) of FP32 depth =
+ * min(1, distance / 1000) of the nearest triangle along each texel-centre ray
+ * from the light, 1 where nothing is hit (depth cleared to 1).  As the
+ * reference's pass draws them: back faces culled (counter-clockwise front
+ * faces, GraphicsPipeline.h:171-172), only hits between the view-depth planes
+ * 0.01 and 1000 of the 90-degree face camera.  Layout [light][face][y][x] in
+ * the reference's face images: faces in Vulkan order, face f's texel (x, y)
+ * looks along R_f v, where v is the Vulkan cube direction of that texel and
+ * R_f = diag(-1, 1, 1) for the +-X and +-Z faces, diag(1, 1, -1) for +-Y
+ * (what its yaw / pitch face cameras, Src/PointLight.cpp:73-131 with
+ * Src/Camera.cpp:49-66, render).  d_mesh positions / indices and d_lights are
+ * device memory; d_cubes holds light_count * 6 * size^2 floats. */
+ALTHEA_API althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights,
+                                             int light_count, int size, float* d_cubes, hipStream_t stream);
+
+/* althea_deferred_resolve (optionally reading a reflection buffer, which may
+ * be NULL) with the point-light shadow test of PBRMaterial.glsl:119-140: light
+ * i is skipped at a pixel when 1000 * cube_i(L.x, -L.y, -L.z) < |Lp - P| - 0.5,
+ * L the unit vector from the pixel to the light, the cube read with the
+ * seamless bilinear filter. */
+ALTHEA_API althea_status althea_deferred_resolve_shadowed(
+    const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl, const althea_point_light* d_lights,
+    int light_count, const althea_globals* globals, const althea_ssao_desc* ssao,
+    const althea_reflection_view* reflection, const float* d_shadow_cubes, int shadow_size, unsigned flags,
+    void* d_out, float* d_ao_out, hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* RGBE (.hdr) files                                                         */
 /* ------------------------------------------------------------------------ */
 

@@ -29,6 +29,9 @@
 //                       same-size impulse -> trimmed binomial row 16
 //   reflection blend    alpha 0 -> plain resolve, alpha 1 / 0.5 -> the
 //                       prefiltered term replaced / halved
+//   shadow cubes        empty mesh -> 1; a lit square -> the analytic
+//                       distance; reversed winding, too near -> not written
+//   shadow test         occluded light skipped; the +-Y face mirror (Q19)
 //   mlp                 central finite differences, Adam closed form
 #pragma once
 
@@ -587,9 +590,132 @@ inline double ndf_ggx(double n_dot_h, double a2) {
   double tmp = n_dot_h * n_dot_h * (a2 - 1.0) + 1.0;
   return a2 / (kPi * tmp * tmp);
 }
+// ---------------------------------------------------------------------------
+// f4. Point-light shadow cubes as the reference lays them out and reads them
+// (Src/PointLight.cpp:73-131 face cameras, Src/Camera.cpp:49-66,
+// PBRMaterial.glsl:119-140 lookup at (L.x, -L.y, -L.z)).  Layer f's texel with
+// Vulkan direction v holds the depth along R_f v, R_f = diag(-1, 1, 1) for
+// the +-X / +-Z layers and diag(1, 1, -1) for +-Y, so the lookup reads -L on
+// the side faces and (L.x, -L.y, L.z) on the +-Y faces (Q19).  Depth =
+// min(1, distance / 1000) of the nearest front-facing (counter-clockwise seen
+// from the light) triangle along the texel-centre ray, view depth in
+// [0.01, 1000]; 1 elsewhere.
+// ---------------------------------------------------------------------------
+constexpr double kShadowNear = 0.01, kShadowFar = 1000.0;
+struct ShadowSet {
+  const float* cubes;  // [light][face][y][x]
+  int size;
+};
+inline dvec3 shadow_face_turn(int face, dvec3 v) {
+  return (face == 2 || face == 3) ? dvec3{v.x, v.y, -v.z} : dvec3{-v.x, v.y, v.z};
+}
+inline double cube1_tap(const float* c, int S, int face, int i, int j) {
+  auto tx = [&](int f, int a, int b) { return static_cast<double>(c[(static_cast<size_t>(f) * S + b) * S + a]); };
+  bool in_i = i >= 0 && i < S, in_j = j >= 0 && j < S;
+  if (in_i && in_j) return tx(face, i, j);
+  int f, a, b;
+  if (in_i || in_j) {
+    cube_fold(face, i, j, S, f, a, b);
+    return tx(f, a, b);
+  }
+  int ic = std::min(std::max(i, 0), S - 1), jc = std::min(std::max(j, 0), S - 1);
+  double acc = tx(face, ic, jc);
+  cube_fold(face, i, jc, S, f, a, b);
+  acc += tx(f, a, b);
+  cube_fold(face, ic, j, S, f, a, b);
+  acc += tx(f, a, b);
+  return acc / 3.0;
+}
+// `contrast` (optional): the largest difference between the four taps.
+inline double cube1_sample(const float* c, int S, dvec3 dir, double* contrast = nullptr) {
+  int face;
+  double s, t;
+  cube_select(dir, face, s, t);
+  double x = s * S - 0.5, y = t * S - 0.5;
+  double fx = std::floor(x), fy = std::floor(y);
+  double ax = x - fx, ay = y - fy;
+  int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+  double t00 = cube1_tap(c, S, face, i0, j0), t10 = cube1_tap(c, S, face, i0 + 1, j0);
+  double t01 = cube1_tap(c, S, face, i0, j0 + 1), t11 = cube1_tap(c, S, face, i0 + 1, j0 + 1);
+  if (contrast)
+    *contrast = std::max({t00, t10, t01, t11}) - std::min({t00, t10, t01, t11});
+  double top = t00 + (t10 - t00) * ax, bot = t01 + (t11 - t01) * ax;
+  return top + (bot - top) * ay;
+}
+// PBRMaterial.glsl:130-140 for light i, L the unit vector to the light.
+// `margin` (optional) is lowered to the test's distance from its threshold in
+// units of what an error in the bilinear weights moves (1000 x the taps'
+// contrast) plus 1e-3 x the light distance for the rounding of the depths.
+inline bool shadow_hidden(const ShadowSet& sh, int light, dvec3 L, double ldist, double* margin = nullptr) {
+  const float* c = sh.cubes + static_cast<size_t>(light) * 6 * sh.size * sh.size;
+  double contrast = 0.0;
+  double closest = cube1_sample(c, sh.size, {L.x, -L.y, -L.z}, &contrast) * kShadowFar;
+  if (margin)
+    *margin = std::min(*margin, std::fabs(closest - (ldist - 0.5)) / (kShadowFar * contrast + 1e-3 * ldist));
+  return closest < ldist - 0.5;
+}
+// The cube of one light (6 S S doubles, [face][y][x]) from a mesh.
+inline void shadow_cube(const float* pos, const uint32_t* idx, int triangles, dvec3 lp, int S, double* out) {
+  for (size_t k = 0; k < static_cast<size_t>(6) * S * S; ++k) out[k] = 1.0;
+  for (int t = 0; t < triangles; ++t) {
+    dvec3 v[3];
+    for (int k = 0; k < 3; ++k) {
+      const float* q = pos + 3 * static_cast<size_t>(idx[3 * t + k]);
+      v[k] = dvec3{q[0], q[1], q[2]} - lp;
+    }
+    dvec3 n0 = cross(v[1], v[2]), n1 = cross(v[2], v[0]), n2 = cross(v[0], v[1]);
+    double det = dot(v[0], n0);
+    if (!(det < 0.0)) continue;  // back face or edge-on: culled
+    for (int face = 0; face < 6; ++face) {
+      double sc[3], tc[3], ma[3];
+      int front = 0;
+      for (int k = 0; k < 3; ++k) {
+        dvec3 q = shadow_face_turn(face, v[k]);
+        switch (face) {
+          case 0: ma[k] = q.x; sc[k] = -q.z; tc[k] = -q.y; break;
+          case 1: ma[k] = -q.x; sc[k] = q.z; tc[k] = -q.y; break;
+          case 2: ma[k] = q.y; sc[k] = q.x; tc[k] = q.z; break;
+          case 3: ma[k] = -q.y; sc[k] = q.x; tc[k] = -q.z; break;
+          case 4: ma[k] = q.z; sc[k] = q.x; tc[k] = -q.y; break;
+          default: ma[k] = -q.z; sc[k] = -q.x; tc[k] = -q.y; break;
+        }
+        front += ma[k] > 0.0;
+      }
+      if (front == 0) continue;
+      int x0 = 0, x1 = S - 1, y0 = 0, y1 = S - 1;
+      if (front == 3) {
+        double lx = 1e300, hx = -1e300, ly = 1e300, hy = -1e300;
+        for (int k = 0; k < 3; ++k) {
+          double x = (sc[k] / ma[k] + 1.0) * 0.5 * S - 0.5, y = (tc[k] / ma[k] + 1.0) * 0.5 * S - 0.5;
+          lx = std::min(lx, x), hx = std::max(hx, x), ly = std::min(ly, y), hy = std::max(hy, y);
+        }
+        x0 = std::max(static_cast<int>(std::ceil(clampd(lx, -2.0, S + 1.0))) - 1, 0);
+        x1 = std::min(static_cast<int>(std::floor(clampd(hx, -2.0, S + 1.0))) + 1, S - 1);
+        y0 = std::max(static_cast<int>(std::ceil(clampd(ly, -2.0, S + 1.0))) - 1, 0);
+        y1 = std::min(static_cast<int>(std::floor(clampd(hy, -2.0, S + 1.0))) + 1, S - 1);
+      }
+      for (int j = y0; j <= y1; ++j)
+        for (int i = x0; i <= x1; ++i) {
+          double pc = (2.0 * i + 1.0) / S - 1.0, pt = (2.0 * j + 1.0) / S - 1.0;
+          dvec3 d = shadow_face_turn(face, face_dir(face, pc, pt));
+          double e0 = dot(d, n0), e1 = dot(d, n1), e2 = dot(d, n2);
+          if (e0 > 0.0 || e1 > 0.0 || e2 > 0.0) continue;
+          double rn = e0 + e1 + e2;
+          if (!(rn < 0.0)) continue;
+          double tt = det / rn;
+          if (tt < kShadowNear || tt > kShadowFar) continue;
+          double depth = std::min(tt * length(d) / kShadowFar, 1.0);
+          double& o = out[(static_cast<size_t>(face) * S + j) * S + i];
+          o = std::min(o, depth);
+        }
+    }
+  }
+}
+
 inline dvec3 pbr_material(dvec3 world_pos, dvec3 V, dvec3 N, dvec3 base, dvec3 reflected,
                           dvec3 irradiance, double metallic, double roughness, double ao,
-                          const Lut& lut, const Light* lights, int light_count) {
+                          const Lut& lut, const Light* lights, int light_count,
+                          const ShadowSet* shadows = nullptr, double* margin = nullptr) {
   double n_dot_v = std::max(dot(N, V * -1.0), 0.0);
   dvec3 F0 = lerp3(dvec3{0.04, 0.04, 0.04}, base, metallic);
   double a = roughness * roughness;
@@ -610,6 +736,7 @@ inline dvec3 pbr_material(dvec3 world_pos, dvec3 V, dvec3 N, dvec3 base, dvec3 r
     double d2 = dot(L, L);
     double d = std::sqrt(d2);
     L = L * (1.0 / d);
+    if (shadows && dot(N, L) > 0.0 && shadow_hidden(*shadows, i, L, d, margin)) continue;
     dvec3 radiance = lights[i].emission * (1.0 / d2);
     dvec3 H = normalize(V + L);
     double n_dot_l = std::max(dot(N, L), 0.0);
@@ -710,6 +837,7 @@ struct ResolveInputs {
   bool tonemap, ssao_on;
   SsaoDesc ssao;
   const struct ReflChain* reflection = nullptr;  // DeferredPass.frag:59-68 when set
+  const ShadowSet* shadows = nullptr;            // PBRMaterial.glsl:119-140 when set
 };
 inline dvec3 view_ray_uv(const ResolveInputs& in, double u, double v) {
   double ndc[4] = {u * 2.0 - 1.0, v * 2.0 - 1.0, 0.0, 1.0}, e[4];
@@ -973,7 +1101,7 @@ inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_o
   }
   if (ao_out) *ao_out = ao;
   dvec3 c = pbr_material(P, I, N, base, refl, irr, metallic, roughness, ao, in.lut, in.lights,
-                         in.light_count);
+                         in.light_count, in.shadows, margin);
   if (in.tonemap) c = tonemap(c, in.exposure);
   return c;
 }

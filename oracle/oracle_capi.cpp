@@ -225,9 +225,12 @@ struct oracle_resolve_args {
   float ssao_bias;
   const float* reflection;  // optional RGBA32F reflection buffer, all levels
   int refl_width, refl_height, refl_mips;
+  const float* shadow_cubes;  // optional [light][face][y][x]
+  int shadow_size;
 };
 
-static ResolveInputs resolve_inputs(const oracle_resolve_args* a, const std::vector<Light>& lights, ReflChain* chain) {
+static ResolveInputs resolve_inputs(const oracle_resolve_args* a, const std::vector<Light>& lights, ReflChain* chain,
+                                    ShadowSet* shadows) {
   ResolveInputs in;
   in.position = a->position;
   in.normal = a->normal;
@@ -253,6 +256,10 @@ static ResolveInputs resolve_inputs(const oracle_resolve_args* a, const std::vec
     *chain = {a->reflection, a->refl_width, a->refl_height, a->refl_mips};
     in.reflection = chain;
   }
+  if (a->shadow_cubes) {
+    *shadows = {a->shadow_cubes, a->shadow_size};
+    in.shadows = shadows;
+  }
   return in;
 }
 
@@ -273,7 +280,8 @@ void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int coun
                        double* out3, double* out_ao, double* out_margin, double* out_query) {
   std::vector<Light> lights = resolve_lights(a);
   ReflChain chain;
-  ResolveInputs in = resolve_inputs(a, lights, &chain);
+  ShadowSet shadows;
+  ResolveInputs in = resolve_inputs(a, lights, &chain, &shadows);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < count; ++i) {
     double ao, margin;
@@ -297,7 +305,8 @@ void oracle_ssr(const oracle_resolve_args* a, const int* pixels, int count, doub
                 int* out_step, double* out_uv, double* out_pos) {
   std::vector<Light> lights = resolve_lights(a);
   ReflChain chain;
-  ResolveInputs in = resolve_inputs(a, lights, &chain);
+  ShadowSet shadows;
+  ResolveInputs in = resolve_inputs(a, lights, &chain, &shadows);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < count; ++i) {
     double margin, uv[2] = {std::nan(""), std::nan("")}, pos[3] = {std::nan(""), std::nan(""), std::nan("")};
@@ -324,6 +333,18 @@ void oracle_reflection_sample(const float* data, int W, int H, int mips, const i
                               int count, double* out4) {
   ReflChain c{data, W, H, mips};
   for (int i = 0; i < count; ++i) reflection_sample(c, pixels[2 * i], pixels[2 * i + 1], lods[i], out4 + 4 * i);
+}
+
+// Shadow cubes of `light_count` lights ([n][8] floats) from a mesh:
+// out [light][face][y][x] doubles.
+void oracle_shadow_cubes(const float* positions, const uint32_t* indices, int triangles, const float* lights,
+                         int light_count, int size, double* out) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int l = 0; l < light_count; ++l) {
+    const float* lp = lights + 8 * l;
+    shadow_cube(positions, indices, triangles, {lp[0], lp[1], lp[2]}, size,
+                out + static_cast<size_t>(l) * 6 * size * size);
+  }
 }
 
 // pbrMaterial for one shading point.  vec: pos, V, N, base, refl, irr (18

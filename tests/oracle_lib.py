@@ -23,7 +23,8 @@ class ResolveArgs(C.Structure):
                 ("projection", vp), ("inverse_projection", vp), ("view", vp), ("inverse_view", vp),
                 ("exposure", C.c_float), ("tonemap", i32), ("ssao_on", i32), ("ssao_kernel", vp),
                 ("ssao_noise", vp), ("ssao_samples", i32), ("ssao_radius", C.c_float), ("ssao_bias", C.c_float),
-                ("reflection", vp), ("refl_width", i32), ("refl_height", i32), ("refl_mips", i32)]
+                ("reflection", vp), ("refl_width", i32), ("refl_height", i32), ("refl_mips", i32),
+                ("shadow_cubes", vp), ("shadow_size", i32)]
 
 
 _L = None
@@ -64,6 +65,7 @@ def lib() -> C.CDLL:
         L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
         L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
+        L.oracle_shadow_cubes.argtypes = [vp, vp, i32, vp, i32, i32, vp]
         L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
         L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
         L.oracle_mlp_param_count.restype = i32
@@ -242,9 +244,11 @@ class Env:
 
 
 # ----------------------------------------------------------------- resolve
-def _resolve_args(scene, views: dict, env, tonemap: bool, gbuffer, reflection, keep: list) -> ResolveArgs:
+def _resolve_args(scene, views: dict, env, tonemap: bool, gbuffer, reflection, keep: list,
+                  shadows=None) -> ResolveArgs:
     """ResolveArgs over numpy copies that `keep` holds alive.  `reflection`:
-    (float32 array of every level, (texels, 4), mips) or None."""
+    (float32 array of every level, (texels, 4), mips) or None; `shadows`:
+    float32 cubes (lights, 6, S, S) or None."""
     planes = gbuffer if gbuffer is not None else (scene.position, scene.normal, scene.albedo, scene.material)
     planes = [_c(p, np.float32) for p in planes]
     pref = _c(views["prefiltered"], np.float32)
@@ -256,26 +260,29 @@ def _resolve_args(scene, views: dict, env, tonemap: bool, gbuffer, reflection, k
     kern = _c(scene.ssao_kernel, np.float32)
     noise = _c(scene.ssao_noise, np.float32)
     chain, mips = (None, 0) if reflection is None else (_c(reflection[0], np.float32), int(reflection[1]))
-    keep += [planes, pref, irr, lut, lights, g, mats, kern, noise, chain]
+    cubes = None if shadows is None else _c(shadows, np.float32)
+    keep += [planes, pref, irr, lut, lights, g, mats, kern, noise, chain, cubes]
     return ResolveArgs(_p(planes[0]), _p(planes[1]), _p(planes[2]), _p(planes[3]), scene.width, scene.height,
                        env.handle if env is not None else None, _p(pref), views["pref_size"], views["pref_mips"],
                        _p(irr), views["irr_size"], _p(lut), views["lut_size"], _p(lights), lights.shape[0],
                        _p(mats[0]), _p(mats[1]), _p(mats[2]), _p(mats[3]), float(g[101]), 1 if tonemap else 0,
                        1 if scene.ssao_enabled else 0, _p(kern), _p(noise), kern.shape[0], scene.ssao_radius,
-                       scene.ssao_bias, _p(chain) if chain is not None else None, scene.width, scene.height, mips)
+                       scene.ssao_bias, _p(chain) if chain is not None else None, scene.width, scene.height, mips,
+                       _p(cubes) if cubes is not None else None, cubes.shape[-1] if cubes is not None else 0)
 
 
 def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer=None, refl=None,
-            query: bool = False, reflection=None):
+            query: bool = False, reflection=None, shadows=None):
     """Oracle resolve of pixels[:, (x, y)] for a synth.Scene and IBL views
     (numpy dict with prefiltered / irradiance / lut arrays and sizes).
     Returns (rgb, ao, decision margin), plus the (R, roughness) query of each
     pixel's prefiltered lookup when `query`.  `refl` (n, 3): prefiltered
     radiance replacing the cube lookup where refl[i, 0] is not NaN.
     `reflection`: (levels as a (texels, 4) float32 array, mips), the
-    reflection buffer blended into the prefiltered term."""
+    reflection buffer blended into the prefiltered term.  `shadows`: float32
+    shadow cubes (lights, 6, S, S) read by the point-light shadow test."""
     keep = []
-    a = _resolve_args(scene, views, env, tonemap, gbuffer, reflection, keep)
+    a = _resolve_args(scene, views, env, tonemap, gbuffer, reflection, keep, shadows)
     px = _c(pixels, np.int32)
     rgb = np.empty((px.shape[0], 3), np.float64)
     ao = np.empty(px.shape[0], np.float64)
@@ -285,6 +292,17 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
     lib().oracle_resolve_ex(C.byref(a), _p(px), px.shape[0], _p(r_in) if r_in is not None else None, _p(rgb),
                             _p(ao), _p(margin), _p(q) if q is not None else None)
     return (rgb, ao, margin, q) if query else (rgb, ao, margin)
+
+
+def shadow_cubes(positions, indices, lights, size: int) -> np.ndarray:
+    """Oracle shadow cubes (lights, 6, size, size), FP64, of a mesh
+    (positions (V, 3), indices (T, 3)) for (n, 8) point lights."""
+    pos = _c(positions, np.float32)
+    idx = _c(indices, np.uint32)
+    lt = _c(lights, np.float32).reshape(-1, 8)
+    out = np.empty((lt.shape[0], 6, size, size), np.float64)
+    lib().oracle_shadow_cubes(_p(pos), _p(idx), idx.size // 3, _p(lt), lt.shape[0], size, _p(out))
+    return out
 
 
 def ssr(scene, views: dict, pixels, gbuffer=None):

@@ -85,6 +85,37 @@ def test_reflection_queries_and_argument_checks_without_gpu():
     assert lib.althea_reflection_convolve(None, None) == 1
 
 
+def test_shadow_argument_checks_without_gpu():
+    """althea_shadow_cubes / althea_deferred_resolve_shadowed reject bad
+    arguments before any device work (dummy pointers are never read)."""
+    from althea_amd import _lib
+
+    lib = _lib.lib()
+    dummy = 0x1000
+    mesh = _lib.MeshViewC(dummy, dummy, 3, 1)
+    assert lib.althea_shadow_cubes(C.byref(mesh), dummy, 1, 0, dummy, None) == 1
+    assert b"size 0" in lib.althea_last_error()
+    assert lib.althea_shadow_cubes(C.byref(mesh), dummy, 1, 4097, dummy, None) == 1
+    assert lib.althea_shadow_cubes(C.byref(mesh), dummy, 0, 16, dummy, None) == 1
+    assert lib.althea_shadow_cubes(C.byref(mesh), None, 1, 16, dummy, None) == 1
+    assert lib.althea_shadow_cubes(None, dummy, 1, 16, dummy, None) == 1
+    no_vertices = _lib.MeshViewC(dummy, dummy, 0, 1)
+    assert lib.althea_shadow_cubes(C.byref(no_vertices), dummy, 1, 16, dummy, None) == 1
+    huge = _lib.MeshViewC(dummy, dummy, 3, 1 << 30)
+    assert lib.althea_shadow_cubes(C.byref(huge), dummy, 2, 16, dummy, None) == 1
+    g = _lib.GBufferViewC(dummy, dummy, dummy, dummy, 64, 32, 0)
+    v = _lib.IBLViewsC()
+    v.environment, v.env_width, v.env_height = dummy, 64, 32
+    v.prefiltered, v.prefiltered_size, v.prefiltered_mips = dummy, 16, 3
+    v.irradiance, v.irradiance_size, v.brdf_lut, v.brdf_lut_size = dummy, 8, dummy, 16
+    glob = _lib.GlobalsC()
+    assert lib.althea_deferred_resolve_shadowed(C.byref(g), C.byref(v), dummy, 1, C.byref(glob), None, None, None,
+                                                256, 0, dummy, None, None) == 1
+    assert b"shadow cubes" in lib.althea_last_error()
+    assert lib.althea_deferred_resolve_shadowed(C.byref(g), C.byref(v), dummy, 1, C.byref(glob), None, None, dummy,
+                                                0, 0, dummy, None, None) == 1
+
+
 def test_facade_header_compiles():
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
                     "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "facade_demo.cpp")], check=True)
