@@ -402,14 +402,21 @@ class PointLightCollection:
     rgb, pad, as PointLight in Include/Althea/PointLight.h), `shadow_size`
     the cube extent (256 in the reference, PointLight.cpp:35).  drawShadowMaps
     renders the cubes of every light from a triangle mesh (world-space float32
-    positions (v, 3), uint32 indices (t, 3), device tensors)."""
+    positions (v, 3), uint32 indices (t, 3), device tensors).  The cubes are
+    stored with a one-texel border per face (`shadow_storage`, (n, 6, S + 2,
+    S + 2)); getShadowMapImage is the (n, 6, S, S) view of the faces."""
 
     def __init__(self, light_count: int, create_shadow_map: bool = True, shadow_size: int = 256):
         torch = _torch()
         self.lights = torch.zeros((light_count, 8), dtype=torch.float32, device="cuda")
         self.shadow_size = shadow_size
-        self.shadow_maps = (torch.ones((light_count, 6, shadow_size, shadow_size), dtype=torch.float32,
-                                       device="cuda") if create_shadow_map else None)
+        P = shadow_size + 2
+        self.shadow_storage = (torch.ones((light_count, 6, P, P), dtype=torch.float32, device="cuda")
+                               if create_shadow_map else None)
+
+    @property
+    def shadow_maps(self):
+        return None if self.shadow_storage is None else self.shadow_storage[:, :, 1:-1, 1:-1]
 
     @classmethod
     def from_lights(cls, lights, create_shadow_map: bool = True, shadow_size: int = 256):
@@ -440,8 +447,9 @@ class PointLightCollection:
         tris = indices.numel() // 3
         mesh = _lib.MeshViewC(_ptr(positions) if tris else None, _ptr(indices) if tris else None,
                               int(positions.numel() // 3), tris)
+        assert self.shadow_storage[0].numel() == lib().althea_shadow_cube_floats(self.shadow_size)
         check(lib().althea_shadow_cubes(C.byref(mesh), _ptr(self.lights), self.getCount(), self.shadow_size,
-                                        _ptr(self.shadow_maps), _stream(stream)),
+                                        _ptr(self.shadow_storage), _stream(stream)),
               "althea_shadow_cubes")
         return self.shadow_maps
 
@@ -474,13 +482,13 @@ class DeferredRendering:
         flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)
         ssao_p = C.byref(ssao) if ssao is not None else None
         if shadows is not None:
-            if shadows.shadow_maps is None or shadows.getCount() < n:
+            if shadows.shadow_storage is None or shadows.getCount() < n:
                 raise ValueError("shadows: need one shadow cube per light")
             rv = reflection.view() if reflection is not None else None
             check(lib().althea_deferred_resolve_shadowed(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
                                                          C.byref(globals_), ssao_p,
                                                          C.byref(rv) if rv is not None else None,
-                                                         _ptr(shadows.shadow_maps), shadows.shadow_size, flags,
+                                                         _ptr(shadows.shadow_storage), shadows.shadow_size, flags,
                                                          _ptr(out), _ptr(ao_out), _stream(stream)),
                   "althea_deferred_resolve_shadowed")
             return out

@@ -125,6 +125,7 @@ PROTOTYPES = {
     "althea_deferred_resolve_reflect": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                               C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),
                                               C.c_uint, vp, vp, vp]),
+    "althea_shadow_cube_floats": (sz, [i32]),
     "althea_shadow_cubes": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp]),
     "althea_deferred_resolve_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                                C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),

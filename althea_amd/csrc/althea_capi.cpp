@@ -546,6 +546,10 @@ althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, in
   return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
 }
 
+size_t althea_shadow_cube_floats(int size) {
+  return size < 1 ? 0 : static_cast<size_t>(6) * (size + 2) * (size + 2);
+}
+
 althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights, int light_count,
                                   int size, float* d_cubes, hipStream_t stream) {
   ALTHEA_REQUIRE(mesh && d_cubes && light_count >= 1 && d_lights, "shadow_cubes: null argument or no light");

@@ -39,35 +39,37 @@ __device__ __forceinline__ void shadow_face_coords(int face, f3 p, float& sc, fl
   }
 }
 
-// Seamless bilinear read of one scalar cube (face-major [face][y][x]) at
-// projected face coordinates, with the same edge and corner rules as the
-// float4 cubes (cube_tap).
-__device__ __forceinline__ float cube1_texel(const float* __restrict__ c, int S, int face, int i, int j) {
-  return c[(face * S + j) * S + i];
+// Storage: each face image with a one-texel border, (S + 2)^2 floats per
+// face, face texel (i, j) at row j + 1, column i + 1.  The border holds what
+// the seamless filter reads one texel beyond the face (the edge texels of the
+// adjacent faces; at the corners the mean of the three texels that meet
+// there), written once by the producer, so a lookup is four plain loads.
+__device__ __forceinline__ int shadow_texel_index(int S, int face, int i, int j) {
+  const int P = S + 2;
+  return (face * P + j + 1) * P + i + 1;
 }
 
-__device__ inline float cube1_tap(const float* __restrict__ c, int S, int face, int i, int j) {
-  const bool in_i = static_cast<unsigned>(i) < static_cast<unsigned>(S);
-  const bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(S);
-  if (in_i && in_j) return cube1_texel(c, S, face, i, j);
-  int f, a, b;
-  if (in_i || in_j) {
-    cube_fold(face, i, j, S, f, a, b);
-    return cube1_texel(c, S, f, a, b);
-  }
-  const int ic = min(max(i, 0), S - 1), jc = min(max(j, 0), S - 1);
-  float acc = cube1_texel(c, S, face, ic, jc);
-  cube_fold(face, i, jc, S, f, a, b);
-  acc += cube1_texel(c, S, f, a, b);
-  cube_fold(face, ic, j, S, f, a, b);
-  acc += cube1_texel(c, S, f, a, b);
-  return acc * (1.0f / 3.0f);
+// Face and projected coordinates of a direction, as cube_select, with
+// selects instead of branches.
+__device__ __forceinline__ void shadow_cube_select(f3 d, int& face, float& pc, float& pt) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  const bool xm = ax >= ay && ax >= az;
+  const bool ym = !xm && ay >= az;
+  const bool px = d.x >= 0.0f, py = d.y >= 0.0f, pz = d.z >= 0.0f;
+  const float ma = xm ? ax : (ym ? ay : az);
+  const float sc = xm ? (px ? -d.z : d.z) : (ym ? d.x : (pz ? d.x : -d.x));
+  const float tc = xm ? -d.y : (ym ? (py ? d.z : -d.z) : -d.y);
+  face = xm ? (px ? 0 : 1) : (ym ? (py ? 2 : 3) : (pz ? 4 : 5));
+  const float inv = rcp_fast(ma);
+  pc = sc * inv;
+  pt = tc * inv;
 }
 
-__device__ inline float cube1_sample(const float* __restrict__ c, int S, f3 dir) {
+// Seamless bilinear read of one bordered scalar cube.
+__device__ __forceinline__ float shadow_cube_sample(const float* __restrict__ c, int S, f3 dir) {
   int face;
   float pc, pt;
-  cube_select(dir, face, pc, pt);
+  shadow_cube_select(dir, face, pc, pt);
   const float half = 0.5f * static_cast<float>(S);
   // pc, pt lie in [-1, 1]; the clamp only keeps a degenerate direction (NaN
   // coordinates) on the texels.
@@ -76,20 +78,10 @@ __device__ inline float cube1_sample(const float* __restrict__ c, int S, f3 dir)
   const float y = fminf(fmaxf(fmaf(pt, half, half - 0.5f), -0.5f), hi);
   const float fx = floorf(x), fy = floorf(y);
   const float ax = x - fx, ay = y - fy;
-  const int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
-  float t00, t10, t01, t11;
-  if (i0 >= 0 && j0 >= 0 && i0 + 1 < S && j0 + 1 < S) {
-    const float* r0 = c + (face * S + j0) * S + i0;
-    t00 = r0[0];
-    t10 = r0[1];
-    t01 = r0[S];
-    t11 = r0[S + 1];
-  } else {
-    t00 = cube1_tap(c, S, face, i0, j0);
-    t10 = cube1_tap(c, S, face, i0 + 1, j0);
-    t01 = cube1_tap(c, S, face, i0, j0 + 1);
-    t11 = cube1_tap(c, S, face, i0 + 1, j0 + 1);
-  }
+  const int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);  // in [-1, S - 1]
+  const float* r0 = c + shadow_texel_index(S, face, i0, j0);
+  const int P = S + 2;
+  const float t00 = r0[0], t10 = r0[1], t01 = r0[P], t11 = r0[P + 1];
   const float top = fmaf(t10 - t00, ax, t00), bot = fmaf(t11 - t01, ax, t01);
   return fmaf(bot - top, ay, top);
 }
@@ -98,11 +90,11 @@ __device__ inline float cube1_sample(const float* __restrict__ c, int S, f3 dir)
 // light is hidden (closestDepth < Ldist - 0.5).  L = Lp - P, unnormalised.
 struct ShadowCubes {
   static constexpr bool kActive = true;
-  const float* cubes;  // [light][face][y][x]
+  const float* cubes;  // [light][face][y + 1][x + 1], bordered faces
   int size;
   __device__ __forceinline__ bool hidden(int light, f3 L, float ldist) const {
-    const float* c = cubes + static_cast<size_t>(light) * 6 * size * size;
-    const float closest = cube1_sample(c, size, mk(L.x, -L.y, -L.z)) * ALTHEA_SHADOW_FAR;
+    const float* c = cubes + static_cast<size_t>(light) * 6 * (size + 2) * (size + 2);
+    const float closest = shadow_cube_sample(c, size, mk(L.x, -L.y, -L.z)) * ALTHEA_SHADOW_FAR;
     return closest < ldist - 0.5f;
   }
 };

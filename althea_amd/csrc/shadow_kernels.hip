@@ -16,13 +16,19 @@
 // light-relative vertices, which cancels in FP32 for small, distant
 // triangles.
 //
-// One lane per (light, triangle): it projects the triangle onto each face,
-// takes the texel box its vertices span (the whole face when a vertex lies
-// behind the face's plane), and writes each hit with an integer atomicMin on
+// One lane per (light, triangle): it clips the triangle to the part of each
+// face's view volume where a hit can be written (view depth ma >= near / 2:
+// the view depth of a hit on a face is its coordinate ma along the face axis,
+// and hits nearer than the near plane are not written), takes the texel box
+// that part projects to (the projection of a convex polygon in front of the
+// camera is spanned by its projected vertices: the triangle's vertices in
+// front and the points where its edges cross ma = near / 2), and writes each
+// hit with an integer atomicMin on
 // the depth's bits (non-negative floats order as their bit patterns), so the
 // result does not depend on the order of the writes.  Small boxes are walked
 // by their own lane; a wave walks larger boxes together, one lane's box at a
-// time.
+// time.  A last pass fills each face's border from the adjacent faces (the
+// seamless filter's reads beyond the edge), so the lookup needs no folding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +41,7 @@
 namespace althea {
 
 constexpr int kShadowLaneTexels = 16;  // boxes up to this many texels stay on their lane
+constexpr float kShadowClip = 0.5f * ALTHEA_SHADOW_NEAR;  // clip plane ma = near / 2
 
 struct ShadowParams {
   const float* positions;
@@ -78,7 +85,7 @@ __device__ __forceinline__ void shadow_texel(const ShadowTri& tri, int face, int
   const float t = tri.det / rn;  // view depth of the hit
   if (t < ALTHEA_SHADOW_NEAR || t > ALTHEA_SHADOW_FAR) return;
   const float depth = fminf(t * len * (1.0f / ALTHEA_SHADOW_FAR), 1.0f);
-  atomicMin(reinterpret_cast<unsigned int*>(cube) + (face * S + j) * S + i, __float_as_uint(depth));
+  atomicMin(reinterpret_cast<unsigned int*>(cube) + shadow_texel_index(S, face, i, j), __float_as_uint(depth));
 }
 
 __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p) {
@@ -104,21 +111,32 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
   const int S = p.size;
   const float inv_s = 1.0f / static_cast<float>(S);
   const float half = 0.5f * static_cast<float>(S);
-  float* cube = p.cubes + static_cast<size_t>(light) * 6 * S * S;
+  const size_t cube_floats = static_cast<size_t>(6) * (S + 2) * (S + 2);
+  float* cube = p.cubes + light * cube_floats;
   for (int face = 0; face < 6; ++face) {
     int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
     if (live) {
       float sc[3], tc[3], ma[3];
-      int front = 0;
+      bool in[3];
       for (int k = 0; k < 3; ++k) {
         shadow_face_coords(face, v[k], sc[k], tc[k], ma[k]);
-        front += ma[k] > 0.0f;
+        in[k] = ma[k] >= kShadowClip;
       }
-      if (front == 3) {
+      if (in[0] || in[1] || in[2]) {
         float lo_x = 1e30f, hi_x = -1e30f, lo_y = 1e30f, hi_y = -1e30f;
-        for (int k = 0; k < 3; ++k) {
-          const float x = fmaf(sc[k] / ma[k], half, half - 0.5f), y = fmaf(tc[k] / ma[k], half, half - 0.5f);
+        auto add = [&](float s_, float t_, float m_) {
+          const float r = 1.0f / m_;
+          const float x = fmaf(s_ * r, half, half - 0.5f), y = fmaf(t_ * r, half, half - 0.5f);
           lo_x = fminf(lo_x, x), hi_x = fmaxf(hi_x, x), lo_y = fminf(lo_y, y), hi_y = fmaxf(hi_y, y);
+        };
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int k2 = k == 2 ? 0 : k + 1;
+          if (in[k]) add(sc[k], tc[k], ma[k]);
+          if (in[k] != in[k2]) {  // the edge crosses the clip plane
+            const float u = (kShadowClip - ma[k]) / (ma[k2] - ma[k]);
+            add(fmaf(u, sc[k2] - sc[k], sc[k]), fmaf(u, tc[k2] - tc[k], tc[k]), kShadowClip);
+          }
         }
         // Texel centres inside the span, one texel of margin for rounding
         // (coordinates clamped to [-2, S + 1] before the integer conversion).
@@ -127,8 +145,6 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
         x1 = min(static_cast<int>(floorf(fminf(fmaxf(hi_x, -2.0f), lim))) + 1, S - 1);
         y0 = max(static_cast<int>(ceilf(fminf(fmaxf(lo_y, -2.0f), lim))) - 1, 0);
         y1 = min(static_cast<int>(floorf(fminf(fmaxf(hi_y, -2.0f), lim))) + 1, S - 1);
-      } else if (front > 0) {
-        x0 = 0, x1 = S - 1, y0 = 0, y1 = S - 1;  // crosses the face's plane
       }
     }
     const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
@@ -149,7 +165,7 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
       w.n = mk(__shfl(tri.n.x, src), __shfl(tri.n.y, src), __shfl(tri.n.z, src));
       w.det = __shfl(tri.det, src);
       const int wx0 = __shfl(x0, src), wy0 = __shfl(y0, src), ww = __shfl(bw, src), wh = __shfl(bh, src);
-      float* wcube = p.cubes + static_cast<size_t>(__shfl(light, src)) * 6 * S * S;
+      float* wcube = p.cubes + __shfl(light, src) * cube_floats;
       const int n = ww * wh;
       for (int k = threadIdx.x & 63; k < n; k += 64) {
         const int j = wy0 + k / ww, i = wx0 + k % ww;
@@ -159,13 +175,55 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
   }
 }
 
+// Border texel k of face f of light l: the seamless filter's tap one texel
+// beyond the face (cube_fold onto the adjacent face; at a corner the mean of
+// the clamped texel and its two folds, summed in that order).
+__device__ __forceinline__ float shadow_interior(const float* c, int S, int face, int i, int j) {
+  return c[shadow_texel_index(S, face, i, j)];
+}
+
+__global__ __launch_bounds__(256) void shadow_border_kernel(float* __restrict__ cubes, int S, int light_count) {
+  const int ring = 4 * S + 4;
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= light_count * 6 * ring) return;
+  const int k = item % ring, lf = item / ring;
+  const int face = lf % 6, light = lf / 6;
+  int i, j;
+  if (k < S + 2) {
+    i = k - 1, j = -1;
+  } else if (k < 2 * (S + 2)) {
+    i = k - (S + 2) - 1, j = S;
+  } else if (k < 2 * (S + 2) + S) {
+    i = -1, j = k - 2 * (S + 2);
+  } else {
+    i = S, j = k - 2 * (S + 2) - S;
+  }
+  float* c = cubes + static_cast<size_t>(light) * 6 * (S + 2) * (S + 2);
+  const bool in_i = i >= 0 && i < S, in_j = j >= 0 && j < S;
+  int f, a, b;
+  float v;
+  if (in_i || in_j) {
+    cube_fold(face, i, j, S, f, a, b);
+    v = shadow_interior(c, S, f, a, b);
+  } else {
+    const int ic = min(max(i, 0), S - 1), jc = min(max(j, 0), S - 1);
+    float acc = shadow_interior(c, S, face, ic, jc);
+    cube_fold(face, i, jc, S, f, a, b);
+    acc += shadow_interior(c, S, f, a, b);
+    cube_fold(face, ic, j, S, f, a, b);
+    acc += shadow_interior(c, S, f, a, b);
+    v = acc * (1.0f / 3.0f);
+  }
+  c[shadow_texel_index(S, face, i, j)] = v;
+}
+
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
                                   int size, float* cubes, hipStream_t s) {
-  const size_t texels = static_cast<size_t>(light_count) * 6 * size * size;
+  const size_t texels = static_cast<size_t>(light_count) * 6 * (size + 2) * (size + 2);
   hipLaunchKernelGGL(shadow_clear_kernel, dim3(static_cast<unsigned>((texels + 255) / 256)), dim3(256), 0, s, cubes,
                      texels);
   althea_status st = check_launch("shadow_clear_kernel");
-  if (st != ALTHEA_OK || mesh->triangle_count == 0) return st;
+  if (st != ALTHEA_OK || mesh->triangle_count == 0) return st;  // borders of a cleared cube are 1 too
   ShadowParams p{};
   p.positions = mesh->positions;
   p.indices = mesh->indices;
@@ -177,7 +235,12 @@ althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_poi
   p.cubes = cubes;
   const size_t items = static_cast<size_t>(light_count) * mesh->triangle_count;
   hipLaunchKernelGGL(shadow_raster_kernel, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, p);
-  return check_launch("shadow_raster_kernel");
+  st = check_launch("shadow_raster_kernel");
+  if (st != ALTHEA_OK) return st;
+  const size_t border = static_cast<size_t>(light_count) * 6 * (4 * size + 4);
+  hipLaunchKernelGGL(shadow_border_kernel, dim3(static_cast<unsigned>((border + 255) / 256)), dim3(256), 0, s, cubes,
+                     size, light_count);
+  return check_launch("shadow_border_kernel");
 }
 
 }  // namespace althea

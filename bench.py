@@ -271,7 +271,7 @@ def shadow_frame(torch, aa, scene, ibl, gbuf, steps: int) -> dict:
     frame_ms = time_events(torch, frame, steps)
     cubes = coll.getShadowMapImage()
     covered = float((cubes < 1.0).float().mean().item())
-    cube_bytes = cubes.numel() * 4
+    cube_bytes = coll.shadow_storage.numel() * 4  # faces with their one-texel borders
     px = WIDTH * HEIGHT
     return {"lights": LIGHTS, "cube_size": SHADOW_SIZE, "triangles": int(idx.shape[0]),
             "draw_ms": draw_ms, "draw_texels_per_s": cubes.numel() / (draw_ms * 1e-3),

@@ -411,13 +411,18 @@ typedef struct althea_mesh_view {
  * from the light, 1 where nothing is hit (depth cleared to 1).  As the
  * reference's pass draws them: back faces culled (counter-clockwise front
  * faces, GraphicsPipeline.h:171-172), only hits between the view-depth planes
- * 0.01 and 1000 of the 90-degree face camera.  Layout [light][face][y][x] in
- * the reference's face images: faces in Vulkan order, face f's texel (x, y)
- * looks along R_f v, where v is the Vulkan cube direction of that texel and
- * R_f = diag(-1, 1, 1) for the +-X and +-Z faces, diag(1, 1, -1) for +-Y
- * (what its yaw / pitch face cameras, Src/PointLight.cpp:73-131 with
- * Src/Camera.cpp:49-66, render).  d_mesh positions / indices and d_lights are
- * device memory; d_cubes holds light_count * 6 * size^2 floats. */
+ * 0.01 and 1000 of the 90-degree face camera.  The face images are the
+ * reference's: faces in Vulkan order, face f's texel (x, y) looks along R_f v,
+ * where v is the Vulkan cube direction of that texel and R_f = diag(-1, 1, 1)
+ * for the +-X and +-Z faces, diag(1, 1, -1) for +-Y (what its yaw / pitch face
+ * cameras, Src/PointLight.cpp:73-131 with Src/Camera.cpp:49-66, render).
+ * Storage: [light][face][y + 1][x + 1] with a one-texel border around each
+ * face, (size + 2)^2 floats per face (althea_shadow_cube_floats): the border
+ * holds what the seamless filter reads beyond the face's edge (the adjacent
+ * face's edge texel; at a corner the mean of the three texels meeting there),
+ * filled by this call.  d_mesh positions / indices and d_lights are device
+ * memory; d_cubes holds light_count * althea_shadow_cube_floats(size) floats. */
+ALTHEA_API size_t althea_shadow_cube_floats(int size);
 ALTHEA_API althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights,
                                              int light_count, int size, float* d_cubes, hipStream_t stream);
 
@@ -425,7 +430,8 @@ ALTHEA_API althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const
  * be NULL) with the point-light shadow test of PBRMaterial.glsl:119-140: light
  * i is skipped at a pixel when 1000 * cube_i(L.x, -L.y, -L.z) < |Lp - P| - 0.5,
  * L the unit vector from the pixel to the light, the cube read with the
- * seamless bilinear filter. */
+ * seamless bilinear filter.  d_shadow_cubes: light_count bordered cubes as
+ * althea_shadow_cubes writes them. */
 ALTHEA_API althea_status althea_deferred_resolve_shadowed(
     const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl, const althea_point_light* d_lights,
     int light_count, const althea_globals* globals, const althea_ssao_desc* ssao,

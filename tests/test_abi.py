@@ -91,6 +91,8 @@ def test_shadow_argument_checks_without_gpu():
     from althea_amd import _lib
 
     lib = _lib.lib()
+    assert lib.althea_shadow_cube_floats(256) == 6 * 258 * 258
+    assert lib.althea_shadow_cube_floats(0) == 0
     dummy = 0x1000
     mesh = _lib.MeshViewC(dummy, dummy, 3, 1)
     assert lib.althea_shadow_cubes(C.byref(mesh), dummy, 1, 0, dummy, None) == 1

@@ -5,6 +5,9 @@ the C ABI:
   cubes of the synthetic heightfield mesh for several lights, texel by texel;
   the clear value without triangles; culled back faces; odd cube sizes; a
   large occluder whose boxes span whole faces (the wave-cooperative walk);
+- the one-texel border of every stored face: the seamless filter's reads
+  beyond the edge (cube_fold onto the adjacent face, corners the mean of
+  three), bit for bit;
 - althea_deferred_resolve_shadowed (PBRMaterial.glsl:119-140): against the
   oracle resolve reading the same cubes, with and without a reflection
   buffer; cubes of 1 give the plain resolve.
@@ -117,6 +120,42 @@ def test_large_occluders_span_whole_faces(torch, size):
     print("box", size, r)
     assert r["hit_disagree"] <= 2e-3 * r["hits"] + 2, r
     assert r["depth_off"] <= 2e-3 * r["hits"] + 2, r
+
+
+def _expected_border(faces, S):
+    """The bordered storage of one cube's faces (6, S, S) float32: each border
+    texel is the seamless filter's tap beyond the edge, in FP32 as the GPU
+    sums it."""
+    P = S + 2
+    out = np.zeros((6, P, P), np.float32)
+    out[:, 1:-1, 1:-1] = faces
+    third = np.float32(1.0 / 3.0)
+    ring = [(i, -1) for i in range(-1, S + 1)] + [(i, S) for i in range(-1, S + 1)]
+    ring += [(-1, j) for j in range(S)] + [(S, j) for j in range(S)]
+    for f in range(6):
+        for i, j in ring:
+            in_i, in_j = 0 <= i < S, 0 <= j < S
+            if in_i or in_j:
+                g, a, b = ol.cube_fold(f, i, j, S)
+                v = faces[g, b, a]
+            else:
+                ic, jc = min(max(i, 0), S - 1), min(max(j, 0), S - 1)
+                g1, a1, b1 = ol.cube_fold(f, i, jc, S)
+                g2, a2, b2 = ol.cube_fold(f, ic, j, S)
+                v = ((faces[f, jc, ic] + faces[g1, b1, a1]) + faces[g2, b2, a2]) * third
+            out[f, j + 1, i + 1] = v
+    return out
+
+
+@pytest.mark.parametrize("size", [1, 2, 37])
+def test_face_borders_hold_the_seamless_taps(torch, size):
+    sc = aa.synth.scene(seed=SEED0 + 35, width=8, height=8, lights=3)
+    pos, idx = aa.synth.heightfield_mesh(SEED0 + 35, n=24)
+    coll, faces = _gpu_cubes(torch, pos, idx, sc.lights, size)
+    stored = coll.shadow_storage.cpu().numpy()
+    assert stored.shape == (3, 6, size + 2, size + 2)
+    for l in range(3):
+        np.testing.assert_array_equal(stored[l], _expected_border(faces[l], size))
 
 
 def _shadow_scene(torch, W, H, lights, seed):
