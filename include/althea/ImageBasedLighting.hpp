@@ -222,6 +222,65 @@ inline void convolveReflectionBuffer(ReflectionBuffer& reflection, hipStream_t s
 
 }  // namespace ScreenSpaceReflection
 
+// PointLightCollection (Include/Althea/PointLight.h:39-126): the point lights
+// in device memory (PointLight layout) and, when created with shadow maps,
+// their cubes (shadowSize^2 x 6 per light, 256 in the reference, stored with
+// the one-texel face border althea_shadow_cubes writes).  setLight changes
+// the host copy; updateResource uploads it.  Owns its device memory.
+class PointLightCollection {
+ public:
+  PointLightCollection() = default;
+  PointLightCollection(size_t lightCount, bool createShadowMap, int shadowSize = 256)
+      : _host(lightCount), _shadowSize(shadowSize) {
+    if (lightCount == 0) throw std::runtime_error("PointLightCollection: no lights");
+    void* lights = nullptr;
+    if (hipMalloc(&lights, lightCount * sizeof(althea_point_light)) != hipSuccess)
+      throw std::runtime_error("PointLightCollection: hipMalloc failed");
+    _lights.reset(lights);
+    if (createShadowMap) {
+      const size_t floats = althea_shadow_cube_floats(shadowSize) * lightCount;
+      if (floats == 0) throw std::runtime_error("PointLightCollection: invalid shadow map size");
+      void* cubes = nullptr;
+      if (hipMalloc(&cubes, floats * sizeof(float)) != hipSuccess)
+        throw std::runtime_error("PointLightCollection: hipMalloc failed");
+      _cubes.reset(cubes);
+    }
+  }
+
+  void setLight(uint32_t lightId, const althea_point_light& light) { _host.at(lightId) = light; }
+  const althea_point_light& getLight(uint32_t lightId) const { return _host.at(lightId); }
+  size_t getCount() const { return _host.size(); }
+
+  void updateResource(hipStream_t stream = nullptr) {
+    throwIfHip(hipMemcpyAsync(_lights.get(), _host.data(), _host.size() * sizeof(althea_point_light),
+                              hipMemcpyHostToDevice, stream),
+               "PointLightCollection::updateResource");
+  }
+
+  // The shadow pass (Src/PointLight.cpp:235-282) over a world-space mesh.
+  void drawShadowMaps(const althea_mesh_view& mesh, hipStream_t stream = nullptr) {
+    if (!_cubes) throw std::runtime_error("PointLightCollection: created without shadow maps");
+    throwIfFailed(althea_shadow_cubes(&mesh, lights(), static_cast<int>(getCount()), _shadowSize,
+                                      static_cast<float*>(_cubes.get()), stream),
+                  "PointLightCollection::drawShadowMaps");
+  }
+
+  const althea_point_light* lights() const { return static_cast<const althea_point_light*>(_lights.get()); }
+  const float* shadowMaps() const { return static_cast<const float*>(_cubes.get()); }
+  int getShadowMapExtent() const { return _shadowSize; }
+
+ private:
+  static void throwIfHip(hipError_t e, const char* where) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(where) + ": " + hipGetErrorString(e));
+  }
+  struct Free {
+    void operator()(void* p) const { (void)hipFree(p); }
+  };
+  std::vector<althea_point_light> _host;
+  std::unique_ptr<void, Free> _lights, _cubes;
+  int _shadowSize = 0;
+};
+
 namespace DeferredRendering {
 
 // The deferred lighting resolve of DeferredPass.frag (one frame).
@@ -242,6 +301,20 @@ inline void resolve(const GBufferResources& gbuffer, const IBLResources& ibl, co
   althea_ibl_views v = ibl.views();
   throwIfFailed(althea_deferred_resolve_reflect(&gbuffer.view(), &v, d_lights, lightCount, &globals, ssao,
                                                 &reflection.view(), flags, d_out, d_ao, stream),
+                "DeferredRendering::resolve");
+}
+
+// The resolve with the lights of `lights` and their shadow test
+// (PBRMaterial.glsl:119-140); `reflection` may be null.
+inline void resolve(const GBufferResources& gbuffer, const IBLResources& ibl, const PointLightCollection& lights,
+                    const althea_globals& globals, const althea_ssao_desc* ssao, const ReflectionBuffer* reflection,
+                    void* d_out, unsigned flags = 0, float* d_ao = nullptr, hipStream_t stream = nullptr) {
+  if (!lights.shadowMaps()) throw std::runtime_error("DeferredRendering::resolve: lights without shadow maps");
+  althea_ibl_views v = ibl.views();
+  throwIfFailed(althea_deferred_resolve_shadowed(&gbuffer.view(), &v, lights.lights(),
+                                                 static_cast<int>(lights.getCount()), &globals, ssao,
+                                                 reflection ? &reflection->view() : nullptr, lights.shadowMaps(),
+                                                 lights.getShadowMapExtent(), flags, d_out, d_ao, stream),
                 "DeferredRendering::resolve");
 }
 

@@ -116,6 +116,31 @@ int main() {
     }
     std::printf("reflections finite %d, extent mismatch threw %d\n", refl_finite ? 1 : 0, extent_threw ? 1 : 0);
     finite = finite && refl_finite && extent_threw;
+    // Point-light shadows: a square between the light and the surface,
+    // facing the light, hides it; the frame is then the frame without lights.
+    PointLightCollection lights(1, true, 64);
+    lights.setLight(0, light);
+    lights.updateResource();
+    const float quad[12] = {-2.0f, -2.0f, -1.0f, 2.0f, -2.0f, -1.0f, 2.0f, 2.0f, -1.0f, -2.0f, 2.0f, -1.0f};
+    const uint32_t tris[6] = {0, 1, 2, 0, 2, 3};  // counter-clockwise seen from the light at z = 1
+    float* d_quad;
+    uint32_t* d_tris;
+    HIP_OK(hipMalloc(&d_quad, sizeof(quad)));
+    HIP_OK(hipMalloc(&d_tris, sizeof(tris)));
+    HIP_OK(hipMemcpy(d_quad, quad, sizeof(quad), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_tris, tris, sizeof(tris), hipMemcpyHostToDevice));
+    lights.drawShadowMaps(althea_mesh_view{d_quad, d_tris, 4, 2});
+    DeferredRendering::resolve(gbuf, ibl, lights, g, nullptr, nullptr, d_out, ALTHEA_RESOLVE_TONEMAP);
+    std::vector<float> shadowed(out.size()), unlit(out.size());
+    HIP_OK(hipMemcpy(shadowed.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost));
+    DeferredRendering::resolve(gbuf, ibl, nullptr, 0, g, nullptr, d_out, ALTHEA_RESOLVE_TONEMAP);
+    HIP_OK(hipMemcpy(unlit.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost));
+    float shadow_diff = 0.0f;
+    for (size_t i = 0; i < out.size(); ++i) shadow_diff = std::fmax(shadow_diff, std::fabs(shadowed[i] - unlit[i]));
+    HIP_OK(hipFree(d_quad));
+    HIP_OK(hipFree(d_tris));
+    std::printf("shadowed frame vs unlit frame: max diff %.3g\n", shadow_diff);
+    finite = finite && shadow_diff <= 1e-6f;
     // Errors come back as exceptions, as the reference throws.
     bool threw = false;
     try {
