@@ -29,7 +29,8 @@ from . import synth
 from ._lib import AltheaError, check, lib
 
 __all__ = ["AltheaError", "IBLDesc", "IBLResources", "ImageBasedLighting", "GBufferResources",
-           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "PointLightCollection", "synth",
+           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "PointLightCollection", "GBufferMesh",
+           "SceneToGBufferPass", "synth",
            "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
            "brdf_lut", "cube_sample", "env_sample", "prefilter_points", "env_levels", "cube_texels"]
 
@@ -338,6 +339,56 @@ class GBufferResources:
         p = self.planes
         return _lib.GBufferViewC(_ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(p[3]), self.width, self.height,
                                  self.format)
+
+
+class GBufferMesh:
+    """The per-vertex inputs of the pass-through G-buffer shader
+    (Shaders/GBuffer/GBufferPassThrough.frag): float32 device tensors
+    positions, colors, mro (v, 3), normals (v, 3) or None (all zero), uint32
+    or int32 indices (t, 3)."""
+
+    def __init__(self, positions, indices, colors, mro, normals=None):
+        torch = _torch()
+        for t in (positions, colors, mro) + ((normals,) if normals is not None else ()):
+            assert t.dtype == torch.float32 and t.is_contiguous() and t.shape == positions.shape, t.shape
+        assert indices.dtype in (torch.int32, torch.uint32) and indices.is_contiguous()
+        self.positions, self.indices, self.colors, self.mro, self.normals = positions, indices, colors, mro, normals
+
+    @property
+    def triangle_count(self) -> int:
+        return self.indices.numel() // 3
+
+    def view(self) -> _lib.GBufferMeshC:
+        n = self.triangle_count
+        return _lib.GBufferMeshC(_ptr(self.positions), _ptr(self.normals), _ptr(self.colors), _ptr(self.mro),
+                                 _ptr(self.indices) if n else None, self.positions.numel() // 3, n)
+
+
+class SceneToGBufferPass:
+    """SceneToGBufferPass (Src/DeferredRendering.cpp:268-332) with the
+    pass-through subpass: `begin(mesh, globals)` rasterizes a GBufferMesh with
+    the camera of `globals` into this pass's GBufferResources (four planes of
+    width x height vec4, FP32 or FP16)."""
+
+    def __init__(self, width: int, height: int, fp16: bool = False):
+        torch = _torch()
+        dt = torch.float16 if fp16 else torch.float32
+        self.gbuffer = GBufferResources(*[torch.empty((height, width, 4), dtype=dt, device="cuda")
+                                          for _ in range(4)])
+        self.workspace = None
+
+    def begin(self, mesh: GBufferMesh, globals_, stream=None) -> GBufferResources:
+        torch = _torch()
+        g = self.gbuffer
+        need = lib().althea_gbuffer_workspace_bytes(g.width, g.height, mesh.triangle_count)
+        if self.workspace is None or self.workspace.numel() < need:
+            self.workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+        mv = mesh.view()
+        gv = g.getHandles()
+        check(lib().althea_gbuffer_draw(C.byref(mv), C.byref(globals_), C.byref(gv), _ptr(self.workspace),
+                                        _stream(stream)),
+              "althea_gbuffer_draw")
+        return g
 
 
 class ReflectionBuffer:

@@ -21,7 +21,8 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 ORACLE = ROOT / "oracle"
 
-HIP_SOURCES = ["ibl_kernels.hip", "resolve_kernels.hip", "ssr_kernels.hip", "shadow_kernels.hip", "mlp_kernels.hip",
+HIP_SOURCES = ["ibl_kernels.hip", "resolve_kernels.hip", "ssr_kernels.hip", "shadow_kernels.hip", "gbuffer_kernels.hip",
+               "mlp_kernels.hip",
                "batched_kernels.hip", "althea_capi.cpp", "hdr_io.cpp"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",

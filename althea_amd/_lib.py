@@ -71,6 +71,11 @@ class ReflectionViewC(C.Structure):
     _fields_ = [("data", vp), ("width", i32), ("height", i32), ("mips", i32), ("format", i32)]
 
 
+class GBufferMeshC(C.Structure):
+    _fields_ = [("positions", vp), ("normals", vp), ("colors", vp), ("mro", vp), ("indices", vp),
+                ("vertex_count", i32), ("triangle_count", i32)]
+
+
 class MeshViewC(C.Structure):
     _fields_ = [("positions", vp), ("indices", vp), ("vertex_count", i32), ("triangle_count", i32)]
 
@@ -125,6 +130,8 @@ PROTOTYPES = {
     "althea_deferred_resolve_reflect": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                               C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),
                                               C.c_uint, vp, vp, vp]),
+    "althea_gbuffer_workspace_bytes": (sz, [i32, i32, i32]),
+    "althea_gbuffer_draw": (i32, [C.POINTER(GBufferMeshC), C.POINTER(GlobalsC), C.POINTER(GBufferViewC), vp, vp]),
     "althea_shadow_cube_floats": (sz, [i32]),
     "althea_shadow_cubes": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp]),
     "althea_deferred_resolve_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,

@@ -546,6 +546,28 @@ althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, in
   return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
 }
 
+size_t althea_gbuffer_workspace_bytes(int width, int height, int triangle_count) {
+  return gbuffer_workspace_bytes(width, height, triangle_count);
+}
+
+althea_status althea_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* globals,
+                                  const althea_gbuffer_view* g, void* d_workspace, hipStream_t stream) {
+  ALTHEA_REQUIRE(mesh && globals && g && d_workspace, "gbuffer_draw: null argument");
+  ALTHEA_REQUIRE(g->width >= 1 && g->height >= 1 && static_cast<long long>(g->width) * g->height < (1ll << 31),
+                 "gbuffer_draw: extent %dx%d", g->width, g->height);
+  ALTHEA_REQUIRE(g->position && g->normal && g->albedo && g->mro, "gbuffer_draw: null G-buffer plane");
+  ALTHEA_REQUIRE(g->format == ALTHEA_GBUFFER_FP32 || g->format == ALTHEA_GBUFFER_FP16, "gbuffer_draw: format %d",
+                 g->format);
+  ALTHEA_REQUIRE(mesh->triangle_count >= 0 && mesh->triangle_count < (1 << 28) &&
+                     (mesh->triangle_count == 0 ||
+                      (mesh->positions && mesh->colors && mesh->mro && mesh->indices &&
+                                                mesh->vertex_count >= 1)),
+                 "gbuffer_draw: mesh with %d triangles, %d vertices", mesh->triangle_count, mesh->vertex_count);
+  ALTHEA_REQUIRE(globals->projection[10] != 0.0f && globals->projection[10] != -1.0f && globals->projection[14] != 0.0f,
+                 "gbuffer_draw: projection is not a [0, 1]-depth perspective");
+  return launch_gbuffer_draw(mesh, globals, g, d_workspace, stream);
+}
+
 size_t althea_shadow_cube_floats(int size) {
   return size < 1 ? 0 : static_cast<size_t>(6) * (size + 2) * (size + 2);
 }

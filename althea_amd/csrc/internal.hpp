@@ -96,6 +96,9 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
                              const float* shadow_cubes = nullptr, int shadow_size = 0);
 
 // Point-light shadow cubes (shadow_kernels.hip).
+size_t gbuffer_workspace_bytes(int width, int height, int triangle_count);
+althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* g,
+                                  const althea_gbuffer_view* out, void* workspace, hipStream_t s);
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
                                   int size, float* cubes, hipStream_t s);
 

@@ -101,10 +101,14 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
       const float* q = p.positions + 3 * static_cast<size_t>(min(vi, static_cast<uint32_t>(p.vertex_count - 1)));
       v[k] = mk(q[0] - lp.x, q[1] - lp.y, q[2] - lp.z);
     }
-    tri.n0 = cross(v[1], v[2]);
-    tri.n1 = cross(v[2], v[0]);
-    tri.n2 = cross(v[0], v[1]);
-    tri.n = cross(v[1] - v[0], v[2] - v[0]);
+    // Edge functions from the edges e1 = b - a, e2 = c - a (c x a = e2 x a,
+    // a x b = a x e1, b x c = n - both): rounding scales with |a| |e|, not
+    // |a| |b|.
+    const f3 e1 = v[1] - v[0], e2 = v[2] - v[0];
+    tri.n = cross(e1, e2);
+    tri.n1 = cross(e2, v[0]);
+    tri.n2 = cross(v[0], e1);
+    tri.n0 = tri.n - tri.n1 - tri.n2;
     tri.det = dot(v[0], tri.n);
     live = tri.det < 0.0f;  // back faces and edge-on triangles are culled
   }

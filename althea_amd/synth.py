@@ -135,6 +135,24 @@ def scene(seed: int, width: int, height: int, lights: int, ssao_samples: int = 1
     return Scene(width, height, *planes, lt, g, k, n)
 
 
+def mesh_attributes(seed: int, positions: np.ndarray, indices: np.ndarray):
+    """Per-vertex inputs of the pass-through G-buffer shader for a mesh:
+    normals (the normalised sum of the adjacent faces' (b - a) x (c - a)),
+    seeded colours in [0.05, 0.95] and metallic / roughness / occlusion in
+    [0, 1] x [0.05, 1] x [0.5, 1], float32 (v, 3) each."""
+    p = positions.astype(np.float64)
+    t = indices.astype(np.int64)
+    fn = np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]])
+    vn = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(vn, t[:, k], fn)
+    vn /= np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-30)
+    rng = np.random.default_rng(seed)
+    col = rng.uniform(0.05, 0.95, p.shape)
+    mro = np.stack([rng.uniform(0.0, 1.0, len(p)), rng.uniform(0.05, 1.0, len(p)), rng.uniform(0.5, 1.0, len(p))], 1)
+    return vn.astype(np.float32), col.astype(np.float32), mro.astype(np.float32)
+
+
 def heightfield_mesh(seed: int, n: int = 128, half: float = 10.0, amplitude: float = 1.5):
     """The scene's heightfield as a triangle mesh: (positions (V, 3) float32,
     indices (T, 3) uint32), an n x n grid over [-half, half]^2, two triangles

@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
     p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
     p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
-    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr", "shadow"],
+    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr", "shadow", "gbuffer"],
                    help="profiling runs: time one kernel family and print its line only "
                         "(copy = a plain 1 GiB device copy, the calibration for HBM byte counters)")
     return p.parse_args()
@@ -293,6 +293,56 @@ def shadow_only(torch, aa, args) -> int:
     return 0
 
 
+def gbuffer_frame(torch, aa, scene, ibl, steps: int, fp16: bool = False) -> dict:
+    """The G-buffer producer on the same camera: the scene's heightfield as a
+    128 x 128-cell mesh (32 768 triangles) with per-vertex normals, colours
+    and MRO, rasterized into the 4K G-buffer, then the resolve of that
+    G-buffer (64 lights, SSAO 16).  Compulsory bytes of the producer: the four
+    planes written once (64 B/px FP32), the mesh read once."""
+    pos, idx = aa.synth.heightfield_mesh(SEED0 + 1, n=128)
+    nrm, col, mro = aa.synth.mesh_attributes(SEED0 + 1, pos, idx)
+    f = lambda a: torch.from_numpy(a).cuda()
+    mesh = aa.GBufferMesh(f(pos), f(idx.view(np.int32)), f(col), f(mro), f(nrm))
+    pas = aa.SceneToGBufferPass(WIDTH, HEIGHT, fp16=fp16)
+    lights = scene.lights_tensor("cuda")
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    glob, ssao = scene.globals, scene.ssao
+
+    def draw():
+        pas.begin(mesh, glob)
+
+    def resolve():
+        aa.DeferredRendering.resolve(pas.gbuffer, ibl, lights, glob, ssao, out=out)
+
+    def frame():
+        draw()
+        resolve()
+
+    frame()
+    draw_ms = time_events(torch, draw, steps)
+    res_ms = time_events(torch, resolve, steps)
+    frame_ms = time_events(torch, frame, steps)
+    covered = float((pas.gbuffer.planes[0][..., 3] == 1.0).float().mean().item())
+    px = WIDTH * HEIGHT
+    nbytes = px * (32 if fp16 else 64) + pos.nbytes * 4 + idx.nbytes
+    return {"triangles": int(idx.shape[0]), "draw_ms": draw_ms, "draw_mpix_per_s": px / (draw_ms * 1e-3) / 1e6,
+            "draw_algorithmic_bytes": nbytes, "draw_achieved_gbs": nbytes / (draw_ms * 1e-3) / 1e9,
+            "covered": covered, "resolve_ms": res_ms, "frame_ms": frame_ms,
+            "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
+
+
+def gbuffer_only(torch, aa, args) -> int:
+    """Profiling run of the G-buffer producer and its resolve on the 4K frame."""
+    env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
+                      brdf_lut_size=128, brdf_lut_samples=1024)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
+    r = gbuffer_frame(torch, aa, scene, ibl, args.steps, fp16=args.fp16_gbuffer)
+    print(json.dumps({"kind": "gbuffer", **r}), flush=True)
+    return 0
+
+
 def fp16_resolve(torch, aa, scene, ibl, lights, steps: int) -> dict:
     """The same frame shaded from an FP16 G-buffer (32 B/px)."""
     g16 = aa.GBufferResources(*scene.gbuffer_tensors("cuda", fp16=True))
@@ -376,6 +426,8 @@ def main() -> int:
         return ssr_only(torch, aa, args)
     if args.only == "shadow":
         return shadow_only(torch, aa, args)
+    if args.only == "gbuffer":
+        return gbuffer_only(torch, aa, args)
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
     desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
                       brdf_lut_size=128, brdf_lut_samples=1024)
@@ -423,6 +475,7 @@ def main() -> int:
         line["secondary"]["resolve_fp16_gbuffer"] = fp16_resolve(torch, aa, scene, ibl, lights, args.steps)
         line["secondary"]["ssr"] = ssr_frame(torch, aa, scene, ibl, gbuf, lights, args.steps)
         line["secondary"]["shadows"] = shadow_frame(torch, aa, scene, ibl, gbuf, args.steps)
+        line["secondary"]["gbuffer"] = gbuffer_frame(torch, aa, scene, ibl, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)

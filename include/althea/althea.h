@@ -390,6 +390,44 @@ ALTHEA_API althea_status althea_deferred_resolve_reflect(const althea_gbuffer_vi
                                                          void* d_out, float* d_ao_out, hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* G-buffer producer                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* A triangle mesh with the per-vertex inputs of the pass-through G-buffer
+ * shader (Shaders/GBuffer/GBufferPassThrough.frag: inPosition, inNormal,
+ * inColor, inMetallicRoughnessOcclusion), float3 each, world space; normals
+ * may be NULL (all zero). */
+typedef struct althea_gbuffer_mesh {
+  const float* positions;
+  const float* normals;
+  const float* colors;
+  const float* mro;
+  const uint32_t* indices; /* 3 per triangle */
+  int vertex_count, triangle_count;
+} althea_gbuffer_mesh;
+
+/* Device workspace althea_gbuffer_draw needs: the per-pixel visibility keys
+ * (8 B/px) and per-triangle setup (76 B/triangle), 256-byte aligned parts. */
+ALTHEA_API size_t althea_gbuffer_workspace_bytes(int width, int height, int triangle_count);
+
+/* SceneToGBufferPass (Src/DeferredRendering.cpp:268-332) with the
+ * pass-through subpass (GBufferPassThrough.frag, as Src/Scene/Floor.cpp
+ * draws it): rasterizes `mesh` with the camera of `globals` (projection *
+ * view; the [0, 1]-depth perspective's near and far planes clip) into the
+ * planes of `gbuffer` (written, FP32 or FP16).  Per pixel: the nearest
+ * front-facing triangle (counter-clockwise on screen, back faces culled;
+ * equal depths keep the earlier triangle, as a LESS depth test does) whose
+ * interior holds the pixel centre, attributes interpolated perspective-
+ * correctly there.  Planes: position (pos, 1), normal (the interpolated
+ * normal, or where it is exactly zero normalize(cross(dFdx(pos), dFdy(pos)))
+ * with the framebuffer's y growing downwards, 1), albedo (color, 1), MRO
+ * (mro, 1); uncovered pixels (0, 0, 0, 1) and position (0, 0, 0, 0).
+ * d_workspace: althea_gbuffer_workspace_bytes bytes. */
+ALTHEA_API althea_status althea_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* globals,
+                                             const althea_gbuffer_view* gbuffer, void* d_workspace,
+                                             hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Point-light shadow cubes                                                  */
 /* ------------------------------------------------------------------------ */
 

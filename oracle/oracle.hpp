@@ -29,6 +29,10 @@
 //                       same-size impulse -> trimmed binomial row 16
 //   reflection blend    alpha 0 -> plain resolve, alpha 1 / 0.5 -> the
 //                       prefiltered term replaced / halved
+//   G-buffer producer   an analytic floor -> the ray / plane point and the
+//                       constant attributes; a brute-force ray / triangle
+//                       test per pixel with a different formulation;
+//                       zero normals -> the face normal away from the eye
 //   shadow cubes        empty mesh -> 1; a lit square -> the analytic
 //                       distance; reversed winding, too near -> not written
 //   shadow test         occluded light skipped; the +-Y face mirror (Q19)
@@ -1104,6 +1108,124 @@ inline dvec3 resolve_pixel(const ResolveInputs& in, int px, int py, double* ao_o
                          in.light_count, in.shadows, margin);
   if (in.tonemap) c = tonemap(c, in.exposure);
   return c;
+}
+
+// ---------------------------------------------------------------------------
+// f1. G-buffer producer: SceneToGBufferPass (Src/DeferredRendering.cpp:268-332)
+// with GBufferPassThrough.frag, per pixel.  The pixel's value is the nearest
+// front-facing triangle (normal (b - a) x (c - a) towards the eye; back faces
+// culled, GraphicsPipeline.h:171-172) whose interior holds the ray through the
+// pixel centre, with view depth in [near, far] of the [0, 1]-depth projection;
+// on equal ray parameters the earlier triangle (LESS depth test).  Attributes
+// are the perspective-correct (world-space barycentric) interpolation at the
+// hit.  A normal that interpolates to exactly zero is replaced by
+// normalize(cross(dFdx(pos), dFdy(pos))): the hits of the rays through the next
+// pixel to the right and the next one down (framebuffer y grows downwards) on
+// the triangle's plane, minus this pixel's hit.  out[16]: position (p, 1),
+// normal (n, 1), albedo (color, 1), mro (mro, 1); an uncovered pixel is
+// (0, 0, 0, 0) and (0, 0, 0, 1) three times.
+// margin: the smallest of, over front-facing triangles whose plane the ray
+// meets within [near, far] no farther than 1 + 1e-5 times the winner's ray
+// parameter: the angular distance (sine) of the ray from each edge plane of
+// the winner, or from the nearest edge plane of a triangle it misses;
+// relative depth gaps to other covering triangles; relative distances of the
+// depth from near and far.
+// ---------------------------------------------------------------------------
+struct GbMesh {
+  const float *pos, *nrm, *col, *mro;
+  const uint32_t* idx;
+  int tris;
+};
+inline void gbuffer_pixel(const GbMesh& mesh, const ResolveInputs& cam, int px, int py, double out[16],
+                          double* margin, int* tri_out) {
+  const dvec3 d = view_ray(cam, px, py);
+  const double* iv = cam.inv_view.m;
+  const dvec3 eye{iv[12], iv[13], iv[14]};
+  const double* vm = cam.view.m;
+  const dvec3 vz{vm[2], vm[6], vm[10]};
+  const double s = -dot(vz, d);  // view depth per unit of ray parameter
+  const double near = cam.proj.m[14] / cam.proj.m[10], far = cam.proj.m[14] / (cam.proj.m[10] + 1.0);
+  auto vtx = [&](const float* a, uint32_t i) { return dvec3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; };
+  struct Cand {
+    int k;
+    double t, e[3], q[3];
+    bool inside;
+  };
+  auto test = [&](int k, Cand& c) {
+    const dvec3 a = vtx(mesh.pos, mesh.idx[3 * k]) - eye, b = vtx(mesh.pos, mesh.idx[3 * k + 1]) - eye,
+                cc = vtx(mesh.pos, mesh.idx[3 * k + 2]) - eye;
+    const dvec3 n = cross(b - a, cc - a);
+    const double det = dot(a, n);
+    if (!(det < 0.0)) return false;
+    const dvec3 en[3] = {cross(b, cc), cross(cc, a), cross(a, b)};
+    const double rn = dot(d, n);
+    if (!(rn < 0.0)) return false;
+    c.k = k;
+    c.t = det / rn;
+    const double depth = c.t * s;
+    if (depth < near || depth > far) return false;
+    c.inside = true;
+    for (int j = 0; j < 3; ++j) {
+      c.e[j] = dot(d, en[j]);
+      c.q[j] = c.e[j] / (length(en[j]) * length(d));
+      if (c.e[j] > 0.0) c.inside = false;
+    }
+    return true;
+  };
+  int best = -1;
+  double bt = 1e300;
+  Cand win{};
+  for (int k = 0; k < mesh.tris; ++k) {
+    Cand c;
+    if (!test(k, c) || !c.inside) continue;
+    if (c.t < bt) {
+      bt = c.t;
+      best = k;
+      win = c;
+    }
+  }
+  if (margin) {
+    double m = 1e30;
+    for (int k = 0; k < mesh.tris; ++k) {
+      Cand c;
+      if (!test(k, c)) continue;
+      if (best >= 0 && c.t > bt * (1.0 + 1e-5)) continue;
+      const double depth = c.t * s;
+      m = std::min({m, std::fabs(depth - near) / near, std::fabs(depth - far) / far});
+      if (c.inside) {
+        for (int j = 0; j < 3; ++j) m = std::min(m, -c.q[j]);
+        if (k != best) m = std::min(m, std::fabs(c.t - bt) / bt);
+      } else {
+        m = std::min(m, std::max({c.q[0], c.q[1], c.q[2]}));
+      }
+    }
+    *margin = m;
+  }
+  if (tri_out) *tri_out = best;
+  for (int j = 0; j < 16; ++j) out[j] = (j % 4 == 3 && j > 3) ? 1.0 : 0.0;
+  if (best < 0) return;
+  const double se = win.e[0] + win.e[1] + win.e[2];
+  const double l[3] = {win.e[0] / se, win.e[1] / se, win.e[2] / se};
+  const uint32_t* vi = mesh.idx + 3 * best;
+  auto interp = [&](const float* a) { return vtx(a, vi[0]) * l[0] + vtx(a, vi[1]) * l[1] + vtx(a, vi[2]) * l[2]; };
+  const dvec3 pos = interp(mesh.pos);
+  dvec3 nrm = mesh.nrm ? interp(mesh.nrm) : dvec3{0.0, 0.0, 0.0};
+  if (nrm.x == 0.0 && nrm.y == 0.0 && nrm.z == 0.0) {
+    const dvec3 a = vtx(mesh.pos, vi[0]) - eye, b = vtx(mesh.pos, vi[1]) - eye, c = vtx(mesh.pos, vi[2]) - eye;
+    const dvec3 n = cross(b - a, c - a);
+    const double det = dot(a, n);
+    auto hit = [&](dvec3 r) { return r * (det / dot(r, n)); };
+    const dvec3 h0 = hit(d), hx = hit(view_ray(cam, px + 1, py)), hy = hit(view_ray(cam, px, py + 1));
+    nrm = normalize(cross(hx - h0, hy - h0));
+  }
+  const dvec3 col = interp(mesh.col), m = interp(mesh.mro);
+  const dvec3 planes[4] = {pos, nrm, col, m};
+  for (int j = 0; j < 4; ++j) {
+    out[4 * j] = planes[j].x;
+    out[4 * j + 1] = planes[j].y;
+    out[4 * j + 2] = planes[j].z;
+    out[4 * j + 3] = 1.0;
+  }
 }
 
 }  // namespace oracle

@@ -335,6 +335,26 @@ void oracle_reflection_sample(const float* data, int W, int H, int mips, const i
   for (int i = 0; i < count; ++i) reflection_sample(c, pixels[2 * i], pixels[2 * i + 1], lods[i], out4 + 4 * i);
 }
 
+// G-buffer producer at pixels (x, y): out n x 16 doubles, margin, triangle
+// (-1 uncovered).  camera: projection, inverse projection, view, inverse
+// view (column-major float[16] each), then width and height.
+void oracle_gbuffer(const float* const* cam, int width, int height, const float* pos, const float* nrm,
+                    const float* col, const float* mro, const uint32_t* idx, int tris, const int* pixels, int count,
+                    double* out, double* margin, int* tri) {
+  ResolveInputs in{};
+  in.W = width;
+  in.H = height;
+  in.proj = dmat4::from_float(cam[0]);
+  in.inv_proj = dmat4::from_float(cam[1]);
+  in.view = dmat4::from_float(cam[2]);
+  in.inv_view = dmat4::from_float(cam[3]);
+  const GbMesh mesh{pos, nrm, col, mro, idx, tris};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < count; ++i)
+    gbuffer_pixel(mesh, in, pixels[2 * i], pixels[2 * i + 1], out + 16 * static_cast<size_t>(i),
+                  margin ? margin + i : nullptr, tri ? tri + i : nullptr);
+}
+
 // Shadow cubes of `light_count` lights ([n][8] floats) from a mesh:
 // out [light][face][y][x] doubles.
 void oracle_shadow_cubes(const float* positions, const uint32_t* indices, int triangles, const float* lights,

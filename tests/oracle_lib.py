@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
         L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
+        L.oracle_gbuffer.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, vp]
         L.oracle_shadow_cubes.argtypes = [vp, vp, i32, vp, i32, i32, vp]
         L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
         L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
@@ -292,6 +293,28 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
     lib().oracle_resolve_ex(C.byref(a), _p(px), px.shape[0], _p(r_in) if r_in is not None else None, _p(rgb),
                             _p(ao), _p(margin), _p(q) if q is not None else None)
     return (rgb, ao, margin, q) if query else (rgb, ao, margin)
+
+
+def gbuffer(scene, positions, indices, colors, mro, normals=None, pixels=None):
+    """Oracle G-buffer producer with the camera of a synth.Scene at
+    pixels[:, (x, y)] (all pixels by default): (planes (n, 4, 4) position /
+    normal / albedo / mro, margin (n,), triangle (n,), -1 uncovered)."""
+    g = _c(scene.globals_arr, np.float32)
+    mats = [_c(g[16 * k:16 * k + 16], np.float32) for k in (0, 1, 2, 4)]
+    cam = (vp * 4)(*[_p(m) for m in mats])
+    if pixels is None:
+        y, x = np.mgrid[0:scene.height, 0:scene.width]
+        pixels = np.stack([x.ravel(), y.ravel()], 1)
+    px = _c(pixels, np.int32)
+    pos, col, m = _c(positions, np.float32), _c(colors, np.float32), _c(mro, np.float32)
+    nrm = _c(normals, np.float32) if normals is not None else None
+    idx = _c(indices, np.uint32)
+    out = np.empty((px.shape[0], 16), np.float64)
+    margin = np.empty(px.shape[0], np.float64)
+    tri = np.empty(px.shape[0], np.int32)
+    lib().oracle_gbuffer(C.cast(cam, vp), scene.width, scene.height, _p(pos), _p(nrm) if nrm is not None else None, _p(col),
+                         _p(m), _p(idx), idx.size // 3, _p(px), px.shape[0], _p(out), _p(margin), _p(tri))
+    return out.reshape(-1, 4, 4), margin, tri
 
 
 def shadow_cubes(positions, indices, lights, size: int) -> np.ndarray:
