@@ -16,12 +16,16 @@
 // Passes (one stream):
 //   gbuffer_setup_kernel   one lane per triangle: edge normals, plane, the
 //                          pixel box of its part in front of half the near
-//                          plane, the box area;
-//   gbuffer_scan_kernel    one block: exclusive prefix sum of the box areas
-//                          (64-bit pixel slots);
-//   gbuffer_raster_kernel  a fixed grid strides over the slots: each lane
-//                          finds its triangle by binary search over the
-//                          prefix sums, casts its pixel's ray and keeps the
+//                          plane, the box area and its prefix sum within the
+//                          block (64-bit pixel slots);
+//   gbuffer_scan_kernel    one block: prefix sum of the block totals;
+//   gbuffer_offset_kernel  block offset added to every triangle's slot;
+//   gbuffer_raster_kernel  a fixed grid of waves, each taking an equal
+//                          contiguous run of slots, 64 at a time: the wave
+//                          follows the triangles of its first and last slot
+//                          forward with wave-uniform galloping searches, each
+//                          lane finds its own within that range, casts its
+//                          pixel's ray and keeps the
 //                          nearest hit as (depth bits << 32 | triangle) with a
 //                          64-bit atomic min, so big and small triangles cost
 //                          the same per pixel and the result does not depend
@@ -41,6 +45,7 @@ namespace althea {
 
 constexpr unsigned long long kGbEmpty = ~0ull;
 constexpr int kGbScanThreads = 1024;
+constexpr int kGbSetupThreads = 256;
 constexpr int kGbRasterBlocks = 4096;
 
 // Per-triangle setup, 64 bytes.
@@ -68,7 +73,7 @@ struct GbParams {
   float near, far;
   GbTri* tris;
   unsigned long long* offsets;  // triangle_count + 1 slot offsets
-  uint32_t* areas;
+  unsigned long long* blocks;   // per setup block: its slot total, then its first slot
   unsigned long long* vis;      // width * height keys
   void* position_out;
   void* normal_out;
@@ -87,14 +92,11 @@ __device__ __forceinline__ f3 gb_ray(const GbParams& p, int x, int y) {
             fmaf(p.rayy[2], cy, fmaf(p.rayx[2], cx, p.ray0[2])));
 }
 
-__global__ __launch_bounds__(256) void gbuffer_setup_kernel(const GbParams p) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.triangle_count) return;
+__device__ void gbuffer_setup_triangle(const GbParams& p, int t, GbTri& tri, unsigned long long& area) {
   const uint32_t last = static_cast<uint32_t>(p.vertex_count - 1);
   const f3 eye = mk(p.eye[0], p.eye[1], p.eye[2]);
   f3 v[3];
   for (int k = 0; k < 3; ++k) v[k] = gb_vertex(p.positions, min(p.indices[3 * t + k], last)) - eye;
-  GbTri tri;
   // Edge functions b x c, c x a, a x b formed from the edges e1 = b - a,
   // e2 = c - a (c x a = e2 x a, a x b = a x e1, b x c = n - both), so their
   // rounding scales with |a| |e| rather than |a| |b|: far, small triangles
@@ -155,20 +157,45 @@ __global__ __launch_bounds__(256) void gbuffer_setup_kernel(const GbParams p) {
   tri.x0 = x0;
   tri.y0 = y0;
   tri.w = bw > 0 ? bw : 1;
-  p.tris[t] = tri;
-  p.areas[t] = (bw > 0 && bh > 0) ? static_cast<uint32_t>(bw) * static_cast<uint32_t>(bh) : 0u;
+  area = (bw > 0 && bh > 0) ? static_cast<unsigned long long>(bw) * static_cast<unsigned long long>(bh) : 0ull;
 }
 
-// Exclusive prefix sum of the box areas in one block: each thread sums a
-// contiguous run, the block scans the run totals, each thread writes its run.
-__global__ __launch_bounds__(kGbScanThreads) void gbuffer_scan_kernel(const uint32_t* __restrict__ areas, int n,
-                                                                      unsigned long long* __restrict__ offsets) {
+// Setup of one triangle per lane, and the exclusive prefix sum of the box
+// areas within the block (wave shuffles, then the waves' totals).
+__global__ __launch_bounds__(kGbSetupThreads) void gbuffer_setup_kernel(const GbParams p) {
+  __shared__ unsigned long long wave_total[kGbSetupThreads / 64];
+  const int t = blockIdx.x * kGbSetupThreads + threadIdx.x;
+  unsigned long long area = 0;
+  if (t < p.triangle_count) {
+    GbTri tri;
+    gbuffer_setup_triangle(p, t, tri, area);
+    p.tris[t] = tri;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long x = area;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wave_total[wave] = x;
+  __syncthreads();
+  unsigned long long before = 0;
+  for (int k = 0; k < wave; ++k) before += wave_total[k];
+  if (t < p.triangle_count) p.offsets[t] = before + x - area;
+  if (threadIdx.x == kGbSetupThreads - 1) p.blocks[blockIdx.x] = before + x;
+}
+
+// Exclusive prefix sum of the n block totals in place, in one block (each
+// thread a contiguous run), and the total slot count into *total.
+__global__ __launch_bounds__(kGbScanThreads) void gbuffer_scan_kernel(unsigned long long* __restrict__ blocks, int n,
+                                                                      unsigned long long* __restrict__ total) {
   __shared__ unsigned long long sums[kGbScanThreads];
   const int tid = threadIdx.x;
   const int run = (n + kGbScanThreads - 1) / kGbScanThreads;
   const int b = min(tid * run, n), e = min(b + run, n);
   unsigned long long s = 0;
-  for (int i = b; i < e; ++i) s += areas[i];
+  for (int i = b; i < e; ++i) s += blocks[i];
   sums[tid] = s;
   __syncthreads();
   for (int d = 1; d < kGbScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
@@ -179,10 +206,17 @@ __global__ __launch_bounds__(kGbScanThreads) void gbuffer_scan_kernel(const uint
   }
   unsigned long long acc = sums[tid] - s;
   for (int i = b; i < e; ++i) {
-    offsets[i] = acc;
-    acc += areas[i];
+    const unsigned long long v = blocks[i];
+    blocks[i] = acc;
+    acc += v;
   }
-  if (tid == kGbScanThreads - 1) offsets[n] = sums[tid];
+  if (tid == kGbScanThreads - 1) *total = sums[tid];
+}
+
+__global__ __launch_bounds__(256) void gbuffer_offset_kernel(unsigned long long* __restrict__ offsets,
+                                                             const unsigned long long* __restrict__ blocks, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) offsets[t] += blocks[t / kGbSetupThreads];
 }
 
 __global__ void gbuffer_clear_kernel(unsigned long long* __restrict__ vis, int n) {
@@ -206,22 +240,60 @@ __device__ __forceinline__ float gb_hit(const GbParams& p, const GbTri& tri, f3 
   return t;
 }
 
+// Largest t in [lo, hi) with offsets[t] <= slot, given offsets[lo] <= slot <
+// offsets[hi].
+__device__ __forceinline__ int gb_find(const unsigned long long* __restrict__ offsets, int lo, int hi,
+                                       unsigned long long slot) {
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offsets[mid] <= slot)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ unsigned long long gb_uniform(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Largest t >= lo with offsets[t] <= slot (offsets[lo] <= slot < offsets[T]):
+// probes lo + 1, lo + 3, lo + 7, ... then a binary search in the last step.
+__device__ __forceinline__ int gb_gallop(const unsigned long long* __restrict__ offsets, int lo, int T,
+                                         unsigned long long slot) {
+  int step = 1;
+  while (lo + step < T && offsets[lo + step] <= slot) {
+    lo += step;
+    step <<= 1;
+  }
+  return gb_find(offsets, lo, min(lo + step, T), slot);
+}
+
 __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
-  const unsigned long long total = p.offsets[p.triangle_count];
-  const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
-  for (unsigned long long slot = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-       slot < total; slot += stride) {
-    // Largest t with offsets[t] <= slot (offsets[0] = 0 <= slot < offsets[T]).
-    int lo = 0, hi = p.triangle_count;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (p.offsets[mid] <= slot)
-        lo = mid;
-      else
-        hi = mid;
-    }
+  const unsigned long long* __restrict__ offsets = p.offsets;
+  const int T = p.triangle_count;
+  const unsigned long long total = offsets[T];
+  const int lane = threadIdx.x & 63;
+  const unsigned waves = gridDim.x * (blockDim.x >> 6);
+  const unsigned wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  // Each wave takes an equal contiguous run of slots, 64 per iteration.
+  const unsigned long long per = ((total + waves - 1) / waves + 63) & ~63ull;
+  const unsigned long long begin = gb_uniform(wave * per);
+  const unsigned long long end = begin + per < total ? begin + per : total;
+  if (begin >= end) return;
+  int tf = gb_find(offsets, 0, T, begin);
+  for (unsigned long long base = begin; base < end; base += 64) {
+    const unsigned long long last = base + 63 < end ? base + 63 : end - 1;
+    tf = gb_gallop(offsets, tf, T, base);
+    const int tl = gb_gallop(offsets, tf, T, last);
+    const unsigned long long slot = base + lane;
+    if (slot > last) continue;
+    const int lo = gb_find(offsets, tf, tl + 1, slot);
     const GbTri tri = p.tris[lo];
-    const unsigned local = static_cast<unsigned>(slot - p.offsets[lo]);
+    const unsigned local = static_cast<unsigned>(slot - offsets[lo]);
     const int x = tri.x0 + static_cast<int>(local % static_cast<unsigned>(tri.w));
     const int y = tri.y0 + static_cast<int>(local / static_cast<unsigned>(tri.w));
     float e[3];
@@ -233,16 +305,22 @@ __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
   }
 }
 
+// Planes are written once and read by a later pass: streaming (non-temporal)
+// vector stores.
+typedef float gb_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned gb_u2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ void gb_store(void* plane, int fp16, size_t i, float x, float y, float z, float w) {
   if (fp16) {
-    ushort4 h;
-    h.x = __half_as_ushort(__float2half_rn(x));
-    h.y = __half_as_ushort(__float2half_rn(y));
-    h.z = __half_as_ushort(__float2half_rn(z));
-    h.w = __half_as_ushort(__float2half_rn(w));
-    static_cast<ushort4*>(plane)[i] = h;
+    const unsigned lo = __half_as_ushort(__float2half_rn(x)) |
+                        (static_cast<unsigned>(__half_as_ushort(__float2half_rn(y))) << 16);
+    const unsigned hi = __half_as_ushort(__float2half_rn(z)) |
+                        (static_cast<unsigned>(__half_as_ushort(__float2half_rn(w))) << 16);
+    const gb_u2 v = {lo, hi};
+    __builtin_nontemporal_store(v, static_cast<gb_u2*>(plane) + i);
   } else {
-    static_cast<float4*>(plane)[i] = make_float4(x, y, z, w);
+    const gb_f4 v = {x, y, z, w};
+    __builtin_nontemporal_store(v, static_cast<gb_f4*>(plane) + i);
   }
 }
 
@@ -305,11 +383,13 @@ namespace {
 size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 }  // namespace
 
+size_t gb_setup_blocks(int triangle_count) { return (static_cast<size_t>(triangle_count) + kGbSetupThreads - 1) / kGbSetupThreads; }
+
 size_t gbuffer_workspace_bytes(int width, int height, int triangle_count) {
   if (width < 1 || height < 1 || triangle_count < 0) return 0;
   const size_t t = static_cast<size_t>(triangle_count);
   return align256(static_cast<size_t>(width) * height * 8) + align256(t * sizeof(GbTri)) + align256((t + 1) * 8) +
-         align256(t * 4);
+         align256(gb_setup_blocks(triangle_count) * 8);
 }
 
 void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]);
@@ -348,7 +428,7 @@ althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_
   ws += align256(t * sizeof(GbTri));
   p.offsets = reinterpret_cast<unsigned long long*>(ws);
   ws += align256((t + 1) * 8);
-  p.areas = reinterpret_cast<uint32_t*>(ws);
+  p.blocks = reinterpret_cast<unsigned long long*>(ws);
   p.position_out = const_cast<void*>(out->position);
   p.normal_out = const_cast<void*>(out->normal);
   p.albedo_out = const_cast<void*>(out->albedo);
@@ -359,11 +439,15 @@ althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_
   althea_status st = check_launch("gbuffer_clear_kernel");
   if (st != ALTHEA_OK) return st;
   if (mesh->triangle_count > 0) {
-    hipLaunchKernelGGL(gbuffer_setup_kernel, dim3((mesh->triangle_count + 255) / 256), dim3(256), 0, s, p);
+    const int nb = static_cast<int>(gb_setup_blocks(mesh->triangle_count));
+    hipLaunchKernelGGL(gbuffer_setup_kernel, dim3(nb), dim3(kGbSetupThreads), 0, s, p);
     if ((st = check_launch("gbuffer_setup_kernel")) != ALTHEA_OK) return st;
-    hipLaunchKernelGGL(gbuffer_scan_kernel, dim3(1), dim3(kGbScanThreads), 0, s, p.areas, mesh->triangle_count,
-                       p.offsets);
+    hipLaunchKernelGGL(gbuffer_scan_kernel, dim3(1), dim3(kGbScanThreads), 0, s, p.blocks, nb,
+                       p.offsets + mesh->triangle_count);
     if ((st = check_launch("gbuffer_scan_kernel")) != ALTHEA_OK) return st;
+    hipLaunchKernelGGL(gbuffer_offset_kernel, dim3((mesh->triangle_count + 255) / 256), dim3(256), 0, s, p.offsets,
+                       p.blocks, mesh->triangle_count);
+    if ((st = check_launch("gbuffer_offset_kernel")) != ALTHEA_OK) return st;
     hipLaunchKernelGGL(gbuffer_raster_kernel, dim3(kGbRasterBlocks), dim3(256), 0, s, p);
     if ((st = check_launch("gbuffer_raster_kernel")) != ALTHEA_OK) return st;
   }

@@ -407,7 +407,7 @@ typedef struct althea_gbuffer_mesh {
 } althea_gbuffer_mesh;
 
 /* Device workspace althea_gbuffer_draw needs: the per-pixel visibility keys
- * (8 B/px) and per-triangle setup (76 B/triangle), 256-byte aligned parts. */
+ * (8 B/px) and per-triangle setup (72 B/triangle), 256-byte aligned parts. */
 ALTHEA_API size_t althea_gbuffer_workspace_bytes(int width, int height, int triangle_count);
 
 /* SceneToGBufferPass (Src/DeferredRendering.cpp:268-332) with the
