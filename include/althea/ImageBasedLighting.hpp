@@ -222,6 +222,37 @@ inline void convolveReflectionBuffer(ReflectionBuffer& reflection, hipStream_t s
 
 }  // namespace ScreenSpaceReflection
 
+// SceneToGBufferPass (Src/DeferredRendering.cpp:268-332) with the
+// pass-through subpass (GBufferPassThrough.frag): begin() rasterizes a mesh
+// with the camera of `globals` into the caller's G-buffer planes.  Owns the
+// workspace (visibility keys and per-triangle setup), grown on demand.
+class SceneToGBufferPass {
+ public:
+  SceneToGBufferPass() = default;
+  explicit SceneToGBufferPass(const GBufferResources& gbuffer) : _gbuffer(gbuffer.view()) {}
+
+  void begin(const althea_gbuffer_mesh& mesh, const althea_globals& globals, hipStream_t stream = nullptr) {
+    const size_t need = althea_gbuffer_workspace_bytes(_gbuffer.width, _gbuffer.height, mesh.triangle_count);
+    if (need == 0) throw std::runtime_error("SceneToGBufferPass: invalid extent or mesh");
+    if (need > _bytes) {
+      void* ws = nullptr;
+      if (hipMalloc(&ws, need) != hipSuccess) throw std::runtime_error("SceneToGBufferPass: hipMalloc failed");
+      _workspace.reset(ws);
+      _bytes = need;
+    }
+    throwIfFailed(althea_gbuffer_draw(&mesh, &globals, &_gbuffer, _workspace.get(), stream),
+                  "SceneToGBufferPass::begin");
+  }
+
+ private:
+  struct Free {
+    void operator()(void* p) const { (void)hipFree(p); }
+  };
+  althea_gbuffer_view _gbuffer{};
+  std::unique_ptr<void, Free> _workspace;
+  size_t _bytes = 0;
+};
+
 // PointLightCollection (Include/Althea/PointLight.h:39-126): the point lights
 // in device memory (PointLight layout) and, when created with shadow maps,
 // their cubes (shadowSize^2 x 6 per light, 256 in the reference, stored with

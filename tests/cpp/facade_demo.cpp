@@ -137,10 +137,56 @@ int main() {
     HIP_OK(hipMemcpy(unlit.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost));
     float shadow_diff = 0.0f;
     for (size_t i = 0; i < out.size(); ++i) shadow_diff = std::fmax(shadow_diff, std::fabs(shadowed[i] - unlit[i]));
-    HIP_OK(hipFree(d_quad));
-    HIP_OK(hipFree(d_tris));
     std::printf("shadowed frame vs unlit frame: max diff %.3g\n", shadow_diff);
     finite = finite && shadow_diff <= 1e-6f;
+    // The G-buffer from a mesh: the occluder square drawn by the camera looking
+    // down -z covers the whole 4x4 frame at z = -1, facing it.
+    {
+      const float white[12] = {0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f, 0.8f};
+      const float mro4[12] = {0.0f, 0.5f, 1.0f, 0.0f, 0.5f, 1.0f, 0.0f, 0.5f, 1.0f, 0.0f, 0.5f, 1.0f};
+      float *d_white, *d_mro4;
+      HIP_OK(hipMalloc(&d_white, sizeof(white)));
+      HIP_OK(hipMalloc(&d_mro4, sizeof(mro4)));
+      HIP_OK(hipMemcpy(d_white, white, sizeof(white), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(d_mro4, mro4, sizeof(mro4), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(d_quad, quad, sizeof(quad), hipMemcpyHostToDevice));
+      althea_globals cam = g;  // a [0, 1]-depth perspective, 90 degrees, near 0.01, far 1000
+      for (float& v : cam.projection) v = 0.0f;
+      for (float& v : cam.inverse_projection) v = 0.0f;
+      const float n = 0.01f, f = 1000.0f;
+      cam.projection[0] = 1.0f;
+      cam.projection[5] = -1.0f;
+      cam.projection[10] = f / (n - f);
+      cam.projection[11] = -1.0f;
+      cam.projection[14] = -(f * n) / (f - n);
+      cam.inverse_projection[0] = 1.0f;
+      cam.inverse_projection[5] = -1.0f;
+      cam.inverse_projection[11] = 1.0f / cam.projection[14];
+      cam.inverse_projection[14] = -1.0f;
+      cam.inverse_projection[15] = cam.projection[10] / cam.projection[14];
+      float *gp, *gn, *ga, *gm;
+      HIP_OK(hipMalloc(&gp, pos.size() * 4));
+      HIP_OK(hipMalloc(&gn, pos.size() * 4));
+      HIP_OK(hipMalloc(&ga, pos.size() * 4));
+      HIP_OK(hipMalloc(&gm, pos.size() * 4));
+      HIP_OK(hipMemcpy(d_tris, tris, sizeof(tris), hipMemcpyHostToDevice));
+      GBufferResources produced(gp, gn, ga, gm, GW, GH);
+      SceneToGBufferPass pass(produced);
+      pass.begin(althea_gbuffer_mesh{d_quad, nullptr, d_white, d_mro4, d_tris, 4, 2}, cam);
+      std::vector<float> gpos(pos.size()), gnrm(pos.size());
+      HIP_OK(hipMemcpy(gpos.data(), gp, gpos.size() * 4, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(gnrm.data(), gn, gnrm.size() * 4, hipMemcpyDeviceToHost));
+      bool gb_ok = true;
+      for (int i = 0; i < GW * GH; ++i)
+        gb_ok = gb_ok && gpos[4 * i + 3] == 1.0f && std::fabs(gpos[4 * i + 2] + 1.0f) < 1e-5f &&
+                std::fabs(gnrm[4 * i + 2] + 1.0f) < 1e-5f;  // derivative normal: away from the eye
+      std::printf("G-buffer from a mesh: every pixel covered at z = -1, normal away from the eye: %d\n",
+                  gb_ok ? 1 : 0);
+      finite = finite && gb_ok;
+      for (float* q : {gp, gn, ga, gm, d_white, d_mro4}) HIP_OK(hipFree(q));
+    }
+    HIP_OK(hipFree(d_quad));
+    HIP_OK(hipFree(d_tris));
     // Errors come back as exceptions, as the reference throws.
     bool threw = false;
     try {
