@@ -29,7 +29,8 @@ from . import synth
 from ._lib import AltheaError, check, lib
 
 __all__ = ["AltheaError", "IBLDesc", "IBLResources", "ImageBasedLighting", "GBufferResources",
-           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "PointLightCollection", "GBufferMesh",
+           "DeferredRendering", "ReflectionBuffer", "ScreenSpaceReflection", "PointLightCollection", "GBufferMesh", "Texture",
+           "GltfMaterial", "GltfPrimitive",
            "SceneToGBufferPass", "synth",
            "hammersley", "env_pyramid", "prefilter_cube", "irradiance_cube",
            "brdf_lut", "cube_sample", "env_sample", "prefilter_points", "env_levels", "cube_texels"]
@@ -364,6 +365,70 @@ class GBufferMesh:
                                  _ptr(self.indices) if n else None, self.positions.numel() // 3, n)
 
 
+class Texture:
+    """An RGBA8 texture with its mip chain (Src/Texture.cpp:50-100) in one
+    uint8 device tensor, levels back to back, and its sampler state
+    (Src/Sampler.cpp): `level0` an (h, w, 4) uint8 array or tensor; the other
+    `mips` - 1 levels are built by the linear blit of Image::generateMipMaps
+    (in linear space for sRGB)."""
+
+    def __init__(self, level0, mips: int = 1, srgb: bool = False, mag_filter: int = _lib.FILTER_LINEAR,
+                 min_filter: int = _lib.FILTER_LINEAR, mipmap_mode: int = _lib.FILTER_LINEAR,
+                 wrap_u: int = _lib.WRAP_REPEAT, wrap_v: int = _lib.WRAP_REPEAT, stream=None):
+        torch = _torch()
+        t0 = torch.as_tensor(level0).to(torch.uint8)
+        self.height, self.width = int(t0.shape[0]), int(t0.shape[1])
+        self.mips, self.srgb = mips, srgb
+        self.sampler = (mag_filter, min_filter, mipmap_mode, wrap_u, wrap_v)
+        n = lib().althea_texture_texels(self.width, self.height, mips)
+        if n == 0:
+            raise ValueError(f"invalid texture {self.width}x{self.height}, {mips} levels")
+        self.data = torch.zeros((n, 4), dtype=torch.uint8, device="cuda")
+        self.data[:self.width * self.height].copy_(t0.reshape(-1, 4))
+        check(lib().althea_texture_build_mips(_ptr(self.data), self.width, self.height, mips, 1 if srgb else 0,
+                                              _stream(stream)),
+              "althea_texture_build_mips")
+
+    def level(self, m: int):
+        w, h = max(1, self.width >> m), max(1, self.height >> m)
+        off = lib().althea_texture_mip_offset(self.width, self.height, m)
+        return self.data[off:off + w * h].view(h, w, 4)
+
+    def view(self) -> _lib.TextureViewC:
+        return _lib.TextureViewC(_ptr(self.data), self.width, self.height, self.mips, 1 if self.srgb else 0,
+                                 *self.sampler)
+
+
+class GltfMaterial:
+    """MaterialConstants of Src/Material.cpp: factors, texture slots (indices
+    into the texture list passed with the draw) and texture-coordinate sets;
+    alpha_cutoff 0.5 as glTF's default."""
+
+    def __init__(self, base_texture: int, normal_texture: int, metallic_roughness_texture: int,
+                 base_color_factor=(1.0, 1.0, 1.0, 1.0), metallic_factor: float = 1.0, roughness_factor: float = 1.0,
+                 normal_scale: float = 1.0, alpha_cutoff: float = 0.5, base_uv: int = 0,
+                 metallic_roughness_uv: int = 0):
+        self.c = _lib.GltfMaterialC((C.c_float * 4)(*base_color_factor), metallic_factor, roughness_factor,
+                                    normal_scale, alpha_cutoff, base_texture, normal_texture,
+                                    metallic_roughness_texture, base_uv, metallic_roughness_uv)
+
+
+class GltfPrimitive:
+    """A drawn glTF primitive in world space: float32 device tensors
+    positions, tangents, bitangents, normals (v, 3), uvs a list of up to four
+    (v, 2) tensors, indices (t, 3) int32 / uint32, and a material index."""
+
+    def __init__(self, positions, tangents, bitangents, normals, uvs, indices, material: int):
+        self.positions, self.tangents, self.bitangents, self.normals = positions, tangents, bitangents, normals
+        self.uvs, self.indices, self.material = list(uvs), indices, material
+
+    def view(self) -> _lib.GltfPrimitiveC:
+        uv = [(_ptr(self.uvs[k]) if k < len(self.uvs) and self.uvs[k] is not None else None) for k in range(4)]
+        return _lib.GltfPrimitiveC(_ptr(self.positions), _ptr(self.tangents), _ptr(self.bitangents),
+                                   _ptr(self.normals), (C.c_void_p * 4)(*uv), _ptr(self.indices),
+                                   self.positions.numel() // 3, self.indices.numel() // 3, self.material)
+
+
 class SceneToGBufferPass:
     """SceneToGBufferPass (Src/DeferredRendering.cpp:268-332) with the
     pass-through subpass: `begin(mesh, globals)` rasterizes a GBufferMesh with
@@ -388,6 +453,26 @@ class SceneToGBufferPass:
         check(lib().althea_gbuffer_draw(C.byref(mv), C.byref(globals_), C.byref(gv), _ptr(self.workspace),
                                         _stream(stream)),
               "althea_gbuffer_draw")
+        return g
+
+    def begin_gltf(self, primitives, materials, textures, globals_, stream=None) -> GBufferResources:
+        """The glTF subpass (Gltf.frag): GltfPrimitive list drawn in order,
+        GltfMaterial list, Texture list."""
+        torch = _torch()
+        g = self.gbuffer
+        prims = (_lib.GltfPrimitiveC * max(1, len(primitives)))(*[p.view() for p in primitives])
+        mats = (_lib.GltfMaterialC * max(1, len(materials)))(*[m.c for m in materials])
+        texs = (_lib.TextureViewC * max(1, len(textures)))(*[t.view() for t in textures])
+        need = lib().althea_gbuffer_gltf_workspace_bytes(g.width, g.height, prims, len(primitives), len(materials),
+                                                         len(textures))
+        if need == 0:
+            raise ValueError("invalid glTF draw")
+        if self.workspace is None or self.workspace.numel() < need:
+            self.workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+        gv = g.getHandles()
+        check(lib().althea_gbuffer_draw_gltf(prims, len(primitives), mats, len(materials), texs, len(textures),
+                                             C.byref(globals_), C.byref(gv), _ptr(self.workspace), _stream(stream)),
+              "althea_gbuffer_draw_gltf")
         return g
 
 

@@ -76,6 +76,27 @@ class GBufferMeshC(C.Structure):
                 ("vertex_count", i32), ("triangle_count", i32)]
 
 
+class TextureViewC(C.Structure):
+    _fields_ = [("texels", vp), ("width", i32), ("height", i32), ("mips", i32), ("srgb", i32), ("mag_filter", i32),
+                ("min_filter", i32), ("mipmap_mode", i32), ("wrap_u", i32), ("wrap_v", i32)]
+
+
+class GltfMaterialC(C.Structure):
+    _fields_ = [("base_color_factor", C.c_float * 4), ("metallic_factor", C.c_float), ("roughness_factor", C.c_float),
+                ("normal_scale", C.c_float), ("alpha_cutoff", C.c_float), ("base_texture", i32),
+                ("normal_texture", i32), ("metallic_roughness_texture", i32), ("base_uv", i32),
+                ("metallic_roughness_uv", i32)]
+
+
+class GltfPrimitiveC(C.Structure):
+    _fields_ = [("positions", vp), ("tangents", vp), ("bitangents", vp), ("normals", vp), ("uvs", vp * 4),
+                ("indices", vp), ("vertex_count", i32), ("triangle_count", i32), ("material", i32)]
+
+
+WRAP_REPEAT, WRAP_MIRRORED_REPEAT, WRAP_CLAMP_TO_EDGE = 0, 1, 2  # ALTHEA_WRAP_*
+FILTER_NEAREST, FILTER_LINEAR = 0, 1                               # ALTHEA_FILTER_*
+
+
 class MeshViewC(C.Structure):
     _fields_ = [("positions", vp), ("indices", vp), ("vertex_count", i32), ("triangle_count", i32)]
 
@@ -132,6 +153,13 @@ PROTOTYPES = {
                                               C.c_uint, vp, vp, vp]),
     "althea_gbuffer_workspace_bytes": (sz, [i32, i32, i32]),
     "althea_gbuffer_draw": (i32, [C.POINTER(GBufferMeshC), C.POINTER(GlobalsC), C.POINTER(GBufferViewC), vp, vp]),
+    "althea_texture_texels": (sz, [i32, i32, i32]),
+    "althea_texture_mip_offset": (sz, [i32, i32, i32]),
+    "althea_texture_build_mips": (i32, [vp, i32, i32, i32, i32, vp]),
+    "althea_gbuffer_gltf_workspace_bytes": (sz, [i32, i32, C.POINTER(GltfPrimitiveC), i32, i32, i32]),
+    "althea_gbuffer_draw_gltf": (i32, [C.POINTER(GltfPrimitiveC), i32, C.POINTER(GltfMaterialC), i32,
+                                       C.POINTER(TextureViewC), i32, C.POINTER(GlobalsC), C.POINTER(GBufferViewC), vp,
+                                       vp]),
     "althea_shadow_cube_floats": (sz, [i32]),
     "althea_shadow_cubes": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp]),
     "althea_deferred_resolve_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,

@@ -568,6 +568,107 @@ althea_status althea_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_
   return launch_gbuffer_draw(mesh, globals, g, d_workspace, stream);
 }
 
+size_t althea_texture_mip_offset(int width, int height, int level) {
+  if (width < 1 || height < 1 || level < 0) return 0;
+  size_t off = 0;
+  for (int l = 0; l < level; ++l)
+    off += static_cast<size_t>(std::max(1, width >> l)) * static_cast<size_t>(std::max(1, height >> l));
+  return off;
+}
+
+size_t althea_texture_texels(int width, int height, int mips) {
+  if (width < 1 || height < 1 || mips < 1 || mips > 16) return 0;
+  return althea_texture_mip_offset(width, height, mips);
+}
+
+static int texture_max_mips(int w, int h) {
+  int m = 1;
+  while ((std::max(w, h) >> m) > 0) ++m;
+  return m;
+}
+
+althea_status althea_texture_build_mips(uint8_t* d_texels, int width, int height, int mips, int srgb,
+                                        hipStream_t stream) {
+  ALTHEA_REQUIRE(d_texels && width >= 1 && height >= 1 && width <= 16384 && height <= 16384,
+                 "texture_build_mips: texels %p, %dx%d", static_cast<void*>(d_texels), width, height);
+  ALTHEA_REQUIRE(mips >= 1 && mips <= texture_max_mips(width, height), "texture_build_mips: %d levels for %dx%d",
+                 mips, width, height);
+  return launch_texture_mips(d_texels, width, height, mips, srgb ? 1 : 0, stream);
+}
+
+static althea_status texture_ok(const althea_texture_view& t, int k) {
+  ALTHEA_REQUIRE(t.texels && t.width >= 1 && t.height >= 1 && t.width <= 16384 && t.height <= 16384,
+                 "texture %d: texels %p, %dx%d", k, static_cast<const void*>(t.texels), t.width, t.height);
+  ALTHEA_REQUIRE(t.mips >= 1 && t.mips <= texture_max_mips(t.width, t.height), "texture %d: %d levels for %dx%d", k,
+                 t.mips, t.width, t.height);
+  auto filt = [](int f) { return f == ALTHEA_FILTER_NEAREST || f == ALTHEA_FILTER_LINEAR; };
+  auto wrap = [](int w) { return w >= ALTHEA_WRAP_REPEAT && w <= ALTHEA_WRAP_CLAMP_TO_EDGE; };
+  ALTHEA_REQUIRE(filt(t.mag_filter) && filt(t.min_filter) && filt(t.mipmap_mode) && wrap(t.wrap_u) && wrap(t.wrap_v),
+                 "texture %d: sampler state", k);
+  return ALTHEA_OK;
+}
+
+static int gltf_triangles(const althea_gltf_primitive* prims, int count) {
+  long long total = 0;
+  for (int k = 0; k < count; ++k) total += prims[k].triangle_count;
+  return total < (1ll << 28) ? static_cast<int>(total) : -1;
+}
+
+size_t althea_gbuffer_gltf_workspace_bytes(int width, int height, const althea_gltf_primitive* primitives,
+                                           int primitive_count, int material_count, int texture_count) {
+  if (primitive_count < 0 || (primitive_count > 0 && !primitives) || material_count < 0 || texture_count < 0)
+    return 0;
+  const int total = gltf_triangles(primitives, primitive_count);
+  if (total < 0) return 0;
+  return gbuffer_gltf_workspace_bytes(width, height, total, primitive_count, material_count, texture_count);
+}
+
+althea_status althea_gbuffer_draw_gltf(const althea_gltf_primitive* primitives, int primitive_count,
+                                       const althea_gltf_material* materials, int material_count,
+                                       const althea_texture_view* textures, int texture_count,
+                                       const althea_globals* globals, const althea_gbuffer_view* g, void* d_workspace,
+                                       hipStream_t stream) {
+  ALTHEA_REQUIRE(globals && g && d_workspace, "gbuffer_draw_gltf: null argument");
+  ALTHEA_REQUIRE(primitive_count >= 0 && (primitive_count == 0 || primitives), "gbuffer_draw_gltf: %d primitives",
+                 primitive_count);
+  ALTHEA_REQUIRE(material_count >= 0 && (material_count == 0 || materials) && texture_count >= 0 &&
+                     (texture_count == 0 || textures),
+                 "gbuffer_draw_gltf: %d materials, %d textures", material_count, texture_count);
+  ALTHEA_REQUIRE(g->width >= 1 && g->height >= 1 && static_cast<long long>(g->width) * g->height < (1ll << 31),
+                 "gbuffer_draw_gltf: extent %dx%d", g->width, g->height);
+  ALTHEA_REQUIRE(g->position && g->normal && g->albedo && g->mro, "gbuffer_draw_gltf: null G-buffer plane");
+  ALTHEA_REQUIRE(g->format == ALTHEA_GBUFFER_FP32 || g->format == ALTHEA_GBUFFER_FP16,
+                 "gbuffer_draw_gltf: format %d", g->format);
+  ALTHEA_REQUIRE(globals->projection[10] != 0.0f && globals->projection[10] != -1.0f && globals->projection[14] != 0.0f,
+                 "gbuffer_draw_gltf: projection is not a [0, 1]-depth perspective");
+  ALTHEA_REQUIRE(gltf_triangles(primitives, primitive_count) >= 0, "gbuffer_draw_gltf: too many triangles");
+  for (int k = 0; k < texture_count; ++k) {
+    const althea_status st = texture_ok(textures[k], k);
+    if (st != ALTHEA_OK) return st;
+  }
+  for (int k = 0; k < material_count; ++k) {
+    const althea_gltf_material& m = materials[k];
+    auto tex = [&](int t) { return t >= 0 && t < texture_count; };
+    ALTHEA_REQUIRE(tex(m.base_texture) && tex(m.normal_texture) && tex(m.metallic_roughness_texture),
+                   "gbuffer_draw_gltf: material %d names a texture outside 0..%d", k, texture_count - 1);
+    ALTHEA_REQUIRE(m.base_uv >= 0 && m.base_uv < 4 && m.metallic_roughness_uv >= 0 && m.metallic_roughness_uv < 4,
+                   "gbuffer_draw_gltf: material %d texture-coordinate sets", k);
+  }
+  for (int k = 0; k < primitive_count; ++k) {
+    const althea_gltf_primitive& q = primitives[k];
+    ALTHEA_REQUIRE(q.triangle_count >= 0 && q.material >= 0 && q.material < material_count,
+                   "gbuffer_draw_gltf: primitive %d: %d triangles, material %d", k, q.triangle_count, q.material);
+    if (q.triangle_count == 0) continue;
+    ALTHEA_REQUIRE(q.positions && q.tangents && q.bitangents && q.normals && q.indices && q.vertex_count >= 1,
+                   "gbuffer_draw_gltf: primitive %d: null vertex data", k);
+    const althea_gltf_material& m = materials[q.material];
+    ALTHEA_REQUIRE(q.uvs[m.base_uv] && q.uvs[m.metallic_roughness_uv],
+                   "gbuffer_draw_gltf: primitive %d lacks the texture coordinates its material reads", k);
+  }
+  return launch_gbuffer_draw_gltf(primitives, primitive_count, materials, material_count, textures, texture_count,
+                                  globals, g, d_workspace, stream);
+}
+
 size_t althea_shadow_cube_floats(int size) {
   return size < 1 ? 0 : static_cast<size_t>(6) * (size + 2) * (size + 2);
 }

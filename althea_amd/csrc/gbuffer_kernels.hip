@@ -32,11 +32,20 @@
 //                          on the order of the writes;
 //   gbuffer_resolve_kernel one lane per pixel: the winning triangle's
 //                          barycentrics at the pixel centre and the four
-//                          G-buffer planes.
+//                          G-buffer planes, from the pass-through inputs or
+//                          the glTF material (textures sampled at the mip
+//                          level of the texture coordinate's screen
+//                          derivatives).
+// A draw holds one or more primitives, their triangles numbered one after
+// the other; with the glTF subpass a candidate fragment whose base colour
+// alpha is below the material's cut-off is discarded before the depth test.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "device_math.hpp"
 #include "internal.hpp"
@@ -57,13 +66,28 @@ struct GbTri {
 };
 static_assert(sizeof(GbTri) == 64, "GbTri is one 64-byte record");
 
-struct GbParams {
+// One drawn primitive.  material < 0: the pass-through inputs (normals may
+// be null: all zero); otherwise the glTF inputs and that material.
+struct GbPrim {
   const float* positions;
-  const float* normals;  // may be null (zero normals)
+  const float* normals;
   const float* colors;
   const float* mro;
+  const float* tangents;
+  const float* bitangents;
+  const float* uvs[4];
   const uint32_t* indices;
-  int vertex_count, triangle_count;
+  int vertex_count, tri_begin, triangle_count, material;
+};
+
+struct GbParams {
+  GbPrim one;                 // the primitive when prims is null
+  const GbPrim* prims;        // device table of prim_count primitives, or null
+  int prim_count;
+  const althea_gltf_material* materials;
+  const althea_texture_view* textures;
+  int alpha_test;             // some primitive is a glTF one
+  int triangle_count;         // over all primitives
   int width, height;
   float eye[3];
   float ray0[3], rayx[3], rayy[3];  // pixel ray = ray0 + rayx (x + 0.5) + rayy (y + 0.5)
@@ -86,6 +110,20 @@ __device__ __forceinline__ f3 gb_vertex(const float* __restrict__ a, uint32_t i)
   return mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
 }
 
+// Primitive of triangle t (numbered over the draw).
+__device__ __forceinline__ const GbPrim& gb_prim(const GbParams& p, int t) {
+  if (!p.prims) return p.one;
+  int lo = 0, hi = p.prim_count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (p.prims[mid].tri_begin <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return p.prims[lo];
+}
+
 __device__ __forceinline__ f3 gb_ray(const GbParams& p, int x, int y) {
   const float cx = static_cast<float>(x) + 0.5f, cy = static_cast<float>(y) + 0.5f;
   return mk(fmaf(p.rayy[0], cy, fmaf(p.rayx[0], cx, p.ray0[0])), fmaf(p.rayy[1], cy, fmaf(p.rayx[1], cx, p.ray0[1])),
@@ -93,10 +131,12 @@ __device__ __forceinline__ f3 gb_ray(const GbParams& p, int x, int y) {
 }
 
 __device__ void gbuffer_setup_triangle(const GbParams& p, int t, GbTri& tri, unsigned long long& area) {
-  const uint32_t last = static_cast<uint32_t>(p.vertex_count - 1);
+  const GbPrim& pr = gb_prim(p, t);
+  const int lt = t - pr.tri_begin;
+  const uint32_t last = static_cast<uint32_t>(pr.vertex_count - 1);
   const f3 eye = mk(p.eye[0], p.eye[1], p.eye[2]);
   f3 v[3];
-  for (int k = 0; k < 3; ++k) v[k] = gb_vertex(p.positions, min(p.indices[3 * t + k], last)) - eye;
+  for (int k = 0; k < 3; ++k) v[k] = gb_vertex(pr.positions, min(pr.indices[3 * lt + k], last)) - eye;
   // Edge functions b x c, c x a, a x b formed from the edges e1 = b - a,
   // e2 = c - a (c x a = e2 x a, a x b = a x e1, b x c = n - both), so their
   // rounding scales with |a| |e| rather than |a| |b|: far, small triangles
@@ -240,6 +280,142 @@ __device__ __forceinline__ float gb_hit(const GbParams& p, const GbTri& tri, f3 
   return t;
 }
 
+// ---------------------------------------------------------------------------
+// Texture sampling (Vulkan rules): wrap per integer texel coordinate; LINEAR
+// = bilinear at (u w - 0.5, v h - 0.5), NEAREST = texel floor(u w); sRGB
+// texels decoded to linear before filtering; the mip level from
+// lambda = log2(max(|d(u w, v h)/dx|, |d(u w, v h)/dy|)): lambda <= 0
+// magnifies (mag filter, level 0), else the min filter at
+// d = min(lambda, mips - 1), the nearest level ceil(d + 0.5) - 1 or the
+// linear blend of levels floor(d) and floor(d) + 1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gb_srgb_to_linear(float c) {
+  return c <= 0.04045f ? c * (1.0f / 12.92f) : powf((c + 0.055f) * (1.0f / 1.055f), 2.4f);
+}
+
+// The sRGB decode of every 8-bit value (the transfer function evaluated in
+// FP64 on the host, rounded to FP32), uploaded once per device.
+__device__ float gb_srgb_lut[256];
+
+__device__ __forceinline__ int gb_wrap(int i, int n, int mode) {
+  if (mode == ALTHEA_WRAP_CLAMP_TO_EDGE) return min(max(i, 0), n - 1);
+  if (mode == ALTHEA_WRAP_MIRRORED_REPEAT) {
+    int m = i % (2 * n);
+    if (m < 0) m += 2 * n;
+    return m < n ? m : 2 * n - 1 - m;
+  }
+  const int m = i % n;
+  return m < 0 ? m + n : m;
+}
+
+template <bool kAlphaOnly>
+__device__ __forceinline__ float4 gb_texel(const althea_texture_view& t, size_t off, int w, int i, int j) {
+  const uchar4 c = reinterpret_cast<const uchar4*>(t.texels)[off + static_cast<size_t>(j) * w + i];
+  if (kAlphaOnly) return make_float4(0.0f, 0.0f, 0.0f, c.w * (1.0f / 255.0f));
+  if (t.srgb) return make_float4(gb_srgb_lut[c.x], gb_srgb_lut[c.y], gb_srgb_lut[c.z], c.w * (1.0f / 255.0f));
+  return make_float4(c.x * (1.0f / 255.0f), c.y * (1.0f / 255.0f), c.z * (1.0f / 255.0f), c.w * (1.0f / 255.0f));
+}
+
+template <bool kAlphaOnly = false>
+__device__ float4 gb_sample_level(const althea_texture_view& t, int level, float u, float v, int filter) {
+  size_t off = 0;
+  for (int l = 0; l < level; ++l) off += static_cast<size_t>(max(1, t.width >> l)) * max(1, t.height >> l);
+  const int w = max(1, t.width >> level), h = max(1, t.height >> level);
+  if (filter == ALTHEA_FILTER_NEAREST) {
+    const int i = gb_wrap(static_cast<int>(floorf(u * w)), w, t.wrap_u);
+    const int j = gb_wrap(static_cast<int>(floorf(v * h)), h, t.wrap_v);
+    return gb_texel<kAlphaOnly>(t, off, w, i, j);
+  }
+  const float x = fmaf(u, static_cast<float>(w), -0.5f), y = fmaf(v, static_cast<float>(h), -0.5f);
+  const float fx = floorf(x), fy = floorf(y);
+  const float ax = x - fx, ay = y - fy;
+  const int i0 = gb_wrap(static_cast<int>(fx), w, t.wrap_u), i1 = gb_wrap(static_cast<int>(fx) + 1, w, t.wrap_u);
+  const int j0 = gb_wrap(static_cast<int>(fy), h, t.wrap_v), j1 = gb_wrap(static_cast<int>(fy) + 1, h, t.wrap_v);
+  const float4 t00 = gb_texel<kAlphaOnly>(t, off, w, i0, j0), t10 = gb_texel<kAlphaOnly>(t, off, w, i1, j0);
+  const float4 t01 = gb_texel<kAlphaOnly>(t, off, w, i0, j1), t11 = gb_texel<kAlphaOnly>(t, off, w, i1, j1);
+  auto lerp2 = [&](float a, float b, float c, float d) {
+    const float top = fmaf(b - a, ax, a), bot = fmaf(d - c, ax, c);
+    return fmaf(bot - top, ay, top);
+  };
+  return make_float4(lerp2(t00.x, t10.x, t01.x, t11.x), lerp2(t00.y, t10.y, t01.y, t11.y),
+                     lerp2(t00.z, t10.z, t01.z, t11.z), lerp2(t00.w, t10.w, t01.w, t11.w));
+}
+
+// Texture coordinate and its screen derivatives (forward differences to the
+// next pixel to the right and below on the triangle's plane).
+struct GbUv {
+  float u, v, dudx, dvdx, dudy, dvdy;
+};
+
+template <bool kAlphaOnly = false>
+__device__ float4 gb_sample(const althea_texture_view& t, const GbUv& c) {
+  const float W = static_cast<float>(t.width), H = static_cast<float>(t.height);
+  const float ax = c.dudx * W, bx = c.dvdx * H, ay = c.dudy * W, by = c.dvdy * H;
+  const float rho2 = fmaxf(fmaf(ax, ax, bx * bx), fmaf(ay, ay, by * by));
+  const float lambda = 0.5f * log2f(rho2);  // log2 of the larger scale factor
+  if (!(lambda > 0.0f)) return gb_sample_level<kAlphaOnly>(t, 0, c.u, c.v, t.mag_filter);
+  const int q = t.mips - 1;
+  if (q == 0) return gb_sample_level<kAlphaOnly>(t, 0, c.u, c.v, t.min_filter);
+  const float d = fminf(lambda, static_cast<float>(q));
+  if (t.mipmap_mode == ALTHEA_FILTER_NEAREST) {
+    const int level = min(max(static_cast<int>(ceilf(d + 0.5f)) - 1, 0), q);
+    return gb_sample_level<kAlphaOnly>(t, level, c.u, c.v, t.min_filter);
+  }
+  const float fl = floorf(d);
+  const int l0 = static_cast<int>(fl), l1 = min(l0 + 1, q);
+  const float delta = d - fl;
+  const float4 a = gb_sample_level<kAlphaOnly>(t, l0, c.u, c.v, t.min_filter);
+  if (delta == 0.0f) return a;
+  const float4 b = gb_sample_level<kAlphaOnly>(t, l1, c.u, c.v, t.min_filter);
+  return make_float4(fmaf(b.x - a.x, delta, a.x), fmaf(b.y - a.y, delta, a.y), fmaf(b.z - a.z, delta, a.z),
+                     fmaf(b.w - a.w, delta, a.w));
+}
+
+// Barycentrics of ray d on the triangle's plane (outside the triangle too).
+__device__ __forceinline__ void gb_bary(const GbTri& tri, f3 d, float l[3]) {
+  const float e0 = fmaf(d.z, tri.n0[2], fmaf(d.y, tri.n0[1], d.x * tri.n0[0]));
+  const float e1 = fmaf(d.z, tri.n1[2], fmaf(d.y, tri.n1[1], d.x * tri.n1[0]));
+  const float e2 = fmaf(d.z, tri.n2[2], fmaf(d.y, tri.n2[1], d.x * tri.n2[0]));
+  const float inv = 1.0f / (e0 + e1 + e2);
+  l[0] = e0 * inv, l[1] = e1 * inv, l[2] = e2 * inv;
+}
+
+__device__ __forceinline__ GbUv gb_uv(const float* __restrict__ uv, const uint32_t vi[3], const float l[3],
+                                      const float lx[3], const float ly[3]) {
+  GbUv c{};
+  if (!uv) return c;
+  const float2 a = reinterpret_cast<const float2*>(uv)[vi[0]], b = reinterpret_cast<const float2*>(uv)[vi[1]],
+               cc = reinterpret_cast<const float2*>(uv)[vi[2]];
+  auto at = [&](const float w[3], float& u, float& v) {
+    u = fmaf(w[2], cc.x, fmaf(w[1], b.x, w[0] * a.x));
+    v = fmaf(w[2], cc.y, fmaf(w[1], b.y, w[0] * a.y));
+  };
+  float ux, vx, uy, vy;
+  at(l, c.u, c.v);
+  at(lx, ux, vx);
+  at(ly, uy, vy);
+  c.dudx = ux - c.u, c.dvdx = vx - c.v, c.dudy = uy - c.u, c.dvdy = vy - c.v;
+  return c;
+}
+
+__device__ __forceinline__ void gb_indices(const GbPrim& pr, int lt, uint32_t vi[3]) {
+  const uint32_t last = static_cast<uint32_t>(pr.vertex_count - 1);
+  for (int k = 0; k < 3; ++k) vi[k] = min(pr.indices[3 * lt + k], last);
+}
+
+// Base colour alpha of a glTF fragment (the alpha test of Gltf.frag:40-42).
+__device__ float gb_base_alpha(const GbParams& p, const GbPrim& pr, const GbTri& tri, int lt, int x, int y, f3 d) {
+  const althea_gltf_material& m = p.materials[pr.material];
+  uint32_t vi[3];
+  gb_indices(pr, lt, vi);
+  float l[3], lx[3], ly[3];
+  gb_bary(tri, d, l);
+  gb_bary(tri, gb_ray(p, x + 1, y), lx);
+  gb_bary(tri, gb_ray(p, x, y + 1), ly);
+  const GbUv c = gb_uv(pr.uvs[m.base_uv], vi, l, lx, ly);
+  return gb_sample<true>(p.textures[m.base_texture], c).w * m.base_color_factor[3];
+}
+
 // Largest t in [lo, hi) with offsets[t] <= slot, given offsets[lo] <= slot <
 // offsets[hi].
 __device__ __forceinline__ int gb_find(const unsigned long long* __restrict__ offsets, int lo, int hi,
@@ -297,8 +473,15 @@ __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
     const int x = tri.x0 + static_cast<int>(local % static_cast<unsigned>(tri.w));
     const int y = tri.y0 + static_cast<int>(local / static_cast<unsigned>(tri.w));
     float e[3];
-    const float t = gb_hit(p, tri, gb_ray(p, x, y), e);
+    const f3 d = gb_ray(p, x, y);
+    const float t = gb_hit(p, tri, d, e);
     if (t < 0.0f) continue;
+    if (p.alpha_test) {
+      const GbPrim& pr = gb_prim(p, lo);
+      if (pr.material >= 0 && gb_base_alpha(p, pr, tri, lo - pr.tri_begin, x, y, d) <
+                                  p.materials[pr.material].alpha_cutoff)
+        continue;  // discarded: no depth written
+    }
     const unsigned long long key =
         (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | static_cast<unsigned>(lo);
     atomicMin(p.vis + static_cast<size_t>(y) * p.width + x, key);
@@ -350,16 +533,42 @@ __global__ __launch_bounds__(256) void gbuffer_resolve_kernel(const GbParams p) 
   const int t = static_cast<int>(key & 0xffffffffu);
   const int x = i % p.width, y = i / p.width;
   const GbTri tri = p.tris[t];
+  const GbPrim& pr = gb_prim(p, t);
+  const int lt = t - pr.tri_begin;
   const f3 d = gb_ray(p, x, y);
   float e[3];
   gb_hit(p, tri, d, e);
   const float inv = 1.0f / (e[0] + e[1] + e[2]);
   const float l[3] = {e[0] * inv, e[1] * inv, e[2] * inv};
-  const uint32_t last = static_cast<uint32_t>(p.vertex_count - 1);
-  const uint32_t vi[3] = {min(p.indices[3 * t], last), min(p.indices[3 * t + 1], last),
-                          min(p.indices[3 * t + 2], last)};
-  const f3 pos = gb_interp(p.positions, vi, l);
-  f3 nrm = p.normals ? gb_interp(p.normals, vi, l) : mk(0.0f, 0.0f, 0.0f);
+  uint32_t vi[3];
+  gb_indices(pr, lt, vi);
+  const f3 pos = gb_interp(pr.positions, vi, l);
+  gb_store(p.position_out, p.fp16, i, pos.x, pos.y, pos.z, 1.0f);
+  if (pr.material >= 0) {
+    // Gltf.frag with fetchMaterial (InstanceData.glsl:27-63).
+    const althea_gltf_material& m = p.materials[pr.material];
+    float lx[3], ly[3];
+    gb_bary(tri, gb_ray(p, x + 1, y), lx);
+    gb_bary(tri, gb_ray(p, x, y + 1), ly);
+    const GbUv cb = gb_uv(pr.uvs[m.base_uv], vi, l, lx, ly);
+    const GbUv cm = gb_uv(pr.uvs[m.metallic_roughness_uv], vi, l, lx, ly);
+    const float4 bt = gb_sample(p.textures[m.base_texture], cb);
+    const float4 base = make_float4(bt.x * m.base_color_factor[0], bt.y * m.base_color_factor[1],
+                                    bt.z * m.base_color_factor[2], bt.w * m.base_color_factor[3]);
+    const float4 nt = gb_sample(p.textures[m.normal_texture], cb);  // the base colour's coordinates
+    const float tx = fmaf(2.0f, nt.x, -1.0f) * m.normal_scale, ty = fmaf(2.0f, nt.y, -1.0f) * m.normal_scale;
+    const float tz = fmaf(2.0f, nt.z, -1.0f);
+    const f3 T = gb_interp(pr.tangents, vi, l), B = gb_interp(pr.bitangents, vi, l), N = gb_interp(pr.normals, vi, l);
+    f3 n = mk(fmaf(tz, N.x, fmaf(ty, B.x, tx * T.x)), fmaf(tz, N.y, fmaf(ty, B.y, tx * T.y)),
+              fmaf(tz, N.z, fmaf(ty, B.z, tx * T.z)));
+    n = n * rsqrtf(dot(n, n));
+    const float4 mr = gb_sample(p.textures[m.metallic_roughness_texture], cm);
+    gb_store(p.normal_out, p.fp16, i, n.x, n.y, n.z, base.w);
+    gb_store(p.albedo_out, p.fp16, i, base.x, base.y, base.z, base.w);
+    gb_store(p.mro_out, p.fp16, i, mr.z * m.metallic_factor, mr.y * m.roughness_factor, 0.0f, base.w);
+    return;
+  }
+  f3 nrm = pr.normals ? gb_interp(pr.normals, vi, l) : mk(0.0f, 0.0f, 0.0f);
   if (nrm.x == 0.0f && nrm.y == 0.0f && nrm.z == 0.0f) {
     // Screen derivatives of the position on the triangle's plane: the hits
     // of the next pixel's ray to the right (x + 1) and below (y + 1, the
@@ -371,9 +580,8 @@ __global__ __launch_bounds__(256) void gbuffer_resolve_kernel(const GbParams p) 
     const f3 c = cross(px, py);
     nrm = c * rsqrtf(dot(c, c));
   }
-  const f3 alb = gb_interp(p.colors, vi, l);
-  const f3 m = gb_interp(p.mro, vi, l);
-  gb_store(p.position_out, p.fp16, i, pos.x, pos.y, pos.z, 1.0f);
+  const f3 alb = gb_interp(pr.colors, vi, l);
+  const f3 m = gb_interp(pr.mro, vi, l);
   gb_store(p.normal_out, p.fp16, i, nrm.x, nrm.y, nrm.z, 1.0f);
   gb_store(p.albedo_out, p.fp16, i, alb.x, alb.y, alb.z, 1.0f);
   gb_store(p.mro_out, p.fp16, i, m.x, m.y, m.z, 1.0f);
@@ -395,16 +603,13 @@ size_t gbuffer_workspace_bytes(int width, int height, int triangle_count) {
 void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]);
 void view_projection(const althea_globals& g, float vp[16]);
 
-althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* g,
-                                  const althea_gbuffer_view* out, void* workspace, hipStream_t s) {
-  GbParams p{};
-  p.positions = mesh->positions;
-  p.normals = mesh->normals;
-  p.colors = mesh->colors;
-  p.mro = mesh->mro;
-  p.indices = mesh->indices;
-  p.vertex_count = mesh->vertex_count;
-  p.triangle_count = mesh->triangle_count;
+namespace {
+
+// Camera, planes and the workspace's fixed parts (keys, setup, offsets,
+// block totals); returns the first byte after them.
+char* gb_common(GbParams& p, const althea_globals* g, const althea_gbuffer_view* out, void* workspace,
+                int triangle_count) {
+  p.triangle_count = triangle_count;
   p.width = out->width;
   p.height = out->height;
   for (int c = 0; c < 3; ++c) {
@@ -421,7 +626,7 @@ althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_
   p.near_clip = 0.5f * p.near;
   char* ws = static_cast<char*>(workspace);
   const size_t npx = static_cast<size_t>(out->width) * out->height;
-  const size_t t = static_cast<size_t>(mesh->triangle_count);
+  const size_t t = static_cast<size_t>(triangle_count);
   p.vis = reinterpret_cast<unsigned long long*>(ws);
   ws += align256(npx * 8);
   p.tris = reinterpret_cast<GbTri*>(ws);
@@ -429,30 +634,185 @@ althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_
   p.offsets = reinterpret_cast<unsigned long long*>(ws);
   ws += align256((t + 1) * 8);
   p.blocks = reinterpret_cast<unsigned long long*>(ws);
+  ws += align256(gb_setup_blocks(triangle_count) * 8);
   p.position_out = const_cast<void*>(out->position);
   p.normal_out = const_cast<void*>(out->normal);
   p.albedo_out = const_cast<void*>(out->albedo);
   p.mro_out = const_cast<void*>(out->mro);
   p.fp16 = out->format == ALTHEA_GBUFFER_FP16 ? 1 : 0;
-  const int n = static_cast<int>(npx);
+  return ws;
+}
+
+althea_status gb_run(const GbParams& p, hipStream_t s) {
+  const int n = p.width * p.height;
+  const int T = p.triangle_count;
   hipLaunchKernelGGL(gbuffer_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p.vis, n);
   althea_status st = check_launch("gbuffer_clear_kernel");
   if (st != ALTHEA_OK) return st;
-  if (mesh->triangle_count > 0) {
-    const int nb = static_cast<int>(gb_setup_blocks(mesh->triangle_count));
+  if (T > 0) {
+    const int nb = static_cast<int>(gb_setup_blocks(T));
     hipLaunchKernelGGL(gbuffer_setup_kernel, dim3(nb), dim3(kGbSetupThreads), 0, s, p);
     if ((st = check_launch("gbuffer_setup_kernel")) != ALTHEA_OK) return st;
-    hipLaunchKernelGGL(gbuffer_scan_kernel, dim3(1), dim3(kGbScanThreads), 0, s, p.blocks, nb,
-                       p.offsets + mesh->triangle_count);
+    hipLaunchKernelGGL(gbuffer_scan_kernel, dim3(1), dim3(kGbScanThreads), 0, s, p.blocks, nb, p.offsets + T);
     if ((st = check_launch("gbuffer_scan_kernel")) != ALTHEA_OK) return st;
-    hipLaunchKernelGGL(gbuffer_offset_kernel, dim3((mesh->triangle_count + 255) / 256), dim3(256), 0, s, p.offsets,
-                       p.blocks, mesh->triangle_count);
+    hipLaunchKernelGGL(gbuffer_offset_kernel, dim3((T + 255) / 256), dim3(256), 0, s, p.offsets, p.blocks, T);
     if ((st = check_launch("gbuffer_offset_kernel")) != ALTHEA_OK) return st;
     hipLaunchKernelGGL(gbuffer_raster_kernel, dim3(kGbRasterBlocks), dim3(256), 0, s, p);
     if ((st = check_launch("gbuffer_raster_kernel")) != ALTHEA_OK) return st;
   }
   hipLaunchKernelGGL(gbuffer_resolve_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p);
   return check_launch("gbuffer_resolve_kernel");
+}
+
+}  // namespace
+
+althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* g,
+                                  const althea_gbuffer_view* out, void* workspace, hipStream_t s) {
+  GbParams p{};
+  p.one.positions = mesh->positions;
+  p.one.normals = mesh->normals;
+  p.one.colors = mesh->colors;
+  p.one.mro = mesh->mro;
+  p.one.indices = mesh->indices;
+  p.one.vertex_count = mesh->vertex_count;
+  p.one.triangle_count = mesh->triangle_count;
+  p.one.material = -1;
+  gb_common(p, g, out, workspace, mesh->triangle_count);
+  return gb_run(p, s);
+}
+
+size_t gbuffer_gltf_workspace_bytes(int width, int height, int triangle_count, int primitive_count,
+                                    int material_count, int texture_count) {
+  const size_t base = gbuffer_workspace_bytes(width, height, triangle_count);
+  if (base == 0) return 0;
+  return base + align256(static_cast<size_t>(primitive_count) * sizeof(GbPrim)) +
+         align256(static_cast<size_t>(material_count) * sizeof(althea_gltf_material)) +
+         align256(static_cast<size_t>(texture_count) * sizeof(althea_texture_view));
+}
+
+// Uploads the sRGB decode table to the current device, once per device.
+hipError_t gb_upload_srgb_lut(hipStream_t s) {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 64 && done[dev]) return hipSuccess;
+  static float lut[256];
+  for (int k = 0; k < 256; ++k) {
+    const double c = k / 255.0;
+    lut[k] = static_cast<float>(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+  }
+  e = hipMemcpyToSymbolAsync(HIP_SYMBOL(gb_srgb_lut), lut, sizeof(lut), 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && dev < 64) done[dev] = true;
+  return e;
+}
+
+althea_status launch_gbuffer_draw_gltf(const althea_gltf_primitive* prims, int prim_count,
+                                       const althea_gltf_material* materials, int material_count,
+                                       const althea_texture_view* textures, int texture_count,
+                                       const althea_globals* g, const althea_gbuffer_view* out, void* workspace,
+                                       hipStream_t s) {
+  std::vector<GbPrim> table(prim_count);
+  int total = 0;
+  for (int k = 0; k < prim_count; ++k) {
+    GbPrim& q = table[k];
+    q = GbPrim{};
+    q.positions = prims[k].positions;
+    q.normals = prims[k].normals;
+    q.tangents = prims[k].tangents;
+    q.bitangents = prims[k].bitangents;
+    for (int u = 0; u < 4; ++u) q.uvs[u] = prims[k].uvs[u];
+    q.indices = prims[k].indices;
+    q.vertex_count = prims[k].vertex_count;
+    q.tri_begin = total;
+    q.triangle_count = prims[k].triangle_count;
+    q.material = prims[k].material;
+    total += prims[k].triangle_count;
+  }
+  GbParams p{};
+  char* ws = gb_common(p, g, out, workspace, total);
+  GbPrim* d_prims = reinterpret_cast<GbPrim*>(ws);
+  ws += align256(static_cast<size_t>(prim_count) * sizeof(GbPrim));
+  althea_gltf_material* d_mat = reinterpret_cast<althea_gltf_material*>(ws);
+  ws += align256(static_cast<size_t>(material_count) * sizeof(althea_gltf_material));
+  althea_texture_view* d_tex = reinterpret_cast<althea_texture_view*>(ws);
+  // Pageable host tables: the copies order after the stream's earlier work.
+  ALTHEA_TRY_HIP(hipMemcpyAsync(d_prims, table.data(), table.size() * sizeof(GbPrim), hipMemcpyHostToDevice, s));
+  ALTHEA_TRY_HIP(hipMemcpyAsync(d_mat, materials, static_cast<size_t>(material_count) * sizeof(althea_gltf_material),
+                                hipMemcpyHostToDevice, s));
+  ALTHEA_TRY_HIP(hipMemcpyAsync(d_tex, textures, static_cast<size_t>(texture_count) * sizeof(althea_texture_view),
+                                hipMemcpyHostToDevice, s));
+  ALTHEA_TRY_HIP(gb_upload_srgb_lut(s));
+  p.prims = d_prims;
+  p.prim_count = prim_count;
+  p.materials = d_mat;
+  p.textures = d_tex;
+  p.alpha_test = 1;
+  return gb_run(p, s);
+}
+
+// Image::generateMipMaps (Src/Image.cpp:135-230): level m from level m - 1 by
+// a linear blit, i.e. the bilinear value (clamped to the edge) at each
+// destination texel centre mapped into the source, in linear space for sRGB
+// (alpha stays linear), rounded to 8 bits.
+__device__ __forceinline__ float gb_linear_to_srgb(float l) {
+  return l <= 0.0031308f ? 12.92f * l : fmaf(1.055f, powf(l, 1.0f / 2.4f), -0.055f);
+}
+
+__device__ __forceinline__ unsigned gb_unorm8(float x) {
+  return static_cast<unsigned>(fminf(fmaxf(floorf(fmaf(x, 255.0f, 0.5f)), 0.0f), 255.0f));
+}
+
+__global__ __launch_bounds__(256) void texture_mip_kernel(uint8_t* __restrict__ texels, size_t src_off, int sw,
+                                                          int sh, size_t dst_off, int dw, int dh, int srgb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dw * dh) return;
+  const int x = i % dw, y = i / dw;
+  const float u = (static_cast<float>(x) + 0.5f) * (static_cast<float>(sw) / static_cast<float>(dw)) - 0.5f;
+  const float v = (static_cast<float>(y) + 0.5f) * (static_cast<float>(sh) / static_cast<float>(dh)) - 0.5f;
+  const float fx = floorf(u), fy = floorf(v);
+  const float ax = u - fx, ay = v - fy;
+  const int i0 = min(max(static_cast<int>(fx), 0), sw - 1), i1 = min(max(static_cast<int>(fx) + 1, 0), sw - 1);
+  const int j0 = min(max(static_cast<int>(fy), 0), sh - 1), j1 = min(max(static_cast<int>(fy) + 1, 0), sh - 1);
+  const uchar4* src = reinterpret_cast<const uchar4*>(texels) + src_off;
+  const uchar4 c[4] = {src[j0 * sw + i0], src[j0 * sw + i1], src[j1 * sw + i0], src[j1 * sw + i1]};
+  float r[4][4];
+  for (int k = 0; k < 4; ++k) {
+    r[k][0] = c[k].x * (1.0f / 255.0f);
+    r[k][1] = c[k].y * (1.0f / 255.0f);
+    r[k][2] = c[k].z * (1.0f / 255.0f);
+    r[k][3] = c[k].w * (1.0f / 255.0f);
+    if (srgb)
+      for (int ch = 0; ch < 3; ++ch) r[k][ch] = gb_srgb_to_linear(r[k][ch]);
+  }
+  unsigned q[4];
+  for (int ch = 0; ch < 4; ++ch) {
+    const float top = fmaf(r[1][ch] - r[0][ch], ax, r[0][ch]), bot = fmaf(r[3][ch] - r[2][ch], ax, r[2][ch]);
+    float val = fmaf(bot - top, ay, top);
+    if (srgb && ch < 3) val = gb_linear_to_srgb(val);
+    q[ch] = gb_unorm8(val);
+  }
+  reinterpret_cast<uchar4*>(texels)[dst_off + i] =
+      make_uchar4(static_cast<unsigned char>(q[0]), static_cast<unsigned char>(q[1]), static_cast<unsigned char>(q[2]),
+                  static_cast<unsigned char>(q[3]));
+}
+
+althea_status launch_texture_mips(uint8_t* texels, int w, int h, int mips, int srgb, hipStream_t s) {
+  size_t off = 0;
+  for (int m = 1; m < mips; ++m) {
+    const int sw = std::max(1, w >> (m - 1)), sh = std::max(1, h >> (m - 1));
+    const int dw = std::max(1, w >> m), dh = std::max(1, h >> m);
+    const size_t dst = off + static_cast<size_t>(sw) * sh;
+    hipLaunchKernelGGL(texture_mip_kernel, dim3((dw * dh + 255) / 256), dim3(256), 0, s, texels, off, sw, sh, dst, dw,
+                       dh, srgb);
+    const althea_status st = check_launch("texture_mip_kernel");
+    if (st != ALTHEA_OK) return st;
+    off = dst;
+  }
+  return ALTHEA_OK;
 }
 
 }  // namespace althea

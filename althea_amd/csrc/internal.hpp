@@ -97,6 +97,14 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
 
 // Point-light shadow cubes (shadow_kernels.hip).
 size_t gbuffer_workspace_bytes(int width, int height, int triangle_count);
+size_t gbuffer_gltf_workspace_bytes(int width, int height, int triangle_count, int primitive_count,
+                                    int material_count, int texture_count);
+althea_status launch_gbuffer_draw_gltf(const althea_gltf_primitive* prims, int prim_count,
+                                       const althea_gltf_material* materials, int material_count,
+                                       const althea_texture_view* textures, int texture_count,
+                                       const althea_globals* g, const althea_gbuffer_view* out, void* workspace,
+                                       hipStream_t s);
+althea_status launch_texture_mips(uint8_t* texels, int w, int h, int mips, int srgb, hipStream_t s);
 althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* g,
                                   const althea_gbuffer_view* out, void* workspace, hipStream_t s);
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,

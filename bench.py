@@ -331,6 +331,53 @@ def gbuffer_frame(torch, aa, scene, ibl, steps: int, fp16: bool = False) -> dict
             "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
 
 
+def gltf_frame(torch, aa, scene, ibl, steps: int) -> dict:
+    """The glTF subpass on the same camera: the heightfield mesh (32 768
+    triangles) as a textured primitive (512^2 sRGB base colour with alpha,
+    512^2 normal map, 256^2 metallic-roughness map, full mip chains, trilinear,
+    texture coordinates repeating 8 times) over a floor primitive, into the
+    4K G-buffer; then the resolve of it."""
+    pos, idx = aa.synth.heightfield_mesh(SEED0 + 1, n=128)
+    nrm, _, _ = aa.synth.mesh_attributes(SEED0 + 1, pos, idx)
+    f = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    tan = np.tile(np.float32([1, 0, 0]), (len(pos), 1))
+    bit = np.tile(np.float32([0, 0, 1]), (len(pos), 1))
+    uv = np.stack([(pos[:, 0] / 20.0 + 0.5) * 8.0, (pos[:, 2] / 20.0 + 0.5) * 8.0], 1).astype(np.float32)
+    rng = np.random.default_rng(SEED0 + 2)
+    base = rng.integers(0, 256, (512, 512, 4), dtype=np.uint8)
+    base[..., 3] = np.where(rng.uniform(size=(512, 512)) < 0.1, 40, 255)
+    nmap = np.concatenate([rng.integers(96, 160, (512, 512, 2)), np.full((512, 512, 2), 255)], -1).astype(np.uint8)
+    mrt = rng.integers(0, 256, (256, 256, 4), dtype=np.uint8)
+    texs = [aa.Texture(f(base), mips=10, srgb=True), aa.Texture(f(nmap), mips=10), aa.Texture(f(mrt), mips=9),
+            aa.Texture(f(np.full((1, 1, 4), 255, np.uint8))), aa.Texture(f(np.array([[[128, 128, 255, 255]]], np.uint8)))]
+    mats = [aa.GltfMaterial(0, 1, 2, base_color_factor=(0.9, 0.85, 0.8, 1.0), metallic_factor=0.6, roughness_factor=0.8),
+            aa.GltfMaterial(3, 4, 3)]
+    terrain = aa.GltfPrimitive(f(pos), f(tan), f(bit), f(nrm), [f(uv)], f(idx.view(np.int32)), 0)
+    fp = np.array([[-30, -2, -30], [30, -2, 30], [30, -2, -30], [-30, -2, 30]], np.float32)
+    floor = aa.GltfPrimitive(f(fp), f(np.tile(np.float32([1, 0, 0]), (4, 1))), f(np.tile(np.float32([0, 0, 1]), (4, 1))),
+                             f(np.tile(np.float32([0, 1, 0]), (4, 1))),
+                             [f(np.array([[0, 0], [1, 1], [1, 0], [0, 1]], np.float32))],
+                             f(np.array([[0, 1, 2], [0, 3, 1]], np.int32)), 1)
+    pas = aa.SceneToGBufferPass(WIDTH, HEIGHT)
+    lights = scene.lights_tensor("cuda")
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    glob, ssao = scene.globals, scene.ssao
+
+    def draw():
+        pas.begin_gltf([terrain, floor], mats, texs, glob)
+
+    def frame():
+        draw()
+        aa.DeferredRendering.resolve(pas.gbuffer, ibl, lights, glob, ssao, out=out)
+
+    frame()
+    draw_ms = time_events(torch, draw, steps)
+    frame_ms = time_events(torch, frame, steps)
+    px = WIDTH * HEIGHT
+    return {"triangles": int(idx.shape[0]) + 2, "draw_ms": draw_ms, "draw_mpix_per_s": px / (draw_ms * 1e-3) / 1e6,
+            "frame_ms": frame_ms, "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
+
+
 def gbuffer_only(torch, aa, args) -> int:
     """Profiling run of the G-buffer producer and its resolve on the 4K frame."""
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
@@ -339,6 +386,7 @@ def gbuffer_only(torch, aa, args) -> int:
     ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
     scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
     r = gbuffer_frame(torch, aa, scene, ibl, args.steps, fp16=args.fp16_gbuffer)
+    r["gltf"] = gltf_frame(torch, aa, scene, ibl, args.steps)
     print(json.dumps({"kind": "gbuffer", **r}), flush=True)
     return 0
 
@@ -476,6 +524,7 @@ def main() -> int:
         line["secondary"]["ssr"] = ssr_frame(torch, aa, scene, ibl, gbuf, lights, args.steps)
         line["secondary"]["shadows"] = shadow_frame(torch, aa, scene, ibl, gbuf, args.steps)
         line["secondary"]["gbuffer"] = gbuffer_frame(torch, aa, scene, ibl, args.steps)
+        line["secondary"]["gbuffer"]["gltf"] = gltf_frame(torch, aa, scene, ibl, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)

@@ -427,6 +427,88 @@ ALTHEA_API althea_status althea_gbuffer_draw(const althea_gbuffer_mesh* mesh, co
                                              const althea_gbuffer_view* gbuffer, void* d_workspace,
                                              hipStream_t stream);
 
+/* glTF material subpass (Shaders/Gltf/Gltf.{vert,frag} with fetchMaterial of
+ * Shaders/InstanceData/InstanceData.glsl:27-63). */
+
+/* Sampler state of a glTF texture (Src/Sampler.cpp:11-114). */
+#define ALTHEA_WRAP_REPEAT 0
+#define ALTHEA_WRAP_MIRRORED_REPEAT 1
+#define ALTHEA_WRAP_CLAMP_TO_EDGE 2
+#define ALTHEA_FILTER_NEAREST 0
+#define ALTHEA_FILTER_LINEAR 1
+
+/* An RGBA8 texture with its mip chain (Src/Texture.cpp:50-100): levels back
+ * to back, level m of max(1, w >> m) x max(1, h >> m) texels
+ * (althea_texture_mip_offset); srgb = 1 for R8G8B8A8_SRGB (decoded to linear
+ * before filtering), 0 for UNORM.  Filtering is the Vulkan LINEAR / NEAREST
+ * rule per axis with the mip level from the screen derivatives of the
+ * texture coordinate, isotropic (the reference enables anisotropic filtering,
+ * whose footprint Vulkan leaves to the implementation). */
+typedef struct althea_texture_view {
+  const uint8_t* texels;
+  int width, height, mips;
+  int srgb;
+  int mag_filter, min_filter, mipmap_mode; /* ALTHEA_FILTER_*; mipmap_mode NEAREST or LINEAR */
+  int wrap_u, wrap_v;                      /* ALTHEA_WRAP_* */
+} althea_texture_view;
+
+ALTHEA_API size_t althea_texture_texels(int width, int height, int mips);
+ALTHEA_API size_t althea_texture_mip_offset(int width, int height, int level);
+/* Image::generateMipMaps (Src/Image.cpp:135-230): level m from level m - 1
+ * by a linear blit to its extent (the bilinear value at each destination
+ * texel centre mapped into the source, clamped to the edge), in linear space
+ * for sRGB textures, rounded to 8 bits.  Level 0 is the caller's. */
+ALTHEA_API althea_status althea_texture_build_mips(uint8_t* d_texels, int width, int height, int mips, int srgb,
+                                                   hipStream_t stream);
+
+/* MaterialConstants as Src/Material.cpp fills them: factors, the texture of
+ * each slot (an index into the texture table; the reference binds its 1x1
+ * white / flat-normal defaults for absent textures, which the caller passes
+ * as textures too) and the texture-coordinate set of the base colour and
+ * metallic-roughness slots (the normal map is read with the base colour's
+ * set, InstanceData.glsl:38-40). */
+typedef struct althea_gltf_material {
+  float base_color_factor[4];
+  float metallic_factor, roughness_factor;
+  float normal_scale;
+  float alpha_cutoff; /* fragments with base colour alpha below it are discarded (0.5 unless set) */
+  int base_texture, normal_texture, metallic_roughness_texture;
+  int base_uv, metallic_roughness_uv; /* 0..3 */
+} althea_gltf_material;
+
+/* One drawn glTF primitive in world space: positions and the TBN frame
+ * after the model transform (Gltf.vert: worldPosition, mat3(model) * tbn),
+ * up to four texture-coordinate sets (float2, NULL when absent), triangles,
+ * and its material. */
+typedef struct althea_gltf_primitive {
+  const float* positions;
+  const float* tangents;
+  const float* bitangents;
+  const float* normals;
+  const float* uvs[4];
+  const uint32_t* indices;
+  int vertex_count, triangle_count;
+  int material;
+} althea_gltf_primitive;
+
+ALTHEA_API size_t althea_gbuffer_gltf_workspace_bytes(int width, int height, const althea_gltf_primitive* primitives,
+                                                      int primitive_count, int material_count, int texture_count);
+
+/* SceneToGBufferPass with the glTF subpass: every primitive drawn in order
+ * into one depth-tested G-buffer (earlier primitives win equal depths).  Per
+ * pixel the nearest front-facing triangle whose fragment survives the alpha
+ * test, then Gltf.frag's outputs: albedo = baseColor (texture x factor),
+ * normal = (normalize(t.x T + t.y B + t.z N), alpha) with t the normal-map
+ * sample mapped to [-1, 1] and scaled by (normal_scale, normal_scale, 1),
+ * MRO = (metallic, roughness, 0, alpha) from the metallic-roughness texture's
+ * blue and green channels times the factors; position (worldPosition, 1).
+ * Tables are host arrays; the texture texels and vertex data device memory. */
+ALTHEA_API althea_status althea_gbuffer_draw_gltf(const althea_gltf_primitive* primitives, int primitive_count,
+                                                  const althea_gltf_material* materials, int material_count,
+                                                  const althea_texture_view* textures, int texture_count,
+                                                  const althea_globals* globals, const althea_gbuffer_view* gbuffer,
+                                                  void* d_workspace, hipStream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Point-light shadow cubes                                                  */
 /* ------------------------------------------------------------------------ */

@@ -33,6 +33,11 @@
 //                       constant attributes; a brute-force ray / triangle
 //                       test per pixel with a different formulation;
 //                       zero normals -> the face normal away from the eye
+//   glTF subpass        constant textures, wrap periodicity / symmetry /
+//                       clamping, a ramp under bilinear filtering, the mip
+//                       level at lambda = 1, 1.5, the nearest-level rule,
+//                       sRGB decode points, 2x2 means of the mip blit; flat
+//                       normal map, factors, MR channels, the alpha cut-off
 //   shadow cubes        empty mesh -> 1; a lit square -> the analytic
 //                       distance; reversed winding, too near -> not written
 //   shadow test         occluded light skipped; the +-Y face mirror (Q19)
@@ -1226,6 +1231,265 @@ inline void gbuffer_pixel(const GbMesh& mesh, const ResolveInputs& cam, int px, 
     out[4 * j + 2] = planes[j].z;
     out[4 * j + 3] = 1.0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// f1b. The glTF subpass: textures as Vulkan samples them (wrap per integer
+// texel coordinate; LINEAR = bilinear at (u w - 0.5, v h - 0.5); NEAREST =
+// texel floor(u w); sRGB decoded before filtering; mip level from
+// lambda = log2 of the larger screen scale factor: lambda <= 0 magnifies with
+// the mag filter at level 0, else the min filter at d = min(lambda, q), the
+// nearest level ceil(d + 0.5) - 1 or the blend of floor(d) and floor(d) + 1),
+// isotropic; mip chains as Image::generateMipMaps blits them (Src/Image.cpp:
+// 135-230); fetchMaterial (InstanceData.glsl:27-63) and Gltf.frag:31-49.
+// `margin` (optional) is lowered to the distance of every discrete decision
+// from its threshold: texel boundaries of NEAREST taps (in texels), the
+// nearest-level rule, the magnification switch when the filters differ.
+// ---------------------------------------------------------------------------
+struct TexView {
+  const uint8_t* texels;
+  int w, h, mips, srgb, mag, min, mipmode, wrap_u, wrap_v;  // ALTHEA_* values
+};
+inline double srgb_to_linear(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
+inline double linear_to_srgb(double l) { return l <= 0.0031308 ? 12.92 * l : 1.055 * std::pow(l, 1.0 / 2.4) - 0.055; }
+inline int tex_wrap(int i, int n, int mode) {
+  if (mode == 2) return std::min(std::max(i, 0), n - 1);  // CLAMP_TO_EDGE
+  if (mode == 1) {                                          // MIRRORED_REPEAT
+    int m = i % (2 * n);
+    if (m < 0) m += 2 * n;
+    return m < n ? m : 2 * n - 1 - m;
+  }
+  int m = i % n;  // REPEAT
+  return m < 0 ? m + n : m;
+}
+inline size_t tex_offset(int w, int h, int level) {
+  size_t off = 0;
+  for (int l = 0; l < level; ++l) off += static_cast<size_t>(std::max(1, w >> l)) * std::max(1, h >> l);
+  return off;
+}
+inline void tex_texel(const TexView& t, size_t off, int w, int i, int j, double out[4]) {
+  const uint8_t* c = t.texels + 4 * (off + static_cast<size_t>(j) * w + i);
+  for (int k = 0; k < 4; ++k) out[k] = c[k] / 255.0;
+  if (t.srgb)
+    for (int k = 0; k < 3; ++k) out[k] = srgb_to_linear(out[k]);
+}
+inline void tex_sample_level(const TexView& t, int level, double u, double v, int filter, double out[4],
+                             double* margin) {
+  const size_t off = tex_offset(t.w, t.h, level);
+  const int w = std::max(1, t.w >> level), h = std::max(1, t.h >> level);
+  if (filter == 0) {
+    const double x = u * w, y = v * h;
+    if (margin) *margin = std::min({*margin, std::fabs(x - std::round(x)), std::fabs(y - std::round(y))});
+    tex_texel(t, off, w, tex_wrap(static_cast<int>(std::floor(x)), w, t.wrap_u),
+              tex_wrap(static_cast<int>(std::floor(y)), h, t.wrap_v), out);
+    return;
+  }
+  const double x = u * w - 0.5, y = v * h - 0.5;
+  const double fx = std::floor(x), fy = std::floor(y), ax = x - fx, ay = y - fy;
+  const int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+  double a[4], b[4], c[4], d[4];
+  tex_texel(t, off, w, tex_wrap(i0, w, t.wrap_u), tex_wrap(j0, h, t.wrap_v), a);
+  tex_texel(t, off, w, tex_wrap(i0 + 1, w, t.wrap_u), tex_wrap(j0, h, t.wrap_v), b);
+  tex_texel(t, off, w, tex_wrap(i0, w, t.wrap_u), tex_wrap(j0 + 1, h, t.wrap_v), c);
+  tex_texel(t, off, w, tex_wrap(i0 + 1, w, t.wrap_u), tex_wrap(j0 + 1, h, t.wrap_v), d);
+  for (int k = 0; k < 4; ++k) {
+    const double top = a[k] + (b[k] - a[k]) * ax, bot = c[k] + (d[k] - c[k]) * ax;
+    out[k] = top + (bot - top) * ay;
+  }
+}
+inline void tex_sample(const TexView& t, double u, double v, double dudx, double dvdx, double dudy, double dvdy,
+                       double out[4], double* margin = nullptr) {
+  const double ax = dudx * t.w, bx = dvdx * t.h, ay = dudy * t.w, by = dvdy * t.h;
+  const double rho2 = std::max(ax * ax + bx * bx, ay * ay + by * by);
+  const double lambda = 0.5 * std::log2(rho2);
+  if (margin && t.mag != t.min) *margin = std::min(*margin, std::fabs(lambda));
+  if (!(lambda > 0.0)) return tex_sample_level(t, 0, u, v, t.mag, out, margin);
+  const int q = t.mips - 1;
+  if (q == 0) return tex_sample_level(t, 0, u, v, t.min, out, margin);
+  const double d = std::min(lambda, static_cast<double>(q));
+  if (t.mipmode == 0) {
+    if (margin && lambda < q) *margin = std::min(*margin, std::fabs(d + 0.5 - std::round(d + 0.5)));
+    const int level = std::min(std::max(static_cast<int>(std::ceil(d + 0.5)) - 1, 0), q);
+    return tex_sample_level(t, level, u, v, t.min, out, margin);
+  }
+  const double fl = std::floor(d);
+  const int l0 = static_cast<int>(fl), l1 = std::min(l0 + 1, q);
+  const double delta = d - fl;
+  double a[4], b[4];
+  tex_sample_level(t, l0, u, v, t.min, a, margin);
+  tex_sample_level(t, l1, u, v, t.min, b, margin);
+  for (int k = 0; k < 4; ++k) out[k] = a[k] + (b[k] - a[k]) * delta;
+}
+// Level m from m - 1: bilinear (clamped) at ((x + 0.5) sw / dw - 0.5,
+// (y + 0.5) sh / dh - 0.5), in linear space for sRGB, rounded to 8 bits.
+inline void texture_mips(uint8_t* texels, int w, int h, int mips, int srgb) {
+  for (int m = 1; m < mips; ++m) {
+    const int sw = std::max(1, w >> (m - 1)), sh = std::max(1, h >> (m - 1));
+    const int dw = std::max(1, w >> m), dh = std::max(1, h >> m);
+    const size_t so = tex_offset(w, h, m - 1), dof = tex_offset(w, h, m);
+    for (int y = 0; y < dh; ++y)
+      for (int x = 0; x < dw; ++x) {
+        const double u = (x + 0.5) * sw / dw - 0.5, v = (y + 0.5) * sh / dh - 0.5;
+        const double fx = std::floor(u), fy = std::floor(v), ax = u - fx, ay = v - fy;
+        const int i0 = std::min(std::max(static_cast<int>(fx), 0), sw - 1);
+        const int i1 = std::min(std::max(static_cast<int>(fx) + 1, 0), sw - 1);
+        const int j0 = std::min(std::max(static_cast<int>(fy), 0), sh - 1);
+        const int j1 = std::min(std::max(static_cast<int>(fy) + 1, 0), sh - 1);
+        for (int ch = 0; ch < 4; ++ch) {
+          auto at = [&](int i, int j) {
+            const double c = texels[4 * (so + static_cast<size_t>(j) * sw + i) + ch] / 255.0;
+            return (srgb && ch < 3) ? srgb_to_linear(c) : c;
+          };
+          const double top = at(i0, j0) + (at(i1, j0) - at(i0, j0)) * ax;
+          const double bot = at(i0, j1) + (at(i1, j1) - at(i0, j1)) * ax;
+          double val = top + (bot - top) * ay;
+          if (srgb && ch < 3) val = linear_to_srgb(val);
+          texels[4 * (dof + static_cast<size_t>(y) * dw + x) + ch] =
+              static_cast<uint8_t>(std::min(std::max(std::floor(val * 255.0 + 0.5), 0.0), 255.0));
+        }
+      }
+  }
+}
+
+struct GltfMaterial {  // althea_gltf_material
+  float base_color_factor[4];
+  float metallic_factor, roughness_factor, normal_scale, alpha_cutoff;
+  int base_texture, normal_texture, metallic_roughness_texture, base_uv, metallic_roughness_uv;
+};
+struct GltfPrim {
+  const float *pos, *tan, *bit, *nrm;
+  const float* uv[4];
+  const uint32_t* idx;
+  int tris, material;
+};
+// One pixel of the glTF subpass over primitives drawn in order; out[16] as
+// gbuffer_pixel; margin adds the alpha test's |alpha - cutoff| and the texture
+// decisions of the winner and of every nearer candidate.
+inline void gltf_pixel(const GltfPrim* prims, int count, const GltfMaterial* mats, const TexView* texs,
+                       const ResolveInputs& cam, int px, int py, double out[16], double* margin, int* tri_out) {
+  const dvec3 d = view_ray(cam, px, py), dx = view_ray(cam, px + 1, py), dy = view_ray(cam, px, py + 1);
+  const double* iv = cam.inv_view.m;
+  const dvec3 eye{iv[12], iv[13], iv[14]};
+  const double* vm = cam.view.m;
+  const dvec3 vz{vm[2], vm[6], vm[10]};
+  const double s = -dot(vz, d);
+  const double near = cam.proj.m[14] / cam.proj.m[10], far = cam.proj.m[14] / (cam.proj.m[10] + 1.0);
+  auto vtx3 = [](const float* a, uint32_t i) { return dvec3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; };
+  struct Hit {
+    double t, l[3], lx[3], ly[3], q[3];
+    bool inside, valid;
+  };
+  auto test = [&](const GltfPrim& pr, int k, Hit& h) {
+    const uint32_t* vi = pr.idx + 3 * k;
+    const dvec3 a = vtx3(pr.pos, vi[0]) - eye, b = vtx3(pr.pos, vi[1]) - eye, c = vtx3(pr.pos, vi[2]) - eye;
+    const dvec3 n = cross(b - a, c - a);
+    const double det = dot(a, n);
+    h.valid = false;
+    if (!(det < 0.0)) return;
+    const dvec3 en[3] = {cross(b, c), cross(c, a), cross(a, b)};
+    const double rn = dot(d, n);
+    if (!(rn < 0.0)) return;
+    h.t = det / rn;
+    const double depth = h.t * s;
+    if (depth < near || depth > far) return;
+    h.valid = true;
+    h.inside = true;
+    double e[3], ex[3], ey[3];
+    for (int j = 0; j < 3; ++j) {
+      e[j] = dot(d, en[j]);
+      ex[j] = dot(dx, en[j]);
+      ey[j] = dot(dy, en[j]);
+      h.q[j] = e[j] / (length(en[j]) * length(d));
+      if (e[j] > 0.0) h.inside = false;
+    }
+    const double se = e[0] + e[1] + e[2], sx = ex[0] + ex[1] + ex[2], sy = ey[0] + ey[1] + ey[2];
+    for (int j = 0; j < 3; ++j) h.l[j] = e[j] / se, h.lx[j] = ex[j] / sx, h.ly[j] = ey[j] / sy;
+  };
+  auto uv_of = [&](const GltfPrim& pr, int k, int set, const Hit& h, double c[6]) {
+    const float* uv = pr.uv[set];
+    for (int j = 0; j < 6; ++j) c[j] = 0.0;
+    if (!uv) return;
+    const uint32_t* vi = pr.idx + 3 * k;
+    double u[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    const double* ws[3] = {h.l, h.lx, h.ly};
+    for (int r = 0; r < 3; ++r)
+      for (int j = 0; j < 3; ++j) {
+        u[r] += ws[r][j] * uv[2 * vi[j]];
+        v[r] += ws[r][j] * uv[2 * vi[j] + 1];
+      }
+    c[0] = u[0], c[1] = v[0], c[2] = u[1] - u[0], c[3] = v[1] - v[0], c[4] = u[2] - u[0], c[5] = v[2] - v[0];
+  };
+  auto alpha_of = [&](const GltfPrim& pr, int k, const Hit& h, double* m) {
+    const GltfMaterial& mt = mats[pr.material];
+    double c[6], o[4];
+    uv_of(pr, k, mt.base_uv, h, c);
+    tex_sample(texs[mt.base_texture], c[0], c[1], c[2], c[3], c[4], c[5], o, m);
+    return o[3] * mt.base_color_factor[3];
+  };
+  // The winner: nearest inside, in range, surviving the alpha test.
+  int best = -1, best_p = -1, best_k = -1, tri_base = 0;
+  double bt = 1e300;
+  Hit win{};
+  {
+    int base = 0;
+    for (int p = 0; p < count; ++p) {
+      for (int k = 0; k < prims[p].tris; ++k) {
+        Hit h;
+        test(prims[p], k, h);
+        if (!h.valid || !h.inside || !(h.t < bt)) continue;
+        if (alpha_of(prims[p], k, h, nullptr) < mats[prims[p].material].alpha_cutoff) continue;
+        bt = h.t, best = base + k, best_p = p, best_k = k, win = h;
+      }
+      base += prims[p].tris;
+    }
+  }
+  (void)tri_base;
+  if (margin) {
+    double m = 1e30;
+    for (int p = 0; p < count; ++p)
+      for (int k = 0; k < prims[p].tris; ++k) {
+        Hit h;
+        test(prims[p], k, h);
+        if (!h.valid) continue;
+        if (best >= 0 && h.t > bt * (1.0 + 1e-5)) continue;
+        const double depth = h.t * s;
+        m = std::min({m, std::fabs(depth - near) / near, std::fabs(depth - far) / far});
+        if (h.inside) {
+          for (int j = 0; j < 3; ++j) m = std::min(m, -h.q[j]);
+          const double a = alpha_of(prims[p], k, h, &m);
+          m = std::min(m, std::fabs(a - mats[prims[p].material].alpha_cutoff));
+          if (!(p == best_p && k == best_k) && best >= 0) m = std::min(m, std::fabs(h.t - bt) / bt);
+        } else {
+          m = std::min(m, std::max({h.q[0], h.q[1], h.q[2]}));
+        }
+      }
+    *margin = m;
+  }
+  if (tri_out) *tri_out = best;
+  for (int j = 0; j < 16; ++j) out[j] = (j % 4 == 3 && j > 3) ? 1.0 : 0.0;
+  if (best < 0) return;
+  const GltfPrim& pr = prims[best_p];
+  const GltfMaterial& mt = mats[pr.material];
+  const uint32_t* vi = pr.idx + 3 * best_k;
+  auto interp = [&](const float* a) {
+    return vtx3(a, vi[0]) * win.l[0] + vtx3(a, vi[1]) * win.l[1] + vtx3(a, vi[2]) * win.l[2];
+  };
+  double cb[6], cm[6], bt4[4], nt[4], mr[4];
+  uv_of(pr, best_k, mt.base_uv, win, cb);
+  uv_of(pr, best_k, mt.metallic_roughness_uv, win, cm);
+  double* mm = margin;
+  tex_sample(texs[mt.base_texture], cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], bt4, mm);
+  tex_sample(texs[mt.normal_texture], cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], nt, mm);
+  tex_sample(texs[mt.metallic_roughness_texture], cm[0], cm[1], cm[2], cm[3], cm[4], cm[5], mr, mm);
+  double base[4];
+  for (int j = 0; j < 4; ++j) base[j] = bt4[j] * mt.base_color_factor[j];
+  const double tx = (2.0 * nt[0] - 1.0) * mt.normal_scale, ty = (2.0 * nt[1] - 1.0) * mt.normal_scale;
+  const double tz = 2.0 * nt[2] - 1.0;
+  const dvec3 n = normalize(interp(pr.tan) * tx + interp(pr.bit) * ty + interp(pr.nrm) * tz);
+  const dvec3 pos = interp(pr.pos);
+  const double planes[16] = {pos.x, pos.y, pos.z, 1.0, n.x, n.y, n.z, base[3], base[0], base[1], base[2], base[3],
+                             mr[2] * mt.metallic_factor, mr[1] * mt.roughness_factor, 0.0, base[3]};
+  for (int j = 0; j < 16; ++j) out[j] = planes[j];
 }
 
 }  // namespace oracle

@@ -355,6 +355,62 @@ void oracle_gbuffer(const float* const* cam, int width, int height, const float*
                   margin ? margin + i : nullptr, tri ? tri + i : nullptr);
 }
 
+// glTF subpass at pixels: prims / mats / texs as the C ABI's structs
+// (althea_gltf_primitive, althea_gltf_material, althea_texture_view with host
+// texel pointers).
+struct OracleGltfPrim {
+  const float *pos, *tan, *bit, *nrm;
+  const float* uv[4];
+  const uint32_t* idx;
+  int vertex_count, tris, material;
+};
+struct OracleTex {
+  const uint8_t* texels;
+  int w, h, mips, srgb, mag, min, mipmode, wrap_u, wrap_v;
+};
+void oracle_gbuffer_gltf(const float* const* cam, int width, int height, const OracleGltfPrim* prims, int count,
+                         const GltfMaterial* mats, const OracleTex* texs, const int* pixels, int n, double* out,
+                         double* margin, int* tri) {
+  ResolveInputs in{};
+  in.W = width;
+  in.H = height;
+  in.proj = dmat4::from_float(cam[0]);
+  in.inv_proj = dmat4::from_float(cam[1]);
+  in.view = dmat4::from_float(cam[2]);
+  in.inv_view = dmat4::from_float(cam[3]);
+  std::vector<GltfPrim> ps(count);
+  for (int k = 0; k < count; ++k)
+    ps[k] = {prims[k].pos, prims[k].tan, prims[k].bit, prims[k].nrm,
+             {prims[k].uv[0], prims[k].uv[1], prims[k].uv[2], prims[k].uv[3]}, prims[k].idx, prims[k].tris,
+             prims[k].material};
+  int nt = 0;
+  for (int k = 0; k < count; ++k) nt = std::max(nt, ps[k].material + 1);
+  int ntex = 0;
+  for (int k = 0; k < nt; ++k)
+    ntex = std::max({ntex, mats[k].base_texture + 1, mats[k].normal_texture + 1, mats[k].metallic_roughness_texture + 1});
+  std::vector<TexView> tv(ntex);
+  for (int k = 0; k < ntex; ++k)
+    tv[k] = {texs[k].texels, texs[k].w, texs[k].h, texs[k].mips, texs[k].srgb, texs[k].mag, texs[k].min,
+             texs[k].mipmode, texs[k].wrap_u, texs[k].wrap_v};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i)
+    gltf_pixel(ps.data(), count, mats, tv.data(), in, pixels[2 * i], pixels[2 * i + 1],
+               out + 16 * static_cast<size_t>(i), margin ? margin + i : nullptr, tri ? tri + i : nullptr);
+}
+
+// Texture sampling at n points (u, v, du/dx, dv/dx, du/dy, dv/dy): out n x 4,
+// margin n.
+void oracle_texture_sample(const OracleTex* t, const double* q, int n, double* out, double* margin) {
+  const TexView tv{t->texels, t->w, t->h, t->mips, t->srgb, t->mag, t->min, t->mipmode, t->wrap_u, t->wrap_v};
+  for (int i = 0; i < n; ++i) {
+    double m = 1e30;
+    tex_sample(tv, q[6 * i], q[6 * i + 1], q[6 * i + 2], q[6 * i + 3], q[6 * i + 4], q[6 * i + 5], out + 4 * i, &m);
+    if (margin) margin[i] = m;
+  }
+}
+
+void oracle_texture_mips(uint8_t* texels, int w, int h, int mips, int srgb) { texture_mips(texels, w, h, mips, srgb); }
+
 // Shadow cubes of `light_count` lights ([n][8] floats) from a mesh:
 // out [light][face][y][x] doubles.
 void oracle_shadow_cubes(const float* positions, const uint32_t* indices, int triangles, const float* lights,

@@ -66,6 +66,9 @@ def lib() -> C.CDLL:
         L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
         L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
         L.oracle_gbuffer.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, vp]
+        L.oracle_gbuffer_gltf.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp, i32, vp, vp, vp]
+        L.oracle_texture_sample.argtypes = [vp, vp, i32, vp, vp]
+        L.oracle_texture_mips.argtypes = [vp, i32, i32, i32, i32]
         L.oracle_shadow_cubes.argtypes = [vp, vp, i32, vp, i32, i32, vp]
         L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
         L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
@@ -314,6 +317,77 @@ def gbuffer(scene, positions, indices, colors, mro, normals=None, pixels=None):
     tri = np.empty(px.shape[0], np.int32)
     lib().oracle_gbuffer(C.cast(cam, vp), scene.width, scene.height, _p(pos), _p(nrm) if nrm is not None else None, _p(col),
                          _p(m), _p(idx), idx.size // 3, _p(px), px.shape[0], _p(out), _p(margin), _p(tri))
+    return out.reshape(-1, 4, 4), margin, tri
+
+
+class OracleTexC(C.Structure):
+    _fields_ = [("texels", vp), ("w", i32), ("h", i32), ("mips", i32), ("srgb", i32), ("mag", i32), ("min", i32),
+                ("mipmode", i32), ("wrap_u", i32), ("wrap_v", i32)]
+
+
+class OracleGltfPrimC(C.Structure):
+    _fields_ = [("pos", vp), ("tan", vp), ("bit", vp), ("nrm", vp), ("uv", vp * 4), ("idx", vp),
+                ("vertex_count", i32), ("tris", i32), ("material", i32)]
+
+
+class OracleTex:
+    """A host RGBA8 mip chain (texels (n, 4) uint8) and sampler state, as
+    the oracle's TexView."""
+
+    def __init__(self, texels, w, h, mips, srgb=0, mag=1, min_=1, mipmode=1, wrap_u=0, wrap_v=0):
+        self.texels = _c(texels, np.uint8)
+        self.c = OracleTexC(_p(self.texels), w, h, mips, srgb, mag, min_, mipmode, wrap_u, wrap_v)
+
+
+def texture_mips(level0, mips: int, srgb: bool) -> np.ndarray:
+    """The whole mip chain (texels (n, 4) uint8) built by the oracle's blit."""
+    h, w = level0.shape[:2]
+    n = sum(max(1, w >> m) * max(1, h >> m) for m in range(mips))
+    out = np.zeros((n, 4), np.uint8)
+    out[:w * h] = level0.reshape(-1, 4)
+    lib().oracle_texture_mips(_p(out), w, h, mips, 1 if srgb else 0)
+    return out
+
+
+def texture_sample(tex: OracleTex, q) -> tuple:
+    """Samples at q (n, 6): u, v, du/dx, dv/dx, du/dy, dv/dy -> (rgba (n, 4),
+    decision margin (n,))."""
+    qq = _c(q, np.float64)
+    out = np.empty((qq.shape[0], 4), np.float64)
+    margin = np.empty(qq.shape[0], np.float64)
+    lib().oracle_texture_sample(C.byref(tex.c), _p(qq), qq.shape[0], _p(out), _p(margin))
+    return out, margin
+
+
+def gbuffer_gltf(scene, prims, mats, texs, pixels=None):
+    """Oracle glTF subpass: prims a list of dicts (pos, tan, bit, nrm, uvs,
+    idx, material) of host arrays, mats a list of GltfMaterialC, texs a list
+    of OracleTex; returns (planes (n, 4, 4), margin, triangle)."""
+    g = _c(scene.globals_arr, np.float32)
+    mats_ = [_c(g[16 * k:16 * k + 16], np.float32) for k in (0, 1, 2, 4)]
+    cam = (vp * 4)(*[_p(m) for m in mats_])
+    keep = []
+    pc = []
+    for pr in prims:
+        arrs = [_c(pr[k], np.float32) for k in ("pos", "tan", "bit", "nrm")]
+        uvs = [_c(u, np.float32) if u is not None else None for u in pr["uvs"]] + [None] * (4 - len(pr["uvs"]))
+        idx = _c(pr["idx"], np.uint32)
+        keep += arrs + uvs + [idx]
+        pc.append(OracleGltfPrimC(*[_p(a) for a in arrs], (vp * 4)(*[_p(u) if u is not None else None for u in uvs]),
+                                  _p(idx), arrs[0].shape[0], idx.size // 3, pr["material"]))
+    prim_arr = (OracleGltfPrimC * len(pc))(*pc)
+    from althea_amd import _lib as L
+    mat_arr = (L.GltfMaterialC * len(mats))(*mats)
+    tex_arr = (OracleTexC * len(texs))(*[t.c for t in texs])
+    if pixels is None:
+        y, x = np.mgrid[0:scene.height, 0:scene.width]
+        pixels = np.stack([x.ravel(), y.ravel()], 1)
+    px = _c(pixels, np.int32)
+    out = np.empty((px.shape[0], 16), np.float64)
+    margin = np.empty(px.shape[0], np.float64)
+    tri = np.empty(px.shape[0], np.int32)
+    lib().oracle_gbuffer_gltf(C.cast(cam, vp), scene.width, scene.height, prim_arr, len(pc), mat_arr, tex_arr,
+                              _p(px), px.shape[0], _p(out), _p(margin), _p(tri))
     return out.reshape(-1, 4, 4), margin, tri
 
 

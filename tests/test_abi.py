@@ -118,6 +118,34 @@ def test_shadow_argument_checks_without_gpu():
                                                 0, 0, dummy, None, None) == 1
 
 
+def test_gltf_and_texture_argument_checks_without_gpu():
+    """Texture and glTF-subpass arguments are checked before any device work."""
+    import ctypes as C
+
+    from althea_amd import _lib
+
+    lib = _lib.lib()
+    assert lib.althea_texture_texels(4, 2, 3) == 4 * 2 + 2 * 1 + 1 * 1
+    assert lib.althea_texture_mip_offset(64, 32, 2) == 64 * 32 + 32 * 16
+    assert lib.althea_texture_build_mips(0x1000, 4, 4, 4, 0, None) == 1  # 4x4 has 3 levels
+    assert b"levels" in lib.althea_last_error()
+    dummy = 0x1000
+    tex = (_lib.TextureViewC * 1)(_lib.TextureViewC(dummy, 4, 4, 3, 0, 1, 1, 1, 0, 0))
+    mat = (_lib.GltfMaterialC * 1)(_lib.GltfMaterialC((C.c_float * 4)(1, 1, 1, 1), 1, 1, 1, 0.5, 0, 0, 1, 0, 0))
+    uv = (C.c_void_p * 4)(dummy, None, None, None)
+    prim = (_lib.GltfPrimitiveC * 1)(_lib.GltfPrimitiveC(dummy, dummy, dummy, dummy, uv, dummy, 3, 1, 0))
+    g = _lib.GBufferViewC(dummy, dummy, dummy, dummy, 64, 32, 0)
+    glob = _lib.GlobalsC()
+    glob.projection[10], glob.projection[14] = -1.00001, -0.01
+    assert lib.althea_gbuffer_draw_gltf(prim, 1, mat, 1, tex, 1, C.byref(glob), C.byref(g), dummy, None) == 1
+    assert b"names a texture outside" in lib.althea_last_error()
+    bad_sampler = (_lib.TextureViewC * 2)(tex[0], _lib.TextureViewC(dummy, 4, 4, 1, 0, 7, 1, 1, 0, 0))
+    assert lib.althea_gbuffer_draw_gltf(prim, 1, mat, 1, bad_sampler, 2, C.byref(glob), C.byref(g), dummy,
+                                        None) == 1
+    assert b"sampler state" in lib.althea_last_error()
+    assert lib.althea_gbuffer_gltf_workspace_bytes(64, 32, prim, 1, 1, 2) > 64 * 32 * 8
+
+
 def test_facade_header_compiles():
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
                     "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "facade_demo.cpp")], check=True)
