@@ -234,17 +234,33 @@ class SceneToGBufferPass {
   void begin(const althea_gbuffer_mesh& mesh, const althea_globals& globals, hipStream_t stream = nullptr) {
     const size_t need = althea_gbuffer_workspace_bytes(_gbuffer.width, _gbuffer.height, mesh.triangle_count);
     if (need == 0) throw std::runtime_error("SceneToGBufferPass: invalid extent or mesh");
-    if (need > _bytes) {
-      void* ws = nullptr;
-      if (hipMalloc(&ws, need) != hipSuccess) throw std::runtime_error("SceneToGBufferPass: hipMalloc failed");
-      _workspace.reset(ws);
-      _bytes = need;
-    }
+    reserve(need);
     throwIfFailed(althea_gbuffer_draw(&mesh, &globals, &_gbuffer, _workspace.get(), stream),
                   "SceneToGBufferPass::begin");
   }
 
+  // The glTF subpass: primitives drawn in order with their materials and
+  // textures (host tables; vertex data and texels in device memory).
+  void beginGltf(const althea_gltf_primitive* primitives, int primitiveCount, const althea_gltf_material* materials,
+                 int materialCount, const althea_texture_view* textures, int textureCount,
+                 const althea_globals& globals, hipStream_t stream = nullptr) {
+    const size_t need = althea_gbuffer_gltf_workspace_bytes(_gbuffer.width, _gbuffer.height, primitives,
+                                                            primitiveCount, materialCount, textureCount);
+    if (need == 0) throw std::runtime_error("SceneToGBufferPass: invalid extent or primitives");
+    reserve(need);
+    throwIfFailed(althea_gbuffer_draw_gltf(primitives, primitiveCount, materials, materialCount, textures,
+                                           textureCount, &globals, &_gbuffer, _workspace.get(), stream),
+                  "SceneToGBufferPass::beginGltf");
+  }
+
  private:
+  void reserve(size_t need) {
+    if (need <= _bytes) return;
+    void* ws = nullptr;
+    if (hipMalloc(&ws, need) != hipSuccess) throw std::runtime_error("SceneToGBufferPass: hipMalloc failed");
+    _workspace.reset(ws);
+    _bytes = need;
+  }
   struct Free {
     void operator()(void* p) const { (void)hipFree(p); }
   };
