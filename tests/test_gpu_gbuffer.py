@@ -129,6 +129,30 @@ def test_overlapping_triangles_zero_normals(torch, seed):
     assert r["err_by_plane"][1] < 1e-3, r
 
 
+@pytest.mark.parametrize("W,H", [(1, 1), (7, 3), (1, 40)])
+def test_small_extents_and_degenerate_triangles(torch, W, H):
+    """Extents down to one pixel; the heightfield mixed with degenerate
+    triangles (collinear vertices, a repeated vertex) that must cover
+    nothing."""
+    sc = aa.synth.scene(seed=SEED0 + 46, width=W, height=H, lights=1)
+    pos, idx = aa.synth.heightfield_mesh(SEED0 + 46, n=16)
+    nrm, col, mro = aa.synth.mesh_attributes(SEED0 + 46, pos, idx)
+    v = len(pos)
+    pos = np.concatenate([pos, np.float32([[0, 1, 0], [1, 1, 0], [2, 1, 0]])])  # exactly collinear
+    nrm, col, mro = (np.concatenate([a, a[:3]]) for a in (nrm, col, mro))
+    extra = np.array([[v, v + 1, v + 2], [5, 5, 9], [3, 3, 3]], np.uint32)
+    idx2 = np.concatenate([extra, idx])
+    got = _draw(torch, sc, (pos, idx2, col, mro, nrm))
+    ref, margin, tri = ol.gbuffer(sc, pos, idx2, col, mro, nrm)
+    r = _compare(got, ref, margin, tri)
+    print("small", W, H, r)
+    assert not np.any((tri >= 0) & (tri < 3))
+    assert r["coverage_disagree_clear"] == 0, r
+    assert r["uncovered_exact"], r
+    if r["covered"]:
+        assert max(r["err_by_plane"]) < 1e-4, r
+
+
 def test_empty_mesh_and_resolve(torch):
     sc = aa.synth.scene(seed=SEED0 + 44, width=200, height=120, lights=4)
     z = np.zeros((0, 3), np.float32)
