@@ -417,8 +417,15 @@ def cpu_baseline(aa, scene, ibl, env) -> dict:
     t0 = time.perf_counter()
     ol.resolve(scene, views, oenv, pix)
     dt = time.perf_counter() - t0
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": pix.shape[0] / dt / 1e6, "unit": "Mpix/s", "cores": ol.max_threads(), "kind": "port",
-            "sample": f"full {WIDTH}x{HEIGHT} frame, {LIGHTS} lights, SSAO 16 (FP64 oracle, {dt:.1f} s)"}
+            "sample": f"full {WIDTH}x{HEIGHT} frame, {LIGHTS} lights, SSAO 16 (FP64 oracle, {dt:.1f} s; "
+                      f"{ol.max_threads()} threads of {os.cpu_count()} on {model})"}
 
 
 def timed_steps(torch, fn, args) -> float:
