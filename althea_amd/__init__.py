@@ -513,13 +513,30 @@ class ScreenSpaceReflection:
     levels (Shaders/SSRGlossyConvolve.comp)."""
 
     @staticmethod
-    def captureReflection(gbuffer, ibl, lights, globals_, reflection: ReflectionBuffer, stream=None):
+    def captureReflection(gbuffer, ibl, lights, globals_, reflection: ReflectionBuffer, shadows=None,
+                          stream=None):
+        """`lights` an (n, 8) device tensor or a PointLightCollection; with
+        `shadows` (a PointLightCollection with shadow maps, or the lights'
+        own collection) the hit shading runs pbrMaterial's shadow test."""
         if not isinstance(gbuffer, GBufferResources):
             gbuffer = GBufferResources(*gbuffer)
+        if isinstance(lights, PointLightCollection):
+            if shadows is None and lights.shadow_storage is not None:
+                shadows = lights
+            lights = lights.lights
         views = ibl.views() if isinstance(ibl, IBLResources) else ibl
         gv = gbuffer.getHandles()
         rv = reflection.view()
         n = int(lights.shape[0]) if lights is not None else 0
+        if shadows is not None:
+            if shadows.shadow_storage is None or shadows.getCount() < n:
+                raise ValueError("shadows: need one shadow cube per light")
+            check(lib().althea_ssr_capture_shadowed(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
+                                                    C.byref(globals_), C.byref(rv), _ptr(shadows.shadow_storage),
+                                                    shadows.shadow_size, _ptr(reflection.workspace),
+                                                    _stream(stream)),
+                  "althea_ssr_capture_shadowed")
+            return reflection
         check(lib().althea_ssr_capture(C.byref(gv), C.byref(views), _ptr(lights) if n else None, n,
                                        C.byref(globals_), C.byref(rv), _ptr(reflection.workspace), _stream(stream)),
               "althea_ssr_capture")

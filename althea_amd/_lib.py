@@ -147,6 +147,8 @@ PROTOTYPES = {
     "althea_ssr_workspace_bytes": (sz, [i32, i32]),
     "althea_ssr_capture": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32, C.POINTER(GlobalsC),
                                  C.POINTER(ReflectionViewC), vp, vp]),
+    "althea_ssr_capture_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
+                                          C.POINTER(GlobalsC), C.POINTER(ReflectionViewC), vp, i32, vp, vp]),
     "althea_reflection_convolve": (i32, [C.POINTER(ReflectionViewC), vp]),
     "althea_deferred_resolve_reflect": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                               C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),

@@ -716,6 +716,24 @@ althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, d_workspace, stream);
 }
 
+althea_status althea_ssr_capture_shadowed(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
+                                          const althea_point_light* d_lights, int light_count,
+                                          const althea_globals* globals, const althea_reflection_view* reflection,
+                                          const float* d_shadow_cubes, int shadow_size, void* d_workspace,
+                                          hipStream_t stream) {
+  althea_status st =
+      resolve_args_ok(g, ibl, d_lights, light_count, globals, nullptr, reflection ? reflection->data : nullptr);
+  if (st != ALTHEA_OK) return st;
+  st = reflection_ok(reflection, g, "ssr_capture_shadowed");
+  if (st != ALTHEA_OK) return st;
+  ALTHEA_REQUIRE(d_workspace, "ssr_capture_shadowed: null workspace");
+  ALTHEA_REQUIRE(d_shadow_cubes && shadow_size >= 1 && shadow_size <= 4096,
+                 "ssr_capture_shadowed: shadow cubes %p, size %d", static_cast<const void*>(d_shadow_cubes),
+                 shadow_size);
+  return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, d_workspace, stream, d_shadow_cubes,
+                            shadow_size);
+}
+
 althea_status althea_reflection_convolve(const althea_reflection_view* reflection, hipStream_t stream) {
   althea_status st = reflection_ok(reflection, nullptr, "reflection_convolve");
   if (st != ALTHEA_OK) return st;

@@ -117,7 +117,8 @@ void refl_chain(const althea_reflection_view& v, ReflChain& c);
 size_t ssr_workspace_bytes(int width, int height);
 althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                  const althea_point_light* lights, int light_count, const althea_globals* globals,
-                                 const althea_reflection_view* reflection, void* workspace, hipStream_t s);
+                                 const althea_reflection_view* reflection, void* workspace, hipStream_t s,
+                                 const float* shadow_cubes = nullptr, int shadow_size = 0);
 althea_status launch_reflection_convolve(const althea_reflection_view* reflection, hipStream_t s);
 
 }  // namespace althea

@@ -36,6 +36,7 @@
 #include "internal.hpp"
 #include "reflection.hpp"
 #include "shading.hpp"
+#include "shadow.hpp"
 
 namespace althea {
 
@@ -65,6 +66,7 @@ struct SsrParams {
   float* depth;                     // depth plane in 8 x 4 texel blocks (taddr)
   int depth_tw;                     // blocks per block row, ceil(W / 8)
   ReflChain out;                    // level 0 is written
+  ShadowCubes shadow;               // shadowed variant only
 };
 
 template <int kFmt>
@@ -175,7 +177,7 @@ __host__ __device__ inline int ssr_band_tiles(int tiles_x, int tiles_y) {
   return ((tiles_x + kXcds - 1) / kXcds) * tiles_y;
 }
 
-template <int kFmt>
+template <int kFmt, bool kShadow = false>
 __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, const float4* __restrict__ lights) {
   const int tiles_x = (p.width + kSsrTile - 1) / kSsrTile, tiles_y = (p.height + kSsrTile - 1) / kSsrTile;
   const int band = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
@@ -279,7 +281,12 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
     lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
     color = pbr_environment(m, irr, refl, A, B, 1.0f);
   }
-  pbr_lights(m, Q, Nh, R, NI, __any(hit), lights, p.light_count, color);
+  // pbrMaterial's light loop, with the shadow test when cubes are given
+  // (PBRMaterial.glsl:119-140 runs it for every caller).
+  if (kShadow)
+    pbr_lights(m, Q, Nh, R, NI, __any(hit), lights, p.light_count, color, p.shadow);
+  else
+    pbr_lights(m, Q, Nh, R, NI, __any(hit), lights, p.light_count, color);
   if (inside) refl_store(p.out, static_cast<uint32_t>(idx), hit ? make_float4(color.x, color.y, color.z, 1.0f) : zero4);
 }
 
@@ -349,7 +356,8 @@ size_t ssr_workspace_bytes(int width, int height) {
 
 althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                  const althea_point_light* lights, int light_count, const althea_globals* globals,
-                                 const althea_reflection_view* reflection, void* workspace, hipStream_t s) {
+                                 const althea_reflection_view* reflection, void* workspace, hipStream_t s,
+                                 const float* shadow_cubes, int shadow_size) {
   SsrParams p{};
   p.position = g->position;
   p.normal = g->normal;
@@ -389,12 +397,21 @@ althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   const int tiles_x = (g->width + kSsrTile - 1) / kSsrTile, tiles_y = (g->height + kSsrTile - 1) / kSsrTile;
   const dim3 grid(kXcds * ssr_band_tiles(tiles_x, tiles_y));
   const float4* lights4 = reinterpret_cast<const float4*>(lights);
+  p.shadow.cubes = shadow_cubes;
+  p.shadow.size = shadow_size;
+  const bool sh = shadow_cubes != nullptr;
   if (g->format == ALTHEA_GBUFFER_FP32) {
     hipLaunchKernelGGL((ssr_depth_kernel<ALTHEA_GBUFFER_FP32>), dgrid, dim3(256), 0, s, p);
-    hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP32>), grid, dim3(256), 0, s, p, lights4);
+    if (sh)
+      hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);
+    else
+      hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP32>), grid, dim3(256), 0, s, p, lights4);
   } else {
     hipLaunchKernelGGL((ssr_depth_kernel<ALTHEA_GBUFFER_FP16>), dgrid, dim3(256), 0, s, p);
-    hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP16>), grid, dim3(256), 0, s, p, lights4);
+    if (sh)
+      hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p, lights4);
+    else
+      hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP16>), grid, dim3(256), 0, s, p, lights4);
   }
   return check_launch("ssr_capture_kernel");
 }

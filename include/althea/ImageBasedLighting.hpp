@@ -328,6 +328,24 @@ class PointLightCollection {
   int _shadowSize = 0;
 };
 
+namespace ScreenSpaceReflection {
+
+// captureReflection with the lights of `lights` and, in the hit shading,
+// their shadow test (pbrMaterial runs it for every caller).
+inline void captureReflection(const GBufferResources& gbuffer, const IBLResources& ibl,
+                              const PointLightCollection& lights, const althea_globals& globals,
+                              ReflectionBuffer& reflection, hipStream_t stream = nullptr) {
+  if (!lights.shadowMaps()) throw std::runtime_error("ScreenSpaceReflection: lights without shadow maps");
+  althea_ibl_views v = ibl.views();
+  throwIfFailed(althea_ssr_capture_shadowed(&gbuffer.view(), &v, lights.lights(),
+                                            static_cast<int>(lights.getCount()), &globals, &reflection.view(),
+                                            lights.shadowMaps(), lights.getShadowMapExtent(), reflection.workspace(),
+                                            stream),
+                "ScreenSpaceReflection::captureReflection");
+}
+
+}  // namespace ScreenSpaceReflection
+
 namespace DeferredRendering {
 
 // The deferred lighting resolve of DeferredPass.frag (one frame).

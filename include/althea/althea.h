@@ -368,6 +368,17 @@ ALTHEA_API althea_status althea_ssr_capture(const althea_gbuffer_view* gbuffer, 
                                             const althea_globals* globals, const althea_reflection_view* reflection,
                                             void* d_workspace, hipStream_t stream);
 
+/* althea_ssr_capture with the point-light shadow test in the hit shading: the
+ * reference's pbrMaterial reads the shadow cubes for every caller
+ * (PBRMaterial.glsl:119-140), SSR.frag's hit shading included.
+ * d_shadow_cubes: light_count cubes as althea_shadow_cubes writes them. */
+ALTHEA_API althea_status althea_ssr_capture_shadowed(const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl,
+                                                     const althea_point_light* d_lights, int light_count,
+                                                     const althea_globals* globals,
+                                                     const althea_reflection_view* reflection,
+                                                     const float* d_shadow_cubes, int shadow_size, void* d_workspace,
+                                                     hipStream_t stream);
+
 /* Levels 1 .. mips-1 of the reflection buffer:
  * ReflectionBuffer::convolveReflectionBuffer (Src/ReflectionBuffer.cpp:163-277)
  * running Shaders/SSRGlossyConvolve.comp per level.  Level m samples level

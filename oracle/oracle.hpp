@@ -1037,7 +1037,7 @@ inline void ssr_pixel(const ResolveInputs& in, int px, int py, double out[4], do
         dvec3 refl = cube_sample(in.prefiltered, Rr, mr[1] * (in.prefiltered.mips - 1));
         dvec3 irr = cube_sample(in.irradiance, Nh, 0.0);
         dvec3 col = pbr_material(Q, V, Nh, {al[0], al[1], al[2]}, refl, irr, mr[0], mr[1], 1.0, in.lut, in.lights,
-                                 in.light_count);
+                                 in.light_count, in.shadows, &m);
         out[0] = col.x; out[1] = col.y; out[2] = col.z; out[3] = 1.0;
         if (hit_step) *hit_step = i;
         if (hit_uv) { hit_uv[0] = u; hit_uv[1] = v; }

@@ -402,11 +402,11 @@ def shadow_cubes(positions, indices, lights, size: int) -> np.ndarray:
     return out
 
 
-def ssr(scene, views: dict, pixels, gbuffer=None):
+def ssr(scene, views: dict, pixels, gbuffer=None, shadows=None):
     """Oracle screen-space reflection of pixels[:, (x, y)]: (rgba, decision
     margin, hit step (-1 miss, -2 uncovered), hit uv, hit position)."""
     keep = []
-    a = _resolve_args(scene, views, None, False, gbuffer, None, keep)
+    a = _resolve_args(scene, views, None, False, gbuffer, None, keep, shadows)
     px = _c(pixels, np.int32)
     n = px.shape[0]
     rgba = np.empty((n, 4), np.float64)

@@ -223,3 +223,34 @@ def test_resolve_shadowed_with_reflection_buffer(torch, ibl):
     print("resolve shadowed + reflection", s, "clear", clear.mean())
     assert clear.mean() > 0.6
     assert s["mean"] < 1e-5 and s["max"] < 1e-3, s
+
+
+def test_ssr_capture_shadowed_matches_oracle(torch, ibl):
+    """The SSR hit shading with the shadow test (pbrMaterial reads the cubes
+    for SSR.frag too): against the oracle with the same cubes; cubes of 1 give
+    the unshadowed capture."""
+    ibl_res, _ = ibl
+    sc, coll, cubes = _shadow_scene(torch, 320, 180, 8, SEED0 + 36)
+    g = sc.gbuffer_tensors("cuda")
+    refl = aa.ReflectionBuffer(sc.width, sc.height, mips=1, fp16=False)
+    aa.ScreenSpaceReflection.captureReflection(g, ibl_res, coll, sc.globals, refl)
+    torch.cuda.synchronize()
+    got = refl.level(0).cpu().numpy().reshape(-1, 4).astype(np.float64)
+    pix = np.stack(np.meshgrid(np.arange(sc.width), np.arange(sc.height)), -1).reshape(-1, 2).astype(np.int32)
+    rgba, margin, step, _, _ = ol.ssr(sc, ibl_res.views_numpy(), pix, shadows=cubes)
+    plain, _, _, _, _ = ol.ssr(sc, ibl_res.views_numpy(), pix)
+    hit = (step >= 0) & (got[:, 3] == 1.0) & (margin > 2e-5)
+    err = rel_err(got[hit, :3], rgba[hit, :3], RADIANCE_FLOOR).max(1)
+    shadowed = np.abs(rgba[hit, :3] - plain[hit, :3]).max(1) > 1e-6
+    s = summary(err)
+    print("ssr shadowed", s, "hits", int(hit.sum()), "shadowed hits", int(shadowed.sum()))
+    assert hit.sum() > 300 and shadowed.sum() > 30
+    assert s["mean"] < 5e-5 and s["max"] < 2e-3, s
+    ones = aa.PointLightCollection.from_lights(sc.lights_tensor("cuda"), shadow_size=8)
+    r1 = aa.ReflectionBuffer(sc.width, sc.height, mips=1, fp16=False)
+    r0 = aa.ReflectionBuffer(sc.width, sc.height, mips=1, fp16=False)
+    aa.ScreenSpaceReflection.captureReflection(g, ibl_res, ones, sc.globals, r1)
+    aa.ScreenSpaceReflection.captureReflection(g, ibl_res, sc.lights_tensor("cuda"), sc.globals, r0)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(r1.level(0).cpu().numpy(), r0.level(0).cpu().numpy(), rtol=2e-6, atol=0)
+
