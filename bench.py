@@ -378,6 +378,37 @@ def gltf_frame(torch, aa, scene, ibl, steps: int) -> dict:
             "frame_ms": frame_ms, "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
 
 
+def full_frame(torch, aa, scene, ibl, steps: int) -> dict:
+    """Every pass of the path on one 4K frame, as an engine runs them: the
+    G-buffer from the 32 768-triangle heightfield mesh (pass-through), the
+    shadow cubes of the 64 lights (256^2) from the same mesh, the SSR capture
+    with shadowed hit shading into a 5-level FP16 buffer, its glossy
+    convolution, and the resolve with shadows and the reflection buffer."""
+    pos, idx = aa.synth.heightfield_mesh(SEED0 + 1, n=128)
+    nrm, col, mro = aa.synth.mesh_attributes(SEED0 + 1, pos, idx)
+    f = lambda a: torch.from_numpy(a).cuda()
+    mesh = aa.GBufferMesh(f(pos), f(idx.view(np.int32)), f(col), f(mro), f(nrm))
+    pas = aa.SceneToGBufferPass(WIDTH, HEIGHT)
+    coll = aa.PointLightCollection.from_lights(scene.lights_tensor("cuda"), shadow_size=SHADOW_SIZE)
+    refl = aa.ReflectionBuffer(WIDTH, HEIGHT, mips=5, fp16=True)
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    glob, ssao = scene.globals, scene.ssao
+    pos_t, idx_t = mesh.positions, mesh.indices
+
+    def frame():
+        g = pas.begin(mesh, glob)
+        coll.drawShadowMaps(pos_t, idx_t)
+        aa.ScreenSpaceReflection.captureReflection(g, ibl, coll, glob, refl)
+        aa.ScreenSpaceReflection.convolveReflectionBuffer(refl)
+        aa.DeferredRendering.resolve(g, ibl, coll, glob, ssao, out=out, reflection=refl)
+
+    frame()
+    ms = time_events(torch, frame, steps)
+    px = WIDTH * HEIGHT
+    return {"passes": "gbuffer + shadow cubes + ssr (shadowed) + convolve + resolve (shadowed, reflections)",
+            "ms": ms, "mpix_per_s": px / (ms * 1e-3) / 1e6}
+
+
 def gbuffer_only(torch, aa, args) -> int:
     """Profiling run of the G-buffer producer and its resolve on the 4K frame."""
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
@@ -387,6 +418,7 @@ def gbuffer_only(torch, aa, args) -> int:
     scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
     r = gbuffer_frame(torch, aa, scene, ibl, args.steps, fp16=args.fp16_gbuffer)
     r["gltf"] = gltf_frame(torch, aa, scene, ibl, args.steps)
+    r["full_frame"] = full_frame(torch, aa, scene, ibl, args.steps)
     print(json.dumps({"kind": "gbuffer", **r}), flush=True)
     return 0
 
@@ -532,6 +564,7 @@ def main() -> int:
         line["secondary"]["shadows"] = shadow_frame(torch, aa, scene, ibl, gbuf, args.steps)
         line["secondary"]["gbuffer"] = gbuffer_frame(torch, aa, scene, ibl, args.steps)
         line["secondary"]["gbuffer"]["gltf"] = gltf_frame(torch, aa, scene, ibl, args.steps)
+        line["secondary"]["full_frame"] = full_frame(torch, aa, scene, ibl, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)
