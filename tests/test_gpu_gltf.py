@@ -106,7 +106,7 @@ def _materials():
     return [mat(f=(0.9, 0.8, 0.7, 1.0), mf=0.7, rf=0.9, ns=1.5, mu=1), mat(bt=3, nt=4, mt=4)]
 
 
-def _gpu(torch, sc, prims, mats, levels):
+def _gpu(torch, sc, prims, mats, levels, fp16=False):
     f = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     texs = [aa.Texture(f(levels[k]), mips=SPECS[k]["mips"], srgb=bool(SPECS[k]["srgb"]), mag_filter=SPECS[k]["mag"],
                        min_filter=SPECS[k]["min_"], mipmap_mode=SPECS[k]["mipmode"], wrap_u=SPECS[k]["wrap_u"],
@@ -120,10 +120,10 @@ def _gpu(torch, sc, prims, mats, levels):
         o = aa.GltfMaterial.__new__(aa.GltfMaterial)
         o.c = m
         mobj.append(o)
-    pas = aa.SceneToGBufferPass(sc.width, sc.height)
+    pas = aa.SceneToGBufferPass(sc.width, sc.height, fp16=fp16)
     g = pas.begin_gltf(gp, mobj, texs, sc.globals)
     torch.cuda.synchronize()
-    planes = np.stack([p.cpu().numpy().reshape(-1, 4) for p in g.planes], 1).astype(np.float64)
+    planes = np.stack([p.float().cpu().numpy().reshape(-1, 4) for p in g.planes], 1).astype(np.float64)
     chains = [t.data.cpu().numpy() for t in texs]
     return planes, chains
 
@@ -166,3 +166,13 @@ def test_invalid_material_texture_index(torch):
     mats[0].normal_texture = 9
     with pytest.raises(aa.AltheaError, match="names a texture outside"):
         _gpu(torch, sc, prims, mats, levels)
+
+
+def test_fp16_planes_are_the_rounded_fp32_planes(torch):
+    sc, prims = _scene(torch, SEED0 + 55)
+    levels = _levels0(SEED0 + 56)
+    mats = _materials()
+    f32, _ = _gpu(torch, sc, prims, mats, levels)
+    f16, _ = _gpu(torch, sc, prims, mats, levels, fp16=True)
+    np.testing.assert_array_equal(f16, f32.astype(np.float32).astype(np.float16).astype(np.float64))
+
