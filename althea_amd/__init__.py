@@ -566,6 +566,7 @@ class PointLightCollection:
         P = shadow_size + 2
         self.shadow_storage = (torch.ones((light_count, 6, P, P), dtype=torch.float32, device="cuda")
                                if create_shadow_map else None)
+        self.workspace = None
 
     @property
     def shadow_maps(self):
@@ -591,7 +592,10 @@ class PointLightCollection:
     def getShadowMapImage(self):
         return self.shadow_maps
 
-    def drawShadowMaps(self, positions, indices, stream=None):
+    def drawShadowMaps(self, positions, indices, scheduled: bool = True, stream=None):
+        """The cubes of every light from a mesh.  `scheduled` (default) uses
+        the workspace path that deals texels out evenly; False the
+        workspace-free one (the same cubes)."""
         if self.shadow_maps is None:
             raise ValueError("PointLightCollection was created without shadow maps")
         torch = _torch()
@@ -601,9 +605,17 @@ class PointLightCollection:
         mesh = _lib.MeshViewC(_ptr(positions) if tris else None, _ptr(indices) if tris else None,
                               int(positions.numel() // 3), tris)
         assert self.shadow_storage[0].numel() == lib().althea_shadow_cube_floats(self.shadow_size)
-        check(lib().althea_shadow_cubes(C.byref(mesh), _ptr(self.lights), self.getCount(), self.shadow_size,
-                                        _ptr(self.shadow_storage), _stream(stream)),
-              "althea_shadow_cubes")
+        if not scheduled:
+            check(lib().althea_shadow_cubes(C.byref(mesh), _ptr(self.lights), self.getCount(), self.shadow_size,
+                                            _ptr(self.shadow_storage), _stream(stream)),
+                  "althea_shadow_cubes")
+            return self.shadow_maps
+        need = lib().althea_shadow_cubes_workspace_bytes(self.getCount(), tris)
+        if self.workspace is None or self.workspace.numel() < need:
+            self.workspace = torch.empty(max(need, 256), dtype=torch.uint8, device="cuda")
+        check(lib().althea_shadow_cubes_ws(C.byref(mesh), _ptr(self.lights), self.getCount(), self.shadow_size,
+                                           _ptr(self.shadow_storage), _ptr(self.workspace), _stream(stream)),
+              "althea_shadow_cubes_ws")
         return self.shadow_maps
 
 

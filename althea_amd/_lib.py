@@ -163,6 +163,8 @@ PROTOTYPES = {
                                        C.POINTER(TextureViewC), i32, C.POINTER(GlobalsC), C.POINTER(GBufferViewC), vp,
                                        vp]),
     "althea_shadow_cube_floats": (sz, [i32]),
+    "althea_shadow_cubes_workspace_bytes": (sz, [i32, i32]),
+    "althea_shadow_cubes_ws": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp, vp]),
     "althea_shadow_cubes": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp]),
     "althea_deferred_resolve_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                                C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),

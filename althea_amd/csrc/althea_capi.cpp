@@ -685,6 +685,26 @@ althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_poi
   return launch_shadow_cubes(mesh, d_lights, light_count, size, d_cubes, stream);
 }
 
+size_t althea_shadow_cubes_workspace_bytes(int light_count, int triangle_count) {
+  if (light_count < 1 || triangle_count < 0 || static_cast<long long>(light_count) * triangle_count >= (1ll << 31))
+    return 0;
+  return shadow_workspace_bytes(light_count, triangle_count);
+}
+
+althea_status althea_shadow_cubes_ws(const althea_mesh_view* mesh, const althea_point_light* d_lights,
+                                     int light_count, int size, float* d_cubes, void* d_workspace,
+                                     hipStream_t stream) {
+  ALTHEA_REQUIRE(mesh && d_cubes && light_count >= 1 && d_lights && d_workspace,
+                 "shadow_cubes_ws: null argument or no light");
+  ALTHEA_REQUIRE(size >= 1 && size <= 4096, "shadow_cubes_ws: size %d", size);
+  ALTHEA_REQUIRE(mesh->triangle_count >= 0 && (mesh->triangle_count == 0 || (mesh->positions && mesh->indices &&
+                                                                               mesh->vertex_count >= 1)),
+                 "shadow_cubes_ws: mesh with %d triangles, %d vertices", mesh->triangle_count, mesh->vertex_count);
+  ALTHEA_REQUIRE(static_cast<long long>(light_count) * mesh->triangle_count < (1ll << 31),
+                 "shadow_cubes_ws: %d lights x %d triangles", light_count, mesh->triangle_count);
+  return launch_shadow_cubes(mesh, d_lights, light_count, size, d_cubes, stream, d_workspace);
+}
+
 althea_status althea_deferred_resolve_shadowed(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                                const althea_point_light* d_lights, int light_count,
                                                const althea_globals* globals, const althea_ssao_desc* ssao,

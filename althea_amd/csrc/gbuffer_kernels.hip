@@ -49,12 +49,13 @@
 
 #include "device_math.hpp"
 #include "internal.hpp"
+#include "slot_scan.hpp"
 
 namespace althea {
 
 constexpr unsigned long long kGbEmpty = ~0ull;
-constexpr int kGbScanThreads = 1024;
-constexpr int kGbSetupThreads = 256;
+constexpr int kGbScanThreads = kSlotScan;
+constexpr int kGbSetupThreads = kSlotBlock;
 constexpr int kGbRasterBlocks = 4096;
 
 // Per-triangle setup, 64 bytes.
@@ -211,19 +212,10 @@ __global__ __launch_bounds__(kGbSetupThreads) void gbuffer_setup_kernel(const Gb
     gbuffer_setup_triangle(p, t, tri, area);
     p.tris[t] = tri;
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned long long x = area;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const unsigned long long y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  if (lane == 63) wave_total[wave] = x;
-  __syncthreads();
-  unsigned long long before = 0;
-  for (int k = 0; k < wave; ++k) before += wave_total[k];
-  if (t < p.triangle_count) p.offsets[t] = before + x - area;
-  if (threadIdx.x == kGbSetupThreads - 1) p.blocks[blockIdx.x] = before + x;
+  unsigned long long block_total;
+  const unsigned long long excl = slot_block_scan(area, wave_total, block_total);
+  if (t < p.triangle_count) p.offsets[t] = excl;
+  if (threadIdx.x == 0) p.blocks[blockIdx.x] = block_total;
 }
 
 // Exclusive prefix sum of the n block totals in place, in one block (each
@@ -231,26 +223,7 @@ __global__ __launch_bounds__(kGbSetupThreads) void gbuffer_setup_kernel(const Gb
 __global__ __launch_bounds__(kGbScanThreads) void gbuffer_scan_kernel(unsigned long long* __restrict__ blocks, int n,
                                                                       unsigned long long* __restrict__ total) {
   __shared__ unsigned long long sums[kGbScanThreads];
-  const int tid = threadIdx.x;
-  const int run = (n + kGbScanThreads - 1) / kGbScanThreads;
-  const int b = min(tid * run, n), e = min(b + run, n);
-  unsigned long long s = 0;
-  for (int i = b; i < e; ++i) s += blocks[i];
-  sums[tid] = s;
-  __syncthreads();
-  for (int d = 1; d < kGbScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
-    const unsigned long long add = tid >= d ? sums[tid - d] : 0ull;
-    __syncthreads();
-    sums[tid] += add;
-    __syncthreads();
-  }
-  unsigned long long acc = sums[tid] - s;
-  for (int i = b; i < e; ++i) {
-    const unsigned long long v = blocks[i];
-    blocks[i] = acc;
-    acc += v;
-  }
-  if (tid == kGbScanThreads - 1) *total = sums[tid];
+  slot_scan_blocks(blocks, n, total, sums);
 }
 
 __global__ __launch_bounds__(256) void gbuffer_offset_kernel(unsigned long long* __restrict__ offsets,
@@ -416,58 +389,23 @@ __device__ float gb_base_alpha(const GbParams& p, const GbPrim& pr, const GbTri&
   return gb_sample<true>(p.textures[m.base_texture], c).w * m.base_color_factor[3];
 }
 
-// Largest t in [lo, hi) with offsets[t] <= slot, given offsets[lo] <= slot <
-// offsets[hi].
-__device__ __forceinline__ int gb_find(const unsigned long long* __restrict__ offsets, int lo, int hi,
-                                       unsigned long long slot) {
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (offsets[mid] <= slot)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ unsigned long long gb_uniform(unsigned long long v) {
-  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
-  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
-  return (static_cast<unsigned long long>(hi) << 32) | lo;
-}
-
-// Largest t >= lo with offsets[t] <= slot (offsets[lo] <= slot < offsets[T]):
-// probes lo + 1, lo + 3, lo + 7, ... then a binary search in the last step.
-__device__ __forceinline__ int gb_gallop(const unsigned long long* __restrict__ offsets, int lo, int T,
-                                         unsigned long long slot) {
-  int step = 1;
-  while (lo + step < T && offsets[lo + step] <= slot) {
-    lo += step;
-    step <<= 1;
-  }
-  return gb_find(offsets, lo, min(lo + step, T), slot);
-}
-
 __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
   const unsigned long long* __restrict__ offsets = p.offsets;
   const int T = p.triangle_count;
   const unsigned long long total = offsets[T];
   const int lane = threadIdx.x & 63;
-  const unsigned waves = gridDim.x * (blockDim.x >> 6);
-  const unsigned wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   // Each wave takes an equal contiguous run of slots, 64 per iteration.
-  const unsigned long long per = ((total + waves - 1) / waves + 63) & ~63ull;
-  const unsigned long long begin = gb_uniform(wave * per);
-  const unsigned long long end = begin + per < total ? begin + per : total;
+  unsigned long long begin, end;
+  slot_range(total, begin, end);
   if (begin >= end) return;
-  int tf = gb_find(offsets, 0, T, begin);
+  int tf = slot_find(offsets, 0, T, begin);
   for (unsigned long long base = begin; base < end; base += 64) {
     const unsigned long long last = base + 63 < end ? base + 63 : end - 1;
-    tf = gb_gallop(offsets, tf, T, base);
-    const int tl = gb_gallop(offsets, tf, T, last);
+    tf = slot_gallop(offsets, tf, T, base);
+    const int tl = slot_gallop(offsets, tf, T, last);
     const unsigned long long slot = base + lane;
     if (slot > last) continue;
-    const int lo = gb_find(offsets, tf, tl + 1, slot);
+    const int lo = slot_find(offsets, tf, tl + 1, slot);
     const GbTri tri = p.tris[lo];
     const unsigned local = static_cast<unsigned>(slot - offsets[lo]);
     const int x = tri.x0 + static_cast<int>(local % static_cast<unsigned>(tri.w));

@@ -107,8 +107,10 @@ althea_status launch_gbuffer_draw_gltf(const althea_gltf_primitive* prims, int p
 althea_status launch_texture_mips(uint8_t* texels, int w, int h, int mips, int srgb, hipStream_t s);
 althea_status launch_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* g,
                                   const althea_gbuffer_view* out, void* workspace, hipStream_t s);
+size_t shadow_workspace_bytes(int light_count, int triangle_count);
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
-                                  int size, float* cubes, hipStream_t s);
+                                  int size, float* cubes, hipStream_t s,
+                                  void* workspace = nullptr);
 
 // Screen-space reflection (ssr_kernels.hip): level 0 of the reflection
 // buffer, then the glossy convolution of levels 1 .. mips-1.

@@ -37,11 +37,24 @@
 #include "device_math.hpp"
 #include "internal.hpp"
 #include "shadow.hpp"
+#include "slot_scan.hpp"
 
 namespace althea {
 
 constexpr int kShadowLaneTexels = 16;  // boxes up to this many texels stay on their lane
 constexpr float kShadowClip = 0.5f * ALTHEA_SHADOW_NEAR;  // clip plane ma = near / 2
+// Box margin in texels.  The FP32 projection of a vertex is off by a few ulp
+// of a coordinate up to S + 1 (under 2e-3 texel at S = 4096), a texel-centre
+// ray by about 2e-4 texel and a non-grazing edge test by about 1e-7 rad (2e-4
+// texel): a texel the edge tests accept lies well within 1/64 texel of the
+// exact span.
+constexpr float kShadowBoxMargin = 1.0f / 64.0f;
+
+struct ShadowSlots {
+  unsigned long long* offsets;  // items + 1 slot offsets
+  uint2* boxes;                 // per item and face: x0 | y0 << 16, w | h << 16
+  unsigned long long* blocks;   // per setup block: slot total, then first slot
+};
 
 struct ShadowParams {
   const float* positions;
@@ -52,6 +65,7 @@ struct ShadowParams {
   int light_count;
   int size;
   float* cubes;
+  ShadowSlots slots;  // the scheduled path only
 };
 
 __global__ void shadow_clear_kernel(float* __restrict__ c, size_t n) {
@@ -88,69 +102,87 @@ __device__ __forceinline__ void shadow_texel(const ShadowTri& tri, int face, int
   atomicMin(reinterpret_cast<unsigned int*>(cube) + shadow_texel_index(S, face, i, j), __float_as_uint(depth));
 }
 
+// Light-relative vertices and edge functions of item = light * T + t; live is
+// false for items past the end and for back-facing or edge-on triangles.
+__device__ __forceinline__ void shadow_item(const ShadowParams& p, int item, int& light, f3 v[3], ShadowTri& tri,
+                                            bool& live) {
+  light = item / max(p.triangle_count, 1);
+  const int t = item - light * p.triangle_count;
+  live = item < p.light_count * p.triangle_count;
+  tri = ShadowTri{};
+  v[0] = v[1] = v[2] = mk(0.0f, 0.0f, 0.0f);
+  if (!live) return;
+  const float4 lp = p.lights[2 * light];
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t vi = p.indices[3 * t + k];
+    const float* q = p.positions + 3 * static_cast<size_t>(min(vi, static_cast<uint32_t>(p.vertex_count - 1)));
+    v[k] = mk(q[0] - lp.x, q[1] - lp.y, q[2] - lp.z);
+  }
+  // Edge functions from the edges e1 = b - a, e2 = c - a (c x a = e2 x a,
+  // a x b = a x e1, b x c = n - both): rounding scales with |a| |e|, not
+  // |a| |b|.
+  const f3 e1 = v[1] - v[0], e2 = v[2] - v[0];
+  tri.n = cross(e1, e2);
+  tri.n1 = cross(e2, v[0]);
+  tri.n2 = cross(v[0], e1);
+  tri.n0 = tri.n - tri.n1 - tri.n2;
+  tri.det = dot(v[0], tri.n);
+  live = tri.det < 0.0f;  // back faces and edge-on triangles are culled
+}
+
+// Texel box [x0, x1] x [y0, y1] of the item on `face` (empty: x1 < x0).
+__device__ __forceinline__ void shadow_face_box(int face, const f3 v[3], bool live, int S, int& x0, int& x1, int& y0,
+                                                int& y1) {
+  const float half = 0.5f * static_cast<float>(S);
+  x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+  if (live) {
+    float sc[3], tc[3], ma[3];
+    bool in[3];
+    for (int k = 0; k < 3; ++k) {
+      shadow_face_coords(face, v[k], sc[k], tc[k], ma[k]);
+      in[k] = ma[k] >= kShadowClip;
+    }
+    if (in[0] || in[1] || in[2]) {
+      float lo_x = 1e30f, hi_x = -1e30f, lo_y = 1e30f, hi_y = -1e30f;
+      auto add = [&](float s_, float t_, float m_) {
+        const float r = 1.0f / m_;
+        const float x = fmaf(s_ * r, half, half - 0.5f), y = fmaf(t_ * r, half, half - 0.5f);
+        lo_x = fminf(lo_x, x), hi_x = fmaxf(hi_x, x), lo_y = fminf(lo_y, y), hi_y = fmaxf(hi_y, y);
+      };
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int k2 = k == 2 ? 0 : k + 1;
+        if (in[k]) add(sc[k], tc[k], ma[k]);
+        if (in[k] != in[k2]) {  // the edge crosses the clip plane
+          const float u = (kShadowClip - ma[k]) / (ma[k2] - ma[k]);
+          add(fmaf(u, sc[k2] - sc[k], sc[k]), fmaf(u, tc[k2] - tc[k], tc[k]), kShadowClip);
+        }
+      }
+      // Texel centres inside the span widened by kShadowBoxMargin
+      // (coordinates clamped to [-2, S + 1] before the integer conversion).
+      const float lim = static_cast<float>(S + 1);
+      x0 = max(static_cast<int>(ceilf(fminf(fmaxf(lo_x - kShadowBoxMargin, -2.0f), lim))), 0);
+      x1 = min(static_cast<int>(floorf(fminf(fmaxf(hi_x + kShadowBoxMargin, -2.0f), lim))), S - 1);
+      y0 = max(static_cast<int>(ceilf(fminf(fmaxf(lo_y - kShadowBoxMargin, -2.0f), lim))), 0);
+      y1 = min(static_cast<int>(floorf(fminf(fmaxf(hi_y + kShadowBoxMargin, -2.0f), lim))), S - 1);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p) {
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
-  const int light = item / max(p.triangle_count, 1), t = item - light * p.triangle_count;
-  bool live = item < p.light_count * p.triangle_count;
-  ShadowTri tri{};
-  f3 v[3] = {mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f)};
-  if (live) {
-    const float4 lp = p.lights[2 * light];
-    for (int k = 0; k < 3; ++k) {
-      const uint32_t vi = p.indices[3 * t + k];
-      const float* q = p.positions + 3 * static_cast<size_t>(min(vi, static_cast<uint32_t>(p.vertex_count - 1)));
-      v[k] = mk(q[0] - lp.x, q[1] - lp.y, q[2] - lp.z);
-    }
-    // Edge functions from the edges e1 = b - a, e2 = c - a (c x a = e2 x a,
-    // a x b = a x e1, b x c = n - both): rounding scales with |a| |e|, not
-    // |a| |b|.
-    const f3 e1 = v[1] - v[0], e2 = v[2] - v[0];
-    tri.n = cross(e1, e2);
-    tri.n1 = cross(e2, v[0]);
-    tri.n2 = cross(v[0], e1);
-    tri.n0 = tri.n - tri.n1 - tri.n2;
-    tri.det = dot(v[0], tri.n);
-    live = tri.det < 0.0f;  // back faces and edge-on triangles are culled
-  }
+  int light;
+  f3 v[3];
+  ShadowTri tri;
+  bool live;
+  shadow_item(p, item, light, v, tri, live);
   const int S = p.size;
   const float inv_s = 1.0f / static_cast<float>(S);
-  const float half = 0.5f * static_cast<float>(S);
   const size_t cube_floats = static_cast<size_t>(6) * (S + 2) * (S + 2);
   float* cube = p.cubes + light * cube_floats;
   for (int face = 0; face < 6; ++face) {
-    int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
-    if (live) {
-      float sc[3], tc[3], ma[3];
-      bool in[3];
-      for (int k = 0; k < 3; ++k) {
-        shadow_face_coords(face, v[k], sc[k], tc[k], ma[k]);
-        in[k] = ma[k] >= kShadowClip;
-      }
-      if (in[0] || in[1] || in[2]) {
-        float lo_x = 1e30f, hi_x = -1e30f, lo_y = 1e30f, hi_y = -1e30f;
-        auto add = [&](float s_, float t_, float m_) {
-          const float r = 1.0f / m_;
-          const float x = fmaf(s_ * r, half, half - 0.5f), y = fmaf(t_ * r, half, half - 0.5f);
-          lo_x = fminf(lo_x, x), hi_x = fmaxf(hi_x, x), lo_y = fminf(lo_y, y), hi_y = fmaxf(hi_y, y);
-        };
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const int k2 = k == 2 ? 0 : k + 1;
-          if (in[k]) add(sc[k], tc[k], ma[k]);
-          if (in[k] != in[k2]) {  // the edge crosses the clip plane
-            const float u = (kShadowClip - ma[k]) / (ma[k2] - ma[k]);
-            add(fmaf(u, sc[k2] - sc[k], sc[k]), fmaf(u, tc[k2] - tc[k], tc[k]), kShadowClip);
-          }
-        }
-        // Texel centres inside the span, one texel of margin for rounding
-        // (coordinates clamped to [-2, S + 1] before the integer conversion).
-        const float lim = static_cast<float>(S + 1);
-        x0 = max(static_cast<int>(ceilf(fminf(fmaxf(lo_x, -2.0f), lim))) - 1, 0);
-        x1 = min(static_cast<int>(floorf(fminf(fmaxf(hi_x, -2.0f), lim))) + 1, S - 1);
-        y0 = max(static_cast<int>(ceilf(fminf(fmaxf(lo_y, -2.0f), lim))) - 1, 0);
-        y1 = min(static_cast<int>(floorf(fminf(fmaxf(hi_y, -2.0f), lim))) + 1, S - 1);
-      }
-    }
+    int x0, x1, y0, y1;
+    shadow_face_box(face, v, live, S, x0, x1, y0, y1);
     const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
     const int area = (bw > 0 && bh > 0) ? bw * bh : 0;
     if (area > 0 && area <= kShadowLaneTexels) {
@@ -176,6 +208,91 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
         shadow_texel(w, face, i, j, S, inv_s, wcube);
       }
     }
+  }
+}
+
+// The scheduled path: every item's six boxes, their total texels as its run
+// of slots, then a grid of waves over equal contiguous slot ranges (the
+// G-buffer rasterizer's scheme), so boxes of every size cost the same per
+// texel.  The item's edge functions are rebuilt per lane when its item changes.
+__global__ __launch_bounds__(kSlotBlock) void shadow_slot_setup_kernel(const ShadowParams p) {
+  __shared__ unsigned long long wave_total[kSlotBlock / 64];
+  const int item = blockIdx.x * kSlotBlock + threadIdx.x;
+  const int items = p.light_count * p.triangle_count;
+  int light;
+  f3 v[3];
+  ShadowTri tri;
+  bool live;
+  shadow_item(p, item, light, v, tri, live);
+  unsigned long long area = 0;
+  for (int face = 0; face < 6; ++face) {
+    int x0, x1, y0, y1;
+    shadow_face_box(face, v, live, p.size, x0, x1, y0, y1);
+    const int bw = x1 >= x0 ? x1 - x0 + 1 : 0, bh = y1 >= y0 ? y1 - y0 + 1 : 0;
+    const int w = bh > 0 ? bw : 0, h = bw > 0 ? bh : 0;
+    area += static_cast<unsigned long long>(w) * static_cast<unsigned long long>(h);
+    if (item < items)
+      p.slots.boxes[static_cast<size_t>(item) * 6 + face] =
+          make_uint2(static_cast<unsigned>(max(x0, 0)) | (static_cast<unsigned>(max(y0, 0)) << 16),
+                     static_cast<unsigned>(w) | (static_cast<unsigned>(h) << 16));
+  }
+  unsigned long long block_total;
+  const unsigned long long excl = slot_block_scan(area, wave_total, block_total);
+  if (item < items) p.slots.offsets[item] = excl;
+  if (threadIdx.x == 0) p.slots.blocks[blockIdx.x] = block_total;
+}
+
+__global__ __launch_bounds__(kSlotScan) void shadow_scan_kernel(unsigned long long* __restrict__ blocks, int n,
+                                                               unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long sums[kSlotScan];
+  slot_scan_blocks(blocks, n, total, sums);
+}
+
+__global__ __launch_bounds__(256) void shadow_offset_kernel(unsigned long long* __restrict__ offsets,
+                                                            const unsigned long long* __restrict__ blocks, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) offsets[t] += blocks[t / kSlotBlock];
+}
+
+__global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowParams p) {
+  const unsigned long long* __restrict__ offsets = p.slots.offsets;
+  const int n = p.light_count * p.triangle_count;
+  const unsigned long long total = offsets[n];
+  const int lane = threadIdx.x & 63;
+  unsigned long long begin, end;
+  slot_range(total, begin, end);
+  if (begin >= end) return;
+  const int S = p.size;
+  const float inv_s = 1.0f / static_cast<float>(S);
+  const size_t cube_floats = static_cast<size_t>(6) * (S + 2) * (S + 2);
+  int tf = slot_find(offsets, 0, n, begin);
+  int cur = -1, light = 0;
+  ShadowTri tri{};
+  for (unsigned long long base = begin; base < end; base += 64) {
+    const unsigned long long last = base + 63 < end ? base + 63 : end - 1;
+    tf = slot_gallop(offsets, tf, n, base);
+    const int tl = slot_gallop(offsets, tf, n, last);
+    const unsigned long long slot = base + lane;
+    if (slot > last) continue;
+    const int item = slot_find(offsets, tf, tl + 1, slot);
+    unsigned local = static_cast<unsigned>(slot - offsets[item]);
+    int face = 0;
+    uint2 b = p.slots.boxes[static_cast<size_t>(item) * 6];
+    for (; face < 5; ++face) {
+      const unsigned a = (b.y & 0xffffu) * (b.y >> 16);
+      if (local < a) break;
+      local -= a;
+      b = p.slots.boxes[static_cast<size_t>(item) * 6 + face + 1];
+    }
+    const unsigned w = b.y & 0xffffu;
+    const int i = static_cast<int>((b.x & 0xffffu) + local % w), j = static_cast<int>((b.x >> 16) + local / w);
+    if (item != cur) {
+      f3 v[3];
+      bool live;
+      shadow_item(p, item, light, v, tri, live);
+      cur = item;
+    }
+    shadow_texel(tri, face, i, j, S, inv_s, p.cubes + light * cube_floats);
   }
 }
 
@@ -221,8 +338,19 @@ __global__ __launch_bounds__(256) void shadow_border_kernel(float* __restrict__ 
   c[shadow_texel_index(S, face, i, j)] = v;
 }
 
+namespace {
+size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+size_t slot_blocks(size_t items) { return (items + kSlotBlock - 1) / kSlotBlock; }
+}  // namespace
+
+size_t shadow_workspace_bytes(int light_count, int triangle_count) {
+  if (light_count < 1 || triangle_count < 0) return 0;
+  const size_t items = static_cast<size_t>(light_count) * triangle_count;
+  return align256((items + 1) * 8) + align256(items * 6 * sizeof(uint2)) + align256(slot_blocks(items) * 8);
+}
+
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
-                                  int size, float* cubes, hipStream_t s) {
+                                  int size, float* cubes, hipStream_t s, void* workspace) {
   const size_t texels = static_cast<size_t>(light_count) * 6 * (size + 2) * (size + 2);
   hipLaunchKernelGGL(shadow_clear_kernel, dim3(static_cast<unsigned>((texels + 255) / 256)), dim3(256), 0, s, cubes,
                      texels);
@@ -238,8 +366,28 @@ althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_poi
   p.size = size;
   p.cubes = cubes;
   const size_t items = static_cast<size_t>(light_count) * mesh->triangle_count;
-  hipLaunchKernelGGL(shadow_raster_kernel, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, p);
-  st = check_launch("shadow_raster_kernel");
+  if (workspace) {
+    char* ws = static_cast<char*>(workspace);
+    p.slots.offsets = reinterpret_cast<unsigned long long*>(ws);
+    ws += align256((items + 1) * 8);
+    p.slots.boxes = reinterpret_cast<uint2*>(ws);
+    ws += align256(items * 6 * sizeof(uint2));
+    p.slots.blocks = reinterpret_cast<unsigned long long*>(ws);
+    const int nb = static_cast<int>(slot_blocks(items));
+    const int ni = static_cast<int>(items);
+    hipLaunchKernelGGL(shadow_slot_setup_kernel, dim3(nb), dim3(kSlotBlock), 0, s, p);
+    if ((st = check_launch("shadow_slot_setup_kernel")) != ALTHEA_OK) return st;
+    hipLaunchKernelGGL(shadow_scan_kernel, dim3(1), dim3(kSlotScan), 0, s, p.slots.blocks, nb, p.slots.offsets + ni);
+    if ((st = check_launch("shadow_scan_kernel")) != ALTHEA_OK) return st;
+    hipLaunchKernelGGL(shadow_offset_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, p.slots.offsets,
+                       p.slots.blocks, ni);
+    if ((st = check_launch("shadow_offset_kernel")) != ALTHEA_OK) return st;
+    hipLaunchKernelGGL(shadow_slot_raster_kernel, dim3(4096), dim3(256), 0, s, p);
+    st = check_launch("shadow_slot_raster_kernel");
+  } else {
+    hipLaunchKernelGGL(shadow_raster_kernel, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, p);
+    st = check_launch("shadow_raster_kernel");
+  }
   if (st != ALTHEA_OK) return st;
   const size_t border = static_cast<size_t>(light_count) * 6 * (4 * size + 4);
   hipLaunchKernelGGL(shadow_border_kernel, dim3(static_cast<unsigned>((border + 255) / 256)), dim3(256), 0, s, cubes,

@@ -304,11 +304,23 @@ class PointLightCollection {
                "PointLightCollection::updateResource");
   }
 
-  // The shadow pass (Src/PointLight.cpp:235-282) over a world-space mesh.
+  // The shadow pass (Src/PointLight.cpp:235-282) over a world-space mesh,
+  // through the workspace variant (the workspace grows to the largest mesh
+  // drawn and is kept).
   void drawShadowMaps(const althea_mesh_view& mesh, hipStream_t stream = nullptr) {
     if (!_cubes) throw std::runtime_error("PointLightCollection: created without shadow maps");
-    throwIfFailed(althea_shadow_cubes(&mesh, lights(), static_cast<int>(getCount()), _shadowSize,
-                                      static_cast<float*>(_cubes.get()), stream),
+    const int n = static_cast<int>(getCount());
+    const size_t need = althea_shadow_cubes_workspace_bytes(n, mesh.triangle_count);
+    if (need == 0) throw std::runtime_error("PointLightCollection::drawShadowMaps: too many light-triangle pairs");
+    if (need > _workspaceBytes) {
+      void* ws = nullptr;
+      if (hipMalloc(&ws, need) != hipSuccess)
+        throw std::runtime_error("PointLightCollection::drawShadowMaps: hipMalloc failed");
+      _workspace.reset(ws);
+      _workspaceBytes = need;
+    }
+    throwIfFailed(althea_shadow_cubes_ws(&mesh, lights(), n, _shadowSize, static_cast<float*>(_cubes.get()),
+                                         _workspace.get(), stream),
                   "PointLightCollection::drawShadowMaps");
   }
 
@@ -324,7 +336,8 @@ class PointLightCollection {
     void operator()(void* p) const { (void)hipFree(p); }
   };
   std::vector<althea_point_light> _host;
-  std::unique_ptr<void, Free> _lights, _cubes;
+  std::unique_ptr<void, Free> _lights, _cubes, _workspace;
+  size_t _workspaceBytes = 0;
   int _shadowSize = 0;
 };
 

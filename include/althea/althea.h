@@ -557,6 +557,15 @@ ALTHEA_API size_t althea_shadow_cube_floats(int size);
 ALTHEA_API althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights,
                                              int light_count, int size, float* d_cubes, hipStream_t stream);
 
+/* althea_shadow_cubes with a caller-provided workspace
+ * (althea_shadow_cubes_workspace_bytes: per light-triangle pair its six
+ * texel boxes and slot offset, 56 B) that deals the boxes' texels out evenly
+ * over the GPU; the same cubes, bit for bit, faster where box sizes vary. */
+ALTHEA_API size_t althea_shadow_cubes_workspace_bytes(int light_count, int triangle_count);
+ALTHEA_API althea_status althea_shadow_cubes_ws(const althea_mesh_view* mesh, const althea_point_light* d_lights,
+                                                int light_count, int size, float* d_cubes, void* d_workspace,
+                                                hipStream_t stream);
+
 /* althea_deferred_resolve (optionally reading a reflection buffer, which may
  * be NULL) with the point-light shadow test of PBRMaterial.glsl:119-140: light
  * i is skipped at a pixel when 1000 * cube_i(L.x, -L.y, -L.z) < |Lp - P| - 0.5,

@@ -105,6 +105,15 @@ def test_shadow_argument_checks_without_gpu():
     assert lib.althea_shadow_cubes(C.byref(no_vertices), dummy, 1, 16, dummy, None) == 1
     huge = _lib.MeshViewC(dummy, dummy, 3, 1 << 30)
     assert lib.althea_shadow_cubes(C.byref(huge), dummy, 2, 16, dummy, None) == 1
+    # The workspace variant: 8 B offset + 48 B boxes per light-triangle pair,
+    # 8 B per 256 pairs, each part 256-byte aligned.
+    assert lib.althea_shadow_cubes_workspace_bytes(2, 1000) == 16128 + 96000 + 256
+    assert lib.althea_shadow_cubes_workspace_bytes(0, 10) == 0
+    assert lib.althea_shadow_cubes_workspace_bytes(2, 1 << 30) == 0
+    assert lib.althea_shadow_cubes_ws(C.byref(mesh), dummy, 1, 16, dummy, None, None) == 1
+    assert b"null" in lib.althea_last_error()
+    assert lib.althea_shadow_cubes_ws(C.byref(mesh), dummy, 1, 0, dummy, dummy, None) == 1
+    assert lib.althea_shadow_cubes_ws(C.byref(huge), dummy, 2, 16, dummy, dummy, None) == 1
     g = _lib.GBufferViewC(dummy, dummy, dummy, dummy, 64, 32, 0)
     v = _lib.IBLViewsC()
     v.environment, v.env_width, v.env_height = dummy, 64, 32

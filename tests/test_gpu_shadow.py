@@ -51,11 +51,22 @@ def _mesh(torch, pos, idx):
             torch.from_numpy(np.ascontiguousarray(idx, np.uint32).view(np.int32)).cuda())
 
 
-def _gpu_cubes(torch, pos, idx, lights, size):
+def _gpu_cubes(torch, pos, idx, lights, size, scheduled=True):
     coll = aa.PointLightCollection.from_lights(torch.from_numpy(lights).cuda(), shadow_size=size)
-    coll.drawShadowMaps(*_mesh(torch, pos, idx))
+    coll.drawShadowMaps(*_mesh(torch, pos, idx), scheduled=scheduled)
     torch.cuda.synchronize()
     return coll, coll.getShadowMapImage().cpu().numpy()
+
+
+def _box(size):
+    rng = np.random.default_rng(size)
+    lights = np.concatenate([rng.uniform(-0.5, 0.5, (3, 3)), np.zeros((3, 1)), np.full((3, 3), 4.0),
+                             np.zeros((3, 1))], 1).astype(np.float32)
+    corners = np.array([[x, y, z] for x in (-2.0, 2.5) for y in (-1.5, 3.0) for z in (-2.2, 1.8)], np.float32)
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    # (a, c, b) winding: every normal points into the box.
+    idx = np.array([t for a, b, c, d in quads for t in ((a, c, b), (a, d, c))], np.uint32)
+    return corners, idx, lights
 
 
 def _compare_cubes(got, ref):
@@ -106,13 +117,7 @@ def test_large_occluders_span_whole_faces(torch, size):
     """A box of six inward-facing squares around the light: every face of
     every cube is covered, through the wave-cooperative walk (boxes larger
     than a lane's share) and for triangles that cross a face's plane."""
-    rng = np.random.default_rng(size)
-    lights = np.concatenate([rng.uniform(-0.5, 0.5, (3, 3)), np.zeros((3, 1)), np.full((3, 3), 4.0),
-                             np.zeros((3, 1))], 1).astype(np.float32)
-    corners = np.array([[x, y, z] for x in (-2.0, 2.5) for y in (-1.5, 3.0) for z in (-2.2, 1.8)], np.float32)
-    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
-    # (a, c, b) winding: every normal points into the box.
-    idx = np.array([t for a, b, c, d in quads for t in ((a, c, b), (a, d, c))], np.uint32)
+    corners, idx, lights = _box(size)
     _, got = _gpu_cubes(torch, corners, idx, lights, size)
     ref = ol.shadow_cubes(corners, idx, lights, size)
     assert np.all(ref < 1.0), "the box must enclose every light"
@@ -120,6 +125,38 @@ def test_large_occluders_span_whole_faces(torch, size):
     print("box", size, r)
     assert r["hit_disagree"] <= 2e-3 * r["hits"] + 2, r
     assert r["depth_off"] <= 2e-3 * r["hits"] + 2, r
+
+
+@pytest.mark.parametrize("case", ["heightfield", "box256", "box3", "mixed"])
+def test_scheduled_path_is_bit_identical(torch, case):
+    """althea_shadow_cubes_ws (texels dealt out evenly over the GPU) writes
+    the cubes of althea_shadow_cubes bit for bit: the same texel tests and an
+    order-free atomic minimum.  Cases: many small boxes, whole-face boxes,
+    boxes of one or two texels, and both mixed with culled and degenerate
+    triangles (zero slots)."""
+    if case == "heightfield":
+        sc = aa.synth.scene(seed=SEED0 + 35, width=8, height=8, lights=9)
+        pos, idx = aa.synth.heightfield_mesh(SEED0 + 35, n=64)
+        lights, size = sc.lights, 96
+    elif case.startswith("box"):
+        size = int(case[3:])
+        pos, idx, lights = _box(size)
+    else:
+        sc = aa.synth.scene(seed=SEED0 + 36, width=8, height=8, lights=5)
+        hp, hi = aa.synth.heightfield_mesh(SEED0 + 36, n=24)
+        bp, bi, _ = _box(7)
+        pos = np.concatenate([hp, bp * 6.0, np.float32([[0, 1, 0], [1, 1, 0], [2, 1, 0]])])
+        extra = np.array([[len(pos) - 3, len(pos) - 2, len(pos) - 1], [4, 4, 4]], np.uint32)
+        idx = np.concatenate([hi, extra, bi + len(hp), hi[::-1, ::-1]])
+        lights, size = sc.lights, 61
+    _, a = _gpu_cubes(torch, pos, idx, lights, size, scheduled=False)
+    coll, b = _gpu_cubes(torch, pos, idx, lights, size, scheduled=True)
+    assert (a < 1.0).any()
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    # A second, smaller draw through the same (kept) workspace.
+    coll.drawShadowMaps(*_mesh(torch, pos, idx[: len(idx) // 3]))
+    _, c = _gpu_cubes(torch, pos, idx[: len(idx) // 3], lights, size, scheduled=False)
+    np.testing.assert_array_equal(coll.getShadowMapImage().cpu().numpy().view(np.uint32), c.view(np.uint32))
 
 
 def _expected_border(faces, S):
