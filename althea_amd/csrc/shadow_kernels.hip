@@ -214,7 +214,10 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
 // The scheduled path: every item's six boxes, their total texels as its run
 // of slots, then a grid of waves over equal contiguous slot ranges (the
 // G-buffer rasterizer's scheme), so boxes of every size cost the same per
-// texel.  The item's edge functions are rebuilt per lane when its item changes.
+// texel.  A wave finds the items of its 64 slots from a window of 64
+// consecutive prefix sums (one coalesced load) by a binary search across the
+// lanes, instead of dependent loads per slot; the item's edge functions are
+// rebuilt per lane when its item changes.
 __global__ __launch_bounds__(kSlotBlock) void shadow_slot_setup_kernel(const ShadowParams p) {
   __shared__ unsigned long long wave_total[kSlotBlock / 64];
   const int item = blockIdx.x * kSlotBlock + threadIdx.x;
@@ -265,25 +268,51 @@ __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowPar
   const int S = p.size;
   const float inv_s = 1.0f / static_cast<float>(S);
   const size_t cube_floats = static_cast<size_t>(6) * (S + 2) * (S + 2);
-  int tf = slot_find(offsets, 0, n, begin);
+  int w0 = slot_find(offsets, 0, n, begin);  // the item holding `begin`
   int cur = -1, light = 0;
   ShadowTri tri{};
   for (unsigned long long base = begin; base < end; base += 64) {
-    const unsigned long long last = base + 63 < end ? base + 63 : end - 1;
-    tf = slot_gallop(offsets, tf, n, base);
-    const int tl = slot_gallop(offsets, tf, n, last);
     const unsigned long long slot = base + lane;
-    if (slot > last) continue;
-    const int item = slot_find(offsets, tf, tl + 1, slot);
-    unsigned local = static_cast<unsigned>(slot - offsets[item]);
-    int face = 0;
-    uint2 b = p.slots.boxes[static_cast<size_t>(item) * 6];
-    for (; face < 5; ++face) {
-      const unsigned a = (b.y & 0xffffu) * (b.y >> 16);
-      if (local < a) break;
-      local -= a;
-      b = p.slots.boxes[static_cast<size_t>(item) * 6 + face + 1];
+    const bool active = slot < end;
+    int item = -1;
+    unsigned item_start = 0;
+    // The items of this step from windows of 64 consecutive prefix sums (two
+    // coalesced loads), each lane's item by a binary search across the lanes.
+    for (bool pending = active; __builtin_amdgcn_ballot_w64(pending);) {
+      const unsigned long long o = offsets[min(w0 + lane, n)];
+      const unsigned long long o_end = offsets[min(w0 + 64, n)];
+      const long long rel = static_cast<long long>(o - base);
+      const int r = static_cast<int>(rel < -1 ? -1 : (rel > 64 ? 64 : rel));
+      const bool here = pending && slot < o_end;
+      int lo = 0;  // largest k with r_k <= lane (r_0 <= lane for every pending lane)
+#pragma unroll
+      for (int h = 32; h >= 1; h >>= 1)
+        if (__shfl(r, lo + h) <= lane) lo += h;
+      const unsigned ostart = static_cast<unsigned>(__shfl(static_cast<int>(static_cast<unsigned>(o)), lo));
+      if (here) item = w0 + lo, item_start = ostart;
+      pending = pending && !here;
+      w0 = min(w0 + 64, n - 1);
     }
+    w0 = __shfl(item, end - base > 64 ? 63 : static_cast<int>(end - base) - 1);  // the item of the step's last slot
+    if (!active) continue;
+    unsigned local = static_cast<unsigned>(slot) - item_start;
+    const uint4* bx = reinterpret_cast<const uint4*>(p.slots.boxes + static_cast<size_t>(item) * 6);
+    const uint4 q0 = bx[0], q1 = bx[1], q2 = bx[2];
+    const uint2 boxes[6] = {make_uint2(q0.x, q0.y), make_uint2(q0.z, q0.w), make_uint2(q1.x, q1.y),
+                            make_uint2(q1.z, q1.w), make_uint2(q2.x, q2.y), make_uint2(q2.z, q2.w)};
+    int face = 0;
+    uint2 b = boxes[0];
+    bool found = false;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {  // the face whose texels hold `local`
+      const unsigned a = (boxes[f].y & 0xffffu) * (boxes[f].y >> 16);
+      if (!found && local < a) {
+        face = f, b = boxes[f], found = true;
+      } else if (!found) {
+        local -= a;
+      }
+    }
+    if (!found) continue;  // not reached: the runs are the boxes' areas
     const unsigned w = b.y & 0xffffu;
     const int i = static_cast<int>((b.x & 0xffffu) + local % w), j = static_cast<int>((b.x >> 16) + local / w);
     if (item != cur) {
