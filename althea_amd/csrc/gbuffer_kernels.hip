@@ -398,16 +398,13 @@ __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
   unsigned long long begin, end;
   slot_range(total, begin, end);
   if (begin >= end) return;
-  int tf = slot_find(offsets, 0, T, begin);
+  int w0 = slot_find(offsets, 0, T, begin);  // the triangle holding `begin`
   for (unsigned long long base = begin; base < end; base += 64) {
-    const unsigned long long last = base + 63 < end ? base + 63 : end - 1;
-    tf = slot_gallop(offsets, tf, T, base);
-    const int tl = slot_gallop(offsets, tf, T, last);
-    const unsigned long long slot = base + lane;
-    if (slot > last) continue;
-    const int lo = slot_find(offsets, tf, tl + 1, slot);
+    unsigned start;
+    const int lo = slot_step_items(offsets, T, base, end, w0, start);
+    if (lo < 0) continue;
     const GbTri tri = p.tris[lo];
-    const unsigned local = static_cast<unsigned>(slot - offsets[lo]);
+    const unsigned local = static_cast<unsigned>(base + lane) - start;
     const int x = tri.x0 + static_cast<int>(local % static_cast<unsigned>(tri.w));
     const int y = tri.y0 + static_cast<int>(local / static_cast<unsigned>(tri.w));
     float e[3];

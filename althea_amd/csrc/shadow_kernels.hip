@@ -261,7 +261,6 @@ __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowPar
   const unsigned long long* __restrict__ offsets = p.slots.offsets;
   const int n = p.light_count * p.triangle_count;
   const unsigned long long total = offsets[n];
-  const int lane = threadIdx.x & 63;
   unsigned long long begin, end;
   slot_range(total, begin, end);
   if (begin >= end) return;
@@ -272,30 +271,10 @@ __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowPar
   int cur = -1, light = 0;
   ShadowTri tri{};
   for (unsigned long long base = begin; base < end; base += 64) {
-    const unsigned long long slot = base + lane;
-    const bool active = slot < end;
-    int item = -1;
-    unsigned item_start = 0;
-    // The items of this step from windows of 64 consecutive prefix sums (two
-    // coalesced loads), each lane's item by a binary search across the lanes.
-    for (bool pending = active; __builtin_amdgcn_ballot_w64(pending);) {
-      const unsigned long long o = offsets[min(w0 + lane, n)];
-      const unsigned long long o_end = offsets[min(w0 + 64, n)];
-      const long long rel = static_cast<long long>(o - base);
-      const int r = static_cast<int>(rel < -1 ? -1 : (rel > 64 ? 64 : rel));
-      const bool here = pending && slot < o_end;
-      int lo = 0;  // largest k with r_k <= lane (r_0 <= lane for every pending lane)
-#pragma unroll
-      for (int h = 32; h >= 1; h >>= 1)
-        if (__shfl(r, lo + h) <= lane) lo += h;
-      const unsigned ostart = static_cast<unsigned>(__shfl(static_cast<int>(static_cast<unsigned>(o)), lo));
-      if (here) item = w0 + lo, item_start = ostart;
-      pending = pending && !here;
-      w0 = min(w0 + 64, n - 1);
-    }
-    w0 = __shfl(item, end - base > 64 ? 63 : static_cast<int>(end - base) - 1);  // the item of the step's last slot
-    if (!active) continue;
-    unsigned local = static_cast<unsigned>(slot) - item_start;
+    unsigned item_start;
+    const int item = slot_step_items(offsets, n, base, end, w0, item_start);
+    if (item < 0) continue;
+    unsigned local = static_cast<unsigned>(base + (threadIdx.x & 63)) - item_start;
     const uint4* bx = reinterpret_cast<const uint4*>(p.slots.boxes + static_cast<size_t>(item) * 6);
     const uint4 q0 = bx[0], q1 = bx[1], q2 = bx[2];
     const uint2 boxes[6] = {make_uint2(q0.x, q0.y), make_uint2(q0.z, q0.w), make_uint2(q1.x, q1.y),

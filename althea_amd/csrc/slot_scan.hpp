@@ -74,16 +74,36 @@ __device__ __forceinline__ int slot_find(const unsigned long long* __restrict__ 
   return lo;
 }
 
-// Largest t >= lo with offsets[t] <= slot (offsets[lo] <= slot < offsets[n]):
-// probes lo + 1, lo + 3, lo + 7, ... then a binary search in the last step.
-__device__ __forceinline__ int slot_gallop(const unsigned long long* __restrict__ offsets, int lo, int n,
-                                           unsigned long long slot) {
-  int step = 1;
-  while (lo + step < n && offsets[lo + step] <= slot) {
-    lo += step;
-    step <<= 1;
+// The item of slot base + lane for every lane of a wave step [base, end)
+// (-1 where base + lane >= end) and the low 32 bits of its first slot, from
+// windows of 64 consecutive prefix sums (one coalesced load each) and a
+// binary search across the lanes; usually one window.  w0 names an item with
+// offsets[w0] <= base and becomes the item of the step's last slot, which
+// serves the next step.  Every lane of the wave must call it.
+__device__ __forceinline__ int slot_step_items(const unsigned long long* __restrict__ offsets, int n,
+                                               unsigned long long base, unsigned long long end, int& w0,
+                                               unsigned& item_start) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long slot = base + lane;
+  int item = -1;
+  item_start = 0;
+  for (bool pending = slot < end; __builtin_amdgcn_ballot_w64(pending);) {
+    const unsigned long long o = offsets[min(w0 + lane, n)];
+    const unsigned long long o_end = offsets[min(w0 + 64, n)];
+    const long long rel = static_cast<long long>(o - base);
+    const int r = static_cast<int>(rel < -1 ? -1 : (rel > 64 ? 64 : rel));
+    const bool here = pending && slot < o_end;
+    int lo = 0;  // largest k with r_k <= lane (r_0 <= lane for every pending lane)
+#pragma unroll
+    for (int h = 32; h >= 1; h >>= 1)
+      if (__shfl(r, lo + h) <= lane) lo += h;
+    const unsigned start = static_cast<unsigned>(__shfl(static_cast<int>(static_cast<unsigned>(o)), lo));
+    if (here) item = w0 + lo, item_start = start;
+    pending = pending && !here;
+    w0 = min(w0 + 64, n - 1);  // reached only while some slot lies past this window
   }
-  return slot_find(offsets, lo, min(lo + step, n), slot);
+  w0 = __shfl(item, end - base > 64 ? 63 : static_cast<int>(end - base) - 1);
+  return item;
 }
 
 __device__ __forceinline__ unsigned long long slot_uniform(unsigned long long v) {
