@@ -81,6 +81,18 @@ struct GbPrim {
   int vertex_count, tri_begin, triangle_count, material;
 };
 
+// Per-triangle alpha-test record of the glTF subpass, written by the setup
+// kernel so that the raster's test waits on one record instead of the chain
+// primitive -> material -> indices -> coordinates -> texture view: the base
+// colour texture coordinates at the three vertices, the alpha factor and
+// cut-off, and the base colour texture's view (null texels: no test, a
+// pass-through primitive).
+struct GbAlpha {
+  float2 uv[3];
+  float factor, cutoff;
+  althea_texture_view tex;
+};
+
 struct GbParams {
   GbPrim one;                 // the primitive when prims is null
   const GbPrim* prims;        // device table of prim_count primitives, or null
@@ -88,6 +100,7 @@ struct GbParams {
   const althea_gltf_material* materials;
   const althea_texture_view* textures;
   int alpha_test;             // some primitive is a glTF one
+  GbAlpha* alpha;             // triangle_count records when alpha_test
   int triangle_count;         // over all primitives
   int width, height;
   float eye[3];
@@ -123,6 +136,26 @@ __device__ __forceinline__ const GbPrim& gb_prim(const GbParams& p, int t) {
       hi = mid;
   }
   return p.prims[lo];
+}
+
+__device__ __forceinline__ void gb_indices(const GbPrim& pr, int lt, uint32_t vi[3]) {
+  const uint32_t last = static_cast<uint32_t>(pr.vertex_count - 1);
+  for (int k = 0; k < 3; ++k) vi[k] = min(pr.indices[3 * lt + k], last);
+}
+
+__device__ GbAlpha gb_alpha_record(const GbParams& p, int t) {
+  GbAlpha a{};
+  const GbPrim& pr = gb_prim(p, t);
+  if (pr.material < 0) return a;  // pass-through: texels stay null
+  const althea_gltf_material& m = p.materials[pr.material];
+  const float2* uv = reinterpret_cast<const float2*>(pr.uvs[m.base_uv]);
+  uint32_t vi[3];
+  gb_indices(pr, t - pr.tri_begin, vi);
+  for (int k = 0; k < 3; ++k) a.uv[k] = uv ? uv[vi[k]] : make_float2(0.0f, 0.0f);
+  a.factor = m.base_color_factor[3];
+  a.cutoff = m.alpha_cutoff;
+  a.tex = p.textures[m.base_texture];
+  return a;
 }
 
 __device__ __forceinline__ f3 gb_ray(const GbParams& p, int x, int y) {
@@ -211,6 +244,7 @@ __global__ __launch_bounds__(kGbSetupThreads) void gbuffer_setup_kernel(const Gb
     GbTri tri;
     gbuffer_setup_triangle(p, t, tri, area);
     p.tris[t] = tri;
+    if (p.alpha) p.alpha[t] = gb_alpha_record(p, t);
   }
   unsigned long long block_total;
   const unsigned long long excl = slot_block_scan(area, wave_total, block_total);
@@ -353,12 +387,9 @@ __device__ __forceinline__ void gb_bary(const GbTri& tri, f3 d, float l[3]) {
   l[0] = e0 * inv, l[1] = e1 * inv, l[2] = e2 * inv;
 }
 
-__device__ __forceinline__ GbUv gb_uv(const float* __restrict__ uv, const uint32_t vi[3], const float l[3],
-                                      const float lx[3], const float ly[3]) {
+__device__ __forceinline__ GbUv gb_uv_of(float2 a, float2 b, float2 cc, const float l[3], const float lx[3],
+                                         const float ly[3]) {
   GbUv c{};
-  if (!uv) return c;
-  const float2 a = reinterpret_cast<const float2*>(uv)[vi[0]], b = reinterpret_cast<const float2*>(uv)[vi[1]],
-               cc = reinterpret_cast<const float2*>(uv)[vi[2]];
   auto at = [&](const float w[3], float& u, float& v) {
     u = fmaf(w[2], cc.x, fmaf(w[1], b.x, w[0] * a.x));
     v = fmaf(w[2], cc.y, fmaf(w[1], b.y, w[0] * a.y));
@@ -371,22 +402,21 @@ __device__ __forceinline__ GbUv gb_uv(const float* __restrict__ uv, const uint32
   return c;
 }
 
-__device__ __forceinline__ void gb_indices(const GbPrim& pr, int lt, uint32_t vi[3]) {
-  const uint32_t last = static_cast<uint32_t>(pr.vertex_count - 1);
-  for (int k = 0; k < 3; ++k) vi[k] = min(pr.indices[3 * lt + k], last);
+__device__ __forceinline__ GbUv gb_uv(const float* __restrict__ uv, const uint32_t vi[3], const float l[3],
+                                      const float lx[3], const float ly[3]) {
+  if (!uv) return GbUv{};
+  const float2* q = reinterpret_cast<const float2*>(uv);
+  return gb_uv_of(q[vi[0]], q[vi[1]], q[vi[2]], l, lx, ly);
 }
 
 // Base colour alpha of a glTF fragment (the alpha test of Gltf.frag:40-42).
-__device__ float gb_base_alpha(const GbParams& p, const GbPrim& pr, const GbTri& tri, int lt, int x, int y, f3 d) {
-  const althea_gltf_material& m = p.materials[pr.material];
-  uint32_t vi[3];
-  gb_indices(pr, lt, vi);
+__device__ float gb_base_alpha(const GbParams& p, const GbAlpha& a, const GbTri& tri, int x, int y, f3 d) {
   float l[3], lx[3], ly[3];
   gb_bary(tri, d, l);
   gb_bary(tri, gb_ray(p, x + 1, y), lx);
   gb_bary(tri, gb_ray(p, x, y + 1), ly);
-  const GbUv c = gb_uv(pr.uvs[m.base_uv], vi, l, lx, ly);
-  return gb_sample<true>(p.textures[m.base_texture], c).w * m.base_color_factor[3];
+  const GbUv c = gb_uv_of(a.uv[0], a.uv[1], a.uv[2], l, lx, ly);
+  return gb_sample<true>(a.tex, c).w * a.factor;
 }
 
 __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
@@ -412,10 +442,8 @@ __global__ __launch_bounds__(256) void gbuffer_raster_kernel(const GbParams p) {
     const float t = gb_hit(p, tri, d, e);
     if (t < 0.0f) continue;
     if (p.alpha_test) {
-      const GbPrim& pr = gb_prim(p, lo);
-      if (pr.material >= 0 && gb_base_alpha(p, pr, tri, lo - pr.tri_begin, x, y, d) <
-                                  p.materials[pr.material].alpha_cutoff)
-        continue;  // discarded: no depth written
+      const GbAlpha& a = p.alpha[lo];
+      if (a.tex.texels && gb_base_alpha(p, a, tri, x, y, d) < a.cutoff) continue;  // discarded: no depth written
     }
     const unsigned long long key =
         (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | static_cast<unsigned>(lo);
@@ -622,7 +650,8 @@ size_t gbuffer_gltf_workspace_bytes(int width, int height, int triangle_count, i
   if (base == 0) return 0;
   return base + align256(static_cast<size_t>(primitive_count) * sizeof(GbPrim)) +
          align256(static_cast<size_t>(material_count) * sizeof(althea_gltf_material)) +
-         align256(static_cast<size_t>(texture_count) * sizeof(althea_texture_view));
+         align256(static_cast<size_t>(texture_count) * sizeof(althea_texture_view)) +
+         align256(static_cast<size_t>(triangle_count) * sizeof(GbAlpha));
 }
 
 // Uploads the sRGB decode table to the current device, once per device.
@@ -674,6 +703,8 @@ althea_status launch_gbuffer_draw_gltf(const althea_gltf_primitive* prims, int p
   althea_gltf_material* d_mat = reinterpret_cast<althea_gltf_material*>(ws);
   ws += align256(static_cast<size_t>(material_count) * sizeof(althea_gltf_material));
   althea_texture_view* d_tex = reinterpret_cast<althea_texture_view*>(ws);
+  ws += align256(static_cast<size_t>(texture_count) * sizeof(althea_texture_view));
+  p.alpha = reinterpret_cast<GbAlpha*>(ws);
   // Pageable host tables: the copies order after the stream's earlier work.
   ALTHEA_TRY_HIP(hipMemcpyAsync(d_prims, table.data(), table.size() * sizeof(GbPrim), hipMemcpyHostToDevice, s));
   ALTHEA_TRY_HIP(hipMemcpyAsync(d_mat, materials, static_cast<size_t>(material_count) * sizeof(althea_gltf_material),
