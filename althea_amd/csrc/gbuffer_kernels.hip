@@ -304,13 +304,22 @@ __device__ __forceinline__ float gb_srgb_to_linear(float c) {
 // FP64 on the host, rounded to FP32), uploaded once per device.
 __device__ float gb_srgb_lut[256];
 
+// Power-of-two extents wrap with a mask (the same value as the modulo,
+// negative coordinates included), others with an integer modulo.
 __device__ __forceinline__ int gb_wrap(int i, int n, int mode) {
   if (mode == ALTHEA_WRAP_CLAMP_TO_EDGE) return min(max(i, 0), n - 1);
+  const bool pow2 = (n & (n - 1)) == 0;
   if (mode == ALTHEA_WRAP_MIRRORED_REPEAT) {
-    int m = i % (2 * n);
-    if (m < 0) m += 2 * n;
+    int m;
+    if (pow2) {
+      m = i & (2 * n - 1);
+    } else {
+      m = i % (2 * n);
+      if (m < 0) m += 2 * n;
+    }
     return m < n ? m : 2 * n - 1 - m;
   }
+  if (pow2) return i & (n - 1);
   const int m = i % n;
   return m < 0 ? m + n : m;
 }

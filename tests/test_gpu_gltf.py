@@ -38,14 +38,21 @@ def torch():
     return t
 
 
-def _levels0(seed):
+# Texture extents (w, h) of the base colour, normal and metallic-roughness
+# maps: powers of two (wrapped by a mask) and others (by an integer modulo).
+EXTENTS = {"pow2": [(64, 64), (64, 64), (32, 32)], "odd": [(48, 40), (40, 24), (24, 20)]}
+
+
+def _levels0(seed, extents=EXTENTS["pow2"]):
     rng = np.random.default_rng(seed)
-    base = rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)
-    base[..., 3] = np.where(rng.uniform(size=(64, 64)) < 0.25, 60, 230)  # alpha below / above the cut-off
-    y, x = np.mgrid[0:64, 0:64] / 64.0
+    (bw, bh), (nw, nh), (mw, mh) = extents
+    base = rng.integers(0, 256, (bh, bw, 4), dtype=np.uint8)
+    base[..., 3] = np.where(rng.uniform(size=(bh, bw)) < 0.25, 60, 230)  # alpha below / above the cut-off
+    y, x = np.mgrid[0:nh, 0:nw]
+    y, x = y / nh, x / nw
     nrm = np.stack([128 + 60 * np.sin(6.28 * x), 128 + 60 * np.cos(6.28 * y), np.full_like(x, 235),
                     np.full_like(x, 255)], -1).astype(np.uint8)
-    mr = rng.integers(0, 256, (32, 32, 4), dtype=np.uint8)
+    mr = rng.integers(0, 256, (mh, mw, 4), dtype=np.uint8)
     return base, nrm, mr
 
 
@@ -106,9 +113,14 @@ def _materials():
     return [mat(f=(0.9, 0.8, 0.7, 1.0), mf=0.7, rf=0.9, ns=1.5, mu=1), mat(bt=3, nt=4, mt=4)]
 
 
+def _mips(k, level0):
+    """SPECS[k]'s level count, at most the texture's full chain."""
+    return min(SPECS[k]["mips"], int(np.log2(max(level0.shape[:2]))) + 1)
+
+
 def _gpu(torch, sc, prims, mats, levels, fp16=False):
     f = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    texs = [aa.Texture(f(levels[k]), mips=SPECS[k]["mips"], srgb=bool(SPECS[k]["srgb"]), mag_filter=SPECS[k]["mag"],
+    texs = [aa.Texture(f(levels[k]), mips=_mips(k, levels[k]), srgb=bool(SPECS[k]["srgb"]), mag_filter=SPECS[k]["mag"],
                        min_filter=SPECS[k]["min_"], mipmap_mode=SPECS[k]["mipmode"], wrap_u=SPECS[k]["wrap_u"],
                        wrap_v=SPECS[k]["wrap_v"]) for k in range(3)]
     texs.append(aa.Texture(f(np.full((1, 1, 4), 255, np.uint8))))
@@ -128,15 +140,17 @@ def _gpu(torch, sc, prims, mats, levels, fp16=False):
     return planes, chains
 
 
-def test_textured_heightfield_matches_oracle(torch):
+@pytest.mark.parametrize("extents", ["pow2", "odd"])
+def test_textured_heightfield_matches_oracle(torch, extents):
     sc, prims = _scene(torch, SEED0 + 51)
-    levels = _levels0(SEED0 + 52)
+    levels = _levels0(SEED0 + 52, EXTENTS[extents])
     mats = _materials()
     got, chains = _gpu(torch, sc, prims, mats, levels)
     # The oracle samples the GPU's own mip chains (their blit is tested above).
     specs = SPECS + [dict(srgb=0, mips=1, mag=1, min_=1, mipmode=1, wrap_u=0, wrap_v=0)] * 2
-    sizes = [(64, 64), (64, 64), (32, 32), (1, 1), (1, 1)]
-    otex = [ol.OracleTex(chains[k], sizes[k][0], sizes[k][1], specs[k]["mips"], specs[k]["srgb"], specs[k]["mag"],
+    sizes = EXTENTS[extents] + [(1, 1), (1, 1)]
+    mips = [_mips(k, levels[k]) for k in range(3)] + [1, 1]
+    otex = [ol.OracleTex(chains[k], sizes[k][0], sizes[k][1], mips[k], specs[k]["srgb"], specs[k]["mag"],
                          specs[k]["min_"], specs[k]["mipmode"], specs[k]["wrap_u"], specs[k]["wrap_v"])
             for k in range(5)]
     ref, margin, tri = ol.gbuffer_gltf(sc, prims, mats, otex)
@@ -151,7 +165,7 @@ def test_textured_heightfield_matches_oracle(torch):
     err = np.abs(got[both] - ref[both]) / np.maximum(np.abs(ref[both]), 1.0)
     r["max_by_plane"] = [float(err[:, k].max()) for k in range(4)]
     r["mean_by_plane"] = [float(err[:, k].mean()) for k in range(4)]
-    print("gltf", r)
+    print("gltf", extents, r)
     assert r["terrain"] > 0.2 * r["pixels"] and r["floor"] > 0.05 * r["pixels"], r
     assert r["clear"] > 0.9, r
     assert r["coverage_disagree_clear"] == 0, r
