@@ -65,7 +65,8 @@ struct ShadowParams {
   int light_count;
   int size;
   float* cubes;
-  ShadowSlots slots;  // the scheduled path only
+  double inv_triangles;  // 1 / triangle_count
+  ShadowSlots slots;     // the scheduled path only
 };
 
 __global__ void shadow_clear_kernel(float* __restrict__ c, size_t n) {
@@ -78,7 +79,7 @@ __global__ void shadow_clear_kernel(float* __restrict__ c, size_t n) {
 __device__ __forceinline__ f3 shadow_texel_dir(int face, int i, int j, float inv_s, float& len) {
   const float pc = fmaf(static_cast<float>(2 * i + 1), inv_s, -1.0f);
   const float pt = fmaf(static_cast<float>(2 * j + 1), inv_s, -1.0f);
-  len = sqrtf(fmaf(pc, pc, fmaf(pt, pt, 1.0f)));
+  len = sqrt_fast(fmaf(pc, pc, fmaf(pt, pt, 1.0f)));
   return shadow_face_turn(face, cube_face_dir(face, pc, pt));
 }
 
@@ -96,7 +97,7 @@ __device__ __forceinline__ void shadow_texel(const ShadowTri& tri, int face, int
   if (e0 > 0.0f || e1 > 0.0f || e2 > 0.0f) return;  // outside (det < 0)
   const float rn = dot(d, tri.n);
   if (!(rn < 0.0f)) return;
-  const float t = tri.det / rn;  // view depth of the hit
+  const float t = tri.det * rcp_fast(rn);  // view depth of the hit
   if (t < ALTHEA_SHADOW_NEAR || t > ALTHEA_SHADOW_FAR) return;
   const float depth = fminf(t * len * (1.0f / ALTHEA_SHADOW_FAR), 1.0f);
   atomicMin(reinterpret_cast<unsigned int*>(cube) + shadow_texel_index(S, face, i, j), __float_as_uint(depth));
@@ -106,8 +107,15 @@ __device__ __forceinline__ void shadow_texel(const ShadowTri& tri, int face, int
 // false for items past the end and for back-facing or edge-on triangles.
 __device__ __forceinline__ void shadow_item(const ShadowParams& p, int item, int& light, f3 v[3], ShadowTri& tri,
                                             bool& live) {
-  light = item / max(p.triangle_count, 1);
-  const int t = item - light * p.triangle_count;
+  // item / T through the FP64 reciprocal: for item < 2^31 the truncated
+  // product is off by at most one, which the remainder corrects.
+  light = static_cast<int>(static_cast<double>(item) * p.inv_triangles);
+  int t = item - light * p.triangle_count;
+  if (t < 0) {
+    --light, t += p.triangle_count;
+  } else if (t >= p.triangle_count) {
+    ++light, t -= p.triangle_count;
+  }
   live = item < p.light_count * p.triangle_count;
   tri = ShadowTri{};
   v[0] = v[1] = v[2] = mk(0.0f, 0.0f, 0.0f);
@@ -292,8 +300,17 @@ __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowPar
       }
     }
     if (!found) continue;  // not reached: the runs are the boxes' areas
+    // Row and column of `local` in the box: local < w h <= 2^24 is exact in
+    // FP32, so the reciprocal's row is off by at most one, corrected here.
     const unsigned w = b.y & 0xffffu;
-    const int i = static_cast<int>((b.x & 0xffffu) + local % w), j = static_cast<int>((b.x >> 16) + local / w);
+    int row = static_cast<int>(static_cast<float>(local) * rcp_fast(static_cast<float>(w)));
+    int col = static_cast<int>(local) - row * static_cast<int>(w);
+    if (col < 0) {
+      --row, col += static_cast<int>(w);
+    } else if (col >= static_cast<int>(w)) {
+      ++row, col -= static_cast<int>(w);
+    }
+    const int i = static_cast<int>(b.x & 0xffffu) + col, j = static_cast<int>(b.x >> 16) + row;
     if (item != cur) {
       f3 v[3];
       bool live;
@@ -373,6 +390,7 @@ althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_poi
   p.light_count = light_count;
   p.size = size;
   p.cubes = cubes;
+  p.inv_triangles = 1.0 / static_cast<double>(mesh->triangle_count);
   const size_t items = static_cast<size_t>(light_count) * mesh->triangle_count;
   if (workspace) {
     char* ws = static_cast<char*>(workspace);
