@@ -52,7 +52,7 @@ constexpr float kShadowBoxMargin = 1.0f / 64.0f;
 
 struct ShadowSlots {
   unsigned long long* offsets;  // items + 1 slot offsets
-  uint2* boxes;                 // per item and face: x0 | y0 << 16, w | h << 16
+  uint2* boxes;                 // per item and face: x0 | y0 << 16, w | h << 16 (items with texels only)
   unsigned long long* blocks;   // per setup block: slot total, then first slot
 };
 
@@ -153,7 +153,7 @@ __device__ __forceinline__ void shadow_face_box(int face, const f3 v[3], bool li
     if (in[0] || in[1] || in[2]) {
       float lo_x = 1e30f, hi_x = -1e30f, lo_y = 1e30f, hi_y = -1e30f;
       auto add = [&](float s_, float t_, float m_) {
-        const float r = 1.0f / m_;
+        const float r = rcp_fast(m_);  // 1 ulp: far inside the box margin
         const float x = fmaf(s_ * r, half, half - 0.5f), y = fmaf(t_ * r, half, half - 0.5f);
         lo_x = fminf(lo_x, x), hi_x = fmaxf(hi_x, x), lo_y = fminf(lo_y, y), hi_y = fmaxf(hi_y, y);
       };
@@ -162,7 +162,7 @@ __device__ __forceinline__ void shadow_face_box(int face, const f3 v[3], bool li
         const int k2 = k == 2 ? 0 : k + 1;
         if (in[k]) add(sc[k], tc[k], ma[k]);
         if (in[k] != in[k2]) {  // the edge crosses the clip plane
-          const float u = (kShadowClip - ma[k]) / (ma[k2] - ma[k]);
+          const float u = (kShadowClip - ma[k]) * rcp_fast(ma[k2] - ma[k]);
           add(fmaf(u, sc[k2] - sc[k], sc[k]), fmaf(u, tc[k2] - tc[k], tc[k]), kShadowClip);
         }
       }
@@ -236,16 +236,24 @@ __global__ __launch_bounds__(kSlotBlock) void shadow_slot_setup_kernel(const Sha
   bool live;
   shadow_item(p, item, light, v, tri, live);
   unsigned long long area = 0;
+  unsigned bx[12];
+#pragma unroll
   for (int face = 0; face < 6; ++face) {
     int x0, x1, y0, y1;
     shadow_face_box(face, v, live, p.size, x0, x1, y0, y1);
     const int bw = x1 >= x0 ? x1 - x0 + 1 : 0, bh = y1 >= y0 ? y1 - y0 + 1 : 0;
     const int w = bh > 0 ? bw : 0, h = bw > 0 ? bh : 0;
     area += static_cast<unsigned long long>(w) * static_cast<unsigned long long>(h);
-    if (item < items)
-      p.slots.boxes[static_cast<size_t>(item) * 6 + face] =
-          make_uint2(static_cast<unsigned>(max(x0, 0)) | (static_cast<unsigned>(max(y0, 0)) << 16),
-                     static_cast<unsigned>(w) | (static_cast<unsigned>(h) << 16));
+    bx[2 * face] = static_cast<unsigned>(max(x0, 0)) | (static_cast<unsigned>(max(y0, 0)) << 16);
+    bx[2 * face + 1] = static_cast<unsigned>(w) | (static_cast<unsigned>(h) << 16);
+  }
+  // Only items with texels are ever looked up: the rest (back faces, boxes
+  // off every face) write no boxes.
+  if (item < items && area > 0) {
+    uint4* q = reinterpret_cast<uint4*>(p.slots.boxes + static_cast<size_t>(item) * 6);
+    q[0] = make_uint4(bx[0], bx[1], bx[2], bx[3]);
+    q[1] = make_uint4(bx[4], bx[5], bx[6], bx[7]);
+    q[2] = make_uint4(bx[8], bx[9], bx[10], bx[11]);
   }
   unsigned long long block_total;
   const unsigned long long excl = slot_block_scan(area, wave_total, block_total);
