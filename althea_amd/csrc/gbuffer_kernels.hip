@@ -48,6 +48,7 @@
 #include <vector>
 
 #include "device_math.hpp"
+#include "fill.hpp"
 #include "internal.hpp"
 #include "slot_scan.hpp"
 
@@ -264,11 +265,6 @@ __global__ __launch_bounds__(256) void gbuffer_offset_kernel(unsigned long long*
                                                              const unsigned long long* __restrict__ blocks, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) offsets[t] += blocks[t / kGbSetupThreads];
-}
-
-__global__ void gbuffer_clear_kernel(unsigned long long* __restrict__ vis, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) vis[i] = kGbEmpty;
 }
 
 // Ray of pixel (x, y) against triangle `tri`: the hit's ray parameter, or a
@@ -618,8 +614,9 @@ char* gb_common(GbParams& p, const althea_globals* g, const althea_gbuffer_view*
 althea_status gb_run(const GbParams& p, hipStream_t s) {
   const int n = p.width * p.height;
   const int T = p.triangle_count;
-  hipLaunchKernelGGL(gbuffer_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p.vis, n);
-  althea_status st = check_launch("gbuffer_clear_kernel");
+  hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(fill_blocks(static_cast<size_t>(n), 8)), dim3(256), 0, s,
+                     p.vis, static_cast<size_t>(n), kGbEmpty);
+  althea_status st = check_launch("fill_kernel<unsigned long long> (visibility clear)");
   if (st != ALTHEA_OK) return st;
   if (T > 0) {
     const int nb = static_cast<int>(gb_setup_blocks(T));

@@ -35,6 +35,7 @@
 #include <cmath>
 
 #include "device_math.hpp"
+#include "fill.hpp"
 #include "internal.hpp"
 #include "shadow.hpp"
 #include "slot_scan.hpp"
@@ -68,11 +69,6 @@ struct ShadowParams {
   double inv_triangles;  // 1 / triangle_count
   ShadowSlots slots;     // the scheduled path only
 };
-
-__global__ void shadow_clear_kernel(float* __restrict__ c, size_t n) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) c[i] = 1.0f;
-}
 
 // Texel (i, j) of face f: the ray direction R_f v (major component +-1) and
 // its length.
@@ -385,9 +381,9 @@ size_t shadow_workspace_bytes(int light_count, int triangle_count) {
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
                                   int size, float* cubes, hipStream_t s, void* workspace) {
   const size_t texels = static_cast<size_t>(light_count) * 6 * (size + 2) * (size + 2);
-  hipLaunchKernelGGL(shadow_clear_kernel, dim3(static_cast<unsigned>((texels + 255) / 256)), dim3(256), 0, s, cubes,
-                     texels);
-  althea_status st = check_launch("shadow_clear_kernel");
+  hipLaunchKernelGGL(fill_kernel<float>, dim3(fill_blocks(texels, sizeof(float))), dim3(256), 0, s, cubes, texels,
+                     1.0f);
+  althea_status st = check_launch("fill_kernel<float> (shadow clear)");
   if (st != ALTHEA_OK || mesh->triangle_count == 0) return st;  // borders of a cleared cube are 1 too
   ShadowParams p{};
   p.positions = mesh->positions;
