@@ -127,14 +127,19 @@ def test_large_occluders_span_whole_faces(torch, size):
     assert r["depth_off"] <= 2e-3 * r["hits"] + 2, r
 
 
-@pytest.mark.parametrize("case", ["heightfield", "box256", "box3", "mixed"])
+@pytest.mark.parametrize("case", ["heightfield", "box256", "box3", "mixed", "full"])
 def test_scheduled_path_is_bit_identical(torch, case):
     """althea_shadow_cubes_ws (texels dealt out evenly over the GPU) writes
     the cubes of althea_shadow_cubes bit for bit: the same texel tests and an
     order-free atomic minimum.  Cases: many small boxes, whole-face boxes,
-    boxes of one or two texels, and both mixed with culled and degenerate
-    triangles (zero slots)."""
-    if case == "heightfield":
+    boxes of one or two texels, both mixed with culled and degenerate
+    triangles (zero slots), and the full benchmark size (64 lights, 256^2
+    cubes, the 128 x 128-cell heightfield: 2 097 152 light-triangle pairs)."""
+    if case == "full":
+        sc = aa.synth.scene(seed=SEED0 + 37, width=8, height=8, lights=64)
+        pos, idx = aa.synth.heightfield_mesh(SEED0 + 37, n=128)
+        lights, size = sc.lights, 256
+    elif case == "heightfield":
         sc = aa.synth.scene(seed=SEED0 + 35, width=8, height=8, lights=9)
         pos, idx = aa.synth.heightfield_mesh(SEED0 + 35, n=64)
         lights, size = sc.lights, 96
