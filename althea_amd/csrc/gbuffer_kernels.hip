@@ -125,9 +125,13 @@ __device__ __forceinline__ f3 gb_vertex(const float* __restrict__ a, uint32_t i)
   return mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
 }
 
-// Primitive of triangle t (numbered over the draw).
+// Primitive of triangle t (numbered over the draw).  kTable: the draw passes
+// a device table (glTF draws always do), so the kernel argument's single
+// primitive is never referenced (a reference to it would copy the argument
+// to scratch memory).
+template <bool kTable = false>
 __device__ __forceinline__ const GbPrim& gb_prim(const GbParams& p, int t) {
-  if (!p.prims) return p.one;
+  if (!kTable && !p.prims) return p.one;
   int lo = 0, hi = p.prim_count;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -144,12 +148,18 @@ __device__ __forceinline__ void gb_indices(const GbPrim& pr, int lt, uint32_t vi
   for (int k = 0; k < 3; ++k) vi[k] = min(pr.indices[3 * lt + k], last);
 }
 
+// Texture coordinate set k of a primitive, selected without indexing the
+// array (a runtime index into it would place it in scratch memory).
+__device__ __forceinline__ const float* gb_uv_set(const GbPrim& pr, int k) {
+  return k == 0 ? pr.uvs[0] : (k == 1 ? pr.uvs[1] : (k == 2 ? pr.uvs[2] : pr.uvs[3]));
+}
+
 __device__ GbAlpha gb_alpha_record(const GbParams& p, int t) {
   GbAlpha a{};
-  const GbPrim& pr = gb_prim(p, t);
+  const GbPrim& pr = gb_prim<true>(p, t);
   if (pr.material < 0) return a;  // pass-through: texels stay null
   const althea_gltf_material& m = p.materials[pr.material];
-  const float2* uv = reinterpret_cast<const float2*>(pr.uvs[m.base_uv]);
+  const float2* uv = reinterpret_cast<const float2*>(gb_uv_set(pr, m.base_uv));
   uint32_t vi[3];
   gb_indices(pr, t - pr.tri_begin, vi);
   for (int k = 0; k < 3; ++k) a.uv[k] = uv ? uv[vi[k]] : make_float2(0.0f, 0.0f);
@@ -360,7 +370,7 @@ struct GbUv {
 };
 
 template <bool kAlphaOnly = false>
-__device__ float4 gb_sample(const althea_texture_view& t, const GbUv& c) {
+__device__ float4 gb_sample(const althea_texture_view& t, const GbUv c) {  // by value: no scratch copy
   const float W = static_cast<float>(t.width), H = static_cast<float>(t.height);
   const float ax = c.dudx * W, bx = c.dvdx * H, ay = c.dudy * W, by = c.dvdy * H;
   const float rho2 = fmaxf(fmaf(ax, ax, bx * bx), fmaf(ay, ay, by * by));
@@ -487,6 +497,9 @@ __device__ __forceinline__ f3 gb_interp(const float* __restrict__ a, const uint3
 // Pixels no triangle covers keep the clear colour (0, 0, 0, 1) of the
 // reference's attachments (DeferredRendering.cpp:274-275), with position
 // (0, 0, 0, 0): position.w is this G-buffer's coverage flag.
+// kGltf: the draw has glTF primitives (their material path is compiled in);
+// the pass-through instance keeps its registers for the plain attributes.
+template <bool kGltf>
 __global__ __launch_bounds__(256) void gbuffer_resolve_kernel(const GbParams p) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.width * p.height) return;
@@ -501,7 +514,7 @@ __global__ __launch_bounds__(256) void gbuffer_resolve_kernel(const GbParams p) 
   const int t = static_cast<int>(key & 0xffffffffu);
   const int x = i % p.width, y = i / p.width;
   const GbTri tri = p.tris[t];
-  const GbPrim& pr = gb_prim(p, t);
+  const GbPrim& pr = gb_prim<kGltf>(p, t);
   const int lt = t - pr.tri_begin;
   const f3 d = gb_ray(p, x, y);
   float e[3];
@@ -512,14 +525,14 @@ __global__ __launch_bounds__(256) void gbuffer_resolve_kernel(const GbParams p) 
   gb_indices(pr, lt, vi);
   const f3 pos = gb_interp(pr.positions, vi, l);
   gb_store(p.position_out, p.fp16, i, pos.x, pos.y, pos.z, 1.0f);
-  if (pr.material >= 0) {
+  if (kGltf && pr.material >= 0) {
     // Gltf.frag with fetchMaterial (InstanceData.glsl:27-63).
     const althea_gltf_material& m = p.materials[pr.material];
     float lx[3], ly[3];
     gb_bary(tri, gb_ray(p, x + 1, y), lx);
     gb_bary(tri, gb_ray(p, x, y + 1), ly);
-    const GbUv cb = gb_uv(pr.uvs[m.base_uv], vi, l, lx, ly);
-    const GbUv cm = gb_uv(pr.uvs[m.metallic_roughness_uv], vi, l, lx, ly);
+    const GbUv cb = gb_uv(gb_uv_set(pr, m.base_uv), vi, l, lx, ly);
+    const GbUv cm = gb_uv(gb_uv_set(pr, m.metallic_roughness_uv), vi, l, lx, ly);
     const float4 bt = gb_sample(p.textures[m.base_texture], cb);
     const float4 base = make_float4(bt.x * m.base_color_factor[0], bt.y * m.base_color_factor[1],
                                     bt.z * m.base_color_factor[2], bt.w * m.base_color_factor[3]);
@@ -629,7 +642,10 @@ althea_status gb_run(const GbParams& p, hipStream_t s) {
     hipLaunchKernelGGL(gbuffer_raster_kernel, dim3(kGbRasterBlocks), dim3(256), 0, s, p);
     if ((st = check_launch("gbuffer_raster_kernel")) != ALTHEA_OK) return st;
   }
-  hipLaunchKernelGGL(gbuffer_resolve_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p);
+  if (p.alpha_test)
+    hipLaunchKernelGGL(gbuffer_resolve_kernel<true>, dim3((n + 255) / 256), dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL(gbuffer_resolve_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, s, p);
   return check_launch("gbuffer_resolve_kernel");
 }
 
