@@ -1,5 +1,6 @@
 // C ABI entry points: argument validation, handle management and workspace
 // plumbing around the kernel launchers.  No exception crosses this boundary.
+#include <cstdint>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -550,9 +551,15 @@ size_t althea_gbuffer_workspace_bytes(int width, int height, int triangle_count)
   return gbuffer_workspace_bytes(width, height, triangle_count);
 }
 
+// Workspaces are carved into 256-byte aligned parts that the kernels access
+// with 16-byte stores and 64-bit atomics: the base must be 256-byte aligned,
+// as hipMalloc and torch allocations are.
+static bool workspace_aligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 255) == 0; }
+
 althea_status althea_gbuffer_draw(const althea_gbuffer_mesh* mesh, const althea_globals* globals,
                                   const althea_gbuffer_view* g, void* d_workspace, hipStream_t stream) {
   ALTHEA_REQUIRE(mesh && globals && g && d_workspace, "gbuffer_draw: null argument");
+  ALTHEA_REQUIRE(workspace_aligned(d_workspace), "gbuffer_draw: workspace not 256-byte aligned");
   ALTHEA_REQUIRE(g->width >= 1 && g->height >= 1 && static_cast<long long>(g->width) * g->height < (1ll << 31),
                  "gbuffer_draw: extent %dx%d", g->width, g->height);
   ALTHEA_REQUIRE(g->position && g->normal && g->albedo && g->mro, "gbuffer_draw: null G-buffer plane");
@@ -629,6 +636,7 @@ althea_status althea_gbuffer_draw_gltf(const althea_gltf_primitive* primitives, 
                                        const althea_globals* globals, const althea_gbuffer_view* g, void* d_workspace,
                                        hipStream_t stream) {
   ALTHEA_REQUIRE(globals && g && d_workspace, "gbuffer_draw_gltf: null argument");
+  ALTHEA_REQUIRE(workspace_aligned(d_workspace), "gbuffer_draw_gltf: workspace not 256-byte aligned");
   ALTHEA_REQUIRE(primitive_count >= 0 && (primitive_count == 0 || primitives), "gbuffer_draw_gltf: %d primitives",
                  primitive_count);
   ALTHEA_REQUIRE(material_count >= 0 && (material_count == 0 || materials) && texture_count >= 0 &&
@@ -696,6 +704,7 @@ althea_status althea_shadow_cubes_ws(const althea_mesh_view* mesh, const althea_
                                      hipStream_t stream) {
   ALTHEA_REQUIRE(mesh && d_cubes && light_count >= 1 && d_lights && d_workspace,
                  "shadow_cubes_ws: null argument or no light");
+  ALTHEA_REQUIRE(workspace_aligned(d_workspace), "shadow_cubes_ws: workspace not 256-byte aligned");
   ALTHEA_REQUIRE(size >= 1 && size <= 4096, "shadow_cubes_ws: size %d", size);
   ALTHEA_REQUIRE(mesh->triangle_count >= 0 && (mesh->triangle_count == 0 || (mesh->positions && mesh->indices &&
                                                                                mesh->vertex_count >= 1)),
@@ -733,6 +742,7 @@ althea_status althea_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   st = reflection_ok(reflection, g, "ssr_capture");
   if (st != ALTHEA_OK) return st;
   ALTHEA_REQUIRE(d_workspace, "ssr_capture: null workspace");
+  ALTHEA_REQUIRE(workspace_aligned(d_workspace), "ssr_capture: workspace not 256-byte aligned");
   return launch_ssr_capture(g, ibl, d_lights, light_count, globals, reflection, d_workspace, stream);
 }
 
@@ -747,6 +757,7 @@ althea_status althea_ssr_capture_shadowed(const althea_gbuffer_view* g, const al
   st = reflection_ok(reflection, g, "ssr_capture_shadowed");
   if (st != ALTHEA_OK) return st;
   ALTHEA_REQUIRE(d_workspace, "ssr_capture_shadowed: null workspace");
+  ALTHEA_REQUIRE(workspace_aligned(d_workspace), "ssr_capture_shadowed: workspace not 256-byte aligned");
   ALTHEA_REQUIRE(d_shadow_cubes && shadow_size >= 1 && shadow_size <= 4096,
                  "ssr_capture_shadowed: shadow cubes %p, size %d", static_cast<const void*>(d_shadow_cubes),
                  shadow_size);

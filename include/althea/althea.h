@@ -14,6 +14,9 @@
  *
  * Conventions
  *   - Every buffer is caller-owned device memory unless a handle says otherwise.
+ *   - A d_workspace argument is 256-byte aligned device memory (hipMalloc and
+ *     torch allocations are) of the size its *_workspace_bytes function gives;
+ *     a misaligned one is rejected with ALTHEA_ERROR_INVALID_ARGUMENT.
  *   - Every call is stream-ordered and asynchronous on the given hipStream_t
  *     (0 = the legacy default stream); results are ready when the stream is.
  *   - Errors come back as althea_status; no C++ exception crosses this ABI.

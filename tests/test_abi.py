@@ -113,6 +113,9 @@ def test_shadow_argument_checks_without_gpu():
     assert lib.althea_shadow_cubes_ws(C.byref(mesh), dummy, 1, 16, dummy, None, None) == 1
     assert b"null" in lib.althea_last_error()
     assert lib.althea_shadow_cubes_ws(C.byref(mesh), dummy, 1, 0, dummy, dummy, None) == 1
+    # Workspaces must be 256-byte aligned (16-byte stores, 64-bit atomics).
+    assert lib.althea_shadow_cubes_ws(C.byref(mesh), dummy, 1, 16, dummy, dummy + 8, None) == 1
+    assert b"256-byte aligned" in lib.althea_last_error()
     assert lib.althea_shadow_cubes_ws(C.byref(huge), dummy, 2, 16, dummy, dummy, None) == 1
     g = _lib.GBufferViewC(dummy, dummy, dummy, dummy, 64, 32, 0)
     v = _lib.IBLViewsC()
