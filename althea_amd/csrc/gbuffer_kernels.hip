@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kGbSetupThreads) void gbuffer_setup_kernel(const Gb
 // thread a contiguous run), and the total slot count into *total.
 __global__ __launch_bounds__(kGbScanThreads) void gbuffer_scan_kernel(unsigned long long* __restrict__ blocks, int n,
                                                                       unsigned long long* __restrict__ total) {
-  __shared__ unsigned long long sums[kGbScanThreads];
+  __shared__ unsigned long long sums[kGbScanThreads / 64];
   slot_scan_blocks(blocks, n, total, sums);
 }
 

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kSlotBlock) void shadow_slot_setup_kernel(const Sha
 
 __global__ __launch_bounds__(kSlotScan) void shadow_scan_kernel(unsigned long long* __restrict__ blocks, int n,
                                                                unsigned long long* __restrict__ total) {
-  __shared__ unsigned long long sums[kSlotScan];
+  __shared__ unsigned long long sums[kSlotScan / 64];
   slot_scan_blocks(blocks, n, total, sums);
 }
 

@@ -34,30 +34,54 @@ __device__ __forceinline__ unsigned long long slot_block_scan(unsigned long long
 }
 
 // Exclusive prefix sum of the n block totals in place by one 1024-thread
-// block (each thread a contiguous run), the total into *total.
+// block (each thread a contiguous run, read and written eight at a time so
+// the loads are in flight together), the total into *total.  The threads'
+// run sums are scanned by wave shuffles, then the 16 wave totals by one wave.
 __device__ __forceinline__ void slot_scan_blocks(unsigned long long* __restrict__ blocks, int n,
                                                  unsigned long long* __restrict__ total,
-                                                 unsigned long long* sums /* LDS [kSlotScan] */) {
-  const int tid = threadIdx.x;
+                                                 unsigned long long* sums /* LDS [kSlotScan / 64] */) {
+  constexpr int kWaves = kSlotScan / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int run = (n + kSlotScan - 1) / kSlotScan;
   const int b = min(tid * run, n), e = min(b + run, n);
   unsigned long long s = 0;
-  for (int i = b; i < e; ++i) s += blocks[i];
-  sums[tid] = s;
+  for (int i0 = b; i0 < e; i0 += 8) {
+    unsigned long long v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = i0 + k < e ? blocks[i0 + k] : 0ull;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  unsigned long long x = s;  // inclusive scan over the wave
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) sums[wave] = x;
   __syncthreads();
-  for (int d = 1; d < kSlotScan; d <<= 1) {  // inclusive Hillis-Steele scan
-    const unsigned long long add = tid >= d ? sums[tid - d] : 0ull;
-    __syncthreads();
-    sums[tid] += add;
-    __syncthreads();
+  if (wave == 0) {  // inclusive scan of the wave totals
+    unsigned long long w = lane < kWaves ? sums[lane] : 0ull;
+#pragma unroll
+    for (int d = 1; d < kWaves; d <<= 1) {
+      const unsigned long long y = __shfl_up(w, d, 64);
+      if (lane >= d) w += y;
+    }
+    if (lane < kWaves) sums[lane] = w;
   }
-  unsigned long long acc = sums[tid] - s;
-  for (int i = b; i < e; ++i) {
-    const unsigned long long v = blocks[i];
-    blocks[i] = acc;
-    acc += v;
+  __syncthreads();
+  unsigned long long acc = (wave > 0 ? sums[wave - 1] : 0ull) + x - s;  // this thread's first exclusive prefix
+  for (int i0 = b; i0 < e; i0 += 8) {
+    unsigned long long v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = i0 + k < e ? blocks[i0 + k] : 0ull;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (i0 + k < e) blocks[i0 + k] = acc;
+      acc += v[k];
+    }
   }
-  if (tid == kSlotScan - 1) *total = sums[tid];
+  if (tid == 0) *total = sums[kWaves - 1];
 }
 
 // Largest t in [lo, hi) with offsets[t] <= slot, given offsets[lo] <= slot <
