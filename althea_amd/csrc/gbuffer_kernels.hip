@@ -14,6 +14,7 @@
 // that is what these kernels compute.
 //
 // Passes (one stream):
+//   fill_kernel            the visibility keys set to empty;
 //   gbuffer_setup_kernel   one lane per triangle: edge normals, plane, the
 //                          pixel box of its part in front of half the near
 //                          plane, the box area and its prefix sum within the
@@ -22,15 +23,16 @@
 //   gbuffer_offset_kernel  block offset added to every triangle's slot;
 //   gbuffer_raster_kernel  a fixed grid of waves, each taking an equal
 //                          contiguous run of slots, 64 at a time: the wave
-//                          follows the triangles of its first and last slot
-//                          forward with wave-uniform galloping searches, each
-//                          lane finds its own within that range, casts its
+//                          finds the triangles of its 64 slots from a window
+//                          of 64 consecutive offsets by a binary search
+//                          across the lanes, each lane casts its
 //                          pixel's ray and keeps the
 //                          nearest hit as (depth bits << 32 | triangle) with a
 //                          64-bit atomic min, so big and small triangles cost
 //                          the same per pixel and the result does not depend
 //                          on the order of the writes;
-//   gbuffer_resolve_kernel one lane per pixel: the winning triangle's
+//   gbuffer_resolve_kernel one lane per pixel (an instance with and one
+//                          without the glTF material path): the winning triangle's
 //                          barycentrics at the pixel centre and the four
 //                          G-buffer planes, from the pass-through inputs or
 //                          the glTF material (textures sampled at the mip
