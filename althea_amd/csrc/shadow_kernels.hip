@@ -25,10 +25,14 @@
 // front and the points where its edges cross ma = near / 2), and writes each
 // hit with an integer atomicMin on
 // the depth's bits (non-negative floats order as their bit patterns), so the
-// result does not depend on the order of the writes.  Small boxes are walked
-// by their own lane; a wave walks larger boxes together, one lane's box at a
-// time.  A last pass fills each face's border from the adjacent faces (the
-// seamless filter's reads beyond the edge), so the lookup needs no folding.
+// result does not depend on the order of the writes.  Two schedules run the
+// same texel tests: with a workspace, every item's boxes are counted and
+// scanned and their texels dealt out evenly over a fixed grid of waves (the
+// shadow_slot_* kernels); without one, small boxes are walked by their own
+// lane and a wave walks larger boxes together, one lane's box at a time
+// (shadow_raster_kernel).  A last pass fills each face's border from the
+// adjacent faces (the seamless filter's reads beyond the edge), so the lookup
+// needs no folding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
