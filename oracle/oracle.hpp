@@ -11,7 +11,7 @@
 // stated at the function.
 //
 // Parity pins (what checks each function, besides the GPU comparison):
-//   hash_rng            hand-computed uint32 vectors (tests/test_oracle_core.py)
+//   hash_rng            literal words written out by hand (tests/test_oracle_core.py)
 //   hammersley          brute-force bit-loop radical inverse, all i < 2^16
 //   local_to_world      orthonormality / handedness invariants
 //   ggx sample + D      quadrature of D*cos = 1, alpha->0 identity, marginal CDF
@@ -19,10 +19,17 @@
 //   env pyramid         numpy box-mean of each level
 //   cube mapping        Vulkan face table round trip, seamless edge continuity
 //   irradiance          closed-form Riemann sum for constant radiance
-//   prefilter           alpha = 0 identity, constant env, dense-quadrature limit
+//   prefilter           alpha = 0 identity, constant env, dense-quadrature limit;
+//                       the source LOD by a one-sample prefilter of a
+//                       level-constant pyramid against the hand-derived LOD;
+//                       the reference's hash-RNG estimator (restated below)
+//                       within its Monte Carlo standard error
 //   brdf_lut            scipy quadrature of the split-sum integral, PNG fixture
-//   pbr_material        closed-form single light at normal incidence
-//   ssao                flat plane -> 1, fully covered point -> 0
+//   pbr_material        closed forms at metallic 1 / roughness 1, and with V and
+//                       L mirror-symmetric about N at roughness 0.5 (NdotH = 0)
+//   ssao                flat plane -> 1, fully covered point -> 0; the
+//                       reference's 24-ray march (restated below) beside it
+//   each quirk pin      fails when the quirk is flipped (ORACLE_MUTATION)
 //   ssr march           flat plane -> no hit; floor + wall scene -> hit at the
 //                       analytic reflection of each floor pixel
 //   glossy convolution  constant -> constant, linear ramp -> closed form,
@@ -53,6 +60,22 @@
 namespace oracle {
 
 constexpr double kPi = 3.14159265358979323846;
+
+// Deliberate single-quirk mutations, for tests/test_oracle_mutations.py only:
+// that test rebuilds this oracle with -DORACLE_MUTATION=k and checks that the
+// pin of quirk k fails.  The shipped oracle is built with 0 (none).
+//   1  direct-light half vector normalize(-V + L) instead of normalize(V + L)  (Q2)
+//   2  direct-light alpha = roughness instead of roughness^2                   (Q3)
+//   3  direct-light Fresnel at NdotV instead of NdotH                          (Q2)
+//   4  source LOD log2(saSample / saTexel) without the factor 0.5              (Q8)
+//   5  saTexel 4 pi / (W H) instead of 4 pi / (6 W H)                          (Q8)
+//   6  pdf denominator 4 HdotV without the + 0.00001                           (Q8)
+//   7  prefilter alpha = roughness^2 instead of roughness                      (Q4)
+//   8  hash RNG q.y >> 2 instead of q.y >> 3                                   (a1)
+//   9  hash RNG seed.yx not swapped                                           (a1)
+#ifndef ORACLE_MUTATION
+#define ORACLE_MUTATION 0
+#endif
 
 struct dvec3 {
   double x = 0, y = 0, z = 0;
@@ -96,9 +119,18 @@ struct HashRng {
   uint32_t nextu() {
     sx += 1u;
     sy += 1u;
+#if ORACLE_MUTATION == 9
+    uint32_t qx = 1103515245u * ((sx >> 1u) ^ sx);
+    uint32_t qy = 1103515245u * ((sy >> 1u) ^ sy);
+#else
     uint32_t qx = 1103515245u * ((sx >> 1u) ^ sy);
     uint32_t qy = 1103515245u * ((sy >> 1u) ^ sx);
+#endif
+#if ORACLE_MUTATION == 8
+    return 1103515245u * (qx ^ (qy >> 2u));
+#else
     return 1103515245u * (qx ^ (qy >> 3u));
+#endif
   }
   float nextf() { return static_cast<float>(nextu()) * (1.0f / 4294967296.0f); }
 };
@@ -462,8 +494,16 @@ inline dvec3 prefilter_dir(const Radiance& radiance, dvec3 raw_dir, double rough
                            int env_w, int env_h, double* weight_out = nullptr) {
   Frame fr = local_to_world(raw_dir);
   dvec3 N = fr.n, V = N;
+#if ORACLE_MUTATION == 7
+  double a = roughness * roughness, a2 = a * a;
+#else
   double a = roughness, a2 = a * a;
+#endif
+#if ORACLE_MUTATION == 5
+  double sa_texel = 4.0 * kPi / (1.0 * env_w * env_h);
+#else
   double sa_texel = 4.0 * kPi / (6.0 * env_w * env_h);
+#endif
   dvec3 acc;
   double total = 0.0;
   for (int i = 0; i < samples; ++i) {
@@ -477,9 +517,17 @@ inline dvec3 prefilter_dir(const Radiance& radiance, dvec3 raw_dir, double rough
     double lod = 0.0;
     if (roughness != 0.0) {
       double D = distribution_ggx(n_dot_h, a2);
+#if ORACLE_MUTATION == 6
+      double pdf = D * n_dot_h / (4.0 * h_dot_v);
+#else
       double pdf = D * n_dot_h / (4.0 * h_dot_v + 0.00001);
+#endif
       double sa_sample = 1.0 / (static_cast<double>(samples) * pdf + 0.0001);
+#if ORACLE_MUTATION == 4
+      lod = std::log2(sa_sample / sa_texel);
+#else
       lod = 0.5 * std::log2(sa_sample / sa_texel);
+#endif
     }
     if (n_dot_l > 0.0) {
       acc = acc + radiance(L, lod) * n_dot_l;
@@ -488,6 +536,72 @@ inline dvec3 prefilter_dir(const Radiance& radiance, dvec3 raw_dir, double rough
   }
   if (weight_out) *weight_out = total;
   return acc * (1.0 / total);
+}
+
+// ---------------------------------------------------------------------------
+// The reference's own prefilter estimator, for the statistical cross-check of
+// the Hammersley estimator above (not used by any parity test of the kernels).
+// PreFilterEnvMap.comp:126-177: output texel (x, y) of a w x h equirect mip
+// has uv = (x, y) / (w, h) without the half texel (Q6), yaw = pi (2u - 1),
+// pitch = pi (v - 0.5), N = (cos p cos y, sin p, cos p sin y); the hash RNG is
+// seeded with the texel position (:127) and draws xi = (rng(), rng()) per
+// GGX sample (:86), `samples` of them (MAX_SAMPLES 10000, :5), the source LOD
+// uses that count (:159).  FP64 apart from the RNG's FP32 conversion (Q17).
+// se_out (optional): the Monte Carlo standard error of the ratio estimator
+// per channel, sqrt(sum w^2 (f - mean)^2) / sum w.
+// ---------------------------------------------------------------------------
+inline dvec3 reference_texel_dir(int x, int y, int w, int h) {
+  double u = static_cast<double>(x) / w, v = static_cast<double>(y) / h;
+  double yaw = kPi * (2.0 * u - 1.0), pitch = kPi * (v - 0.5);
+  return {std::cos(pitch) * std::cos(yaw), std::sin(pitch), std::cos(pitch) * std::sin(yaw)};
+}
+template <class Radiance>
+inline dvec3 prefilter_reference_estimator(const Radiance& radiance, dvec3 N, double roughness, int samples,
+                                           uint32_t seed_x, uint32_t seed_y, int env_w, int env_h,
+                                           double* se_out = nullptr) {
+  Frame fr = local_to_world(N);
+  dvec3 V = fr.n;
+  double a2 = roughness * roughness;
+  double sa_texel = 4.0 * kPi / (6.0 * env_w * env_h);
+  HashRng rng{seed_x, seed_y};
+  std::vector<double> w, f[3];
+  for (int i = 0; i < samples; ++i) {
+    double x1 = rng.nextf(), x2 = rng.nextf();
+    double den = 1.0 + (a2 - 1.0) * x2;
+    if (!(den > 0.0)) continue;  // xi = 1 at a2 = 0: 0/0 in the shader, NdotL = max(NaN, 0) = 0
+    double cos_t = std::sqrt((1.0 - x2) / den);
+    double sin_t = std::sqrt(std::max(0.0, 1.0 - cos_t * cos_t));
+    double phi = 2.0 * kPi * x1;
+    dvec3 H = fr.to_world({std::cos(phi) * sin_t, std::sin(phi) * sin_t, cos_t});
+    dvec3 L = normalize(H * (2.0 * dot(V, H)) - V);
+    double n_dot_l = std::max(dot(fr.n, L), 0.0);
+    double n_dot_h = std::max(dot(fr.n, H), 0.0);
+    double h_dot_v = std::max(dot(H, V), 0.0);
+    double pdf = distribution_ggx(n_dot_h, a2) * n_dot_h / (4.0 * h_dot_v + 0.00001);
+    double sa_sample = 1.0 / (static_cast<double>(samples) * pdf + 0.0001);
+    double lod = roughness == 0.0 ? 0.0 : 0.5 * std::log2(sa_sample / sa_texel);
+    if (n_dot_l > 0.0) {
+      dvec3 c = radiance(L, lod);
+      w.push_back(n_dot_l);
+      f[0].push_back(c.x);
+      f[1].push_back(c.y);
+      f[2].push_back(c.z);
+    }
+  }
+  double sw = 0.0, mean[3] = {0, 0, 0};
+  for (size_t k = 0; k < w.size(); ++k) {
+    sw += w[k];
+    for (int c = 0; c < 3; ++c) mean[c] += w[k] * f[c][k];
+  }
+  for (int c = 0; c < 3; ++c) mean[c] /= sw;
+  if (se_out) {
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      for (size_t k = 0; k < w.size(); ++k) v += w[k] * w[k] * (f[c][k] - mean[c]) * (f[c][k] - mean[c]);
+      se_out[c] = std::sqrt(v) / sw;
+    }
+  }
+  return {mean[0], mean[1], mean[2]};
 }
 
 // ---------------------------------------------------------------------------
@@ -727,7 +841,11 @@ inline dvec3 pbr_material(dvec3 world_pos, dvec3 V, dvec3 N, dvec3 base, dvec3 r
                           const ShadowSet* shadows = nullptr, double* margin = nullptr) {
   double n_dot_v = std::max(dot(N, V * -1.0), 0.0);
   dvec3 F0 = lerp3(dvec3{0.04, 0.04, 0.04}, base, metallic);
+#if ORACLE_MUTATION == 2
+  double a = roughness;
+#else
   double a = roughness * roughness;
+#endif
   double a2 = a * a;
   double k_direct = (a + 1.0) * (a + 1.0) / 8.0;
   dvec3 kd = base * (1.0 - metallic);  // mix(baseColor, vec3(0), metallic)
@@ -747,10 +865,18 @@ inline dvec3 pbr_material(dvec3 world_pos, dvec3 V, dvec3 N, dvec3 base, dvec3 r
     L = L * (1.0 / d);
     if (shadows && dot(N, L) > 0.0 && shadow_hidden(*shadows, i, L, d, margin)) continue;
     dvec3 radiance = lights[i].emission * (1.0 / d2);
+#if ORACLE_MUTATION == 1
+    dvec3 H = normalize(L - V);
+#else
     dvec3 H = normalize(V + L);
+#endif
     double n_dot_l = std::max(dot(N, L), 0.0);
     double n_dot_h = std::max(dot(N, H), 0.0);
+#if ORACLE_MUTATION == 3
+    dvec3 F = fresnel_schlick(n_dot_v, F0, roughness);
+#else
     dvec3 F = fresnel_schlick(n_dot_h, F0, roughness);
+#endif
     dvec3 diffuse = mul(dvec3{1.0, 1.0, 1.0} - F, kd) * (1.0 / kPi);
     double s = ndf_ggx(n_dot_h, a2) * geometry_schlick_ggx(n_dot_v, k_direct) *
                geometry_schlick_ggx(n_dot_l, k_direct) / (4.0 * n_dot_l * n_dot_v + 0.0001);
@@ -896,6 +1022,59 @@ inline void rgba_bilinear(const float* img, int W, int H, double x, double y, do
 // texture(img, uv) with the LINEAR / CLAMP_TO_EDGE sampler.
 inline void rgba_sample_uv(const float* img, int W, int H, double u, double v, double out[4]) {
   rgba_bilinear(img, W, H, u * W - 0.5, v * H - 0.5, out);
+}
+
+// ---------------------------------------------------------------------------
+// The reference's own SSAO, Shaders/SSAO.glsl:29-85, kept as a cross-check of
+// the hemisphere kernel above (Q18), not used by any parity test of the
+// kernels.  Seeded with uvec2(gl_FragCoord.xy) = (px, py) (DeferredPass.frag:42);
+// 24 rays xi = (rng, rng, rng), direction LocalToWorld(N) normalize(2 xi.xy - 1,
+// xi.z); end point P + 0.5 dir projected to uvEnd; 12 steps at
+// mix(uvStart, uvEnd, i / 12), stopping outside [0, 1]; the G-buffer position
+// and normal read with the LINEAR / CLAMP_TO_EDGE sampler
+// (DeferredRendering.cpp:79-97, Sampler.h:16-28); a hit is a sign change of
+// (Q - P) . perpRef between consecutive steps (i > 0) with a world step of at
+// most 2 and a normal facing against the ray.  AO = 1 - hits / 24.
+// ---------------------------------------------------------------------------
+inline double ssao_reference(const float* position, const float* normal, int W, int H, int px, int py,
+                             dvec3 P, dvec3 N, const dmat4& view, const dmat4& proj) {
+  HashRng rng{static_cast<uint32_t>(px), static_cast<uint32_t>(py)};
+  Frame fr = local_to_world(N);
+  double uv0[2] = {(px + 0.5) / W, (py + 0.5) / H};
+  double ao = 0.0;
+  for (int ray = 0; ray < 24; ++ray) {
+    double x0 = rng.nextf(), x1 = rng.nextf(), x2 = rng.nextf();
+    dvec3 dir = fr.to_world(normalize({2.0 * x0 - 1.0, 2.0 * x1 - 1.0, x2}));
+    dvec3 end = P + dir * 0.5;
+    double e4[4] = {end.x, end.y, end.z, 1.0}, ve[4], clip[4];
+    view.apply(e4, ve);
+    proj.apply(ve, clip);
+    double uv1[2] = {0.5 * clip[0] / clip[3] + 0.5, 0.5 * clip[1] / clip[3] + 0.5};
+    dvec3 perp = normalize(cross(cross(dir, N), dir));
+    dvec3 prev_pos = P;
+    double prev_proj = 0.0;
+    for (int i = 0; i < 12; ++i) {
+      double t = i / 12.0;
+      double u = uv0[0] + (uv1[0] - uv0[0]) * t, v = uv0[1] + (uv1[1] - uv0[1]) * t;
+      if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) break;
+      double q[4];
+      rgba_sample_uv(position, W, H, u, v, q);
+      dvec3 cur{q[0], q[1], q[2]};
+      double cur_proj = dot(cur - P, perp);
+      double step = length(cur - prev_pos);
+      if (cur_proj * prev_proj < 0.0 && step <= 2.0 && i > 0) {
+        double nq[4];
+        rgba_sample_uv(normal, W, H, u, v, nq);
+        if (dot(normalize({nq[0], nq[1], nq[2]}), dir) < 0.0) {
+          ao += 1.0;
+          break;
+        }
+      }
+      prev_pos = cur;
+      prev_proj = cur_proj;
+    }
+  }
+  return 1.0 - ao / 24.0;
 }
 
 // Shaders/SSRGlossyConvolve.comp:22-55 for one texel (x, y) of a dw x dh

@@ -157,6 +157,55 @@ void oracle_irradiance_texels(void* env, int face_size, int theta_samples, const
   }
 }
 
+// Cross-check of the two prefilter estimators at the reference's equirect
+// output texels (x, y) of a w x h mip (PreFilterEnvMap.comp:126-177): the
+// reference's hash-RNG estimator with `ref_samples` (and its standard error)
+// and the Hammersley estimator with `ham_samples` at the same direction.
+void oracle_prefilter_crosscheck(void* env, const int* texels, int count, int w, int h, double roughness,
+                                 int ref_samples, int ham_samples, double* out_ref3, double* out_se3,
+                                 double* out_ham3) {
+  const OracleEnv* e = static_cast<OracleEnv*>(env);
+  auto rad = [&](dvec3 d, double lod) { return sample_env_precompute(e->pyr, d, lod); };
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < count; ++i) {
+    const int x = texels[2 * i], y = texels[2 * i + 1];
+    dvec3 N = reference_texel_dir(x, y, w, h);
+    dvec3 r = prefilter_reference_estimator(rad, N, roughness, ref_samples, static_cast<uint32_t>(x),
+                                            static_cast<uint32_t>(y), e->w, e->h, out_se3 + 3 * i);
+    dvec3 q = prefilter_dir(rad, N, roughness, ham_samples, e->w, e->h);
+    out_ref3[3 * i] = r.x; out_ref3[3 * i + 1] = r.y; out_ref3[3 * i + 2] = r.z;
+    out_ham3[3 * i] = q.x; out_ham3[3 * i + 1] = q.y; out_ham3[3 * i + 2] = q.z;
+  }
+}
+
+// Prefilter of a pyramid whose level l holds the constant 2^l in every
+// channel (a W x H source's levels, filled directly rather than blitted), at
+// raw directions: pins the source LOD of PreFilterEnvMap.comp:155-161 through
+// the real trilinear lookup, since the output is then a function of the LOD
+// alone.
+void oracle_prefilter_level_constant(const double* raw_dirs, int count, double roughness, int samples, int env_w,
+                                     int env_h, double* out3) {
+  Pyramid p;
+  p.levels = mip_count(env_w, env_h);
+  size_t total = 0;
+  for (int l = 0; l < p.levels; ++l) {
+    p.w.push_back(std::max(1, env_w >> l));
+    p.h.push_back(std::max(1, env_h >> l));
+    p.offset.push_back(total);
+    total += static_cast<size_t>(p.w[l]) * p.h[l];
+  }
+  p.rgba.assign(4 * total, 0.0);
+  for (int l = 0; l < p.levels; ++l)
+    for (size_t t = p.offset[l]; t < p.offset[l] + static_cast<size_t>(p.w[l]) * p.h[l]; ++t)
+      for (int c = 0; c < 4; ++c) p.rgba[4 * t + c] = std::ldexp(1.0, l);
+  auto rad = [&](dvec3 d, double lod) { return sample_env_precompute(p, d, lod); };
+  for (int i = 0; i < count; ++i) {
+    dvec3 r = prefilter_dir(rad, {raw_dirs[3 * i], raw_dirs[3 * i + 1], raw_dirs[3 * i + 2]}, roughness, samples,
+                            env_w, env_h);
+    out3[3 * i] = r.x; out3[3 * i + 1] = r.y; out3[3 * i + 2] = r.z;
+  }
+}
+
 // Analytic radiance fields for closed-form pins: kind 0 constant (p0,p1,p2),
 // kind 1 linear p . dir per channel (p[0..2] for r, p[3..5] g, p[6..8] b).
 static dvec3 analytic(int kind, const double* p, dvec3 d) {
@@ -296,6 +345,32 @@ void oracle_resolve_ex(const oracle_resolve_args* a, const int* pixels, int coun
 void oracle_resolve(const oracle_resolve_args* a, const int* pixels, int count, double* out3, double* out_ao,
                     double* out_margin) {
   oracle_resolve_ex(a, pixels, count, nullptr, out3, out_ao, out_margin, nullptr);
+}
+
+// The reference's SSAO (ssao_reference) and the hemisphere kernel's AO at
+// covered pixels[:, (x, y)]; uncovered pixels give NaN in both.
+void oracle_ssao_compare(const oracle_resolve_args* a, const int* pixels, int count, double* out_ref,
+                         double* out_kernel) {
+  std::vector<Light> lights = resolve_lights(a);
+  ReflChain chain;
+  ShadowSet shadows;
+  ResolveInputs in = resolve_inputs(a, lights, &chain, &shadows);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < count; ++i) {
+    const int px = pixels[2 * i], py = pixels[2 * i + 1];
+    const size_t idx = static_cast<size_t>(py) * in.W + px;
+    const float* p = in.position + 4 * idx;
+    const float* n = in.normal + 4 * idx;
+    if (p[3] == 0.0f) {
+      out_ref[i] = out_kernel[i] = std::nan("");
+      continue;
+    }
+    dvec3 P{p[0], p[1], p[2]};
+    dvec3 N = normalize({n[0], n[1], n[2]});
+    out_ref[i] = ssao_reference(in.position, in.normal, in.W, in.H, px, py, P, N, in.view, in.proj);
+    double margin;
+    out_kernel[i] = ssao(in.position, in.W, in.H, px, py, P, N, in.view, in.proj, in.ssao, &margin);
+  }
 }
 
 // Screen-space reflection of pixels[:, (x, y)]: out4 rgba, margin, hit step

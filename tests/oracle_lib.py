@@ -35,56 +35,65 @@ def lib() -> C.CDLL:
     if _L is None:
         if not ORACLE_PATH.exists():
             raise RuntimeError(f"{ORACLE_PATH} missing: run __graft_entry__.build()")
-        L = C.CDLL(str(ORACLE_PATH))
-        L.oracle_hash_rng.argtypes = [u32, u32, i32, vp, vp]
-        L.oracle_hammersley.argtypes = [u32, u32, u32, vp, vp, vp]
-        L.oracle_frame.argtypes = [vp, i32, vp]
-        L.oracle_ggx_sample.argtypes = [vp, i32, dbl, vp]
-        L.oracle_ggx_d.argtypes = [dbl, dbl]
-        L.oracle_ggx_d.restype = dbl
-        L.oracle_mip_count.argtypes = [i32, i32]
-        L.oracle_env_create.argtypes = [vp, i32, i32]
-        L.oracle_env_create.restype = vp
-        L.oracle_env_destroy.argtypes = [vp]
-        L.oracle_env_texels.argtypes = [vp]
-        L.oracle_env_texels.restype = C.c_size_t
-        L.oracle_env_pyramid.argtypes = [vp, vp]
-        L.oracle_env_lookup.argtypes = [vp, vp, vp, i32, i32, vp]
-        L.oracle_cube_texel_dirs.argtypes = [i32, vp, i32, vp]
-        L.oracle_cube_select.argtypes = [vp, i32, vp, vp]
-        L.oracle_cube_fold.argtypes = [i32, i32, i32, i32, vp]
-        L.oracle_cube_sample.argtypes = [vp, i32, i32, vp, vp, i32, vp]
-        L.oracle_prefilter_texels.argtypes = [vp, i32, i32, i32, vp, i32, vp]
-        L.oracle_prefilter_points.argtypes = [vp, i32, vp, i32, vp]
-        L.oracle_irradiance_texels.argtypes = [vp, i32, i32, vp, i32, vp]
-        L.oracle_prefilter_analytic.argtypes = [i32, vp, vp, i32, dbl, i32, i32, i32, vp]
-        L.oracle_irradiance_analytic.argtypes = [i32, vp, vp, i32, i32, i32, i32, vp]
-        L.oracle_brdf_lut.argtypes = [i32, i32, vp, i32, vp]
-        L.oracle_brdf_lut_point.argtypes = [dbl, dbl, i32, vp]
-        L.oracle_resolve.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp]
-        L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
-        L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
-        L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
-        L.oracle_gbuffer.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, vp]
-        L.oracle_gbuffer_gltf.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp, i32, vp, vp, vp]
-        L.oracle_texture_sample.argtypes = [vp, vp, i32, vp, vp]
-        L.oracle_texture_mips.argtypes = [vp, i32, i32, i32, i32]
-        L.oracle_shadow_cubes.argtypes = [vp, vp, i32, vp, i32, i32, vp]
-        L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
-        L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
-        L.oracle_mlp_param_count.restype = i32
-        L.oracle_mlp_encode.argtypes = [vp, i32, vp]
-        L.oracle_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
-        L.oracle_mlp_loss_grad.argtypes = [vp, vp, vp, i32, i32, vp]
-        L.oracle_mlp_loss_grad.restype = dbl
-        L.oracle_mlp_loss_grad64.argtypes = [vp, vp, vp, i32, vp]
-        L.oracle_mlp_loss_grad64.restype = dbl
-        L.oracle_adam.argtypes = [vp, vp, vp, vp, i32, i32, dbl, dbl, dbl, dbl]
-        L.oracle_bf16_round.argtypes = [C.c_float]
-        L.oracle_bf16_round.restype = C.c_float
-        L.oracle_max_threads.restype = i32
-        _L = L
+        _L = load(ORACLE_PATH)
     return _L
+
+
+def load(path) -> C.CDLL:
+    """An oracle library at `path` with its argument types declared (the
+    mutation tests load rebuilt variants next to the shipped one)."""
+    L = C.CDLL(str(path))
+    L.oracle_hash_rng.argtypes = [u32, u32, i32, vp, vp]
+    L.oracle_hammersley.argtypes = [u32, u32, u32, vp, vp, vp]
+    L.oracle_frame.argtypes = [vp, i32, vp]
+    L.oracle_ggx_sample.argtypes = [vp, i32, dbl, vp]
+    L.oracle_ggx_d.argtypes = [dbl, dbl]
+    L.oracle_ggx_d.restype = dbl
+    L.oracle_mip_count.argtypes = [i32, i32]
+    L.oracle_env_create.argtypes = [vp, i32, i32]
+    L.oracle_env_create.restype = vp
+    L.oracle_env_destroy.argtypes = [vp]
+    L.oracle_env_texels.argtypes = [vp]
+    L.oracle_env_texels.restype = C.c_size_t
+    L.oracle_env_pyramid.argtypes = [vp, vp]
+    L.oracle_env_lookup.argtypes = [vp, vp, vp, i32, i32, vp]
+    L.oracle_cube_texel_dirs.argtypes = [i32, vp, i32, vp]
+    L.oracle_cube_select.argtypes = [vp, i32, vp, vp]
+    L.oracle_cube_fold.argtypes = [i32, i32, i32, i32, vp]
+    L.oracle_cube_sample.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+    L.oracle_prefilter_texels.argtypes = [vp, i32, i32, i32, vp, i32, vp]
+    L.oracle_prefilter_points.argtypes = [vp, i32, vp, i32, vp]
+    L.oracle_irradiance_texels.argtypes = [vp, i32, i32, vp, i32, vp]
+    L.oracle_prefilter_analytic.argtypes = [i32, vp, vp, i32, dbl, i32, i32, i32, vp]
+    L.oracle_irradiance_analytic.argtypes = [i32, vp, vp, i32, i32, i32, i32, vp]
+    L.oracle_brdf_lut.argtypes = [i32, i32, vp, i32, vp]
+    L.oracle_brdf_lut_point.argtypes = [dbl, dbl, i32, vp]
+    L.oracle_resolve.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp]
+    L.oracle_resolve_ex.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
+    L.oracle_pbr_material.argtypes = [vp, dbl, dbl, dbl, vp, i32, vp, i32, vp]
+    L.oracle_ssr.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp, vp, vp, vp]
+    L.oracle_gbuffer.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, vp]
+    L.oracle_gbuffer_gltf.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp, i32, vp, vp, vp]
+    L.oracle_texture_sample.argtypes = [vp, vp, i32, vp, vp]
+    L.oracle_texture_mips.argtypes = [vp, i32, i32, i32, i32]
+    L.oracle_shadow_cubes.argtypes = [vp, vp, i32, vp, i32, i32, vp]
+    L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
+    L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
+    L.oracle_mlp_param_count.restype = i32
+    L.oracle_mlp_encode.argtypes = [vp, i32, vp]
+    L.oracle_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
+    L.oracle_mlp_loss_grad.argtypes = [vp, vp, vp, i32, i32, vp]
+    L.oracle_mlp_loss_grad.restype = dbl
+    L.oracle_mlp_loss_grad64.argtypes = [vp, vp, vp, i32, vp]
+    L.oracle_mlp_loss_grad64.restype = dbl
+    L.oracle_adam.argtypes = [vp, vp, vp, vp, i32, i32, dbl, dbl, dbl, dbl]
+    L.oracle_bf16_round.argtypes = [C.c_float]
+    L.oracle_bf16_round.restype = C.c_float
+    L.oracle_max_threads.restype = i32
+    L.oracle_prefilter_level_constant.argtypes = [vp, i32, dbl, i32, i32, i32, vp]
+    L.oracle_prefilter_crosscheck.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, vp]
+    L.oracle_ssao_compare.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp]
+    return L
 
 
 def _p(a: np.ndarray) -> int:
@@ -173,6 +182,14 @@ def prefilter_analytic(kind: int, params, raw_dirs, roughness: float, samples: i
     return out
 
 
+def prefilter_level_constant(raw_dirs, roughness: float, samples: int, env_w: int, env_h: int) -> np.ndarray:
+    """Prefilter over a pyramid whose level l is the constant 2^l."""
+    d = _c(raw_dirs, np.float64)
+    out = np.empty((d.shape[0], 3), np.float64)
+    lib().oracle_prefilter_level_constant(_p(d), d.shape[0], roughness, samples, env_w, env_h, _p(out))
+    return out
+
+
 def irradiance_analytic(kind: int, params, raw_dirs, env_w: int, env_h: int, theta: int = 300) -> np.ndarray:
     p = _c(params, np.float64)
     d = _c(raw_dirs, np.float64)
@@ -240,6 +257,17 @@ class Env:
         lib().oracle_prefilter_points(self._h, samples, _p(d), d.shape[0], _p(out))
         return out
 
+    def prefilter_crosscheck(self, texels, w: int, h: int, roughness: float, ref_samples: int = 10000,
+                             ham_samples: int = 10000):
+        """Reference hash-RNG estimator (value, standard error) and the
+        Hammersley estimator at the reference's equirect texels (x, y) of a
+        w x h mip."""
+        t = _c(texels, np.int32)
+        ref, se, ham = (np.empty((t.shape[0], 3), np.float64) for _ in range(3))
+        lib().oracle_prefilter_crosscheck(self._h, _p(t), t.shape[0], w, h, roughness, ref_samples, ham_samples,
+                                          _p(ref), _p(se), _p(ham))
+        return ref, se, ham
+
     def irradiance_texels(self, face_size: int, ids, theta: int = 300) -> np.ndarray:
         i = _c(ids, np.int32)
         out = np.empty((i.shape[0], 3), np.float64)
@@ -296,6 +324,20 @@ def resolve(scene, views: dict, env: Env, pixels, tonemap: bool = False, gbuffer
     lib().oracle_resolve_ex(C.byref(a), _p(px), px.shape[0], _p(r_in) if r_in is not None else None, _p(rgb),
                             _p(ao), _p(margin), _p(q) if q is not None else None)
     return (rgb, ao, margin, q) if query else (rgb, ao, margin)
+
+
+def ssao_compare(scene, pixels):
+    """AO of the reference's 24-ray march (SSAO.glsl:29-85) and of the
+    hemisphere kernel at pixels[:, (x, y)]; NaN where uncovered."""
+    keep = []
+    views = dict(prefiltered=np.zeros((6, 4), np.float32), pref_size=1, pref_mips=1,
+                 irradiance=np.zeros((6, 4), np.float32), irr_size=1, lut=np.zeros((1, 2), np.float32), lut_size=1)
+    a = _resolve_args(scene, views, None, False, None, None, keep)
+    px = _c(pixels, np.int32)
+    ref = np.empty(px.shape[0], np.float64)
+    ker = np.empty(px.shape[0], np.float64)
+    lib().oracle_ssao_compare(C.byref(a), _p(px), px.shape[0], _p(ref), _p(ker))
+    return ref, ker
 
 
 def gbuffer(scene, positions, indices, colors, mro, normals=None, pixels=None):

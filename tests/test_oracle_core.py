@@ -11,25 +11,35 @@ from tests import oracle_lib as ol
 
 
 # ------------------------------------------------------------ a1 hash RNG
-def _hash_rng_py(sx, sy, count):
-    """Independent uint32 statement of Shaders/Misc/Sampling.glsl:13-24."""
-    M = 0xFFFFFFFF
-    out = []
-    for _ in range(count):
-        sx = (sx + 1) & M
-        sy = (sy + 1) & M
-        qx = (1103515245 * ((sx >> 1) ^ sy)) & M
-        qy = (1103515245 * ((sy >> 1) ^ sx)) & M
-        out.append((1103515245 * (qx ^ (qy >> 3))) & M)
-    return np.array(out, np.uint32)
+# Literal words of Shaders/Misc/Sampling.glsl:13-24 (rngu), written out once
+# by hand and kept as constants.  First word of seed (0, 0), step by step:
+#   seed += 1                     -> (1, 1)
+#   (seed >> 1) ^ seed.yx         -> (0 ^ 1, 0 ^ 1) = (1, 1)
+#   q = 0x41C64E6D * (1, 1)       -> (0x41C64E6D, 0x41C64E6D)   (1103515245)
+#   q.y >> 3                      -> 0x0838C9CD
+#   q.x ^ (q.y >> 3)              -> 0x49FE87A0
+#   n = 0x41C64E6D * 0x49FE87A0   -> 0x94727F20  (mod 2^32)
+HASH_RNG_WORDS = {
+    (0, 0): [0x94727F20, 0x21C1E403, 0x6AAB4CAD, 0xE54A1673],
+    (1, 2): [0xE030B4CA, 0x0114F97C, 0x6FC823E2, 0x6227B801],
+    (0xFFFFFFFF, 7): [0x35899006, 0x3A4176FD, 0xA95F8670, 0xF3026A97],  # seed.x wraps to 0
+    (640, 360): [0xCA43121B, 0x38743C7A, 0x5F36E839, 0x50E0C893],
+}
 
 
-@pytest.mark.parametrize("seed", [(0, 0), (1, 2), (123456, 789), (0xFFFFFFFF, 7)])
-def test_hash_rng_bit_exact(seed):
-    u, f = ol.hash_rng(seed[0], seed[1], 64)
-    np.testing.assert_array_equal(u, _hash_rng_py(seed[0], seed[1], 64))
-    # float(n) * (1 / float(0xffffffff)) == float32(n) * 2^-32 exactly (Q17)
-    np.testing.assert_array_equal(f, u.astype(np.float32) * np.float32(2.0 ** -32))
+def pin_hash_rng_words():
+    for (sx, sy), words in HASH_RNG_WORDS.items():
+        u, f = ol.hash_rng(sx, sy, len(words))
+        np.testing.assert_array_equal(u, np.array(words, np.uint32), err_msg=f"seed {(sx, sy)}")
+        # float(n) * (1 / float(0xffffffff)) == float32(n) * 2^-32 exactly (Q17)
+        np.testing.assert_array_equal(f, u.astype(np.float32) * np.float32(2.0 ** -32))
+    # seed (0, 0): xi = 0.5798720717430115, 0.1318647861480713, ...
+    _, f = ol.hash_rng(0, 0, 2)
+    assert f.tolist() == [0.5798720717430115, 0.1318647861480713]
+
+
+def test_hash_rng_literal_words():
+    pin_hash_rng_words()
 
 
 def test_hash_rng_can_return_one():
@@ -309,6 +319,80 @@ def test_prefilter_linear_field_matches_quadrature(rough):
     np.testing.assert_allclose(out[:, 0], expect, atol=2e-3 * np.linalg.norm(a))
 
 
+# Source LOD (Q8, PreFilterEnvMap.comp:155-161) through a one-sample prefilter.
+# Hammersley point 0 is (0, 0): cos(theta_h) = 1, so H = N = V = L, NdotL = 1
+# and the output is the source at that sample's LOD alone.  Over a pyramid
+# whose level l is the constant 2^l, that is lerp(2^l0, 2^l1, frac) of the
+# LOD.  By hand, with alpha = roughness (Q4), a2 = r^2:
+#   D(NdotH = 1) = a2 / (pi (a2 - 1 + 1)^2) = 1 / (pi a2)
+#   pdf = D NdotH / (4 HdotV + 0.00001) = 1 / (pi a2 4.00001)
+#   saSample = 1 / (1 * pdf + 0.0001),  saTexel = 4 pi / (6 W H)
+#   lod = 0.5 log2(saSample / saTexel), clamped to [0, levels - 1]
+def _q8_expected(r, w, h):
+    a2 = r * r
+    pdf = 1.0 / (math.pi * a2 * 4.00001)
+    sa_sample = 1.0 / (pdf + 0.0001)
+    sa_texel = 4.0 * math.pi / (6.0 * w * h)
+    lod = 0.5 * math.log2(sa_sample / sa_texel)
+    levels = 1 + int(math.floor(math.log2(max(w, h))))
+    lod = min(max(lod, 0.0), levels - 1.0)
+    l0 = int(math.floor(lod))
+    l1 = min(l0 + 1, levels - 1)
+    t = lod - l0
+    return 2.0 ** l0 * (1 - t) + 2.0 ** l1 * t, lod
+
+
+def pin_prefilter_source_lod():
+    dirs = np.array([[0.3, 0.8, -0.5], [1.0, 0.0, 0.0], [-0.2, -0.9, 0.4]])
+    # r = 0.25 / 0.5 / 1 at 512x256 put the LOD at 7.29, 8.29 and 9 (the
+    # clamp); 0.05 at 64x32 gives 1.11.  Roughness 0 has LOD 0 by definition.
+    for r, w, h in [(0.25, 512, 256), (0.5, 512, 256), (1.0, 512, 256), (0.05, 64, 32), (0.35, 4096, 2048)]:
+        expect, lod = _q8_expected(r, w, h)
+        got = ol.prefilter_level_constant(dirs, r, 1, w, h)
+        np.testing.assert_allclose(got, expect, rtol=1e-12, err_msg=f"r={r} {w}x{h} lod={lod}")
+    np.testing.assert_allclose(ol.prefilter_level_constant(dirs, 0.0, 1, 512, 256), 1.0, rtol=0)
+
+
+def test_prefilter_source_lod_one_sample():
+    pin_prefilter_source_lod()
+
+
+def test_q8_expected_lods_are_interior():
+    # The cases above exercise both mips of the blend, not only the clamp.
+    lods = [_q8_expected(r, w, h)[1] for r, w, h in [(0.25, 512, 256), (0.5, 512, 256), (0.05, 64, 32)]]
+    assert all(0 < lod < 9 and lod != math.floor(lod) for lod in lods), lods
+
+
+# The reference's own estimator (PreFilterEnvMap.comp:126-177: hash RNG seeded
+# by the texel, MAX_SAMPLES 10 000, equirect output directions at
+# uv = texelPos / size, Q6) against the Hammersley estimator at the same
+# directions and sample count, on the config-0 source (512x256 log-normal +
+# suns) at the reference's prefiltered mip sizes (W >> (i+1), roughness i/4,
+# ImageBasedLighting.cpp:192-203, :383).  Both estimate the same integral; the
+# reference's is pseudo-random, so they must agree within its Monte Carlo
+# standard error: z = (hammersley - reference) / SE has E[z^2] = 1.
+def pin_hammersley_vs_reference_estimator():
+    import althea_amd as aa
+
+    env = ol.Env(aa.synth.env(seed=0x414C0000, width=512, height=256))
+    rng = np.random.default_rng(5)
+    for m in range(1, 5):
+        r = m / 4.0
+        w, h = 512 >> (m + 1), 256 >> (m + 1)
+        tex = np.stack([rng.integers(0, w, 40), rng.integers(1, h - 1, 40)], 1)
+        ref, se, ham = env.prefilter_crosscheck(tex, w, h, r, 10000, 10000)
+        z = (ham - ref) / se
+        # 120 values (40 texels x 3 channels, channels correlated): the mean of
+        # z^2 within [0.5, 1.6] and no |z| beyond 4.5.
+        assert 0.5 < float(np.mean(z * z)) < 1.6 and float(np.abs(z).max()) < 4.5, (r, np.mean(z * z), z)
+        # The Monte Carlo error itself is a few percent at 10 000 samples.
+        assert 0.005 < float(np.median(se / ref)) < 0.1
+
+
+def test_hammersley_estimator_agrees_with_reference_estimator():
+    pin_hammersley_vs_reference_estimator()
+
+
 # ------------------------------------------------------------ a10 BRDF LUT
 def _split_sum_quadrature(nv, r):
     alpha = r * r
@@ -413,6 +497,36 @@ def test_pbr_material_light_below_horizon_contributes_nothing():
     np.testing.assert_array_equal(out, 0.0)
 
 
+# Direct light with V and L mirror-symmetric about N (PBRMaterial.glsl:117-159).
+# N = +z, the light at (-3, 0, 4) (exact in FP32, distance 5), so
+# L = (-0.6, 0, 0.8), and the eye on the mirror side: V = camera -> surface =
+# (-0.6, 0, -0.8).  Then V + L = (-1.2, 0, 0), so the reference's
+# H = normalize(V + L) (:145, Q2) is perpendicular to N: NdotH = 0.  By hand at
+# roughness 0.5, metallic 0, F0 = 0.04:
+#   alpha = r^2 = 0.25, a2 = 0.0625 (Q3);  D = a2 / (pi (0 (a2 - 1) + 1)^2) = 0.0625 / pi
+#   F(NdotH = 0) = F0 + (max(1 - r, F0) - F0) (1 - 0)^5 = 0.5            (:152, :51-53)
+#   kDirect = (alpha + 1)^2 / 8 = 1.5625 / 8 = 0.1953125                 (:96)
+#   NdotV = NdotL = 0.8;  g = 0.8 / (0.8 (1 - k) + k) = 0.8 / 0.8390625
+#   spec = D g^2 F / (4 * 0.8 * 0.8 + 1e-4),  diffuse = (1 - F) base / pi
+#   color = (diffuse + spec) (E / d^2) NdotL,  E = 10, d^2 = 25
+def pin_pbr_material_mirror_symmetric():
+    base = np.array([0.8, 0.5, 0.2])
+    V = np.array([-0.6, 0.0, -0.8])
+    N = np.array([0.0, 0.0, 1.0])
+    light = np.array([-3.0, 0.0, 4.0, 0, 10.0, 10.0, 10.0, 0], np.float32)
+    out = ol.pbr_material(np.zeros(3), V, N, base, np.zeros(3), np.zeros(3), 0.0, 0.5, 1.0, _lut_const(0, 0), 4,
+                          light)
+    g = 0.8 / 0.8390625
+    spec = (0.0625 / math.pi) * g * g * 0.5 / (4 * 0.8 * 0.8 + 1e-4)
+    diffuse = 0.5 * base / math.pi
+    expect = (diffuse + spec) * (10.0 / 25.0) * 0.8
+    np.testing.assert_allclose(out, expect, rtol=1e-12)
+
+
+def test_pbr_material_mirror_symmetric_closed_form():
+    pin_pbr_material_mirror_symmetric()
+
+
 # ------------------------------------------------- a13/a14 SSAO and resolve
 def _views_const(pref=1.0, irr=1.0, lut=(0.5, 0.1)):
     import althea_amd as aa
@@ -436,6 +550,36 @@ def test_ssao_flat_plane_is_unoccluded():
     pix = np.stack([xs[pick], ys[pick]], 1)
     _, ao, _ = ol.resolve(sc, _views_const(), env, pix)
     np.testing.assert_array_equal(ao, 1.0)
+
+
+def test_reference_ssao_march_beside_hemisphere_kernel():
+    """SSAO.glsl:29-85 (24 hashed rays x 12 screen steps, Q18) restated as a
+    cross-check of the hemisphere kernel that replaces it.  On the camera-facing
+    plane no ray changes side of its reference plane, so both give 1.  On the
+    config-0 frame (1280x720 fBm heightfield) they are different estimators of
+    occlusion: measured means 0.928 (reference) and 0.990 (kernel), Pearson
+    correlation 0.43 over 2 887 covered pixels.  The test keeps them positively
+    associated and both in [0, 1]."""
+    import althea_amd as aa
+
+    flat = aa.synth.scene(seed=3, width=96, height=64, lights=2, amplitude=0.0, eye=(0.0, 10.0, 0.01))
+    ys, xs = np.nonzero(flat.position[..., 3] > 0)
+    pick = np.random.default_rng(0).choice(len(xs), 200, replace=False)
+    ref, ker = ol.ssao_compare(flat, np.stack([xs[pick], ys[pick]], 1))
+    np.testing.assert_array_equal(ref, 1.0)
+    np.testing.assert_array_equal(ker, 1.0)
+
+    sc = aa.synth.scene(seed=0x414C0000, width=1280, height=720, lights=8)
+    rng = np.random.default_rng(3)
+    pix = np.stack([rng.integers(0, 1280, 4000), rng.integers(0, 720, 4000)], 1)
+    ref, ker = ol.ssao_compare(sc, pix)
+    ok = np.isfinite(ref)
+    assert np.array_equal(ok, np.isfinite(ker)) and ok.mean() > 0.5
+    r, k = ref[ok], ker[ok]
+    assert r.min() >= 0.0 and r.max() <= 1.0 and k.min() >= 0.0 and k.max() <= 1.0
+    assert np.all(np.isclose(r * 24.0, np.round(r * 24.0)))  # 1 - hits / 24
+    assert np.corrcoef(r, k)[0, 1] > 0.3, np.corrcoef(r, k)[0, 1]
+    assert 0.85 < r.mean() < 0.98 and 0.97 < k.mean() <= 1.0, (r.mean(), k.mean())
 
 
 def test_ssao_covered_point_closed_form():
