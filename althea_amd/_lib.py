@@ -11,6 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libalthea_amd.so"
 
 ALTHEA_OK = 0
+ALTHEA_ERROR_INVALID_ARGUMENT = 1
 PRECOMPUTE_FP32_SUMS = 0x1
 RESOLVE_TONEMAP = 0x1
 RESOLVE_OUT_FP16 = 0x2
@@ -127,6 +128,8 @@ PROTOTYPES = {
     "althea_cube_sample": (i32, [vp, i32, i32, vp, i32, vp, vp]),
     "althea_env_sample": (i32, [vp, i32, i32, vp, i32, vp, vp]),
     "althea_prefilter_cube": (i32, [vp, i32, i32, i32, i32, i32, C.c_uint, vp, vp]),
+    "althea_prefilter_cube_scratch_bytes": (sz, [i32, i32]),
+    "althea_prefilter_cube_noalloc": (i32, [vp, i32, i32, i32, i32, i32, C.c_uint, vp, vp]),
     "althea_prefilter_points": (i32, [vp, i32, i32, i32, vp, i32, vp, vp]),
     "althea_irradiance_cube": (i32, [vp, i32, i32, i32, i32, C.c_uint, vp, vp]),
     "althea_brdf_lut": (i32, [i32, i32, vp, vp]),

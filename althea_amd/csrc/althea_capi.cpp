@@ -226,6 +226,28 @@ althea_status althea_prefilter_cube(const float* d_pyramid, int env_width, int e
                                stream);
 }
 
+size_t althea_prefilter_cube_scratch_bytes(int mips, int samples) {
+  if (mips <= 0 || samples <= 0) return 0;
+  return prefilter_workspace_bytes(0, 0, 0, mips, samples);
+}
+
+althea_status althea_prefilter_cube_noalloc(const float* d_pyramid, int env_width, int env_height, int size,
+                                            int mips, int samples, unsigned flags, float* d_cube,
+                                            hipStream_t stream) {
+  ALTHEA_REQUIRE(d_pyramid && d_cube, "prefilter_cube_noalloc: null buffer");
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_cube_noalloc: %dx%d is not a power-of-two extent",
+                 env_width, env_height);
+  ALTHEA_REQUIRE(cube_ok(size, mips), "prefilter_cube_noalloc: size %d / mips %d unsupported", size, mips);
+  ALTHEA_REQUIRE(samples_ok(samples), "prefilter_cube_noalloc: samples %d out of range", samples);
+  const size_t need = prefilter_workspace_bytes(env_width, env_height, size, mips, samples);
+  ALTHEA_REQUIRE(need <= ALTHEA_PREFILTER_SCRATCH_MAX_BYTES,
+                 "prefilter_cube_noalloc: %d mips x %d samples need %zu bytes of sample tables, more than the "
+                 "fixed %u-byte array (use althea_prefilter_cube)",
+                 mips, samples, need, ALTHEA_PREFILTER_SCRATCH_MAX_BYTES);
+  return launch_prefilter_cube_noalloc(d_pyramid, env_width, env_height, size, mips, samples, flags, d_cube,
+                                       stream);
+}
+
 althea_status althea_prefilter_points(const float* d_pyramid, int env_width, int env_height, int samples,
                                       const float* d_dir_rough, int count, float* d_out, hipStream_t stream) {
   ALTHEA_REQUIRE(count >= 0, "prefilter_points: negative count");

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "device_math.hpp"
 #include "internal.hpp"
@@ -272,6 +273,49 @@ althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, Ggx
 size_t prefilter_workspace_bytes(int, int, int, int mips, int samples) {
   return align256(sizeof(GgxEntry) * static_cast<size_t>(mips) * samples) + align256(sizeof(int) * mips) +
          align256(sizeof(double) * mips) + align256(sizeof(float) * mips);
+}
+
+// ------------------------------------------- allocation-free prefilter
+// The sample tables of althea_prefilter_cube_noalloc: a fixed device array,
+// one copy per GPU, held until the process ends.  In host code `scratch`
+// names a host-side stand-in, so its device address is looked up once per
+// device with hipGetSymbolAddress and only that address is handed to kernels.
+static_assert(ALTHEA_PREFILTER_SCRATCH_MAX_BYTES % 256 == 0, "scratch size");
+static __device__ __align__(256) unsigned char scratch[ALTHEA_PREFILTER_SCRATCH_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device.
+static althea_status prefilter_scratch(void** out) {
+  int dev = 0;
+  ALTHEA_TRY_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("prefilter_cube_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+    return ALTHEA_ERROR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    ALTHEA_TRY_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return ALTHEA_OK;
+}
+
+// Every table entry, count and weight sum the prefilter reads is written by
+// the table kernel of the same call before the texel kernel runs, so nothing
+// left in the array by an earlier call is read and nothing needs clearing.
+althea_status launch_prefilter_cube_noalloc(const float* pyr, int env_w, int env_h, int size, int mips,
+                                            int samples, unsigned flags, float* cube, hipStream_t s) {
+  void* ws = nullptr;
+  althea_status st = prefilter_scratch(&ws);
+  if (st != ALTHEA_OK) return st;
+  return launch_prefilter_cube(pyr, env_w, env_h, size, mips, samples, flags, cube, ws, s);
 }
 
 althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,

@@ -60,6 +60,9 @@ althea_status launch_pyramid(const float* level0, int w, int h, float* pyr, hipS
 althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,
                                     unsigned flags, float* cube, void* workspace, hipStream_t s);
 size_t prefilter_workspace_bytes(int env_w, int env_h, int size, int mips, int samples);
+// The same prefilter with its tables in the library's fixed device array.
+althea_status launch_prefilter_cube_noalloc(const float* pyr, int env_w, int env_h, int size, int mips,
+                                            int samples, unsigned flags, float* cube, hipStream_t s);
 althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, int samples,
                                       const float* dir_rough, int count, float* out, void* workspace,
                                       hipStream_t s);

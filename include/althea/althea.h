@@ -126,6 +126,25 @@ ALTHEA_API althea_status althea_prefilter_cube(const float* d_pyramid, int env_w
                                                int size, int mips, int samples, unsigned flags,
                                                float* d_cube, hipStream_t stream);
 
+/* Largest sample-table scratch althea_prefilter_cube_noalloc accepts:
+ * althea_prefilter_cube_scratch_bytes(mips, samples) must not exceed it
+ * (4 MiB: about 32 B per mip and sample, e.g. 6 mips x 16384 samples). */
+#define ALTHEA_PREFILTER_SCRATCH_MAX_BYTES (4u << 20)
+ALTHEA_API size_t althea_prefilter_cube_scratch_bytes(int mips, int samples);
+
+/* althea_prefilter_cube without any device allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-roughness GGX sample tables live in a fixed 4 MiB device array that the
+ * library holds on every GPU it runs on until the process ends; requests whose
+ * tables exceed ALTHEA_PREFILTER_SCRATCH_MAX_BYTES are rejected with
+ * ALTHEA_ERROR_INVALID_ARGUMENT before any device work.  The cube written is
+ * bit-identical to althea_prefilter_cube's.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+ALTHEA_API althea_status althea_prefilter_cube_noalloc(const float* d_pyramid, int env_width, int env_height,
+                                                       int size, int mips, int samples, unsigned flags,
+                                                       float* d_cube, hipStream_t stream);
+
 /* Prefiltered radiance at arbitrary points: d_dir_rough float4 (dir.xyz,
  * roughness) per point, d_out float4.  Used for the learned surrogate's labels. */
 ALTHEA_API althea_status althea_prefilter_points(const float* d_pyramid, int env_width, int env_height,
