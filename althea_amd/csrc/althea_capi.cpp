@@ -466,7 +466,7 @@ althea_status althea_deferred_resolve_mlp(const althea_gbuffer_view* g, const al
   if (st != ALTHEA_OK) return st;
   ALTHEA_REQUIRE(mlp, "deferred_resolve_mlp: null MLP");
   MlpInferenceView v{};
-  st = mlp_inference_view(mlp, &v);
+  st = mlp_resolve_queue(mlp, static_cast<size_t>(g->width) * g->height, &v);
   if (st != ALTHEA_OK) return st;
   ALTHEA_REQUIRE(v.mon_mips > 0 && v.cell_ok, "deferred_resolve_mlp: the MLP's error monitor is not built");
   ALTHEA_REQUIRE(v.mon_size == ibl->prefiltered_size && v.mon_mips == ibl->prefiltered_mips,

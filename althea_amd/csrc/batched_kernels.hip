@@ -72,12 +72,18 @@ __global__ __launch_bounds__(256) void prefilter_batch_kernel(const BatchMap* __
     const int x = (t % tpr) * kTW + (lane % kTW);
     const int y = (t / tpr) * kTH + (lane / kTW);
     const bool active = x < S && y < S;
-    const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
     const int set_mip = mp.table_set * p.mips + m;
     const LevelC* lv = levels + mp.level_base;
-    const float4 out = prefilter_texel<kF64Sums>(
-        mp.pyr, [&](int l) -> const LevelC& { return lv[l]; }, tables + static_cast<size_t>(set_mip) * p.samples,
-        counts[set_mip], wsum[set_mip], fr);
+    const auto level = [&](int l) -> const LevelC& { return lv[l]; };
+    float4 out;
+    if (m == 0 || p.mips == 1) {
+      // roughness 0: the mirror lookup, as the single-map kernel
+      out = prefilter_texel_mirror(mp.pyr, level, face, min(x, S - 1), min(y, S - 1), S);
+    } else {
+      const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+      out = prefilter_texel<kF64Sums>(mp.pyr, level, tables + static_cast<size_t>(set_mip) * p.samples,
+                                      counts[set_mip], wsum[set_mip], fr);
+    }
     if (active) {
       size_t off = 0;
       for (int k = 0; k < m; ++k) {

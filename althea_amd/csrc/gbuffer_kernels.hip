@@ -583,9 +583,6 @@ size_t gbuffer_workspace_bytes(int width, int height, int triangle_count) {
          align256(gb_setup_blocks(triangle_count) * 8);
 }
 
-void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]);
-void view_projection(const althea_globals& g, float vp[16]);
-
 namespace {
 
 // Camera, planes and the workspace's fixed parts (keys, setup, offsets,

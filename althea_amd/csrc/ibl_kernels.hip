@@ -19,6 +19,7 @@
 
 #include "device_math.hpp"
 #include "internal.hpp"
+#include "precise_lookup.hpp"
 #include "prefilter_common.hpp"
 
 namespace althea {
@@ -236,11 +237,16 @@ __global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restri
   const int x = (t % tpf) * kPrefilterTile + (threadIdx.x & 7);
   const int y = (t / tpf) * kPrefilterTile + (threadIdx.x >> 3);
   const bool active = x < S && y < S;
-  const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
-
-  const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
-  const float4 out = prefilter_texel<kF64Sums>(
-      pyr, [&](int l) -> const LevelC& { return p.lv[l]; }, tab, counts[m], wsum[m], fr);
+  const auto level = [&](int l) -> const LevelC& { return p.lv[l]; };
+  float4 out;
+  if (m == 0 || p.cube.mips == 1) {
+    // roughness m / (mips - 1) == 0: the mirror lookup (wave-uniform branch)
+    out = prefilter_texel_mirror(pyr, level, face, min(x, S - 1), min(y, S - 1), S);
+  } else {
+    const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+    const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
+    out = prefilter_texel<kF64Sums>(pyr, level, tab, counts[m], wsum[m], fr);
+  }
   if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
 }
 
@@ -709,10 +715,30 @@ __global__ __launch_bounds__(256) void env_sample_kernel(const char* __restrict_
   __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.count) return;
-  float4 q = dir_lod[i];
-  float yaw, pitch;
-  equirect_angles(mk(q.x, q.y, q.z), yaw, pitch);
-  f3 v = lds_trilinear(pyr, s_lv, p.levels, yaw, pitch, q.w);
+  // The mirror-lookup path: angles and texel coordinates in df32 (the
+  // addressing of the roughness-0 prefilter), the two levels blended with
+  // weights 1 - delta and delta.
+  const float4 q = dir_lod[i];
+  const float len_xz = sqrtf(fmaf(q.x, q.x, q.z * q.z));
+  const int pole = len_xz > 0.001f ? 0 : (q.y > 0.0f ? 1 : -1);
+  df32 yaw, pitch;
+  equirect_angles_df(df3{df_of(q.x), df_of(q.y), df_of(q.z)}, pole, yaw, pitch);
+  const float d = fminf(fmaxf(q.w, 0.0f), static_cast<float>(p.levels - 1));
+  const float fl = floorf(d);
+  const int l0 = static_cast<int>(fl);
+  const float delta = d - fl;
+  auto at = [&](int l) {
+    const LevelC& c = s_lv[l];
+    df32 x, y;
+    equirect_texel_df(yaw, pitch, static_cast<int>(c.wmask + 1), static_cast<int>(c.hmask + 1), x, y);
+    return level_bilinear_df(pyr, c, x, y);
+  };
+  f3 v = at(l0);
+  if (delta != 0.0f) {
+    const f3 b = at(min(l0 + 1, p.levels - 1));
+    const float w0 = 1.0f - delta;
+    v = mk(fmaf(b.x, delta, v.x * w0), fmaf(b.y, delta, v.y * w0), fmaf(b.z, delta, v.z * w0));
+  }
   out[i] = make_float4(v.x, v.y, v.z, 1.0f);
 }
 

@@ -85,7 +85,9 @@ althea_status prefilter_batch_into(const althea_env_view* envs, int count, const
 // Host: the view ray of DeferredPass.vert:10-22 / SSR.vert:15-22 is linear
 // in NDC, so affine in the pixel centre (p + 0.5): ray0 + rayx x + rayy y;
 // and projection * view, column-major (resolve_kernels.hip).
-void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]);
+struct ViewRayDf;
+void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3],
+                     ViewRayDf* df = nullptr);
 void view_projection(const althea_globals& g, float vp[16]);
 
 // Deferred resolve (resolve_kernels.hip); with `mlp` the prefiltered term

@@ -496,6 +496,9 @@ struct MlpState {
   float* cell_error = nullptr;
   uint8_t* cell_ok = nullptr;
   int mon_size = 0, mon_mips = 0, mon_accepted = 0;
+  // Queue of the MLP-IBL resolve's deferred pixels: a counter, then records.
+  void* queue_mem = nullptr;
+  size_t queue_records = 0;
 };
 
 static constexpr int kFwdBlocks = 256;    // training forward grid
@@ -626,6 +629,29 @@ althea_status mlp_inference_view(const althea_mlp* m, MlpInferenceView* v) {
   v->mon_size = m->s.mon_size;
   v->mon_mips = m->s.mon_mips;
   v->mon_accepted = m->s.mon_accepted;
+  v->queue = nullptr;
+  v->queue_count = nullptr;
+  v->queue_capacity = 0;
+  return ALTHEA_OK;
+}
+
+althea_status mlp_resolve_queue(const althea_mlp* m, size_t records, MlpInferenceView* v) {
+  althea_status st = mlp_inference_view(m, v);
+  if (st != ALTHEA_OK) return st;
+  // The queue is the handle's scratch (like its activations), so a const
+  // handle may grow it.
+  MlpState* s = const_cast<MlpState*>(&m->s);
+  if (records > s->queue_records) {
+    ALTHEA_TRY_HIP(hipDeviceSynchronize());  // an earlier frame may still read the old queue
+    (void)hipFree(s->queue_mem);
+    s->queue_mem = nullptr;
+    s->queue_records = 0;
+    ALTHEA_TRY_HIP(hipMalloc(&s->queue_mem, 256 + records * 3 * sizeof(float4)));
+    s->queue_records = records;
+  }
+  v->queue_count = static_cast<uint32_t*>(s->queue_mem);
+  v->queue = reinterpret_cast<float4*>(static_cast<char*>(s->queue_mem) + 256);
+  v->queue_capacity = s->queue_records;
   return ALTHEA_OK;
 }
 }  // namespace althea
@@ -702,6 +728,7 @@ althea_status althea_mlp_destroy(althea_mlp* m) {
   (void)hipFree(s->bpartial);
   (void)hipFree(s->cell_error);
   (void)hipFree(s->cell_ok);
+  (void)hipFree(s->queue_mem);
   delete m;
   return ALTHEA_OK;
 }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "device_math.hpp"
+#include "precise_lookup.hpp"
 
 namespace althea {
 
@@ -61,6 +62,28 @@ __device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, 
                        1.0f);
   }
   return make_float4(sr / sw, sg / sw, sb / sw, 1.0f);
+}
+
+// Roughness 0 (PreFilterEnvMap.comp:139-161 with a = 0): every half vector
+// is N, so every kept sample is the mirror lookup at N, LOD 0, and the
+// NdotL-weighted mean is that one lookup.  It reads the full texel contrast
+// of the source, so it is addressed in df32 from the texel's exact lattice
+// direction (yaw and pitch do not depend on the direction's length); the pole
+// branch (|xz| <= 0.001 of the unit direction, PreFilterEnvMap.comp:104-112)
+// is decided in FP32 as the sampled path decides it.
+template <class LevelOf>
+__device__ __forceinline__ float4 prefilter_texel_mirror(const char* __restrict__ pyr, const LevelOf& level, int face,
+                                                         int x, int y, int S) {
+  const f3 raw = cube_texel_raw_dir(face, x, y, S);
+  const f3 n = normalize(raw);
+  const float len_xz = sqrtf(fmaf(n.x, n.x, n.z * n.z));
+  const int pole = len_xz > 0.001f ? 0 : (n.y > 0.0f ? 1 : -1);
+  const LevelC& c = level(0);
+  df32 tx, ty;
+  equirect_coords_df(df3{df_of(raw.x), df_of(raw.y), df_of(raw.z)}, static_cast<int>(c.wmask + 1),
+                     static_cast<int>(c.hmask + 1), pole, tx, ty);
+  const f3 v = level_bilinear_df(pyr, c, tx, ty);
+  return make_float4(v.x, v.y, v.z, 1.0f);
 }
 
 }  // namespace althea

@@ -4,10 +4,14 @@
 //
 // Per pixel (DeferredPass.vert:10-22, DeferredPass.frag:41-94):
 //   view ray, affine in the pixel centre and precomputed on the host from
-//     mat3(inverseView) * (inverseProjection * (ndc, 0, 1)).xyz;
-//   background equirect lookup (LOD 0, CLAMP) when position.w == 0 (Q13);
+//     mat3(inverseView) * (inverseProjection * (ndc, 0, 1)).xyz, in FP32 and
+//     (for the mirror-like lookups) with df32 coefficients;
+//   background equirect lookup (LOD 0, CLAMP) when position.w == 0 (Q13),
+//     addressed in df32;
 //   reflection = seamless trilinear prefiltered-cube lookup at
-//     LOD roughness*(mips-1) (PBRMaterial.glsl:4-11 with cube maps);
+//     LOD roughness*(mips-1) (PBRMaterial.glsl:4-11 with cube maps); where the
+//     LOD is below 1 the mip-0 tap (a mirror of the source's texel contrast)
+//     is addressed in df32 from the df32 reflection of the view ray;
 //   irradiance = seamless bilinear irradiance-cube lookup (PBRMaterial.glsl:13-19);
 //   SSAO hemisphere kernel into the occlusion channel (Q9);
 //   pbrMaterial (PBRMaterial.glsl:72-162, live convention of SSR.frag:65-75);
@@ -19,8 +23,10 @@
 #include <cmath>
 
 #include "device_math.hpp"
+#include "fill.hpp"
 #include "internal.hpp"
 #include "mlp_eval.hpp"
+#include "precise_lookup.hpp"
 #include "reflection.hpp"
 #include "shading.hpp"
 #include "shadow.hpp"
@@ -43,6 +49,7 @@ struct ResolveParams {
   int lut_size;
   int light_count;
   float ray0[3], rayx[3], rayy[3];  // view ray = ray0 + rayx*(px+0.5) + rayy*(py+0.5)
+  ViewRayDf ray_df;                 // the same coefficients as df32 pairs
   float view_z[4];                  // row 2 of the view matrix
   float vp[16];                     // projection * view, column-major
   float exposure;
@@ -167,8 +174,8 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
   return fmaf(-occ, p.ssao_inv_samples, 1.0f);
 }
 
-// LDS of the MLP variant: the per-wave query staging, then the weights.
-struct MlpResolveSmem {
+// LDS of the MLP finishing kernel: the per-wave query staging, then the weights.
+struct MlpFinishSmem {
   MlpWaveStage stage[4];
   MlpSmem mlp;
 };
@@ -177,10 +184,17 @@ struct MlpResolveSmem {
 // mapping that gives each XCD (with its own L2) contiguous runs of tiles was
 // measured slower at 4K (runs of 2, 8 or 68 tile rows: 0.87, 0.89, 0.84 ms
 // against 0.83 ms).
+//
+// kMlp (the MLP-IBL resolve): a pixel whose reflection cell the MLP's error
+// monitor accepts at both mips the cube lookup would blend skips the cube
+// lookup and is not stored here; it goes to the handle's queue as three
+// float4 (R, roughness | (F A + B) ao per channel, pixel index | the colour
+// without the prefiltered term), which resolve_mlp_finish_kernel completes
+// with the MLP's radiance.  Every other pixel is shaded exactly as by the
+// plain resolve.
 template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false, bool kShadow = false>
 __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, int tiles_x,
-                                             const float4* __restrict__ lights, const float4* s_noise,
-                                             MlpWaveStage* s_stage, const MlpSmem* s_mlp) {
+                                             const float4* __restrict__ lights, const float4* s_noise) {
   const int px = (tile % tiles_x) * 64 + (threadIdx.x & 63);
   const int py = (tile / tiles_x) * 4 + (threadIdx.x >> 6);
   const bool inside = px < p.width && py < p.height;
@@ -192,10 +206,11 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
   const float4 pos = load_gbuf<kFmt>(p.position, idx);
   const bool background = pos.w == 0.0f;
   if (background) {
-    // Background (DeferredPass.frag:44-53): env LOD 0, CLAMP_TO_EDGE.
-    const float yaw = atan2_fast(dir.z, dir.x);
-    const float pitch = atan2_xpos(dir.y, sqrt_fast(fmaf(dir.x, dir.x, dir.z * dir.z)));
-    f3 c = env_bilinear_clamp(p.env, p.env_w, p.env_h, equirect_x(yaw, p.env_w), equirect_y(pitch, p.env_h));
+    // Background (DeferredPass.frag:44-53): env LOD 0, CLAMP_TO_EDGE, the atan2
+    // form without a pole branch; a mirror of the source, addressed in df32.
+    df32 tx, ty;
+    equirect_coords_df(view_ray_df(p.ray_df, cxp, cyp), p.env_w, p.env_h, 0, tx, ty);
+    f3 c = clamp_bilinear_df(p.env, p.env_w, p.env_h, tx, ty);
     if (p.tonemap) c = tonemap(c, p.exposure);
     if (inside) {
       if (p.ao_out) p.ao_out[idx] = 1.0f;
@@ -235,19 +250,31 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     const int per_mip = 6 * ALTHEA_MLP_MONITOR_CELLS;
     from_mlp = shade && p.mlp.cell_ok[m0 * per_mip + cell] != 0 &&
                (d == fl || p.mlp.cell_ok[m1 * per_mip + cell] != 0);
-    if (__any(from_mlp)) {
-      const f3 ly = mlp_wave_eval(*s_mlp, s_stage[threadIdx.x >> 6], R.x, R.y, R.z, roughness, from_mlp);
-      refl = mk(expf(ly.x), expf(ly.y), expf(ly.z));
-    }
     if (inside && p.mlp_used) p.mlp_used[idx] = from_mlp ? 1 : 0;
-    // Keep the remaining G-buffer reads after the evaluation (register pressure).
-    asm volatile("" ::: "memory");
   }
   const float4 alb = shade ? load_gbuf<kFmt>(p.albedo, idx) : zero4;
   const PbrPoint m = pbr_point(mk(alb.x, alb.y, alb.z), metallic, roughness, NI);
   f3 color = mk(0.0f, 0.0f, 0.0f);
   if (shade) {
-    if (!from_mlp) refl = cube_sample(p.prefiltered, p.pref, R, lod);
+    if (!from_mlp) {
+      const float d = fminf(fmaxf(lod, 0.0f), static_cast<float>(p.pref.mips - 1));
+      if (d < 1.0f) {
+        // The mip-0 tap mirrors the source's texel contrast: reflection and
+        // face coordinates in df32, the blend with mip 1 in weight form.
+        const df3 Rd = reflect_df(view_ray_df(p.ray_df, cxp, cyp), mk(nrm.x, nrm.y, nrm.z));
+        refl = cube_bilinear_df(p.prefiltered, p.pref.size, Rd);
+        if (d > 0.0f) {
+          int face;
+          float pc, pt;
+          cube_select(R, face, pc, pt);
+          const f3 b = cube_bilinear(p.prefiltered + p.pref.off[1], max(1, p.pref.size >> 1), face, pc, pt);
+          const float w0 = 1.0f - d;
+          refl = mk(fmaf(b.x, d, refl.x * w0), fmaf(b.y, d, refl.y * w0), fmaf(b.z, d, refl.z * w0));
+        }
+      } else {
+        refl = cube_sample(p.prefiltered, p.pref, R, lod);
+      }
+    }
     if (kRefl) {
       // Reflection buffer at LOD roughness * (mips - 1); where its alpha a
       // reaches 0.01, mix(prefiltered, rgb / a, a) (DeferredPass.frag:59-68).
@@ -267,53 +294,90 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     float A, B;
     lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
     color = pbr_environment(m, irr, refl, A, B, ao);
+    if (kMlp && from_mlp) {
+      // The prefiltered radiance's factor in pbr_environment, for the finishing kernel.
+      const float fx = pow5(1.0f - m.n_dot_v);
+      refl = mk(fmaf(fmaf(m.dF.x, fx, m.F0.x), A, B) * ao, fmaf(fmaf(m.dF.y, fx, m.F0.y), A, B) * ao,
+                fmaf(fmaf(m.dF.z, fx, m.F0.z), A, B) * ao);
+    }
   }
+  const auto view_df = [&]() { return view_ray_df(p.ray_df, cxp, cyp); };
   if (kShadow)
-    pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color, p.shadow);
+    pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color, p.shadow, view_df);
   else
-    pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color);
-  if (shade) {
+    pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color, NoShadow(), view_df);
+  if (kMlp) {
+    // Deferred pixels take consecutive queue slots: one atomic per wave.
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(from_mlp);
+    if (mask != 0) {
+      const int lane = threadIdx.x & 63;
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(p.mlp.queue_count, static_cast<uint32_t>(__popcll(mask)));
+      base = __shfl(base, 0, 64);
+      const uint32_t slot = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+      if (from_mlp && slot < p.mlp.queue_capacity) {
+        float4* q = p.mlp.queue + 3 * static_cast<size_t>(slot);
+        q[0] = make_float4(R.x, R.y, R.z, roughness);
+        q[1] = make_float4(refl.x, refl.y, refl.z, __uint_as_float(static_cast<uint32_t>(idx)));
+        q[2] = make_float4(color.x, color.y, color.z, 0.0f);
+      }
+    }
+  }
+  if (shade && !(kMlp && from_mlp)) {
     if (p.tonemap) color = tonemap(color, p.exposure);
     store_out<kOutF16>(p.out, idx, color);
   }
 }
 
-// Shared prologue of both kernels: the SSAO noise tile (and the MLP weights)
-// in LDS, visible to every wave.
-template <bool kMlp>
-__device__ __forceinline__ void resolve_prologue(const ResolveParams& p, float4* s_noise, MlpSmem* s_mlp) {
+// The resolve: one block per tile, the SSAO noise tile in LDS; kRefl blends
+// the reflection buffer, kShadow applies the point-light shadow test, kMlp
+// defers the pixels the MLP serves.
+template <int kFmt, bool kOutF16, bool kRefl = false, bool kShadow = false, bool kMlp = false>
+__global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, const float4* __restrict__ lights) {
+  __shared__ float4 s_noise[16];
   if (threadIdx.x < 16)
     s_noise[threadIdx.x] =
         make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
-  if (kMlp) mlp_stage_weights(*s_mlp, p.mlp.w, threadIdx.x, 256);
   __syncthreads();
+  const int tiles_x = (p.width + 63) / 64;
+  resolve_tile<kFmt, kOutF16, kMlp, kRefl, kShadow>(p, blockIdx.x, tiles_x, lights, s_noise);
 }
 
-// Plain resolve: one block per tile; kRefl blends the reflection buffer,
-// kShadow applies the point-light shadow test.
-template <int kFmt, bool kOutF16, bool kRefl = false, bool kShadow = false>
-__global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, const float4* __restrict__ lights) {
-  __shared__ float4 s_noise[16];
-  resolve_prologue<false>(p, s_noise, nullptr);
-  const int tiles_x = (p.width + 63) / 64;
-  resolve_tile<kFmt, kOutF16, false, kRefl, kShadow>(p, blockIdx.x, tiles_x, lights, s_noise, nullptr, nullptr);
+// Completes the pixels the MLP-IBL resolve deferred: the MLP (weights in LDS,
+// each wave 64 consecutive queue records as four 16-sample MFMA tiles) gives
+// the log-radiance at (R, roughness), and the pixel is
+// colour + exp(y) (F A + B) ao, tonemapped as the resolve would.  Persistent
+// blocks; a block with no record exits before staging the weights.
+template <bool kOutF16>
+__global__ __launch_bounds__(256) void resolve_mlp_finish_kernel(const ResolveParams p) {
+  __shared__ MlpFinishSmem s;
+  const uint32_t n = min(*p.mlp.queue_count, static_cast<uint32_t>(p.mlp.queue_capacity));
+  if (blockIdx.x * 256u >= n) return;  // uniform over the block
+  mlp_stage_weights(s.mlp, p.mlp.w, threadIdx.x, 256);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const uint32_t r = base + 64u * wave + lane;
+    const bool have = r < n;
+    float4 q0 = make_float4(0.0f, 0.0f, 1.0f, 1.0f), q1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q2 = q1;
+    if (have) {
+      const float4* q = p.mlp.queue + 3 * static_cast<size_t>(r);
+      q0 = q[0];
+      q1 = q[1];
+      q2 = q[2];
+    }
+    const f3 ly = mlp_wave_eval(s.mlp, s.stage[wave], q0.x, q0.y, q0.z, q0.w, have);
+    if (have) {
+      f3 c = mk(fmaf(expf(ly.x), q1.x, q2.x), fmaf(expf(ly.y), q1.y, q2.y), fmaf(expf(ly.z), q1.z, q2.z));
+      if (p.tonemap) c = tonemap(c, p.exposure);
+      store_out<kOutF16>(p.out, __float_as_uint(q1.w), c);
+    }
+  }
 }
 
-// MLP-IBL resolve: persistent blocks keep the MLP weights in LDS for several
-// tiles.  The kernel needs about 150 VGPRs (3 waves per SIMD, 3 blocks per
-// CU); a budget of 128 for 4 waves per SIMD would spill.
-constexpr int kMlpBlocksPerCu = 3;
-template <int kFmt, bool kOutF16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMlpBlocksPerCu))) void resolve_mlp_kernel(
-    const ResolveParams p, const float4* __restrict__ lights) {
-  __shared__ float4 s_noise[16];
-  __shared__ MlpResolveSmem s_m;
-  resolve_prologue<true>(p, s_noise, &s_m.mlp);
-  const int tiles_x = (p.width + 63) / 64;
-  const int tiles = tiles_x * ((p.height + 3) / 4);
-  for (int vt = blockIdx.x; vt < tiles; vt += gridDim.x)
-    resolve_tile<kFmt, kOutF16, true>(p, vt, tiles_x, lights, s_noise, s_m.stage, &s_m.mlp);
-}
+// Persistent blocks of the MLP finishing kernel per CU (about 150 VGPRs: 3
+// waves per SIMD).
+constexpr int kMlpFinishBlocksPerCu = 3;
 
 __global__ void fill_u8_kernel(uint8_t* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -322,7 +386,8 @@ __global__ void fill_u8_kernel(uint8_t* __restrict__ p, int n) {
 
 // Host: the view ray of DeferredPass.vert is linear in NDC, so it is affine in
 // the pixel centre; fold mat3(inverseView) * inverseProjection into it.
-void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3]) {
+void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float rayx[3], float rayy[3],
+                     ViewRayDf* df) {
   const float* ip = g.inverse_projection;
   const float* iv = g.inverse_view;
   auto e_of = [&](double nx, double ny, double out[3]) {
@@ -334,10 +399,20 @@ void view_ray_affine(const althea_globals& g, int W, int H, float ray0[3], float
   e_of(-1.0, -1.0, o);
   e_of(1.0, -1.0, ax);
   e_of(-1.0, 1.0, ay);
+  auto split = [](double v, float out[2]) {
+    out[0] = static_cast<float>(v);
+    out[1] = static_cast<float>(v - static_cast<double>(out[0]));
+  };
   for (int r = 0; r < 3; ++r) {
-    ray0[r] = static_cast<float>(o[r]);
-    rayx[r] = static_cast<float>((ax[r] - o[r]) / W);
-    rayy[r] = static_cast<float>((ay[r] - o[r]) / H);
+    const double c0 = o[r], cx = (ax[r] - o[r]) / W, cy = (ay[r] - o[r]) / H;
+    ray0[r] = static_cast<float>(c0);
+    rayx[r] = static_cast<float>(cx);
+    rayy[r] = static_cast<float>(cy);
+    if (df) {
+      split(c0, df->ray0[r]);
+      split(cx, df->rayx[r]);
+      split(cy, df->rayy[r]);
+    }
   }
 }
 
@@ -376,7 +451,7 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
   p.lut_size = ibl->brdf_lut_size;
   p.light_count = light_count;
-  view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy);
+  view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy, &p.ray_df);
   for (int c = 0; c < 4; ++c) p.view_z[c] = globals->view[c * 4 + 2];
   view_projection(*globals, p.vp);
   p.exposure = globals->exposure;
@@ -410,23 +485,38 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
     mlp = nullptr;
   }
   if (mlp) {
+    // The resolve at full occupancy with the MLP's pixels queued, then the
+    // finishing kernel over the queue: the MLP's cost follows its pixels.
     p.mlp = *mlp;
     p.mlp_used = mlp_used;
-    // Persistent blocks, as many as are resident at once.
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(fill_blocks(1, sizeof(uint32_t))), dim3(256), 0, s,
+                       p.mlp.queue_count, static_cast<size_t>(1), 0u);
+    althea_status st = check_launch("fill_kernel");
+    if (st != ALTHEA_OK) return st;
+    const dim3 grid(tiles);
+    if (fp32 && f16_out)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true, false, false, true>), grid, dim3(256), 0, s, p,
+                         lights4);
+    else if (fp32)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false, false, false, true>), grid, dim3(256), 0, s, p,
+                         lights4);
+    else if (f16_out)
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true, false, false, true>), grid, dim3(256), 0, s, p,
+                         lights4);
+    else
+      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false, false, false, true>), grid, dim3(256), 0, s, p,
+                         lights4);
+    st = check_launch("resolve_kernel<mlp>");
+    if (st != ALTHEA_OK) return st;
     int dev = 0, cus = 256;
     ALTHEA_TRY_HIP(hipGetDevice(&dev));
     ALTHEA_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int blocks = std::min(tiles, kMlpBlocksPerCu * cus);
-    const dim3 grid(blocks);
-    if (fp32 && f16_out)
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);
-    else if (fp32)
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p, lights4);
-    else if (f16_out)
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p, lights4);
+    const dim3 fgrid(kMlpFinishBlocksPerCu * cus);
+    if (f16_out)
+      hipLaunchKernelGGL(resolve_mlp_finish_kernel<true>, fgrid, dim3(256), 0, s, p);
     else
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p, lights4);
-    return check_launch("resolve_mlp_kernel");
+      hipLaunchKernelGGL(resolve_mlp_finish_kernel<false>, fgrid, dim3(256), 0, s, p);
+    return check_launch("resolve_mlp_finish_kernel");
   }
   const dim3 grid(tiles);
   if (reflection || shadow_cubes) {

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_math.hpp"
+#include "precise_lookup.hpp"
 
 namespace althea {
 
@@ -96,10 +97,42 @@ struct NoShadow {
   __device__ __forceinline__ bool hidden(int, f3, float) const { return false; }
 };
 
-template <class Shadow = NoShadow>
+// The reference's half vector H = normalize(V + L) (PBRMaterial.glsl:145, Q2)
+// is ill-conditioned where L nearly equals -V, a light near the line of
+// sight behind the viewer: |V + L|^2 = 2 + 2 V.L cancels, so the FP32 form
+// carries an error of about 1e-7 / |V + L|^2 into NdotH.  Where some lane of
+// the wave has |V + L|^2 < kHalfVecRefine (about one light per wave in the 4K
+// frame), NdotH is recomputed from the explicit vector h = V + L, whose error
+// is about 1e-7 / |V + L|; below kHalfVecDf (one light in 60 waves) from V and
+// L in df32, V from the caller's df32 view ray (`view_df`, any positive scale).
+constexpr float kHalfVecRefine = 1e-2f;
+constexpr float kHalfVecDf = 1e-4f;
+
+// The view direction as given (FP32), for callers without a df32 view ray.
+struct ViewOfF3 {
+  f3 v;
+  __device__ __forceinline__ df3 operator()() const { return {df_of(v.x), df_of(v.y), df_of(v.z)}; }
+};
+
+template <class ViewDf>
+__device__ __forceinline__ float n_dot_h_refined(f3 N, f3 I, f3 P, float4 lp, f3 L, float inv_d, float h2,
+                                              const ViewDf& view_df) {
+  if (h2 >= kHalfVecDf) {
+    const f3 h = mk(fmaf(L.x, inv_d, I.x), fmaf(L.y, inv_d, I.y), fmaf(L.z, inv_d, I.z));
+    return fmaxf(dot(N, h) * rsq_fast(dot(h, h)), 0.0f);
+  }
+  const df3 Lu = normalize_df({df_two_sum(lp.x, -P.x), df_two_sum(lp.y, -P.y), df_two_sum(lp.z, -P.z)});
+  const df3 Vu = normalize_df(view_df());
+  const df3 h = {df_add(Vu.x, Lu.x), df_add(Vu.y, Lu.y), df_add(Vu.z, Lu.z)};
+  const df32 hh = df_add(df_add(df_mul(h.x, h.x), df_mul(h.y, h.y)), df_mul(h.z, h.z));
+  const df32 nh = df_add(df_add(df_mul_f(h.x, N.x), df_mul_f(h.y, N.y)), df_mul_f(h.z, N.z));
+  return fmaxf(df_div(nh, df_sqrt(hh)).hi, 0.0f);
+}
+
+template <class Shadow = NoShadow, class ViewDf = ViewOfF3>
 __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, float NI, bool any_lane,
                                            const float4* __restrict__ lights, int light_count, f3& color,
-                                           const Shadow& shadow = Shadow()) {
+                                           const Shadow& shadow = Shadow(), const ViewDf& view_df = ViewDf{}) {
   const float c1 = m.a2 * gv_of(m.n_dot_v, m.kdir) * kInvPi;
   const float four_nv = 4.0f * m.n_dot_v;
   const float om_k = 1.0f - m.kdir;
@@ -118,7 +151,11 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
       const float vl = dot(I, L);
       const float h2 = fmaf(2.0f * vl, inv_d, 2.0f);
       const float n_dot_l = fmaxf(nl_raw * inv_d, 0.0f);
-      const float n_dot_h = fmaxf(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2), 0.0f);
+      float n_dot_h = fmaxf(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2), 0.0f);
+      const bool refine = h2 < kHalfVecRefine && nl_raw > 0.0f;
+      if (__builtin_amdgcn_ballot_w64(refine) != 0) {
+        if (refine) n_dot_h = n_dot_h_refined(N, I, P, lp, L, inv_d, h2, view_df);
+      }
       const float x = 1.0f - n_dot_h;
       const float x5 = pow5(x);
       const float t = fmaf(n_dot_h * n_dot_h, a2m1, 1.0f);

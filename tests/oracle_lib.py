@@ -16,6 +16,16 @@ ORACLE_PATH = Path(__file__).resolve().parent.parent / "oracle" / "liboracle.so"
 vp, i32, u32, dbl = C.c_void_p, C.c_int, C.c_uint32, C.c_double
 
 
+class PortArgs(C.Structure):
+    """port_resolve_args of oracle/port_resolve.cpp."""
+    _fields_ = [("position", vp), ("normal", vp), ("albedo", vp), ("mro", vp), ("width", i32), ("height", i32),
+                ("env", vp), ("env_w", i32), ("env_h", i32), ("prefiltered", vp), ("pref_size", i32),
+                ("pref_mips", i32), ("irradiance", vp), ("irr_size", i32), ("lut", vp), ("lut_size", i32),
+                ("lights", vp), ("light_count", i32), ("view", vp), ("proj_view", vp), ("ray_affine", vp),
+                ("exposure", C.c_float), ("tonemap", i32), ("ssao_on", i32), ("ssao_samples", i32),
+                ("ssao_radius", C.c_float), ("ssao_bias", C.c_float), ("ssao_kernel", vp), ("ssao_noise", vp)]
+
+
 class ResolveArgs(C.Structure):
     _fields_ = [("position", vp), ("normal", vp), ("albedo", vp), ("mro", vp), ("width", i32), ("height", i32),
                 ("env", vp), ("prefiltered", vp), ("pref_size", i32), ("pref_mips", i32), ("irradiance", vp),
@@ -93,6 +103,8 @@ def load(path) -> C.CDLL:
     L.oracle_prefilter_level_constant.argtypes = [vp, i32, dbl, i32, i32, i32, vp]
     L.oracle_prefilter_crosscheck.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, vp]
     L.oracle_ssao_compare.argtypes = [C.POINTER(ResolveArgs), vp, i32, vp, vp]
+    if hasattr(L, "port_resolve"):  # the FP32 port (absent from oracle-only rebuilds)
+        L.port_resolve.argtypes = [C.POINTER(PortArgs), vp, i32, i32, C.c_uint, vp]
     return L
 
 
@@ -338,6 +350,47 @@ def ssao_compare(scene, pixels):
     ker = np.empty(px.shape[0], np.float64)
     lib().oracle_ssao_compare(C.byref(a), _p(px), px.shape[0], _p(ref), _p(ker))
     return ref, ker
+
+
+# Steps of the FP32 port that can be put back to plain FP32 addressing
+# (error attribution): background, mip-0 reflection tap, half vector.
+PORT_FP32_BACKGROUND, PORT_FP32_REFLECTION, PORT_FP32_HALF_VECTOR = 1, 2, 4
+
+
+def port_resolve(scene, views: dict, env_level0, pixels=None, threads: int = 0, fp32_steps: int = 0,
+                 tonemap: bool = False) -> np.ndarray:
+    """FP32 CPU port of the resolve (oracle/port_resolve.cpp) for a
+    synth.Scene, IBL views (as for resolve()) and the environment's level 0
+    (h, w, 4): float32 (n, 4) at pixels[:, (x, y)], or the whole frame in row
+    order when pixels is None.  `threads` 0 uses every OpenMP thread."""
+    W, H = scene.width, scene.height
+    g = scene.globals_arr.astype(np.float64)
+    proj, iproj = g[0:16].reshape(4, 4).T, g[16:32].reshape(4, 4).T
+    view, iview = g[32:48].reshape(4, 4).T, g[64:80].reshape(4, 4).T
+
+    def e_of(nx, ny):
+        e = iproj @ np.array([nx, ny, 0.0, 1.0])
+        return iview[:3, :3] @ e[:3]
+
+    o, ax, ay = e_of(-1.0, -1.0), e_of(1.0, -1.0), e_of(-1.0, 1.0)
+    aff = _c(np.concatenate([o, (ax - o) / W, (ay - o) / H]), np.float64)
+    pv = _c((proj @ view).T.ravel(), np.float32)
+    vw = _c(scene.globals_arr[32:48], np.float32)
+    planes = [_c(p, np.float32) for p in (scene.position, scene.normal, scene.albedo, scene.material)]
+    env = _c(env_level0, np.float32)
+    pref, irr, lut = (_c(views[k], np.float32) for k in ("prefiltered", "irradiance", "lut"))
+    lights = _c(scene.lights, np.float32)
+    kern, noise = _c(scene.ssao_kernel, np.float32), _c(scene.ssao_noise, np.float32)
+    a = PortArgs(*[_p(x) for x in planes], W, H, _p(env), env.shape[1], env.shape[0], _p(pref), views["pref_size"],
+                 views["pref_mips"], _p(irr), views["irr_size"], _p(lut), views["lut_size"], _p(lights),
+                 lights.shape[0], _p(vw), _p(pv), _p(aff), float(scene.globals_arr[101]), 1 if tonemap else 0,
+                 1 if scene.ssao_enabled else 0, kern.shape[0], scene.ssao_radius, scene.ssao_bias, _p(kern),
+                 _p(noise))
+    px = None if pixels is None else _c(pixels, np.int32)
+    n = W * H if px is None else px.shape[0]
+    out = np.empty((n, 4), np.float32)
+    lib().port_resolve(C.byref(a), None if px is None else _p(px), n, threads, fp32_steps, _p(out))
+    return out
 
 
 def gbuffer(scene, positions, indices, colors, mro, normals=None, pixels=None):

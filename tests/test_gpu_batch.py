@@ -84,6 +84,34 @@ def test_batch_autotune_and_oracle_spot_check(torch):
     assert err.max() < MAX_REL, summary(err)
 
 
+def test_config3_full_batch_matches_single_map_kernel(torch):
+    """The configured batch (config 3): 64 environment maps, widths drawn from
+    {256 .. 4096} by batch_widths(64) (height W/2, the config-0 radiance
+    distribution seeded by the map index), faces W/4, 5 roughness mips, 1024
+    Hammersley samples, the launch shape chosen by the autotuner, one graph
+    launch.  Every map's cube is bit-identical to the single-map kernel's
+    (whose parity the other prefilter tests pin against the oracle), and the
+    largest map is spot-checked against the FP64 oracle."""
+    widths = batch_widths(64)
+    assert len(set(widths)) == 5 and max(widths) == 4096
+    maps = [torch.from_numpy(aa.synth.env(seed=i, width=w, height=w // 2)).cuda() for i, w in enumerate(widths)]
+    b = PrefilterBatch(maps, [w // 4 for w in widths], mips=5, samples=1024)
+    t = b.autotune()
+    print(f"config-3 batch: widths {dict((w, widths.count(w)) for w in sorted(set(widths)))}, tuning {t}", flush=True)
+    b.run()
+    torch.cuda.synchronize()
+    for i, w in enumerate(widths):
+        ref = aa.prefilter_cube(aa.env_pyramid(maps[i]), w, w // 2, w // 4, 5, 1024)
+        assert torch.equal(b.cube(i), ref), f"map {i} ({w} wide) differs from the single-map kernel"
+        del ref
+    i = widths.index(4096)
+    ids = cube_texel_ids(1024, 5, per_face=2, seed=11)
+    got = cube_gather(b.cube(i).cpu().numpy(), 1024, ids)
+    ref = ol.Env(maps[i].cpu().numpy()).prefilter_texels(1024, 5, 1024, ids)[:, :3]
+    err = rel_err(got, ref, RADIANCE_FLOOR)
+    assert err.max() < MAX_REL, summary(err)
+
+
 def test_batch_rejects_bad_shapes(torch):
     maps = _maps(torch, [64])
     with pytest.raises(aa.AltheaError, match="face size"):

@@ -3,15 +3,13 @@ the C ABI.  Sizes are the benchmark's small configuration (512x256 source,
 128^2 x 6 x 5 prefiltered mips at 1024 samples, 32^2 x 6 irradiance, 128^2
 BRDF LUT) plus edge cases; the oracle evaluates a subset of texels.
 
-Two sources:
-  smooth  background 1 + four 8-degree sun lobes (peak 100): FP32 texel
-          addressing error times the source's per-texel contrast stays far
-          below 1e-4, so the 1e-4 max-relative contract is asserted.
+Two sources, both held to the 1e-4 max-relative contract:
+  smooth  background 1 + four 8-degree sun lobes (peak 100);
   config0 per-texel log-normal(0, 1.5) noise + three 2-degree suns of 1e3:
-          where a lookup lands between two texels whose values differ by
-          orders of magnitude, the FP32 texel coordinate (about 1e-5 texel of
-          rounding at 512 wide) moves the result by 1e-5 x that contrast; the
-          test asserts the mean and a 99.9th-percentile bound and reports max.
+          neighbouring texels differ by up to 1000x, so a mirror-like lookup
+          (the roughness-0 mip, the env_sample probe) carries its texel
+          coordinate in df32; rough mips average 1000+ FP32 lookups whose
+          addressing errors cancel.
 """
 import numpy as np
 import pytest
@@ -95,11 +93,7 @@ def test_env_sample_matches_oracle(env_case, torch):
     got = aa.env_sample(pyr, 512, 256, torch.from_numpy(q).cuda()).cpu().numpy()[:, :3]
     ref = oenv.lookup(q[:, :3].astype(np.float64), q[:, 3].astype(np.float64))
     err = rel_err(got, ref, RADIANCE_FLOOR)
-    s = summary(err)
-    if name == "smooth":
-        assert s["max"] < MAX_REL, s
-    else:
-        assert s["mean"] < 1e-5 and s["p99"] < MAX_REL, s
+    assert err.max() < MAX_REL, summary(err)
 
 
 def test_cube_sample_matches_oracle(torch):
@@ -141,14 +135,9 @@ def test_prefilter_cube_matches_oracle(env_case, torch):
     got = cube_gather(cube, 128, ids)
     ref = oenv.prefilter_texels(128, 5, 1024, ids)
     err = rel_err(got, ref[:, :3], RADIANCE_FLOOR)
-    s = summary(err)
-    if name == "smooth":
-        assert s["max"] < MAX_REL, s
-    else:
-        assert s["mean"] < 1e-5 and s["p99"] < MAX_REL, s
-        # Rough mips average 1000+ lookups, so addressing error cancels.
-        rough = ids[:, 0] >= 1
-        assert err[rough].max() < MAX_REL, summary(err[rough])
+    assert err.max() < MAX_REL, summary(err)
+    mirror = ids[:, 0] == 0
+    assert err[mirror].max() < 1e-5, summary(err[mirror])  # one df32-addressed lookup per texel
     # Alpha channel and the untouched layout.
     assert np.all(cube[:, 3] == 1.0)
 

@@ -71,10 +71,9 @@ def test_prefilter_fp64_vs_fp32_sums(torch, source):
             _row(f"  mip {m} (roughness {m / 4:.2f}), {tag}", errs[fp32][sel])
     s64 = summary(errs[False])
     rough = ids[:, 0] >= 1
-    # Rough mips average thousands of lookups; mip 0 is a mirror lookup of
-    # per-texel noise (FP32 addressing x neighbour contrast).
-    assert summary(errs[False][rough])["max"] < 1e-4, summary(errs[False][rough])
-    assert s64["mean"] < 1e-4, s64
+    # Every mip within the contract: rough mips average thousands of FP32
+    # lookups, mip 0 (a mirror of per-texel noise) is one df32-addressed lookup.
+    assert s64["max"] < 1e-4, s64
     assert errs[False][rough].mean() <= errs[True][rough].mean() * 1.5 + 1e-8
 
 
@@ -114,5 +113,6 @@ def test_resolve_fp16_vs_fp32_gbuffer(torch, source):
         res[fp16] = rel_err(got, ref, RADIANCE_FLOOR)
         _row(f"shaded 3840x2160 from {'FP16' if fp16 else 'FP32'} G-buffer", res[fp16])
     clear = margin > 2e-3
-    assert summary(res[False][clear])["mean"] < 1e-4
+    _row("shaded 3840x2160 from FP32 G-buffer, pixels clear of SSAO thresholds", res[False][clear])
+    assert summary(res[False][clear])["max"] < 1e-4, summary(res[False][clear])
     print("PRECISION_REPORT " + json.dumps(REPORT))

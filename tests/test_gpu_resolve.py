@@ -1,5 +1,9 @@
 """GPU parity of the deferred resolve (DeferredPass.frag + pbrMaterial + SSAO)
-against the FP64 oracle, through the C ABI.
+against the FP64 oracle, through the C ABI, at the 1e-4 max-relative contract
+on the smooth and on the configured (per-texel noise) sources.  The lookups
+that mirror the source's texel contrast (the background, the mip-0
+reflection tap) and the half vector near V = -L are what keep the noisy
+configurations inside it (df32 addressing in the kernel).
 
 The oracle shades the same FP32 G-buffer with the IBL state the GPU produced,
 so this isolates the resolve from the precompute.  SSAO makes discrete
@@ -58,7 +62,7 @@ def _run(torch, scene, ibl, fp16=False, tonemap=False, out_fp16=False):
                                                                                                         scene.width)
 
 
-def _check(scene, ibl, oenv, img, ao, pix, strict, tonemap=False, gbuffer=None):
+def _check(scene, ibl, oenv, img, ao, pix, tonemap=False, gbuffer=None):
     ref, ref_ao, margin = ol.resolve(scene, ibl.views_numpy(), oenv, pix, tonemap=tonemap, gbuffer=gbuffer)
     got = img[pix[:, 1], pix[:, 0], :3]
     got_ao = ao[pix[:, 1], pix[:, 0]]
@@ -71,14 +75,7 @@ def _check(scene, ibl, oenv, img, ao, pix, strict, tonemap=False, gbuffer=None):
     # texel boundary or depth threshold; they are checked for flips only.
     assert clear.mean() > 0.6, clear.mean()
     assert ao_err[clear].max() < 1e-5, summary(ao_err[clear])
-    if strict:
-        assert s["max"] < MAX_REL, s
-    else:
-        # Noisy source: the mirror-like reflection lookups read mip 0 of the
-        # prefiltered cube, which is the per-texel log-normal noise; FP32
-        # texel addressing (1e-5..1e-4 texel at 128..512 faces) times a texel
-        # contrast of up to 100x bounds these.
-        assert s["mean"] < 1e-4 and s["p99"] < 1e-3, s
+    assert s["max"] < MAX_REL, s
     assert flips <= max(2, 0.01 * (~clear).sum()), flips
     return s
 
@@ -88,7 +85,7 @@ def test_resolve_config0_matches_oracle(torch, kind):
     ibl, oenv = _ibl(torch, kind)
     scene = aa.synth.scene(seed=SEED0, width=1280, height=720, lights=8)
     img, ao = _run(torch, scene, ibl)
-    _check(scene, ibl, oenv, img, ao, _pixels(scene, 6000, 1), strict=(kind == "smooth"))
+    _check(scene, ibl, oenv, img, ao, _pixels(scene, 6000, 1))
     assert np.isfinite(img).all()
     assert np.all(img[..., 3] == 1.0)
 
@@ -97,7 +94,7 @@ def test_resolve_tonemap_and_fp16_output(torch):
     ibl, oenv = _ibl(torch, "smooth")
     scene = aa.synth.scene(seed=SEED0 + 1, width=320, height=200, lights=8)
     img, ao = _run(torch, scene, ibl, tonemap=True)
-    _check(scene, ibl, oenv, img, ao, _pixels(scene, 3000, 2), strict=True, tonemap=True)
+    _check(scene, ibl, oenv, img, ao, _pixels(scene, 3000, 2), tonemap=True)
     img16, _ = _run(torch, scene, ibl, tonemap=True, out_fp16=True)
     np.testing.assert_allclose(img16, img, rtol=1e-3, atol=1e-4)
 
@@ -108,7 +105,7 @@ def test_resolve_fp16_gbuffer(torch):
     img, ao = _run(torch, scene, ibl, fp16=True)
     planes16 = [p.astype(np.float16).astype(np.float32) for p in
                 (scene.position, scene.normal, scene.albedo, scene.material)]
-    _check(scene, ibl, oenv, img, ao, _pixels(scene, 3000, 3), strict=True, gbuffer=planes16)
+    _check(scene, ibl, oenv, img, ao, _pixels(scene, 3000, 3), gbuffer=planes16)
 
 
 def test_resolve_config1_full_size(torch):
@@ -120,7 +117,7 @@ def test_resolve_config1_full_size(torch):
     scene = aa.synth.scene(seed=SEED0 + 1, width=3840, height=2160, lights=64)
     img, ao = _run(torch, scene, ibl)
     assert np.isfinite(img).all()
-    _check(scene, ibl, ol.Env(env), img, ao, _pixels(scene, 1500, 4), strict=False)
+    _check(scene, ibl, ol.Env(env), img, ao, _pixels(scene, 4000, 4))
 
 
 def test_resolve_no_lights_no_ssao(torch):
