@@ -48,6 +48,19 @@ def _stream(stream=None) -> int:
     return s.cuda_stream
 
 
+def _empty(stream, *args, **kw):
+    """torch.empty allocated on the stream the work is launched on, so the
+    caching allocator orders reuse of the block after that stream's use of
+    it (a block allocated on another stream could be handed out again while
+    the launch still reads or writes it)."""
+    torch = _torch()
+    kw.setdefault("device", "cuda")
+    if stream is None:
+        return torch.empty(*args, **kw)
+    with torch.cuda.stream(stream):
+        return torch.empty(*args, **kw)
+
+
 def _ptr(t) -> int:
     if t is None:
         return None
@@ -84,8 +97,8 @@ def hammersley(n: int, first: int = 0, count: int | None = None, stream=None):
     """(bits uint32 tensor, xi float32 (count, 2)) from the device Hammersley."""
     torch = _torch()
     count = n - first if count is None else count
-    bits = torch.empty(count, dtype=torch.int32, device="cuda")
-    xi = torch.empty((count, 2), dtype=torch.float32, device="cuda")
+    bits = _empty(stream, count, dtype=torch.int32, device="cuda")
+    xi = _empty(stream, (count, 2), dtype=torch.float32, device="cuda")
     check(lib().althea_hammersley(n, first, count, _ptr(bits), _ptr(xi), _stream(stream)), "althea_hammersley")
     return bits, xi
 
@@ -94,7 +107,7 @@ def env_pyramid(level0, stream=None):
     """Full pyramid of an equirect RGBA32F map (h, w, 4) -> flat (texels, 4)."""
     torch = _torch()
     h, w = int(level0.shape[0]), int(level0.shape[1])
-    out = torch.empty((lib().althea_env_pyramid_texels(w, h), 4), dtype=torch.float32, device=level0.device)
+    out = _empty(stream, (lib().althea_env_pyramid_texels(w, h), 4), dtype=torch.float32, device=level0.device)
     check(lib().althea_env_pyramid_build(_ptr(level0), w, h, _ptr(out), _stream(stream)), "althea_env_pyramid_build")
     return out
 
@@ -102,7 +115,7 @@ def env_pyramid(level0, stream=None):
 def prefilter_cube(pyramid, env_w: int, env_h: int, size: int, mips: int, samples: int, fp32_sums: bool = False,
                    stream=None):
     torch = _torch()
-    out = torch.empty((cube_texels(size, mips), 4), dtype=torch.float32, device=pyramid.device)
+    out = _empty(stream, (cube_texels(size, mips), 4), dtype=torch.float32, device=pyramid.device)
     flags = _lib.PRECOMPUTE_FP32_SUMS if fp32_sums else 0
     check(lib().althea_prefilter_cube(_ptr(pyramid), env_w, env_h, size, mips, samples, flags, _ptr(out),
                                       _stream(stream)), "althea_prefilter_cube")
@@ -111,7 +124,7 @@ def prefilter_cube(pyramid, env_w: int, env_h: int, size: int, mips: int, sample
 
 def prefilter_points(pyramid, env_w: int, env_h: int, samples: int, dir_rough, stream=None):
     torch = _torch()
-    out = torch.empty((dir_rough.shape[0], 4), dtype=torch.float32, device=pyramid.device)
+    out = _empty(stream, (dir_rough.shape[0], 4), dtype=torch.float32, device=pyramid.device)
     check(lib().althea_prefilter_points(_ptr(pyramid), env_w, env_h, samples, _ptr(dir_rough),
                                         int(dir_rough.shape[0]), _ptr(out), _stream(stream)),
           "althea_prefilter_points")
@@ -121,7 +134,7 @@ def prefilter_points(pyramid, env_w: int, env_h: int, samples: int, dir_rough, s
 def irradiance_cube(pyramid, env_w: int, env_h: int, size: int, theta_samples: int = 300, fp32_sums: bool = False,
                     stream=None):
     torch = _torch()
-    out = torch.empty((6 * size * size, 4), dtype=torch.float32, device=pyramid.device)
+    out = _empty(stream, (6 * size * size, 4), dtype=torch.float32, device=pyramid.device)
     flags = _lib.PRECOMPUTE_FP32_SUMS if fp32_sums else 0
     check(lib().althea_irradiance_cube(_ptr(pyramid), env_w, env_h, size, theta_samples, flags, _ptr(out),
                                        _stream(stream)), "althea_irradiance_cube")
@@ -130,14 +143,14 @@ def irradiance_cube(pyramid, env_w: int, env_h: int, size: int, theta_samples: i
 
 def brdf_lut(size: int, samples: int, stream=None):
     torch = _torch()
-    out = torch.empty((size, size, 2), dtype=torch.float32, device="cuda")
+    out = _empty(stream, (size, size, 2), dtype=torch.float32, device="cuda")
     check(lib().althea_brdf_lut(size, samples, _ptr(out), _stream(stream)), "althea_brdf_lut")
     return out
 
 
 def cube_sample(cube, size: int, mips: int, dir_lod, stream=None):
     torch = _torch()
-    out = torch.empty((dir_lod.shape[0], 4), dtype=torch.float32, device=cube.device)
+    out = _empty(stream, (dir_lod.shape[0], 4), dtype=torch.float32, device=cube.device)
     check(lib().althea_cube_sample(_ptr(cube), size, mips, _ptr(dir_lod), int(dir_lod.shape[0]), _ptr(out),
                                    _stream(stream)), "althea_cube_sample")
     return out
@@ -145,7 +158,7 @@ def cube_sample(cube, size: int, mips: int, dir_lod, stream=None):
 
 def env_sample(pyramid, env_w: int, env_h: int, dir_lod, stream=None):
     torch = _torch()
-    out = torch.empty((dir_lod.shape[0], 4), dtype=torch.float32, device=pyramid.device)
+    out = _empty(stream, (dir_lod.shape[0], 4), dtype=torch.float32, device=pyramid.device)
     check(lib().althea_env_sample(_ptr(pyramid), env_w, env_h, _ptr(dir_lod), int(dir_lod.shape[0]), _ptr(out),
                                   _stream(stream)), "althea_env_sample")
     return out
@@ -447,7 +460,7 @@ class SceneToGBufferPass:
         g = self.gbuffer
         need = lib().althea_gbuffer_workspace_bytes(g.width, g.height, mesh.triangle_count)
         if self.workspace is None or self.workspace.numel() < need:
-            self.workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+            self.workspace = _empty(stream, need, dtype=torch.uint8, device="cuda")
         mv = mesh.view()
         gv = g.getHandles()
         check(lib().althea_gbuffer_draw(C.byref(mv), C.byref(globals_), C.byref(gv), _ptr(self.workspace),
@@ -468,7 +481,7 @@ class SceneToGBufferPass:
         if need == 0:
             raise ValueError("invalid glTF draw")
         if self.workspace is None or self.workspace.numel() < need:
-            self.workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+            self.workspace = _empty(stream, need, dtype=torch.uint8, device="cuda")
         gv = g.getHandles()
         check(lib().althea_gbuffer_draw_gltf(prims, len(primitives), mats, len(materials), texs, len(textures),
                                              C.byref(globals_), C.byref(gv), _ptr(self.workspace), _stream(stream)),
@@ -612,7 +625,7 @@ class PointLightCollection:
             return self.shadow_maps
         need = lib().althea_shadow_cubes_workspace_bytes(self.getCount(), tris)
         if self.workspace is None or self.workspace.numel() < need:
-            self.workspace = torch.empty(max(need, 256), dtype=torch.uint8, device="cuda")
+            self.workspace = _empty(stream, max(need, 256), dtype=torch.uint8, device="cuda")
         check(lib().althea_shadow_cubes_ws(C.byref(mesh), _ptr(self.lights), self.getCount(), self.shadow_size,
                                            _ptr(self.shadow_storage), _ptr(self.workspace), _stream(stream)),
               "althea_shadow_cubes_ws")
@@ -643,7 +656,7 @@ class DeferredRendering:
         n = int(lights.shape[0]) if lights is not None else 0
         if out is None:
             dt = torch.float16 if out_fp16 else torch.float32
-            out = torch.empty((gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
+            out = _empty(stream, (gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
         flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)
         ssao_p = C.byref(ssao) if ssao is not None else None
         if shadows is not None:
@@ -685,7 +698,7 @@ class DeferredRendering:
         n = int(lights.shape[0]) if lights is not None else 0
         if out is None:
             dt = torch.float16 if out_fp16 else torch.float32
-            out = torch.empty((gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
+            out = _empty(stream, (gbuffer.height * gbuffer.width, 4), dtype=dt, device="cuda")
         if mlp_used is not None:
             assert mlp_used.dtype == torch.uint8 and mlp_used.numel() >= gbuffer.height * gbuffer.width
         flags = (_lib.RESOLVE_TONEMAP if tonemap else 0) | (_lib.RESOLVE_OUT_FP16 if out_fp16 else 0)

@@ -1,5 +1,6 @@
 // C ABI entry points: argument validation, handle management and workspace
 // plumbing around the kernel launchers.  No exception crosses this boundary.
+#include <cstddef>
 #include <cstdint>
 #include <hip/hip_runtime.h>
 
@@ -18,6 +19,23 @@
 #include "mlp_view.hpp"
 
 namespace althea {
+
+// The uniform and light layouts are the reference's byte for byte, so a
+// consumer passes its existing buffers: GlobalUniforms
+// (Include/Althea/GlobalUniforms.h:15-31, six mat4 then the scalars, 416 B)
+// and PointLight (Include/Althea/PointLight.h:31-34, two alignas(16) vec3).
+static_assert(sizeof(althea_globals) == 416, "GlobalUniforms is 416 bytes");
+static_assert(offsetof(althea_globals, inverse_projection) == 64 && offsetof(althea_globals, view) == 128 &&
+                  offsetof(althea_globals, prev_view) == 192 && offsetof(althea_globals, inverse_view) == 256 &&
+                  offsetof(althea_globals, prev_inverse_view) == 320 && offsetof(althea_globals, mouse_uv) == 384,
+              "GlobalUniforms matrices");
+static_assert(offsetof(althea_globals, light_count) == 392 && offsetof(althea_globals, light_buffer_handle) == 396 &&
+                  offsetof(althea_globals, time) == 400 && offsetof(althea_globals, exposure) == 404 &&
+                  offsetof(althea_globals, input_mask) == 408 && offsetof(althea_globals, frame_count) == 412,
+              "GlobalUniforms scalars");
+static_assert(sizeof(althea_point_light) == 32 && offsetof(althea_point_light, position) == 0 &&
+                  offsetof(althea_point_light, emission) == 16,
+              "PointLight: alignas(16) position and emission");
 
 namespace {
 thread_local char g_error[512] = "";
@@ -466,7 +484,8 @@ althea_status althea_deferred_resolve_mlp(const althea_gbuffer_view* g, const al
   if (st != ALTHEA_OK) return st;
   ALTHEA_REQUIRE(mlp, "deferred_resolve_mlp: null MLP");
   MlpInferenceView v{};
-  st = mlp_resolve_queue(mlp, static_cast<size_t>(g->width) * g->height, &v);
+  // One queue region per resolve wave: 64 x 4 pixel tiles, four waves each.
+  st = mlp_resolve_queue(mlp, static_cast<size_t>((g->width + 63) / 64) * ((g->height + 3) / 4) * 4, &v);
   if (st != ALTHEA_OK) return st;
   ALTHEA_REQUIRE(v.mon_mips > 0 && v.cell_ok, "deferred_resolve_mlp: the MLP's error monitor is not built");
   ALTHEA_REQUIRE(v.mon_size == ibl->prefiltered_size && v.mon_mips == ibl->prefiltered_mips,

@@ -67,21 +67,21 @@ __device__ __forceinline__ float atan_unit(float t) {
   p = fmaf(p, s, 9.999998808e-01f);
   return p * t;
 }
+// atan2 for (y, x) != (0, 0); at the origin the result is NaN (mn / mx =
+// 0 * inf), which equirect_angles replaces by its pole branch.
 __device__ __forceinline__ float atan2_fast(float y, float x) {
   const float ax = fabsf(x), ay = fabsf(y);
   const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-  const float t = mx > 0.0f ? mn * rcp_fast(mx) : 0.0f;
-  float r = atan_unit(t);
+  float r = atan_unit(mn * rcp_fast(mx));
   r = ay > ax ? 1.570796327f - r : r;
   r = x < 0.0f ? 3.141592654f - r : r;
   return copysignf(r, y);
 }
-// atan2(y, x) for x >= 0 (the elevation angle atan(y / |xz|)).
+// atan2(y, x) for x >= 0 (the elevation angle atan(y / |xz|)), (y, x) != 0.
 __device__ __forceinline__ float atan2_xpos(float y, float x) {
   const float ay = fabsf(y);
   const float mx = fmaxf(x, ay), mn = fminf(x, ay);
-  const float t = mx > 0.0f ? mn * rcp_fast(mx) : 0.0f;
-  float r = atan_unit(t);
+  float r = atan_unit(mn * rcp_fast(mx));
   r = ay > x ? 1.570796327f - r : r;
   return copysignf(r, y);
 }
@@ -141,23 +141,70 @@ __host__ inline LevelC make_level(int w, int h, uint32_t off) {
   return c;
 }
 
-// Bilinear REPEAT fetch of (yaw, pitch) from a level: the FP32 texel
-// coordinate is one FMA from the angle.
-__device__ __forceinline__ f3 level_bilinear(const char* __restrict__ pyr_bytes, const LevelC& c, float yaw,
-                                             float pitch) {
+// floor(x) as an integer in one instruction (v_cvt_flr_i32_f32).
+__device__ __forceinline__ int cvt_floor(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// The four REPEAT texels around (yaw, pitch) on a level (the FP32 texel
+// coordinate is one FMA from the angle) and the bilinear fractions.  Rows
+// wrap through the level's texel mask: the row after the last is row 0.
+struct LevelTaps {
+  f3 t00, t10, t01, t11;
+  float ax, ay;
+};
+__device__ __forceinline__ LevelTaps level_taps(const char* __restrict__ pyr_bytes, const LevelC& c, float yaw,
+                                                float pitch) {
   const float x = fmaf(yaw, c.kx, c.cx), y = fmaf(pitch, c.ky, c.cy);
-  const float fx = floorf(x), fy = floorf(y);
-  const float ax = x - fx, ay = y - fy;
-  const int ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+  LevelTaps t;
+  t.ax = __builtin_amdgcn_fractf(x);
+  t.ay = __builtin_amdgcn_fractf(y);
+  const int ix = cvt_floor(x), iy = cvt_floor(y);
   const uint32_t x0 = static_cast<uint32_t>(ix) & c.wmask, x1 = static_cast<uint32_t>(ix + 1) & c.wmask;
-  const uint32_t r0 = (static_cast<uint32_t>(iy) & c.hmask) << c.wshift;
-  const uint32_t r1 = (static_cast<uint32_t>(iy + 1) & c.hmask) << c.wshift;
+  const uint32_t lmask = ((c.hmask + 1u) << c.wshift) - 1u;  // texels of the level - 1 (wave-uniform)
+  const uint32_t r0 = (static_cast<uint32_t>(iy) << c.wshift) & lmask;
+  const uint32_t r1 = (r0 + c.wmask + 1u) & lmask;
   // The level offset goes into the (wave-uniform, scalar) base address.
   const char* lvl = pyr_bytes + (static_cast<size_t>(c.off) << 4);
-  auto ld = [&](uint32_t t) { return xyz(*reinterpret_cast<const float4*>(lvl + (t << 4))); };
-  const f3 t00 = ld(r0 + x0), t10 = ld(r0 + x1), t01 = ld(r1 + x0), t11 = ld(r1 + x1);
-  const f3 top = lerp3(t00, t10, ax), bot = lerp3(t01, t11, ax);
-  return lerp3(top, bot, ay);
+  auto ld = [&](uint32_t i) { return xyz(*reinterpret_cast<const float4*>(lvl + (i << 4))); };
+  t.t00 = ld(r0 + x0);
+  t.t10 = ld(r0 + x1);
+  t.t01 = ld(r1 + x0);
+  t.t11 = ld(r1 + x1);
+  return t;
+}
+
+// Bilinear REPEAT fetch of (yaw, pitch) from a level.
+__device__ __forceinline__ f3 level_bilinear(const char* __restrict__ pyr_bytes, const LevelC& c, float yaw,
+                                             float pitch) {
+  const LevelTaps t = level_taps(pyr_bytes, c, yaw, pitch);
+  const f3 top = lerp3(t.t00, t.t10, t.ax), bot = lerp3(t.t01, t.t11, t.ax);
+  return lerp3(top, bot, t.ay);
+}
+
+// acc += w * bilinear(yaw, pitch) in weight form: the four texel weights
+// carry w, so a sample's two source levels (w = its weight times 1 - delta
+// and times delta) add straight into the running sums.
+__device__ __forceinline__ void level_bilinear_acc(const char* __restrict__ pyr_bytes, const LevelC& c, float yaw,
+                                                   float pitch, float w, float& ar, float& ag, float& ab) {
+  const LevelTaps t = level_taps(pyr_bytes, c, yaw, pitch);
+  const float wy0 = (1.0f - t.ay) * w, wy1 = t.ay * w;
+  const float bx = 1.0f - t.ax;
+  const float w00 = bx * wy0, w10 = t.ax * wy0, w01 = bx * wy1, w11 = t.ax * wy1;
+  ar = fmaf(t.t00.x, w00, ar);
+  ag = fmaf(t.t00.y, w00, ag);
+  ab = fmaf(t.t00.z, w00, ab);
+  ar = fmaf(t.t10.x, w10, ar);
+  ag = fmaf(t.t10.y, w10, ag);
+  ab = fmaf(t.t10.z, w10, ab);
+  ar = fmaf(t.t01.x, w01, ar);
+  ag = fmaf(t.t01.y, w01, ag);
+  ab = fmaf(t.t01.z, w01, ab);
+  ar = fmaf(t.t11.x, w11, ar);
+  ag = fmaf(t.t11.y, w11, ag);
+  ab = fmaf(t.t11.z, w11, ab);
 }
 
 // Bilinear CLAMP_TO_EDGE fetch (the shading-side samplers).

@@ -46,7 +46,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <mutex>
 #include <vector>
 
 #include "device_math.hpp"
@@ -308,9 +307,55 @@ __device__ __forceinline__ float gb_srgb_to_linear(float c) {
   return c <= 0.04045f ? c * (1.0f / 12.92f) : powf((c + 0.055f) * (1.0f / 1.055f), 2.4f);
 }
 
-// The sRGB decode of every 8-bit value (the transfer function evaluated in
-// FP64 on the host, rounded to FP32), uploaded once per device.
-__device__ float gb_srgb_lut[256];
+// The sRGB decode of every 8-bit value: the transfer function
+// (c <= 0.04045 ? c / 12.92 : ((c + 0.055) / 1.055)^2.4, c = k / 255)
+// evaluated in FP64 and rounded to FP32, a constant of the code object (no
+// upload, so nothing here waits on a stream).
+__device__ const float gb_srgb_lut[256] = {
+    0.000000000e+00f, 3.035269910e-04f, 6.070539821e-04f, 9.105809731e-04f, 1.214107964e-03f, 1.517634955e-03f,
+    1.821161946e-03f, 2.124688821e-03f, 2.428215928e-03f, 2.731742803e-03f, 3.035269910e-03f, 3.346535843e-03f,
+    3.676507389e-03f, 4.024717025e-03f, 4.391442053e-03f, 4.776953254e-03f, 5.181516521e-03f, 5.605391692e-03f,
+    6.048833020e-03f, 6.512090564e-03f, 6.995410193e-03f, 7.499032188e-03f, 8.023193106e-03f, 8.568125777e-03f,
+    9.134058841e-03f, 9.721217677e-03f, 1.032982301e-02f, 1.096009370e-02f, 1.161224488e-02f, 1.228648797e-02f,
+    1.298303250e-02f, 1.370208338e-02f, 1.444384363e-02f, 1.520851441e-02f, 1.599629410e-02f, 1.680737548e-02f,
+    1.764195412e-02f, 1.850022003e-02f, 1.938236132e-02f, 2.028856240e-02f, 2.121900953e-02f, 2.217388526e-02f,
+    2.315336652e-02f, 2.415763214e-02f, 2.518685907e-02f, 2.624122240e-02f, 2.732089162e-02f, 2.842603996e-02f,
+    2.955683507e-02f, 3.071344458e-02f, 3.189603239e-02f, 3.310476616e-02f, 3.433980793e-02f, 3.560131416e-02f,
+    3.688944876e-02f, 3.820437193e-02f, 3.954623640e-02f, 4.091519862e-02f, 4.231141135e-02f, 4.373503104e-02f,
+    4.518620297e-02f, 4.666508734e-02f, 4.817182571e-02f, 4.970656708e-02f, 5.126945674e-02f, 5.286064744e-02f,
+    5.448027700e-02f, 5.612849072e-02f, 5.780543014e-02f, 5.951123685e-02f, 6.124605238e-02f, 6.301001459e-02f,
+    6.480326504e-02f, 6.662593782e-02f, 6.847816706e-02f, 7.036009431e-02f, 7.227185369e-02f, 7.421357185e-02f,
+    7.618538290e-02f, 7.818742096e-02f, 8.021982014e-02f, 8.228270710e-02f, 8.437620848e-02f, 8.650045842e-02f,
+    8.865558356e-02f, 9.084171057e-02f, 9.305896610e-02f, 9.530746937e-02f, 9.758734703e-02f, 9.989872575e-02f,
+    1.022417322e-01f, 1.046164855e-01f, 1.070231050e-01f, 1.094617099e-01f, 1.119324267e-01f, 1.144353747e-01f,
+    1.169706658e-01f, 1.195384264e-01f, 1.221387759e-01f, 1.247718185e-01f, 1.274376810e-01f, 1.301364750e-01f,
+    1.328683197e-01f, 1.356333345e-01f, 1.384316087e-01f, 1.412632912e-01f, 1.441284716e-01f, 1.470272690e-01f,
+    1.499597877e-01f, 1.529261470e-01f, 1.559264660e-01f, 1.589608341e-01f, 1.620293707e-01f, 1.651321948e-01f,
+    1.682693958e-01f, 1.714411080e-01f, 1.746474057e-01f, 1.778884232e-01f, 1.811642498e-01f, 1.844749898e-01f,
+    1.878207773e-01f, 1.912016869e-01f, 1.946178377e-01f, 1.980693191e-01f, 2.015562505e-01f, 2.050787359e-01f,
+    2.086368650e-01f, 2.122307569e-01f, 2.158605009e-01f, 2.195262015e-01f, 2.232279629e-01f, 2.269658744e-01f,
+    2.307400554e-01f, 2.345505804e-01f, 2.383975685e-01f, 2.422811240e-01f, 2.462013215e-01f, 2.501582801e-01f,
+    2.541520894e-01f, 2.581828535e-01f, 2.622506618e-01f, 2.663556039e-01f, 2.704977989e-01f, 2.746773064e-01f,
+    2.788942754e-01f, 2.831487358e-01f, 2.874408364e-01f, 2.917706370e-01f, 2.961382568e-01f, 3.005437851e-01f,
+    3.049873114e-01f, 3.094689250e-01f, 3.139887154e-01f, 3.185467720e-01f, 3.231432140e-01f, 3.277781010e-01f,
+    3.324515224e-01f, 3.371636271e-01f, 3.419144154e-01f, 3.467040658e-01f, 3.515326083e-01f, 3.564001322e-01f,
+    3.613067865e-01f, 3.662526011e-01f, 3.712376952e-01f, 3.762621284e-01f, 3.813260198e-01f, 3.864294291e-01f,
+    3.915724754e-01f, 3.967552185e-01f, 4.019777775e-01f, 4.072402120e-01f, 4.125426114e-01f, 4.178850651e-01f,
+    4.232676625e-01f, 4.286904931e-01f, 4.341536462e-01f, 4.396571815e-01f, 4.452011883e-01f, 4.507857859e-01f,
+    4.564110339e-01f, 4.620769918e-01f, 4.677838087e-01f, 4.735314846e-01f, 4.793201685e-01f, 4.851499498e-01f,
+    4.910208583e-01f, 4.969329834e-01f, 5.028864741e-01f, 5.088813305e-01f, 5.149176717e-01f, 5.209955573e-01f,
+    5.271151066e-01f, 5.332763791e-01f, 5.394794941e-01f, 5.457244515e-01f, 5.520114303e-01f, 5.583403707e-01f,
+    5.647115111e-01f, 5.711248517e-01f, 5.775804520e-01f, 5.840784311e-01f, 5.906188488e-01f, 5.972017646e-01f,
+    6.038273573e-01f, 6.104955673e-01f, 6.172065735e-01f, 6.239603758e-01f, 6.307571530e-01f, 6.375968456e-01f,
+    6.444796920e-01f, 6.514056325e-01f, 6.583748460e-01f, 6.653872728e-01f, 6.724431515e-01f, 6.795424819e-01f,
+    6.866853237e-01f, 6.938717365e-01f, 7.011018991e-01f, 7.083757520e-01f, 7.156934738e-01f, 7.230551243e-01f,
+    7.304607630e-01f, 7.379103899e-01f, 7.454041839e-01f, 7.529422045e-01f, 7.605245113e-01f, 7.681511641e-01f,
+    7.758222222e-01f, 7.835378051e-01f, 7.912979126e-01f, 7.991027236e-01f, 8.069522381e-01f, 8.148465753e-01f,
+    8.227857351e-01f, 8.307698965e-01f, 8.387989998e-01f, 8.468732238e-01f, 8.549926281e-01f, 8.631572127e-01f,
+    8.713670969e-01f, 8.796223998e-01f, 8.879231215e-01f, 8.962693810e-01f, 9.046611786e-01f, 9.130986333e-01f,
+    9.215818644e-01f, 9.301108718e-01f, 9.386857152e-01f, 9.473065138e-01f, 9.559733272e-01f, 9.646862745e-01f,
+    9.734452963e-01f, 9.822505713e-01f, 9.911020994e-01f, 1.000000000e+00f,
+};
 
 // Power-of-two extents wrap with a mask (the same value as the modulo,
 // negative coordinates included), others with an integer modulo.
@@ -675,26 +720,6 @@ size_t gbuffer_gltf_workspace_bytes(int width, int height, int triangle_count, i
          align256(static_cast<size_t>(triangle_count) * sizeof(GbAlpha));
 }
 
-// Uploads the sRGB decode table to the current device, once per device.
-hipError_t gb_upload_srgb_lut(hipStream_t s) {
-  static std::mutex mu;
-  static bool done[64] = {};
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  std::lock_guard<std::mutex> lock(mu);
-  if (dev < 64 && done[dev]) return hipSuccess;
-  static float lut[256];
-  for (int k = 0; k < 256; ++k) {
-    const double c = k / 255.0;
-    lut[k] = static_cast<float>(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
-  }
-  e = hipMemcpyToSymbolAsync(HIP_SYMBOL(gb_srgb_lut), lut, sizeof(lut), 0, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e == hipSuccess && dev < 64) done[dev] = true;
-  return e;
-}
-
 althea_status launch_gbuffer_draw_gltf(const althea_gltf_primitive* prims, int prim_count,
                                        const althea_gltf_material* materials, int material_count,
                                        const althea_texture_view* textures, int texture_count,
@@ -726,13 +751,14 @@ althea_status launch_gbuffer_draw_gltf(const althea_gltf_primitive* prims, int p
   althea_texture_view* d_tex = reinterpret_cast<althea_texture_view*>(ws);
   ws += align256(static_cast<size_t>(texture_count) * sizeof(althea_texture_view));
   p.alpha = reinterpret_cast<GbAlpha*>(ws);
-  // Pageable host tables: the copies order after the stream's earlier work.
+  // Pageable host tables: the copies order after the stream's earlier work
+  // (this draw reads its tables from host memory, so it is not captured in
+  // graphs).
   ALTHEA_TRY_HIP(hipMemcpyAsync(d_prims, table.data(), table.size() * sizeof(GbPrim), hipMemcpyHostToDevice, s));
   ALTHEA_TRY_HIP(hipMemcpyAsync(d_mat, materials, static_cast<size_t>(material_count) * sizeof(althea_gltf_material),
                                 hipMemcpyHostToDevice, s));
   ALTHEA_TRY_HIP(hipMemcpyAsync(d_tex, textures, static_cast<size_t>(texture_count) * sizeof(althea_texture_view),
                                 hipMemcpyHostToDevice, s));
-  ALTHEA_TRY_HIP(gb_upload_srgb_lut(s));
   p.prims = d_prims;
   p.prim_count = prim_count;
   p.materials = d_mat;

@@ -159,9 +159,11 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
         entry.lx = static_cast<float>(2.0 * hz * hx);
         entry.ly = static_cast<float>(2.0 * hz * hy);
         entry.lz = static_cast<float>(lz);
-        entry.delta = lodf - fl;
+        const double delta = static_cast<double>(lodf - fl);
         entry.l0 = static_cast<int>(fl);
         entry.l1 = min(entry.l0 + 1, p.env_levels - 1);
+        entry.w0 = static_cast<float>(lz * (1.0 - delta));
+        entry.w1 = static_cast<float>(lz * delta);
         entry.weight = static_cast<float>(lz);
         my_sum += lz;
         w32 = static_cast<float>(lz);
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
     // entry carrying their summed weight gives the same weighted mean (v * n /
     // n, exact for n <= 2^24) with one lookup instead of n.
     if (rough == 0.0 && s_base > 1) {
-      tab[0].weight = static_cast<float>(s_base);
+      tab[0].weight = tab[0].w0 = static_cast<float>(s_base);
       counts[m] = 1;
     }
   }
@@ -473,8 +475,10 @@ struct IrradianceParams {
   double scale;  // pi / theta_samples / phi_samples
 };
 
-__global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float2* __restrict__ theta_tab,
-                                         float4* __restrict__ phi_tab) {
+// phi_tab[j] = (cos, sin, cos sin (1 - delta), cos sin delta): the sample
+// weight split over the two source levels (delta = the LOD's fraction).
+__global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float delta,
+                                         float2* __restrict__ theta_tab, float4* __restrict__ phi_tab) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double pi = 3.14159265358979323846;
   if (i < theta_samples) {
@@ -483,8 +487,9 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
   }
   if (i < phi_samples) {
     double ph = i * 0.5 * pi / phi_samples;
-    double c = cos(ph), s = sin(ph);
-    phi_tab[i] = make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(c * s), 0.0f);
+    double c = cos(ph), s = sin(ph), d = static_cast<double>(delta);
+    phi_tab[i] = make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(c * s * (1.0 - d)),
+                             static_cast<float>(c * s * d));
   }
 }
 
@@ -518,11 +523,8 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __rest
                       fmaf(a.z, ph.y, fr.n.z * ph.x));
       float yaw, pitch;
       equirect_angles(d, yaw, pitch);
-      f3 v = level_bilinear(pyr, p.lv0, yaw, pitch);
-      if (p.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, p.lv1, yaw, pitch), p.delta);
-      sr = fmaf(v.x, ph.z, sr);
-      sg = fmaf(v.y, ph.z, sg);
-      sb = fmaf(v.z, ph.z, sb);
+      level_bilinear_acc(pyr, p.lv0, yaw, pitch, ph.z, sr, sg, sb);
+      if (p.delta != 0.0f) level_bilinear_acc(pyr, p.lv1, yaw, pitch, ph.w, sr, sg, sb);
     }
     if (kF64Sums) {
       ar += sr;
@@ -574,11 +576,6 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   char* ws = static_cast<char*>(workspace);
   float2* theta_tab = reinterpret_cast<float2*>(ws);
   float4* phi_tab = reinterpret_cast<float4*>(ws + align256(sizeof(float2) * theta_samples));
-  int n = std::max(theta_samples, phi);
-  hipLaunchKernelGGL(irradiance_tables_kernel, dim3((n + 255) / 256), dim3(256), 0, s, theta_samples, phi,
-                     theta_tab, phi_tab);
-  althea_status st = check_launch("irradiance_tables_kernel");
-  if (st != ALTHEA_OK) return st;
 
   IrradianceParams p{};
   PyramidGeometry g = pyramid_geometry(env_w, env_h);
@@ -596,6 +593,11 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.lv0 = make_level(g.w[p.l0], g.h[p.l0], g.off[p.l0]);
   p.lv1 = make_level(g.w[p.l1], g.h[p.l1], g.off[p.l1]);
   p.scale = M_PI / theta_samples / phi;
+  const int n = std::max(theta_samples, phi);
+  hipLaunchKernelGGL(irradiance_tables_kernel, dim3((n + 255) / 256), dim3(256), 0, s, theta_samples, phi, p.delta,
+                     theta_tab, phi_tab);
+  althea_status st = check_launch("irradiance_tables_kernel");
+  if (st != ALTHEA_OK) return st;
   const dim3 grid((6 * size * size + 3) / 4);
   const char* pyr4 = reinterpret_cast<const char*>(pyr);
   float4* cube4 = reinterpret_cast<float4*>(cube);

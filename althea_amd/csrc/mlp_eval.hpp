@@ -1,5 +1,5 @@
 // Device building blocks of the prefiltered-radiance MLP shared by the
-// training kernels, the error monitor and the fused MLP-IBL resolve.
+// training kernels, the error monitor and the MLP-IBL resolve.
 //
 // MFMA lane layout (v_mfma_f32_16x16x32_bf16, lane l = 16 g + n):
 //   A (16 x 32): lane holds W[row = n][k = 8 g + j], j = 0..7
@@ -72,6 +72,67 @@ __device__ __forceinline__ void encode_all(float dx, float dy, float dz, float r
   }
   f[27] = rough;
   f[28] = f[29] = f[30] = f[31] = 0.0f;
+}
+
+// -------------------------------------------- register-resident inference
+// The forward A operands of this lane (feature group g = lane / 16, row
+// n = lane % 16 of each 16-row block), 88 VGPRs, loaded once per wave; the
+// streaming inference layout of mlp_fwd_kernel.
+struct MlpRegWeights {
+  bf16x8 a1[4], a2[4][2], a3[4][2], a4[2];
+};
+__device__ __forceinline__ void mlp_load_reg_weights(const MlpDeviceWeights& w, int g, int n, MlpRegWeights& r) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    r.a1[b] = load8(w.w1 + (16 * b + n) * 32 + 8 * g);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      r.a2[b][kb] = load8(w.w2 + (16 * b + n) * 64 + 32 * kb + 8 * g);
+      r.a3[b][kb] = load8(w.w3 + (16 * b + n) * 64 + 32 * kb + 8 * g);
+    }
+  }
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) r.a4[kb] = load8(w.w4 + n * 64 + 32 * kb + 8 * g);
+}
+
+// Log-radiance of one 16-query tile: lane (g, n) passes query n (every group
+// of the tile passes the same query n); the result for query n sits in lane
+// n of group 0 (rows 0..2 of y).  Arithmetic identical to mlp_fwd_kernel's
+// inference: same features, MFMA order, bias and SiLU.
+__device__ __forceinline__ f32x4 mlp_tile_eval(const MlpRegWeights& r, const MlpDeviceWeights& w, int g, float dx,
+                                               float dy, float dz, float rough) {
+  float f[32];
+  encode_all(dx, dy, dz, rough, f);
+  bf16x8 x;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    x[j] = static_cast<__bf16>(g == 0 ? f[j] : (g == 1 ? f[8 + j] : (g == 2 ? f[16 + j] : f[24 + j])));
+  f32x4 h[4];
+  auto hidden = [&](auto&& wtile, const bf16x8* in, int nk, const float* bias) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < nk; ++kb) acc = mfma(wtile(b, kb), in[kb], acc);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float z = acc[i] + bias[16 * b + 4 * g + i];
+        h[b][i] = z * sigmoidf(z);
+      }
+    }
+  };
+  hidden([&](int b, int) { return r.a1[b]; }, &x, 1, w.b1);
+  const bf16x8 hb1[2] = {pack8(h[0], h[1]), pack8(h[2], h[3])};
+  hidden([&](int b, int kb) { return r.a2[b][kb]; }, hb1, 2, w.b2);
+  const bf16x8 hb2[2] = {pack8(h[0], h[1]), pack8(h[2], h[3])};
+  hidden([&](int b, int kb) { return r.a3[b][kb]; }, hb2, 2, w.b3);
+  const bf16x8 hb3[2] = {pack8(h[0], h[1]), pack8(h[2], h[3])};
+  f32x4 y = {0.f, 0.f, 0.f, 0.f};
+  y = mfma(r.a4[0], hb3[0], y);
+  y = mfma(r.a4[1], hb3[1], y);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) y[i] += w.b4[4 * g + i];
+  return y;
 }
 
 // ------------------------------------------------- LDS-resident inference

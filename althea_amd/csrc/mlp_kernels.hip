@@ -81,11 +81,12 @@ struct FwdParams {
   int count;
   float grad_scale;       // 2 / (3 * batch)
   MlpActivations act;     // feature-major copies (training)
-  float* bias_partial;    // [gridDim.x][kMlpBiasPartial] (training)
+  float* bias_partial;    // [gridDim.x][kMlpBiasPartial] (training: the loss column)
 };
 
 // One 16-sample tile per iteration per wave.  kTrain adds the loss, the
-// backward-data chain and the feature-major stores.
+// backward-data chain and the feature-major stores (the weight-gradient
+// kernel forms every weight and bias gradient from those).
 template <bool kTrain>
 __global__ __launch_bounds__(256) void mlp_fwd_kernel(const FwdParams p) {
   const int lane = threadIdx.x & 63;
@@ -115,9 +116,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const FwdParams p) {
   auto A3 = [&](int b, int kb) { return kTrain ? load8(p.w.w3 + (16 * b + n) * 64 + 32 * kb + 8 * g) : a3[b][kb]; };
   auto A4 = [&](int kb) { return kTrain ? load8(p.w.w4 + n * 64 + 32 * kb + 8 * g) : a4[kb]; };
 
-  // Training accumulators: bias gradients for this lane's features, loss.
-  float db1[4][4] = {}, db2[4][4] = {}, db3[4][4] = {}, db4[4] = {};
-  float loss = 0.0f;
+  float loss = 0.0f;  // training: this lane's squared log errors
 
   for (int tile = blockIdx.x * 4 + wave; tile < tiles; tile += waves_total) {
     const int s = tile * 16 + n;
@@ -194,8 +193,6 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const FwdParams p) {
       dy[1] = p.grad_scale * e1;
       dy[2] = p.grad_scale * e2;
     }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) db4[i] += dy[i];
 
     bf16x8 dyb;
 #pragma unroll
@@ -206,8 +203,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const FwdParams p) {
     }
 
     // ---- backward-data: dH = W^T dZ, dZ = dH * SiLU'(z), layer by layer.
-    auto backward = [&](auto&& wtile, const bf16x8* in, int nk, const f32x4* sp, float (*db)[4], __bf16* store,
-                        f32x4* dz) {
+    auto backward = [&](auto&& wtile, const bf16x8* in, int nk, const f32x4* sp, __bf16* store, f32x4* dz) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -217,67 +213,48 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const FwdParams p) {
         for (int i = 0; i < 4; ++i) {
           const float d = acc[i] * sp[b][i];
           dz[b][i] = d;
-          db[b][i] += d;
           store[(16u * b + 4u * g + i) * B + col] = static_cast<__bf16>(d);
         }
       }
     };
     f32x4 dz[4];
-    backward([&](int b, int) { return load8(p.w.w4t + (16 * b + n) * 32 + 8 * g); }, &dyb, 1, sp3, db3, A.dz3, dz);
+    backward([&](int b, int) { return load8(p.w.w4t + (16 * b + n) * 32 + 8 * g); }, &dyb, 1, sp3, A.dz3, dz);
     bf16x8 d_in[2] = {pack8(dz[0], dz[1]), pack8(dz[2], dz[3])};
     backward([&](int b, int kb) { return load8(p.w.w3t + (16 * b + n) * 64 + 32 * kb + 8 * g); }, d_in, 2, sp2,
-             db2, A.dz2, dz);
+             A.dz2, dz);
     d_in[0] = pack8(dz[0], dz[1]);
     d_in[1] = pack8(dz[2], dz[3]);
     backward([&](int b, int kb) { return load8(p.w.w2t + (16 * b + n) * 64 + 32 * kb + 8 * g); }, d_in, 2, sp1,
-             db1, A.dz1, dz);
+             A.dz1, dz);
   }
 
   if (!kTrain) return;
-  // Reduce the bias partials over the 16 samples of each lane group and over
-  // the block's waves; one row of partials per block, fixed order.
-  __shared__ float s_red[4][kMlpBiasPartial];
-  auto reduce16 = [&](float v) {
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
-  };
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float r1 = reduce16(db1[b][i]), r2 = reduce16(db2[b][i]), r3 = reduce16(db3[b][i]);
-      if (n == 0) {
-        s_red[wave][kBiasB1 + 16 * b + 4 * g + i] = r1;
-        s_red[wave][kBiasB2 + 16 * b + 4 * g + i] = r2;
-        s_red[wave][kBiasB3 + 16 * b + 4 * g + i] = r3;
-      }
-    }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float r4 = reduce16(db4[i]);
-    if (n == 0 && g == 0 && i < kMlpOut) s_red[wave][kBiasB4 + i] = r4;
-  }
+  // The block's loss in the loss column of its partial row, fixed order.
+  __shared__ float s_loss[4];
   float l = loss;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) l += __shfl_xor(l, o, 64);
-  if (lane == 0) s_red[wave][kBiasLoss] = l;
+  if (lane == 0) s_loss[wave] = l;
   __syncthreads();
-  for (int k = threadIdx.x; k < kMlpBiasPartial; k += blockDim.x)
-    p.bias_partial[static_cast<size_t>(blockIdx.x) * kMlpBiasPartial + k] =
-        s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
+  if (threadIdx.x == 0)
+    p.bias_partial[static_cast<size_t>(blockIdx.x) * kMlpBiasPartial + kBiasLoss] =
+        s_loss[0] + s_loss[1] + s_loss[2] + s_loss[3];
 }
 
 // ------------------------------------------------------ weight gradients
 // dW_l[o][k] = sum_s dZ_l[o][s] H_{l-1}[k][s] with samples on the MFMA K axis:
 // A = dZ tile (lane: 8 consecutive samples of one output row), B = H tile.
-// Each block owns a contiguous sample range; wave w owns output block w of
-// layers 1-3 and input block w of layer 4.  Partials: one row per block.
+// The bias gradients db_l[o] = sum_s dZ_l[o][s] are the same product with a
+// B tile of ones.  Each block owns a contiguous sample range; wave w owns
+// output block w of layers 1-3 and input block w of layer 4 (wave 0 also the
+// layer-4 bias).  Partials: one row of weights and one row of biases per
+// block (the forward kernel writes the loss column of the bias row).
 struct WgradParams {
   MlpActivations act;
   int count;         // samples (multiple of 32 after padding)
   int chunks_per_block;
-  float* partial;    // [gridDim.x][kMlpWeights]
+  float* partial;       // [gridDim.x][kMlpWeights]
+  float* bias_partial;  // [gridDim.x][kMlpBiasPartial], columns before kBiasLoss
 };
 
 __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const WgradParams p) {
@@ -287,6 +264,10 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const WgradParams p) {
   const MlpActivations& A = p.act;
   const size_t B = A.stride;
   f32x4 d1[2] = {}, d2[4] = {}, d3[4] = {}, d4 = {};
+  f32x4 e1 = {}, e2 = {}, e3 = {}, e4 = {};  // bias sums (every column of D the same)
+  bf16x8 ones;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) ones[j] = static_cast<__bf16>(1.0f);
   const int chunk0 = blockIdx.x * p.chunks_per_block;
   const int nchunks = (p.count + 31) / 32;
   for (int c = chunk0; c < min(chunk0 + p.chunks_per_block, nchunks); ++c) {
@@ -303,6 +284,21 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const WgradParams p) {
       d3[kb] = mfma(z3, load8(A.h2 + (16 * kb + n) * B + s0), d3[kb]);
     }
     d4 = mfma(y, load8(A.h3 + (16 * w + n) * B + s0), d4);
+    e1 = mfma(z1, ones, e1);
+    e2 = mfma(z2, ones, e2);
+    e3 = mfma(z3, ones, e3);
+    if (w == 0) e4 = mfma(y, ones, e4);
+  }
+  float* bout = p.bias_partial + static_cast<size_t>(blockIdx.x) * kMlpBiasPartial;
+  if (n == 0) {  // column 0 of D: row o = 16 w + 4 g + i
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = 16 * w + 4 * g + i;
+      bout[kBiasB1 + o] = e1[i];
+      bout[kBiasB2 + o] = e2[i];
+      bout[kBiasB3 + o] = e3[i];
+      if (w == 0 && o < kMlpOut) bout[kBiasB4 + o] = e4[i];
+    }
   }
   float* out = p.partial + static_cast<size_t>(blockIdx.x) * kMlpWeights;
   // D: lane holds dW[o = 16 w + 4 g + i][k = 16 kb + n].
@@ -328,8 +324,9 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const WgradParams p) {
 struct AdamParams {
   const float* wpartial;  // [wblocks][kMlpWeights]
   int wblocks;
-  const float* bpartial;  // [bblocks][kMlpBiasPartial]
+  const float* bpartial;  // [lblocks][kMlpBiasPartial]: bias columns in rows < bblocks, loss in rows < lblocks
   int bblocks;
+  int lblocks;
   float* params;          // FP32 master copy, oracle order
   float* m;
   float* v;
@@ -351,7 +348,7 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(const AdamParams p) {
     if (!p.loss_out) return;
     __shared__ float s_loss[256];
     float l = 0.0f;
-    for (int b = threadIdx.x; b < p.bblocks; b += 256)
+    for (int b = threadIdx.x; b < p.lblocks; b += 256)
       l += p.bpartial[static_cast<size_t>(b) * kMlpBiasPartial + kBiasLoss];
     s_loss[threadIdx.x] = l;
     __syncthreads();
@@ -498,11 +495,17 @@ struct MlpState {
   int mon_size = 0, mon_mips = 0, mon_accepted = 0;
   // Queue of the MLP-IBL resolve's deferred pixels: a counter, then records.
   void* queue_mem = nullptr;
-  size_t queue_records = 0;
+  size_t queue_regions = 0;
 };
 
-static constexpr int kFwdBlocks = 256;    // training forward grid
-static constexpr int kWgradBlocks = 256;  // weight-gradient grid
+// Training grids.  The forward (about 240 registers per lane: two waves per
+// SIMD) runs two waves per SIMD; the weight gradients keep one partial row
+// per block, so their grid stays at one wave per SIMD (10 MB of partials).
+// Row b of the bias partials holds the weight-gradient block b's bias sums
+// and the forward block b's loss.
+static constexpr int kFwdBlocks = 512;
+static constexpr int kWgradBlocks = 256;
+static_assert(kFwdBlocks >= kWgradBlocks, "bias partial rows");
 
 static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
@@ -579,6 +582,7 @@ static althea_status run_train(MlpState* s, const float* dir_rough, const float*
   int nchunks = (count + 31) / 32;
   wp.chunks_per_block = (nchunks + kWgradBlocks - 1) / kWgradBlocks;
   wp.partial = s->wpartial;
+  wp.bias_partial = s->bpartial;
   hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(kWgradBlocks), dim3(256), 0, stream, wp);
   st = check_launch("mlp_wgrad_kernel");
   if (st != ALTHEA_OK) return st;
@@ -586,7 +590,8 @@ static althea_status run_train(MlpState* s, const float* dir_rough, const float*
   ap.wpartial = s->wpartial;
   ap.wblocks = kWgradBlocks;
   ap.bpartial = s->bpartial;
-  ap.bblocks = kFwdBlocks;
+  ap.bblocks = kWgradBlocks;
+  ap.lblocks = kFwdBlocks;
   ap.params = s->params;
   ap.m = s->m;
   ap.v = s->v;
@@ -630,28 +635,32 @@ althea_status mlp_inference_view(const althea_mlp* m, MlpInferenceView* v) {
   v->mon_mips = m->s.mon_mips;
   v->mon_accepted = m->s.mon_accepted;
   v->queue = nullptr;
-  v->queue_count = nullptr;
-  v->queue_capacity = 0;
+  v->region_count = nullptr;
+  v->queue_regions = 0;
   return ALTHEA_OK;
 }
 
-althea_status mlp_resolve_queue(const althea_mlp* m, size_t records, MlpInferenceView* v) {
+// Queue storage: the region counts (padded to 256 bytes), then the records.
+static size_t queue_count_bytes(size_t regions) { return (regions * sizeof(uint32_t) + 255) / 256 * 256; }
+
+althea_status mlp_resolve_queue(const althea_mlp* m, size_t regions, MlpInferenceView* v) {
   althea_status st = mlp_inference_view(m, v);
   if (st != ALTHEA_OK) return st;
   // The queue is the handle's scratch (like its activations), so a const
   // handle may grow it.
   MlpState* s = const_cast<MlpState*>(&m->s);
-  if (records > s->queue_records) {
+  if (regions > s->queue_regions) {
     ALTHEA_TRY_HIP(hipDeviceSynchronize());  // an earlier frame may still read the old queue
     (void)hipFree(s->queue_mem);
     s->queue_mem = nullptr;
-    s->queue_records = 0;
-    ALTHEA_TRY_HIP(hipMalloc(&s->queue_mem, 256 + records * 3 * sizeof(float4)));
-    s->queue_records = records;
+    s->queue_regions = 0;
+    ALTHEA_TRY_HIP(hipMalloc(&s->queue_mem,
+                             queue_count_bytes(regions) + regions * kMlpQueueRegion * 3 * sizeof(float4)));
+    s->queue_regions = regions;
   }
-  v->queue_count = static_cast<uint32_t*>(s->queue_mem);
-  v->queue = reinterpret_cast<float4*>(static_cast<char*>(s->queue_mem) + 256);
-  v->queue_capacity = s->queue_records;
+  v->region_count = static_cast<uint32_t*>(s->queue_mem);
+  v->queue = reinterpret_cast<float4*>(static_cast<char*>(s->queue_mem) + queue_count_bytes(s->queue_regions));
+  v->queue_regions = s->queue_regions;
   return ALTHEA_OK;
 }
 }  // namespace althea

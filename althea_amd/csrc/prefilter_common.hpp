@@ -8,21 +8,24 @@
 namespace althea {
 
 // Sample-table entry (one per Hammersley sample with NdotL > 0, in sample
-// order): L in the texel frame (T, B, N), the source LOD split into
-// (l0, l1, delta) and the sample's weight in the NdotL-weighted mean (NdotL =
-// lz, or the summed weight of coinciding samples).  32 bytes, one scalar load.
+// order): L in the texel frame (T, B, N), the source LOD split into levels
+// (l0, l1) and fraction delta, and the sample's weight in the NdotL-weighted
+// mean (NdotL = lz, or the summed weight of coinciding samples) with its
+// shares of the two levels, w0 = weight (1 - delta) and w1 = weight delta
+// (rounded once from FP64; w1 = 0 where the sample reads one level).
+// 32 bytes, one scalar load.
 struct GgxEntry {
-  float lx, ly, lz, delta;
+  float lx, ly, lz, w0;
   int l0, l1;
-  float weight;
-  int pad;
+  float w1, weight;
 };
 static_assert(sizeof(GgxEntry) == 32, "one scalar load per entry");
 
 constexpr int kBlockSums = 32;  // FP32 partial sums flushed to FP64 every 32 samples
 
 // NdotL-weighted mean of the source over the table (PreFilterEnvMap.comp:147-169):
-// the FP32 per-sample body, FP64 running sums (kF64Sums) or all-FP32 sums.
+// the FP32 per-sample body (each level's bilinear in weight form, added into
+// the running sums), FP64 running sums (kF64Sums) or all-FP32 sums.
 // `level(l)` returns the LevelC of source level l.
 template <bool kF64Sums, class LevelOf>
 __device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, const LevelOf& level,
@@ -37,11 +40,8 @@ __device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, 
                     fmaf(fr.t.z, en.lx, fmaf(fr.b.z, en.ly, fr.n.z * en.lz)));
     float yaw, pitch;
     equirect_angles(L, yaw, pitch);
-    f3 v = level_bilinear(pyr, level(en.l0), yaw, pitch);
-    if (en.delta != 0.0f) v = lerp3(v, level_bilinear(pyr, level(en.l1), yaw, pitch), en.delta);
-    sr = fmaf(v.x, en.weight, sr);
-    sg = fmaf(v.y, en.weight, sg);
-    sb = fmaf(v.z, en.weight, sb);
+    level_bilinear_acc(pyr, level(en.l0), yaw, pitch, en.w0, sr, sg, sb);
+    if (__float_as_uint(en.w1) != 0u) level_bilinear_acc(pyr, level(en.l1), yaw, pitch, en.w1, sr, sg, sb);
     if (kF64Sums) {
       if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
         ar += sr;

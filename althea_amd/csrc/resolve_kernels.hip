@@ -23,7 +23,6 @@
 #include <cmath>
 
 #include "device_math.hpp"
-#include "fill.hpp"
 #include "internal.hpp"
 #include "mlp_eval.hpp"
 #include "precise_lookup.hpp"
@@ -164,6 +163,11 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
     float4 q[kSsaoBatch];
 #pragma unroll
     for (int j = 0; j < kSsaoBatch; ++j) q[j] = load_gbuf<kFmt>(p.position, idx[j]);
+    // The first gather is consumed here, so it is issued with the others:
+    // otherwise the compiler moves it under that tap's validity test and
+    // splits it into two dependent loads (w, then xyz).  (Pinning all four
+    // this way costs 4 VGPRs, one wave per SIMD.)
+    asm volatile("" : "+v"(q[0].x), "+v"(q[0].y), "+v"(q[0].z), "+v"(q[0].w));
 #pragma unroll
     for (int j = 0; j < kSsaoBatch; ++j) {
       if (!ok[j] || q[j].w == 0.0f) continue;
@@ -174,11 +178,6 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
   return fmaf(-occ, p.ssao_inv_samples, 1.0f);
 }
 
-// LDS of the MLP finishing kernel: the per-wave query staging, then the weights.
-struct MlpFinishSmem {
-  MlpWaveStage stage[4];
-  MlpSmem mlp;
-};
 
 // One 64 x 4 tile, row-major tile id.  Tiles are taken in launch order: a
 // mapping that gives each XCD (with its own L2) contiguous runs of tiles was
@@ -191,7 +190,10 @@ struct MlpFinishSmem {
 // float4 (R, roughness | (F A + B) ao per channel, pixel index | the colour
 // without the prefiltered term), which resolve_mlp_finish_kernel completes
 // with the MLP's radiance.  Every other pixel is shaded exactly as by the
-// plain resolve.
+// plain resolve.  Wave w of the grid (block * 4 + wave) owns queue records
+// [64 w, 64 w + 64): its deferred lanes take consecutive records in lane
+// order and lane 0 writes the wave's count, zero included, so the queue needs
+// neither atomics nor clearing.
 template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false, bool kShadow = false>
 __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, int tiles_x,
                                              const float4* __restrict__ lights, const float4* s_noise) {
@@ -236,6 +238,7 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
   const float lod = roughness * static_cast<float>(p.pref.mips - 1);
   f3 refl = mk(0.0f, 0.0f, 0.0f);
   bool from_mlp = false;
+  uint32_t mlp_slot = 0;
   if (kMlp) {
     // Monitor cell of R at the two mips the cube lookup would blend.
     int rface;
@@ -251,6 +254,13 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     from_mlp = shade && p.mlp.cell_ok[m0 * per_mip + cell] != 0 &&
                (d == fl || p.mlp.cell_ok[m1 * per_mip + cell] != 0);
     if (inside && p.mlp_used) p.mlp_used[idx] = from_mlp ? 1 : 0;
+    // The query goes out now, the colour after the lights.
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(from_mlp);
+    const int lane = threadIdx.x & 63;
+    const uint32_t region = static_cast<uint32_t>(tile) * 4u + (threadIdx.x >> 6);
+    if (lane == 0) p.mlp.region_count[region] = static_cast<uint32_t>(__popcll(mask));
+    mlp_slot = region * kMlpQueueRegion + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (from_mlp) p.mlp.queue[3 * static_cast<size_t>(mlp_slot)] = make_float4(R.x, R.y, R.z, roughness);
   }
   const float4 alb = shade ? load_gbuf<kFmt>(p.albedo, idx) : zero4;
   const PbrPoint m = pbr_point(mk(alb.x, alb.y, alb.z), metallic, roughness, NI);
@@ -295,10 +305,12 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
     color = pbr_environment(m, irr, refl, A, B, ao);
     if (kMlp && from_mlp) {
-      // The prefiltered radiance's factor in pbr_environment, for the finishing kernel.
+      // The prefiltered term's factor (F A + B) ao per channel, F at NdotV
+      // (pbr_environment), and the pixel index.
       const float fx = pow5(1.0f - m.n_dot_v);
-      refl = mk(fmaf(fmaf(m.dF.x, fx, m.F0.x), A, B) * ao, fmaf(fmaf(m.dF.y, fx, m.F0.y), A, B) * ao,
-                fmaf(fmaf(m.dF.z, fx, m.F0.z), A, B) * ao);
+      p.mlp.queue[3 * static_cast<size_t>(mlp_slot) + 1] =
+          make_float4(fmaf(fmaf(m.dF.x, fx, m.F0.x), A, B) * ao, fmaf(fmaf(m.dF.y, fx, m.F0.y), A, B) * ao,
+                      fmaf(fmaf(m.dF.z, fx, m.F0.z), A, B) * ao, __uint_as_float(static_cast<uint32_t>(idx)));
     }
   }
   const auto view_df = [&]() { return view_ray_df(p.ray_df, cxp, cyp); };
@@ -306,23 +318,8 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color, p.shadow, view_df);
   else
     pbr_lights(m, P, N, I, NI, __any(shade), lights, p.light_count, color, NoShadow(), view_df);
-  if (kMlp) {
-    // Deferred pixels take consecutive queue slots: one atomic per wave.
-    const uint64_t mask = __builtin_amdgcn_ballot_w64(from_mlp);
-    if (mask != 0) {
-      const int lane = threadIdx.x & 63;
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(p.mlp.queue_count, static_cast<uint32_t>(__popcll(mask)));
-      base = __shfl(base, 0, 64);
-      const uint32_t slot = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-      if (from_mlp && slot < p.mlp.queue_capacity) {
-        float4* q = p.mlp.queue + 3 * static_cast<size_t>(slot);
-        q[0] = make_float4(R.x, R.y, R.z, roughness);
-        q[1] = make_float4(refl.x, refl.y, refl.z, __uint_as_float(static_cast<uint32_t>(idx)));
-        q[2] = make_float4(color.x, color.y, color.z, 0.0f);
-      }
-    }
-  }
+  if (kMlp && from_mlp)  // the colour without the prefiltered term
+    p.mlp.queue[3 * static_cast<size_t>(mlp_slot) + 2] = make_float4(color.x, color.y, color.z, 0.0f);
   if (shade && !(kMlp && from_mlp)) {
     if (p.tonemap) color = tonemap(color, p.exposure);
     store_out<kOutF16>(p.out, idx, color);
@@ -343,40 +340,60 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, con
   resolve_tile<kFmt, kOutF16, kMlp, kRefl, kShadow>(p, blockIdx.x, tiles_x, lights, s_noise);
 }
 
-// Completes the pixels the MLP-IBL resolve deferred: the MLP (weights in LDS,
-// each wave 64 consecutive queue records as four 16-sample MFMA tiles) gives
-// the log-radiance at (R, roughness), and the pixel is
-// colour + exp(y) (F A + B) ao, tonemapped as the resolve would.  Persistent
-// blocks; a block with no record exits before staging the weights.
-template <bool kOutF16>
-__global__ __launch_bounds__(256) void resolve_mlp_finish_kernel(const ResolveParams p) {
-  __shared__ MlpFinishSmem s;
-  const uint32_t n = min(*p.mlp.queue_count, static_cast<uint32_t>(p.mlp.queue_capacity));
-  if (blockIdx.x * 256u >= n) return;  // uniform over the block
-  mlp_stage_weights(s.mlp, p.mlp.w, threadIdx.x, 256);
+// The MLP-IBL resolve held to the plain resolve's six waves per SIMD (80
+// registers per lane), so that a frame whose MLP serves few pixels costs what
+// the cube-IBL frame costs.
+template <int kFmt, bool kOutF16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void resolve_mlp_kernel(
+    const ResolveParams p, const float4* __restrict__ lights) {
+  __shared__ float4 s_noise[16];
+  if (threadIdx.x < 16)
+    s_noise[threadIdx.x] =
+        make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
-    const uint32_t r = base + 64u * wave + lane;
-    const bool have = r < n;
-    float4 q0 = make_float4(0.0f, 0.0f, 1.0f, 1.0f), q1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q2 = q1;
-    if (have) {
-      const float4* q = p.mlp.queue + 3 * static_cast<size_t>(r);
-      q0 = q[0];
-      q1 = q[1];
-      q2 = q[2];
-    }
-    const f3 ly = mlp_wave_eval(s.mlp, s.stage[wave], q0.x, q0.y, q0.z, q0.w, have);
-    if (have) {
-      f3 c = mk(fmaf(expf(ly.x), q1.x, q2.x), fmaf(expf(ly.y), q1.y, q2.y), fmaf(expf(ly.z), q1.z, q2.z));
-      if (p.tonemap) c = tonemap(c, p.exposure);
-      store_out<kOutF16>(p.out, __float_as_uint(q1.w), c);
+  const int tiles_x = (p.width + 63) / 64;
+  resolve_tile<kFmt, kOutF16, true>(p, blockIdx.x, tiles_x, lights, s_noise);
+}
+
+// Completes the pixels the MLP-IBL resolve deferred.  Persistent waves take
+// queue regions in turn (wave-uniform counts); a wave loads the MLP's forward
+// operands into registers at its first non-empty region and evaluates each
+// region in 16-record tiles: lane (g, n) evaluates feature group g of record
+// n, and lane n of group 0 forms the pixel colour + exp(y) (F A + B) ao,
+// tonemapped as the resolve would, and stores it.  The MLP arithmetic is the
+// streaming inference kernel's (althea_mlp_infer).  No barriers, no atomics.
+template <bool kOutF16>
+__global__ __launch_bounds__(256) void resolve_mlp_finish_kernel(const ResolveParams p, uint32_t regions) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+  const uint32_t waves = gridDim.x * 4u;
+  auto count_of = [&](uint32_t region) {
+    return min(__builtin_amdgcn_readfirstlane(p.mlp.region_count[region]), static_cast<uint32_t>(kMlpQueueRegion));
+  };
+  uint32_t region = blockIdx.x * 4u + static_cast<uint32_t>(threadIdx.x >> 6);
+  while (region < regions && count_of(region) == 0) region += waves;
+  if (region >= regions) return;
+  MlpRegWeights r;
+  mlp_load_reg_weights(p.mlp.w, g, n, r);
+  for (; region < regions; region += waves) {
+    const uint32_t count = count_of(region);
+    for (uint32_t t = 0; t < count; t += 16u) {
+      const uint32_t k = t + static_cast<uint32_t>(n);
+      const bool valid = k < count;
+      const float4* q = p.mlp.queue + 3 * (static_cast<size_t>(region) * kMlpQueueRegion + (valid ? k : 0u));
+      const float4 q0 = valid ? q[0] : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+      const f32x4 y = mlp_tile_eval(r, p.mlp.w, g, q0.x, q0.y, q0.z, q0.w);
+      if (g == 0 && valid) {
+        const float4 q1 = q[1], q2 = q[2];
+        f3 c = mk(fmaf(expf(y[0]), q1.x, q2.x), fmaf(expf(y[1]), q1.y, q2.y), fmaf(expf(y[2]), q1.z, q2.z));
+        if (p.tonemap) c = tonemap(c, p.exposure);
+        store_out<kOutF16>(p.out, __float_as_uint(q1.w), c);
+      }
     }
   }
 }
 
-// Persistent blocks of the MLP finishing kernel per CU (about 150 VGPRs: 3
-// waves per SIMD).
+// Blocks of the MLP finishing kernel per CU: about 160 registers per lane
+// (88 of them the weights) allow 3 waves per SIMD.
 constexpr int kMlpFinishBlocksPerCu = 3;
 
 __global__ void fill_u8_kernel(uint8_t* __restrict__ p, int n) {
@@ -489,33 +506,34 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
     // finishing kernel over the queue: the MLP's cost follows its pixels.
     p.mlp = *mlp;
     p.mlp_used = mlp_used;
-    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(fill_blocks(1, sizeof(uint32_t))), dim3(256), 0, s,
-                       p.mlp.queue_count, static_cast<size_t>(1), 0u);
-    althea_status st = check_launch("fill_kernel");
-    if (st != ALTHEA_OK) return st;
+    const uint32_t regions = static_cast<uint32_t>(tiles) * 4u;
+    if (p.mlp.queue_regions < regions) {
+      set_error("deferred_resolve_mlp: queue of %zu regions, the frame needs %u", p.mlp.queue_regions, regions);
+      return ALTHEA_ERROR_INVALID_ARGUMENT;
+    }
     const dim3 grid(tiles);
     if (fp32 && f16_out)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true, false, false, true>), grid, dim3(256), 0, s, p,
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p,
                          lights4);
     else if (fp32)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false, false, false, true>), grid, dim3(256), 0, s, p,
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p,
                          lights4);
     else if (f16_out)
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true, false, false, true>), grid, dim3(256), 0, s, p,
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p,
                          lights4);
     else
-      hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false, false, false, true>), grid, dim3(256), 0, s, p,
+      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p,
                          lights4);
-    st = check_launch("resolve_kernel<mlp>");
+    althea_status st = check_launch("resolve_mlp_kernel");
     if (st != ALTHEA_OK) return st;
     int dev = 0, cus = 256;
     ALTHEA_TRY_HIP(hipGetDevice(&dev));
     ALTHEA_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const dim3 fgrid(kMlpFinishBlocksPerCu * cus);
     if (f16_out)
-      hipLaunchKernelGGL(resolve_mlp_finish_kernel<true>, fgrid, dim3(256), 0, s, p);
+      hipLaunchKernelGGL(resolve_mlp_finish_kernel<true>, fgrid, dim3(256), 0, s, p, regions);
     else
-      hipLaunchKernelGGL(resolve_mlp_finish_kernel<false>, fgrid, dim3(256), 0, s, p);
+      hipLaunchKernelGGL(resolve_mlp_finish_kernel<false>, fgrid, dim3(256), 0, s, p, regions);
     return check_launch("resolve_mlp_finish_kernel");
   }
   const dim3 grid(tiles);

@@ -84,7 +84,9 @@ __device__ __forceinline__ f3 pbr_environment(const PbrPoint& m, f3 irr, f3 refl
 // E.z) accumulate four per-pixel scalars instead of three channels:
 //   sum (F_c s + (1 - F_c) kdp_c) e w
 //     = F0_c (A2 - kdp_c A1) + dF_c (A4 - kdp_c A3) + kdp_c A1
-//   with ew = e w, S = s ew, A1 = sum ew, A2 = sum S, A3 = sum x5 ew, A4 = sum x5 S.
+//   with ew = e w, S = s ew, A1 = sum ew, A2 = sum S, A3 = sum x5 ew, A4 = sum x5 S;
+//   the per-pixel factor c1 = a2 gv / pi of s is applied once to A2 and A4.
+// V.L enters only as 2 V.L, formed as (2V).L (exact scaling).
 // A light below the horizon of every lane of the wave adds exactly zero and
 // is skipped with a wave-uniform branch; lanes with N = 0 pass no horizon.
 // Lights are wave-uniform: scalar loads into SGPRs, and the grey test is a
@@ -138,6 +140,7 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
   const float om_k = 1.0f - m.kdir;
   const float a2m1 = m.a2 - 1.0f;
   const f3 kdp = m.kd * kInvPi;
+  const f3 I2 = I * 2.0f;
   float A1 = 0.0f, A2 = 0.0f, A3 = 0.0f, A4 = 0.0f;
   if (any_lane) {
     for (int i = 0; i < light_count; ++i) {
@@ -145,34 +148,35 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
       const float4 em = lights[2 * i + 1];
       const f3 L = mk(lp.x - P.x, lp.y - P.y, lp.z - P.z);
       const float nl_raw = dot(N, L);
-      if (__builtin_amdgcn_ballot_w64(nl_raw > 0.0f) == 0) continue;
+      const uint64_t lit = __builtin_amdgcn_ballot_w64(nl_raw > 0.0f);
+      if (lit == 0) continue;
       const float d2 = dot(L, L);
       const float inv_d = rsq_fast(d2);
-      const float vl = dot(I, L);
-      const float h2 = fmaf(2.0f * vl, inv_d, 2.0f);
+      const float h2 = fmaf(dot(I2, L), inv_d, 2.0f);
       const float n_dot_l = fmaxf(nl_raw * inv_d, 0.0f);
       float n_dot_h = fmaxf(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2), 0.0f);
-      const bool refine = h2 < kHalfVecRefine && nl_raw > 0.0f;
-      if (__builtin_amdgcn_ballot_w64(refine) != 0) {
-        if (refine) n_dot_h = n_dot_h_refined(N, I, P, lp, L, inv_d, h2, view_df);
+      // Lanes above the horizon with |V + L|^2 below the refinement bound.
+      if ((__builtin_amdgcn_ballot_w64(h2 < kHalfVecRefine) & lit) != 0) {
+        if (h2 < kHalfVecRefine && nl_raw > 0.0f) n_dot_h = n_dot_h_refined(N, I, P, lp, L, inv_d, h2, view_df);
       }
       const float x = 1.0f - n_dot_h;
       const float x5 = pow5(x);
       const float t = fmaf(n_dot_h * n_dot_h, a2m1, 1.0f);
       const float den = (t * t) * fmaf(n_dot_l, om_k, m.kdir) * fmaf(n_dot_l, four_nv, 0.0001f);
-      const float s = c1 * n_dot_l * rcp_fast(den);
+      const float s1 = n_dot_l * rcp_fast(den);  // s / c1
       float w = (inv_d * inv_d) * n_dot_l;
       if (Shadow::kActive && n_dot_l > 0.0f && shadow.hidden(i, L, d2 * inv_d)) w = 0.0f;
       const bool grey =
           __float_as_uint(em.x) == __float_as_uint(em.y) && __float_as_uint(em.y) == __float_as_uint(em.z);
       if (grey) {
         const float ew = em.x * w;
-        const float S = s * ew;
+        const float S = s1 * ew;
         A1 += ew;
         A2 += S;
         A3 = fmaf(x5, ew, A3);
         A4 = fmaf(x5, S, A4);
       } else {
+        const float s = c1 * s1;
         const f3 F = mk(fmaf(m.dF.x, x5, m.F0.x), fmaf(m.dF.y, x5, m.F0.y), fmaf(m.dF.z, x5, m.F0.z));
         color.x = fmaf(fmaf(F.x, s, fmaf(-F.x, kdp.x, kdp.x)) * em.x, w, color.x);
         color.y = fmaf(fmaf(F.y, s, fmaf(-F.y, kdp.y, kdp.y)) * em.y, w, color.y);
@@ -180,6 +184,8 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
       }
     }
   }
+  A2 *= c1;
+  A4 *= c1;
   color.x += fmaf(m.F0.x, fmaf(-kdp.x, A1, A2), fmaf(m.dF.x, fmaf(-kdp.x, A3, A4), kdp.x * A1));
   color.y += fmaf(m.F0.y, fmaf(-kdp.y, A1, A2), fmaf(m.dF.y, fmaf(-kdp.y, A3, A4), kdp.y * A1));
   color.z += fmaf(m.F0.z, fmaf(-kdp.z, A1, A2), fmaf(m.dF.z, fmaf(-kdp.z, A3, A4), kdp.z * A1));

@@ -28,11 +28,17 @@ SEED0 = 0x414C0000
 WIDTH, HEIGHT, LIGHTS = 3840, 2160, 64
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes one resolve of this workload moves (FP32 G-buffer), from
-# rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters on a
-# 1 GiB copy (bench.py --only copy); commands and figures in
-# profiles/resolve_kernel_4k_fp32_64lights_ssao16.md.
-RESOLVE_TRAFFIC_BYTES = {"fp32": 3.362e9}
+# Per-launch counters of the FP32-G-buffer resolve of this workload, from the
+# rocprofv3 passes in profiles/resolve_kernel_4k_fp32_64lights_ssao16.md (the
+# commands are there): HBM bytes from FETCH_SIZE and WRITE_SIZE scaled by the
+# same counters on a 1 GiB copy (bench.py --only copy), and the VALU wave
+# instructions (SQ_INSTS_VALU).  They describe the kernel source they were
+# measured on and are refreshed with it.
+RESOLVE_TRAFFIC_BYTES = {"fp32": 3.342e9}
+RESOLVE_VALU_WAVE_INSTS = {"fp32": 468068682}
+# Nominal VALU issue: one wave64 instruction per SIMD per 4 cycles, 1024 SIMDs
+# at 2.4 GHz.
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 
 
 def parse():
@@ -43,7 +49,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
     p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
     p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
-    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr", "shadow", "gbuffer"],
+    p.add_argument("--only", choices=["resolve", "prefilter", "copy", "ssr", "shadow", "gbuffer", "irradiance", "mlp"],
                    help="profiling runs: time one kernel family and print its line only "
                         "(copy = a plain 1 GiB device copy, the calibration for HBM byte counters)")
     return p.parse_args()
@@ -118,6 +124,94 @@ def batched(torch, aa) -> dict:
             "tuning": tuning, "autotune_s": tune_s}
 
 
+def mlp_errors_vs_oracle(torch, aa, ibl, scene, env, net, gbuf, lights, n_pix: int = 2000) -> dict:
+    """Shading error of the cube-IBL frame and of the MLP-IBL frame with every
+    cell accepted, against the FP64 oracle at sampled covered pixels.  The
+    oracle shades each pixel with the exact prefiltered radiance at its own
+    reflection query (R, roughness): the FP64 prefilter with 1024 Hammersley
+    samples (the plain method's estimator) and with 16384 (converged)."""
+    from tests import oracle_lib as ol
+
+    rng = np.random.default_rng(SEED0 + 9)
+    cov = np.nonzero(scene.position[..., 3].ravel() != 0)[0]
+    lin = rng.choice(cov, n_pix, replace=False)
+    pix = np.stack([lin % WIDTH, lin // WIDTH], 1).astype(np.int32)
+    views, oenv = ibl.views_numpy(), ol.Env(env)
+    _, _, margin, q = ol.resolve(scene, views, oenv, pix, query=True)
+    dr = q.astype(np.float32)
+    cube = aa.DeferredRendering.resolve(gbuf, ibl, lights, scene.globals, scene.ssao).cpu().numpy()[lin, :3]
+    used = torch.zeros(WIDTH * HEIGHT, dtype=torch.uint8, device="cuda")
+    mlp = aa.DeferredRendering.resolve_mlp(gbuf, ibl, net, lights, scene.globals, scene.ssao,
+                                           mlp_used=used).cpu().numpy()[lin, :3]
+    served = used.cpu().numpy()[lin] != 0
+    clear = margin > 2e-3
+    out = {"pixels": int(n_pix), "clear_of_ssao_thresholds": int(clear.sum()), "mlp_served": int(served.sum())}
+    for spp in (1024, 16384):
+        refl = oenv.prefilter_points(spp, dr)
+        ref, _, _ = ol.resolve(scene, views, oenv, pix, refl=refl)
+        for name, img in (("cube_ibl", cube), ("mlp_ibl", mlp)):
+            e = (np.abs(img - ref) / np.maximum(np.abs(ref), 1e-2)).max(1)[clear]
+            out[f"{name}_vs_fp64_prefilter_{spp}spp"] = {"mean": float(e.mean()), "max": float(e.max()),
+                                                          "p99": float(np.percentile(e, 99))}
+    return out
+
+
+CONFIG4_EPOCHS = 32
+
+
+def config4_recipe(torch, aa, maps_per_width: int = 1) -> dict:
+    """The learned surrogate as configured (config 4): one MLP per environment
+    of the 64-map batch (here the first map of each source width, 256 to
+    4096), trained with Adam (step size 1e-3, batch 65536) on 2^22 queries,
+    directions uniform on the sphere and roughness uniform in [0, 1],
+    labelled by the plain 1024-sample prefilter of that map; 32 epochs (2048
+    steps).  Per map: the training time, the loss, and the mean relative
+    error over 2^14 held-out queries against 1024-sample labels and against
+    converged 16384-sample labels, with the cube path (the map's own 5-mip
+    cube, faces W/4) judged against the same labels."""
+    from althea_amd import mlp as M
+    from althea_amd.batch import batch_widths
+
+    widths = batch_widths(64)
+    picks = [widths.index(w) for w in sorted(set(widths))]
+    res = {"recipe": {"queries": 1 << 22, "batch": 65536, "lr": 1e-3, "roughness": [0.0, 1.0], "label_spp": 1024,
+                      "epochs": CONFIG4_EPOCHS, "steps": CONFIG4_EPOCHS * ((1 << 22) // 65536)}, "maps": {}}
+    for i in picks:
+        w = widths[i]
+        env = torch.from_numpy(aa.synth.env(seed=i, width=w, height=w // 2)).cuda()
+        pyr = aa.env_pyramid(env)
+        t0 = time.perf_counter()
+        dr, labels = M.training_set(pyr, w, w // 2, 1 << 22, seed=SEED0 + 40 + i, samples=1024)
+        net = M.PrefilterMLP(seed=SEED0 + 50 + i, lr=1e-3,
+                             init_output_bias=float(np.log(labels[:, :3].mean().item())))
+        torch.cuda.synchronize()
+        label_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        hist = M.fit(net, dr, labels, batch=65536, epochs=CONFIG4_EPOCHS)
+        torch.cuda.synchronize()
+        train_s = time.perf_counter() - t0
+        q, t1024 = M.training_set(pyr, w, w // 2, 1 << 14, seed=SEED0 + 60 + i, samples=1024)
+        t16k = aa.prefilter_points(pyr, w, w // 2, 16384, q)
+        pred = net.infer(q)
+        cube = aa.prefilter_cube(pyr, w, w // 2, w // 4, 5, 1024)
+        qn = q.cpu().numpy()
+        dir_lod = torch.from_numpy(np.concatenate([qn[:, :3], qn[:, 3:4] * 4.0], 1).astype(np.float32)).cuda()
+        cube_pred = aa.cube_sample(cube, w // 4, 5, dir_lod)
+
+        def err(a, b):
+            return float(((a[:, :3] - b[:, :3]).abs() / b[:, :3].abs().clamp_min(1e-2)).amax(1).mean().item())
+
+        res["maps"][f"map{i}_{w}x{w // 2}"] = {
+            "label_s": label_s, "train_s": train_s, "ms_per_step": 1e3 * train_s / res["recipe"]["steps"],
+            "loss_first": hist[0], "loss_last": hist[-1],
+            "mlp_mean_rel_err_vs_1024spp": err(pred, t1024), "mlp_mean_rel_err_vs_16384spp": err(pred, t16k),
+            "cube_mean_rel_err_vs_1024spp": err(cube_pred, t1024), "cube_mean_rel_err_vs_16384spp": err(cube_pred, t16k),
+            "labels_1024_vs_16384spp": err(t1024, t16k),
+            "ibl_state_bytes": {"mlp": net.state_bytes(), "cube": 16 * aa.cube_texels(w // 4, 5)}}
+        del env, pyr, dr, labels, net, cube
+    return res
+
+
 def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
     """The learned stand-in on the same frame.  2^22 (direction, roughness)
     queries, roughness uniform in [1/(mips-1), 1] (the range whose cube
@@ -125,9 +219,12 @@ def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
     per-texel noise), labelled by the 1024-sample prefilter of the frame's
     environment; Adam, batch 16384, step size 3e-3 decaying to 1e-5 over 256
     epochs (65536 steps); then the error monitor against the 512^2 x6 cube
-    and the fused MLP-IBL resolve at monitor thresholds 1e-3 (the target),
-    3e-3, 5e-3 and 1e-2.  Each MLP-IBL frame is compared with the cube-IBL frame
-    over every pixel (relative difference, floor 1e-2)."""
+    and the MLP-IBL resolve (the resolve queues the pixels the MLP serves, a
+    dense MFMA kernel completes them) at monitor thresholds 1e-3 (the target),
+    3e-3, 5e-3, 1e-2 and with every cell accepted (the full-frame MLP
+    inference).  Each MLP-IBL frame is compared with the cube-IBL frame over
+    every pixel (relative difference, floor 1e-2), and the cube-IBL and
+    full-frame MLP-IBL shading against the FP64 oracle."""
     from althea_amd import mlp as M
 
     t0 = time.perf_counter()
@@ -152,7 +249,7 @@ def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
     plain_ms = time_events(torch, lambda: aa.DeferredRendering.resolve(gbuf, ibl, lights, scene.globals, scene.ssao,
                                                                         out=out), steps)
     per_threshold = {}
-    for thr in (1e-3, 3e-3, 5e-3, 1e-2):
+    for thr in (1e-3, 3e-3, 5e-3, 1e-2, float("inf")):
         t0 = time.perf_counter()
         cell_err, accepted = net.build_monitor(cube, v.prefiltered_size, v.prefiltered_mips, threshold=thr)
         torch.cuda.synchronize()
@@ -165,14 +262,17 @@ def mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, steps: int) -> dict:
         step()
         ms = time_events(torch, step, steps)
         d = ((out[:, :3] - plain[:, :3]).abs() / plain[:, :3].abs().clamp_min(1e-2)).double()
-        per_threshold[f"{thr:g}"] = {
+        per_threshold["every_cell" if thr == float("inf") else f"{thr:g}"] = {
             "accepted_cells": accepted, "cells": cell_err.numel(), "monitor_s": monitor_s,
             "mlp_pixel_fraction": float(used.float().mean().item()),
             "ms": ms, "mpix_per_s": WIDTH * HEIGHT / (ms * 1e-3) / 1e6,
             "rel_diff_vs_cube_ibl_frame": {"mean": float(d.mean().item()), "max": float(d.max().item())}}
     e = cell_err.cpu().numpy()
+    # The monitor now accepts every cell: the full-frame MLP inference.
+    errors = mlp_errors_vs_oracle(torch, aa, ibl, scene, env, net, gbuf, lights)
     return {"cube_ibl": {"ms": plain_ms, "mpix_per_s": WIDTH * HEIGHT / (plain_ms * 1e-3) / 1e6},
             "mlp_ibl_by_monitor_threshold": per_threshold,
+            "shading_error_vs_fp64_oracle": errors,
             "cell_error_quantiles_by_mip": {m: {q: float(np.quantile(e[m], float(q))) for q in ("0.1", "0.5", "0.9")}
                                             for m in range(e.shape[0])},
             "ibl_state_bytes": {"cube_ibl": ibl.state_bytes(), "mlp_weights": net.state_bytes(),
@@ -473,6 +573,17 @@ def cpu_baseline(aa, scene, ibl, env) -> dict:
                        "threads": ol.max_threads(), "what": "FP64 oracle, same frame, once"}}
 
 
+def valu_issue(ms: float, fp16: bool):
+    """VALU wave instructions per launch (profiled) over this run's launch
+    time, against the nominal issue rate."""
+    n = RESOLVE_VALU_WAVE_INSTS.get("fp16" if fp16 else "fp32")
+    if n is None:
+        return None
+    rate = n / (ms * 1e-3)
+    return {"wave_insts_per_launch": n, "issue_per_s": rate, "nominal_issue_per_s": VALU_ISSUE_PEAK,
+            "frac_of_nominal": rate / VALU_ISSUE_PEAK}
+
+
 def timed_steps(torch, fn, args) -> float:
     for _ in range(args.warmup):
         fn()
@@ -505,6 +616,49 @@ def prefilter_only(torch, aa, args) -> int:
     return 0
 
 
+def irradiance_only(torch, aa, args) -> int:
+    """The irradiance convolution of the 512x256 source into a 32^2 x6 cube,
+    45000 samples per texel (300 theta x 150 phi rows)."""
+    env = aa.synth.env(seed=SEED0, width=512, height=256)
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    ms = timed_steps(torch, lambda: aa.irradiance_cube(pyr, 512, 256, 32), args)
+    print(json.dumps({"kind": "irradiance", "workload": "512x256 -> 32^2x6, 45000 samples per texel", "ms": ms,
+                      "samples_per_s": 6 * 32 * 32 * 45000 / (ms * 1e-3)}), flush=True)
+    return 0
+
+
+def mlp_only(torch, aa, args) -> int:
+    """The MLP kernels at their configured sizes: training steps at batch
+    65536 on 2^22 labelled queries of the headline 2048x1024 map, then the
+    4K MLP-IBL resolve with every monitor cell accepted (the full-frame
+    inference: every covered pixel with roughness above mip 0's band goes
+    through the MFMA finishing kernel)."""
+    from althea_amd import mlp as M
+
+    env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
+    desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
+                      brdf_lut_size=128, brdf_lut_samples=1024)
+    ibl = aa.ImageBasedLighting.createResources(torch.from_numpy(env).cuda(), desc)
+    scene = aa.synth.scene(seed=SEED0 + 1, width=WIDTH, height=HEIGHT, lights=LIGHTS)
+    gbuf = aa.GBufferResources(*scene.gbuffer_tensors("cuda"))
+    lights = scene.lights_tensor("cuda")
+    dr, labels = M.training_set(ibl.environment_tensor(), 2048, 1024, 1 << 22, seed=SEED0 + 4, samples=1024)
+    net = M.PrefilterMLP(seed=SEED0 + 5, lr=1e-3, init_output_bias=float(np.log(labels[:, :3].mean().item())))
+    batch = 65536
+    x, y = dr[:batch].contiguous(), labels[:batch].contiguous()
+    train_ms = timed_steps(torch, lambda: net.train_step(x, y), args)
+    v = ibl.views()
+    net.build_monitor(ibl.prefiltered_tensor(), v.prefiltered_size, v.prefiltered_mips, threshold=float("inf"))
+    out = torch.empty((WIDTH * HEIGHT, 4), dtype=torch.float32, device="cuda")
+    used = torch.zeros(WIDTH * HEIGHT, dtype=torch.uint8, device="cuda")
+    ms = timed_steps(torch, lambda: aa.DeferredRendering.resolve_mlp(gbuf, ibl, net, lights, scene.globals, scene.ssao,
+                                                                      out=out, mlp_used=used), args)
+    print(json.dumps({"kind": "mlp", "train_batch": batch, "train_ms_per_step": train_ms,
+                      "resolve_every_cell_ms": ms, "mlp_pixels": int(used.sum().item()),
+                      "weights_bytes": net.state_bytes()}), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -528,6 +682,10 @@ def main() -> int:
         return shadow_only(torch, aa, args)
     if args.only == "gbuffer":
         return gbuffer_only(torch, aa, args)
+    if args.only == "irradiance":
+        return irradiance_only(torch, aa, args)
+    if args.only == "mlp":
+        return mlp_only(torch, aa, args)
     env = aa.synth.env(seed=SEED0 + 1, width=2048, height=1024)
     desc = aa.IBLDesc(irradiance_size=32, prefiltered_size=512, prefiltered_mips=6, prefilter_samples=1024,
                       brdf_lut_size=128, brdf_lut_samples=1024)
@@ -568,7 +726,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": RESOLVE_TRAFFIC_BYTES.get("fp16" if args.fp16_gbuffer else "fp32"),
-                     "algorithmic_bytes": nbytes, "kernel": "resolve_kernel"},
+                     "algorithmic_bytes": nbytes, "kernel": "resolve_kernel",
+                     # What binds the kernel: VALU issue (64 lights per pixel), not HBM.
+                     "limiter": "valu", "valu": valu_issue(ms, args.fp16_gbuffer),
+                     "counters_from": "profiles/resolve_kernel_4k_fp32_64lights_ssao16.md"},
     }
     if not args.no_secondary and args.only is None:
         line["secondary"] = secondary(torch, aa)
@@ -579,6 +740,7 @@ def main() -> int:
         line["secondary"]["gbuffer"]["gltf"] = gltf_frame(torch, aa, scene, ibl, args.steps)
         line["secondary"]["full_frame"] = full_frame(torch, aa, scene, ibl, args.steps)
         line["secondary"]["mlp_ibl_resolve"] = mlp_ibl(torch, aa, ibl, scene, gbuf, lights, env, args.steps)
+        line["secondary"]["mlp_config4_recipe"] = config4_recipe(torch, aa)
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(aa, scene, ibl, env)
     print(json.dumps(line), flush=True)

@@ -105,7 +105,9 @@ ALTHEA_API althea_status althea_cube_sample(const float* d_cube, int size, int m
 
 /* Precompute lookups into an environment pyramid (REPEAT sampler, trilinear,
  * the pole branch of PreFilterEnvMap.comp:98-122): d_dir_lod float4 per
- * query, d_out float4. */
+ * query, d_out float4.  The mirror-lookup path: angles and texel coordinates
+ * in double-float (as the roughness-0 prefilter mip addresses its lookups),
+ * each level's bilinear weights rounded once to FP32. */
 ALTHEA_API althea_status althea_env_sample(const float* d_pyramid, int width, int height,
                                            const float* d_dir_lod, int count, float* d_out,
                                            hipStream_t stream);
@@ -574,7 +576,13 @@ typedef struct althea_mesh_view {
  * holds what the seamless filter reads beyond the face's edge (the adjacent
  * face's edge texel; at a corner the mean of the three texels meeting there),
  * filled by this call.  d_mesh positions / indices and d_lights are device
- * memory; d_cubes holds light_count * althea_shadow_cube_floats(size) floats. */
+ * memory; d_cubes holds light_count * althea_shadow_cube_floats(size) floats.
+ * Deviation: every triangle given is opaque.  The reference's shadow shaders
+ * discard fragments whose base-colour alpha is below the material's cut-off
+ * (ShadowMapBindless.frag, ShadowMap.frag); the mesh here carries no
+ * material, so alpha-masked geometry casts solid shadows while the glTF
+ * G-buffer subpass (althea_gbuffer_draw_gltf) cuts it out.  Pass only opaque
+ * triangles where that matters. */
 ALTHEA_API size_t althea_shadow_cube_floats(int size);
 ALTHEA_API althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights,
                                              int light_count, int size, float* d_cubes, hipStream_t stream);

@@ -14,7 +14,8 @@
 //   bf16   - the rounding points of the BF16 MFMA kernels: weights, the 28
 //            encoded inputs, every hidden activation and every back-propagated
 //            pre-activation gradient are rounded to bfloat16 (nearest even)
-//            before they enter a product; sums stay FP64.
+//            before they enter a product (the bias gradients are the sums of
+//            the rounded pre-activation gradients); sums stay FP64.
 #pragma once
 
 #include <cmath>
@@ -131,7 +132,7 @@ inline double loss_grad(const P* params, const float* dir_rough, const float* ta
     }
     for (int o = 0; o < kOut; ++o) {
       double g = q(dy[o], bf16);
-      grad[kB4 + o] += dy[o];
+      grad[kB4 + o] += g;
       for (int i = 0; i < kHidden; ++i) grad[kW4 + o * kHidden + i] += g * a.h3[i];
     }
     for (int i = 0; i < kHidden; ++i) {
@@ -141,7 +142,7 @@ inline double loss_grad(const P* params, const float* dir_rough, const float* ta
     }
     for (int o = 0; o < kHidden; ++o) {
       double g = q(dz3[o], bf16);
-      grad[kB3 + o] += dz3[o];
+      grad[kB3 + o] += g;
       for (int i = 0; i < kHidden; ++i) grad[kW3 + o * kHidden + i] += g * a.h2[i];
     }
     for (int i = 0; i < kHidden; ++i) {
@@ -151,7 +152,7 @@ inline double loss_grad(const P* params, const float* dir_rough, const float* ta
     }
     for (int o = 0; o < kHidden; ++o) {
       double g = q(dz2[o], bf16);
-      grad[kB2 + o] += dz2[o];
+      grad[kB2 + o] += g;
       for (int i = 0; i < kHidden; ++i) grad[kW2 + o * kHidden + i] += g * a.h1[i];
     }
     for (int i = 0; i < kHidden; ++i) {
@@ -161,7 +162,7 @@ inline double loss_grad(const P* params, const float* dir_rough, const float* ta
     }
     for (int o = 0; o < kHidden; ++o) {
       double g = q(dz1[o], bf16);
-      grad[kB1 + o] += dz1[o];
+      grad[kB1 + o] += g;
       for (int i = 0; i < kIn; ++i) grad[kW1 + o * kIn + i] += g * a.x[i];
     }
   }

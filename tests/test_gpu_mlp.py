@@ -117,3 +117,25 @@ def test_training_reduces_loss(torch):
         if it % 50 == 0 or it == 399:
             losses.append(loss.item())
     assert losses[-1] < losses[0] / 10, losses
+
+
+def test_handles_alive_at_interpreter_exit_are_released_quietly(torch):
+    """Handles still referenced by module globals when the interpreter exits
+    are destroyed through the function bound at creation, with no exception
+    reported from __del__."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import torch, althea_amd as aa\n"
+            "from althea_amd import mlp as M\n"
+            "ibl = aa.IBLResources(aa.IBLDesc(prefiltered_size=16, prefiltered_mips=3, irradiance_size=4,"
+            " brdf_lut_size=8))\n"
+            "net = M.PrefilterMLP(seed=3)\n"
+            "torch.cuda.synchronize()\n"
+            "print('handles made', flush=True)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "handles made" in r.stdout
+    assert "Exception ignored" not in r.stderr and "TypeError" not in r.stderr, r.stderr[-2000:]

@@ -221,3 +221,41 @@ def test_mlp_error_is_the_fit_not_bf16(torch):
     print(f"MLP_ERROR_SOURCES bf16-vs-labels {e_gpu:.3e} fp64-vs-labels {e_f64:.3e} bf16-vs-fp64 {e_bf16:.3e}")
     assert abs(e_gpu - e_f64) < 0.1 * e_f64, (e_gpu, e_f64)
     assert e_bf16 < 0.5 * e_f64, (e_bf16, e_f64)
+
+
+def test_config4_recipe_one_mlp_per_environment(torch):
+    """Config 4 as stated, on two environments of the config-3 batch (the
+    first 256- and 1024-wide maps of batch_widths(64), seeded by their index):
+    per environment one MLP, 2^22 queries (directions uniform on the sphere,
+    roughness uniform in [0, 1]) labelled by the plain 1024-sample prefilter,
+    Adam with step size 1e-3 and batch 65536, 16 epochs (1024 steps).  Each
+    network's loss falls tenfold, the two networks differ, and on held-out
+    rough queries (roughness 0.25 .. 1, the lobes a 10 k-parameter network can
+    follow) each stays within 5 % mean relative error of its own map's
+    labels and is further from the other map's."""
+    from althea_amd.batch import batch_widths
+
+    widths = batch_widths(64)
+    picks = [widths.index(256), widths.index(1024)]
+    nets, pyrs = [], []
+    for i in picks:
+        w = widths[i]
+        env = aa.synth.env(seed=i, width=w, height=w // 2)
+        pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+        dr, labels = M.training_set(pyr, w, w // 2, 1 << 22, seed=SEED0 + 40 + i, samples=1024)
+        net = M.PrefilterMLP(seed=SEED0 + 50 + i, lr=1e-3, init_output_bias=float(np.log(labels[:, :3].mean().item())))
+        hist = M.fit(net, dr, labels, batch=65536, epochs=16)
+        print(f"config-4 map {i} ({w} wide): loss {hist[0]:.3e} -> {hist[-1]:.3e}", flush=True)
+        assert hist[-1] < hist[0] / 10, hist
+        nets.append(net)
+        pyrs.append((pyr, w))
+        del dr, labels
+    errs = np.zeros((2, 2))
+    for a, (pyr, w) in enumerate(pyrs):
+        q, t = M.training_set(pyr, w, w // 2, 1 << 13, seed=SEED0 + 70 + a, samples=1024, rmin=0.25)
+        for b, net in enumerate(nets):
+            pred = net.infer(q)
+            errs[a, b] = float(((pred[:, :3] - t[:, :3]).abs() / t[:, :3].abs().clamp_min(1e-2)).amax(1).mean())
+    print(f"config-4 held-out mean rel error [map, net]: {errs.tolist()}", flush=True)
+    assert errs[0, 0] < 0.05 and errs[1, 1] < 0.05, errs
+    assert errs[0, 1] > errs[0, 0] and errs[1, 0] > errs[1, 1], errs
