@@ -59,6 +59,7 @@ struct ResolveParams {
   float ssao_radius, ssao_bias;
   float kernel[ALTHEA_SSAO_MAX_SAMPLES][4];
   float noise[16][4];
+  const float* ssao_depth;  // per pixel view_depth(position), +inf where uncovered
   void* out;
   float* ao_out;
   MlpInferenceView mlp;  // MLP variant only
@@ -135,10 +136,14 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
   py0 *= hh, ty *= hh, by *= hh, ny *= hh;
   auto vz = [&](f3 v) { return -fmaf(p.view_z[0], v.x, fmaf(p.view_z[1], v.y, p.view_z[2] * v.z)); };
   const float dT = vz(Tr), dB = vz(Br), dN = vz(Nr);
-  // Taps in groups of kSsaoBatch: their position loads are issued together
+  // Taps in groups of kSsaoBatch: their depth loads are issued together
   // (an invalid tap loads texel 0 and is ignored), then evaluated in tap
   // order, so the per-tap arithmetic and the sum order are those of a
-  // one-tap-at-a-time loop.
+  // one-tap-at-a-time loop.  The depths come from the frame's view-depth
+  // plane (ssao_depth_kernel): 4 bytes per tap instead of the 16-byte
+  // position, the same value view_depth gives the position, and +inf where
+  // the G-buffer is uncovered, which fails the occlusion compare exactly as
+  // the uncovered-tap skip does.
   float occ = 0.0f;
   const int n = p.ssao_samples;
   for (int k0 = 0; k0 < n; k0 += kSsaoBatch) {
@@ -160,19 +165,16 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
       const int qy = min(static_cast<int>(sy), p.height - 1);
       idx[j] = ok[j] ? static_cast<uint32_t>(qy) * static_cast<uint32_t>(p.width) + static_cast<uint32_t>(qx) : 0u;
     }
-    float4 q[kSsaoBatch];
+    float qd[kSsaoBatch];
 #pragma unroll
-    for (int j = 0; j < kSsaoBatch; ++j) q[j] = load_gbuf<kFmt>(p.position, idx[j]);
-    // The first gather is consumed here, so it is issued with the others:
-    // otherwise the compiler moves it under that tap's validity test and
-    // splits it into two dependent loads (w, then xyz).  (Pinning all four
-    // this way costs 4 VGPRs, one wave per SIMD.)
-    asm volatile("" : "+v"(q[0].x), "+v"(q[0].y), "+v"(q[0].z), "+v"(q[0].w));
+    for (int j = 0; j < kSsaoBatch; ++j) qd[j] = p.ssao_depth[idx[j]];
+    // The first load is consumed here, so it is issued with the others
+    // rather than moved under that tap's validity test.
+    asm volatile("" : "+v"(qd[0]));
 #pragma unroll
     for (int j = 0; j < kSsaoBatch; ++j) {
-      if (!ok[j] || q[j].w == 0.0f) continue;
-      const float qd = view_depth(p, mk(q[j].x, q[j].y, q[j].z));
-      if (qd <= sd[j] - p.ssao_bias) occ += smoothstep01(p.ssao_radius * rcp_fast(fabsf(frag_depth - qd)));
+      if (ok[j] && qd[j] <= sd[j] - p.ssao_bias)
+        occ += smoothstep01(p.ssao_radius * rcp_fast(fabsf(frag_depth - qd[j])));
     }
   }
   return fmaf(-occ, p.ssao_inv_samples, 1.0f);
@@ -326,6 +328,18 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
   }
 }
 
+// The frame's SSAO view-depth plane: view_depth(position) per pixel, +inf
+// where the G-buffer is uncovered (position.w == 0); the expression and the
+// FP16 conversion are the ones the resolve applies to a tap's position.
+template <int kFmt>
+__global__ __launch_bounds__(256) void ssao_depth_kernel(const ResolveParams p, float* __restrict__ plane) {
+  const size_t n = static_cast<size_t>(p.width) * p.height;
+  const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = load_gbuf<kFmt>(p.position, i);
+  plane[i] = q.w == 0.0f ? __builtin_inff() : view_depth(p, mk(q.x, q.y, q.z));
+}
+
 // The resolve: one block per tile, the SSAO noise tile in LDS; kRefl blends
 // the reflection buffer, kShadow applies the point-light shadow test, kMlp
 // defers the pixels the MLP serves.
@@ -442,6 +456,14 @@ void view_projection(const althea_globals& g, float vp[16]) {
     }
 }
 
+static althea_status launch_resolve_with(ResolveParams& p, const althea_gbuffer_view* g,
+                                        const althea_point_light* lights, unsigned flags,
+                                        const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s,
+                                        const althea_reflection_view* reflection, const float* shadow_cubes,
+                                        int shadow_size);
+
+// The SSAO view-depth plane lives in a stream-ordered allocation for the
+// duration of the call (W x H floats).
 althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                              const althea_point_light* lights, int light_count, const althea_globals* globals,
                              const althea_ssao_desc* ssao, unsigned flags, void* out, float* ao_out,
@@ -486,6 +508,34 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   }
   p.out = out;
   p.ao_out = ao_out;
+  if (!p.ssao_on)
+    return launch_resolve_with(p, g, lights, flags, mlp, mlp_used, s, reflection, shadow_cubes, shadow_size);
+  const size_t pixels = static_cast<size_t>(g->width) * g->height;
+  float* plane = nullptr;
+  const hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&plane), pixels * sizeof(float), s);
+  if (e != hipSuccess) {
+    set_error("resolve: hipMallocAsync of the %zu-byte SSAO depth plane failed: %s", pixels * sizeof(float),
+              hipGetErrorString(e));
+    return ALTHEA_ERROR_HIP;
+  }
+  p.ssao_depth = plane;
+  const dim3 dgrid(static_cast<unsigned>((pixels + 255) / 256));
+  if (g->format == ALTHEA_GBUFFER_FP32)
+    hipLaunchKernelGGL(ssao_depth_kernel<ALTHEA_GBUFFER_FP32>, dgrid, dim3(256), 0, s, p, plane);
+  else
+    hipLaunchKernelGGL(ssao_depth_kernel<ALTHEA_GBUFFER_FP16>, dgrid, dim3(256), 0, s, p, plane);
+  althea_status st = check_launch("ssao_depth_kernel");
+  if (st == ALTHEA_OK)
+    st = launch_resolve_with(p, g, lights, flags, mlp, mlp_used, s, reflection, shadow_cubes, shadow_size);
+  (void)hipFreeAsync(plane, s);
+  return st;
+}
+
+static althea_status launch_resolve_with(ResolveParams& p, const althea_gbuffer_view* g,
+                                        const althea_point_light* lights, unsigned flags,
+                                        const MlpInferenceView* mlp, uint8_t* mlp_used, hipStream_t s,
+                                        const althea_reflection_view* reflection, const float* shadow_cubes,
+                                        int shadow_size) {
   const int tiles = ((g->width + 63) / 64) * ((g->height + 3) / 4);
   const float4* lights4 = reinterpret_cast<const float4*>(lights);
   const bool f16_out = (flags & ALTHEA_RESOLVE_OUT_FP16) != 0;

@@ -34,8 +34,11 @@ FP32_PEAK_TFLOPS = 157.3
 # same counters on a 1 GiB copy (bench.py --only copy), and the VALU wave
 # instructions (SQ_INSTS_VALU).  They describe the kernel source they were
 # measured on and are refreshed with it.
-RESOLVE_TRAFFIC_BYTES = {"fp32": 3.342e9}
-RESOLVE_VALU_WAVE_INSTS = {"fp32": 468068682}
+RESOLVE_TRAFFIC_BYTES = {"fp32": 1.427e9}
+RESOLVE_VALU_WAVE_INSTS = {"fp32": 460595456}
+# The SSAO view-depth pre-pass of the same call (ssao_depth_kernel): the
+# position plane read once, the 4-byte plane written once.
+DEPTH_PASS_TRAFFIC_BYTES = {"fp32": 1.659e8}
 # Nominal VALU issue: one wave64 instruction per SIMD per 4 cycles, 1024 SIMDs
 # at 2.4 GHz.
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
@@ -44,8 +47,8 @@ VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=100, help="untimed steps first: the GPU reaches its steady clock after about 50 frames")
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing (profiling runs)")
     p.add_argument("--no-secondary", action="store_true", help="skip the precompute timings")
     p.add_argument("--fp16-gbuffer", action="store_true", help="shade from an FP16 G-buffer")
@@ -575,13 +578,15 @@ def cpu_baseline(aa, scene, ibl, env) -> dict:
 
 def valu_issue(ms: float, fp16: bool):
     """VALU wave instructions per launch (profiled) over this run's launch
-    time, against the nominal issue rate."""
+    time: per SIMD per cycle at 2.4 GHz, and against one wave64 instruction
+    per SIMD per 4 cycles (a ratio at or above 1 means the SIMDs issue VALU
+    as fast as that model allows, or faster)."""
     n = RESOLVE_VALU_WAVE_INSTS.get("fp16" if fp16 else "fp32")
     if n is None:
         return None
     rate = n / (ms * 1e-3)
-    return {"wave_insts_per_launch": n, "issue_per_s": rate, "nominal_issue_per_s": VALU_ISSUE_PEAK,
-            "frac_of_nominal": rate / VALU_ISSUE_PEAK}
+    return {"wave_insts_per_launch": n, "issue_per_s": rate, "wave_insts_per_simd_cycle": rate / (1024 * 2.4e9),
+            "ratio_to_one_per_4_cycles": rate / VALU_ISSUE_PEAK}
 
 
 def timed_steps(torch, fn, args) -> float:
@@ -727,6 +732,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": RESOLVE_TRAFFIC_BYTES.get("fp16" if args.fp16_gbuffer else "fp32"),
                      "algorithmic_bytes": nbytes, "kernel": "resolve_kernel",
+                     "traffic_depth_pass": DEPTH_PASS_TRAFFIC_BYTES.get("fp16" if args.fp16_gbuffer else "fp32"),
                      # What binds the kernel: VALU issue (64 lights per pixel), not HBM.
                      "limiter": "valu", "valu": valu_issue(ms, args.fp16_gbuffer),
                      "counters_from": "profiles/resolve_kernel_4k_fp32_64lights_ssao16.md"},
