@@ -341,7 +341,10 @@ typedef struct althea_ssao_desc {
  * reflection from the prefiltered cube at LOD roughness*(mips-1), irradiance,
  * SSAO into the occlusion channel, pbrMaterial with `light_count` point lights,
  * optional tonemap.  d_out: width*height RGBA (FP32 or FP16).  d_ao_out
- * (optional, may be NULL): width*height floats of the occlusion used. */
+ * (optional, may be NULL): width*height floats of the occlusion used.
+ * With SSAO enabled, every resolve entry point first writes the frame's view
+ * depth (width*height floats) into a stream-ordered allocation
+ * (hipMallocAsync / hipFreeAsync on `stream`) that the SSAO taps read. */
 ALTHEA_API althea_status althea_deferred_resolve(const althea_gbuffer_view* gbuffer, const althea_ibl_views* ibl,
                                                  const althea_point_light* d_lights, int light_count,
                                                  const althea_globals* globals, const althea_ssao_desc* ssao,
