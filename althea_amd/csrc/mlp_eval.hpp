@@ -46,12 +46,35 @@ __device__ __forceinline__ float sigmoidf(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
 }
 
+// sin(pi x) and cos(pi x) for the components of a unit direction (|x| <= 1
+// up to rounding): x is reflected into r in [-1/2, 1/2] (for |x| > 1/2,
+// r = sign(x) - x: sin(pi r) = sin(pi x), cos(pi r) = -cos(pi x)), then
+// sin(pi r) = r P(r^2) and cos(pi r) = Q(r^2) with P, Q of degree 4 fitted
+// to the functions on [0, 1/2] by iteratively reweighted least squares on
+// Chebyshev nodes (3.3e-9 and 4.7e-8 absolute in exact arithmetic, about
+// 2e-7 evaluated in FP32: far below the BF16 rounding of the features).
+__device__ __forceinline__ void sincospi_unit(float x, float& s, float& c) {
+  const bool far = fabsf(x) > 0.5f;
+  const float r = far ? copysignf(1.0f, x) - x : x;
+  const float u = r * r;
+  float ps = fmaf(u, 0.0772201270f, -0.5980451703f);
+  ps = fmaf(ps, u, 2.5500314236f);
+  ps = fmaf(ps, u, -5.1677069664f);
+  ps = fmaf(ps, u, 3.1415925026f);
+  float pc = fmaf(u, 0.2196981013f, -1.3318809271f);
+  pc = fmaf(pc, u, 4.0584120750f);
+  pc = fmaf(pc, u, -4.9347929955f);
+  pc = fmaf(pc, u, 0.9999999404f);
+  s = r * ps;
+  c = far ? -pc : pc;
+}
+
 // The input features of one query (28, zero-padded to 32): dir (0..2), then
 // for k = 0..3 sin(2^k pi d_c) at 3 + 6k + c and cos(2^k pi d_c) at
 // 3 + 6k + 3 + c (c = x, y, z), then roughness (27).  sin/cos(pi d) come
-// from sincospif, the higher octaves from the double-angle identities
+// from sincospi_unit, the higher octaves from the double-angle identities
 // sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a) (about
-// 5e-7 absolute after three doublings, far below the BF16 input rounding).
+// 1e-6 absolute after three doublings, far below the BF16 input rounding).
 __device__ __forceinline__ void encode_all(float dx, float dy, float dz, float rough, float f[32]) {
   f[0] = dx;
   f[1] = dy;
@@ -60,7 +83,7 @@ __device__ __forceinline__ void encode_all(float dx, float dy, float dz, float r
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float s, co;
-    sincospif(d[c], &s, &co);
+    sincospi_unit(d[c], s, co);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       f[3 + 6 * k + c] = s;

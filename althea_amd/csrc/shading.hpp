@@ -139,6 +139,9 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
   const float four_nv = 4.0f * m.n_dot_v;
   const float om_k = 1.0f - m.kdir;
   const float a2m1 = m.a2 - 1.0f;
+  // gl * (4 NdotV NdotL + 1e-4) = (NdotL (1 - k) + k)(4 NdotV NdotL + 1e-4),
+  // a quadratic in NdotL: (q2 NdotL + q1) NdotL + q0.
+  const float q2 = om_k * four_nv, q1 = fmaf(om_k, 0.0001f, m.kdir * four_nv), q0 = m.kdir * 0.0001f;
   const f3 kdp = m.kd * kInvPi;
   const f3 I2 = I * 2.0f;
   float A1 = 0.0f, A2 = 0.0f, A3 = 0.0f, A4 = 0.0f;
@@ -162,7 +165,7 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
       const float x = 1.0f - n_dot_h;
       const float x5 = pow5(x);
       const float t = fmaf(n_dot_h * n_dot_h, a2m1, 1.0f);
-      const float den = (t * t) * fmaf(n_dot_l, om_k, m.kdir) * fmaf(n_dot_l, four_nv, 0.0001f);
+      const float den = (t * t) * fmaf(fmaf(q2, n_dot_l, q1), n_dot_l, q0);
       const float s1 = n_dot_l * rcp_fast(den);  // s / c1
       float w = (inv_d * inv_d) * n_dot_l;
       if (Shadow::kActive && n_dot_l > 0.0f && shadow.hidden(i, L, d2 * inv_d)) w = 0.0f;
