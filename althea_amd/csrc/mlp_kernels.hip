@@ -335,7 +335,39 @@ struct AdamParams {
   float loss_scale;       // 1 / (3 * batch)
   int update;
   float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+  MlpDeviceWeights w;  // the BF16 operand copies, rewritten for each updated parameter
 };
+
+// Writes the operand copies of master parameter t (value v): the scatter form
+// of mlp_pack_kernel for one parameter (the zero padding is never touched).
+__device__ __forceinline__ void mlp_pack_one(const MlpDeviceWeights& w, int t, float v) {
+  const __bf16 b = static_cast<__bf16>(v);
+  auto slot64 = [](int k) { return (k & ~31) + mlp_perm_inv(k & 31); };
+  if (t < kOffB1) {
+    const int i = t - kOffW1, o = i / kMlpIn, k = i - o * kMlpIn;
+    w.w1[o * 32 + k] = b;
+  } else if (t < kOffW2) {
+    w.b1[t - kOffB1] = v;
+  } else if (t < kOffB2) {
+    const int i = t - kOffW2, o = i >> 6, k = i & 63;
+    w.w2[o * 64 + slot64(k)] = b;
+    w.w2t[k * 64 + slot64(o)] = b;
+  } else if (t < kOffW3) {
+    w.b2[t - kOffB2] = v;
+  } else if (t < kOffB3) {
+    const int i = t - kOffW3, o = i >> 6, k = i & 63;
+    w.w3[o * 64 + slot64(k)] = b;
+    w.w3t[k * 64 + slot64(o)] = b;
+  } else if (t < kOffW4) {
+    w.b3[t - kOffB3] = v;
+  } else if (t < kOffB4) {
+    const int i = t - kOffW4, o = i >> 6, k = i & 63;
+    w.w4[o * 64 + slot64(k)] = b;
+    w.w4t[k * 32 + mlp_perm_inv(o)] = b;
+  } else {
+    w.b4[t - kOffB4] = v;
+  }
+}
 
 // kAdamLanes lanes per parameter each sum every kAdamLanes-th partial row,
 // then a fixed xor tree over those lanes (every lane ends with the same
@@ -382,7 +414,9 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(const AdamParams p) {
   p.m[t] = m;
   p.v[t] = v;
   float mh = m / p.c1, vh = v / p.c2;
-  p.params[t] -= p.lr * mh / (sqrtf(vh) + p.eps);
+  const float np = p.params[t] - p.lr * mh / (sqrtf(vh) + p.eps);
+  p.params[t] = np;
+  mlp_pack_one(p.w, t, np);
 }
 
 // --------------------------------------------------------- layout probe
@@ -599,6 +633,7 @@ static althea_status run_train(MlpState* s, const float* dir_rough, const float*
   ap.loss_out = loss_out;
   ap.loss_scale = 1.0f / (3.0f * static_cast<float>(count));
   ap.update = update ? 1 : 0;
+  ap.w = s->w;
   if (update) {
     s->step += 1;
     ap.lr = s->desc.learning_rate;
@@ -612,7 +647,7 @@ static althea_status run_train(MlpState* s, const float* dir_rough, const float*
                      dim3(256), 0, stream, ap);
   st = check_launch("mlp_adam_kernel");
   if (st != ALTHEA_OK) return st;
-  return update ? launch_pack(s, stream) : ALTHEA_OK;
+  return ALTHEA_OK;
 }
 
 }  // namespace althea

@@ -42,6 +42,14 @@ __host__ __device__ constexpr int mlp_perm(int slot) {
   return (slot & 7) < 4 ? 4 * (slot >> 3) + (slot & 7) : 16 + 4 * (slot >> 3) + (slot & 7) - 4;
 }
 
+// The slot of feature k (0..31) in a 32-wide block: mlp_perm's inverse.
+__host__ __device__ constexpr int mlp_perm_inv(int k) {
+  return k < 16 ? 8 * (k >> 2) + (k & 3) : 8 * ((k - 16) >> 2) + 4 + ((k - 16) & 3);
+}
+static_assert(mlp_perm_inv(mlp_perm(0)) == 0 && mlp_perm_inv(mlp_perm(5)) == 5 && mlp_perm_inv(mlp_perm(13)) == 13 &&
+                  mlp_perm_inv(mlp_perm(22)) == 22 && mlp_perm_inv(mlp_perm(31)) == 31,
+              "mlp_perm_inv inverts mlp_perm");
+
 // Master-copy index -> weight-gradient index (or -1 for a bias).
 __host__ __device__ inline int mlp_param_to_wgrad(int t) {
   if (t < kOffB1) return kWgW1 + (t - kOffW1);
