@@ -107,8 +107,11 @@ static bool cube_ok(int size, int mips) {
   return is_pow2(size) && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1;
 }
 
+// Power-of-two extents up to 2^15, at most 2^27 texels (16384 x 8192): every
+// level's byte offsets, padded or not, fit 32 bits.
 static bool env_dims_ok(int w, int h) {
-  return is_pow2(w) && is_pow2(h) && w <= (1 << 15) && h <= (1 << 15);
+  return is_pow2(w) && is_pow2(h) && w <= (1 << 15) && h <= (1 << 15) &&
+         static_cast<int64_t>(w) * h <= (int64_t{1} << 27);
 }
 
 }  // namespace althea
@@ -204,7 +207,7 @@ size_t althea_env_pyramid_texels(int width, int height) {
 althea_status althea_env_pyramid_build(const float* d_level0, int width, int height, float* d_pyramid,
                                        hipStream_t stream) {
   ALTHEA_REQUIRE(d_level0 && d_pyramid, "env_pyramid_build: null buffer");
-  ALTHEA_REQUIRE(env_dims_ok(width, height), "env_pyramid_build: %dx%d is not a power-of-two extent", width, height);
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "env_pyramid_build: %dx%d is not a power-of-two extent of at most 2^27 texels", width, height);
   return launch_pyramid(d_level0, width, height, d_pyramid, stream);
 }
 
@@ -225,7 +228,7 @@ althea_status althea_env_sample(const float* d_pyramid, int width, int height, c
                                 float* d_out, hipStream_t stream) {
   ALTHEA_REQUIRE(count >= 0, "env_sample: negative count");
   ALTHEA_REQUIRE(count == 0 || (d_pyramid && d_dir_lod && d_out), "env_sample: null buffer");
-  ALTHEA_REQUIRE(env_dims_ok(width, height), "env_sample: %dx%d is not a power-of-two extent", width, height);
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "env_sample: %dx%d is not a power-of-two extent of at most 2^27 texels", width, height);
   return launch_env_sample(d_pyramid, width, height, d_dir_lod, count, d_out, nullptr, stream);
 }
 
@@ -234,7 +237,7 @@ static bool samples_ok(int samples) { return samples >= 1 && samples <= (1 << 20
 althea_status althea_prefilter_cube(const float* d_pyramid, int env_width, int env_height, int size, int mips,
                                     int samples, unsigned flags, float* d_cube, hipStream_t stream) {
   ALTHEA_REQUIRE(d_pyramid && d_cube, "prefilter_cube: null buffer");
-  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_cube: %dx%d is not a power-of-two extent",
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_cube: %dx%d is not a power-of-two extent of at most 2^27 texels",
                  env_width, env_height);
   ALTHEA_REQUIRE(cube_ok(size, mips), "prefilter_cube: size %d / mips %d unsupported", size, mips);
   ALTHEA_REQUIRE(samples_ok(samples), "prefilter_cube: samples %d out of range", samples);
@@ -246,18 +249,18 @@ althea_status althea_prefilter_cube(const float* d_pyramid, int env_width, int e
 
 size_t althea_prefilter_cube_scratch_bytes(int mips, int samples) {
   if (mips <= 0 || samples <= 0) return 0;
-  return prefilter_workspace_bytes(0, 0, 0, mips, samples);
+  return prefilter_table_bytes(mips, samples);
 }
 
 althea_status althea_prefilter_cube_noalloc(const float* d_pyramid, int env_width, int env_height, int size,
                                             int mips, int samples, unsigned flags, float* d_cube,
                                             hipStream_t stream) {
   ALTHEA_REQUIRE(d_pyramid && d_cube, "prefilter_cube_noalloc: null buffer");
-  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_cube_noalloc: %dx%d is not a power-of-two extent",
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_cube_noalloc: %dx%d is not a power-of-two extent of at most 2^27 texels",
                  env_width, env_height);
   ALTHEA_REQUIRE(cube_ok(size, mips), "prefilter_cube_noalloc: size %d / mips %d unsupported", size, mips);
   ALTHEA_REQUIRE(samples_ok(samples), "prefilter_cube_noalloc: samples %d out of range", samples);
-  const size_t need = prefilter_workspace_bytes(env_width, env_height, size, mips, samples);
+  const size_t need = prefilter_table_bytes(mips, samples);
   ALTHEA_REQUIRE(need <= ALTHEA_PREFILTER_SCRATCH_MAX_BYTES,
                  "prefilter_cube_noalloc: %d mips x %d samples need %zu bytes of sample tables, more than the "
                  "fixed %u-byte array (use althea_prefilter_cube)",
@@ -270,7 +273,7 @@ althea_status althea_prefilter_points(const float* d_pyramid, int env_width, int
                                       const float* d_dir_rough, int count, float* d_out, hipStream_t stream) {
   ALTHEA_REQUIRE(count >= 0, "prefilter_points: negative count");
   ALTHEA_REQUIRE(count == 0 || (d_pyramid && d_dir_rough && d_out), "prefilter_points: null buffer");
-  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_points: %dx%d is not a power-of-two extent",
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "prefilter_points: %dx%d is not a power-of-two extent of at most 2^27 texels",
                  env_width, env_height);
   ALTHEA_REQUIRE(samples_ok(samples), "prefilter_points: samples %d out of range", samples);
   return launch_prefilter_points(d_pyramid, env_width, env_height, samples, d_dir_rough, count, d_out, nullptr,
@@ -280,7 +283,7 @@ althea_status althea_prefilter_points(const float* d_pyramid, int env_width, int
 althea_status althea_irradiance_cube(const float* d_pyramid, int env_width, int env_height, int size,
                                      int theta_samples, unsigned flags, float* d_cube, hipStream_t stream) {
   ALTHEA_REQUIRE(d_pyramid && d_cube, "irradiance_cube: null buffer");
-  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "irradiance_cube: %dx%d is not a power-of-two extent",
+  ALTHEA_REQUIRE(env_dims_ok(env_width, env_height), "irradiance_cube: %dx%d is not a power-of-two extent of at most 2^27 texels",
                  env_width, env_height);
   ALTHEA_REQUIRE(size >= 1 && size <= 4096, "irradiance_cube: size %d out of range", size);
   ALTHEA_REQUIRE(theta_samples >= 1 && theta_samples <= 100000, "irradiance_cube: theta_samples %d out of range",
@@ -386,7 +389,7 @@ extern "C" {
 althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, int width, int height,
                                     hipStream_t stream) {
   ALTHEA_REQUIRE(ibl && d_equirect, "ibl_precompute: null argument");
-  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_precompute: %dx%d is not a power-of-two extent", width, height);
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_precompute: %dx%d is not a power-of-two extent of at most 2^27 texels", width, height);
   const althea_ibl_desc& d = ibl->desc;
   althea_status st = ibl_prepare(ibl, width, height, stream);
   if (st != ALTHEA_OK) return st;
@@ -553,7 +556,7 @@ althea_status althea_ibl_save_cache(const althea_ibl* ibl, const char* dir, hipS
 althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, int width, int height, const char* dir,
                                     hipStream_t stream) {
   ALTHEA_REQUIRE(ibl && d_equirect && dir && dir[0], "ibl_load_cache: null argument");
-  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_load_cache: %dx%d is not a power-of-two extent", width, height);
+  ALTHEA_REQUIRE(env_dims_ok(width, height), "ibl_load_cache: %dx%d is not a power-of-two extent of at most 2^27 texels", width, height);
   const althea_ibl_desc& d = ibl->desc;
   // Every file present, each with the handle's extent, before any device work.
   for (int m = -1; m < d.prefiltered_mips; ++m) {

@@ -29,11 +29,12 @@
 namespace althea {
 
 struct BatchMap {
-  const char* pyr;  // source pyramid (bytes)
-  float4* cube;     // output cube, [mip][face][y][x]
-  int size;         // face size at mip 0
-  int table_set;    // sample-table set of this map's source extent
-  int level_base;   // first LevelC of this map
+  const char* pyr;   // source pyramid (bytes)
+  const char* gpyr;  // its gather copy (bytes)
+  float4* cube;      // output cube, [mip][face][y][x]
+  int size;          // face size at mip 0
+  int table_set;     // sample-table set of this map's source extent
+  int level_base;    // first LevelC / GatherLevel of this map
   int pad;
 };
 
@@ -52,6 +53,7 @@ __global__ void batch_reset_kernel(int* counter) {
 template <bool kF64Sums, int kTW, int kTH>
 __global__ __launch_bounds__(256) void prefilter_batch_kernel(const BatchMap* __restrict__ maps,
                                                               const LevelC* __restrict__ levels,
+                                                              const GatherLevel* __restrict__ glevels,
                                                               const GgxEntry* __restrict__ tables,
                                                               const int* __restrict__ counts,
                                                               const double* __restrict__ wsum,
@@ -81,8 +83,9 @@ __global__ __launch_bounds__(256) void prefilter_batch_kernel(const BatchMap* __
       out = prefilter_texel_mirror(mp.pyr, level, face, min(x, S - 1), min(y, S - 1), S);
     } else {
       const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
-      out = prefilter_texel<kF64Sums>(mp.pyr, level, tables + static_cast<size_t>(set_mip) * p.samples,
-                                      counts[set_mip], wsum[set_mip], fr);
+      out = prefilter_texel<kF64Sums>(GatherAcc{mp.gpyr, glevels + mp.level_base},
+                                      tables + static_cast<size_t>(set_mip) * p.samples, counts[set_mip],
+                                      wsum[set_mip], fr);
     }
     if (active) {
       size_t off = 0;
@@ -130,10 +133,12 @@ struct althea_prefilter_batch {
   std::vector<int> sizes;
   // Device state.
   std::vector<float*> pyramids;
+  std::vector<float*> gathers;  // gather copies of the pyramids, owned by the batch
   std::vector<float*> cubes;
   bool owns_buffers = true;  // false when the pyramids and cubes belong to IBL handles
   BatchMap* d_maps = nullptr;
   LevelC* d_levels = nullptr;
+  GatherLevel* d_glevels = nullptr;
   GgxEntry* d_tables = nullptr;
   int* d_counts = nullptr;
   double* d_wsum = nullptr;
@@ -208,15 +213,15 @@ void launch_persistent_t(const althea_prefilter_batch* b, int shape, int blocks,
   switch (shape) {
     case 0:
       hipLaunchKernelGGL((prefilter_batch_kernel<kF64, 8, 8>), dim3(blocks), dim3(256), 0, s, b->d_maps, b->d_levels,
-                         b->d_tables, b->d_counts, b->d_wsum, work, b->d_counter, p);
+                         b->d_glevels, b->d_tables, b->d_counts, b->d_wsum, work, b->d_counter, p);
       break;
     case 1:
       hipLaunchKernelGGL((prefilter_batch_kernel<kF64, 16, 4>), dim3(blocks), dim3(256), 0, s, b->d_maps,
-                         b->d_levels, b->d_tables, b->d_counts, b->d_wsum, work, b->d_counter, p);
+                         b->d_levels, b->d_glevels, b->d_tables, b->d_counts, b->d_wsum, work, b->d_counter, p);
       break;
     default:
       hipLaunchKernelGGL((prefilter_batch_kernel<kF64, 4, 16>), dim3(blocks), dim3(256), 0, s, b->d_maps,
-                         b->d_levels, b->d_tables, b->d_counts, b->d_wsum, work, b->d_counter, p);
+                         b->d_levels, b->d_glevels, b->d_tables, b->d_counts, b->d_wsum, work, b->d_counter, p);
       break;
   }
 }
@@ -235,10 +240,14 @@ althea_status launch_persistent(const althea_prefilter_batch* b, int shape, int 
   return check_launch("prefilter_batch_kernel");
 }
 
-// Pyramids of all maps and the sample tables of every source extent.
+// Pyramids of all maps with their gather copies, and the sample tables of
+// every source extent.
 althea_status launch_prep(const althea_prefilter_batch* b, hipStream_t s) {
   for (int i = 0; i < b->count; ++i) {
-    althea_status st = launch_pyramid(b->envs[i].rgba, b->envs[i].width, b->envs[i].height, b->pyramids[i], s);
+    const int w = b->envs[i].width, h = b->envs[i].height;
+    althea_status st = launch_pyramid(b->envs[i].rgba, w, h, b->pyramids[i], s);
+    if (st != ALTHEA_OK) return st;
+    st = launch_gather(b->pyramids[i], w, h, gather_geometry(w, h, 0, althea_env_levels(w, h) - 1), b->gathers[i], s);
     if (st != ALTHEA_OK) return st;
   }
   for (size_t k = 0; k < b->extents.size(); ++k) {
@@ -288,8 +297,11 @@ static althea_status batch_create(const althea_env_view* envs, int count, const 
   ALTHEA_REQUIRE(samples >= 1 && samples <= (1 << 20), "prefilter_batch_create: samples %d", samples);
   ALTHEA_REQUIRE(mips >= 1 && mips <= 16, "prefilter_batch_create: mips %d", mips);
   for (int i = 0; i < count; ++i) {
-    ALTHEA_REQUIRE(envs[i].rgba && is_pow2(envs[i].width) && is_pow2(envs[i].height),
-                   "prefilter_batch_create: map %d extent %dx%d", i, envs[i].width, envs[i].height);
+    ALTHEA_REQUIRE(envs[i].rgba && is_pow2(envs[i].width) && is_pow2(envs[i].height) && envs[i].width <= (1 << 15) &&
+                       envs[i].height <= (1 << 15) &&
+                       static_cast<int64_t>(envs[i].width) * envs[i].height <= (int64_t{1} << 27),
+                   "prefilter_batch_create: map %d extent %dx%d (powers of two, at most 2^27 texels)", i,
+                   envs[i].width, envs[i].height);
     ALTHEA_REQUIRE(is_pow2(face_sizes[i]) && (face_sizes[i] >> (mips - 1)) >= 1,
                    "prefilter_batch_create: map %d face size %d with %d mips", i, face_sizes[i], mips);
   }
@@ -314,7 +326,9 @@ static althea_status batch_create(const althea_env_view* envs, int count, const 
   std::map<std::pair<int, int>, int> set_of;
   std::vector<BatchMap> maps(count);
   std::vector<LevelC> levels(static_cast<size_t>(count) * kMaxLevels);
+  std::vector<GatherLevel> glevels(static_cast<size_t>(count) * kMaxLevels);
   b->pyramids.assign(count, nullptr);
+  b->gathers.assign(count, nullptr);
   b->cubes.assign(count, nullptr);
   b->owns_buffers = ext_pyramids == nullptr;
   for (int i = 0; i < count; ++i) {
@@ -333,13 +347,21 @@ static althea_status batch_create(const althea_env_view* envs, int count, const 
       set_error("prefilter_batch_create: out of device memory at map %d", i);
       return fail(ALTHEA_ERROR_OUT_OF_MEMORY);
     }
-    pyramid_levels(envs[i].width, envs[i].height, &levels[static_cast<size_t>(i) * kMaxLevels]);
-    maps[i] = BatchMap{reinterpret_cast<const char*>(b->pyramids[i]), reinterpret_cast<float4*>(b->cubes[i]),
-                       face_sizes[i], it->second, i * kMaxLevels, 0};
+    const int w = envs[i].width, h = envs[i].height;
+    const GatherGeometry gg = gather_geometry(w, h, 0, althea_env_levels(w, h) - 1);
+    if (hipMalloc(&b->gathers[i], 16 * gg.texels) != hipSuccess) {
+      set_error("prefilter_batch_create: out of device memory at map %d", i);
+      return fail(ALTHEA_ERROR_OUT_OF_MEMORY);
+    }
+    pyramid_levels(w, h, &levels[static_cast<size_t>(i) * kMaxLevels]);
+    gather_levels(w, h, gg, &glevels[static_cast<size_t>(i) * kMaxLevels]);
+    maps[i] = BatchMap{reinterpret_cast<const char*>(b->pyramids[i]), reinterpret_cast<const char*>(b->gathers[i]),
+                       reinterpret_cast<float4*>(b->cubes[i]), face_sizes[i], it->second, i * kMaxLevels, 0};
   }
   const size_t sets = b->extents.size() * mips;
   if (hipMalloc(&b->d_maps, sizeof(BatchMap) * count) != hipSuccess ||
       hipMalloc(&b->d_levels, sizeof(LevelC) * levels.size()) != hipSuccess ||
+      hipMalloc(&b->d_glevels, sizeof(GatherLevel) * glevels.size()) != hipSuccess ||
       hipMalloc(&b->d_tables, sizeof(GgxEntry) * sets * samples) != hipSuccess ||
       hipMalloc(&b->d_counts, sizeof(int) * sets) != hipSuccess ||
       hipMalloc(&b->d_wsum, sizeof(double) * sets) != hipSuccess ||
@@ -350,6 +372,8 @@ static althea_status batch_create(const althea_env_view* envs, int count, const 
   }
   if (hipMemcpy(b->d_maps, maps.data(), sizeof(BatchMap) * count, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_levels, levels.data(), sizeof(LevelC) * levels.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_glevels, glevels.data(), sizeof(GatherLevel) * glevels.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
       hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking) != hipSuccess) {
     set_error("prefilter_batch_create: upload failed");
     return fail(ALTHEA_ERROR_HIP);
@@ -375,8 +399,10 @@ althea_status althea_prefilter_batch_destroy(althea_prefilter_batch* b) {
     for (float* p : b->pyramids) (void)hipFree(p);
     for (float* p : b->cubes) (void)hipFree(p);
   }
+  for (float* p : b->gathers) (void)hipFree(p);
   (void)hipFree(b->d_maps);
   (void)hipFree(b->d_levels);
+  (void)hipFree(b->d_glevels);
   (void)hipFree(b->d_tables);
   (void)hipFree(b->d_counts);
   (void)hipFree(b->d_wsum);

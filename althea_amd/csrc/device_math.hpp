@@ -45,6 +45,8 @@ __device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+// min(max(x, 0), 1); folds into the producing instruction's clamp bit.
+__device__ __forceinline__ float saturate(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 __device__ __forceinline__ f3 normalize_fast(f3 a) { return a * rsq_fast(fmaf(a.x, a.x, fmaf(a.y, a.y, a.z * a.z))); }
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
   return {fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z), fmaf(b.w - a.w, t, a.w)};
@@ -184,15 +186,72 @@ __device__ __forceinline__ f3 level_bilinear(const char* __restrict__ pyr_bytes,
   return lerp3(top, bot, t.ay);
 }
 
-// acc += w * bilinear(yaw, pitch) in weight form: the four texel weights
-// carry w, so a sample's two source levels (w = its weight times 1 - delta
-// and times delta) add straight into the running sums.
-__device__ __forceinline__ void level_bilinear_acc(const char* __restrict__ pyr_bytes, const LevelC& c, float yaw,
-                                                   float pitch, float w, float& ar, float& ag, float& ab) {
-  const LevelTaps t = level_taps(pyr_bytes, c, yaw, pitch);
-  const float wy0 = (1.0f - t.ay) * w, wy1 = t.ay * w;
-  const float bx = 1.0f - t.ax;
-  const float w00 = bx * wy0, w10 = t.ax * wy0, w01 = bx * wy1, w11 = t.ax * wy1;
+// Gather copy of a pyramid level, read by the precompute's sample loops:
+// the level with its REPEAT neighbours around it, (w + 2) x (h + 2) texels,
+// padded (r, c) = level((r - 1) mod h, (c - 1) mod w).  The texel coordinate
+// is formed exactly as level_taps forms it; with the wrap stored, a tap pair
+// sits at one byte offset and offset + 16, the next row at + rowbytes.
+struct GatherLevel {
+  float kx, cx, ky, cy;  // x = yaw kx + cx, y = pitch ky + cy (as LevelC)
+  uint32_t rowbytes;     // (w + 2) * 16
+  uint32_t off;          // first texel of the padded level, float4 units
+  uint32_t pad0, pad1;
+};
+
+__host__ inline GatherLevel make_gather_level(int w, int h, uint32_t off) {
+  const LevelC c = make_level(w, h, 0);
+  GatherLevel g{};
+  g.kx = c.kx;
+  g.cx = c.cx;
+  g.ky = c.ky;
+  g.cy = c.cy;
+  g.rowbytes = static_cast<uint32_t>(w + 2) * 16u;
+  g.off = off;
+  return g;
+}
+
+// level_taps on a gather level: the same four texels and fractions.  The
+// angles are those of equirect_angles, |yaw| <= float(pi) and |pitch| <=
+// float(pi) / 2, so x lies in [-0.5, w - 0.5] and y in [-0.5, h - 0.5] up to
+// a rounding of w * 5e-8 texel: floor(x) + 1 is in [0, w] and floor(y) + 1
+// in [0, h], inside the padded level (levels are at most 2^15 wide; a NaN
+// angle converts to index 0).
+__device__ __forceinline__ LevelTaps gather_taps(const char* __restrict__ gpyr_bytes, const GatherLevel& g, float yaw,
+                                                 float pitch) {
+  const float x = fmaf(yaw, g.kx, g.cx), y = fmaf(pitch, g.ky, g.cy);
+  LevelTaps t;
+  t.ax = __builtin_amdgcn_fractf(x);
+  t.ay = __builtin_amdgcn_fractf(y);
+  const int ix = cvt_floor(x), iy = cvt_floor(y);
+  // (iy + 1) rowbytes + (ix + 1) 16 as iy * rowbytes + (16 ix + rowbytes + 16):
+  // one shift-add and one signed 24-bit multiply-add (iy >= -1, rowbytes < 2^20).
+  const int rb = static_cast<int>(g.rowbytes);
+  int col, o;
+  asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(col) : "v"(ix), "s"(rb + 16));
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(o) : "v"(iy), "s"(rb), "v"(col));
+  const uint32_t o0 = static_cast<uint32_t>(o);
+  const uint32_t o1 = o0 + g.rowbytes;
+  // Global address space: the loads take the scalar base plus a 32-bit lane
+  // offset and the immediate +16, also where the base comes from memory.
+  typedef const __attribute__((address_space(1))) char gchar;
+  typedef const __attribute__((address_space(1))) float gfloat;
+  gchar* lvl = (gchar*)gpyr_bytes + (static_cast<size_t>(g.off) << 4);
+  gfloat* r0 = (gfloat*)(lvl + o0);
+  gfloat* r1 = (gfloat*)(lvl + o1);
+  t.t00 = mk(r0[0], r0[1], r0[2]);
+  t.t10 = mk(r0[4], r0[5], r0[6]);
+  t.t01 = mk(r1[0], r1[1], r1[2]);
+  t.t11 = mk(r1[4], r1[5], r1[6]);
+  return t;
+}
+
+// acc += w * bilinear in weight form: the four texel weights carry w, so a
+// sample's two source levels (w = its weight times 1 - delta and times
+// delta) add straight into the running sums.  w (1 - a) is one FMA.
+__device__ __forceinline__ void taps_acc(const LevelTaps& t, float w, float& ar, float& ag, float& ab) {
+  const float wy1 = t.ay * w, wy0 = fmaf(-t.ay, w, w);
+  const float w10 = t.ax * wy0, w00 = fmaf(-t.ax, wy0, wy0);
+  const float w11 = t.ax * wy1, w01 = fmaf(-t.ax, wy1, wy1);
   ar = fmaf(t.t00.x, w00, ar);
   ag = fmaf(t.t00.y, w00, ag);
   ab = fmaf(t.t00.z, w00, ab);
@@ -205,6 +264,17 @@ __device__ __forceinline__ void level_bilinear_acc(const char* __restrict__ pyr_
   ar = fmaf(t.t11.x, w11, ar);
   ag = fmaf(t.t11.y, w11, ag);
   ab = fmaf(t.t11.z, w11, ab);
+}
+
+__device__ __forceinline__ void level_bilinear_acc(const char* __restrict__ pyr_bytes, const LevelC& c, float yaw,
+                                                   float pitch, float w, float& ar, float& ag, float& ab) {
+  taps_acc(level_taps(pyr_bytes, c, yaw, pitch), w, ar, ag, ab);
+}
+
+__device__ __forceinline__ void gather_bilinear_acc(const char* __restrict__ gpyr_bytes, const GatherLevel& g,
+                                                    float yaw, float pitch, float w, float& ar, float& ag,
+                                                    float& ab) {
+  taps_acc(gather_taps(gpyr_bytes, g, yaw, pitch), w, ar, ag, ab);
 }
 
 // Bilinear CLAMP_TO_EDGE fetch (the shading-side samplers).

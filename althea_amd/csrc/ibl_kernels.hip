@@ -214,7 +214,8 @@ __global__ __launch_bounds__(kTableThreads) void ggx_table_kernel(TableParams p,
 
 // ------------------------------------------------------------ prefilter
 struct PrefilterParams {
-  LevelC lv[kMaxLevels];  // source pyramid levels
+  LevelC lv[kMaxLevels];        // source pyramid levels
+  GatherLevel glv[kMaxLevels];  // the same levels in the gather copy
   CubeMips cube;
   int samples;
   int tile_start[kMaxLevels + 1];  // 8x8-texel tiles before each mip
@@ -222,8 +223,11 @@ struct PrefilterParams {
 
 constexpr int kPrefilterTile = 8;
 
-template <bool kF64Sums>
+// kGather: the sample loop reads the padded gather copy `gpyr`; otherwise the
+// pyramid (bit-identical sums, more address arithmetic per lookup).
+template <bool kF64Sums, bool kGather>
 __global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restrict__ pyr,
+                                                            const char* __restrict__ gpyr,
                                                             const GgxEntry* __restrict__ tables,
                                                             const int* __restrict__ counts,
                                                             const double* __restrict__ wsum,
@@ -247,14 +251,88 @@ __global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restri
   } else {
     const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
     const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
-    out = prefilter_texel<kF64Sums>(pyr, level, tab, counts[m], wsum[m], fr);
+    if (kGather)
+      out = prefilter_texel<kF64Sums>(GatherAcc{gpyr, p.glv}, tab, counts[m], wsum[m], fr);
+    else
+      out = prefilter_texel<kF64Sums>(PyramidAcc{pyr, p.lv}, tab, counts[m], wsum[m], fr);
   }
   if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
 }
 
+static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
 void pyramid_levels(int w, int h, LevelC* out) {
   PyramidGeometry g = pyramid_geometry(w, h);
   for (int l = 0; l < g.levels; ++l) out[l] = make_level(g.w[l], g.h[l], g.off[l]);
+}
+
+// ------------------------------------------------------------ gather copy
+// Levels first..last of a pyramid, each padded with its REPEAT neighbours
+// ((w + 2) x (h + 2) texels, GatherLevel), concatenated from texel 0.
+GatherGeometry gather_geometry(int w, int h, int first, int last) {
+  const PyramidGeometry g = pyramid_geometry(w, h);
+  GatherGeometry out{};
+  out.first = first;
+  out.last = last;
+  size_t total = 0;
+  for (int l = first; l <= last; ++l) {
+    out.off[l] = static_cast<uint32_t>(total);
+    total += static_cast<size_t>(g.w[l] + 2) * (g.h[l] + 2);
+  }
+  out.texels = total;
+  return out;
+}
+
+size_t gather_bytes(int w, int h, int first, int last) {
+  if (w <= 0 || h <= 0) return 0;
+  return align256(16 * gather_geometry(w, h, first, last).texels);
+}
+
+void gather_levels(int w, int h, const GatherGeometry& gg, GatherLevel* out) {
+  const PyramidGeometry g = pyramid_geometry(w, h);
+  for (int l = gg.first; l <= gg.last; ++l) out[l] = make_gather_level(g.w[l], g.h[l], gg.off[l]);
+}
+
+struct PadParams {
+  int first, last;
+  int w[kMaxLevels], h[kMaxLevels];
+  uint32_t src[kMaxLevels];      // pyramid texel offset of each level
+  uint32_t dst[kMaxLevels + 1];  // gather texel offset of each level; dst[last + 1] = total
+};
+
+// One thread per gather texel: padded (r, c) of level l is the pyramid texel
+// ((r - 1) mod h, (c - 1) mod w).
+__global__ __launch_bounds__(256) void gather_pad_kernel(const float4* __restrict__ pyr, const PadParams p,
+                                                         float4* __restrict__ gpyr) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= p.dst[p.last + 1]) return;
+  int l = p.first;
+  while (l < p.last && i >= p.dst[l + 1]) ++l;
+  const int w = p.w[l], h = p.h[l];
+  const uint32_t k = i - p.dst[l];
+  const int r = static_cast<int>(k / static_cast<uint32_t>(w + 2));
+  const int c = static_cast<int>(k) - r * (w + 2);
+  const int sr = r == 0 ? h - 1 : (r == h + 1 ? 0 : r - 1);
+  const int sc = c == 0 ? w - 1 : (c == w + 1 ? 0 : c - 1);
+  gpyr[i] = pyr[p.src[l] + static_cast<size_t>(sr) * w + sc];
+}
+
+althea_status launch_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr, hipStream_t s) {
+  const PyramidGeometry g = pyramid_geometry(w, h);
+  PadParams p{};
+  p.first = gg.first;
+  p.last = gg.last;
+  for (int l = gg.first; l <= gg.last; ++l) {
+    p.w[l] = g.w[l];
+    p.h[l] = g.h[l];
+    p.src[l] = g.off[l];
+    p.dst[l] = gg.off[l];
+  }
+  p.dst[gg.last + 1] = static_cast<uint32_t>(gg.texels);
+  const uint32_t blocks = static_cast<uint32_t>((gg.texels + 255) / 256);
+  hipLaunchKernelGGL(gather_pad_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4*>(pyr), p,
+                     reinterpret_cast<float4*>(gpyr));
+  return check_launch("gather_pad_kernel");
 }
 
 static CubeMips cube_mips_of(int size, int mips) {
@@ -265,7 +343,6 @@ static CubeMips cube_mips_of(int size, int mips) {
   return c;
 }
 
-static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, GgxEntry* tables, int* counts,
                                 double* wsum, float* wsum32, hipStream_t s) {
@@ -278,9 +355,14 @@ althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, Ggx
   return check_launch("ggx_table_kernel");
 }
 
-size_t prefilter_workspace_bytes(int, int, int, int mips, int samples) {
+size_t prefilter_table_bytes(int mips, int samples) {
   return align256(sizeof(GgxEntry) * static_cast<size_t>(mips) * samples) + align256(sizeof(int) * mips) +
          align256(sizeof(double) * mips) + align256(sizeof(float) * mips);
+}
+
+// The sample tables, then the gather copy of every level.
+size_t prefilter_workspace_bytes(int env_w, int env_h, int, int mips, int samples) {
+  return prefilter_table_bytes(mips, samples) + gather_bytes(env_w, env_h, 0, althea_env_levels(env_w, env_h) - 1);
 }
 
 // ------------------------------------------- allocation-free prefilter
@@ -318,16 +400,23 @@ static althea_status prefilter_scratch(void** out) {
 // Every table entry, count and weight sum the prefilter reads is written by
 // the table kernel of the same call before the texel kernel runs, so nothing
 // left in the array by an earlier call is read and nothing needs clearing.
+static althea_status prefilter_cube_tables_in(const float* pyr, int env_w, int env_h, int size, int mips,
+                                              int samples, unsigned flags, float* cube, void* workspace,
+                                              float* gather, hipStream_t s);
+
 althea_status launch_prefilter_cube_noalloc(const float* pyr, int env_w, int env_h, int size, int mips,
                                             int samples, unsigned flags, float* cube, hipStream_t s) {
   void* ws = nullptr;
   althea_status st = prefilter_scratch(&ws);
   if (st != ALTHEA_OK) return st;
-  return launch_prefilter_cube(pyr, env_w, env_h, size, mips, samples, flags, cube, ws, s);
+  return prefilter_cube_tables_in(pyr, env_w, env_h, size, mips, samples, flags, cube, ws, nullptr, s);
 }
 
-althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,
-                                    unsigned flags, float* cube, void* workspace, hipStream_t s) {
+// Tables in `workspace`; the sample loop reads the gather copy `gather` when
+// one is given (written here first), else the pyramid.
+static althea_status prefilter_cube_tables_in(const float* pyr, int env_w, int env_h, int size, int mips,
+                                              int samples, unsigned flags, float* cube, void* workspace,
+                                              float* gather, hipStream_t s) {
   char* ws = static_cast<char*>(workspace);
   GgxEntry* tables = reinterpret_cast<GgxEntry*>(ws);
   ws += align256(sizeof(GgxEntry) * static_cast<size_t>(mips) * samples);
@@ -342,6 +431,12 @@ althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int 
 
   PrefilterParams p{};
   pyramid_levels(env_w, env_h, p.lv);
+  if (gather) {
+    const GatherGeometry gg = gather_geometry(env_w, env_h, 0, althea_env_levels(env_w, env_h) - 1);
+    st = launch_gather(pyr, env_w, env_h, gg, gather, s);
+    if (st != ALTHEA_OK) return st;
+    gather_levels(env_w, env_h, gg, p.glv);
+  }
   p.cube = cube_mips_of(size, mips);
   p.samples = samples;
   int tiles = 0;
@@ -353,14 +448,19 @@ althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int 
   }
   p.tile_start[mips] = tiles;
   const char* pyr_b = reinterpret_cast<const char*>(pyr);
+  const char* g_b = reinterpret_cast<const char*>(gather);
   float4* cube4 = reinterpret_cast<float4*>(cube);
-  if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
-    hipLaunchKernelGGL(prefilter_cube_kernel<false>, dim3(tiles), dim3(64), 0, s, pyr_b, tables, counts, wsum, p,
-                       cube4);
-  else
-    hipLaunchKernelGGL(prefilter_cube_kernel<true>, dim3(tiles), dim3(64), 0, s, pyr_b, tables, counts, wsum, p,
-                       cube4);
+  const bool f64 = !(flags & ALTHEA_PRECOMPUTE_FP32_SUMS);
+  auto kernel = f64 ? (gather ? prefilter_cube_kernel<true, true> : prefilter_cube_kernel<true, false>)
+                    : (gather ? prefilter_cube_kernel<false, true> : prefilter_cube_kernel<false, false>);
+  hipLaunchKernelGGL(kernel, dim3(tiles), dim3(64), 0, s, pyr_b, g_b, tables, counts, wsum, p, cube4);
   return check_launch("prefilter_cube_kernel");
+}
+
+althea_status launch_prefilter_cube(const float* pyr, int env_w, int env_h, int size, int mips, int samples,
+                                    unsigned flags, float* cube, void* workspace, hipStream_t s) {
+  float* gather = reinterpret_cast<float*>(static_cast<char*>(workspace) + prefilter_table_bytes(mips, samples));
+  return prefilter_cube_tables_in(pyr, env_w, env_h, size, mips, samples, flags, cube, workspace, gather, s);
 }
 
 // -------------------------------------------------------- point prefilter
@@ -459,13 +559,12 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
 }
 
 // ------------------------------------------------------------ irradiance
-// GenIrradianceMap.comp:106-155 per cube texel.  A 256-thread block owns a
-// 4x4 texel tile; lane = (texel 0..15, theta slice 0..15).  Slice k walks
-// theta_i for i = k, k+16, ... with the phi loop wave-uniform (scalar table
-// loads); per-theta FP32 sums are flushed into FP64, then reduced over the
-// four slices of a wave with shuffles and over the four waves in LDS.
+// GenIrradianceMap.comp:106-155 per cube texel: the Riemann sum over
+// theta_samples azimuths x phi_samples polar angles of radiance(d) cos sin,
+// read at the fixed source LOD log2(W / theta_samples), times
+// pi / (theta_samples phi_samples).
 struct IrradianceParams {
-  LevelC lv0, lv1;  // the two source levels around LOD log2(W / theta_samples)
+  GatherLevel lv0, lv1;  // the two source levels around LOD log2(W / theta_samples), gather copies
   int levels;
   int size;
   int theta_samples;
@@ -493,13 +592,36 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
   }
 }
 
-// One wave per output texel, lane = theta row (rows lane, lane + 64, ...):
-// for a fixed phi the 64 lanes look up 64 consecutive directions on one ring
-// around the texel normal, so a wave's lookups are neighbours.  Each lane
-// sums its row over phi in FP32 in phi order; the rows are combined in FP64
-// (kF64Sums) or FP32 by a fixed xor tree across the wave.
+// One wave per output texel.  A lane owns one polar column (phi index
+// 16 k + lane / 4) per pass and walks the azimuth rows theta = 4 s + lane % 4,
+// so for a fixed step the wave's 64 lookups cover a compact patch of 16 polar
+// by 4 azimuth samples (at most 9.6 x 4.8 degrees at the configured 300 x 150)
+// and the gathers of a step share cache lines; a ring of 64 azimuths reaches
+// 77 degrees.  Each lane sums its column's samples over theta in FP32 in theta
+// order and flushes the column into FP64 (kF64Sums) or FP32; the lanes are
+// combined by a fixed xor tree across the wave.
+constexpr int kIrrPhiLanes = 16, kIrrThetaLanes = 4;
+static_assert(kIrrPhiLanes * kIrrThetaLanes == 64, "one wave per texel");
+
+template <bool kTwoLevels>
+__device__ __forceinline__ void irradiance_column(const char* __restrict__ gpyr, const float2* __restrict__ theta_tab,
+                                                  const IrradianceParams& p, f3 u, f3 v, f3 w, float w0, float w1,
+                                                  int a, float& sr, float& sg, float& sb) {
+  for (int i = a; i < p.theta_samples; i += kIrrThetaLanes) {
+    const float2 th = theta_tab[i];
+    // n cos(phi) + (t cos(theta) + b sin(theta)) sin(phi), with u = t sin(phi),
+    // v = b sin(phi), w = n cos(phi).
+    const f3 d = mk(fmaf(u.x, th.x, fmaf(v.x, th.y, w.x)), fmaf(u.y, th.x, fmaf(v.y, th.y, w.y)),
+                    fmaf(u.z, th.x, fmaf(v.z, th.y, w.z)));
+    float yaw, pitch;
+    equirect_angles(d, yaw, pitch);
+    gather_bilinear_acc(gpyr, p.lv0, yaw, pitch, w0, sr, sg, sb);
+    if (kTwoLevels) gather_bilinear_acc(gpyr, p.lv1, yaw, pitch, w1, sr, sg, sb);
+  }
+}
+
 template <bool kF64Sums>
-__global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ pyr,
+__global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ gpyr,
                                                               const float2* __restrict__ theta_tab,
                                                               const float4* __restrict__ phi_tab,
                                                               const IrradianceParams p, float4* __restrict__ cube) {
@@ -511,21 +633,17 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __rest
   const int rem = texel - face * S * S;
   const int y = rem / S, x = rem - y * S;
   const Frame fr = local_to_world(cube_texel_raw_dir(face, x, y, S));
+  const int a = lane % kIrrThetaLanes, b = lane / kIrrThetaLanes;
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float fr32 = 0.0f, fg32 = 0.0f, fb32 = 0.0f;
-  for (int i = lane; i < p.theta_samples; i += 64) {
-    const float2 th = theta_tab[i];
-    const f3 a = fr.t * th.x + fr.b * th.y;
+  for (int j = b; j < p.phi_samples; j += kIrrPhiLanes) {
+    const float4 ph = phi_tab[j];
+    const f3 u = fr.t * ph.y, v = fr.b * ph.y, w = fr.n * ph.x;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-    for (int j = 0; j < p.phi_samples; ++j) {
-      const float4 ph = phi_tab[j];
-      const f3 d = mk(fmaf(a.x, ph.y, fr.n.x * ph.x), fmaf(a.y, ph.y, fr.n.y * ph.x),
-                      fmaf(a.z, ph.y, fr.n.z * ph.x));
-      float yaw, pitch;
-      equirect_angles(d, yaw, pitch);
-      level_bilinear_acc(pyr, p.lv0, yaw, pitch, ph.z, sr, sg, sb);
-      if (p.delta != 0.0f) level_bilinear_acc(pyr, p.lv1, yaw, pitch, ph.w, sr, sg, sb);
-    }
+    if (p.delta != 0.0f)
+      irradiance_column<true>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
+    else
+      irradiance_column<false>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
     if (kF64Sums) {
       ar += sr;
       ag += sg;
@@ -564,9 +682,31 @@ static int irradiance_phi_samples(int w, int h, int theta) {
   return static_cast<int>(static_cast<float>(h) * static_cast<float>(theta) / static_cast<float>(w));
 }
 
+// Source LOD log2(W / thetaSamples) (GenIrradianceMap.comp:139) in FP32 as
+// the shader computes it, clamped to the pyramid: levels l0, l1 and the
+// fraction delta.
+struct IrradianceLod {
+  int l0, l1;
+  float delta;
+};
+static IrradianceLod irradiance_lod(int w, int h, int theta) {
+  const int levels = althea_env_levels(w, h);
+  float lod = std::log2(static_cast<float>(w) / static_cast<float>(theta));
+  lod = std::min(std::max(lod, 0.0f), static_cast<float>(levels - 1));
+  IrradianceLod r;
+  r.l0 = static_cast<int>(std::floor(lod));
+  r.l1 = std::min(r.l0 + 1, levels - 1);
+  r.delta = lod - static_cast<float>(r.l0);
+  return r;
+}
+
+// The theta and phi tables, then the gather copy of levels l0..l1.
 size_t irradiance_workspace_bytes(int w, int h, int, int theta) {
   int phi = irradiance_phi_samples(w, h, theta);
-  return align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
+  const size_t tables = align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
+  if (w <= 0 || h <= 0 || theta <= 0) return tables;
+  const IrradianceLod lod = irradiance_lod(w, h, theta);
+  return tables + gather_bytes(w, h, lod.l0, lod.l1);
 }
 
 althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
@@ -575,7 +715,10 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   ALTHEA_REQUIRE(phi >= 1, "irradiance: int(H*theta/W) = %d phi samples", phi);
   char* ws = static_cast<char*>(workspace);
   float2* theta_tab = reinterpret_cast<float2*>(ws);
-  float4* phi_tab = reinterpret_cast<float4*>(ws + align256(sizeof(float2) * theta_samples));
+  ws += align256(sizeof(float2) * theta_samples);
+  float4* phi_tab = reinterpret_cast<float4*>(ws);
+  ws += align256(sizeof(float4) * phi);
+  float* gather = reinterpret_cast<float*>(ws);
 
   IrradianceParams p{};
   PyramidGeometry g = pyramid_geometry(env_w, env_h);
@@ -583,28 +726,28 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.size = size;
   p.theta_samples = theta_samples;
   p.phi_samples = phi;
-  // Source LOD log2(W / thetaSamples) (GenIrradianceMap.comp:139) in FP32 as
-  // the shader computes it, clamped to the pyramid.
-  float lod = std::log2(static_cast<float>(env_w) / static_cast<float>(theta_samples));
-  lod = std::min(std::max(lod, 0.0f), static_cast<float>(p.levels - 1));
-  p.l0 = static_cast<int>(std::floor(lod));
-  p.l1 = std::min(p.l0 + 1, p.levels - 1);
-  p.delta = lod - static_cast<float>(p.l0);
-  p.lv0 = make_level(g.w[p.l0], g.h[p.l0], g.off[p.l0]);
-  p.lv1 = make_level(g.w[p.l1], g.h[p.l1], g.off[p.l1]);
+  const IrradianceLod lod = irradiance_lod(env_w, env_h, theta_samples);
+  p.l0 = lod.l0;
+  p.l1 = lod.l1;
+  p.delta = lod.delta;
+  const GatherGeometry gg = gather_geometry(env_w, env_h, p.l0, p.l1);
+  althea_status st = launch_gather(pyr, env_w, env_h, gg, gather, s);
+  if (st != ALTHEA_OK) return st;
+  p.lv0 = make_gather_level(g.w[p.l0], g.h[p.l0], gg.off[p.l0]);
+  p.lv1 = make_gather_level(g.w[p.l1], g.h[p.l1], gg.off[p.l1]);
   p.scale = M_PI / theta_samples / phi;
   const int n = std::max(theta_samples, phi);
   hipLaunchKernelGGL(irradiance_tables_kernel, dim3((n + 255) / 256), dim3(256), 0, s, theta_samples, phi, p.delta,
                      theta_tab, phi_tab);
-  althea_status st = check_launch("irradiance_tables_kernel");
+  st = check_launch("irradiance_tables_kernel");
   if (st != ALTHEA_OK) return st;
   const dim3 grid((6 * size * size + 3) / 4);
-  const char* pyr4 = reinterpret_cast<const char*>(pyr);
+  const char* g4 = reinterpret_cast<const char*>(gather);
   float4* cube4 = reinterpret_cast<float4*>(cube);
   if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
-    hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+    hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, g4, theta_tab, phi_tab, p, cube4);
   else
-    hipLaunchKernelGGL(irradiance_cube_kernel<true>, grid, dim3(256), 0, s, pyr4, theta_tab, phi_tab, p, cube4);
+    hipLaunchKernelGGL(irradiance_cube_kernel<true>, grid, dim3(256), 0, s, g4, theta_tab, phi_tab, p, cube4);
   return check_launch("irradiance_cube_kernel");
 }
 

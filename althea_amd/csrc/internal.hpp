@@ -49,6 +49,23 @@ size_t cube_mip_offset(int size, int mip);
 
 struct GgxEntry;
 struct LevelC;
+struct GatherLevel;
+
+// Gather copy of pyramid levels first..last (each padded with its REPEAT
+// neighbours, (w + 2) x (h + 2) texels, concatenated): texel offset of each
+// level and the total.
+struct GatherGeometry {
+  int first, last;
+  uint32_t off[16];
+  size_t texels;
+};
+GatherGeometry gather_geometry(int w, int h, int first, int last);
+size_t gather_bytes(int w, int h, int first, int last);
+void gather_levels(int w, int h, const GatherGeometry& gg, GatherLevel* out);
+althea_status launch_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr, hipStream_t s);
+// Sample tables of a prefilter (the part of its workspace the allocation-free
+// entry point keeps in the library's fixed array).
+size_t prefilter_table_bytes(int mips, int samples);
 
 // Kernel launchers (ibl_kernels.hip).
 althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, GgxEntry* tables, int* counts,

@@ -23,14 +23,33 @@ static_assert(sizeof(GgxEntry) == 32, "one scalar load per entry");
 
 constexpr int kBlockSums = 32;  // FP32 partial sums flushed to FP64 every 32 samples
 
+// Source lookups of the sample loop: acc(l, yaw, pitch, w, r, g, b) adds w
+// times the bilinear value of level l.  GatherAcc reads the padded gather
+// copy, PyramidAcc the pyramid itself; both read the same texels with the
+// same coordinates and weights, so their sums are bit-identical.
+struct GatherAcc {
+  const char* __restrict__ gpyr;
+  const GatherLevel* __restrict__ lv;
+  __device__ __forceinline__ void operator()(int l, float yaw, float pitch, float w, float& r, float& g,
+                                             float& b) const {
+    gather_bilinear_acc(gpyr, lv[l], yaw, pitch, w, r, g, b);
+  }
+};
+struct PyramidAcc {
+  const char* __restrict__ pyr;
+  const LevelC* __restrict__ lv;
+  __device__ __forceinline__ void operator()(int l, float yaw, float pitch, float w, float& r, float& g,
+                                             float& b) const {
+    level_bilinear_acc(pyr, lv[l], yaw, pitch, w, r, g, b);
+  }
+};
+
 // NdotL-weighted mean of the source over the table (PreFilterEnvMap.comp:147-169):
 // the FP32 per-sample body (each level's bilinear in weight form, added into
 // the running sums), FP64 running sums (kF64Sums) or all-FP32 sums.
-// `level(l)` returns the LevelC of source level l.
-template <bool kF64Sums, class LevelOf>
-__device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, const LevelOf& level,
-                                                  const GgxEntry* __restrict__ tab, int count, double wsum,
-                                                  const Frame& fr) {
+template <bool kF64Sums, class Acc>
+__device__ __forceinline__ float4 prefilter_texel(const Acc& acc, const GgxEntry* __restrict__ tab, int count,
+                                                  double wsum, const Frame& fr) {
   double ar = 0.0, ag = 0.0, ab = 0.0;
   float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
   for (int e = 0; e < count; ++e) {
@@ -40,8 +59,8 @@ __device__ __forceinline__ float4 prefilter_texel(const char* __restrict__ pyr, 
                     fmaf(fr.t.z, en.lx, fmaf(fr.b.z, en.ly, fr.n.z * en.lz)));
     float yaw, pitch;
     equirect_angles(L, yaw, pitch);
-    level_bilinear_acc(pyr, level(en.l0), yaw, pitch, en.w0, sr, sg, sb);
-    if (__float_as_uint(en.w1) != 0u) level_bilinear_acc(pyr, level(en.l1), yaw, pitch, en.w1, sr, sg, sb);
+    acc(en.l0, yaw, pitch, en.w0, sr, sg, sb);
+    if (__float_as_uint(en.w1) != 0u) acc(en.l1, yaw, pitch, en.w1, sr, sg, sb);
     if (kF64Sums) {
       if ((e & (kBlockSums - 1)) == kBlockSums - 1) {
         ar += sr;

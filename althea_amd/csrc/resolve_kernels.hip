@@ -98,7 +98,7 @@ __device__ __forceinline__ f3 tonemap(f3 c, float exposure) {
 }
 
 __device__ __forceinline__ float smoothstep01(float x) {
-  float t = fminf(fmaxf(x, 0.0f), 1.0f);
+  const float t = saturate(x);
   return t * t * fmaf(-2.0f, t, 3.0f);
 }
 
@@ -160,10 +160,13 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
       const float cw = fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
       const float inv_w = rcp_fast(cw);
       const float sx = fmaf(cx, inv_w, hw), sy = fmaf(cy, inv_w, hh);  // pixels
-      ok[j] = k0 + j < n && cw > 0.0f && sx >= 0.0f && sx < fw && sy >= 0.0f && sy < fh;
-      const int qx = min(static_cast<int>(sx), p.width - 1);
-      const int qy = min(static_cast<int>(sy), p.height - 1);
-      idx[j] = ok[j] ? static_cast<uint32_t>(qy) * static_cast<uint32_t>(p.width) + static_cast<uint32_t>(qx) : 0u;
+      // floor(s) lies in [0, extent) exactly where s does, and equals the
+      // truncation there: one unsigned compare per axis is the bounds test.
+      const uint32_t qx = static_cast<uint32_t>(cvt_floor(sx)), qy = static_cast<uint32_t>(cvt_floor(sy));
+      ok[j] = (k0 + j < n) & (cw > 0.0f) & (qx < static_cast<uint32_t>(p.width)) &
+              (qy < static_cast<uint32_t>(p.height));
+      const uint32_t lin = qy * static_cast<uint32_t>(p.width) + qx;
+      idx[j] = ok[j] ? lin : 0u;
     }
     float qd[kSsaoBatch];
 #pragma unroll

@@ -156,8 +156,10 @@ __device__ __forceinline__ void pbr_lights(const PbrPoint& m, f3 P, f3 N, f3 I, 
       const float d2 = dot(L, L);
       const float inv_d = rsq_fast(d2);
       const float h2 = fmaf(dot(I2, L), inv_d, 2.0f);
-      const float n_dot_l = fmaxf(nl_raw * inv_d, 0.0f);
-      float n_dot_h = fmaxf(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2), 0.0f);
+      // max(., 0) of the reference, saturated: the cosines of unit vectors
+      // exceed 1 only by rounding, and the clamp costs no instruction.
+      const float n_dot_l = saturate(nl_raw * inv_d);
+      float n_dot_h = saturate(fmaf(nl_raw, inv_d, NI) * rsq_fast(h2));
       // Lanes above the horizon with |V + L|^2 below the refinement bound.
       if ((__builtin_amdgcn_ballot_w64(h2 < kHalfVecRefine) & lit) != 0) {
         if (h2 < kHalfVecRefine && nl_raw > 0.0f) n_dot_h = n_dot_h_refined(N, I, P, lp, L, inv_d, h2, view_df);

@@ -215,7 +215,7 @@ def test_irradiance_fp32_sums_variant(env_case, torch):
     assert err.max() < 1e-3, summary(err)
 
 
-@pytest.mark.parametrize("size,theta", [(1, 300), (5, 17), (32, 300)])
+@pytest.mark.parametrize("size,theta", [(1, 300), (5, 17), (8, 100), (32, 300)])
 def test_irradiance_edge_shapes(torch, size, theta):
     env = _env("smooth")
     pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
