@@ -31,6 +31,8 @@
  *     convention of Shaders/Global/GlobalResources.glsl:48-54).  Power-of-two
  *     extents.  Its pyramid holds althea_env_levels(w,h) levels
  *     (Src/Utilities.cpp:97-100) concatenated, level l of max(1,w>>l) x max(1,h>>l).
+ *     Sources have at most 2^27 texels (e.g. 16384 x 8192), so that every
+ *     level's byte offsets fit 32 bits.
  *   - Cube map: RGBA32F [mip][face][y][x], faces +X,-X,+Y,-Y,+Z,-Z with the
  *     Vulkan cube-face-selection table; mip m has faces of max(1, size>>m).
  *   - BRDF LUT: RG32F [roughness row][NdotV column], texel centres at (i+0.5)/size.
@@ -123,7 +125,10 @@ ALTHEA_API althea_status althea_env_sample(const float* d_pyramid, int width, in
  * mip per roughness m/(mips-1) as ImageBasedLighting.cpp:352-402 does with i/4):
  * `samples` Hammersley half vectors, alpha = roughness, NdotL-weighted mean,
  * source LOD from the sample pdf.  d_cube holds althea_cube_texels(size, mips)
- * float4.  Sums over samples are FP64 unless ALTHEA_PRECOMPUTE_FP32_SUMS. */
+ * float4.  Sums over samples are FP64 unless ALTHEA_PRECOMPUTE_FP32_SUMS.  The
+ * call's stream-ordered workspace holds the sample tables and a padded copy of
+ * the pyramid (each level with its wrap-around neighbours stored around it,
+ * about 1.34x the pyramid) that the sample loop reads. */
 ALTHEA_API althea_status althea_prefilter_cube(const float* d_pyramid, int env_width, int env_height,
                                                int size, int mips, int samples, unsigned flags,
                                                float* d_cube, hipStream_t stream);
@@ -140,8 +145,9 @@ ALTHEA_API size_t althea_prefilter_cube_scratch_bytes(int mips, int samples);
  * per-roughness GGX sample tables live in a fixed 4 MiB device array that the
  * library holds on every GPU it runs on until the process ends; requests whose
  * tables exceed ALTHEA_PREFILTER_SCRATCH_MAX_BYTES are rejected with
- * ALTHEA_ERROR_INVALID_ARGUMENT before any device work.  The cube written is
- * bit-identical to althea_prefilter_cube's.  Every call on one GPU shares that
+ * ALTHEA_ERROR_INVALID_ARGUMENT before any device work.  It reads the pyramid
+ * itself, where althea_prefilter_cube reads its padded copy; the cube written
+ * is bit-identical to althea_prefilter_cube's.  Every call on one GPU shares that
  * array: calls that overlap on one GPU (other streams, other threads) race. */
 ALTHEA_API althea_status althea_prefilter_cube_noalloc(const float* d_pyramid, int env_width, int env_height,
                                                        int size, int mips, int samples, unsigned flags,
@@ -155,7 +161,9 @@ ALTHEA_API althea_status althea_prefilter_points(const float* d_pyramid, int env
 
 /* Diffuse irradiance cube (GenIrradianceMap.comp:106-155 per output texel):
  * theta_samples x int(H*theta_samples/W) Riemann sum (300 in the reference)
- * at source LOD log2(W/theta_samples).  d_cube holds 6*size*size float4. */
+ * at source LOD log2(W/theta_samples).  d_cube holds 6*size*size float4.  The
+ * call's stream-ordered workspace holds the angle tables and a padded copy of
+ * the two source levels read. */
 ALTHEA_API althea_status althea_irradiance_cube(const float* d_pyramid, int env_width, int env_height,
                                                 int size, int theta_samples, unsigned flags, float* d_cube,
                                                 hipStream_t stream);
