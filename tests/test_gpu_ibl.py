@@ -196,6 +196,41 @@ def test_prefilter_points_agree_with_cube_texels(torch):
     assert err.max() < 2e-4, summary(err)
 
 
+@pytest.mark.parametrize("w,h", [(1, 1), (2, 1), (1, 4), (4, 16), (16, 4), (64, 8)])
+def test_gather_copy_edge_sources(torch, w, h):
+    """Sources down to one texel and of either aspect.  The prefilter and the
+    irradiance read a padded copy of each level (its REPEAT neighbours stored
+    around it); both match the oracle, and the allocation-free prefilter, which
+    reads the pyramid itself, writes the same bits."""
+    env = aa.synth.env(seed=SEED0 + 131 * w + h, width=w, height=h, sigma=0.5, suns=1, peak_min=10.0,
+                       peak_max=10.0, sun_sigma_deg=20.0)
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    oenv = ol.Env(env)
+    size, mips, samples = 8, 4, 64
+    n = aa.cube_texels(size, mips)
+    ref = torch.full((n, 4), float("nan"), dtype=torch.float32, device="cuda")
+    got = torch.full((n, 4), float("nan"), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    L = aa.lib()
+    print(f"gather edge source {w}x{h}", flush=True)
+    assert L.althea_prefilter_cube(pyr.data_ptr(), w, h, size, mips, samples, 0, ref.data_ptr(), stream) == 0, \
+        L.althea_last_error()
+    assert L.althea_prefilter_cube_noalloc(pyr.data_ptr(), w, h, size, mips, samples, 0, got.data_ptr(),
+                                           stream) == 0, L.althea_last_error()
+    torch.cuda.synchronize()
+    cube = ref.cpu().numpy()
+    np.testing.assert_array_equal(got.cpu().numpy().view(np.uint32), cube.view(np.uint32))
+    ids = cube_texel_ids(size, mips, per_face=4, seed=11)
+    err = rel_err(cube_gather(cube, size, ids), oenv.prefilter_texels(size, mips, samples, ids)[:, :3],
+                  RADIANCE_FLOOR)
+    assert err.max() < MAX_REL, summary(err)
+    theta = 16
+    irr = aa.irradiance_cube(pyr, w, h, 4, theta).cpu().numpy()
+    ids = cube_texel_ids(4, 1, per_face=3, seed=12)
+    err = rel_err(cube_gather(irr, 4, ids), oenv.irradiance_texels(4, ids[:, 1:], theta), RADIANCE_FLOOR)
+    assert err.max() < MAX_REL, summary(err)
+
+
 # -------------------------------------------------------------- irradiance
 def test_irradiance_cube_matches_oracle(env_case, torch):
     name, env, dev, pyr, oenv = env_case
