@@ -16,8 +16,10 @@
 // about 1e-13 rad.
 //
 // Host and device: the functions are __host__ __device__ and use only IEEE
-// FP32 operations (+, -, *, /, sqrtf, fmaf, floorf), so a host build gives
-// the same results as the kernels.
+// FP32 operations (+, -, *, /, sqrtf, fmaf, floorf) and a reciprocal (df_rcp:
+// v_rcp_f32 in device code, the division on the host), so a host build gives
+// the kernels' results up to the last bits of quotients, which df_div's
+// remainder correction holds to the same bound on both.
 #pragma once
 
 #include <math.h>
@@ -99,6 +101,28 @@ __host__ __device__ __forceinline__ df32 df_add(df32 a, df32 b) {
   return df_fast_two_sum(s.hi, s.lo);
 }
 __host__ __device__ __forceinline__ df32 df_sub(df32 a, df32 b) { return df_add(a, df_neg(b)); }
+// a + b with the low parts added in FP32 (Dekker's add): absolute error about
+// 2^-44 (|a| + |b|), so relative only to the operands, not to a + b.  For sums
+// whose result must be accurate on the scale of the operands (the components
+// of a direction, against its length), not under cancellation.
+__host__ __device__ __forceinline__ df32 df_add_fast(df32 a, df32 b) {
+  ALTHEA_DF32_NO_CONTRACT
+  df32 s = df_two_sum(a.hi, b.hi);
+  s.lo += a.lo + b.lo;
+  return df_fast_two_sum(s.hi, s.lo);
+}
+__host__ __device__ __forceinline__ df32 df_sub_fast(df32 a, df32 b) { return df_add_fast(a, df_neg(b)); }
+// 1 / x to within an ulp or two: v_rcp_f32 in device code, the division on
+// the host.  Quotients built on it are corrected by an exact remainder, so
+// host and device results meet the same bound and may differ in the last bits
+// of the low part.
+__host__ __device__ __forceinline__ float df_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
 __host__ __device__ __forceinline__ df32 df_add_f(df32 a, float b) {
   ALTHEA_DF32_NO_CONTRACT
   df32 s = df_two_sum(a.hi, b);
@@ -117,13 +141,16 @@ __host__ __device__ __forceinline__ df32 df_mul_f(df32 a, float b) {
   p.lo = fmaf(a.lo, b, p.lo);
   return df_fast_two_sum(p.hi, p.lo);
 }
-// a / b: an FP32 quotient corrected by the exact remainder.
+// a / b: an FP32 quotient from the reciprocal, corrected by the exact
+// remainder (the correction q2 is about 2^-23 of q1, so its own 2-ulp error is
+// about 2^-46 of the result).
 __host__ __device__ __forceinline__ df32 df_div(df32 a, df32 b) {
   ALTHEA_DF32_NO_CONTRACT
-  const float q1 = a.hi / b.hi;
+  const float rb = df_rcp(b.hi);
+  const float q1 = a.hi * rb;
   const df32 p = df_mul_f(b, q1);
   const df32 r = df_sub(a, p);
-  const float q2 = r.hi / b.hi;
+  const float q2 = r.hi * rb;
   return df_fast_two_sum(q1, q2);
 }
 // sqrt(a), a >= 0: one Newton correction of the FP32 root.
@@ -135,10 +162,10 @@ __host__ __device__ __forceinline__ df32 df_sqrt(df32 a) {
   const float r = ((a.hi - p.hi) - p.lo) + a.lo;
   return df_fast_two_sum(s, r / (2.0f * s));
 }
-// x^2 + z^2 of FP32 inputs (exact products, one df sum).
+// x^2 + z^2 (two non-negative terms: no cancellation, so the fast add).
 __host__ __device__ __forceinline__ df32 df_sum_squares(df32 x, df32 z) {
   ALTHEA_DF32_NO_CONTRACT
-  return df_add(df_mul(x, x), df_mul(z, z));
+  return df_add_fast(df_mul(x, x), df_mul(z, z));
 }
 
 constexpr df32 kDfPi = {3.141592741e+00f, -8.742277657e-08f};
@@ -156,7 +183,8 @@ __host__ __device__ __forceinline__ df32 df_atan_unit(df32 t) {
   const df32 u = df_div(num, den);
   const float p = u.hi * u.hi;
   const float tail = u.hi * p * fmaf(p, fmaf(p, -0.142857149f, 0.2f), -0.333333343f);
-  return df_add_f(df_add(df32{kAtanK64Hi[k], kAtanK64Lo[k]}, u), tail);
+  // atan(k/64) >= |u| for k >= 1 and 0 for k = 0: no cancellation.
+  return df_add_f(df_add_fast(df32{kAtanK64Hi[k], kAtanK64Lo[k]}, u), tail);
 }
 
 // atan2(y, x) in (-pi, pi].
@@ -166,8 +194,9 @@ __host__ __device__ __forceinline__ df32 df_atan2(df32 y, df32 x) {
   const bool swap = ay.hi > ax.hi;
   const df32 num = swap ? ax : ay, den = swap ? ay : ax;
   df32 r = den.hi > 0.0f ? df_atan_unit(df_div(num, den)) : df32{0.0f, 0.0f};
-  if (swap) r = df_sub(kDfHalfPi, r);
-  if (x.hi < 0.0f) r = df_sub(kDfPi, r);
+  // r <= pi/4 before the first reflection and <= pi/2 before the second.
+  if (swap) r = df_sub_fast(kDfHalfPi, r);
+  if (x.hi < 0.0f) r = df_sub_fast(kDfPi, r);
   return y.hi < 0.0f ? df_neg(r) : r;
 }
 

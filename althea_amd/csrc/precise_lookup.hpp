@@ -125,30 +125,35 @@ __device__ __forceinline__ f3 cube_bilinear_df(const float4* __restrict__ mip, i
 // the normal n (any positive scale), up to a positive scale:
 // reflect(v/|v|, n/|n|) |v| |n|^2 = v (n.n) - 2 (n.v) n.  Cube lookups depend
 // only on the direction, so no normalisation is needed.
+// The sums are accurate on the scale of |v| |n|^2 (df_add_fast), which is
+// what a direction needs.
 __device__ __forceinline__ df3 reflect_df(const df3& v, f3 n) {
-  const df32 nn = df_add(df_add(df_two_prod(n.x, n.x), df_two_prod(n.y, n.y)), df_two_prod(n.z, n.z));
-  const df32 nv = df_add(df_add(df_mul_f(v.x, n.x), df_mul_f(v.y, n.y)), df_mul_f(v.z, n.z));
+  const df32 nn =
+      df_add_fast(df_add_fast(df_two_prod(n.x, n.x), df_two_prod(n.y, n.y)), df_two_prod(n.z, n.z));
+  const df32 nv = df_add_fast(df_add_fast(df_mul_f(v.x, n.x), df_mul_f(v.y, n.y)), df_mul_f(v.z, n.z));
   const df32 two_nv = {2.0f * nv.hi, 2.0f * nv.lo};
-  return {df_sub(df_mul(v.x, nn), df_mul_f(two_nv, n.x)), df_sub(df_mul(v.y, nn), df_mul_f(two_nv, n.y)),
-          df_sub(df_mul(v.z, nn), df_mul_f(two_nv, n.z))};
+  return {df_sub_fast(df_mul(v.x, nn), df_mul_f(two_nv, n.x)), df_sub_fast(df_mul(v.y, nn), df_mul_f(two_nv, n.y)),
+          df_sub_fast(df_mul(v.z, nn), df_mul_f(two_nv, n.z))};
 }
 
 // The view ray of pixel centre (cx, cy) from df32 affine coefficients:
-// ray0 + rayx cx + rayy cy (DeferredPass.vert:10-22 is linear in NDC).
+// ray0 + rayx cx + rayy cy (DeferredPass.vert:10-22 is linear in NDC), each
+// component accurate on the scale of the terms (df_add_fast).
 struct ViewRayDf {
   float ray0[3][2], rayx[3][2], rayy[3][2];  // (hi, lo) per component
 };
 __device__ __forceinline__ df3 view_ray_df(const ViewRayDf& r, float cx, float cy) {
   auto comp = [&](int k) {
-    return df_add(df_add(df32{r.ray0[k][0], r.ray0[k][1]}, df_mul_f(df32{r.rayx[k][0], r.rayx[k][1]}, cx)),
-                  df_mul_f(df32{r.rayy[k][0], r.rayy[k][1]}, cy));
+    return df_add_fast(
+        df_add_fast(df32{r.ray0[k][0], r.ray0[k][1]}, df_mul_f(df32{r.rayx[k][0], r.rayx[k][1]}, cx)),
+        df_mul_f(df32{r.rayy[k][0], r.rayy[k][1]}, cy));
   };
   return {comp(0), comp(1), comp(2)};
 }
 
 // Unit vector of a df32 vector.
 __device__ __forceinline__ df3 normalize_df(const df3& v) {
-  const df32 n2 = df_add(df_add(df_mul(v.x, v.x), df_mul(v.y, v.y)), df_mul(v.z, v.z));
+  const df32 n2 = df_add_fast(df_add_fast(df_mul(v.x, v.x), df_mul(v.y, v.y)), df_mul(v.z, v.z));
   const df32 inv = df_div(df_of(1.0f), df_sqrt(n2));
   return {df_mul(v.x, inv), df_mul(v.y, inv), df_mul(v.z, inv)};
 }

@@ -1,6 +1,8 @@
 // Host build of althea_amd/csrc/df32.hpp for tests/test_df32.py: the header's
-// functions are __host__ __device__ and use only IEEE FP32 operations, so the
-// host results are the kernels' results.
+// functions are __host__ __device__ and use only IEEE FP32 operations; the
+// host results are the kernels' results except where df_rcp's hardware
+// reciprocal (device) and division (host) differ, which df_div's remainder
+// correction brings to the same bound.
 #define __host__
 #define __device__
 #define __forceinline__ inline
