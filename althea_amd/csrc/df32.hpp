@@ -141,16 +141,19 @@ __host__ __device__ __forceinline__ df32 df_mul_f(df32 a, float b) {
   p.lo = fmaf(a.lo, b, p.lo);
   return df_fast_two_sum(p.hi, p.lo);
 }
-// a / b: an FP32 quotient from the reciprocal, corrected by the exact
-// remainder (the correction q2 is about 2^-23 of q1, so its own 2-ulp error is
-// about 2^-46 of the result).
+// a / b: an FP32 quotient q1 from the reciprocal, corrected by the
+// remainder r = a - q1 b.  q1 b.hi is taken exactly (two_prod); q1 is within
+// a few ulps of a.hi / b.hi, so a.hi - (q1 b.hi).hi is exact (Sterbenz) and
+// the rest of r is a few FP32 roundings relative to r itself.  The correction
+// q2 = r / b.hi is about 2^-23 of q1, so its own error is about 2^-46 of the
+// result.
 __host__ __device__ __forceinline__ df32 df_div(df32 a, df32 b) {
   ALTHEA_DF32_NO_CONTRACT
   const float rb = df_rcp(b.hi);
   const float q1 = a.hi * rb;
-  const df32 p = df_mul_f(b, q1);
-  const df32 r = df_sub(a, p);
-  const float q2 = r.hi * rb;
+  const df32 p = df_two_prod(q1, b.hi);
+  const float r = (((a.hi - p.hi) - p.lo) + a.lo) - q1 * b.lo;
+  const float q2 = r * rb;
   return df_fast_two_sum(q1, q2);
 }
 // sqrt(a), a >= 0: one Newton correction of the FP32 root.
@@ -160,7 +163,7 @@ __host__ __device__ __forceinline__ df32 df_sqrt(df32 a) {
   const float s = sqrtf(a.hi);
   const df32 p = df_two_prod(s, s);
   const float r = ((a.hi - p.hi) - p.lo) + a.lo;
-  return df_fast_two_sum(s, r / (2.0f * s));
+  return df_fast_two_sum(s, r * df_rcp(2.0f * s));
 }
 // x^2 + z^2 (two non-negative terms: no cancellation, so the fast add).
 __host__ __device__ __forceinline__ df32 df_sum_squares(df32 x, df32 z) {
@@ -211,6 +214,8 @@ struct TexelSplit {
 __host__ __device__ __forceinline__ TexelSplit df_texel(df32 x) {
   ALTHEA_DF32_NO_CONTRACT
   float f = floorf(x.hi);
+  // x.hi - floor(x.hi) is not always exact (x.hi in (-1, 0) has finer bits
+  // than x.hi + 1 can hold), so it is a two_sum.
   df32 a = df_add_f(df_two_sum(x.hi, -f), x.lo);  // x - floor(x.hi), in (-1, 2)
   // The fraction is the df value: 1 - 1e-9 has hi = 1 and lo < 0.
   if (a.hi < 0.0f || (a.hi == 0.0f && a.lo < 0.0f)) {

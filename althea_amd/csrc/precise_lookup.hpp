@@ -80,7 +80,7 @@ __device__ __forceinline__ f3 clamp_bilinear_df(const float4* __restrict__ level
                    xyz(level[y1 * w + x1]), sx, sy);
 }
 
-// Seamless bilinear fetch of one cube mip (faces of size S) in direction R:
+// Seamless bilinear fetch of one cube mip (faces of size S, a power of two) in direction R:
 // face by the major axis of R's leading parts (the Vulkan table of
 // cube_select), (sc / ma, tc / ma) and the texel coordinates in df32.
 __device__ __forceinline__ f3 cube_bilinear_df(const float4* __restrict__ mip, int S, const df3& R) {
@@ -100,9 +100,12 @@ __device__ __forceinline__ f3 cube_bilinear_df(const float4* __restrict__ mip, i
     if (R.z.hi >= 0.0f) { face = 4; sc = R.x; tc = df_neg(R.y); }
     else { face = 5; sc = df_neg(R.x); tc = df_neg(R.y); }
   }
+  // S is a power of two (the prefiltered cube's sizes are), so scaling by S / 2
+  // is exact on both parts.
   const float half = 0.5f * static_cast<float>(S);
-  const df32 x = df_add_f(df_mul_f(df_div(sc, ma), half), half - 0.5f);
-  const df32 y = df_add_f(df_mul_f(df_div(tc, ma), half), half - 0.5f);
+  const df32 u = df_div(sc, ma), v = df_div(tc, ma);
+  const df32 x = df_add_f(df32{u.hi * half, u.lo * half}, half - 0.5f);
+  const df32 y = df_add_f(df32{v.hi * half, v.lo * half}, half - 0.5f);
   const TexelSplit sx = df_texel(x), sy = df_texel(y);
   const int i0 = sx.i, j0 = sy.i;
   f3 t00, t10, t01, t11;

@@ -6,6 +6,18 @@
 #define __host__
 #define __device__
 #define __forceinline__ inline
+#ifdef DF32_DEVICE_RCP
+// The device code path: df_rcp takes the hardware reciprocal, accurate to
+// 1 ulp.  Modelled as the correctly rounded 1 / x moved one ulp down or up
+// (alternately), so every quotient starts from a wrong last bit.
+#include <math.h>
+#define __HIP_DEVICE_COMPILE__ 1
+static unsigned g_rcp_flip = 0;
+static inline float __builtin_amdgcn_rcpf(float x) {
+  const float r = 1.0f / x;
+  return nextafterf(r, (g_rcp_flip++ & 1u) ? INFINITY : -INFINITY);
+}
+#endif
 #include "../../althea_amd/csrc/df32.hpp"
 
 using namespace althea;

@@ -1,5 +1,6 @@
 """Double-float arithmetic of althea_amd/csrc/df32.hpp (the kernels' precise
-addressing), built for the host and checked against FP64: atan2 to about
+addressing), built for the host (once as the host path, once as the device
+path with a reciprocal one ulp off) and checked against FP64: atan2 to about
 1e-13 rad over every octant, division and square root to ~2^-44 relative,
 and the texel split (floor plus weights a, b = 1 - a, each one FP32 rounding
 of the exact fraction) including coordinates just below and above integers."""
@@ -14,10 +15,13 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 
 
-@pytest.fixture(scope="module")
-def lib(tmp_path_factory):
-    so = tmp_path_factory.mktemp("df32") / "libdf32_host.so"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+@pytest.fixture(scope="module", params=["host", "device_rcp"])
+def lib(request, tmp_path_factory):
+    """The header built for the host (df_rcp = 1 / x) and with the device code
+    path (df_rcp = the hardware reciprocal, modelled as 1 / x one ulp off)."""
+    so = tmp_path_factory.mktemp("df32") / f"libdf32_{request.param}.so"
+    defs = ["-DDF32_DEVICE_RCP"] if request.param == "device_rcp" else []
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", *defs,
                     str(ROOT / "tests" / "cpp" / "df32_host.cpp"), "-o", str(so)], check=True, capture_output=True)
     L = C.CDLL(str(so))
     vp = C.c_void_p
