@@ -156,6 +156,15 @@ def cube_sample(cube, size: int, mips: int, dir_lod, stream=None):
     return out
 
 
+def cube_border(cube, size: int, mips: int, stream=None):
+    """Seamless-bordered copy of a cube (althea_cube_border): float32
+    (althea_cube_bordered_texels(size, mips), 4)."""
+    torch = _torch()
+    out = _empty(stream, (lib().althea_cube_bordered_texels(size, mips), 4), dtype=torch.float32, device=cube.device)
+    check(lib().althea_cube_border(_ptr(cube), size, mips, _ptr(out), _stream(stream)), "althea_cube_border")
+    return out
+
+
 def env_sample(pyramid, env_w: int, env_h: int, dir_lod, stream=None):
     torch = _torch()
     out = _empty(stream, (dir_lod.shape[0], 4), dtype=torch.float32, device=pyramid.device)

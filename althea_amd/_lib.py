@@ -34,7 +34,8 @@ class IBLDescC(C.Structure):
 class IBLViewsC(C.Structure):
     _fields_ = [("environment", vp), ("env_width", i32), ("env_height", i32), ("env_levels", i32),
                 ("prefiltered", vp), ("prefiltered_size", i32), ("prefiltered_mips", i32),
-                ("irradiance", vp), ("irradiance_size", i32), ("brdf_lut", vp), ("brdf_lut_size", i32)]
+                ("irradiance", vp), ("irradiance_size", i32), ("brdf_lut", vp), ("brdf_lut_size", i32),
+                ("prefiltered_bordered", vp), ("irradiance_bordered", vp)]
 
 
 class GBufferViewC(C.Structure):
@@ -125,6 +126,8 @@ PROTOTYPES = {
     "althea_env_pyramid_texels": (sz, [i32, i32]),
     "althea_env_pyramid_build": (i32, [vp, i32, i32, vp, vp]),
     "althea_cube_texels": (sz, [i32, i32]),
+    "althea_cube_bordered_texels": (sz, [i32, i32]),
+    "althea_cube_border": (i32, [vp, i32, i32, vp, vp]),
     "althea_cube_sample": (i32, [vp, i32, i32, vp, i32, vp, vp]),
     "althea_env_sample": (i32, [vp, i32, i32, vp, i32, vp, vp]),
     "althea_prefilter_cube": (i32, [vp, i32, i32, i32, i32, i32, C.c_uint, vp, vp]),

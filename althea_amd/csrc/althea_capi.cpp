@@ -103,8 +103,10 @@ class Workspace {
   hipError_t e_ = hipSuccess;
 };
 
+// Power-of-two faces up to 16384 (texel offsets of a cube and of its bordered
+// copy fit 32 bits).
 static bool cube_ok(int size, int mips) {
-  return is_pow2(size) && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1;
+  return is_pow2(size) && size <= 16384 && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1;
 }
 
 // Power-of-two extents up to 2^15, at most 2^27 texels (16384 x 8192): every
@@ -127,6 +129,10 @@ struct althea_ibl {
   float* prefiltered = nullptr;
   float* irradiance = nullptr;
   float* lut = nullptr;
+  // Seamless-bordered copies of the two cubes (althea_cube_border), rewritten
+  // after every precompute or cache load; the resolve and the SSR capture read them.
+  float* prefiltered_bordered = nullptr;
+  float* irradiance_bordered = nullptr;
   void* workspace = nullptr;
   size_t workspace_bytes = 0;
 };
@@ -214,6 +220,18 @@ althea_status althea_env_pyramid_build(const float* d_level0, int width, int hei
 size_t althea_cube_texels(int size, int mips) {
   if (size <= 0 || mips <= 0) return 0;
   return cube_mip_offset(size, mips);
+}
+
+size_t althea_cube_bordered_texels(int size, int mips) {
+  if (size < 1 || mips < 1 || mips > 16) return 0;
+  return cube_bordered_texels(size, mips);
+}
+
+althea_status althea_cube_border(const float* d_cube, int size, int mips, float* d_bordered, hipStream_t stream) {
+  ALTHEA_REQUIRE(d_cube && d_bordered, "cube_border: null buffer");
+  ALTHEA_REQUIRE(size >= 1 && size <= 16384 && mips >= 1 && mips <= 16 && (size >> (mips - 1)) >= 1,
+                 "cube_border: size %d / mips %d unsupported", size, mips);
+  return launch_cube_border(d_cube, size, mips, d_bordered, stream);
 }
 
 althea_status althea_cube_sample(const float* d_cube, int size, int mips, const float* d_dir_lod, int count,
@@ -332,8 +350,11 @@ althea_status althea_ibl_create(const althea_ibl_desc* desc, althea_ibl** out) {
   size_t pref = sizeof(float) * 4 * althea_cube_texels(desc->prefiltered_size, desc->prefiltered_mips);
   size_t irr = sizeof(float) * 4 * 6 * static_cast<size_t>(desc->irradiance_size) * desc->irradiance_size;
   size_t lut = sizeof(float) * 2 * static_cast<size_t>(desc->brdf_lut_size) * desc->brdf_lut_size;
+  size_t pref_b = sizeof(float) * 4 * cube_bordered_texels(desc->prefiltered_size, desc->prefiltered_mips);
+  size_t irr_b = sizeof(float) * 4 * cube_bordered_texels(desc->irradiance_size, 1);
   if (hipMalloc(&ibl->prefiltered, pref) != hipSuccess || hipMalloc(&ibl->irradiance, irr) != hipSuccess ||
-      hipMalloc(&ibl->lut, lut) != hipSuccess) {
+      hipMalloc(&ibl->lut, lut) != hipSuccess || hipMalloc(&ibl->prefiltered_bordered, pref_b) != hipSuccess ||
+      hipMalloc(&ibl->irradiance_bordered, irr_b) != hipSuccess) {
     set_error("ibl_create: device allocation failed");
     althea_ibl_destroy(ibl);
     return ALTHEA_ERROR_OUT_OF_MEMORY;
@@ -348,12 +369,23 @@ althea_status althea_ibl_destroy(althea_ibl* ibl) {
   (void)hipFree(ibl->prefiltered);
   (void)hipFree(ibl->irradiance);
   (void)hipFree(ibl->lut);
+  (void)hipFree(ibl->prefiltered_bordered);
+  (void)hipFree(ibl->irradiance_bordered);
   (void)hipFree(ibl->workspace);
   delete ibl;
   return ALTHEA_OK;
 }
 
 }  // extern "C"
+
+// The handle's bordered cubes from its prefiltered and irradiance cubes.
+static althea_status ibl_border(althea_ibl* ibl, hipStream_t stream) {
+  const althea_ibl_desc& d = ibl->desc;
+  althea_status st =
+      launch_cube_border(ibl->prefiltered, d.prefiltered_size, d.prefiltered_mips, ibl->prefiltered_bordered, stream);
+  if (st != ALTHEA_OK) return st;
+  return launch_cube_border(ibl->irradiance, d.irradiance_size, 1, ibl->irradiance_bordered, stream);
+}
 
 // Sizes the handle's pyramid and workspace for a width x height source.
 static althea_status ibl_prepare(althea_ibl* ibl, int width, int height, hipStream_t stream) {
@@ -401,6 +433,8 @@ althea_status althea_ibl_precompute(althea_ibl* ibl, const float* d_equirect, in
   st = launch_prefilter_cube(ibl->env, width, height, d.prefiltered_size, d.prefiltered_mips, d.prefilter_samples,
                              d.flags, ibl->prefiltered, ibl->workspace, stream);
   if (st != ALTHEA_OK) return st;
+  st = ibl_border(ibl, stream);
+  if (st != ALTHEA_OK) return st;
   return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
 }
 
@@ -439,6 +473,8 @@ althea_status althea_ibl_precompute_batched(althea_ibl* const* ibls, const althe
     if (st != ALTHEA_OK) return st;
     st = launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibls[i]->lut, stream);
     if (st != ALTHEA_OK) return st;
+    st = ibl_border(ibls[i], stream);
+    if (st != ALTHEA_OK) return st;
   }
   return ALTHEA_OK;
 }
@@ -456,6 +492,8 @@ althea_status althea_ibl_get_views(const althea_ibl* ibl, althea_ibl_views* v) {
   v->irradiance_size = ibl->desc.irradiance_size;
   v->brdf_lut = ibl->lut;
   v->brdf_lut_size = ibl->desc.brdf_lut_size;
+  v->prefiltered_bordered = ibl->prefiltered_bordered;
+  v->irradiance_bordered = ibl->irradiance_bordered;
   return ALTHEA_OK;
 }
 
@@ -587,6 +625,8 @@ althea_status althea_ibl_load_cache(althea_ibl* ibl, const float* d_equirect, in
     if (st != ALTHEA_OK) return st;
   }
   st = load(ibl->irradiance, d.irradiance_size, -1);
+  if (st != ALTHEA_OK) return st;
+  st = ibl_border(ibl, stream);
   if (st != ALTHEA_OK) return st;
   return launch_brdf_lut(d.brdf_lut_size, d.brdf_lut_samples, ibl->lut, stream);
 }

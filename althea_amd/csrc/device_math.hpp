@@ -449,6 +449,35 @@ struct CubeMips {
   uint32_t off[kMaxLevels];  // texel offset of each mip
 };
 
+// Seamless-bordered copy of a cube: mip m (faces of S = max(1, size >> m))
+// stored as 6 faces of (S + 2) x (S + 2) texels, bordered (r, c) holding
+// cube_tap(face, c - 1, r - 1): the face's own texel inside, beyond an edge
+// the neighbouring face's edge texel, at a corner the mean of the three
+// texels meeting there.  Those are exactly the taps cube_bilinear reads, so a
+// lookup becomes four plain loads with the same result.  off[m] is in texels.
+struct BorderedCube {
+  int size, mips;
+  uint32_t off[kMaxLevels];
+};
+
+// cube_bilinear on a bordered mip (`mip` at that mip's first texel).  The
+// projected coordinates of cube_select satisfy |pc|, |pt| <= 1 (up to the
+// reciprocal's rounding), so floor(x), floor(y) lie in [-1, S - 1] and the
+// taps (floor + 1, floor + 2) stay inside the bordered face; a NaN direction
+// converts to index 0.
+__device__ __forceinline__ f3 bcube_bilinear(const float4* __restrict__ mip, int S, int face, float pc, float pt) {
+  const float half = 0.5f * static_cast<float>(S);
+  const float x = fmaf(pc, half, half - 0.5f), y = fmaf(pt, half, half - 0.5f);
+  const float fx = floorf(x), fy = floorf(y);
+  const float ax = x - fx, ay = y - fy;
+  const int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+  const int W = S + 2;
+  const float4* r0 = mip + (face * W + j0 + 1) * W + (i0 + 1);
+  const f3 t00 = xyz(r0[0]), t10 = xyz(r0[1]), t01 = xyz(r0[W]), t11 = xyz(r0[W + 1]);
+  const f3 top = lerp3(t00, t10, ax), bot = lerp3(t01, t11, ax);
+  return lerp3(top, bot, ay);
+}
+
 __device__ inline f3 cube_sample(const float4* __restrict__ cube, const CubeMips& cm, f3 dir, float lod) {
   int face;
   float pc, pt;
@@ -461,6 +490,22 @@ __device__ inline f3 cube_sample(const float4* __restrict__ cube, const CubeMips
   f3 a = cube_bilinear(cube + cm.off[l0], max(1, cm.size >> l0), face, pc, pt);
   if (delta == 0.0f) return a;
   f3 b = cube_bilinear(cube + cm.off[l1], max(1, cm.size >> l1), face, pc, pt);
+  return lerp3(a, b, delta);
+}
+
+// cube_sample on a bordered copy: the same trilinear value.
+__device__ inline f3 bcube_sample(const float4* __restrict__ cube, const BorderedCube& cm, f3 dir, float lod) {
+  int face;
+  float pc, pt;
+  cube_select(dir, face, pc, pt);
+  float d = fminf(fmaxf(lod, 0.0f), static_cast<float>(cm.mips - 1));
+  float fl = floorf(d);
+  int l0 = static_cast<int>(fl);
+  int l1 = min(l0 + 1, cm.mips - 1);
+  float delta = d - fl;
+  f3 a = bcube_bilinear(cube + cm.off[l0], max(1, cm.size >> l0), face, pc, pt);
+  if (delta == 0.0f) return a;
+  f3 b = bcube_bilinear(cube + cm.off[l1], max(1, cm.size >> l1), face, pc, pt);
   return lerp3(a, b, delta);
 }
 

@@ -846,6 +846,56 @@ althea_status launch_cube_sample(const float* cube, int size, int mips, const fl
   return check_launch("cube_sample_kernel");
 }
 
+// --------------------------------------------------- seamless-bordered cubes
+BorderedCube bordered_cube_of(int size, int mips) {
+  BorderedCube c{};
+  c.size = size;
+  c.mips = mips;
+  size_t off = 0;
+  for (int m = 0; m < mips; ++m) {
+    c.off[m] = static_cast<uint32_t>(off);
+    const size_t w = static_cast<size_t>(std::max(1, size >> m)) + 2;
+    off += 6 * w * w;
+  }
+  return c;
+}
+
+size_t cube_bordered_texels(int size, int mips) {
+  size_t n = 0;
+  for (int m = 0; m < mips; ++m) {
+    const size_t w = static_cast<size_t>(std::max(1, size >> m)) + 2;
+    n += 6 * w * w;
+  }
+  return n;
+}
+
+// One thread per bordered texel: bordered (r, c) of face f at mip m is
+// cube_tap(f, c - 1, r - 1), the tap a seamless bilinear lookup reads there.
+__global__ __launch_bounds__(256) void cube_border_kernel(const float4* __restrict__ cube, const CubeMips cm,
+                                                          const BorderedCube bc, uint32_t total,
+                                                          float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= total) return;
+  int m = 0;
+  while (m + 1 < bc.mips && i >= bc.off[m + 1]) ++m;
+  const int S = max(1, bc.size >> m), W = S + 2;
+  const uint32_t k = i - bc.off[m];
+  const int face = static_cast<int>(k / static_cast<uint32_t>(W * W));
+  const int rem = static_cast<int>(k) - face * W * W;
+  const int r = rem / W, c = rem - r * W;
+  const f3 v = cube_tap(cube + cm.off[m], S, face, c - 1, r - 1);
+  out[i] = make_float4(v.x, v.y, v.z, 1.0f);
+}
+
+althea_status launch_cube_border(const float* cube, int size, int mips, float* bordered, hipStream_t s) {
+  const CubeMips cm = cube_mips_of(size, mips);
+  const BorderedCube bc = bordered_cube_of(size, mips);
+  const uint32_t total = static_cast<uint32_t>(cube_bordered_texels(size, mips));
+  hipLaunchKernelGGL(cube_border_kernel, dim3((total + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(cube), cm, bc, total, reinterpret_cast<float4*>(bordered));
+  return check_launch("cube_border_kernel");
+}
+
 struct EnvSampleParams {
   LevelC lv[kMaxLevels];
   int levels;

@@ -47,6 +47,13 @@ PyramidGeometry pyramid_geometry(int width, int height);
 // Cube mip offsets (texels).
 size_t cube_mip_offset(int size, int mip);
 
+// Seamless-bordered copies of cubes (BorderedCube in device_math.hpp): mip
+// offsets, total texels, and the kernel that writes one from a cube.
+struct BorderedCube;
+BorderedCube bordered_cube_of(int size, int mips);
+size_t cube_bordered_texels(int size, int mips);
+althea_status launch_cube_border(const float* cube, int size, int mips, float* bordered, hipStream_t s);
+
 struct GgxEntry;
 struct LevelC;
 struct GatherLevel;

@@ -80,7 +80,8 @@ __device__ __forceinline__ f3 clamp_bilinear_df(const float4* __restrict__ level
                    xyz(level[y1 * w + x1]), sx, sy);
 }
 
-// Seamless bilinear fetch of one cube mip (faces of size S, a power of two) in direction R:
+// Seamless bilinear fetch of one bordered cube mip (faces of size S, a power
+// of two, stored with their seamless border as BorderedCube) in direction R:
 // face by the major axis of R's leading parts (the Vulkan table of
 // cube_select), (sc / ma, tc / ma) and the texel coordinates in df32.
 __device__ __forceinline__ f3 cube_bilinear_df(const float4* __restrict__ mip, int S, const df3& R) {
@@ -106,22 +107,12 @@ __device__ __forceinline__ f3 cube_bilinear_df(const float4* __restrict__ mip, i
   const df32 u = df_div(sc, ma), v = df_div(tc, ma);
   const df32 x = df_add_f(df32{u.hi * half, u.lo * half}, half - 0.5f);
   const df32 y = df_add_f(df32{v.hi * half, v.lo * half}, half - 0.5f);
+  // |sc|, |tc| <= ma, so floor(x), floor(y) lie in [-1, S - 1]: the taps are
+  // inside the bordered face (BorderedCube), four plain loads.
   const TexelSplit sx = df_texel(x), sy = df_texel(y);
-  const int i0 = sx.i, j0 = sy.i;
-  f3 t00, t10, t01, t11;
-  if (i0 >= 0 && j0 >= 0 && i0 + 1 < S && j0 + 1 < S) {
-    const float4* r0 = mip + (face * S + j0) * S + i0;
-    t00 = xyz(r0[0]);
-    t10 = xyz(r0[1]);
-    t01 = xyz(r0[S]);
-    t11 = xyz(r0[S + 1]);
-  } else {
-    t00 = cube_tap(mip, S, face, i0, j0);
-    t10 = cube_tap(mip, S, face, i0 + 1, j0);
-    t01 = cube_tap(mip, S, face, i0, j0 + 1);
-    t11 = cube_tap(mip, S, face, i0 + 1, j0 + 1);
-  }
-  return weighted4(t00, t10, t01, t11, sx, sy);
+  const int W = S + 2;
+  const float4* r0 = mip + (face * W + sy.i + 1) * W + (sx.i + 1);
+  return weighted4(xyz(r0[0]), xyz(r0[1]), xyz(r0[W]), xyz(r0[W + 1]), sx, sy);
 }
 
 // Reflection of the view ray v (camera -> surface, any positive scale) about

@@ -40,9 +40,9 @@ struct ResolveParams {
   int width, height;
   const float4* env;  // equirect level 0
   int env_w, env_h;
-  const float4* prefiltered;
-  CubeMips pref;
-  const float4* irradiance;
+  const float4* prefiltered;  // seamless-bordered copy (BorderedCube)
+  BorderedCube pref;
+  const float4* irradiance;  // seamless-bordered copy, one mip
   int irr_size;
   const float2* lut;
   int lut_size;
@@ -277,17 +277,17 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
         // The mip-0 tap mirrors the source's texel contrast: reflection and
         // face coordinates in df32, the blend with mip 1 in weight form.
         const df3 Rd = reflect_df(view_ray_df(p.ray_df, cxp, cyp), mk(nrm.x, nrm.y, nrm.z));
-        refl = cube_bilinear_df(p.prefiltered, p.pref.size, Rd);
+        refl = cube_bilinear_df(p.prefiltered + p.pref.off[0], p.pref.size, Rd);
         if (d > 0.0f) {
           int face;
           float pc, pt;
           cube_select(R, face, pc, pt);
-          const f3 b = cube_bilinear(p.prefiltered + p.pref.off[1], max(1, p.pref.size >> 1), face, pc, pt);
+          const f3 b = bcube_bilinear(p.prefiltered + p.pref.off[1], max(1, p.pref.size >> 1), face, pc, pt);
           const float w0 = 1.0f - d;
           refl = mk(fmaf(b.x, d, refl.x * w0), fmaf(b.y, d, refl.y * w0), fmaf(b.z, d, refl.z * w0));
         }
       } else {
-        refl = cube_sample(p.prefiltered, p.pref, R, lod);
+        refl = bcube_sample(p.prefiltered, p.pref, R, lod);
       }
     }
     if (kRefl) {
@@ -303,7 +303,7 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     int face;
     float pc, pt;
     cube_select(N, face, pc, pt);
-    const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
+    const f3 irr = bcube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
     if (p.ssao_on) ao = ssao<kFmt>(p, s_noise, px, py, P, N);
     if (p.ao_out) p.ao_out[idx] = ao;
     float A, B;
@@ -483,12 +483,9 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   p.env = reinterpret_cast<const float4*>(ibl->environment);
   p.env_w = ibl->env_width;
   p.env_h = ibl->env_height;
-  p.prefiltered = reinterpret_cast<const float4*>(ibl->prefiltered);
-  p.pref.size = ibl->prefiltered_size;
-  p.pref.mips = ibl->prefiltered_mips;
-  for (int m = 0; m < ibl->prefiltered_mips; ++m)
-    p.pref.off[m] = static_cast<uint32_t>(cube_mip_offset(ibl->prefiltered_size, m));
-  p.irradiance = reinterpret_cast<const float4*>(ibl->irradiance);
+  p.prefiltered = reinterpret_cast<const float4*>(ibl->prefiltered_bordered);
+  p.pref = bordered_cube_of(ibl->prefiltered_size, ibl->prefiltered_mips);
+  p.irradiance = reinterpret_cast<const float4*>(ibl->irradiance_bordered);
   p.irr_size = ibl->irradiance_size;
   p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
   p.lut_size = ibl->brdf_lut_size;
@@ -511,26 +508,46 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   }
   p.out = out;
   p.ao_out = ao_out;
-  if (!p.ssao_on)
-    return launch_resolve_with(p, g, lights, flags, mlp, mlp_used, s, reflection, shadow_cubes, shadow_size);
+  // One stream-ordered allocation for what the call must build: the SSAO
+  // depth plane (SSAO on) and the bordered cubes (views without them).
   const size_t pixels = static_cast<size_t>(g->width) * g->height;
-  float* plane = nullptr;
-  const hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&plane), pixels * sizeof(float), s);
+  const bool border = !p.prefiltered || !p.irradiance;
+  const size_t plane_bytes = p.ssao_on ? (pixels * sizeof(float) + 255) & ~static_cast<size_t>(255) : 0;
+  const size_t pref_bytes =
+      border ? 16 * cube_bordered_texels(ibl->prefiltered_size, ibl->prefiltered_mips) : 0;
+  const size_t irr_bytes = border ? 16 * cube_bordered_texels(ibl->irradiance_size, 1) : 0;
+  const size_t bytes = plane_bytes + ((pref_bytes + 255) & ~static_cast<size_t>(255)) + irr_bytes;
+  if (bytes == 0)
+    return launch_resolve_with(p, g, lights, flags, mlp, mlp_used, s, reflection, shadow_cubes, shadow_size);
+  char* mem = nullptr;
+  const hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&mem), bytes, s);
   if (e != hipSuccess) {
-    set_error("resolve: hipMallocAsync of the %zu-byte SSAO depth plane failed: %s", pixels * sizeof(float),
+    set_error("resolve: hipMallocAsync of %zu bytes (SSAO depth plane, bordered cubes) failed: %s", bytes,
               hipGetErrorString(e));
     return ALTHEA_ERROR_HIP;
   }
-  p.ssao_depth = plane;
-  const dim3 dgrid(static_cast<unsigned>((pixels + 255) / 256));
-  if (g->format == ALTHEA_GBUFFER_FP32)
-    hipLaunchKernelGGL(ssao_depth_kernel<ALTHEA_GBUFFER_FP32>, dgrid, dim3(256), 0, s, p, plane);
-  else
-    hipLaunchKernelGGL(ssao_depth_kernel<ALTHEA_GBUFFER_FP16>, dgrid, dim3(256), 0, s, p, plane);
-  althea_status st = check_launch("ssao_depth_kernel");
+  althea_status st = ALTHEA_OK;
+  if (border) {
+    float* pref_b = reinterpret_cast<float*>(mem + plane_bytes);
+    float* irr_b = reinterpret_cast<float*>(mem + plane_bytes + ((pref_bytes + 255) & ~static_cast<size_t>(255)));
+    st = launch_cube_border(ibl->prefiltered, ibl->prefiltered_size, ibl->prefiltered_mips, pref_b, s);
+    if (st == ALTHEA_OK) st = launch_cube_border(ibl->irradiance, ibl->irradiance_size, 1, irr_b, s);
+    p.prefiltered = reinterpret_cast<const float4*>(pref_b);
+    p.irradiance = reinterpret_cast<const float4*>(irr_b);
+  }
+  if (st == ALTHEA_OK && p.ssao_on) {
+    float* plane = reinterpret_cast<float*>(mem);
+    p.ssao_depth = plane;
+    const dim3 dgrid(static_cast<unsigned>((pixels + 255) / 256));
+    if (g->format == ALTHEA_GBUFFER_FP32)
+      hipLaunchKernelGGL(ssao_depth_kernel<ALTHEA_GBUFFER_FP32>, dgrid, dim3(256), 0, s, p, plane);
+    else
+      hipLaunchKernelGGL(ssao_depth_kernel<ALTHEA_GBUFFER_FP16>, dgrid, dim3(256), 0, s, p, plane);
+    st = check_launch("ssao_depth_kernel");
+  }
   if (st == ALTHEA_OK)
     st = launch_resolve_with(p, g, lights, flags, mlp, mlp_used, s, reflection, shadow_cubes, shadow_size);
-  (void)hipFreeAsync(plane, s);
+  (void)hipFreeAsync(mem, s);
   return st;
 }
 

@@ -247,7 +247,23 @@ typedef struct althea_ibl_views {
   int irradiance_size;
   const float* brdf_lut;
   int brdf_lut_size;
+  /* Optional seamless-bordered copies of the prefiltered and irradiance cubes
+   * (althea_cube_border), which the deferred resolves read; an IBL handle
+   * keeps them.  NULL: each resolve builds them in its stream-ordered
+   * allocation (a copy of the two cubes per call). */
+  const float* prefiltered_bordered;
+  const float* irradiance_bordered;
 } althea_ibl_views;
+
+/* Seamless-bordered copy of a cube (size x size faces, mips levels): every
+ * face of mip m stored as (S + 2) x (S + 2) RGBA32F texels (S = size >> m),
+ * the border holding what the seamless bilinear filter reads one texel beyond
+ * each edge (the adjacent face's edge texel; at a corner the mean of the three
+ * texels meeting there), so a lookup is four plain loads with the cube's own
+ * result.  althea_cube_bordered_texels gives its size in float4. */
+ALTHEA_API size_t althea_cube_bordered_texels(int size, int mips);
+ALTHEA_API althea_status althea_cube_border(const float* d_cube, int size, int mips, float* d_bordered,
+                                            hipStream_t stream);
 
 ALTHEA_API void althea_ibl_desc_default(althea_ibl_desc* desc);
 ALTHEA_API althea_status althea_ibl_create(const althea_ibl_desc* desc, althea_ibl** out);

@@ -111,6 +111,67 @@ def test_cube_sample_matches_oracle(torch):
     assert err.max() < 2e-5, summary(err)
 
 
+def _bordered_sample(bc, size, mips, dirs, lods):
+    """Trilinear lookup of a seamless-bordered cube in FP64: four plain taps
+    per mip at (floor + 1, floor + 2) of the bordered face, no folding."""
+    face, st = ol.cube_select(dirs)
+    offs, o = [], 0
+    for m in range(mips):
+        offs.append(o)
+        o += 6 * (max(1, size >> m) + 2) ** 2
+
+    def bil(m):
+        S = max(1, size >> m)
+        W = S + 2
+        x, y = st[:, 0] * S - 0.5, st[:, 1] * S - 0.5  # the oracle's (s, t) are in [0, 1]
+        fx, fy = np.floor(x), np.floor(y)
+        ax, ay = (x - fx)[:, None], (y - fy)[:, None]
+        base = offs[m] + (face * W + fy.astype(np.int64) + 1) * W + fx.astype(np.int64) + 1
+        t = lambda k: bc[k, :3].astype(np.float64)
+        top = t(base) * (1 - ax) + t(base + 1) * ax
+        bot = t(base + W) * (1 - ax) + t(base + W + 1) * ax
+        return top * (1 - ay) + bot * ay
+
+    d = np.clip(lods, 0, mips - 1)
+    fl = np.floor(d)
+    l0 = fl.astype(np.int64)
+    delta = (d - fl)[:, None]
+    out = np.zeros((dirs.shape[0], 3))
+    for m in range(mips):
+        sel = l0 == m
+        if not sel.any():
+            continue
+        a = bil(m)[sel]
+        b = bil(min(m + 1, mips - 1))[sel]
+        out[sel] = a * (1 - delta[sel]) + b * delta[sel]
+    return out
+
+
+@pytest.mark.parametrize("size,mips", [(16, 5), (1, 1), (5, 1)])
+def test_cube_border_matches_seamless_oracle(torch, size, mips):
+    """The bordered copy the resolve reads: interior texels are the cube's,
+    and lookups near edges and corners (four plain taps, no folding) equal the
+    oracle's seamless trilinear lookup; 5 is an irradiance-like odd size."""
+    rng = np.random.default_rng(3 + size)
+    n = aa.cube_texels(size, mips)
+    cube = rng.uniform(0.2, 3.0, size=(n, 4)).astype(np.float32)
+    bc = aa.cube_border(torch.from_numpy(cube).cuda(), size, mips).cpu().numpy()
+    assert bc.shape[0] == aa.lib().althea_cube_bordered_texels(size, mips)
+    o, co = 0, 0
+    for m in range(mips):
+        S = max(1, size >> m)
+        faces = bc[o:o + 6 * (S + 2) ** 2].reshape(6, S + 2, S + 2, 4)
+        np.testing.assert_array_equal(faces[:, 1:S + 1, 1:S + 1, :3], cube[co:co + 6 * S * S].reshape(6, S, S, 4)[..., :3])
+        o += 6 * (S + 2) ** 2
+        co += 6 * S * S
+    q = np.concatenate([rng.normal(size=(20000, 3)), rng.uniform(-0.5, mips - 0.5, size=(20000, 1))], 1)
+    q[:8000, :3] = np.sign(rng.normal(size=(8000, 3))) * rng.uniform(0.97, 1.0, size=(8000, 3))
+    got = _bordered_sample(bc, size, mips, q[:, :3], q[:, 3])
+    ref = ol.cube_sample(cube, size, mips, q[:, :3], q[:, 3])
+    err = rel_err(got, ref, RADIANCE_FLOOR)
+    assert err.max() < 2e-6, summary(err)
+
+
 # ---------------------------------------------------------------- BRDF LUT
 def test_brdf_lut_matches_oracle(torch):
     got = aa.brdf_lut(128, 1024).cpu().numpy().reshape(-1, 2)

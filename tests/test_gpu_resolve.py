@@ -130,3 +130,23 @@ def test_resolve_no_lights_no_ssao(torch):
     err = rel_err(img[pix[:, 1], pix[:, 0], :3], ref, RADIANCE_FLOOR)
     assert err.max() < MAX_REL, summary(err)
     np.testing.assert_array_equal(ao[pix[:, 1], pix[:, 0]], ref_ao)
+
+
+def test_views_without_bordered_cubes_build_them_per_call(torch):
+    """The resolve reads seamless-bordered copies of the prefiltered and
+    irradiance cubes.  With the handle's copies and with views that carry none
+    (the resolve builds them in its own allocation), the frames are the same
+    bits."""
+    ibl, _ = _ibl(torch, "config0")
+    v = ibl.getHandles()
+    assert v.prefiltered_bordered and v.irradiance_bordered
+    bare = aa._lib.IBLViewsC()
+    for name, _t in aa._lib.IBLViewsC._fields_:
+        setattr(bare, name, getattr(v, name))
+    bare.prefiltered_bordered = None
+    bare.irradiance_bordered = None
+    scene = aa.synth.scene(seed=SEED0 + 4, width=640, height=360, lights=8)
+    a, ao_a = _run(torch, scene, v)
+    b, ao_b = _run(torch, scene, bare)
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    np.testing.assert_array_equal(ao_a.view(np.uint32), ao_b.view(np.uint32))
