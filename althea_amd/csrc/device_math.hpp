@@ -245,6 +245,49 @@ __device__ __forceinline__ LevelTaps gather_taps(const char* __restrict__ gpyr_b
   return t;
 }
 
+// Split gather copy: the padded texels of a gather level as an RG plane
+// (float2 per texel) and a B plane (float per texel).  A tap pair's red and
+// green are one 16-byte load and its blue one 8-byte load, so a wave whose
+// lanes read a patch of neighbouring texels touches about half the cache
+// lines that the float4 copy's two 12-byte loads per pair touch.
+struct SplitLevel {
+  float kx, cx, ky, cy;  // as GatherLevel
+  uint32_t rowbytes;     // (w + 2) * 8, one row of the RG plane
+  uint32_t rg;           // byte offset of the level's RG plane
+  uint32_t b;            // byte offset of the level's B plane
+  uint32_t pad;
+};
+
+// gather_taps on a split level: the same four texels and fractions.
+__device__ __forceinline__ LevelTaps split_taps(const char* __restrict__ base, const SplitLevel& g, float yaw,
+                                                float pitch) {
+  const float x = fmaf(yaw, g.kx, g.cx), y = fmaf(pitch, g.ky, g.cy);
+  LevelTaps t;
+  t.ax = __builtin_amdgcn_fractf(x);
+  t.ay = __builtin_amdgcn_fractf(y);
+  const int ix = cvt_floor(x), iy = cvt_floor(y);
+  // RG offset (iy + 1) rowbytes + (ix + 1) 8 as iy * rowbytes + (8 ix +
+  // rowbytes + 8); the B offset is half of it.
+  const int rb = static_cast<int>(g.rowbytes);
+  int col, o;
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(col) : "v"(ix), "s"(rb + 8));
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(o) : "v"(iy), "s"(rb), "v"(col));
+  const uint32_t o0 = static_cast<uint32_t>(o), ob = o0 >> 1;
+  typedef const __attribute__((address_space(1))) char gchar;
+  typedef const __attribute__((address_space(1))) float gfloat;
+  gchar* rgp = (gchar*)base + g.rg;
+  gchar* bp = (gchar*)base + g.b;
+  gfloat* a0 = (gfloat*)(rgp + o0);
+  gfloat* a1 = (gfloat*)(rgp + (o0 + g.rowbytes));
+  gfloat* b0 = (gfloat*)(bp + ob);
+  gfloat* b1 = (gfloat*)(bp + (ob + (g.rowbytes >> 1)));
+  t.t00 = mk(a0[0], a0[1], b0[0]);
+  t.t10 = mk(a0[2], a0[3], b0[1]);
+  t.t01 = mk(a1[0], a1[1], b1[0]);
+  t.t11 = mk(a1[2], a1[3], b1[1]);
+  return t;
+}
+
 // acc += w * bilinear in weight form: the four texel weights carry w, so a
 // sample's two source levels (w = its weight times 1 - delta and times
 // delta) add straight into the running sums.  w (1 - a) is one FMA.
@@ -275,6 +318,11 @@ __device__ __forceinline__ void gather_bilinear_acc(const char* __restrict__ gpy
                                                     float yaw, float pitch, float w, float& ar, float& ag,
                                                     float& ab) {
   taps_acc(gather_taps(gpyr_bytes, g, yaw, pitch), w, ar, ag, ab);
+}
+
+__device__ __forceinline__ void split_bilinear_acc(const char* __restrict__ base, const SplitLevel& g, float yaw,
+                                                   float pitch, float w, float& ar, float& ag, float& ab) {
+  taps_acc(split_taps(base, g, yaw, pitch), w, ar, ag, ab);
 }
 
 // Bilinear CLAMP_TO_EDGE fetch (the shading-side samplers).

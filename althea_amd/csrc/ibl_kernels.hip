@@ -300,10 +300,36 @@ struct PadParams {
   uint32_t dst[kMaxLevels + 1];  // gather texel offset of each level; dst[last + 1] = total
 };
 
+// Texels of a split gather copy's RG plane (the B plane starts 8 times this
+// many bytes in): the total rounded up to 64, so both planes start 256-byte
+// aligned.
+static size_t split_plane_texels(const GatherGeometry& gg) { return (gg.texels + 63) & ~static_cast<size_t>(63); }
+
+size_t split_gather_bytes(int w, int h, int first, int last) {
+  if (w <= 0 || h <= 0) return 0;
+  return 12 * split_plane_texels(gather_geometry(w, h, first, last));
+}
+
+SplitLevel split_level(int w, int h, const GatherGeometry& gg, int l) {
+  const PyramidGeometry g = pyramid_geometry(w, h);
+  const LevelC c = make_level(g.w[l], g.h[l], 0);
+  SplitLevel s{};
+  s.kx = c.kx;
+  s.cx = c.cx;
+  s.ky = c.ky;
+  s.cy = c.cy;
+  s.rowbytes = static_cast<uint32_t>(g.w[l] + 2) * 8u;
+  s.rg = 8u * gg.off[l];
+  s.b = static_cast<uint32_t>(8 * split_plane_texels(gg)) + 4u * gg.off[l];
+  return s;
+}
+
 // One thread per gather texel: padded (r, c) of level l is the pyramid texel
-// ((r - 1) mod h, (c - 1) mod w).
+// ((r - 1) mod h, (c - 1) mod w).  kSplit writes it as the split copy's RG
+// and B planes (plane_texels apart) instead of one float4.
+template <bool kSplit>
 __global__ __launch_bounds__(256) void gather_pad_kernel(const float4* __restrict__ pyr, const PadParams p,
-                                                         float4* __restrict__ gpyr) {
+                                                         float4* __restrict__ gpyr, size_t plane_texels) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= p.dst[p.last + 1]) return;
   int l = p.first;
@@ -314,10 +340,17 @@ __global__ __launch_bounds__(256) void gather_pad_kernel(const float4* __restric
   const int c = static_cast<int>(k) - r * (w + 2);
   const int sr = r == 0 ? h - 1 : (r == h + 1 ? 0 : r - 1);
   const int sc = c == 0 ? w - 1 : (c == w + 1 ? 0 : c - 1);
-  gpyr[i] = pyr[p.src[l] + static_cast<size_t>(sr) * w + sc];
+  const float4 v = pyr[p.src[l] + static_cast<size_t>(sr) * w + sc];
+  if (kSplit) {
+    reinterpret_cast<float2*>(gpyr)[i] = make_float2(v.x, v.y);
+    reinterpret_cast<float*>(reinterpret_cast<float2*>(gpyr) + plane_texels)[i] = v.z;
+  } else {
+    gpyr[i] = v;
+  }
 }
 
-althea_status launch_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr, hipStream_t s) {
+static althea_status launch_gather_as(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr,
+                                      bool split, hipStream_t s) {
   const PyramidGeometry g = pyramid_geometry(w, h);
   PadParams p{};
   p.first = gg.first;
@@ -330,9 +363,22 @@ althea_status launch_gather(const float* pyr, int w, int h, const GatherGeometry
   }
   p.dst[gg.last + 1] = static_cast<uint32_t>(gg.texels);
   const uint32_t blocks = static_cast<uint32_t>((gg.texels + 255) / 256);
-  hipLaunchKernelGGL(gather_pad_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4*>(pyr), p,
-                     reinterpret_cast<float4*>(gpyr));
+  if (split)
+    hipLaunchKernelGGL(gather_pad_kernel<true>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4*>(pyr),
+                       p, reinterpret_cast<float4*>(gpyr), split_plane_texels(gg));
+  else
+    hipLaunchKernelGGL(gather_pad_kernel<false>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4*>(pyr),
+                       p, reinterpret_cast<float4*>(gpyr), size_t{0});
   return check_launch("gather_pad_kernel");
+}
+
+althea_status launch_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr, hipStream_t s) {
+  return launch_gather_as(pyr, w, h, gg, gpyr, false, s);
+}
+
+althea_status launch_split_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr,
+                                  hipStream_t s) {
+  return launch_gather_as(pyr, w, h, gg, gpyr, true, s);
 }
 
 static CubeMips cube_mips_of(int size, int mips) {
@@ -564,7 +610,7 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
 // read at the fixed source LOD log2(W / theta_samples), times
 // pi / (theta_samples phi_samples).
 struct IrradianceParams {
-  GatherLevel lv0, lv1;  // the two source levels around LOD log2(W / theta_samples), gather copies
+  SplitLevel lv0, lv1;  // the two source levels around LOD log2(W / theta_samples), split gather copies
   int levels;
   int size;
   int theta_samples;
@@ -615,8 +661,8 @@ __device__ __forceinline__ void irradiance_column(const char* __restrict__ gpyr,
                     fmaf(u.z, th.x, fmaf(v.z, th.y, w.z)));
     float yaw, pitch;
     equirect_angles(d, yaw, pitch);
-    gather_bilinear_acc(gpyr, p.lv0, yaw, pitch, w0, sr, sg, sb);
-    if (kTwoLevels) gather_bilinear_acc(gpyr, p.lv1, yaw, pitch, w1, sr, sg, sb);
+    split_bilinear_acc(gpyr, p.lv0, yaw, pitch, w0, sr, sg, sb);
+    if (kTwoLevels) split_bilinear_acc(gpyr, p.lv1, yaw, pitch, w1, sr, sg, sb);
   }
 }
 
@@ -700,13 +746,13 @@ static IrradianceLod irradiance_lod(int w, int h, int theta) {
   return r;
 }
 
-// The theta and phi tables, then the gather copy of levels l0..l1.
+// The theta and phi tables, then the split gather copy of levels l0..l1.
 size_t irradiance_workspace_bytes(int w, int h, int, int theta) {
   int phi = irradiance_phi_samples(w, h, theta);
   const size_t tables = align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
   if (w <= 0 || h <= 0 || theta <= 0) return tables;
   const IrradianceLod lod = irradiance_lod(w, h, theta);
-  return tables + gather_bytes(w, h, lod.l0, lod.l1);
+  return tables + split_gather_bytes(w, h, lod.l0, lod.l1);
 }
 
 althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
@@ -731,10 +777,10 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.l1 = lod.l1;
   p.delta = lod.delta;
   const GatherGeometry gg = gather_geometry(env_w, env_h, p.l0, p.l1);
-  althea_status st = launch_gather(pyr, env_w, env_h, gg, gather, s);
+  althea_status st = launch_split_gather(pyr, env_w, env_h, gg, gather, s);
   if (st != ALTHEA_OK) return st;
-  p.lv0 = make_gather_level(g.w[p.l0], g.h[p.l0], gg.off[p.l0]);
-  p.lv1 = make_gather_level(g.w[p.l1], g.h[p.l1], gg.off[p.l1]);
+  p.lv0 = split_level(env_w, env_h, gg, p.l0);
+  p.lv1 = split_level(env_w, env_h, gg, p.l1);
   p.scale = M_PI / theta_samples / phi;
   const int n = std::max(theta_samples, phi);
   hipLaunchKernelGGL(irradiance_tables_kernel, dim3((n + 255) / 256), dim3(256), 0, s, theta_samples, phi, p.delta,

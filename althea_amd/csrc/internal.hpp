@@ -70,6 +70,13 @@ GatherGeometry gather_geometry(int w, int h, int first, int last);
 size_t gather_bytes(int w, int h, int first, int last);
 void gather_levels(int w, int h, const GatherGeometry& gg, GatherLevel* out);
 althea_status launch_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr, hipStream_t s);
+// The split form (SplitLevel): the same padded texels as an RG plane of
+// float2 followed by a B plane of floats.
+struct SplitLevel;
+size_t split_gather_bytes(int w, int h, int first, int last);
+SplitLevel split_level(int w, int h, const GatherGeometry& gg, int l);
+althea_status launch_split_gather(const float* pyr, int w, int h, const GatherGeometry& gg, float* gpyr,
+                                  hipStream_t s);
 // Sample tables of a prefilter (the part of its workspace the allocation-free
 // entry point keeps in the library's fixed array).
 size_t prefilter_table_bytes(int mips, int samples);

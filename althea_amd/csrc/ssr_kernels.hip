@@ -51,9 +51,9 @@ struct SsrParams {
   const void* albedo;
   const void* mro;
   int width, height;
-  const float4* prefiltered;
-  CubeMips pref;
-  const float4* irradiance;
+  const float4* prefiltered;  // seamless-bordered copy (BorderedCube)
+  BorderedCube pref;
+  const float4* irradiance;  // seamless-bordered copy, one mip
   int irr_size;
   const float2* lut;
   int lut_size;
@@ -272,11 +272,11 @@ __global__ __launch_bounds__(256) void ssr_capture_kernel(const SsrParams p, con
   if (hit) {
     const float nd = 2.0f * NI;
     const f3 Rr = mk(fmaf(-nd, Nh.x, R.x), fmaf(-nd, Nh.y, R.y), fmaf(-nd, Nh.z, R.z));
-    const f3 refl = cube_sample(p.prefiltered, p.pref, Rr, roughness * static_cast<float>(p.pref.mips - 1));
+    const f3 refl = bcube_sample(p.prefiltered, p.pref, Rr, roughness * static_cast<float>(p.pref.mips - 1));
     int face;
     float pc, pt;
     cube_select(Nh, face, pc, pt);
-    const f3 irr = cube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
+    const f3 irr = bcube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
     float A, B;
     lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
     color = pbr_environment(m, irr, refl, A, B, 1.0f);
@@ -365,12 +365,9 @@ althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   p.mro = g->mro;
   p.width = g->width;
   p.height = g->height;
-  p.prefiltered = reinterpret_cast<const float4*>(ibl->prefiltered);
-  p.pref.size = ibl->prefiltered_size;
-  p.pref.mips = ibl->prefiltered_mips;
-  for (int m = 0; m < ibl->prefiltered_mips; ++m)
-    p.pref.off[m] = static_cast<uint32_t>(cube_mip_offset(ibl->prefiltered_size, m));
-  p.irradiance = reinterpret_cast<const float4*>(ibl->irradiance);
+  p.prefiltered = reinterpret_cast<const float4*>(ibl->prefiltered_bordered);
+  p.pref = bordered_cube_of(ibl->prefiltered_size, ibl->prefiltered_mips);
+  p.irradiance = reinterpret_cast<const float4*>(ibl->irradiance_bordered);
   p.irr_size = ibl->irradiance_size;
   p.lut = reinterpret_cast<const float2*>(ibl->brdf_lut);
   p.lut_size = ibl->brdf_lut_size;
@@ -400,6 +397,29 @@ althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
   p.shadow.cubes = shadow_cubes;
   p.shadow.size = shadow_size;
   const bool sh = shadow_cubes != nullptr;
+  // Views without the seamless-bordered cubes: build them in a
+  // stream-ordered allocation for this call.
+  char* border_mem = nullptr;
+  if (!p.prefiltered || !p.irradiance) {
+    const size_t pref_bytes = (16 * cube_bordered_texels(ibl->prefiltered_size, ibl->prefiltered_mips) + 255) &
+                              ~static_cast<size_t>(255);
+    const size_t bytes = pref_bytes + 16 * cube_bordered_texels(ibl->irradiance_size, 1);
+    const hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&border_mem), bytes, s);
+    if (e != hipSuccess) {
+      set_error("ssr_capture: hipMallocAsync of %zu bytes (bordered cubes) failed: %s", bytes, hipGetErrorString(e));
+      return ALTHEA_ERROR_HIP;
+    }
+    float* pref_b = reinterpret_cast<float*>(border_mem);
+    float* irr_b = reinterpret_cast<float*>(border_mem + pref_bytes);
+    althea_status st = launch_cube_border(ibl->prefiltered, ibl->prefiltered_size, ibl->prefiltered_mips, pref_b, s);
+    if (st == ALTHEA_OK) st = launch_cube_border(ibl->irradiance, ibl->irradiance_size, 1, irr_b, s);
+    if (st != ALTHEA_OK) {
+      (void)hipFreeAsync(border_mem, s);
+      return st;
+    }
+    p.prefiltered = reinterpret_cast<const float4*>(pref_b);
+    p.irradiance = reinterpret_cast<const float4*>(irr_b);
+  }
   if (g->format == ALTHEA_GBUFFER_FP32) {
     hipLaunchKernelGGL((ssr_depth_kernel<ALTHEA_GBUFFER_FP32>), dgrid, dim3(256), 0, s, p);
     if (sh)
@@ -413,7 +433,9 @@ althea_status launch_ssr_capture(const althea_gbuffer_view* g, const althea_ibl_
     else
       hipLaunchKernelGGL((ssr_capture_kernel<ALTHEA_GBUFFER_FP16>), grid, dim3(256), 0, s, p, lights4);
   }
-  return check_launch("ssr_capture_kernel");
+  const althea_status st = check_launch("ssr_capture_kernel");
+  if (border_mem) (void)hipFreeAsync(border_mem, s);
+  return st;
 }
 
 althea_status launch_reflection_convolve(const althea_reflection_view* reflection, hipStream_t s) {

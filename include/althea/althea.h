@@ -248,9 +248,9 @@ typedef struct althea_ibl_views {
   const float* brdf_lut;
   int brdf_lut_size;
   /* Optional seamless-bordered copies of the prefiltered and irradiance cubes
-   * (althea_cube_border), which the deferred resolves read; an IBL handle
-   * keeps them.  NULL: each resolve builds them in its stream-ordered
-   * allocation (a copy of the two cubes per call). */
+   * (althea_cube_border), which the deferred resolves and the SSR capture's
+   * hit shading read; an IBL handle keeps them.  NULL: each such call builds
+   * them in a stream-ordered allocation (a copy of the two cubes per call). */
   const float* prefiltered_bordered;
   const float* irradiance_bordered;
 } althea_ibl_views;

@@ -95,6 +95,25 @@ def test_ssr_capture_heightfield_matches_oracle(torch, ibl):
     assert r["err"]["mean"] < 5e-5 and r["err"]["max"] < 2e-3, r
 
 
+def test_ssr_capture_builds_bordered_cubes_for_bare_views(torch, ibl):
+    """The hit shading reads seamless-bordered copies of the prefiltered and
+    irradiance cubes: the handle's, or for views that carry none, copies the
+    capture builds in its own allocation.  Both give the same bits."""
+    ibl_res, _ = ibl
+    v = ibl_res.getHandles()
+    assert v.prefiltered_bordered and v.irradiance_bordered
+    bare = aa._lib.IBLViewsC()
+    for name, _t in aa._lib.IBLViewsC._fields_:
+        setattr(bare, name, getattr(v, name))
+    bare.prefiltered_bordered = None
+    bare.irradiance_bordered = None
+    scene = aa.synth.scene(seed=SEED0 + 21, width=480, height=270, lights=8)
+    a = _capture(torch, scene, v).level(0).cpu().numpy()
+    b = _capture(torch, scene, bare).level(0).cpu().numpy()
+    assert (a[..., 3] == 1.0).sum() > 2000
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 @pytest.mark.parametrize("W,H", [(37, 23), (1, 16), (130, 3)])
 def test_ssr_capture_odd_extents_match_oracle(torch, ibl, W, H):
     """Extents that are not multiples of the 16 x 16 tiles or of the depth
