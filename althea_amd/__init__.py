@@ -640,6 +640,28 @@ class PointLightCollection:
               "althea_shadow_cubes_ws")
         return self.shadow_maps
 
+    def drawShadowMapsGltf(self, primitives, materials, textures, stream=None):
+        """The cubes of every light from glTF primitives (GltfPrimitive list,
+        GltfMaterial list, Texture list), with the shadow shaders' alpha test:
+        fragments whose base colour alpha is below the material's cut-off
+        cast no shadow."""
+        if self.shadow_maps is None:
+            raise ValueError("PointLightCollection was created without shadow maps")
+        torch = _torch()
+        prims = (_lib.GltfPrimitiveC * max(1, len(primitives)))(*[p.view() for p in primitives])
+        mats = (_lib.GltfMaterialC * max(1, len(materials)))(*[m.c for m in materials])
+        texs = (_lib.TextureViewC * max(1, len(textures)))(*[t.view() for t in textures])
+        need = lib().althea_shadow_cubes_gltf_workspace_bytes(prims, len(primitives), self.getCount())
+        if need == 0:
+            raise ValueError("invalid glTF shadow draw")
+        if self.workspace is None or self.workspace.numel() < need:
+            self.workspace = _empty(stream, need, dtype=torch.uint8, device="cuda")
+        check(lib().althea_shadow_cubes_gltf(prims, len(primitives), mats, len(materials), texs, len(textures),
+                                             _ptr(self.lights), self.getCount(), self.shadow_size,
+                                             _ptr(self.shadow_storage), _ptr(self.workspace), _stream(stream)),
+              "althea_shadow_cubes_gltf")
+        return self.shadow_maps
+
 
 class DeferredRendering:
     @staticmethod

@@ -172,6 +172,9 @@ PROTOTYPES = {
     "althea_shadow_cubes_workspace_bytes": (sz, [i32, i32]),
     "althea_shadow_cubes_ws": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp, vp]),
     "althea_shadow_cubes": (i32, [C.POINTER(MeshViewC), vp, i32, i32, vp, vp]),
+    "althea_shadow_cubes_gltf_workspace_bytes": (sz, [C.POINTER(GltfPrimitiveC), i32, i32]),
+    "althea_shadow_cubes_gltf": (i32, [C.POINTER(GltfPrimitiveC), i32, C.POINTER(GltfMaterialC), i32,
+                                       C.POINTER(TextureViewC), i32, vp, i32, i32, vp, vp, vp]),
     "althea_deferred_resolve_shadowed": (i32, [C.POINTER(GBufferViewC), C.POINTER(IBLViewsC), vp, i32,
                                                C.POINTER(GlobalsC), C.POINTER(SsaoDescC), C.POINTER(ReflectionViewC),
                                                vp, i32, C.c_uint, vp, vp, vp]),

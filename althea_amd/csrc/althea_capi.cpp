@@ -798,6 +798,55 @@ althea_status althea_shadow_cubes_ws(const althea_mesh_view* mesh, const althea_
   return launch_shadow_cubes(mesh, d_lights, light_count, size, d_cubes, stream, d_workspace);
 }
 
+size_t althea_shadow_cubes_gltf_workspace_bytes(const althea_gltf_primitive* primitives, int primitive_count,
+                                                int light_count) {
+  if (primitive_count < 0 || (primitive_count > 0 && !primitives) || light_count < 1) return 0;
+  const int total = gltf_triangles(primitives, primitive_count);
+  if (total < 0 || static_cast<long long>(light_count) * total >= (1ll << 31)) return 0;
+  return shadow_gltf_workspace_bytes(light_count, total, primitive_count);
+}
+
+althea_status althea_shadow_cubes_gltf(const althea_gltf_primitive* primitives, int primitive_count,
+                                       const althea_gltf_material* materials, int material_count,
+                                       const althea_texture_view* textures, int texture_count,
+                                       const althea_point_light* d_lights, int light_count, int size, float* d_cubes,
+                                       void* d_workspace, hipStream_t stream) {
+  ALTHEA_REQUIRE(d_cubes && light_count >= 1 && d_lights && d_workspace,
+                 "shadow_cubes_gltf: null argument or no light");
+  ALTHEA_REQUIRE(workspace_aligned(d_workspace), "shadow_cubes_gltf: workspace not 256-byte aligned");
+  ALTHEA_REQUIRE(size >= 1 && size <= 4096, "shadow_cubes_gltf: size %d", size);
+  ALTHEA_REQUIRE(primitive_count >= 0 && (primitive_count == 0 || primitives), "shadow_cubes_gltf: %d primitives",
+                 primitive_count);
+  ALTHEA_REQUIRE(material_count >= 0 && (material_count == 0 || materials) && texture_count >= 0 &&
+                     (texture_count == 0 || textures),
+                 "shadow_cubes_gltf: %d materials, %d textures", material_count, texture_count);
+  const int total = gltf_triangles(primitives, primitive_count);
+  ALTHEA_REQUIRE(total >= 0 && static_cast<long long>(light_count) * total < (1ll << 31),
+                 "shadow_cubes_gltf: %d lights x %d triangles", light_count, total);
+  for (int k = 0; k < texture_count; ++k) {
+    const althea_status st = texture_ok(textures[k], k);
+    if (st != ALTHEA_OK) return st;
+  }
+  for (int k = 0; k < material_count; ++k) {
+    const althea_gltf_material& m = materials[k];
+    ALTHEA_REQUIRE(m.base_texture >= 0 && m.base_texture < texture_count && m.base_uv >= 0 && m.base_uv < 4,
+                   "shadow_cubes_gltf: material %d: base colour texture %d, coordinate set %d", k, m.base_texture,
+                   m.base_uv);
+  }
+  for (int k = 0; k < primitive_count; ++k) {
+    const althea_gltf_primitive& q = primitives[k];
+    ALTHEA_REQUIRE(q.triangle_count >= 0 && q.material >= 0 && q.material < material_count,
+                   "shadow_cubes_gltf: primitive %d: %d triangles, material %d", k, q.triangle_count, q.material);
+    if (q.triangle_count == 0) continue;
+    ALTHEA_REQUIRE(q.positions && q.indices && q.vertex_count >= 1, "shadow_cubes_gltf: primitive %d: null vertex data",
+                   k);
+    ALTHEA_REQUIRE(q.uvs[materials[q.material].base_uv],
+                   "shadow_cubes_gltf: primitive %d lacks the texture coordinates its material reads", k);
+  }
+  return launch_shadow_cubes_gltf(primitives, primitive_count, materials, textures, d_lights, light_count, size,
+                                  d_cubes, d_workspace, stream);
+}
+
 althea_status althea_deferred_resolve_shadowed(const althea_gbuffer_view* g, const althea_ibl_views* ibl,
                                                const althea_point_light* d_lights, int light_count,
                                                const althea_globals* globals, const althea_ssao_desc* ssao,

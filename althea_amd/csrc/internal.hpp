@@ -147,6 +147,14 @@ size_t shadow_workspace_bytes(int light_count, int triangle_count);
 althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
                                   int size, float* cubes, hipStream_t s,
                                   void* workspace = nullptr);
+// The glTF shadow pass (alpha-tested): the mesh path's workspace for the
+// draw's triangles, then the triangle soup, the alpha-test records and the
+// primitive table.
+size_t shadow_gltf_workspace_bytes(int light_count, int triangles, int prim_count);
+althea_status launch_shadow_cubes_gltf(const althea_gltf_primitive* prims, int prim_count,
+                                       const althea_gltf_material* materials, const althea_texture_view* textures,
+                                       const althea_point_light* lights, int light_count, int size, float* cubes,
+                                       void* workspace, hipStream_t s);
 
 // Screen-space reflection (ssr_kernels.hip): level 0 of the reflection
 // buffer, then the glossy convolution of levels 1 .. mips-1.

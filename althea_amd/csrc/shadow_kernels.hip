@@ -33,13 +33,21 @@
 // (shadow_raster_kernel).  A last pass fills each face's border from the
 // adjacent faces (the seamless filter's reads beyond the edge), so the lookup
 // needs no folding.
+//
+// glTF primitives (althea_shadow_cubes_gltf) add the shadow shaders' alpha
+// test (ShadowMapBindless.frag, ShadowMap.frag): a setup pass flattens the
+// primitives into a triangle soup with one alpha-test record per triangle,
+// and a texel whose base colour alpha (texture x factor) is below the
+// material's cut-off is discarded before its depth is written.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "device_math.hpp"
 #include "fill.hpp"
+#include "gltf_texture.hpp"
 #include "internal.hpp"
 #include "shadow.hpp"
 #include "slot_scan.hpp"
@@ -62,8 +70,9 @@ struct ShadowSlots {
 };
 
 struct ShadowParams {
-  const float* positions;
-  const uint32_t* indices;
+  const float* positions;    // float3 vertices, or with no indices a triangle soup (9 floats a triangle)
+  const uint32_t* indices;   // 3 a triangle, or null
+  const GbAlpha* alpha;      // per triangle (glTF draws), or null: every triangle opaque
   int vertex_count;
   int triangle_count;
   const float4* lights;  // [light][2]: position, emission
@@ -89,8 +98,34 @@ struct ShadowTri {
   float det;      // a . n
 };
 
+// The shadow shaders' alpha test at texel (i, j) of `face`, e the texel-centre
+// ray's edge functions: the base colour alpha (texture x factor) at the ray's
+// point on the triangle, the texture coordinate interpolated with the
+// barycentrics e / (e0 + e1 + e2) and its derivatives taken as forward
+// differences to the rays of texels (i + 1, j) and (i, j + 1) on the
+// triangle's plane (the face image's x and y), below the cut-off.
+__device__ bool shadow_alpha_discards(const GbAlpha& a, const ShadowTri& tri, float e0, float e1, float e2, int face,
+                                      int i, int j, float inv_s) {
+  auto bary = [&](f3 d, float l[3]) {
+    const float f0 = dot(d, tri.n0), f1 = dot(d, tri.n1), f2 = dot(d, tri.n2);
+    const float inv = 1.0f / (f0 + f1 + f2);
+    l[0] = f0 * inv, l[1] = f1 * inv, l[2] = f2 * inv;
+  };
+  float len;
+  const float inv = 1.0f / (e0 + e1 + e2);
+  const float l[3] = {e0 * inv, e1 * inv, e2 * inv};
+  float lx[3], ly[3];
+  bary(shadow_texel_dir(face, i + 1, j, inv_s, len), lx);
+  bary(shadow_texel_dir(face, i, j + 1, inv_s, len), ly);
+  const GbUv c = gb_uv_of(a.uv[0], a.uv[1], a.uv[2], l, lx, ly);
+  return gb_sample<true>(a.tex, c).w * a.factor < a.cutoff;
+}
+
+// kAlpha: `a` is the triangle's alpha-test record (no test where its texels
+// are null).
+template <bool kAlpha = false>
 __device__ __forceinline__ void shadow_texel(const ShadowTri& tri, int face, int i, int j, int S, float inv_s,
-                                             float* __restrict__ cube) {
+                                             float* __restrict__ cube, const GbAlpha* a = nullptr) {
   float len;
   const f3 d = shadow_texel_dir(face, i, j, inv_s, len);
   const float e0 = dot(d, tri.n0), e1 = dot(d, tri.n1), e2 = dot(d, tri.n2);
@@ -99,14 +134,16 @@ __device__ __forceinline__ void shadow_texel(const ShadowTri& tri, int face, int
   if (!(rn < 0.0f)) return;
   const float t = tri.det * rcp_fast(rn);  // view depth of the hit
   if (t < ALTHEA_SHADOW_NEAR || t > ALTHEA_SHADOW_FAR) return;
+  if (kAlpha && a->tex.texels && shadow_alpha_discards(*a, tri, e0, e1, e2, face, i, j, inv_s)) return;
   const float depth = fminf(t * len * (1.0f / ALTHEA_SHADOW_FAR), 1.0f);
   atomicMin(reinterpret_cast<unsigned int*>(cube) + shadow_texel_index(S, face, i, j), __float_as_uint(depth));
 }
 
-// Light-relative vertices and edge functions of item = light * T + t; live is
-// false for items past the end and for back-facing or edge-on triangles.
+// Light-relative vertices and edge functions of item = light * T + t (t into
+// `tri_index`); live is false for items past the end and for back-facing or
+// edge-on triangles.
 __device__ __forceinline__ void shadow_item(const ShadowParams& p, int item, int& light, f3 v[3], ShadowTri& tri,
-                                            bool& live) {
+                                            bool& live, int& tri_index) {
   // item / T through the FP64 reciprocal: for item < 2^31 the truncated
   // product is off by at most one, which the remainder corrects.
   light = static_cast<int>(static_cast<double>(item) * p.inv_triangles);
@@ -116,14 +153,20 @@ __device__ __forceinline__ void shadow_item(const ShadowParams& p, int item, int
   } else if (t >= p.triangle_count) {
     ++light, t -= p.triangle_count;
   }
+  tri_index = t;
   live = item < p.light_count * p.triangle_count;
   tri = ShadowTri{};
   v[0] = v[1] = v[2] = mk(0.0f, 0.0f, 0.0f);
   if (!live) return;
   const float4 lp = p.lights[2 * light];
   for (int k = 0; k < 3; ++k) {
-    const uint32_t vi = p.indices[3 * t + k];
-    const float* q = p.positions + 3 * static_cast<size_t>(min(vi, static_cast<uint32_t>(p.vertex_count - 1)));
+    const float* q;
+    if (p.indices) {
+      const uint32_t vi = p.indices[3 * t + k];
+      q = p.positions + 3 * static_cast<size_t>(min(vi, static_cast<uint32_t>(p.vertex_count - 1)));
+    } else {
+      q = p.positions + 9 * static_cast<size_t>(t) + 3 * k;
+    }
     v[k] = mk(q[0] - lp.x, q[1] - lp.y, q[2] - lp.z);
   }
   // Edge functions from the edges e1 = b - a, e2 = c - a (c x a = e2 x a,
@@ -177,13 +220,14 @@ __device__ __forceinline__ void shadow_face_box(int face, const f3 v[3], bool li
   }
 }
 
+template <bool kAlpha>
 __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p) {
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
-  int light;
+  int light, t;
   f3 v[3];
   ShadowTri tri;
   bool live;
-  shadow_item(p, item, light, v, tri, live);
+  shadow_item(p, item, light, v, tri, live, t);
   const int S = p.size;
   const float inv_s = 1.0f / static_cast<float>(S);
   const size_t cube_floats = static_cast<size_t>(6) * (S + 2) * (S + 2);
@@ -195,7 +239,7 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
     const int area = (bw > 0 && bh > 0) ? bw * bh : 0;
     if (area > 0 && area <= kShadowLaneTexels) {
       for (int j = y0; j <= y1; ++j)
-        for (int i = x0; i <= x1; ++i) shadow_texel(tri, face, i, j, S, inv_s, cube);
+        for (int i = x0; i <= x1; ++i) shadow_texel<kAlpha>(tri, face, i, j, S, inv_s, cube, kAlpha ? p.alpha + t : nullptr);
     }
     // Larger boxes: the wave walks them one lane's box at a time.
     uint64_t big = __builtin_amdgcn_ballot_w64(area > kShadowLaneTexels);
@@ -210,10 +254,11 @@ __global__ __launch_bounds__(256) void shadow_raster_kernel(const ShadowParams p
       w.det = __shfl(tri.det, src);
       const int wx0 = __shfl(x0, src), wy0 = __shfl(y0, src), ww = __shfl(bw, src), wh = __shfl(bh, src);
       float* wcube = p.cubes + __shfl(light, src) * cube_floats;
+      const GbAlpha* wa = kAlpha ? p.alpha + __shfl(t, src) : nullptr;
       const int n = ww * wh;
       for (int k = threadIdx.x & 63; k < n; k += 64) {
         const int j = wy0 + k / ww, i = wx0 + k % ww;
-        shadow_texel(w, face, i, j, S, inv_s, wcube);
+        shadow_texel<kAlpha>(w, face, i, j, S, inv_s, wcube, wa);
       }
     }
   }
@@ -230,11 +275,11 @@ __global__ __launch_bounds__(kSlotBlock) void shadow_slot_setup_kernel(const Sha
   __shared__ unsigned long long wave_total[kSlotBlock / 64];
   const int item = blockIdx.x * kSlotBlock + threadIdx.x;
   const int items = p.light_count * p.triangle_count;
-  int light;
+  int light, t;
   f3 v[3];
   ShadowTri tri;
   bool live;
-  shadow_item(p, item, light, v, tri, live);
+  shadow_item(p, item, light, v, tri, live, t);
   unsigned long long area = 0;
   unsigned bx[12];
 #pragma unroll
@@ -273,6 +318,7 @@ __global__ __launch_bounds__(256) void shadow_offset_kernel(unsigned long long* 
   if (t < n) offsets[t] += blocks[t / kSlotBlock];
 }
 
+template <bool kAlpha>
 __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowParams p) {
   const unsigned long long* __restrict__ offsets = p.slots.offsets;
   const int n = p.light_count * p.triangle_count;
@@ -284,7 +330,7 @@ __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowPar
   const float inv_s = 1.0f / static_cast<float>(S);
   const size_t cube_floats = static_cast<size_t>(6) * (S + 2) * (S + 2);
   int w0 = slot_find(offsets, 0, n, begin);  // the item holding `begin`
-  int cur = -1, light = 0;
+  int cur = -1, light = 0, t = 0;
   ShadowTri tri{};
   for (unsigned long long base = begin; base < end; base += 64) {
     unsigned item_start;
@@ -322,10 +368,10 @@ __global__ __launch_bounds__(256) void shadow_slot_raster_kernel(const ShadowPar
     if (item != cur) {
       f3 v[3];
       bool live;
-      shadow_item(p, item, light, v, tri, live);
+      shadow_item(p, item, light, v, tri, live, t);
       cur = item;
     }
-    shadow_texel(tri, face, i, j, S, inv_s, p.cubes + light * cube_floats);
+    shadow_texel<kAlpha>(tri, face, i, j, S, inv_s, p.cubes + light * cube_floats, kAlpha ? p.alpha + t : nullptr);
   }
 }
 
@@ -382,24 +428,19 @@ size_t shadow_workspace_bytes(int light_count, int triangle_count) {
   return align256((items + 1) * 8) + align256(items * 6 * sizeof(uint2)) + align256(slot_blocks(items) * 8);
 }
 
-althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
-                                  int size, float* cubes, hipStream_t s, void* workspace) {
+// The cubes' clear, the raster (scheduled with a workspace, else the
+// lane-walk) and the borders, for the triangles p describes.
+static althea_status shadow_passes(ShadowParams& p, void* workspace, hipStream_t s) {
+  const int light_count = p.light_count, size = p.size;
+  float* cubes = p.cubes;
   const size_t texels = static_cast<size_t>(light_count) * 6 * (size + 2) * (size + 2);
   hipLaunchKernelGGL(fill_kernel<float>, dim3(fill_blocks(texels, sizeof(float))), dim3(256), 0, s, cubes, texels,
                      1.0f);
   althea_status st = check_launch("fill_kernel<float> (shadow clear)");
-  if (st != ALTHEA_OK || mesh->triangle_count == 0) return st;  // borders of a cleared cube are 1 too
-  ShadowParams p{};
-  p.positions = mesh->positions;
-  p.indices = mesh->indices;
-  p.vertex_count = mesh->vertex_count;
-  p.triangle_count = mesh->triangle_count;
-  p.lights = reinterpret_cast<const float4*>(lights);
-  p.light_count = light_count;
-  p.size = size;
-  p.cubes = cubes;
-  p.inv_triangles = 1.0 / static_cast<double>(mesh->triangle_count);
-  const size_t items = static_cast<size_t>(light_count) * mesh->triangle_count;
+  if (st != ALTHEA_OK || p.triangle_count == 0) return st;  // borders of a cleared cube are 1 too
+  p.inv_triangles = 1.0 / static_cast<double>(p.triangle_count);
+  const size_t items = static_cast<size_t>(light_count) * p.triangle_count;
+  const bool alpha = p.alpha != nullptr;
   if (workspace) {
     char* ws = static_cast<char*>(workspace);
     p.slots.offsets = reinterpret_cast<unsigned long long*>(ws);
@@ -416,10 +457,17 @@ althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_poi
     hipLaunchKernelGGL(shadow_offset_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, p.slots.offsets,
                        p.slots.blocks, ni);
     if ((st = check_launch("shadow_offset_kernel")) != ALTHEA_OK) return st;
-    hipLaunchKernelGGL(shadow_slot_raster_kernel, dim3(4096), dim3(256), 0, s, p);
+    if (alpha)
+      hipLaunchKernelGGL(shadow_slot_raster_kernel<true>, dim3(4096), dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL(shadow_slot_raster_kernel<false>, dim3(4096), dim3(256), 0, s, p);
     st = check_launch("shadow_slot_raster_kernel");
   } else {
-    hipLaunchKernelGGL(shadow_raster_kernel, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, p);
+    const dim3 grid(static_cast<unsigned>((items + 255) / 256));
+    if (alpha)
+      hipLaunchKernelGGL(shadow_raster_kernel<true>, grid, dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL(shadow_raster_kernel<false>, grid, dim3(256), 0, s, p);
     st = check_launch("shadow_raster_kernel");
   }
   if (st != ALTHEA_OK) return st;
@@ -427,6 +475,125 @@ althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_poi
   hipLaunchKernelGGL(shadow_border_kernel, dim3(static_cast<unsigned>((border + 255) / 256)), dim3(256), 0, s, cubes,
                      size, light_count);
   return check_launch("shadow_border_kernel");
+}
+
+althea_status launch_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* lights, int light_count,
+                                  int size, float* cubes, hipStream_t s, void* workspace) {
+  ShadowParams p{};
+  p.positions = mesh->positions;
+  p.indices = mesh->indices;
+  p.vertex_count = mesh->vertex_count;
+  p.triangle_count = mesh->triangle_count;
+  p.lights = reinterpret_cast<const float4*>(lights);
+  p.light_count = light_count;
+  p.size = size;
+  p.cubes = cubes;
+  return shadow_passes(p, workspace, s);
+}
+
+// ------------------------------------------------------- glTF primitives
+// A primitive of the shadow pass as the setup reads it: world-space geometry,
+// the base colour coordinate set its material reads, and the alpha test's
+// factor, cut-off and texture.
+struct ShadowPrim {
+  const float* positions;
+  const float* uv;  // float2 per vertex
+  const uint32_t* indices;
+  int vertex_count, tri_begin, triangle_count;
+  float factor, cutoff;
+  althea_texture_view tex;
+};
+
+// One lane per triangle of the draw (primitives numbered one after the
+// other): its three world-space vertices into the triangle soup the raster
+// reads, and its alpha-test record.
+__global__ __launch_bounds__(256) void shadow_gltf_setup_kernel(const ShadowPrim* __restrict__ prims, int prim_count,
+                                                                int triangles, float* __restrict__ soup,
+                                                                GbAlpha* __restrict__ alpha) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= triangles) return;
+  int lo = 0, hi = prim_count;  // the last primitive whose first triangle is <= t
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prims[mid].tri_begin <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const ShadowPrim& pr = prims[lo];
+  const int lt = t - pr.tri_begin;
+  const uint32_t last = static_cast<uint32_t>(pr.vertex_count - 1);
+  GbAlpha a{};
+  const float2* uv = reinterpret_cast<const float2*>(pr.uv);
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t vi = min(pr.indices[3 * lt + k], last);
+    for (int c = 0; c < 3; ++c) soup[9 * static_cast<size_t>(t) + 3 * k + c] = pr.positions[3 * static_cast<size_t>(vi) + c];
+    a.uv[k] = uv[vi];
+  }
+  a.factor = pr.factor;
+  a.cutoff = pr.cutoff;
+  a.tex = pr.tex;
+  alpha[t] = a;
+}
+
+static size_t shadow_gltf_tables_bytes(int triangles, int prim_count) {
+  return align256(static_cast<size_t>(triangles) * 9 * sizeof(float)) +
+         align256(static_cast<size_t>(triangles) * sizeof(GbAlpha)) +
+         align256(static_cast<size_t>(prim_count) * sizeof(ShadowPrim));
+}
+
+size_t shadow_gltf_workspace_bytes(int light_count, int triangles, int prim_count) {
+  return shadow_workspace_bytes(light_count, triangles) + shadow_gltf_tables_bytes(triangles, prim_count);
+}
+
+althea_status launch_shadow_cubes_gltf(const althea_gltf_primitive* prims, int prim_count,
+                                       const althea_gltf_material* materials, const althea_texture_view* textures,
+                                       const althea_point_light* lights, int light_count, int size, float* cubes,
+                                       void* workspace, hipStream_t s) {
+  std::vector<ShadowPrim> table;
+  int total = 0;
+  for (int k = 0; k < prim_count; ++k) {
+    if (prims[k].triangle_count == 0) continue;
+    const althea_gltf_material& m = materials[prims[k].material];
+    ShadowPrim q{};
+    q.positions = prims[k].positions;
+    q.uv = prims[k].uvs[m.base_uv];
+    q.indices = prims[k].indices;
+    q.vertex_count = prims[k].vertex_count;
+    q.tri_begin = total;
+    q.triangle_count = prims[k].triangle_count;
+    q.factor = m.base_color_factor[3];
+    q.cutoff = m.alpha_cutoff;
+    q.tex = textures[m.base_texture];
+    table.push_back(q);
+    total += prims[k].triangle_count;
+  }
+  char* ws = static_cast<char*>(workspace);
+  float* soup = reinterpret_cast<float*>(ws + shadow_workspace_bytes(light_count, total));
+  GbAlpha* alpha = reinterpret_cast<GbAlpha*>(reinterpret_cast<char*>(soup) +
+                                              align256(static_cast<size_t>(total) * 9 * sizeof(float)));
+  ShadowPrim* d_prims = reinterpret_cast<ShadowPrim*>(reinterpret_cast<char*>(alpha) +
+                                                      align256(static_cast<size_t>(total) * sizeof(GbAlpha)));
+  if (total > 0) {
+    // Pageable host table: the copy orders after the stream's earlier work
+    // (the draw reads it from host memory, so it is not captured in graphs).
+    ALTHEA_TRY_HIP(hipMemcpyAsync(d_prims, table.data(), table.size() * sizeof(ShadowPrim), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(shadow_gltf_setup_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
+                       d_prims, static_cast<int>(table.size()), total, soup, alpha);
+    const althea_status st = check_launch("shadow_gltf_setup_kernel");
+    if (st != ALTHEA_OK) return st;
+  }
+  ShadowParams p{};
+  p.positions = soup;
+  p.indices = nullptr;
+  p.alpha = alpha;
+  p.vertex_count = 3 * total;
+  p.triangle_count = total;
+  p.lights = reinterpret_cast<const float4*>(lights);
+  p.light_count = light_count;
+  p.size = size;
+  p.cubes = cubes;
+  return shadow_passes(p, workspace, s);
 }
 
 }  // namespace althea

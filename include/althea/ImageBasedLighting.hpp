@@ -312,15 +312,26 @@ class PointLightCollection {
     const int n = static_cast<int>(getCount());
     const size_t need = althea_shadow_cubes_workspace_bytes(n, mesh.triangle_count);
     if (need == 0) throw std::runtime_error("PointLightCollection::drawShadowMaps: too many light-triangle pairs");
-    if (need > _workspaceBytes) {
-      void* ws = nullptr;
-      if (hipMalloc(&ws, need) != hipSuccess)
-        throw std::runtime_error("PointLightCollection::drawShadowMaps: hipMalloc failed");
-      _workspace.reset(ws);
-      _workspaceBytes = need;
-    }
+    reserveWorkspace(need);
     throwIfFailed(althea_shadow_cubes_ws(&mesh, lights(), n, _shadowSize, static_cast<float*>(_cubes.get()),
                                          _workspace.get(), stream),
+                  "PointLightCollection::drawShadowMaps");
+  }
+
+  // The shadow pass over a model's glTF primitives with their materials: the
+  // shadow shaders' alpha test (ShadowMapBindless.frag) discards fragments
+  // whose base colour alpha is below the material's cut-off.
+  void drawShadowMaps(const althea_gltf_primitive* primitives, int primitiveCount,
+                      const althea_gltf_material* materials, int materialCount, const althea_texture_view* textures,
+                      int textureCount, hipStream_t stream = nullptr) {
+    if (!_cubes) throw std::runtime_error("PointLightCollection: created without shadow maps");
+    const int n = static_cast<int>(getCount());
+    const size_t need = althea_shadow_cubes_gltf_workspace_bytes(primitives, primitiveCount, n);
+    if (need == 0) throw std::runtime_error("PointLightCollection::drawShadowMaps: invalid glTF draw");
+    reserveWorkspace(need);
+    throwIfFailed(althea_shadow_cubes_gltf(primitives, primitiveCount, materials, materialCount, textures,
+                                           textureCount, lights(), n, _shadowSize, static_cast<float*>(_cubes.get()),
+                                           _workspace.get(), stream),
                   "PointLightCollection::drawShadowMaps");
   }
 
@@ -331,6 +342,16 @@ class PointLightCollection {
  private:
   static void throwIfHip(hipError_t e, const char* where) {
     if (e != hipSuccess) throw std::runtime_error(std::string(where) + ": " + hipGetErrorString(e));
+  }
+  // Grows the kept workspace to `need` bytes (hipFree waits for the work
+  // that used the old one).
+  void reserveWorkspace(size_t need) {
+    if (need <= _workspaceBytes) return;
+    void* ws = nullptr;
+    if (hipMalloc(&ws, need) != hipSuccess)
+      throw std::runtime_error("PointLightCollection::drawShadowMaps: hipMalloc failed");
+    _workspace.reset(ws);
+    _workspaceBytes = need;
   }
   struct Free {
     void operator()(void* p) const { (void)hipFree(p); }

@@ -604,12 +604,8 @@ typedef struct althea_mesh_view {
  * face's edge texel; at a corner the mean of the three texels meeting there),
  * filled by this call.  d_mesh positions / indices and d_lights are device
  * memory; d_cubes holds light_count * althea_shadow_cube_floats(size) floats.
- * Deviation: every triangle given is opaque.  The reference's shadow shaders
- * discard fragments whose base-colour alpha is below the material's cut-off
- * (ShadowMapBindless.frag, ShadowMap.frag); the mesh here carries no
- * material, so alpha-masked geometry casts solid shadows while the glTF
- * G-buffer subpass (althea_gbuffer_draw_gltf) cuts it out.  Pass only opaque
- * triangles where that matters. */
+ * Every triangle of the mesh is opaque: alpha-masked geometry goes through
+ * althea_shadow_cubes_gltf, which runs the shadow shaders' alpha test. */
 ALTHEA_API size_t althea_shadow_cube_floats(int size);
 ALTHEA_API althea_status althea_shadow_cubes(const althea_mesh_view* mesh, const althea_point_light* d_lights,
                                              int light_count, int size, float* d_cubes, hipStream_t stream);
@@ -622,6 +618,27 @@ ALTHEA_API size_t althea_shadow_cubes_workspace_bytes(int light_count, int trian
 ALTHEA_API althea_status althea_shadow_cubes_ws(const althea_mesh_view* mesh, const althea_point_light* d_lights,
                                                 int light_count, int size, float* d_cubes, void* d_workspace,
                                                 hipStream_t stream);
+
+/* The shadow cubes of glTF primitives, as the reference's shadow pass draws
+ * a model's primitives with their materials: ShadowMapBindless.frag and
+ * ShadowMap.frag discard a fragment whose base colour alpha (the base colour
+ * texture's alpha times the factor's) is below the material's cut-off, before
+ * the depth test.  Here a texel's candidate hit is discarded when that alpha,
+ * sampled at the hit (the material's base colour coordinate set,
+ * interpolated perspective-correctly) with the mip level of the coordinate's
+ * derivatives across the face's texels (forward differences to the next
+ * texel right and below), is below the cut-off.  Everything else as
+ * althea_shadow_cubes_ws, the same cubes for opaque materials.  Tables are
+ * host arrays as for althea_gbuffer_draw_gltf (positions, indices and the
+ * base colour coordinates of each primitive, and the texels, in device
+ * memory); d_workspace: althea_shadow_cubes_gltf_workspace_bytes bytes. */
+ALTHEA_API size_t althea_shadow_cubes_gltf_workspace_bytes(const althea_gltf_primitive* primitives,
+                                                           int primitive_count, int light_count);
+ALTHEA_API althea_status althea_shadow_cubes_gltf(const althea_gltf_primitive* primitives, int primitive_count,
+                                                  const althea_gltf_material* materials, int material_count,
+                                                  const althea_texture_view* textures, int texture_count,
+                                                  const althea_point_light* d_lights, int light_count, int size,
+                                                  float* d_cubes, void* d_workspace, hipStream_t stream);
 
 /* althea_deferred_resolve (optionally reading a reflection buffer, which may
  * be NULL) with the point-light shadow test of PBRMaterial.glsl:119-140: light

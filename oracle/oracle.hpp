@@ -777,18 +777,16 @@ inline bool shadow_hidden(const ShadowSet& sh, int light, dvec3 L, double ldist,
     *margin = std::min(*margin, std::fabs(closest - (ldist - 0.5)) / (kShadowFar * contrast + 1e-3 * ldist));
   return closest < ldist - 0.5;
 }
-// The cube of one light (6 S S doubles, [face][y][x]) from a mesh.
-inline void shadow_cube(const float* pos, const uint32_t* idx, int triangles, dvec3 lp, int S, double* out) {
-  for (size_t k = 0; k < static_cast<size_t>(6) * S * S; ++k) out[k] = 1.0;
-  for (int t = 0; t < triangles; ++t) {
-    dvec3 v[3];
-    for (int k = 0; k < 3; ++k) {
-      const float* q = pos + 3 * static_cast<size_t>(idx[3 * t + k]);
-      v[k] = dvec3{q[0], q[1], q[2]} - lp;
-    }
+// One triangle (light-relative vertices v) into the cube of one light
+// (6 S S doubles, [face][y][x]): every texel whose centre ray meets it as a
+// front face between the planes, and that `keep(face, i, j, e)` accepts
+// (e: the ray's edge functions d . (b x c), d . (c x a), d . (a x b)).
+template <class Keep>
+inline void shadow_triangle(const dvec3 v[3], int S, double* out, const Keep& keep) {
+  {
     dvec3 n0 = cross(v[1], v[2]), n1 = cross(v[2], v[0]), n2 = cross(v[0], v[1]);
     double det = dot(v[0], n0);
-    if (!(det < 0.0)) continue;  // back face or edge-on: culled
+    if (!(det < 0.0)) return;  // back face or edge-on: culled
     for (int face = 0; face < 6; ++face) {
       double sc[3], tc[3], ma[3];
       int front = 0;
@@ -827,11 +825,26 @@ inline void shadow_cube(const float* pos, const uint32_t* idx, int triangles, dv
           if (!(rn < 0.0)) continue;
           double tt = det / rn;
           if (tt < kShadowNear || tt > kShadowFar) continue;
+          const double e[3] = {e0, e1, e2};
+          if (!keep(face, i, j, e)) continue;
           double depth = std::min(tt * length(d) / kShadowFar, 1.0);
           double& o = out[(static_cast<size_t>(face) * S + j) * S + i];
           o = std::min(o, depth);
         }
     }
+  }
+}
+
+// The cube of one light (6 S S doubles, [face][y][x]) from a mesh.
+inline void shadow_cube(const float* pos, const uint32_t* idx, int triangles, dvec3 lp, int S, double* out) {
+  for (size_t k = 0; k < static_cast<size_t>(6) * S * S; ++k) out[k] = 1.0;
+  for (int t = 0; t < triangles; ++t) {
+    dvec3 v[3];
+    for (int k = 0; k < 3; ++k) {
+      const float* q = pos + 3 * static_cast<size_t>(idx[3 * t + k]);
+      v[k] = dvec3{q[0], q[1], q[2]} - lp;
+    }
+    shadow_triangle(v, S, out, [](int, int, int, const double*) { return true; });
   }
 }
 
@@ -1541,6 +1554,61 @@ struct GltfPrim {
   const uint32_t* idx;
   int tris, material;
 };
+// f4b. The shadow pass over glTF primitives (Src/PointLight.cpp:235-282 drawing
+// each primitive with its material; ShadowMapBindless.frag, ShadowMap.frag):
+// a candidate texel is discarded when
+// alpha = texture(baseColorTexture, uv).a * baseColorFactor.a < alphaCutoff,
+// uv the material's base colour set interpolated at the texel-centre ray's
+// hit (barycentrics e_k / sum e), its derivatives the differences to the
+// rays of texels (i + 1, j) and (i, j + 1) on the triangle's plane (the face
+// image's x and y, the rasterizer's screen derivatives).  `margin`
+// (optional, 6 S S, preset by the caller) is lowered per texel to the
+// distance of each candidate's alpha decision from its threshold
+// (|alpha - cutoff| and the sampler's decisions).
+inline void shadow_cube_gltf(const GltfPrim* prims, int count, const GltfMaterial* mats, const TexView* texs,
+                             dvec3 lp, int S, double* out, double* margin) {
+  for (size_t k = 0; k < static_cast<size_t>(6) * S * S; ++k) out[k] = 1.0;
+  auto texel_ray = [&](int face, int i, int j) {
+    const double pc = (2.0 * i + 1.0) / S - 1.0, pt = (2.0 * j + 1.0) / S - 1.0;
+    return shadow_face_turn(face, face_dir(face, pc, pt));
+  };
+  for (int p = 0; p < count; ++p) {
+    const GltfPrim& pr = prims[p];
+    const GltfMaterial& mt = mats[pr.material];
+    const float* uv = pr.uv[mt.base_uv];
+    for (int t = 0; t < pr.tris; ++t) {
+      const uint32_t* vi = pr.idx + 3 * t;
+      dvec3 v[3];
+      for (int k = 0; k < 3; ++k) {
+        const float* q = pr.pos + 3 * static_cast<size_t>(vi[k]);
+        v[k] = dvec3{q[0], q[1], q[2]} - lp;
+      }
+      const dvec3 en[3] = {cross(v[1], v[2]), cross(v[2], v[0]), cross(v[0], v[1])};
+      auto keep = [&](int face, int i, int j, const double* e) {
+        const dvec3 rx = texel_ray(face, i + 1, j), ry = texel_ray(face, i, j + 1);
+        double ex[3], ey[3];
+        for (int k = 0; k < 3; ++k) ex[k] = dot(rx, en[k]), ey[k] = dot(ry, en[k]);
+        const double se = e[0] + e[1] + e[2], sx = ex[0] + ex[1] + ex[2], sy = ey[0] + ey[1] + ey[2];
+        double u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+          const double us = uv[2 * vi[k]], vs = uv[2 * vi[k] + 1];
+          u[0] += e[k] / se * us, w[0] += e[k] / se * vs;
+          u[1] += ex[k] / sx * us, w[1] += ex[k] / sx * vs;
+          u[2] += ey[k] / sy * us, w[2] += ey[k] / sy * vs;
+        }
+        double o[4], m = 1e30;
+        tex_sample(texs[mt.base_texture], u[0], w[0], u[1] - u[0], w[1] - w[0], u[2] - u[0], w[2] - w[0], o, &m);
+        const double alpha = o[3] * mt.base_color_factor[3];
+        if (margin) {
+          double& mg = margin[(static_cast<size_t>(face) * S + j) * S + i];
+          mg = std::min({mg, m, std::fabs(alpha - mt.alpha_cutoff)});
+        }
+        return !(alpha < mt.alpha_cutoff);
+      };
+      shadow_triangle(v, S, out, keep);
+    }
+  }
+}
 // One pixel of the glTF subpass over primitives drawn in order; out[16] as
 // gbuffer_pixel; margin adds the alpha test's |alpha - cutoff| and the texture
 // decisions of the winner and of every nearer candidate.

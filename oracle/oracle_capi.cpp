@@ -443,6 +443,26 @@ struct OracleTex {
   const uint8_t* texels;
   int w, h, mips, srgb, mag, min, mipmode, wrap_u, wrap_v;
 };
+// The oracle's primitive and texture tables from the C views (every texture
+// a material names).
+static void gltf_tables(const OracleGltfPrim* prims, int count, const GltfMaterial* mats, const OracleTex* texs,
+                        std::vector<GltfPrim>& ps, std::vector<TexView>& tv) {
+  ps.resize(count);
+  for (int k = 0; k < count; ++k)
+    ps[k] = {prims[k].pos, prims[k].tan, prims[k].bit, prims[k].nrm,
+             {prims[k].uv[0], prims[k].uv[1], prims[k].uv[2], prims[k].uv[3]}, prims[k].idx, prims[k].tris,
+             prims[k].material};
+  int nt = 0;
+  for (int k = 0; k < count; ++k) nt = std::max(nt, ps[k].material + 1);
+  int ntex = 0;
+  for (int k = 0; k < nt; ++k)
+    ntex = std::max({ntex, mats[k].base_texture + 1, mats[k].normal_texture + 1, mats[k].metallic_roughness_texture + 1});
+  tv.resize(ntex);
+  for (int k = 0; k < ntex; ++k)
+    tv[k] = {texs[k].texels, texs[k].w, texs[k].h, texs[k].mips, texs[k].srgb, texs[k].mag, texs[k].min,
+             texs[k].mipmode, texs[k].wrap_u, texs[k].wrap_v};
+}
+
 void oracle_gbuffer_gltf(const float* const* cam, int width, int height, const OracleGltfPrim* prims, int count,
                          const GltfMaterial* mats, const OracleTex* texs, const int* pixels, int n, double* out,
                          double* margin, int* tri) {
@@ -453,20 +473,9 @@ void oracle_gbuffer_gltf(const float* const* cam, int width, int height, const O
   in.inv_proj = dmat4::from_float(cam[1]);
   in.view = dmat4::from_float(cam[2]);
   in.inv_view = dmat4::from_float(cam[3]);
-  std::vector<GltfPrim> ps(count);
-  for (int k = 0; k < count; ++k)
-    ps[k] = {prims[k].pos, prims[k].tan, prims[k].bit, prims[k].nrm,
-             {prims[k].uv[0], prims[k].uv[1], prims[k].uv[2], prims[k].uv[3]}, prims[k].idx, prims[k].tris,
-             prims[k].material};
-  int nt = 0;
-  for (int k = 0; k < count; ++k) nt = std::max(nt, ps[k].material + 1);
-  int ntex = 0;
-  for (int k = 0; k < nt; ++k)
-    ntex = std::max({ntex, mats[k].base_texture + 1, mats[k].normal_texture + 1, mats[k].metallic_roughness_texture + 1});
-  std::vector<TexView> tv(ntex);
-  for (int k = 0; k < ntex; ++k)
-    tv[k] = {texs[k].texels, texs[k].w, texs[k].h, texs[k].mips, texs[k].srgb, texs[k].mag, texs[k].min,
-             texs[k].mipmode, texs[k].wrap_u, texs[k].wrap_v};
+  std::vector<GltfPrim> ps;
+  std::vector<TexView> tv;
+  gltf_tables(prims, count, mats, texs, ps, tv);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < n; ++i)
     gltf_pixel(ps.data(), count, mats, tv.data(), in, pixels[2 * i], pixels[2 * i + 1],
@@ -495,6 +504,22 @@ void oracle_shadow_cubes(const float* positions, const uint32_t* indices, int tr
     const float* lp = lights + 8 * l;
     shadow_cube(positions, indices, triangles, {lp[0], lp[1], lp[2]}, size,
                 out + static_cast<size_t>(l) * 6 * size * size);
+  }
+}
+
+// Alpha-tested shadow cubes of glTF primitives: out and margin
+// [light][face][y][x] doubles (margin 1e30 where no candidate was tested).
+void oracle_shadow_cubes_gltf(const OracleGltfPrim* prims, int count, const GltfMaterial* mats, const OracleTex* texs,
+                              const float* lights, int light_count, int size, double* out, double* margin) {
+  std::vector<GltfPrim> ps;
+  std::vector<TexView> tv;
+  gltf_tables(prims, count, mats, texs, ps, tv);
+  const size_t per = static_cast<size_t>(6) * size * size;
+  for (size_t k = 0; k < per * light_count; ++k) margin[k] = 1e30;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int l = 0; l < light_count; ++l) {
+    const float* lp = lights + 8 * l;
+    shadow_cube_gltf(ps.data(), count, mats, tv.data(), {lp[0], lp[1], lp[2]}, size, out + l * per, margin + l * per);
   }
 }
 

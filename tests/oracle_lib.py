@@ -87,6 +87,7 @@ def load(path) -> C.CDLL:
     L.oracle_texture_sample.argtypes = [vp, vp, i32, vp, vp]
     L.oracle_texture_mips.argtypes = [vp, i32, i32, i32, i32]
     L.oracle_shadow_cubes.argtypes = [vp, vp, i32, vp, i32, i32, vp]
+    L.oracle_shadow_cubes_gltf.argtypes = [vp, i32, vp, vp, vp, i32, i32, vp, vp]
     L.oracle_convolve_level.argtypes = [vp, i32, i32, i32, i32, i32, vp]
     L.oracle_reflection_sample.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
     L.oracle_mlp_param_count.restype = i32
@@ -484,6 +485,40 @@ def gbuffer_gltf(scene, prims, mats, texs, pixels=None):
     lib().oracle_gbuffer_gltf(C.cast(cam, vp), scene.width, scene.height, prim_arr, len(pc), mat_arr, tex_arr,
                               _p(px), px.shape[0], _p(out), _p(margin), _p(tri))
     return out.reshape(-1, 4, 4), margin, tri
+
+
+def _gltf_prims(prims):
+    """OracleGltfPrimC array of prims (dicts of host arrays as gbuffer_gltf
+    takes them) and the arrays it points into."""
+    keep = []
+    pc = []
+    for pr in prims:
+        arrs = [_c(pr[k], np.float32) if pr.get(k) is not None else None for k in ("pos", "tan", "bit", "nrm")]
+        uvs = [_c(u, np.float32) if u is not None else None for u in pr["uvs"]] + [None] * (4 - len(pr["uvs"]))
+        idx = _c(pr["idx"], np.uint32)
+        keep += arrs + uvs + [idx]
+        pc.append(OracleGltfPrimC(*[_p(a) if a is not None else None for a in arrs],
+                                  (vp * 4)(*[_p(u) if u is not None else None for u in uvs]), _p(idx),
+                                  arrs[0].shape[0], idx.size // 3, pr["material"]))
+    return (OracleGltfPrimC * len(pc))(*pc), keep
+
+
+def shadow_cubes_gltf(prims, mats, texs, lights, size: int):
+    """Oracle alpha-tested shadow cubes of glTF primitives (dicts as
+    gbuffer_gltf takes them; mats GltfMaterialC, texs OracleTex) for (n, 8)
+    point lights: (cubes (lights, 6, size, size), alpha-decision margin of the
+    same shape, 1e30 where no candidate was tested)."""
+    prim_arr, keep = _gltf_prims(prims)
+    from althea_amd import _lib as L
+    mat_arr = (L.GltfMaterialC * len(mats))(*mats)
+    tex_arr = (OracleTexC * len(texs))(*[t.c for t in texs])
+    lt = _c(lights, np.float32).reshape(-1, 8)
+    out = np.empty((lt.shape[0], 6, size, size), np.float64)
+    margin = np.empty_like(out)
+    lib().oracle_shadow_cubes_gltf(prim_arr, len(prims), mat_arr, tex_arr, _p(lt), lt.shape[0], size, _p(out),
+                                   _p(margin))
+    del keep
+    return out, margin
 
 
 def shadow_cubes(positions, indices, lights, size: int) -> np.ndarray:

@@ -296,3 +296,77 @@ def test_ssr_capture_shadowed_matches_oracle(torch, ibl):
     torch.cuda.synchronize()
     np.testing.assert_allclose(r1.level(0).cpu().numpy(), r0.level(0).cpu().numpy(), rtol=2e-6, atol=0)
 
+
+
+# ------------------------------------------------- glTF (alpha-tested) pass
+def _gltf_shadow_scene(seed, n=40):
+    """The heightfield as a glTF primitive whose base colour alpha cuts about
+    a quarter of its texture (sRGB, LINEAR with 7 mips, coordinates
+    repeating four times across the terrain, so the mip level varies over
+    the cubes), over an opaque floor primitive (a 1 x 1 white texture)."""
+    pos, idx = aa.synth.heightfield_mesh(seed, n=n)
+    uv = np.stack([(pos[:, 0] / 20.0 + 0.5) * 4.0, (pos[:, 2] / 20.0 + 0.5) * 4.0], 1).astype(np.float32)
+    floor_pos = np.array([[-30, -3, -30], [30, -3, 30], [30, -3, -30], [-30, -3, 30]], np.float32)
+    floor_idx = np.array([[0, 1, 2], [0, 3, 1]], np.uint32)
+    floor_uv = np.array([[0, 0], [1, 1], [1, 0], [0, 1]], np.float32)
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)
+    base[..., 3] = np.where(rng.uniform(size=(64, 64)) < 0.25, 60, 230)
+    return [(pos, idx, uv, 0), (floor_pos, floor_idx, floor_uv, 1)], base
+
+
+def _gltf_shadow_gpu(torch, prims, base, lights, size, opaque=False):
+    import ctypes
+
+    f = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    lvl = base.copy()
+    if opaque:
+        lvl[..., 3] = 255
+    texs = [aa.Texture(f(lvl), mips=7, srgb=True), aa.Texture(f(np.full((1, 1, 4), 255, np.uint8)))]
+    mats = [aa.GltfMaterial(0, 0, 0), aa.GltfMaterial(1, 1, 1)]
+    gp = [aa.GltfPrimitive(f(p), None, None, None, [f(u)], f(i.view(np.int32)), m) for p, i, u, m in prims]
+    coll = aa.PointLightCollection.from_lights(torch.from_numpy(lights).cuda(), shadow_size=size)
+    coll.drawShadowMapsGltf(gp, mats, texs)
+    torch.cuda.synchronize()
+    chains = [t.data.cpu().numpy() for t in texs]
+    return coll, coll.getShadowMapImage().cpu().numpy(), mats, chains
+
+
+@pytest.mark.parametrize("size", [96, 37])
+def test_gltf_alpha_tested_cubes_match_oracle(torch, size):
+    """althea_shadow_cubes_gltf against the oracle's alpha-tested pass (which
+    samples the GPU's own mip chain).  Texels whose alpha decision (or a
+    texture decision) lies within MARGIN of its threshold may go either way
+    and are only counted."""
+    sc = aa.synth.scene(seed=SEED0 + 61, width=8, height=8, lights=5)
+    prims, base = _gltf_shadow_scene(SEED0 + 61)
+    _, got, mats, chains = _gltf_shadow_gpu(torch, prims, base, sc.lights, size)
+    otex = [ol.OracleTex(chains[0], 64, 64, 7, 1, 1, 1, 1, 0, 0), ol.OracleTex(chains[1], 1, 1, 1, 0, 1, 1, 1, 0, 0)]
+    oprims = [{"pos": p, "uvs": [u], "idx": i, "material": m} for p, i, u, m in prims]
+    ref, margin = ol.shadow_cubes_gltf(oprims, [m.c for m in mats], otex, sc.lights, size)
+    plain = ol.shadow_cubes(np.concatenate([prims[0][0], prims[1][0]]),
+                            np.concatenate([prims[0][1], prims[1][1] + len(prims[0][0])]), sc.lights, size)
+    clear = margin > 2e-4
+    r = _compare_cubes(got[clear], ref[clear])
+    r["clear"] = float(clear.mean())
+    r["cut"] = int(((plain < 1.0) & (ref != plain)).sum())
+    print("gltf shadow cubes", size, r)
+    assert r["cut"] > 0.02 * r["hits"], r  # the alpha test removed texels
+    assert r["clear"] > 0.97, r
+    assert r["hit_disagree"] <= 2e-3 * r["hits"] + 4, r
+    assert r["depth_off"] <= 2e-3 * r["hits"] + 4, r
+    assert r["err"]["mean"] < 1e-6, r
+
+
+def test_gltf_opaque_material_is_the_mesh_pass(torch):
+    """With every base colour alpha 1 the alpha test keeps every fragment:
+    the glTF pass writes the mesh pass's cubes bit for bit (the same
+    light-relative vertices and texel tests)."""
+    sc = aa.synth.scene(seed=SEED0 + 62, width=8, height=8, lights=4)
+    prims, base = _gltf_shadow_scene(SEED0 + 62, n=32)
+    _, got, _, _ = _gltf_shadow_gpu(torch, prims, base, sc.lights, 64, opaque=True)
+    pos = np.concatenate([prims[0][0], prims[1][0]])
+    idx = np.concatenate([prims[0][1], prims[1][1] + len(prims[0][0])])
+    _, want = _gpu_cubes(torch, pos, idx, sc.lights, 64)
+    assert (want < 1.0).any()
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))

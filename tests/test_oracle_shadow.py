@@ -150,6 +150,112 @@ def test_random_triangles_match_brute_force(seed):
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=0)
 
 
+# ------------------------------------------------- glTF (alpha-tested) pass
+def _gltf_material(factor_alpha=1.0, cutoff=0.5):
+    import ctypes
+
+    from althea_amd import _lib as L
+    return L.GltfMaterialC((ctypes.c_float * 4)(1.0, 1.0, 1.0, factor_alpha), 1.0, 1.0, 1.0, cutoff, 0, 0, 0, 0, 0)
+
+
+def _alpha_texture(alpha, nearest=True):
+    """RGBA8 texture (one level) whose alpha channel is `alpha` (h, w)."""
+    h, w = alpha.shape
+    lvl = np.full((h, w, 4), 200, np.uint8)
+    lvl[..., 3] = alpha
+    f = 0 if nearest else 1
+    return ol.OracleTex(lvl.reshape(-1, 4), w, h, 1, srgb=0, mag=f, min_=f, mipmode=0)
+
+
+def _gltf_prim(pos, idx, uv):
+    return {"pos": pos, "tan": None, "bit": None, "nrm": None, "uvs": [uv], "idx": idx, "material": 0}
+
+
+def _random_tris(seed, T=40):
+    rng = np.random.default_rng(seed)
+    centres = rng.normal(0.0, 3.0, (T, 1, 3))
+    pos = (centres + rng.normal(0.0, 1.5, (T, 3, 3))).reshape(-1, 3).astype(np.float32)
+    idx = np.arange(3 * T, dtype=np.uint32).reshape(T, 3)
+    uv = rng.uniform(0.0, 1.0, (3 * T, 2)).astype(np.float32)
+    return pos, idx, uv
+
+
+@pytest.mark.parametrize("alpha,factor,kept", [(255, 1.0, True), (0, 1.0, False), (200, 0.5, False),
+                                               (200, 0.7, True)])
+def test_gltf_uniform_alpha_keeps_or_drops_every_fragment(alpha, factor, kept):
+    """A texture of one alpha value: alpha / 255 x the factor's alpha against
+    the 0.5 cut-off keeps every fragment (the cubes of the plain mesh pass) or
+    none (every texel cleared to 1): 200/255 x 0.5 = 0.392 is cut, x 0.7 =
+    0.549 is kept."""
+    S = 16
+    lp = np.array([0.3, -0.2, 0.1])
+    pos, idx, uv = _random_tris(5)
+    tex = _alpha_texture(np.full((4, 4), alpha, np.uint8))
+    got, margin = ol.shadow_cubes_gltf([_gltf_prim(pos, idx, uv)], [_gltf_material(factor)], [tex], _light(lp), S)
+    plain = ol.shadow_cubes(pos, idx, _light(lp), S)
+    assert np.mean(plain < 1.0) > 0.1
+    if kept:
+        np.testing.assert_array_equal(got, plain)
+    else:
+        assert np.all(got == 1.0)
+    # Every texel with a candidate fragment had its alpha decision taken.
+    np.testing.assert_array_equal(margin[0] < 1e30, plain[0] < 1.0)
+
+
+def test_gltf_checkerboard_cuts_the_analytic_squares():
+    """A square below the light covering its -Y face, textured with a 2 x 2
+    NEAREST checkerboard of alpha 0 and 255 (magnified: one texture texel
+    spans many face texels).  Each -Y texel's ray meets the plane y = -h at
+    P = d h / |d.y|; uv is P's affine coordinate across the square, and the
+    texel is written (with the analytic distance) exactly where floor(2 uv)
+    names an opaque texture texel."""
+    S, h = 32, 1.0
+    c = np.array([0.123, -h, -0.217])
+    u, v = np.array([3.0, 0, 0]), np.array([0, 0, -3.0])
+    pos, idx = _square(c, u, v, +1)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    board = np.array([[0, 255], [255, 0]], np.uint8)  # row j, column i
+    tex = _alpha_texture(board)
+    got, margin = ol.shadow_cubes_gltf([_gltf_prim(pos, idx, uv)], [_gltf_material()], [tex], _light((0, 0, 0)), S)
+    d = _face_rays(S)[3]
+    p = d * (np.float32(h) / np.abs(d[..., 1:2]))
+    corner = pos[0].astype(np.float64)
+    uu = (p - corner) @ (2 * u) / np.dot(2 * u, 2 * u)
+    vv = (p - corner) @ (2 * v) / np.dot(2 * v, 2 * v)
+    ti, tj = np.floor(2 * uu).astype(int), np.floor(2 * vv).astype(int)
+    opaque = board[tj % 2, ti % 2] == 255
+    clear = (np.abs(2 * uu - np.round(2 * uu)) > 1e-6) & (np.abs(2 * vv - np.round(2 * vv)) > 1e-6)
+    want = np.where(opaque, float(np.float32(h)) * np.linalg.norm(d, axis=-1) / FAR, 1.0)
+    assert 0.3 < opaque.mean() < 0.7 and clear.mean() > 0.9
+    np.testing.assert_allclose(got[0, 3][clear], want[clear], rtol=1e-12)
+    assert np.all(got[0, 2] == 1.0)
+
+
+def test_gltf_derivatives_pick_the_mip_level():
+    """The mip level comes from the texture coordinate's differences to the
+    next texel right and below.  A two-level texture whose level 0 is opaque
+    and level 1 transparent (LINEAR, nearest mip): a quad whose uv spans few
+    texture texels per face texel reads level 0 (kept); with the uv scaled up
+    so that one face texel spans more than 1.5 texture texels, level 1
+    (cut)."""
+    S, h = 16, 1.0
+    pos, idx = _square((0.05, -h, 0.07), (3.0, 0, 0), (0, 0, -3.0), +1)
+    lvl = np.full((8 * 8 + 4 * 4, 4), 200, np.uint8)
+    lvl[:64, 3] = 255
+    lvl[64:, 3] = 0
+    tex = ol.OracleTex(lvl, 8, 8, 2, srgb=0, mag=1, min_=1, mipmode=0)
+    out = {}
+    for scale in (1.0, 40.0):
+        uv = (np.array([[0, 0], [1, 0], [1, 1], [0, 1]]) * scale).astype(np.float32)
+        got, _ = ol.shadow_cubes_gltf([_gltf_prim(pos, idx, uv)], [_gltf_material()], [tex], _light((0, 0, 0)), S)
+        out[scale] = got[0, 3]
+    # One face texel at the -Y face centre spans 2h/S = 0.125 of the 6-wide
+    # square: 0.125 / 6 x 8 = 0.17 texture texels (magnified) at scale 1 and
+    # 6.7 (level 1 of the 2) at scale 40.
+    assert np.all(out[1.0] < 1.0)
+    assert np.all(out[40.0] == 1.0)
+
+
 # ---------------------------------------------------------------- consumer
 @pytest.fixture(scope="module")
 def plane_scene():
