@@ -54,6 +54,7 @@ struct ResolveParams {
   float exposure;
   int tonemap;
   int ssao_on;
+  int ssao_w_depth;  // clip w is the view depth bit for bit (w row of vp = -view_z)
   int ssao_samples;
   float ssao_inv_samples;
   float ssao_radius, ssao_bias;
@@ -108,7 +109,11 @@ __device__ __forceinline__ float view_depth(const ResolveParams& p, f3 q) {
 
 constexpr int kSsaoBatch = 4;  // position gathers issued together
 
-template <int kFmt>
+// kWDepth: the projection's w row is (0, 0, -1, 0), so the w row of
+// projection * view is -view_z entry for entry (the host checks the bits)
+// and the clip w of every tap, formed by the same fused operations with
+// every operand negated, equals its view depth: the depth stands in for w.
+template <int kFmt, bool kWDepth>
 __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int py, f3 P, f3 N) {
   const float4 nz = s_noise[(px & 3) + 4 * (py & 3)];
   const f3 r = mk(nz.x, nz.y, nz.z);
@@ -157,7 +162,7 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
       sd[j] = fmaf(dT, kx, fmaf(dB, ky, fmaf(dN, kz, frag_depth)));
       const float cx = fmaf(tx, kx, fmaf(bx, ky, fmaf(nx, kz, px0)));
       const float cy = fmaf(ty, kx, fmaf(by, ky, fmaf(ny, kz, py0)));
-      const float cw = fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
+      const float cw = kWDepth ? sd[j] : fmaf(tw, kx, fmaf(bw, ky, fmaf(nw, kz, pw0)));
       const float inv_w = rcp_fast(cw);
       const float sx = fmaf(cx, inv_w, hw), sy = fmaf(cy, inv_w, hh);  // pixels
       // floor(s) lies in [0, extent) exactly where s does, and equals the
@@ -199,7 +204,7 @@ __device__ float ssao(const ResolveParams& p, const float4* s_noise, int px, int
 // [64 w, 64 w + 64): its deferred lanes take consecutive records in lane
 // order and lane 0 writes the wave's count, zero included, so the queue needs
 // neither atomics nor clearing.
-template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false, bool kShadow = false>
+template <int kFmt, bool kOutF16, bool kMlp, bool kRefl = false, bool kShadow = false, bool kWDepth = false>
 __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, int tiles_x,
                                              const float4* __restrict__ lights, const float4* s_noise) {
   const int px = (tile % tiles_x) * 64 + (threadIdx.x & 63);
@@ -304,7 +309,7 @@ __device__ __forceinline__ void resolve_tile(const ResolveParams& p, int tile, i
     float pc, pt;
     cube_select(N, face, pc, pt);
     const f3 irr = bcube_bilinear(p.irradiance, p.irr_size, face, pc, pt);
-    if (p.ssao_on) ao = ssao<kFmt>(p, s_noise, px, py, P, N);
+    if (p.ssao_on) ao = ssao<kFmt, kWDepth>(p, s_noise, px, py, P, N);
     if (p.ao_out) p.ao_out[idx] = ao;
     float A, B;
     lut_sample(p.lut, p.lut_size, m.n_dot_v, roughness, A, B);
@@ -345,8 +350,9 @@ __global__ __launch_bounds__(256) void ssao_depth_kernel(const ResolveParams p, 
 
 // The resolve: one block per tile, the SSAO noise tile in LDS; kRefl blends
 // the reflection buffer, kShadow applies the point-light shadow test, kMlp
-// defers the pixels the MLP serves.
-template <int kFmt, bool kOutF16, bool kRefl = false, bool kShadow = false, bool kMlp = false>
+// defers the pixels the MLP serves; kWDepth (the plain resolve under a
+// perspective projection) takes each SSAO tap's view depth as its clip w.
+template <int kFmt, bool kOutF16, bool kRefl = false, bool kShadow = false, bool kMlp = false, bool kWDepth = false>
 __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, const float4* __restrict__ lights) {
   __shared__ float4 s_noise[16];
   if (threadIdx.x < 16)
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, con
         make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
   __syncthreads();
   const int tiles_x = (p.width + 63) / 64;
-  resolve_tile<kFmt, kOutF16, kMlp, kRefl, kShadow>(p, blockIdx.x, tiles_x, lights, s_noise);
+  resolve_tile<kFmt, kOutF16, kMlp, kRefl, kShadow, kWDepth>(p, blockIdx.x, tiles_x, lights, s_noise);
 }
 
 // The MLP-IBL resolve held to the plain resolve's six waves per SIMD (80
@@ -493,6 +499,8 @@ althea_status launch_resolve(const althea_gbuffer_view* g, const althea_ibl_view
   view_ray_affine(*globals, g->width, g->height, p.ray0, p.rayx, p.rayy, &p.ray_df);
   for (int c = 0; c < 4; ++c) p.view_z[c] = globals->view[c * 4 + 2];
   view_projection(*globals, p.vp);
+  p.ssao_w_depth = p.vp[3] == -p.view_z[0] && p.vp[7] == -p.view_z[1] && p.vp[11] == -p.view_z[2] &&
+                   p.vp[15] == -p.view_z[3];
   p.exposure = globals->exposure;
   p.tonemap = (flags & ALTHEA_RESOLVE_TONEMAP) ? 1 : 0;
   p.ssao_on = (ssao && ssao->enabled) ? 1 : 0;
@@ -634,14 +642,23 @@ static althea_status launch_resolve_with(ResolveParams& p, const althea_gbuffer_
     }
     return check_launch("resolve_kernel");
   }
-  if (fp32 && f16_out)
-    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p, lights4);
-  else if (fp32)
-    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p, lights4);
-  else if (f16_out)
-    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p, lights4);
-  else
-    hipLaunchKernelGGL((resolve_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p, lights4);
+  const int variant = (f16_out ? 1 : 0) | (fp32 ? 0 : 2) | (p.ssao_w_depth ? 4 : 0);
+  switch (variant) {
+#define ALTHEA_RESOLVE_CASE(v, fmt, f16, wdepth)                                                                  \
+  case v:                                                                                                         \
+    hipLaunchKernelGGL((resolve_kernel<fmt, f16, false, false, false, wdepth>), grid, dim3(256), 0, s, p, lights4); \
+    break;
+    ALTHEA_RESOLVE_CASE(0, ALTHEA_GBUFFER_FP32, false, false)
+    ALTHEA_RESOLVE_CASE(1, ALTHEA_GBUFFER_FP32, true, false)
+    ALTHEA_RESOLVE_CASE(2, ALTHEA_GBUFFER_FP16, false, false)
+    ALTHEA_RESOLVE_CASE(3, ALTHEA_GBUFFER_FP16, true, false)
+    ALTHEA_RESOLVE_CASE(4, ALTHEA_GBUFFER_FP32, false, true)
+    ALTHEA_RESOLVE_CASE(5, ALTHEA_GBUFFER_FP32, true, true)
+    ALTHEA_RESOLVE_CASE(6, ALTHEA_GBUFFER_FP16, false, true)
+    ALTHEA_RESOLVE_CASE(7, ALTHEA_GBUFFER_FP16, true, true)
+#undef ALTHEA_RESOLVE_CASE
+    default: break;
+  }
   return check_launch("resolve_kernel");
 }
 
