@@ -363,11 +363,12 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveParams p, con
   resolve_tile<kFmt, kOutF16, kMlp, kRefl, kShadow, kWDepth>(p, blockIdx.x, tiles_x, lights, s_noise);
 }
 
-// The MLP-IBL resolve held to the plain resolve's six waves per SIMD (80
-// registers per lane), so that a frame whose MLP serves few pixels costs what
-// the cube-IBL frame costs.
-template <int kFmt, bool kOutF16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void resolve_mlp_kernel(
+// The MLP-IBL resolve held to the plain resolve's waves per SIMD (six, or
+// seven where SSAO takes the depth for w: 80 or 72 registers per lane), so
+// that a frame whose MLP serves few pixels costs what the cube-IBL frame
+// costs.
+template <int kFmt, bool kOutF16, bool kWDepth>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWDepth ? 7 : 6))) void resolve_mlp_kernel(
     const ResolveParams p, const float4* __restrict__ lights) {
   __shared__ float4 s_noise[16];
   if (threadIdx.x < 16)
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void r
         make_float4(p.noise[threadIdx.x][0], p.noise[threadIdx.x][1], p.noise[threadIdx.x][2], 0.0f);
   __syncthreads();
   const int tiles_x = (p.width + 63) / 64;
-  resolve_tile<kFmt, kOutF16, true>(p, blockIdx.x, tiles_x, lights, s_noise);
+  resolve_tile<kFmt, kOutF16, true, false, false, kWDepth>(p, blockIdx.x, tiles_x, lights, s_noise);
 }
 
 // Completes the pixels the MLP-IBL resolve deferred.  Persistent waves take
@@ -590,18 +591,23 @@ static althea_status launch_resolve_with(ResolveParams& p, const althea_gbuffer_
       return ALTHEA_ERROR_INVALID_ARGUMENT;
     }
     const dim3 grid(tiles);
-    if (fp32 && f16_out)
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, true>), grid, dim3(256), 0, s, p,
-                         lights4);
-    else if (fp32)
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP32, false>), grid, dim3(256), 0, s, p,
-                         lights4);
-    else if (f16_out)
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, true>), grid, dim3(256), 0, s, p,
-                         lights4);
-    else
-      hipLaunchKernelGGL((resolve_mlp_kernel<ALTHEA_GBUFFER_FP16, false>), grid, dim3(256), 0, s, p,
-                         lights4);
+    const int mv = (f16_out ? 1 : 0) | (fp32 ? 0 : 2) | (p.ssao_w_depth ? 4 : 0);
+    switch (mv) {
+#define ALTHEA_RESOLVE_MLP_CASE(v, fmt, f16, wdepth)                                                    \
+  case v:                                                                                             \
+    hipLaunchKernelGGL((resolve_mlp_kernel<fmt, f16, wdepth>), grid, dim3(256), 0, s, p, lights4); \
+    break;
+      ALTHEA_RESOLVE_MLP_CASE(0, ALTHEA_GBUFFER_FP32, false, false)
+      ALTHEA_RESOLVE_MLP_CASE(1, ALTHEA_GBUFFER_FP32, true, false)
+      ALTHEA_RESOLVE_MLP_CASE(2, ALTHEA_GBUFFER_FP16, false, false)
+      ALTHEA_RESOLVE_MLP_CASE(3, ALTHEA_GBUFFER_FP16, true, false)
+      ALTHEA_RESOLVE_MLP_CASE(4, ALTHEA_GBUFFER_FP32, false, true)
+      ALTHEA_RESOLVE_MLP_CASE(5, ALTHEA_GBUFFER_FP32, true, true)
+      ALTHEA_RESOLVE_MLP_CASE(6, ALTHEA_GBUFFER_FP16, false, true)
+      ALTHEA_RESOLVE_MLP_CASE(7, ALTHEA_GBUFFER_FP16, true, true)
+#undef ALTHEA_RESOLVE_MLP_CASE
+      default: break;
+    }
     althea_status st = check_launch("resolve_mlp_kernel");
     if (st != ALTHEA_OK) return st;
     int dev = 0, cus = 256;
@@ -619,12 +625,16 @@ static althea_status launch_resolve_with(ResolveParams& p, const althea_gbuffer_
     if (reflection) refl_chain(*reflection, p.refl);
     p.shadow.cubes = shadow_cubes;
     p.shadow.size = shadow_size;
-    const int variant = (reflection ? 1 : 0) | (shadow_cubes ? 2 : 0) | (f16_out ? 4 : 0) | (fp32 ? 0 : 8);
+    const int variant = (reflection ? 1 : 0) | (shadow_cubes ? 2 : 0) | (f16_out ? 4 : 0) | (fp32 ? 0 : 8) |
+                        (p.ssao_w_depth ? 16 : 0);
     switch (variant) {
-#define ALTHEA_RESOLVE_CASE(v, fmt, f16, refl, shadow)                                                            \
-  case v:                                                                                                         \
-    hipLaunchKernelGGL((resolve_kernel<fmt, f16, refl, shadow>), grid, dim3(256), 0, s, p, lights4);              \
+#define ALTHEA_RESOLVE_CASE2(v, fmt, f16, refl, shadow, wdepth)                                                  \
+  case v:                                                                                                        \
+    hipLaunchKernelGGL((resolve_kernel<fmt, f16, refl, shadow, false, wdepth>), grid, dim3(256), 0, s, p, lights4); \
     break;
+#define ALTHEA_RESOLVE_CASE(v, fmt, f16, refl, shadow)    \
+  ALTHEA_RESOLVE_CASE2(v, fmt, f16, refl, shadow, false) \
+  ALTHEA_RESOLVE_CASE2(v + 16, fmt, f16, refl, shadow, true)
       ALTHEA_RESOLVE_CASE(1, ALTHEA_GBUFFER_FP32, false, true, false)
       ALTHEA_RESOLVE_CASE(2, ALTHEA_GBUFFER_FP32, false, false, true)
       ALTHEA_RESOLVE_CASE(3, ALTHEA_GBUFFER_FP32, false, true, true)
@@ -638,6 +648,7 @@ static althea_status launch_resolve_with(ResolveParams& p, const althea_gbuffer_
       ALTHEA_RESOLVE_CASE(14, ALTHEA_GBUFFER_FP16, true, false, true)
       ALTHEA_RESOLVE_CASE(15, ALTHEA_GBUFFER_FP16, true, true, true)
 #undef ALTHEA_RESOLVE_CASE
+#undef ALTHEA_RESOLVE_CASE2
       default: break;
     }
     return check_launch("resolve_kernel");
