@@ -34,8 +34,8 @@ FP32_PEAK_TFLOPS = 157.3
 # same counters on a 1 GiB copy (bench.py --only copy), and the VALU wave
 # instructions (SQ_INSTS_VALU).  They describe the kernel source they were
 # measured on and are refreshed with it.
-RESOLVE_TRAFFIC_BYTES = {"fp32": 1.420e9}
-RESOLVE_VALU_WAVE_INSTS = {"fp32": 363306524}
+RESOLVE_TRAFFIC_BYTES = {"fp32": 1.428e9}
+RESOLVE_VALU_WAVE_INSTS = {"fp32": 360039216}
 # The SSAO view-depth pre-pass of the same call (ssao_depth_kernel): the
 # position plane read once, the 4-byte plane written once.
 DEPTH_PASS_TRAFFIC_BYTES = {"fp32": 1.659e8}
