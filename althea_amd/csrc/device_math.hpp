@@ -258,14 +258,13 @@ struct SplitLevel {
   uint32_t pad;
 };
 
-// gather_taps on a split level: the same four texels and fractions.
-__device__ __forceinline__ LevelTaps split_taps(const char* __restrict__ base, const SplitLevel& g, float yaw,
-                                                float pitch) {
-  const float x = fmaf(yaw, g.kx, g.cx), y = fmaf(pitch, g.ky, g.cy);
+// The four taps of a split level around padded entry (ix + 1, iy + 1) with
+// fractions (ax, ay).
+__device__ __forceinline__ LevelTaps split_taps_at(const char* __restrict__ base, const SplitLevel& g, int ix, int iy,
+                                                   float ax, float ay) {
   LevelTaps t;
-  t.ax = __builtin_amdgcn_fractf(x);
-  t.ay = __builtin_amdgcn_fractf(y);
-  const int ix = cvt_floor(x), iy = cvt_floor(y);
+  t.ax = ax;
+  t.ay = ay;
   // RG offset (iy + 1) rowbytes + (ix + 1) 8 as iy * rowbytes + (8 ix +
   // rowbytes + 8); the B offset is half of it.
   const int rb = static_cast<int>(g.rowbytes);
@@ -287,6 +286,35 @@ __device__ __forceinline__ LevelTaps split_taps(const char* __restrict__ base, c
   t.t11 = mk(a1[2], a1[3], b1[1]);
   return t;
 }
+
+// gather_taps on a split level: the same four texels and fractions.
+__device__ __forceinline__ LevelTaps split_taps(const char* __restrict__ base, const SplitLevel& g, float yaw,
+                                                float pitch) {
+  const float x = fmaf(yaw, g.kx, g.cx), y = fmaf(pitch, g.ky, g.cy);
+  return split_taps_at(base, g, cvt_floor(x), cvt_floor(y), __builtin_amdgcn_fractf(x), __builtin_amdgcn_fractf(y));
+}
+
+// Blend grid of a two-level lookup at a fixed LOD (the irradiance's): the
+// value (1 - delta) B0 + delta B1 of the fine level's bilinear B0 and the
+// next level's B1 is bilinear between the lines where either piecewise
+// bilinear function bends, the fine level's texel centres (integer fine
+// coordinates x0) and the coarse level's (x0 = 2k + 1/2).  Its nodes sit at
+// x0 = 2k, 2k + 1/2 and 2k + 1 (three per two fine texels, along each
+// axis), so one bilinear lookup of the node values replaces the two levels'
+// lookups.  Stored as a split level of (3 w0 / 2 + 2) x (3 h0 / 2 + 2)
+// nodes (REPEAT border); kx, cx, ky, cy are the fine level's.
+//
+// Lower node of the cell holding x0 and the fraction across it: r = x0 - 2k
+// in [0, 2) is exact (Sterbenz) for k >= 0 and rounded once for x0 in
+// [-1/2, 0); the fractions 2r, 2r - 1 and r - 1 are exact.
+__device__ __forceinline__ int blend_cell(float x0, float& t) {
+  const float k = floorf(0.5f * x0);
+  const float r = fmaf(-2.0f, k, x0);
+  const bool c1 = r >= 0.5f, c2 = r >= 1.0f;
+  t = c2 ? r - 1.0f : (c1 ? fmaf(2.0f, r, -1.0f) : 2.0f * r);
+  return 3 * static_cast<int>(k) + static_cast<int>(c1) + static_cast<int>(c2);
+}
+
 
 // acc += w * bilinear in weight form: the four texel weights carry w, so a
 // sample's two source levels (w = its weight times 1 - delta and times
@@ -323,6 +351,16 @@ __device__ __forceinline__ void gather_bilinear_acc(const char* __restrict__ gpy
 __device__ __forceinline__ void split_bilinear_acc(const char* __restrict__ base, const SplitLevel& g, float yaw,
                                                    float pitch, float w, float& ar, float& ag, float& ab) {
   taps_acc(split_taps(base, g, yaw, pitch), w, ar, ag, ab);
+}
+
+__device__ __forceinline__ void blend_bilinear_acc(const char* __restrict__ base, const SplitLevel& g, float yaw,
+                                                   float pitch, float w, float& ar, float& ag, float& ab) {
+  // Coordinates below -1/2 by rounding (and NaN) go to -1/2, whose cell's
+  // lower node is the border node -1.
+  const float x0 = fmaxf(fmaf(yaw, g.kx, g.cx), -0.5f), y0 = fmaxf(fmaf(pitch, g.ky, g.cy), -0.5f);
+  float ax, ay;
+  const int nx = blend_cell(x0, ax), ny = blend_cell(y0, ay);
+  taps_acc(split_taps_at(base, g, nx, ny, ax, ay), w, ar, ag, ab);
 }
 
 // Bilinear CLAMP_TO_EDGE fetch (the shading-side samplers).

@@ -611,6 +611,8 @@ althea_status launch_prefilter_points(const float* pyr, int env_w, int env_h, in
 // pi / (theta_samples phi_samples).
 struct IrradianceParams {
   SplitLevel lv0, lv1;  // the two source levels around LOD log2(W / theta_samples), split gather copies
+  SplitLevel grid;      // their blend grid (blend)
+  int blend;
   int levels;
   int size;
   int theta_samples;
@@ -621,8 +623,9 @@ struct IrradianceParams {
 };
 
 // phi_tab[j] = (cos, sin, cos sin (1 - delta), cos sin delta): the sample
-// weight split over the two source levels (delta = the LOD's fraction).
-__global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float delta,
+// weight split over the two source levels (delta = the LOD's fraction);
+// with the blend grid, whose values carry the split, (cos, sin, cos sin, 0).
+__global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, float delta, int blend,
                                          float2* __restrict__ theta_tab, float4* __restrict__ phi_tab) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double pi = 3.14159265358979323846;
@@ -633,8 +636,9 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
   if (i < phi_samples) {
     double ph = i * 0.5 * pi / phi_samples;
     double c = cos(ph), s = sin(ph), d = static_cast<double>(delta);
-    phi_tab[i] = make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(c * s * (1.0 - d)),
-                             static_cast<float>(c * s * d));
+    phi_tab[i] = blend ? make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(c * s), 0.0f)
+                       : make_float4(static_cast<float>(c), static_cast<float>(s),
+                                     static_cast<float>(c * s * (1.0 - d)), static_cast<float>(c * s * d));
   }
 }
 
@@ -649,7 +653,9 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
 constexpr int kIrrPhiLanes = 16, kIrrThetaLanes = 4;
 static_assert(kIrrPhiLanes * kIrrThetaLanes == 64, "one wave per texel");
 
-template <bool kTwoLevels>
+// kLevels: 1 (one source level), 2 (two levels, each its own lookup) or 3
+// (the two levels' blend grid, one lookup of weight w0).
+template <int kLevels>
 __device__ __forceinline__ void irradiance_column(const char* __restrict__ gpyr, const float2* __restrict__ theta_tab,
                                                   const IrradianceParams& p, f3 u, f3 v, f3 w, float w0, float w1,
                                                   int a, float& sr, float& sg, float& sb) {
@@ -661,8 +667,12 @@ __device__ __forceinline__ void irradiance_column(const char* __restrict__ gpyr,
                     fmaf(u.z, th.x, fmaf(v.z, th.y, w.z)));
     float yaw, pitch;
     equirect_angles(d, yaw, pitch);
-    split_bilinear_acc(gpyr, p.lv0, yaw, pitch, w0, sr, sg, sb);
-    if (kTwoLevels) split_bilinear_acc(gpyr, p.lv1, yaw, pitch, w1, sr, sg, sb);
+    if (kLevels == 3) {
+      blend_bilinear_acc(gpyr, p.grid, yaw, pitch, w0, sr, sg, sb);
+    } else {
+      split_bilinear_acc(gpyr, p.lv0, yaw, pitch, w0, sr, sg, sb);
+      if (kLevels == 2) split_bilinear_acc(gpyr, p.lv1, yaw, pitch, w1, sr, sg, sb);
+    }
   }
 }
 
@@ -686,10 +696,12 @@ __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __rest
     const float4 ph = phi_tab[j];
     const f3 u = fr.t * ph.y, v = fr.b * ph.y, w = fr.n * ph.x;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-    if (p.delta != 0.0f)
-      irradiance_column<true>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
+    if (p.blend)
+      irradiance_column<3>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
+    else if (p.delta != 0.0f)
+      irradiance_column<2>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
     else
-      irradiance_column<false>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
+      irradiance_column<1>(gpyr, theta_tab, p, u, v, w, ph.z, ph.w, a, sr, sg, sb);
     if (kF64Sums) {
       ar += sr;
       ag += sg;
@@ -746,13 +758,77 @@ static IrradianceLod irradiance_lod(int w, int h, int theta) {
   return r;
 }
 
-// The theta and phi tables, then the split gather copy of levels l0..l1.
+// Blend grid of levels l0 and l1 = l0 + 1 (two levels, each extent of l0
+// even and l1 exactly half of it): nodes per axis 3 w0 / 2, plus the REPEAT
+// border; RG plane texels rounded up to 64.
+struct BlendGrid {
+  bool ok;
+  int w0, h0, nx, ny;
+  size_t plane;
+};
+static BlendGrid blend_grid(int w, int h, const IrradianceLod& lod) {
+  const PyramidGeometry g = pyramid_geometry(w, h);
+  BlendGrid b{};
+  if (lod.delta == 0.0f || lod.l1 != lod.l0 + 1) return b;
+  b.w0 = g.w[lod.l0];
+  b.h0 = g.h[lod.l0];
+  b.ok = b.w0 >= 2 && b.h0 >= 2 && b.w0 % 2 == 0 && b.h0 % 2 == 0 && g.w[lod.l1] == b.w0 / 2 &&
+         g.h[lod.l1] == b.h0 / 2;
+  b.nx = 3 * b.w0 / 2;
+  b.ny = 3 * b.h0 / 2;
+  b.plane = (static_cast<size_t>(b.nx + 2) * (b.ny + 2) + 63) & ~static_cast<size_t>(63);
+  return b;
+}
+
+// The theta and phi tables, then the blend grid of levels l0, l1 or their
+// split gather copy.
 size_t irradiance_workspace_bytes(int w, int h, int, int theta) {
   int phi = irradiance_phi_samples(w, h, theta);
   const size_t tables = align256(sizeof(float2) * theta) + align256(sizeof(float4) * std::max(phi, 1));
   if (w <= 0 || h <= 0 || theta <= 0) return tables;
   const IrradianceLod lod = irradiance_lod(w, h, theta);
+  const BlendGrid b = blend_grid(w, h, lod);
+  if (b.ok) return tables + 12 * b.plane;
   return tables + split_gather_bytes(w, h, lod.l0, lod.l1);
+}
+
+struct BlendGridParams {
+  int w0, h0, nx, ny;
+  uint32_t off0, off1;  // pyramid texel offsets of the two levels
+  double delta;
+  size_t plane;
+};
+
+// One thread per padded node (r, c): node (c - 1, r - 1) wrapped to the
+// grid, at fine coordinates (2 (n / 3) + (n % 3) / 2) per axis; its value
+// (1 - delta) B0 + delta B1 in FP64 (B1 at x0 / 2 - 1/4, REPEAT), rounded
+// once, into the RG and B planes.
+__global__ __launch_bounds__(256) void blend_grid_kernel(const float4* __restrict__ pyr, const BlendGridParams p,
+                                                         float* __restrict__ out) {
+  const size_t W = static_cast<size_t>(p.nx) + 2, i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= W * (p.ny + 2)) return;
+  const int r = static_cast<int>(i / W), c = static_cast<int>(i - r * W);
+  auto wrap = [](int k, int n) { return k < 0 ? k + n : (k >= n ? k - n : k); };
+  const int n = wrap(c - 1, p.nx), m = wrap(r - 1, p.ny);
+  const double x0 = 2.0 * (n / 3) + 0.5 * (n % 3), y0 = 2.0 * (m / 3) + 0.5 * (m % 3);
+  auto bilinear = [&](uint32_t off, int w, int h, double x, double y, double* v) {
+    const double fx = floor(x), fy = floor(y), ax = x - fx, ay = y - fy;
+    const int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+    auto at = [&](int i, int j) {
+      return pyr[off + static_cast<size_t>((j % h + h) % h) * w + (i % w + w) % w];
+    };
+    const float4 t00 = at(i0, j0), t10 = at(i0 + 1, j0), t01 = at(i0, j0 + 1), t11 = at(i0 + 1, j0 + 1);
+    const float a[4][3] = {{t00.x, t00.y, t00.z}, {t10.x, t10.y, t10.z}, {t01.x, t01.y, t01.z}, {t11.x, t11.y, t11.z}};
+    for (int k = 0; k < 3; ++k)
+      v[k] = (1.0 - ay) * ((1.0 - ax) * a[0][k] + ax * a[1][k]) + ay * ((1.0 - ax) * a[2][k] + ax * a[3][k]);
+  };
+  double b0[3], b1[3];
+  bilinear(p.off0, p.w0, p.h0, x0, y0, b0);
+  bilinear(p.off1, p.w0 / 2, p.h0 / 2, 0.5 * x0 - 0.25, 0.5 * y0 - 0.25, b1);
+  float f[3];
+  for (int k = 0; k < 3; ++k) f[k] = static_cast<float>((1.0 - p.delta) * b0[k] + p.delta * b1[k]);
+  reinterpret_cast<float2*>(out)[i] = make_float2(f[0], f[1]);
+  out[2 * p.plane + i] = f[2];
 }
 
 althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size, int theta_samples,
@@ -776,15 +852,43 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   p.l0 = lod.l0;
   p.l1 = lod.l1;
   p.delta = lod.delta;
-  const GatherGeometry gg = gather_geometry(env_w, env_h, p.l0, p.l1);
-  althea_status st = launch_split_gather(pyr, env_w, env_h, gg, gather, s);
-  if (st != ALTHEA_OK) return st;
-  p.lv0 = split_level(env_w, env_h, gg, p.l0);
-  p.lv1 = split_level(env_w, env_h, gg, p.l1);
+  const BlendGrid bg = blend_grid(env_w, env_h, lod);
+  althea_status st = ALTHEA_OK;
+  if (bg.ok) {
+    BlendGridParams q{};
+    q.w0 = bg.w0;
+    q.h0 = bg.h0;
+    q.nx = bg.nx;
+    q.ny = bg.ny;
+    q.off0 = g.off[p.l0];
+    q.off1 = g.off[p.l1];
+    q.delta = static_cast<double>(lod.delta);
+    q.plane = bg.plane;
+    const size_t nodes = static_cast<size_t>(bg.nx + 2) * (bg.ny + 2);
+    hipLaunchKernelGGL(blend_grid_kernel, dim3(static_cast<unsigned>((nodes + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(pyr), q, gather);
+    st = check_launch("blend_grid_kernel");
+    if (st != ALTHEA_OK) return st;
+    const LevelC c0 = make_level(bg.w0, bg.h0, 0);
+    p.grid.kx = c0.kx;
+    p.grid.cx = c0.cx;
+    p.grid.ky = c0.ky;
+    p.grid.cy = c0.cy;
+    p.grid.rowbytes = static_cast<uint32_t>(bg.nx + 2) * 8u;
+    p.grid.rg = 0;
+    p.grid.b = static_cast<uint32_t>(8 * bg.plane);
+    p.blend = 1;
+  } else {
+    const GatherGeometry gg = gather_geometry(env_w, env_h, p.l0, p.l1);
+    st = launch_split_gather(pyr, env_w, env_h, gg, gather, s);
+    if (st != ALTHEA_OK) return st;
+    p.lv0 = split_level(env_w, env_h, gg, p.l0);
+    p.lv1 = split_level(env_w, env_h, gg, p.l1);
+  }
   p.scale = M_PI / theta_samples / phi;
   const int n = std::max(theta_samples, phi);
   hipLaunchKernelGGL(irradiance_tables_kernel, dim3((n + 255) / 256), dim3(256), 0, s, theta_samples, phi, p.delta,
-                     theta_tab, phi_tab);
+                     p.blend, theta_tab, phi_tab);
   st = check_launch("irradiance_tables_kernel");
   if (st != ALTHEA_OK) return st;
   const dim3 grid((6 * size * size + 3) / 4);
