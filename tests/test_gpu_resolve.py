@@ -120,6 +120,23 @@ def test_resolve_config1_full_size(torch):
     _check(scene, ibl, ol.Env(env), img, ao, _pixels(scene, 4000, 4))
 
 
+def test_resolve_general_projection_path(torch):
+    """Under a perspective projection whose w row is (0, 0, -1, 0) the SSAO
+    taps take their view depth as clip w (bit for bit the same value).  The
+    projection scaled by 2 (the same projective map, w row (0, 0, -2, 0))
+    takes the general path, which forms w from the matrix: checked against
+    the oracle, and close to the frame of the unscaled projection."""
+    ibl, oenv = _ibl(torch, "smooth")
+    scene = aa.synth.scene(seed=SEED0 + 6, width=640, height=360, lights=8)
+    base, base_ao = _run(torch, scene, ibl)
+    scene.globals_arr[0:16] *= 2.0
+    scene.globals_arr[16:32] *= 0.5
+    img, ao = _run(torch, scene, ibl)
+    _check(scene, ibl, oenv, img, ao, _pixels(scene, 3000, 7))
+    assert np.mean(np.abs(ao - base_ao) > 1e-4) < 0.01
+    np.testing.assert_allclose(img[ao == base_ao], base[ao == base_ao], rtol=1e-4, atol=1e-6)
+
+
 def test_resolve_no_lights_no_ssao(torch):
     ibl, oenv = _ibl(torch, "smooth")
     scene = aa.synth.scene(seed=SEED0 + 3, width=64, height=48, lights=0)
