@@ -35,7 +35,7 @@ struct BatchMap {
   int size;          // face size at mip 0
   int table_set;     // sample-table set of this map's source extent
   int level_base;    // first LevelC / GatherLevel of this map
-  int pad;
+  int chunk;         // table entries per partial sum, as the single-map kernel cuts this map's sums
 };
 
 // Work item: map, mip, face, tile index within the face.
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void prefilter_batch_kernel(const BatchMap* __
       const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
       out = prefilter_texel<kF64Sums>(GatherAcc{mp.gpyr, glevels + mp.level_base},
                                       tables + static_cast<size_t>(set_mip) * p.samples, counts[set_mip],
-                                      wsum[set_mip], fr);
+                                      wsum[set_mip], fr, mp.chunk);
     }
     if (active) {
       size_t off = 0;
@@ -357,6 +357,7 @@ static althea_status batch_create(const althea_env_view* envs, int count, const 
     gather_levels(w, h, gg, &glevels[static_cast<size_t>(i) * kMaxLevels]);
     maps[i] = BatchMap{reinterpret_cast<const char*>(b->pyramids[i]), reinterpret_cast<const char*>(b->gathers[i]),
                        reinterpret_cast<float4*>(b->cubes[i]), face_sizes[i], it->second, i * kMaxLevels, 0};
+    prefilter_split(face_sizes[i], mips, samples, maps[i].chunk);
   }
   const size_t sets = b->extents.size() * mips;
   if (hipMalloc(&b->d_maps, sizeof(BatchMap) * count) != hipSuccess ||

@@ -218,45 +218,117 @@ struct PrefilterParams {
   GatherLevel glv[kMaxLevels];  // the same levels in the gather copy
   CubeMips cube;
   int samples;
-  int tile_start[kMaxLevels + 1];  // 8x8-texel tiles before each mip
+  int chunk;                        // table entries per partial sum (prefilter_sums)
+  int block_start[kMaxLevels + 1];  // blocks before each mip
 };
 
 constexpr int kPrefilterTile = 8;
 
 // kGather: the sample loop reads the padded gather copy `gpyr`; otherwise the
 // pyramid (bit-identical sums, more address arithmetic per lookup).
-template <bool kF64Sums, bool kGather>
-__global__ __launch_bounds__(64) void prefilter_cube_kernel(const char* __restrict__ pyr,
-                                                            const char* __restrict__ gpyr,
-                                                            const GgxEntry* __restrict__ tables,
-                                                            const int* __restrict__ counts,
-                                                            const double* __restrict__ wsum,
-                                                            const PrefilterParams p, float4* __restrict__ cube) {
+// kSplit waves per block.  A rough mip's block is one 8x8 tile whose table
+// is cut into chunks of p.chunk entries, wave k summing chunk k; wave 0 adds
+// the partials in chunk order (prefilter_sums: the bits of one pass).  A
+// roughness-0 block holds kSplit tiles, one per wave.  kSplit > 1 fills the
+// GPU when the rough mips have few tiles (a 128^2 cube: 510 tiles, half a
+// wave per SIMD).
+template <bool kF64Sums, bool kGather, int kSplit>
+__global__ __launch_bounds__(64 * kSplit) void prefilter_cube_kernel(const char* __restrict__ pyr,
+                                                                     const char* __restrict__ gpyr,
+                                                                     const GgxEntry* __restrict__ tables,
+                                                                     const int* __restrict__ counts,
+                                                                     const double* __restrict__ wsum,
+                                                                     const PrefilterParams p,
+                                                                     float4* __restrict__ cube) {
   const int b = blockIdx.x;
   int m = 0;
-  while (m + 1 < p.cube.mips && b >= p.tile_start[m + 1]) ++m;
+  while (m + 1 < p.cube.mips && b >= p.block_start[m + 1]) ++m;
   const int S = max(1, p.cube.size >> m);
   const int tpf = (S + kPrefilterTile - 1) / kPrefilterTile;
-  const int local = b - p.tile_start[m];
-  const int face = local / (tpf * tpf);
-  const int t = local - face * tpf * tpf;
-  const int x = (t % tpf) * kPrefilterTile + (threadIdx.x & 7);
-  const int y = (t / tpf) * kPrefilterTile + (threadIdx.x >> 3);
-  const bool active = x < S && y < S;
   const auto level = [&](int l) -> const LevelC& { return p.lv[l]; };
-  float4 out;
-  if (m == 0 || p.cube.mips == 1) {
-    // roughness m / (mips - 1) == 0: the mirror lookup (wave-uniform branch)
-    out = prefilter_texel_mirror(pyr, level, face, min(x, S - 1), min(y, S - 1), S);
-  } else {
-    const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
-    const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
-    if (kGather)
-      out = prefilter_texel<kF64Sums>(GatherAcc{gpyr, p.glv}, tab, counts[m], wsum[m], fr);
-    else
-      out = prefilter_texel<kF64Sums>(PyramidAcc{pyr, p.lv}, tab, counts[m], wsum[m], fr);
+  const bool mirror = m == 0 || p.cube.mips == 1;  // roughness m / (mips - 1) == 0 (block-uniform)
+  if (kSplit == 1) {
+    const int local = b - p.block_start[m];
+    const int face = local / (tpf * tpf);
+    const int t = local - face * tpf * tpf;
+    const int x = (t % tpf) * kPrefilterTile + (threadIdx.x & 7);
+    const int y = (t / tpf) * kPrefilterTile + (threadIdx.x >> 3);
+    const bool active = x < S && y < S;
+    float4 out;
+    if (mirror) {
+      // the mirror lookup
+      out = prefilter_texel_mirror(pyr, level, face, min(x, S - 1), min(y, S - 1), S);
+    } else {
+      const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+      const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
+      if (kGather)
+        out = prefilter_value<kF64Sums>(prefilter_sums_whole<kF64Sums>(GatherAcc{gpyr, p.glv}, tab, 0, counts[m], fr),
+                                        wsum[m]);
+      else
+        out = prefilter_value<kF64Sums>(prefilter_sums_whole<kF64Sums>(PyramidAcc{pyr, p.lv}, tab, 0, counts[m], fr),
+                                        wsum[m]);
+    }
+    if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
+    return;
   }
-  if (active) cube[p.cube.off[m] + (static_cast<size_t>(face) * S + y) * S + x] = out;
+  using T = typename PrefilterSums<kF64Sums>::T;
+  __shared__ T s_part[kSplit][4][64];
+  const int tiles = 6 * tpf * tpf;
+  const int wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+  const int local = mirror ? (b - p.block_start[m]) * kSplit + wave : b - p.block_start[m];
+  const int lt = min(local, tiles - 1);
+  const int face = lt / (tpf * tpf);
+  const int t = lt - face * tpf * tpf;
+  const int x = (t % tpf) * kPrefilterTile + (lane & 7);
+  const int y = (t / tpf) * kPrefilterTile + (lane >> 3);
+  const bool active = local < tiles && x < S && y < S;
+  const size_t dst = p.cube.off[m] + (static_cast<size_t>(face) * S + min(y, S - 1)) * S + min(x, S - 1);
+  if (mirror) {
+    const float4 out = prefilter_texel_mirror(pyr, level, face, min(x, S - 1), min(y, S - 1), S);
+    if (active) cube[dst] = out;
+    return;
+  }
+  const Frame fr = local_to_world(cube_texel_raw_dir(face, min(x, S - 1), min(y, S - 1), S));
+  const GgxEntry* __restrict__ tab = tables + static_cast<size_t>(m) * p.samples;
+  const int count = counts[m];
+  const int lo = min(count, wave * p.chunk), hi = min(count, lo + p.chunk);
+  // A wave's range lies inside one chunk: its sums are one pass.
+  const PrefilterSums<kF64Sums> part =
+      kGather ? prefilter_sums_whole<kF64Sums>(GatherAcc{gpyr, p.glv}, tab, lo, hi, fr)
+              : prefilter_sums_whole<kF64Sums>(PyramidAcc{pyr, p.lv}, tab, lo, hi, fr);
+  s_part[wave][0][lane] = part.r;
+  s_part[wave][1][lane] = part.g;
+  s_part[wave][2][lane] = part.b;
+  s_part[wave][3][lane] = static_cast<T>(part.w);
+  __syncthreads();
+  if (wave == 0) {
+    PrefilterSums<kF64Sums> tot;
+    for (int k = 0; k < kSplit; ++k) {
+      tot.r += s_part[k][0][lane];
+      tot.g += s_part[k][1][lane];
+      tot.b += s_part[k][2][lane];
+      tot.w += static_cast<float>(s_part[k][3][lane]);
+    }
+    if (active) cube[dst] = prefilter_value<kF64Sums>(tot, wsum[m]);
+  }
+}
+
+// Waves per block of the single-map prefilter and the table entries per
+// partial sum: the rough mips' tiles times the waves reach 4096 (four waves
+// per SIMD) with at most 8 waves and 64 entries per chunk; chunks are
+// multiples of the 32-entry FP32 blocks.  One wave: a chunk longer than any
+// table.  The batched kernel cuts each map's sums the same way.
+int prefilter_split(int size, int mips, int samples, int& chunk) {
+  long long rough = 0;
+  for (int m = 1; m < mips; ++m) {
+    const int S = std::max(1, size >> m);
+    const int tpf = (S + kPrefilterTile - 1) / kPrefilterTile;
+    rough += 6ll * tpf * tpf;
+  }
+  int k = 1;
+  while (k < 8 && rough > 0 && rough * k < 4096 && samples / (2 * k) >= 64) k *= 2;
+  chunk = k == 1 ? (1 << 30) : ((samples + k - 1) / k + 31) / 32 * 32;
+  return k;
 }
 
 static size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
@@ -485,21 +557,39 @@ static althea_status prefilter_cube_tables_in(const float* pyr, int env_w, int e
   }
   p.cube = cube_mips_of(size, mips);
   p.samples = samples;
-  int tiles = 0;
+  const int split = prefilter_split(size, mips, samples, p.chunk);
+  int blocks = 0;
   for (int m = 0; m < mips; ++m) {
-    p.tile_start[m] = tiles;
+    p.block_start[m] = blocks;
     int S = std::max(1, size >> m);
     int tpf = (S + kPrefilterTile - 1) / kPrefilterTile;
-    tiles += 6 * tpf * tpf;
+    const int tiles = 6 * tpf * tpf;
+    blocks += (m == 0 || mips == 1) ? (tiles + split - 1) / split : tiles;
   }
-  p.tile_start[mips] = tiles;
+  p.block_start[mips] = blocks;
   const char* pyr_b = reinterpret_cast<const char*>(pyr);
   const char* g_b = reinterpret_cast<const char*>(gather);
   float4* cube4 = reinterpret_cast<float4*>(cube);
   const bool f64 = !(flags & ALTHEA_PRECOMPUTE_FP32_SUMS);
-  auto kernel = f64 ? (gather ? prefilter_cube_kernel<true, true> : prefilter_cube_kernel<true, false>)
-                    : (gather ? prefilter_cube_kernel<false, true> : prefilter_cube_kernel<false, false>);
-  hipLaunchKernelGGL(kernel, dim3(tiles), dim3(64), 0, s, pyr_b, g_b, tables, counts, wsum, p, cube4);
+  const int variant = (f64 ? 1 : 0) | (gather ? 2 : 0);
+  switch (split) {
+#define ALTHEA_PREFILTER_LAUNCH(k)                                                                                   \
+  case k: {                                                                                                          \
+    auto kernel = variant == 3   ? prefilter_cube_kernel<true, true, k>                                              \
+                  : variant == 1 ? prefilter_cube_kernel<true, false, k>                                             \
+                  : variant == 2 ? prefilter_cube_kernel<false, true, k>                                             \
+                                 : prefilter_cube_kernel<false, false, k>;                                           \
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * k), 0, s, pyr_b, g_b, tables, counts, wsum, p, cube4);        \
+    break;                                                                                                           \
+  }
+    ALTHEA_PREFILTER_LAUNCH(1)
+    ALTHEA_PREFILTER_LAUNCH(2)
+    ALTHEA_PREFILTER_LAUNCH(4)
+    ALTHEA_PREFILTER_LAUNCH(8)
+#undef ALTHEA_PREFILTER_LAUNCH
+    default:
+      break;
+  }
   return check_launch("prefilter_cube_kernel");
 }
 

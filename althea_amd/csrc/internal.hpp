@@ -80,6 +80,9 @@ althea_status launch_split_gather(const float* pyr, int w, int h, const GatherGe
 // Sample tables of a prefilter (the part of its workspace the allocation-free
 // entry point keeps in the library's fixed array).
 size_t prefilter_table_bytes(int mips, int samples);
+// Waves per block of the single-map prefilter for a cube and the table
+// entries per partial sum (`chunk`), which the batched kernel uses too.
+int prefilter_split(int size, int mips, int samples, int& chunk);
 
 // Kernel launchers (ibl_kernels.hip).
 althea_status launch_ggx_tables(int samples, int mips, int env_w, int env_h, GgxEntry* tables, int* counts,

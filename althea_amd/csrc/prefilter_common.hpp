@@ -2,6 +2,8 @@
 // (persistent) kernels, so both produce bit-identical texels.
 #pragma once
 
+#include <type_traits>
+
 #include "device_math.hpp"
 #include "precise_lookup.hpp"
 
@@ -44,15 +46,33 @@ struct PyramidAcc {
   }
 };
 
-// NdotL-weighted mean of the source over the table (PreFilterEnvMap.comp:147-169):
-// the FP32 per-sample body (each level's bilinear in weight form, added into
-// the running sums), FP64 running sums (kF64Sums) or all-FP32 sums.
+// Sums of the NdotL-weighted source over table entries [lo, hi) for one
+// texel (PreFilterEnvMap.comp:147-169): the FP32 per-sample body (each
+// level's bilinear in weight form, added into running FP32 sums).  kF64Sums:
+// the FP32 sums are flushed into FP64 every kBlockSums entries (blocks
+// aligned at multiples of kBlockSums), into a partial that is added into the
+// total t at every multiple of `chunk` (itself a multiple of kBlockSums) and
+// at the end.  Otherwise all-FP32, the same chunking, and the weight sum in
+// tw.  A texel's entries cut into chunks summed apart (by several waves) and
+// their partials added in chunk order give the same bits as one pass.
+template <bool kF64Sums>
+struct PrefilterSums {
+  using T = typename std::conditional<kF64Sums, double, float>::type;
+  T r = 0, g = 0, b = 0;
+  float w = 0.0f;
+};
+
+// One pass over [lo, hi) with no chunk boundary inside (chunk >= hi - lo):
+// the running FP64 sum of the FP32 blocks, as t = 0 + that sum.
 template <bool kF64Sums, class Acc>
-__device__ __forceinline__ float4 prefilter_texel(const Acc& acc, const GgxEntry* __restrict__ tab, int count,
-                                                  double wsum, const Frame& fr) {
-  double ar = 0.0, ag = 0.0, ab = 0.0;
+__device__ __forceinline__ PrefilterSums<kF64Sums> prefilter_sums_whole(const Acc& acc,
+                                                                        const GgxEntry* __restrict__ tab, int lo,
+                                                                        int hi, const Frame& fr) {
+  using T = typename PrefilterSums<kF64Sums>::T;
+  PrefilterSums<kF64Sums> t;
+  T ar = 0, ag = 0, ab = 0;
   float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
-  for (int e = 0; e < count; ++e) {
+  for (int e = lo; e < hi; ++e) {
     const GgxEntry en = tab[e];
     const f3 L = mk(fmaf(fr.t.x, en.lx, fmaf(fr.b.x, en.ly, fr.n.x * en.lz)),
                     fmaf(fr.t.y, en.lx, fmaf(fr.b.y, en.ly, fr.n.y * en.lz)),
@@ -72,15 +92,79 @@ __device__ __forceinline__ float4 prefilter_texel(const Acc& acc, const GgxEntry
       sw += en.weight;
     }
   }
+  ar += sr;
+  ag += sg;
+  ab += sb;
+  t.r = ar;
+  t.g = ag;
+  t.b = ab;
+  t.w = sw;
+  return t;
+}
+
+template <bool kF64Sums, class Acc>
+__device__ __forceinline__ PrefilterSums<kF64Sums> prefilter_sums(const Acc& acc, const GgxEntry* __restrict__ tab,
+                                                                  int lo, int hi, int chunk, const Frame& fr) {
+  if (hi - lo <= chunk) return prefilter_sums_whole<kF64Sums>(acc, tab, lo, hi, fr);
+  using T = typename PrefilterSums<kF64Sums>::T;
+  PrefilterSums<kF64Sums> t;
+  T ar = 0, ag = 0, ab = 0;  // the chunk's partial
+  float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+  int chunk_end = lo + chunk;
+  for (int e = lo; e < hi; ++e) {
+    const GgxEntry en = tab[e];
+    const f3 L = mk(fmaf(fr.t.x, en.lx, fmaf(fr.b.x, en.ly, fr.n.x * en.lz)),
+                    fmaf(fr.t.y, en.lx, fmaf(fr.b.y, en.ly, fr.n.y * en.lz)),
+                    fmaf(fr.t.z, en.lx, fmaf(fr.b.z, en.ly, fr.n.z * en.lz)));
+    float yaw, pitch;
+    equirect_angles(L, yaw, pitch);
+    acc(en.l0, yaw, pitch, en.w0, sr, sg, sb);
+    if (__float_as_uint(en.w1) != 0u) acc(en.l1, yaw, pitch, en.w1, sr, sg, sb);
+    if (!kF64Sums) sw += en.weight;
+    if (kF64Sums ? (e & (kBlockSums - 1)) == kBlockSums - 1 : e + 1 == chunk_end) {
+      ar += sr;
+      ag += sg;
+      ab += sb;
+      sr = sg = sb = 0.0f;
+    }
+    if (e + 1 == chunk_end) {
+      t.r += ar;
+      t.g += ag;
+      t.b += ab;
+      t.w += sw;
+      ar = ag = ab = 0;
+      sw = 0.0f;
+      chunk_end += chunk;
+    }
+  }
+  ar += sr;
+  ag += sg;
+  ab += sb;
+  t.r += ar;
+  t.g += ag;
+  t.b += ab;
+  t.w += sw;
+  return t;
+}
+
+// The texel's value from its sums: FP64 sums over the FP64 weight sum, or
+// all-FP32 sums over their own weight sum.
+template <bool kF64Sums>
+__device__ __forceinline__ float4 prefilter_value(const PrefilterSums<kF64Sums>& t, double wsum) {
   if (kF64Sums) {
-    ar += sr;
-    ag += sg;
-    ab += sb;
     const double inv = 1.0 / wsum;
-    return make_float4(static_cast<float>(ar * inv), static_cast<float>(ag * inv), static_cast<float>(ab * inv),
+    return make_float4(static_cast<float>(t.r * inv), static_cast<float>(t.g * inv), static_cast<float>(t.b * inv),
                        1.0f);
   }
-  return make_float4(sr / sw, sg / sw, sb / sw, 1.0f);
+  return make_float4(static_cast<float>(t.r) / t.w, static_cast<float>(t.g) / t.w, static_cast<float>(t.b) / t.w,
+                     1.0f);
+}
+
+// The whole table in one pass, cut into chunks of `chunk` entries as above.
+template <bool kF64Sums, class Acc>
+__device__ __forceinline__ float4 prefilter_texel(const Acc& acc, const GgxEntry* __restrict__ tab, int count,
+                                                  double wsum, const Frame& fr, int chunk) {
+  return prefilter_value<kF64Sums>(prefilter_sums<kF64Sums>(acc, tab, 0, count, chunk, fr), wsum);
 }
 
 // Roughness 0 (PreFilterEnvMap.comp:139-161 with a = 0): every half vector
