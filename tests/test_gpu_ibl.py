@@ -229,6 +229,31 @@ def test_prefilter_edge_shapes(torch, size, mips, samples):
     assert err.max() < MAX_REL, summary(err)
 
 
+@pytest.mark.parametrize("size,mips,samples", [(64, 5, 128), (256, 5, 1024), (128, 5, 1000), (64, 4, 100)])
+def test_prefilter_sample_split_shapes(torch, size, mips, samples):
+    """Cubes whose rough mips have few 8x8 tiles split each tile's sample
+    table over a block of waves (chunks of a multiple of 32 entries, FP64
+    partials added in order), as long as every chunk keeps 64 entries.
+    These shapes take 2 (132 rough tiles, 128 samples), 4 (2 040 tiles),
+    8 (510 tiles; 1 000 samples, so the last chunk is short) and 1 wave
+    (100 samples) per tile."""
+    env = _env("smooth")
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    cube = aa.prefilter_cube(pyr, 512, 256, size, mips, samples).cpu().numpy()
+    ids = cube_texel_ids(size, mips, per_face=6, seed=6)
+    err = rel_err(cube_gather(cube, size, ids), ol.Env(env).prefilter_texels(size, mips, samples, ids)[:, :3],
+                  RADIANCE_FLOOR)
+    assert err.max() < MAX_REL, summary(err)
+    # The allocation-free entry point runs the same kernel: the same bits.
+    from althea_amd import _lib
+    again = torch.full((aa.cube_texels(size, mips), 4), float("nan"), dtype=torch.float32, device="cuda")
+    st = aa.lib().althea_prefilter_cube_noalloc(pyr.data_ptr(), 512, 256, size, mips, samples, 0, again.data_ptr(),
+                                                torch.cuda.current_stream().cuda_stream)
+    assert st == _lib.ALTHEA_OK, aa.lib().althea_last_error()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(again.cpu().numpy().view(np.uint32), cube.view(np.uint32))
+
+
 def test_prefilter_points_match_oracle(env_case, torch):
     name, env, dev, pyr, oenv = env_case
     q = aa.synth.dir_rough(seed=9, n=3000, rmin=0.05, rmax=1.0)
