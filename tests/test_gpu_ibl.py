@@ -317,6 +317,22 @@ def test_gather_copy_edge_sources(torch, w, h):
     assert err.max() < MAX_REL, summary(err)
 
 
+@pytest.mark.parametrize("w,h,theta", [(64, 64, 24), (32, 128, 12), (1024, 4, 300), (256, 128, 100)])
+def test_irradiance_blend_grid_aspects(torch, w, h, theta):
+    """Two source levels (a fractional LOD log2(W / theta)) read through their
+    blend grid, on square, tall and 256:1 sources and at a 1.36 LOD: the
+    grid's nodes follow each axis of the fine level separately."""
+    env = aa.synth.env(seed=SEED0 + 7 * w + h, width=w, height=h, sigma=0.5, suns=2, peak_min=10.0, peak_max=20.0,
+                       sun_sigma_deg=15.0)
+    pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
+    lod = np.log2(w / theta)
+    assert lod % 1.0 != 0.0
+    irr = aa.irradiance_cube(pyr, w, h, 8, theta).cpu().numpy()
+    ids = cube_texel_ids(8, 1, per_face=6, seed=13)
+    err = rel_err(cube_gather(irr, 8, ids), ol.Env(env).irradiance_texels(8, ids[:, 1:], theta), RADIANCE_FLOOR)
+    assert err.max() < MAX_REL, summary(err)
+
+
 # -------------------------------------------------------------- irradiance
 def test_irradiance_cube_matches_oracle(env_case, torch):
     name, env, dev, pyr, oenv = env_case
