@@ -376,9 +376,27 @@ def shadow_frame(torch, aa, scene, ibl, gbuf, steps: int) -> dict:
     covered = float((cubes < 1.0).float().mean().item())
     cube_bytes = coll.shadow_storage.numel() * 4  # faces with their one-texel borders
     px = WIDTH * HEIGHT
+    # The same mesh as a glTF primitive with an alpha-masked base colour (a
+    # quarter of a 512^2 sRGB texture below the 0.5 cut-off, 10 mips,
+    # coordinates repeating 8 times): the shadow shaders' alpha test.
+    rng = np.random.default_rng(SEED0 + 11)
+    base = rng.integers(0, 256, (512, 512, 4), dtype=np.uint8)
+    base[..., 3] = np.where(rng.uniform(size=(512, 512)) < 0.25, 60, 230)
+    tex = aa.Texture(torch.from_numpy(base).cuda(), mips=10, srgb=True)
+    uv = np.stack([(pos[:, 0] / 20.0 + 0.5) * 8.0, (pos[:, 2] / 20.0 + 0.5) * 8.0], 1).astype(np.float32)
+    prim = aa.GltfPrimitive(pos_t, None, None, None, [torch.from_numpy(uv).cuda()], idx_t, 0)
+    mat = aa.GltfMaterial(0, 0, 0)
+
+    def draw_gltf():
+        coll.drawShadowMapsGltf([prim], [mat], [tex])
+
+    draw_gltf()
+    gltf_ms = time_events(torch, draw_gltf, steps)
+    gltf_covered = float((coll.getShadowMapImage() < 1.0).float().mean().item())
     return {"lights": LIGHTS, "cube_size": SHADOW_SIZE, "triangles": int(idx.shape[0]),
             "draw_ms": draw_ms, "draw_texels_per_s": cubes.numel() / (draw_ms * 1e-3),
             "draw_algorithmic_bytes": 2 * cube_bytes + pos.nbytes + idx.nbytes, "texels_covered": covered,
+            "draw_gltf_alpha_tested_ms": gltf_ms, "texels_covered_gltf": gltf_covered,
             "resolve_shadowed_ms": res_ms, "resolve_shadowed_mpix_per_s": px / (res_ms * 1e-3) / 1e6,
             "frame_ms": frame_ms, "frame_mpix_per_s": px / (frame_ms * 1e-3) / 1e6}
 
