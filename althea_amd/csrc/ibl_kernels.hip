@@ -733,14 +733,17 @@ __global__ void irradiance_tables_kernel(int theta_samples, int phi_samples, flo
 }
 
 // One wave per output texel.  A lane owns one polar column (phi index
-// 16 k + lane / 4) per pass and walks the azimuth rows theta = 4 s + lane % 4,
-// so for a fixed step the wave's 64 lookups cover a compact patch of 16 polar
-// by 4 azimuth samples (at most 9.6 x 4.8 degrees at the configured 300 x 150)
-// and the gathers of a step share cache lines; a ring of 64 azimuths reaches
-// 77 degrees.  Each lane sums its column's samples over theta in FP32 in theta
+// 8 k + lane / 8) per pass and walks the azimuth rows theta = 8 s + lane % 8,
+// so for a fixed step the wave's 64 lookups cover a compact patch of 8 polar
+// by 8 azimuth samples (4.8 degrees of polar angle by at most 9.6 degrees,
+// times sin(phi), of azimuth at the configured 300 x 150) and the gathers of
+// a step share cache lines; a ring of 64 azimuths reaches 77 degrees.  Of
+// the patches 4 x 16, 8 x 8, 16 x 4 and 32 x 2 (polar x azimuth), 8 x 8
+// wastes the fewest lanes on the configured 150 x 300 samples: 19 passes of
+// 38 steps (722 wave-steps) against 10 of 75 (750) for 16 x 4.  Each lane sums its column's samples over theta in FP32 in theta
 // order and flushes the column into FP64 (kF64Sums) or FP32; the lanes are
 // combined by a fixed xor tree across the wave.
-constexpr int kIrrPhiLanes = 16, kIrrThetaLanes = 4;
+constexpr int kIrrPhiLanes = 8, kIrrThetaLanes = 8;
 static_assert(kIrrPhiLanes * kIrrThetaLanes == 64, "one wave per texel");
 
 // kLevels: 1 (one source level), 2 (two levels, each its own lookup) or 3
