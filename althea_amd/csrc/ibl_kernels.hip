@@ -769,11 +769,21 @@ __device__ __forceinline__ void irradiance_column(const char* __restrict__ gpyr,
   }
 }
 
-template <bool kF64Sums>
+// Azimuth tables of up to this many rows are read from LDS (kLds): the
+// table's loads then leave the texture data return, which bounds the kernel.
+constexpr int kIrrThetaLdsMax = 4096;
+
+template <bool kF64Sums, bool kLds>
 __global__ __launch_bounds__(256) void irradiance_cube_kernel(const char* __restrict__ gpyr,
-                                                              const float2* __restrict__ theta_tab,
+                                                              const float2* __restrict__ theta_glob,
                                                               const float4* __restrict__ phi_tab,
                                                               const IrradianceParams p, float4* __restrict__ cube) {
+  extern __shared__ float2 s_theta[];  // theta_samples rows when kLds
+  if (kLds) {
+    for (int i = threadIdx.x; i < p.theta_samples; i += 256) s_theta[i] = theta_glob[i];
+    __syncthreads();
+  }
+  const float2* theta_tab = kLds ? s_theta : theta_glob;
   const int S = p.size;
   const int texel = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
   if (texel >= 6 * S * S) return;  // whole waves; no block barrier follows
@@ -987,10 +997,21 @@ althea_status launch_irradiance(const float* pyr, int env_w, int env_h, int size
   const dim3 grid((6 * size * size + 3) / 4);
   const char* g4 = reinterpret_cast<const char*>(gather);
   float4* cube4 = reinterpret_cast<float4*>(cube);
-  if (flags & ALTHEA_PRECOMPUTE_FP32_SUMS)
-    hipLaunchKernelGGL(irradiance_cube_kernel<false>, grid, dim3(256), 0, s, g4, theta_tab, phi_tab, p, cube4);
-  else
-    hipLaunchKernelGGL(irradiance_cube_kernel<true>, grid, dim3(256), 0, s, g4, theta_tab, phi_tab, p, cube4);
+  const bool f64 = !(flags & ALTHEA_PRECOMPUTE_FP32_SUMS);
+  if (theta_samples <= kIrrThetaLdsMax) {
+    const size_t lds = sizeof(float2) * static_cast<size_t>(theta_samples);
+    if (f64)
+      hipLaunchKernelGGL((irradiance_cube_kernel<true, true>), grid, dim3(256), lds, s, g4, theta_tab, phi_tab, p,
+                         cube4);
+    else
+      hipLaunchKernelGGL((irradiance_cube_kernel<false, true>), grid, dim3(256), lds, s, g4, theta_tab, phi_tab, p,
+                         cube4);
+  } else if (f64) {
+    hipLaunchKernelGGL((irradiance_cube_kernel<true, false>), grid, dim3(256), 0, s, g4, theta_tab, phi_tab, p, cube4);
+  } else {
+    hipLaunchKernelGGL((irradiance_cube_kernel<false, false>), grid, dim3(256), 0, s, g4, theta_tab, phi_tab, p,
+                       cube4);
+  }
   return check_launch("irradiance_cube_kernel");
 }
 

@@ -352,8 +352,10 @@ def test_irradiance_fp32_sums_variant(env_case, torch):
     assert err.max() < 1e-3, summary(err)
 
 
-@pytest.mark.parametrize("size,theta", [(1, 300), (5, 17), (8, 100), (32, 300)])
+@pytest.mark.parametrize("size,theta", [(1, 300), (5, 17), (8, 100), (32, 300), (1, 4096), (1, 4097)])
 def test_irradiance_edge_shapes(torch, size, theta):
+    """Cube sizes and azimuth counts; up to 4096 azimuth rows the kernel
+    reads its azimuth table from LDS, above it from global memory."""
     env = _env("smooth")
     pyr = aa.env_pyramid(torch.from_numpy(env).cuda())
     cube = aa.irradiance_cube(pyr, 512, 256, size, theta).cpu().numpy()
