@@ -172,6 +172,38 @@ def test_resolve_mlp_accepting_every_cell_matches_oracle(torch):
     assert s["mean"] < 1e-5 and s["p99"] < 1e-4, s
 
 
+def test_resolve_mlp_ragged_frames_every_pixel(torch):
+    """Frames smaller than one 64 x 4 tile or one past it through the MLP-IBL
+    resolve with every cell accepted: the deferred queue and the finishing
+    kernel serve exactly the shaded pixels, and every pixel of the frames
+    (pooled) meets the bar of the full-size test against the oracle."""
+    env = _smooth_env()
+    ibl = _ibl(torch, env, size=64, mips=5)
+    net, _, _ = _trained(torch, env, steps_epochs=2, count=1 << 15)
+    _, accepted = net.build_monitor(ibl.prefiltered_tensor(), 64, 5, threshold=1e30)
+    assert accepted == 5 * 6 * 64
+    views, oenv = ibl.views_numpy(), ol.Env(env)
+    errs = []
+    for W, H in [(1, 1), (7, 3), (65, 2), (130, 7)]:
+        scene = aa.synth.scene(seed=SEED0 + 9 + W, width=W, height=H, lights=8)
+        img, mask = _scene_run(torch, scene, ibl, net)
+        assert np.isfinite(img).all()
+        shaded = scene.position[..., 3].reshape(H, W) != 0
+        assert np.array_equal(mask.astype(bool), shaded), (W, H)
+        xs, ys = np.meshgrid(np.arange(W), np.arange(H))
+        pix = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+        _, _, _, q = ol.resolve(scene, views, oenv, pix, query=True)
+        refl = np.full((pix.shape[0], 3), np.nan)
+        fg = ~np.isnan(q[:, 0])
+        if fg.any():
+            refl[fg] = net.infer(torch.from_numpy(q[fg].astype(np.float32)).cuda()).cpu().numpy()[:, :3]
+        ref, _, margin = ol.resolve(scene, views, oenv, pix, refl=refl)
+        e = rel_err(img[pix[:, 1], pix[:, 0], :3], ref, RADIANCE_FLOOR).max(1)
+        errs.append(e[margin > 2e-3])
+    s = summary(np.concatenate(errs))
+    assert s["mean"] < 1e-5 and s["p99"] < 1e-4, s
+
+
 def test_mlp_gain_over_cube_of_more_bytes(torch):
     """Rough lobes (roughness 0.5..1): the trained MLP (BF16 weights, 21 KB)
     against a 16^2 x6 cube with 5 mips (RGBA32F, 32.7 KB) prefiltered from
